@@ -1,0 +1,143 @@
+"""ctypes binding of libfastkcna.so (the C ABI declared in include/fk_api.h).
+
+The structures mirror fk_api.h field for field; tests compare their sizes
+with what a C compiler reports for the header.  Loading fails loudly when the
+library has not been built: there is no CPU fallback in the product path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfastkcna.so")
+
+FK_OK, FK_E_INVALID, FK_E_NOMEM, FK_E_DEVICE, FK_E_IO, FK_E_INTERNAL, FK_E_NODEVICE = range(7)
+FK_PG_NSG, FK_PG_HNSW_LAYER, FK_PG_TAUMNG, FK_PG_ALPHAPG = 1, 2, 3, 4
+FK_FLAG_VERBOSE = 0x1
+FK_FLAG_NO_CONTROLS = 0x2
+FK_MAX_LOOPS = 16
+
+STATUS_NAMES = {
+    FK_OK: "FK_OK",
+    FK_E_INVALID: "FK_E_INVALID",
+    FK_E_NOMEM: "FK_E_NOMEM",
+    FK_E_DEVICE: "FK_E_DEVICE",
+    FK_E_IO: "FK_E_IO",
+    FK_E_INTERNAL: "FK_E_INTERNAL",
+    FK_E_NODEVICE: "FK_E_NODEVICE",
+}
+
+u32, u64, f32, f64 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_float, ctypes.c_double
+
+
+class FkParams(ctypes.Structure):
+    _fields_ = [
+        ("K", u32), ("L", u32), ("S", u32), ("R", u32), ("iterations", u32), ("controls", u32), ("seed", u32),
+        ("delta", f32), ("recall", f32),
+        ("nsg_R", u32), ("search_L", u32), ("search_K", u32), ("step", u32), ("massq_S", u32), ("loop_i", u32),
+        ("tau", f32), ("angle", f32),
+        ("flags", u32),
+    ]
+
+
+class FkStats(ctypes.Structure):
+    _fields_ = [
+        ("d_init", u64), ("d_ctrl", u64), ("d_join", u64), ("d_prune", u64), ("d_prune_rev", u64),
+        ("d_inter", u64), ("d_ep", u64), ("d_bv", u64), ("d_conn", u64),
+        ("t_total", f64), ("t_ctrl", f64), ("t_nnd", f64), ("t_prune", f64), ("t_inter", f64), ("t_ep", f64),
+        ("t_conn", f64), ("t_bv", f64), ("t_upload", f64),
+        ("nnd_iterations", u32), ("nnd_stop", u32), ("nnd_recall", f32), ("nnd_delta", f32),
+        ("n_loops", u32), ("n_kgraph", u32), ("n_recall", u32),
+        ("kgraph_time", f64 * (FK_MAX_LOOPS + 64)),
+        ("recall", f32 * (FK_MAX_LOOPS + 64)),
+        ("prune_time", f64 * FK_MAX_LOOPS), ("inter_time", f64 * FK_MAX_LOOPS),
+        ("tree_grow_time", f64 * FK_MAX_LOOPS),
+        ("buildnsg_time", f64 * FK_MAX_LOOPS), ("search_time", f64 * FK_MAX_LOOPS),
+        ("con_time", f64 * FK_MAX_LOOPS),
+        ("prune_cc", f32 * FK_MAX_LOOPS), ("search_cc", f32 * FK_MAX_LOOPS),
+        ("aod", u32), ("max_degree", u32), ("min_degree", u32), ("n_attached", u32),
+        ("bv_rounds", u32 * FK_MAX_LOOPS),
+    ]
+
+    def to_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
+class FkGraph(ctypes.Structure):
+    _fields_ = [
+        ("n", u32), ("ep", u32), ("width", u32),
+        ("offsets", ctypes.POINTER(u64)),
+        ("ids", ctypes.POINTER(u32)),
+        ("stats", FkStats),
+    ]
+
+
+_P = ctypes.c_void_p
+_PROTOS = {
+    "fk_open": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
+    "fk_close": (None, [_P]),
+    "fk_last_error": (ctypes.c_char_p, [_P]),
+    "fk_version": (ctypes.c_char_p, []),
+    "fk_dataset_upload": (ctypes.c_int, [_P, _P, ctypes.c_size_t, u32, u32]),
+    "fk_build": (ctypes.c_int, [_P, ctypes.c_int, u32, ctypes.POINTER(FkParams), ctypes.POINTER(FkGraph)]),
+    "fk_graph_free": (None, [ctypes.POINTER(FkGraph)]),
+    "fk_pair_distances": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, _P]),
+    "fk_join_distances": (ctypes.c_int, [_P, _P, _P, _P, u32, _P, ctypes.POINTER(u64)]),
+    "fk_prune": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, u32, _P, u32, u32, ctypes.POINTER(FkParams),
+                                ctypes.c_int, _P, _P, u32, _P, ctypes.POINTER(u64)]),
+    "fk_nndescent": (ctypes.c_int, [_P, u32, ctypes.POINTER(FkParams), _P, _P, u32, ctypes.POINTER(FkStats)]),
+    "fk_knn_bruteforce": (ctypes.c_int, [_P, _P, u32, u32, u32, _P, _P, _P]),
+    "fk_search": (ctypes.c_int, [_P, _P, _P, u32, u32, _P, u32, u32, u32, _P, ctypes.POINTER(u64)]),
+}
+
+_lib = None
+
+
+class FkError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+def load():
+    """Load libfastkcna.so; raises if it is missing (build it first)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "or `make -C fastkcna_amd/csrc`"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _PROTOS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return sorted(_PROTOS)
+
+
+def ptr(a):
+    """Raw pointer of a C-contiguous numpy array (None for None)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def params_struct(p):
+    """FkParams from any object with the IndexParams attribute names."""
+    s = FkParams()
+    for name, _ in FkParams._fields_:
+        setattr(s, name, getattr(p, name))
+    return s

@@ -1,0 +1,262 @@
+// C-ABI entry points: handle lifetime, dataset residency and the unit-level
+// entry points (distances, prune, brute force).  The full build and
+// NN-descent live in fk_build.hip, the query search in fk_search.hip.
+#include <math.h>
+
+#include <new>
+
+#include "fk_api_impl.h"
+#include "fk_device.h"
+#include "fk_internal.h"
+
+namespace fk {
+
+float alpha_cos_threshold(float angle) {
+    // src/kgraph.cpp:3160-3161: float angle; std::cos(angle / 180 * kPi) with
+    // kPi a double constant; the float quotient is widened, cos is taken in
+    // double and the result narrowed to float.
+    const double kPi = 3.14159265358979323846264;
+    return (float)cos((double)(angle / 180.0f) * kPi);
+}
+
+int rule_of(fk_pg_type t) {
+    switch (t) {
+        case FK_PG_TAUMNG: return FK_RULE_TAU;
+        case FK_PG_ALPHAPG: return FK_RULE_ALPHA;
+        default: return FK_RULE_NSG;
+    }
+}
+
+void select_device(fk_handle *h) { FK_HIP(hipSetDevice(h->device)); }
+
+void require_dataset(fk_handle *h) {
+    if (h->ds.n_total == 0) throw Error(FK_E_INVALID, "no dataset uploaded (fk_dataset_upload)");
+}
+
+}  // namespace fk
+
+using namespace fk;
+
+extern "C" {
+
+const char *fk_version(void) { return "fastkcna-amd-0.1 gfx950"; }
+
+fk_status fk_open(int device, fk_handle **out) {
+    if (!out) return FK_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0 || device < 0 || device >= n) return FK_E_NODEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return FK_E_NODEVICE;
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return FK_E_NODEVICE;
+    fk_handle *h = new (std::nothrow) fk_handle();
+    if (!h) return FK_E_NOMEM;
+    h->device = device;
+    fk_status st = guard(h, [&] {
+        select_device(h);
+        FK_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->counters.alloc(64);
+    });
+    if (st != FK_OK) {
+        delete h;
+        return st;
+    }
+    *out = h;
+    return FK_OK;
+}
+
+void fk_close(fk_handle *h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    h->ds.rows.release();
+    h->counters.release();
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+const char *fk_last_error(const fk_handle *h) { return h ? h->last_error.c_str() : "null handle"; }
+
+fk_status fk_dataset_upload(fk_handle *h, const float *rows, size_t row_stride_bytes, uint32_t n_total,
+                            uint32_t d) {
+    if (!h) return FK_E_INVALID;
+    return guard(h, [&] {
+        FK_CHECK_ARG(rows != nullptr && n_total > 0 && d > 0, "rows must be non-empty");
+        FK_CHECK_ARG(row_stride_bytes >= (size_t)d * sizeof(float), "row stride smaller than a row");
+        select_device(h);
+        Dataset &ds = h->ds;
+        ds.rows.release();
+        ds.n_total = 0;
+        const uint32_t dpad = (d + 7u) & ~7u;
+        ds.rows.alloc((size_t)n_total * dpad);
+        if (dpad != d) FK_HIP(hipMemsetAsync(ds.rows.p, 0, ds.rows.bytes(), h->stream));
+        FK_HIP(hipMemcpy2DAsync(ds.rows.p, (size_t)dpad * sizeof(float), rows, row_stride_bytes,
+                                (size_t)d * sizeof(float), n_total, hipMemcpyHostToDevice, h->stream));
+        FK_HIP(hipStreamSynchronize(h->stream));
+        ds.n_total = n_total;
+        ds.d = d;
+        ds.dpad = dpad;
+        ds.stride = dpad;
+    });
+}
+
+fk_status fk_pair_distances(fk_handle *h, const uint32_t *a, const uint32_t *b, size_t n, float *out) {
+    if (!h) return FK_E_INVALID;
+    return guard(h, [&] {
+        require_dataset(h);
+        FK_CHECK_ARG(n == 0 || (a && b && out), "null pointer");
+        for (size_t i = 0; i < n; ++i)
+            FK_CHECK_ARG(a[i] < h->ds.n_total && b[i] < h->ds.n_total, "row id out of range");
+        if (n == 0) return;
+        select_device(h);
+        DevBuf<uint32_t> da(n), db(n);
+        DevBuf<float> dout(n);
+        FK_HIP(hipMemcpyAsync(da.p, a, n * 4, hipMemcpyHostToDevice, h->stream));
+        FK_HIP(hipMemcpyAsync(db.p, b, n * 4, hipMemcpyHostToDevice, h->stream));
+        launch_pair_distances(h->stream, h->ds, da.p, db.p, n, dout.p);
+        FK_HIP(hipMemcpyAsync(out, dout.p, n * 4, hipMemcpyDeviceToHost, h->stream));
+        FK_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+fk_status fk_join_distances(fk_handle *h, const uint32_t *ids, const uint64_t *list_off, const uint32_t *n_new,
+                            uint32_t n_nodes, float *out, uint64_t *n_pairs) {
+    if (!h) return FK_E_INVALID;
+    return guard(h, [&] {
+        require_dataset(h);
+        FK_CHECK_ARG(list_off && n_new, "null pointer");
+        const uint64_t n_ids = list_off[n_nodes];
+        FK_CHECK_ARG(n_ids == 0 || ids, "null ids");
+        std::vector<uint64_t> pair_off(n_nodes + 1, 0);
+        for (uint32_t t = 0; t < n_nodes; ++t) {
+            FK_CHECK_ARG(list_off[t + 1] >= list_off[t], "list offsets must be non-decreasing");
+            const uint64_t T = list_off[t + 1] - list_off[t];
+            const uint64_t nn = n_new[t];
+            FK_CHECK_ARG(nn <= T, "n_new larger than the list");
+            pair_off[t + 1] = pair_off[t] + nn * (nn - (nn ? 1 : 0)) / 2 + nn * (T - nn);
+        }
+        for (uint64_t i = 0; i < n_ids; ++i) FK_CHECK_ARG(ids[i] < h->ds.n_total, "id out of range");
+        if (n_pairs) *n_pairs = pair_off[n_nodes];
+        if (pair_off[n_nodes] == 0 || n_nodes == 0) return;
+        FK_CHECK_ARG(out, "null output");
+        select_device(h);
+        DevBuf<uint32_t> dids(n_ids), dnn(n_nodes);
+        DevBuf<uint64_t> dlo(n_nodes + 1), dpo(n_nodes + 1);
+        DevBuf<float> dout(pair_off[n_nodes]);
+        FK_HIP(hipMemcpyAsync(dids.p, ids, n_ids * 4, hipMemcpyHostToDevice, h->stream));
+        FK_HIP(hipMemcpyAsync(dnn.p, n_new, (size_t)n_nodes * 4, hipMemcpyHostToDevice, h->stream));
+        FK_HIP(hipMemcpyAsync(dlo.p, list_off, (size_t)(n_nodes + 1) * 8, hipMemcpyHostToDevice, h->stream));
+        FK_HIP(hipMemcpyAsync(dpo.p, pair_off.data(), (size_t)(n_nodes + 1) * 8, hipMemcpyHostToDevice,
+                              h->stream));
+        launch_join_distances(h->stream, h->ds, dids.p, dlo.p, dnn.p, dpo.p, n_nodes, dout.p);
+        FK_HIP(hipMemcpyAsync(out, dout.p, pair_off[n_nodes] * 4, hipMemcpyDeviceToHost, h->stream));
+        FK_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+fk_status fk_prune(fk_handle *h, fk_pg_type type, const uint64_t *pools, const uint32_t *pool_len,
+                   uint32_t pool_stride, const uint32_t *q, uint32_t n_tasks, uint32_t oracle_size,
+                   const fk_params *p, int reversed, uint32_t *out_ids, float *out_dists, uint32_t out_stride,
+                   uint32_t *out_len, uint64_t *cc) {
+    if (!h) return FK_E_INVALID;
+    return guard(h, [&] {
+        require_dataset(h);
+        FK_CHECK_ARG(p && pools && pool_len && out_ids && out_len, "null pointer");
+        FK_CHECK_ARG(out_stride >= p->nsg_R, "out_stride < nsg_R");
+        FK_CHECK_ARG(p->nsg_R <= 256, "nsg_R above 256 is not supported");
+        const uint32_t lim = reversed ? h->ds.n_total : (oracle_size < h->ds.n_total ? oracle_size : h->ds.n_total);
+        for (uint32_t t = 0; t < n_tasks; ++t) {
+            FK_CHECK_ARG(pool_len[t] <= pool_stride, "pool_len > pool_stride");
+            const uint64_t *c = pools + (uint64_t)t * pool_stride;
+            // Rows the pruner reads must be resident: pool[start] is always
+            // kept; the sync form skips later ids >= oracle_size by itself,
+            // the reversed form reads every entry.
+            const uint32_t qt = q ? q[t] : t;
+            const uint32_t start = (pool_len[t] > 0 && (uint32_t)c[0] == qt) ? 1u : 0u;
+            if (start < pool_len[t]) FK_CHECK_ARG((uint32_t)c[start] < h->ds.n_total, "pool id out of range");
+            if (reversed)
+                for (uint32_t k = 0; k < pool_len[t]; ++k)
+                    FK_CHECK_ARG((uint32_t)c[k] < h->ds.n_total, "pool id out of range");
+        }
+        if (n_tasks == 0) {
+            if (cc) *cc = 0;
+            return;
+        }
+        select_device(h);
+        const size_t pool_n = (size_t)n_tasks * pool_stride;
+        DevBuf<uint64_t> dpool(pool_n);
+        DevBuf<uint32_t> dlen(n_tasks), dq(n_tasks), doid((size_t)n_tasks * out_stride), dolen(n_tasks);
+        DevBuf<float> dod((size_t)n_tasks * out_stride);
+        FK_HIP(hipMemcpyAsync(dpool.p, pools, pool_n * 8, hipMemcpyHostToDevice, h->stream));
+        FK_HIP(hipMemcpyAsync(dlen.p, pool_len, (size_t)n_tasks * 4, hipMemcpyHostToDevice, h->stream));
+        if (q) FK_HIP(hipMemcpyAsync(dq.p, q, (size_t)n_tasks * 4, hipMemcpyHostToDevice, h->stream));
+        FK_HIP(hipMemsetAsync(h->counters.p, 0, 8, h->stream));
+        PruneLaunch a;
+        a.cand = dpool.p;
+        a.cand_len = dlen.p;
+        a.cand_stride = pool_stride;
+        a.qid = q ? dq.p : nullptr;
+        a.n_tasks = n_tasks;
+        a.oracle_n = reversed ? 0xFFFFFFFFu : lim;
+        a.maxc = reversed ? 0xFFFFFFFFu : p->search_K;
+        a.R = p->nsg_R;
+        a.tau3 = 3.0f * p->tau;
+        a.cos_thr = alpha_cos_threshold(p->angle);
+        a.rule = rule_of(type);
+        a.out_ids = doid.p;
+        a.out_d = dod.p;
+        a.out_stride = out_stride;
+        a.out_len = dolen.p;
+        a.cc = h->counters.p;
+        launch_prune(h->stream, h->ds, a);
+        FK_HIP(hipMemcpyAsync(out_ids, doid.p, (size_t)n_tasks * out_stride * 4, hipMemcpyDeviceToHost,
+                              h->stream));
+        if (out_dists)
+            FK_HIP(hipMemcpyAsync(out_dists, dod.p, (size_t)n_tasks * out_stride * 4, hipMemcpyDeviceToHost,
+                                  h->stream));
+        FK_HIP(hipMemcpyAsync(out_len, dolen.p, (size_t)n_tasks * 4, hipMemcpyDeviceToHost, h->stream));
+        unsigned long long c = 0;
+        FK_HIP(hipMemcpyAsync(&c, h->counters.p, 8, hipMemcpyDeviceToHost, h->stream));
+        FK_HIP(hipStreamSynchronize(h->stream));
+        if (cc) *cc = c;
+    });
+}
+
+fk_status fk_knn_bruteforce(fk_handle *h, const float *queries, uint32_t n_q, uint32_t n_used, uint32_t K,
+                            const uint32_t *self_ids, uint32_t *out_ids, float *out_dists) {
+    if (!h) return FK_E_INVALID;
+    return guard(h, [&] {
+        require_dataset(h);
+        FK_CHECK_ARG(n_used >= 1 && n_used <= h->ds.n_total, "n_used out of range");
+        FK_CHECK_ARG(K >= 1 && K < n_used + (self_ids ? 0u : 1u), "K must be below the number of rows");
+        if (n_q == 0) return;
+        FK_CHECK_ARG(queries && out_ids, "null pointer");
+        select_device(h);
+        const Dataset &ds = h->ds;
+        DevBuf<float> dq((size_t)n_q * ds.dpad);
+        if (ds.dpad != ds.d) FK_HIP(hipMemsetAsync(dq.p, 0, dq.bytes(), h->stream));
+        FK_HIP(hipMemcpy2DAsync(dq.p, (size_t)ds.dpad * 4, queries, (size_t)ds.d * 4, (size_t)ds.d * 4, n_q,
+                                hipMemcpyHostToDevice, h->stream));
+        DevBuf<uint32_t> dself, dout((size_t)n_q * K);
+        DevBuf<float> dd((size_t)n_q * K);
+        if (self_ids) {
+            dself.alloc(n_q);
+            FK_HIP(hipMemcpyAsync(dself.p, self_ids, (size_t)n_q * 4, hipMemcpyHostToDevice, h->stream));
+        }
+        bruteforce_knn(h->stream, ds, dq.p, ds.dpad, n_q, n_used, K, self_ids ? dself.p : nullptr, dout.p, dd.p);
+        FK_HIP(hipMemcpyAsync(out_ids, dout.p, (size_t)n_q * K * 4, hipMemcpyDeviceToHost, h->stream));
+        if (out_dists)
+            FK_HIP(hipMemcpyAsync(out_dists, dd.p, (size_t)n_q * K * 4, hipMemcpyDeviceToHost, h->stream));
+        FK_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+void fk_graph_free(fk_graph *g) {
+    if (!g) return;
+    free(g->offsets);
+    free(g->ids);
+    g->offsets = nullptr;
+    g->ids = nullptr;
+}
+
+}  // extern "C"

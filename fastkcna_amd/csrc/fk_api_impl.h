@@ -1,0 +1,33 @@
+// Shared helpers of the C-ABI translation units.
+#pragma once
+#include <new>
+
+#include "fk_internal.h"
+
+namespace fk {
+
+// Runs f, converting any failure into a status and the handle's last error.
+template <class F>
+fk_status guard(fk_handle *h, F &&f) {
+    try {
+        f();
+        if (h) h->last_error.clear();
+        return FK_OK;
+    } catch (const Error &e) {
+        if (h) h->last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        if (h) h->last_error = "host allocation failed";
+        return FK_E_NOMEM;
+    } catch (const std::exception &e) {
+        if (h) h->last_error = e.what();
+        return FK_E_INTERNAL;
+    }
+}
+
+float alpha_cos_threshold(float angle);
+int rule_of(fk_pg_type t);
+void select_device(fk_handle *h);
+void require_dataset(fk_handle *h);
+
+}  // namespace fk

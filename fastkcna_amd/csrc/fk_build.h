@@ -1,0 +1,120 @@
+// Device state of one graph build and the phase entry points shared by the
+// NN-descent, loop and search translation units.
+//
+// Layout in HBM (structure of arrays, N = nodes of the build):
+//   pool      u64[N][cap]   packed (dist bits << 32 | id), each node's first
+//                           cnt[i] entries sorted ascending; cap = MAXL
+//   pflg      u8 [N][cap]   bit0 = flag (not yet expanded), bit1 = isnew
+//   cnt       u32[N]        valid pool entries (the reference's Nhood::L)
+//   nsg_id    u32[N][ncap]  pruned out-lists, ncap = nsg_R + slack for the
+//   nsg_d     f32[N][ncap]  connectivity edges; nsg_len u32[N]
+//   nsg_new   u8 [N][ncap]  isnew bit of each out-list entry
+//   old_id    u32[N][ncap]  previous loop's out-list (UpdateNhoods)
+#pragma once
+#include "fk_internal.h"
+
+namespace fk {
+
+#define FK_FLG_NEW 0x1u
+#define FK_FLG_ISNEW 0x2u
+#define FK_POOL_MAX 1024   // largest supported MAXL
+#define FK_LIST_MAX 2048   // largest supported NN-descent sample list
+#define FK_NSG_SLACK 64    // out-list room for connectivity edges
+
+struct BuildParams {
+    // effective (shrunk) parameters, src/kgraph.cpp:1288-1334
+    uint32_t K, L, S, R, iterations, controls, seed;
+    float delta, recall;
+    uint32_t nsg_R, search_L, search_K, step, massq_S, loop_i;
+    float tau, angle;
+    uint32_t flags;
+    uint32_t MAXL;
+};
+
+BuildParams shrink_params(const fk_params &p, uint32_t N);
+
+struct BuildState {
+    fk_handle *h = nullptr;
+    hipStream_t s = nullptr;
+    const Dataset *ds = nullptr;
+    uint32_t N = 0;        // oracle.size()
+    uint32_t cap = 0;      // pool capacity (MAXL)
+    uint32_t ncap = 0;     // out-list capacity
+    BuildParams P{};
+    fk_stats *st = nullptr;
+    bool verbose = false;
+
+    DevBuf<uint64_t> pool;
+    DevBuf<uint8_t> pflg;
+    DevBuf<uint32_t> cnt;
+    DevBuf<float> thr;       // join filter threshold per node
+    DevBuf<uint32_t> M;
+    DevBuf<float> radiusM;
+    DevBuf<uint8_t> found;
+
+    // sampling lists: [fwd new | rev new | fwd old | rev old] per node
+    DevBuf<uint64_t> list_off;  // N + 1
+    DevBuf<uint32_t> n_new;     // N
+    DevBuf<uint32_t> lists;
+    DevBuf<uint64_t> pair_off;  // N + 1
+    DevBuf<uint32_t> pair_cnt;  // scratch N
+    DevBuf<float> pdist;
+    DevBuf<uint32_t> resv;
+    DevBuf<uint32_t> bcnt;      // N
+    DevBuf<uint64_t> boff;      // N + 1
+    DevBuf<uint64_t> bkey;
+    DevBuf<uint32_t> bsrc;
+    // reverse-sample buckets
+    DevBuf<uint32_t> fcnt_new, fcnt_old, rcnt_new, rcnt_old;  // N each
+    DevBuf<uint64_t> roff_new, roff_old;                       // N + 1
+    DevBuf<uint32_t> rbuf_new, rbuf_old;
+
+    // sampled ground truth (GenerateControl)
+    uint32_t n_ctrl = 0;
+    DevBuf<uint32_t> ctrl_ids;   // n_ctrl
+    DevBuf<uint32_t> ctrl_gt;    // n_ctrl x MAXL ids
+    DevBuf<unsigned long long> acc;  // reduction scratch
+
+    // pruned graph
+    DevBuf<uint32_t> nsg_id, nsg_len, old_id, old_len;
+    DevBuf<float> nsg_d;
+    DevBuf<uint8_t> nsg_new;
+    uint32_t ep = 0;
+    uint32_t width = 0;
+    uint32_t loopcount = 0;
+    std::vector<float> center;  // Calcenter over all resident rows (cached)
+    uint64_t host_rng = 0;      // the reference's global rand() stream
+
+    // scratch
+    DevBuf<uint64_t> scal;      // 16 u64
+    DevBuf<uint64_t> scan_tmp;  // prefix-scan partials
+
+    uint64_t next_rand() {
+        host_rng += 0x9E3779B97F4A7C15ull;
+        uint64_t x = host_rng;
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+        return x ^ (x >> 31);
+    }
+    void sync();
+};
+
+// Phases (fk_nnd.hip)
+void nnd_alloc(BuildState &b);
+void nnd_controls(BuildState &b);
+void nnd_run(BuildState &b);            // init + iterations (constructor)
+float nnd_evaluate(BuildState &b, uint32_t K);  // recall over controls at K
+
+// Pool merge and BridgeView (fk_nnd.hip)
+void bridgeview(BuildState &b);
+
+// Loops (fk_build.hip)
+void build_loops(BuildState &b, fk_pg_type type);
+
+// Search / graph utilities (fk_search.hip)
+void compute_ep(BuildState &b);
+void tree_grow(BuildState &b);
+
+double now_sec();
+
+}  // namespace fk

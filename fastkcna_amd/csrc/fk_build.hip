@@ -1,0 +1,453 @@
+// The FastKCNA refinement loops on the GPU and the fk_build / fk_nndescent
+// entry points.
+//
+// Per loop (buildNSG / buildtauMG / buildAlphaPG / buildHNSWLayer,
+// src/kgraph.cpp:2314-2337, 3143-3154, 3482-3493):
+//   Link      prune every pool into an out-list (full rule), add reverse
+//             edges, re-prune over-full lists (nsg_Link :2087-2143)
+//   ComputeEp centroid + beam search (fk_search.hip)
+//   tree_grow connectivity (fk_search.hip; not for HNSW layers)
+//   UpdateNhoods  isnew bits of the out-lists (:3537-3584)
+// and between loops BridgeView (fk_nnd.hip).
+//
+// The pruner always applies the full rule (*_sync_prune).  The reference
+// switches to the incremental *_sync_prune2 for nodes that were not
+// re-pruned in the previous loop; the paper states that the two agree
+// (result-preserving Opt2), and the CPU oracle diffs them.
+#include <math.h>
+#include <stdlib.h>
+
+#include <algorithm>
+
+#include "fk_api_impl.h"
+#include "fk_build.h"
+#include "fk_device.h"
+#include "fk_sort.h"
+
+namespace fk {
+
+// Parameter shrinking of the KGraphConstructor constructor
+// (src/kgraph.cpp:1288-1334) and MAXL.
+BuildParams shrink_params(const fk_params &p, uint32_t N) {
+    BuildParams P;
+    P.K = p.K; P.L = p.L; P.S = p.S; P.R = p.R; P.iterations = p.iterations; P.controls = p.controls;
+    P.seed = p.seed; P.delta = p.delta; P.recall = p.recall; P.nsg_R = p.nsg_R; P.search_L = p.search_L;
+    P.search_K = p.search_K; P.step = p.step; P.massq_S = p.massq_S; P.loop_i = p.loop_i; P.tau = p.tau;
+    P.angle = p.angle; P.flags = p.flags;
+    if (N <= P.controls) P.controls = N - 1;
+    if (N <= P.L) P.L = N - 1;
+    if (N <= P.S) P.S = N - 1;
+    if (N <= P.search_L) P.search_L = N - 1;
+    if (N <= P.search_K) P.search_K = N - 1;
+    if (N <= P.K) {
+        P.K = N - 1;
+        P.loop_i = 0;
+        P.S = N - 1;
+        P.iterations = 0;
+    }
+    P.MAXL = std::max(std::max(P.L, P.K), std::max(P.search_L, P.search_K));
+    return P;
+}
+
+// ---------------------------------------------------------------------------
+// Reverse edges (InterInsert, src/kgraph.cpp:2051-2084): every edge n -> t
+// offers n to t's list unless n is already there.  In-edges are gathered by
+// count / scan / fill as packed (d(n, t), n) keys.
+// ---------------------------------------------------------------------------
+__global__ void k_indeg(const uint32_t *__restrict__ nsg_id, const uint32_t *__restrict__ nsg_len, uint32_t N,
+                        uint32_t ncap, uint32_t *__restrict__ indeg) {
+    const uint32_t n = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (n >= N) return;
+    const uint32_t deg = nsg_len[n];
+    for (uint32_t k = lane_id(); k < deg; k += 64) atomicAdd(&indeg[nsg_id[(uint64_t)n * ncap + k]], 1u);
+}
+
+__global__ void k_in_fill(const uint32_t *__restrict__ nsg_id, const float *__restrict__ nsg_d,
+                          const uint32_t *__restrict__ nsg_len, uint32_t N, uint32_t ncap,
+                          const uint64_t *__restrict__ in_off, uint32_t *__restrict__ cur, uint64_t *__restrict__ in_key) {
+    const uint32_t n = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (n >= N) return;
+    const uint32_t deg = nsg_len[n];
+    for (uint32_t k = lane_id(); k < deg; k += 64) {
+        const uint32_t t = nsg_id[(uint64_t)n * ncap + k];
+        const uint64_t slot = in_off[t] + atomicAdd(&cur[t], 1u);
+        in_key[slot] = make_key(nsg_d[(uint64_t)n * ncap + k], n);
+    }
+}
+
+#define FK_INTER_MAX 4096
+
+// One workgroup per node: out-list A plus the in-edges B not already in A.
+// If the union fits nsg_R it is A followed by B in (dist, id) order;
+// otherwise the union, sorted, is staged for the reversed prune.
+__global__ __launch_bounds__(256) void k_inter_merge(uint32_t N, uint32_t ncap, uint32_t R,
+                                                     uint32_t *__restrict__ nsg_id, float *__restrict__ nsg_d,
+                                                     uint32_t *__restrict__ nsg_len,
+                                                     const uint32_t *__restrict__ indeg,
+                                                     const uint64_t *__restrict__ in_off,
+                                                     const uint64_t *__restrict__ in_key, uint64_t *__restrict__ rp_key,
+                                                     uint64_t *__restrict__ rp_off, uint32_t *__restrict__ rp_len,
+                                                     uint32_t *__restrict__ task_list, uint32_t *__restrict__ n_tasks,
+                                                     uint32_t *__restrict__ err) {
+    __shared__ uint64_t s_b[FK_INTER_MAX];
+    __shared__ uint32_t s_a[FK_NSG_SLACK + 256];
+    __shared__ uint32_t s_nb;
+    const uint32_t des = blockIdx.x;
+    const uint32_t nA = nsg_len[des];
+    const uint32_t nB = indeg[des];
+    if (nB == 0) {
+        if (threadIdx.x == 0) rp_len[des] = 0;
+        return;
+    }
+    if (nA + nB > FK_INTER_MAX) {
+        if (threadIdx.x == 0) { atomicMax(err, nA + nB); rp_len[des] = 0; }
+        return;
+    }
+    uint32_t *ai = nsg_id + (uint64_t)des * ncap;
+    float *ad = nsg_d + (uint64_t)des * ncap;
+    for (uint32_t k = threadIdx.x; k < nA; k += blockDim.x) s_a[k] = ai[k];
+    if (threadIdx.x == 0) s_nb = 0;
+    __syncthreads();
+    const uint64_t *bk = in_key + in_off[des];
+    // B minus A (reference: dedupe against the current list, :2063-2076)
+    for (uint32_t k = threadIdx.x; k < nB; k += blockDim.x) {
+        const uint64_t key = bk[k];
+        const uint32_t src = key_id(key);
+        bool dup = false;
+        for (uint32_t j = 0; j < nA; ++j)
+            if (s_a[j] == src) { dup = true; break; }
+        if (!dup) s_b[atomicAdd(&s_nb, 1u)] = key;
+    }
+    __syncthreads();
+    const uint32_t nb = s_nb;
+    if (nb == 0) {
+        if (threadIdx.x == 0) rp_len[des] = 0;
+        return;
+    }
+    if (nA + nb <= R) {
+        const uint32_t np2 = next_pow2(nb < 2 ? 2 : nb);
+        for (uint32_t k = nb + threadIdx.x; k < np2; k += blockDim.x) s_b[k] = FK_KEY_EMPTY;
+        __syncthreads();
+        block_bitonic_sort(s_b, np2);
+        for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) {
+            ai[nA + k] = key_id(s_b[k]);
+            ad[nA + k] = key_dist(s_b[k]);
+        }
+        if (threadIdx.x == 0) { nsg_len[des] = nA + nb; rp_len[des] = 0; }
+        return;
+    }
+    // over-full: stage A U B sorted for the reversed prune
+    const uint32_t tot = nA + nb;
+    for (uint32_t k = threadIdx.x; k < nA; k += blockDim.x) s_b[nb + k] = make_key(ad[k], s_a[k]);
+    const uint32_t np2 = next_pow2(tot);
+    for (uint32_t k = tot + threadIdx.x; k < np2; k += blockDim.x) s_b[k] = FK_KEY_EMPTY;
+    __syncthreads();
+    block_bitonic_sort(s_b, np2);
+    const uint64_t off = in_off[des] + (uint64_t)des * R;
+    for (uint32_t k = threadIdx.x; k < tot; k += blockDim.x) rp_key[off + k] = s_b[k];
+    if (threadIdx.x == 0) {
+        rp_off[des] = off;
+        rp_len[des] = tot;
+        task_list[atomicAdd(n_tasks, 1u)] = des;
+    }
+}
+
+// UpdateNhoods: isnew = (first loop) or (id not in the previous out-list);
+// then the previous out-list becomes the current one.
+__global__ void k_update_nhoods(const uint32_t *__restrict__ nsg_id, const uint32_t *__restrict__ nsg_len,
+                                uint8_t *__restrict__ nsg_new, uint32_t *__restrict__ old_id,
+                                uint32_t *__restrict__ old_len, uint32_t N, uint32_t ncap, uint32_t first) {
+    const uint32_t n = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (n >= N) return;
+    const uint32_t deg = nsg_len[n];
+    const uint32_t od = first ? 0u : old_len[n];
+    const uint32_t *ids = nsg_id + (uint64_t)n * ncap;
+    uint32_t *oi = old_id + (uint64_t)n * ncap;
+    for (uint32_t k = lane_id(); k < deg; k += 64) {
+        uint8_t isnew = 1;
+        if (!first) {
+            const uint32_t id = ids[k];
+            for (uint32_t j = 0; j < od; ++j)
+                if (oi[j] == id) { isnew = 0; break; }
+        }
+        nsg_new[(uint64_t)n * ncap + k] = isnew;
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t k = lane_id(); k < deg; k += 64) oi[k] = ids[k];
+    if (lane_id() == 0) old_len[n] = deg;
+}
+
+// CalculateNSG (src/kgraph.cpp:2273-2293): degree statistics.
+__global__ void k_degree_stats(const uint32_t *__restrict__ nsg_len, uint32_t N, unsigned long long *__restrict__ acc) {
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const uint32_t d = nsg_len[n];
+    atomicAdd(acc, (unsigned long long)d);
+    atomicMax(acc + 1, (unsigned long long)d);
+    atomicMin(acc + 2, (unsigned long long)d);
+}
+
+__global__ void k_pack_csr(const uint32_t *__restrict__ nsg_id, const uint32_t *__restrict__ nsg_len, uint32_t N,
+                           uint32_t ncap, const uint64_t *__restrict__ off, uint32_t *__restrict__ out) {
+    const uint32_t n = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (n >= N) return;
+    const uint32_t deg = nsg_len[n];
+    for (uint32_t k = lane_id(); k < deg; k += 64) out[off[n] + k] = nsg_id[(uint64_t)n * ncap + k];
+}
+
+static void link(BuildState &b, fk_pg_type type, uint32_t li) {
+    const double t0 = now_sec();
+    const uint32_t N = b.N;
+    const BuildParams &P = b.P;
+    const unsigned gw = (N + 3) / 4;
+    // 1. prune every pool
+    FK_HIP(hipMemsetAsync(b.acc.p, 0, 16, b.s));
+    PruneLaunch a;
+    a.cand = b.pool.p;
+    a.cand_len = b.cnt.p;
+    a.cand_stride = b.cap;
+    a.n_tasks = N;
+    a.oracle_n = N;
+    a.maxc = P.search_K;
+    a.R = P.nsg_R;
+    a.tau3 = 3.0f * P.tau;
+    a.cos_thr = alpha_cos_threshold(P.angle);
+    a.rule = rule_of(type);
+    a.out_ids = b.nsg_id.p;
+    a.out_d = b.nsg_d.p;
+    a.out_stride = b.ncap;
+    a.out_len = b.nsg_len.p;
+    a.cc = b.acc.p;
+    launch_prune(b.s, *b.ds, a);
+    unsigned long long cc = 0;
+    FK_HIP(hipMemcpyAsync(&cc, b.acc.p, 8, hipMemcpyDeviceToHost, b.s));
+    b.sync();
+    b.width = P.nsg_R;
+    b.st->d_prune += cc;
+    const double t1 = now_sec();
+    if (li < FK_MAX_LOOPS) {
+        b.st->prune_cc[li] = (float)((double)cc / ((double)N * (double)(N - 1) / 2.0));
+        b.st->prune_time[li] = t1 - t0;
+    }
+    b.st->t_prune += t1 - t0;
+    // 2. reverse edges
+    FK_HIP(hipMemsetAsync(b.bcnt.p, 0, (size_t)N * 4, b.s));
+    hipLaunchKernelGGL(k_indeg, dim3(gw), dim3(256), 0, b.s, b.nsg_id.p, b.nsg_len.p, N, b.ncap, b.bcnt.p);
+    FK_LAUNCH_CHECK();
+    exclusive_scan_u32_to_u64(b.s, b.bcnt.p, b.boff.p, N, b.boff.p + N, b.scan_tmp);
+    uint64_t n_in = 0;
+    FK_HIP(hipMemcpyAsync(&n_in, b.boff.p + N, 8, hipMemcpyDeviceToHost, b.s));
+    b.sync();
+    b.bkey.ensure(n_in);
+    FK_HIP(hipMemsetAsync(b.pair_cnt.p, 0, (size_t)N * 4, b.s));
+    hipLaunchKernelGGL(k_in_fill, dim3(gw), dim3(256), 0, b.s, b.nsg_id.p, b.nsg_d.p, b.nsg_len.p, N, b.ncap,
+                       b.boff.p, b.pair_cnt.p, b.bkey.p);
+    FK_LAUNCH_CHECK();
+    DevBuf<uint64_t> rp_key(n_in + (uint64_t)N * P.nsg_R), rp_off(N);
+    DevBuf<uint32_t> rp_len(N), tasks(N), counters(2);
+    FK_HIP(hipMemsetAsync(counters.p, 0, 8, b.s));
+    hipLaunchKernelGGL(k_inter_merge, dim3(N), dim3(256), 0, b.s, N, b.ncap, P.nsg_R, b.nsg_id.p, b.nsg_d.p,
+                       b.nsg_len.p, b.bcnt.p, b.boff.p, b.bkey.p, rp_key.p, rp_off.p, rp_len.p, tasks.p, counters.p,
+                       counters.p + 1);
+    FK_LAUNCH_CHECK();
+    uint32_t hc[2] = {0, 0};
+    FK_HIP(hipMemcpyAsync(hc, counters.p, 8, hipMemcpyDeviceToHost, b.s));
+    b.sync();
+    if (hc[1]) throw Error(FK_E_INTERNAL, "in-degree above the reverse-edge merge capacity (" +
+                                              std::to_string(hc[1]) + " > 4096)");
+    // 3. reversed prune of over-full lists (no search_K cap, no size guard)
+    if (hc[0]) {
+        FK_HIP(hipMemsetAsync(b.acc.p, 0, 8, b.s));
+        PruneLaunch r = a;
+        r.cand = rp_key.p;
+        r.cand_off = rp_off.p;
+        r.cand_len = rp_len.p;
+        r.task_list = tasks.p;
+        r.n_tasks = hc[0];
+        r.oracle_n = 0xFFFFFFFFu;
+        r.maxc = 0xFFFFFFFFu;
+        launch_prune(b.s, *b.ds, r);
+        unsigned long long rc = 0;
+        FK_HIP(hipMemcpyAsync(&rc, b.acc.p, 8, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+        b.st->d_prune_rev += rc;
+    }
+    const double t2 = now_sec();
+    if (li < FK_MAX_LOOPS) b.st->inter_time[li] = t2 - t1;
+    b.st->t_inter += t2 - t1;
+    if (b.verbose)
+        fprintf(stderr, "[fk] link: prune dists %llu, in-edges %llu, re-pruned %u (%.3f s)\n", cc,
+                (unsigned long long)n_in, hc[0], t2 - t0);
+}
+
+static void update_nhoods(BuildState &b) {
+    const unsigned gw = (b.N + 3) / 4;
+    hipLaunchKernelGGL(k_update_nhoods, dim3(gw), dim3(256), 0, b.s, b.nsg_id.p, b.nsg_len.p, b.nsg_new.p, b.old_id.p,
+                       b.old_len.p, b.N, b.ncap, b.loopcount == 0 ? 1u : 0u);
+    FK_LAUNCH_CHECK();
+}
+
+static void calculate_nsg(BuildState &b) {
+    unsigned long long init[3] = {0, 0, ~0ull};
+    FK_HIP(hipMemcpyAsync(b.acc.p, init, 24, hipMemcpyHostToDevice, b.s));
+    hipLaunchKernelGGL(k_degree_stats, dim3((b.N + 255) / 256), dim3(256), 0, b.s, b.nsg_len.p, b.N, b.acc.p);
+    FK_LAUNCH_CHECK();
+    unsigned long long r[3];
+    FK_HIP(hipMemcpyAsync(r, b.acc.p, 24, hipMemcpyDeviceToHost, b.s));
+    b.sync();
+    // avg is an unsigned in the reference: the mean truncated to an integer
+    b.st->aod = (uint32_t)(r[0] / b.N);
+    b.st->max_degree = (uint32_t)r[1];
+    b.st->min_degree = (uint32_t)r[2];
+    if (b.verbose)
+        fprintf(stderr, "[fk] Degree Statistics: Max = %u, Min = %u, Avg = %u\n", b.st->max_degree,
+                b.st->min_degree, b.st->aod);
+}
+
+void build_loops(BuildState &b, fk_pg_type type) {
+    const uint32_t N = b.N;
+    b.ncap = b.P.nsg_R + FK_NSG_SLACK;
+    b.nsg_id.alloc((size_t)N * b.ncap);
+    b.nsg_d.alloc((size_t)N * b.ncap);
+    b.nsg_new.alloc((size_t)N * b.ncap);
+    b.old_id.alloc((size_t)N * b.ncap);
+    b.nsg_len.alloc(N);
+    b.old_len.alloc(N);
+    FK_HIP(hipMemsetAsync(b.old_len.p, 0, (size_t)N * 4, b.s));
+    b.loopcount = 0;
+    const uint32_t iters = b.P.loop_i + 1;
+    const double s0 = b.st->t_total;  // build start offset is tracked by the caller
+    (void)s0;
+    for (uint32_t iter = 1; iter <= iters; ++iter) {
+        const uint32_t li = iter - 1;
+        const double tb = now_sec();
+        link(b, type, li);
+        compute_ep(b);
+        if (type != FK_PG_HNSW_LAYER) tree_grow(b);
+        update_nhoods(b);
+        b.sync();
+        if (li < FK_MAX_LOOPS) b.st->buildnsg_time[li] = now_sec() - tb;
+        b.loopcount++;
+        b.st->n_loops = iter;
+        if (iter == iters) {
+            calculate_nsg(b);
+            break;
+        }
+        const double ts = now_sec();
+        bridgeview(b);
+        if (li < FK_MAX_LOOPS) b.st->search_time[li] = now_sec() - ts;
+        const float rec = nnd_evaluate(b, b.P.MAXL);
+        if (b.st->n_recall < FK_MAX_LOOPS + 64) b.st->recall[b.st->n_recall++] = rec;
+        if (b.verbose) fprintf(stderr, "[fk] recall after BridgeView: %g\n", rec);
+    }
+}
+
+}  // namespace fk
+
+using namespace fk;
+
+static void check_params(const fk_params *p) {
+    FK_CHECK_ARG(p != nullptr, "null params");
+    FK_CHECK_ARG(p->K >= 1 && p->L >= 1 && p->S >= 1, "K, L and S must be positive");
+    FK_CHECK_ARG(p->nsg_R >= 1 && p->nsg_R <= 256, "nsg_R must be in [1, 256]");
+    FK_CHECK_ARG(p->R >= 1, "R = 0 (uncapped reverse lists) is not supported");
+    FK_CHECK_ARG((uint64_t)p->L + 2ull * p->R <= FK_LIST_MAX, "L + 2R must not exceed 2048");
+    FK_CHECK_ARG(p->massq_S >= 1, "massq_S must be positive");
+}
+
+static void stats_to_out(const BuildState &b, fk_graph *out, std::vector<uint64_t> &off, uint32_t *ids) {
+    out->n = b.N;
+    out->ep = b.ep;
+    out->width = b.width;
+    out->offsets = (uint64_t *)malloc(off.size() * sizeof(uint64_t));
+    if (!out->offsets) throw std::bad_alloc();
+    memcpy(out->offsets, off.data(), off.size() * sizeof(uint64_t));
+    out->ids = ids;
+}
+
+extern "C" {
+
+fk_status fk_build(fk_handle *h, fk_pg_type type, uint32_t n_used, const fk_params *p, fk_graph *out) {
+    if (!h || !out) return FK_E_INVALID;
+    memset(out, 0, sizeof(*out));
+    return guard(h, [&] {
+        require_dataset(h);
+        check_params(p);
+        FK_CHECK_ARG(type == FK_PG_NSG || type == FK_PG_HNSW_LAYER || type == FK_PG_TAUMNG || type == FK_PG_ALPHAPG,
+                     "unknown graph type");
+        FK_CHECK_ARG(n_used >= 2 && n_used <= h->ds.n_total, "n_used must be in [2, n_total]");
+        select_device(h);
+        const double t0 = now_sec();
+        BuildState b;
+        b.h = h;
+        b.s = h->stream;
+        b.ds = &h->ds;
+        b.N = n_used;
+        b.P = shrink_params(*p, n_used);
+        b.cap = b.P.MAXL;
+        b.st = &out->stats;
+        b.verbose = (p->flags & FK_FLAG_VERBOSE) != 0;
+        b.host_rng = ((uint64_t)b.P.seed << 32) ^ 0x5DEECE66Dull;
+        nnd_controls(b);
+        nnd_run(b);
+        // buildFast*: evaluate() at MAXL, then the constructor's time
+        const float rec = nnd_evaluate(b, b.P.MAXL);
+        if (b.st->n_recall < FK_MAX_LOOPS + 64) b.st->recall[b.st->n_recall++] = rec;
+        if (b.st->n_kgraph < FK_MAX_LOOPS + 64) b.st->kgraph_time[b.st->n_kgraph++] = now_sec() - t0;
+        build_loops(b, type);
+        // output CSR
+        std::vector<uint32_t> len(b.N);
+        FK_HIP(hipMemcpyAsync(len.data(), b.nsg_len.p, (size_t)b.N * 4, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+        std::vector<uint64_t> off((size_t)b.N + 1, 0);
+        for (uint32_t i = 0; i < b.N; ++i) off[i + 1] = off[i] + len[i];
+        DevBuf<uint64_t> doff(b.N + 1);
+        DevBuf<uint32_t> dids(off[b.N]);
+        FK_HIP(hipMemcpyAsync(doff.p, off.data(), off.size() * 8, hipMemcpyHostToDevice, b.s));
+        hipLaunchKernelGGL(k_pack_csr, dim3((b.N + 3) / 4), dim3(256), 0, b.s, b.nsg_id.p, b.nsg_len.p, b.N, b.ncap,
+                           doff.p, dids.p);
+        FK_LAUNCH_CHECK();
+        uint32_t *ids = (uint32_t *)malloc(std::max<uint64_t>(off[b.N], 1) * 4);
+        if (!ids) throw std::bad_alloc();
+        FK_HIP(hipMemcpyAsync(ids, dids.p, off[b.N] * 4, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+        stats_to_out(b, out, off, ids);
+        out->stats.t_total = now_sec() - t0;
+    });
+}
+
+fk_status fk_nndescent(fk_handle *h, uint32_t n_used, const fk_params *p, uint64_t *pools, uint32_t *pool_len,
+                       uint32_t pool_stride, fk_stats *stats) {
+    if (!h) return FK_E_INVALID;
+    return guard(h, [&] {
+        require_dataset(h);
+        check_params(p);
+        FK_CHECK_ARG(n_used >= 2 && n_used <= h->ds.n_total, "n_used must be in [2, n_total]");
+        FK_CHECK_ARG(pools && pool_len, "null output");
+        select_device(h);
+        const double t0 = now_sec();
+        fk_stats local;
+        memset(&local, 0, sizeof(local));
+        BuildState b;
+        b.h = h;
+        b.s = h->stream;
+        b.ds = &h->ds;
+        b.N = n_used;
+        b.P = shrink_params(*p, n_used);
+        FK_CHECK_ARG(pool_stride >= b.P.L, "pool_stride < L");
+        b.cap = b.P.MAXL;
+        b.st = stats ? stats : &local;
+        memset(b.st, 0, sizeof(*b.st));
+        b.verbose = (p->flags & FK_FLAG_VERBOSE) != 0;
+        b.host_rng = ((uint64_t)b.P.seed << 32) ^ 0x5DEECE66Dull;
+        nnd_controls(b);
+        nnd_run(b);
+        FK_HIP(hipMemcpyAsync(pool_len, b.cnt.p, (size_t)b.N * 4, hipMemcpyDeviceToHost, b.s));
+        FK_HIP(hipMemcpy2DAsync(pools, (size_t)pool_stride * 8, b.pool.p, (size_t)b.cap * 8,
+                                (size_t)std::min(b.cap, pool_stride) * 8, b.N, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+        b.st->t_total = now_sec() - t0;
+    });
+}
+
+}  // extern "C"

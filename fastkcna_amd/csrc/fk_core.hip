@@ -1,0 +1,430 @@
+// Core kernels: pair distances, the local-join distance block, the occlusion
+// pruner, prefix scans and the brute-force exact kNN.
+#include "fk_device.h"
+#include "fk_internal.h"
+#include "fk_sort.h"
+
+namespace fk {
+
+// ---------------------------------------------------------------------------
+// Pair distances: one thread per (a[k], b[k]) pair.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pair_distances(const float *__restrict__ rows, uint32_t stride,
+                                                        uint32_t dpad, const uint32_t *__restrict__ a,
+                                                        const uint32_t *__restrict__ b, size_t n,
+                                                        float *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = l2_canon(rows + (size_t)a[i] * stride, rows + (size_t)b[i] * stride, dpad);
+}
+
+void launch_pair_distances(hipStream_t s, const Dataset &ds, const uint32_t *a, const uint32_t *b,
+                           size_t n, float *out) {
+    if (n == 0) return;
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_pair_distances, dim3(grid), dim3(256), 0, s, ds.rows.p, ds.stride, ds.dpad, a, b,
+                       n, out);
+    FK_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Local-join distance block.  One workgroup per node; the node's sample rows
+// are staged in LDS (row stride dpad + 4 floats, so lanes reading the same
+// offset of different rows hit different banks) when they fit, and read from
+// HBM/L2 otherwise.  Pair (x, y), x < n_new, x < y < T, goes to
+// out[pair_off + rowstart(x) + y - x - 1], rowstart(x) = x(T-1) - x(x-1)/2.
+// ---------------------------------------------------------------------------
+#define FK_JOIN_LDS_BYTES (48 * 1024)
+
+__global__ __launch_bounds__(256) void k_join_distances(const float *__restrict__ rows, uint32_t stride,
+                                                        uint32_t dpad, const uint32_t *__restrict__ ids,
+                                                        const uint64_t *__restrict__ list_off,
+                                                        const uint32_t *__restrict__ n_new,
+                                                        const uint64_t *__restrict__ pair_off,
+                                                        float *__restrict__ out) {
+    extern __shared__ __align__(16) float s_rows[];
+    const uint32_t node = blockIdx.x;
+    const uint64_t lo = list_off[node];
+    const uint32_t T = (uint32_t)(list_off[node + 1] - lo);
+    const uint32_t nn = n_new[node];
+    const uint32_t *lst = ids + lo;
+    float *o = out + pair_off[node];
+    const uint32_t ls = dpad + 4;
+    const bool in_lds = (size_t)T * ls * sizeof(float) <= FK_JOIN_LDS_BYTES;
+    if (in_lds) {
+        const uint32_t v4 = dpad >> 2;  // float4 per row
+        for (uint32_t i = threadIdx.x; i < T * v4; i += blockDim.x) {
+            const uint32_t r = i / v4, c = i - r * v4;
+            const float4 v = *reinterpret_cast<const float4 *>(rows + (size_t)lst[r] * stride + c * 4);
+            *reinterpret_cast<float4 *>(s_rows + r * ls + c * 4) = v;
+        }
+        __syncthreads();
+    }
+    const uint64_t cells = (uint64_t)nn * T;
+    for (uint64_t c = threadIdx.x; c < cells; c += blockDim.x) {
+        const uint32_t x = (uint32_t)(c / T), y = (uint32_t)(c - (uint64_t)x * T);
+        if (y <= x) continue;
+        const float *ra = in_lds ? s_rows + x * ls : rows + (size_t)lst[x] * stride;
+        const float *rb = in_lds ? s_rows + y * ls : rows + (size_t)lst[y] * stride;
+        const uint64_t rs = (uint64_t)x * (T - 1) - (uint64_t)x * (x - 1) / 2;
+        o[rs + (y - x - 1)] = l2_canon(ra, rb, dpad);
+    }
+}
+
+void launch_join_distances(hipStream_t s, const Dataset &ds, const uint32_t *ids, const uint64_t *list_off,
+                           const uint32_t *n_new, const uint64_t *pair_off, uint32_t n_nodes, float *out) {
+    if (n_nodes == 0) return;
+    hipLaunchKernelGGL(k_join_distances, dim3(n_nodes), dim3(256), FK_JOIN_LDS_BYTES, s, ds.rows.p, ds.stride,
+                       ds.dpad, ids, list_off, n_new, pair_off, out);
+    FK_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Occlusion prune, one wavefront per pool.
+//
+// Sequential-accept semantics of *_sync_prune (src/kgraph.cpp:1809-1851):
+// pool[start] is always kept (start skips a leading self entry); every later
+// candidate at position < min(len, maxc) is examined in order until nsg_R are
+// kept; it is dropped if an already kept entry has its id or occludes it.
+// The wave takes 64 candidates at a time, one per lane: every lane checks its
+// candidate against the kept list (in kept order, stopping at the first
+// occluder, as the reference does), then the lowest surviving lane is
+// accepted and only the lanes above it check the new member, repeatedly.
+// That reproduces the reference's decisions and its distance count exactly.
+// ---------------------------------------------------------------------------
+#define FK_PRUNE_WPB 4
+#define FK_PRUNE_RMAX 256
+
+template <int RULE>
+__global__ __launch_bounds__(256) void k_prune(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
+                                               PruneLaunch a) {
+    __shared__ uint32_t s_oid[FK_PRUNE_WPB][FK_PRUNE_RMAX];
+    __shared__ float s_od[FK_PRUNE_WPB][FK_PRUNE_RMAX];
+    const uint32_t lane = lane_id();
+    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t task = blockIdx.x * FK_PRUNE_WPB + w;
+    if (task >= a.n_tasks) return;
+    const uint32_t slot = a.task_list ? a.task_list[task] : task;
+    const uint64_t *c = a.cand + (a.cand_off ? a.cand_off[slot] : (uint64_t)slot * a.cand_stride);
+    const uint32_t len = a.cand_len[slot];
+    const uint32_t q = a.qid ? a.qid[slot] : slot;
+    uint32_t *oid = a.out_ids + (uint64_t)slot * a.out_stride;
+    float *od = a.out_d ? a.out_d + (uint64_t)slot * a.out_stride : nullptr;
+    uint32_t *olen = a.out_len + slot;
+    uint32_t *so = s_oid[w];
+    float *sd = s_od[w];
+
+    uint32_t start = 0;
+    if (len > 0 && key_id(c[0]) == q) start = 1;
+    if (start >= len || a.R == 0) {
+        if (lane == 0) *olen = 0;
+        return;
+    }
+    uint32_t nout = 1;
+    {
+        const uint64_t k0 = c[start];
+        if (lane == 0) { so[0] = key_id(k0); sd[0] = key_dist(k0); }
+    }
+    const uint32_t limit = len < a.maxc ? len : a.maxc;
+    uint32_t pos = start + 1;
+    unsigned long long cc = 0;
+    bool stop = nout >= a.R;
+    while (!stop && pos < limit) {
+        const uint32_t nb = (limit - pos) < 64u ? (limit - pos) : 64u;
+        const bool valid = lane < nb;
+        const uint64_t key = valid ? c[pos + lane] : FK_KEY_EMPTY;
+        const uint32_t pid = key_id(key);
+        const float pd = key_dist(key);
+        const bool skip = !valid || pid >= a.oracle_n;
+        const float *prow = rows + (size_t)(skip ? 0u : pid) * stride;
+        bool occl = false;
+        uint32_t mycc = 0;
+        __builtin_amdgcn_wave_barrier();
+        if (!skip) {
+            for (uint32_t t = 0; t < nout; ++t) {
+                const uint32_t tid = so[t];
+                if (tid == pid) { occl = true; break; }
+                ++mycc;
+                const float djk = l2_canon(rows + (size_t)tid * stride, prow, dpad);
+                if (occludes<RULE>(djk, pd, sd[t], a.tau3, a.cos_thr)) { occl = true; break; }
+            }
+        }
+        uint32_t done = 0;
+        uint32_t stop_lane = 63;
+        while (true) {
+            const uint64_t m = wave_ballot(!skip && !occl && lane >= done);
+            if (m == 0) break;
+            const uint32_t al = (uint32_t)__builtin_ctzll(m);
+            const uint32_t aid = __shfl(pid, al, 64);
+            const float ad = __shfl(pd, al, 64);
+            if (lane == 0) { so[nout] = aid; sd[nout] = ad; }
+            ++nout;
+            done = al + 1;
+            if (nout >= a.R) { stop = true; stop_lane = al; break; }
+            if (!skip && !occl && lane > al) {
+                if (aid == pid) {
+                    occl = true;
+                } else {
+                    ++mycc;
+                    const float djk = l2_canon(rows + (size_t)aid * stride, prow, dpad);
+                    if (occludes<RULE>(djk, pd, ad, a.tau3, a.cos_thr)) occl = true;
+                }
+            }
+        }
+        if (lane <= stop_lane) cc += mycc;
+        pos += nb;
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t k = lane; k < nout; k += 64) {
+        oid[k] = so[k];
+        if (od) od[k] = sd[k];
+    }
+    if (lane == 0) *olen = nout;
+    if (a.cc) {
+        cc = wave_sum(cc);
+        if (lane == 0 && cc) atomicAdd(a.cc, cc);
+    }
+}
+
+void launch_prune(hipStream_t s, const Dataset &ds, const PruneLaunch &a) {
+    if (a.n_tasks == 0) return;
+    FK_CHECK_ARG(a.R <= FK_PRUNE_RMAX, "nsg_R above the supported maximum (256)");
+    const unsigned grid = (a.n_tasks + FK_PRUNE_WPB - 1) / FK_PRUNE_WPB;
+    switch (a.rule) {
+        case FK_RULE_NSG:
+            hipLaunchKernelGGL(k_prune<FK_RULE_NSG>, dim3(grid), dim3(64 * FK_PRUNE_WPB), 0, s, ds.rows.p,
+                               ds.stride, ds.dpad, a);
+            break;
+        case FK_RULE_TAU:
+            hipLaunchKernelGGL(k_prune<FK_RULE_TAU>, dim3(grid), dim3(64 * FK_PRUNE_WPB), 0, s, ds.rows.p,
+                               ds.stride, ds.dpad, a);
+            break;
+        default:
+            hipLaunchKernelGGL(k_prune<FK_RULE_ALPHA>, dim3(grid), dim3(64 * FK_PRUNE_WPB), 0, s, ds.rows.p,
+                               ds.stride, ds.dpad, a);
+            break;
+    }
+    FK_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive prefix sums (three phases: block sums, scan of block sums, fix-up).
+// ---------------------------------------------------------------------------
+#define FK_SCAN_BLOCK 256
+#define FK_SCAN_PER_THREAD 8
+#define FK_SCAN_TILE (FK_SCAN_BLOCK * FK_SCAN_PER_THREAD)
+
+template <typename T>
+__device__ uint64_t block_exclusive_scan(uint64_t v, uint64_t *s_tmp, uint64_t *total) {
+    // Hillis-Steele over the block in LDS.
+    s_tmp[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t o = 1; o < blockDim.x; o <<= 1) {
+        const uint64_t t = threadIdx.x >= o ? s_tmp[threadIdx.x - o] : 0;
+        __syncthreads();
+        s_tmp[threadIdx.x] += t;
+        __syncthreads();
+    }
+    const uint64_t incl = s_tmp[threadIdx.x];
+    *total = s_tmp[blockDim.x - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+template <typename TI>
+__global__ __launch_bounds__(FK_SCAN_BLOCK) void k_scan_partials(const TI *__restrict__ in, size_t n,
+                                                                 uint64_t *__restrict__ part) {
+    __shared__ uint64_t s_tmp[FK_SCAN_BLOCK];
+    const size_t base = (size_t)blockIdx.x * FK_SCAN_TILE + (size_t)threadIdx.x * FK_SCAN_PER_THREAD;
+    uint64_t v = 0;
+    for (int k = 0; k < FK_SCAN_PER_THREAD; ++k)
+        if (base + k < n) v += (uint64_t)in[base + k];
+    uint64_t tot;
+    block_exclusive_scan<uint64_t>(v, s_tmp, &tot);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(FK_SCAN_BLOCK) void k_scan_single(uint64_t *__restrict__ part, size_t n,
+                                                               uint64_t *__restrict__ total) {
+    __shared__ uint64_t s_tmp[FK_SCAN_BLOCK];
+    uint64_t carry = 0;
+    for (size_t base = 0; base < n; base += FK_SCAN_BLOCK) {
+        const size_t i = base + threadIdx.x;
+        const uint64_t v = i < n ? part[i] : 0;
+        uint64_t tot;
+        const uint64_t ex = block_exclusive_scan<uint64_t>(v, s_tmp, &tot);
+        if (i < n) part[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+template <typename TI>
+__global__ __launch_bounds__(FK_SCAN_BLOCK) void k_scan_apply(const TI *__restrict__ in, size_t n,
+                                                              const uint64_t *__restrict__ part,
+                                                              uint64_t *__restrict__ out) {
+    __shared__ uint64_t s_tmp[FK_SCAN_BLOCK];
+    const size_t base = (size_t)blockIdx.x * FK_SCAN_TILE + (size_t)threadIdx.x * FK_SCAN_PER_THREAD;
+    TI vals[FK_SCAN_PER_THREAD];
+    uint64_t v = 0;
+    for (int k = 0; k < FK_SCAN_PER_THREAD; ++k) {
+        vals[k] = base + k < n ? in[base + k] : (TI)0;
+        v += (uint64_t)vals[k];
+    }
+    uint64_t tot;
+    uint64_t run = block_exclusive_scan<uint64_t>(v, s_tmp, &tot) + part[blockIdx.x];
+    for (int k = 0; k < FK_SCAN_PER_THREAD; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += (uint64_t)vals[k];
+    }
+}
+
+template <typename TI>
+static void exclusive_scan_impl(hipStream_t s, const TI *in, uint64_t *out, size_t n, uint64_t *total,
+                                DevBuf<uint64_t> &part) {
+    if (n == 0) {
+        if (total) FK_HIP(hipMemsetAsync(total, 0, sizeof(uint64_t), s));
+        return;
+    }
+    const size_t nb = (n + FK_SCAN_TILE - 1) / FK_SCAN_TILE;
+    if (part.n < nb) {
+        FK_HIP(hipStreamSynchronize(s));  // the old scratch may still be in use
+        part.alloc(nb);
+    }
+    hipLaunchKernelGGL(k_scan_partials<TI>, dim3((unsigned)nb), dim3(FK_SCAN_BLOCK), 0, s, in, n, part.p);
+    FK_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(FK_SCAN_BLOCK), 0, s, part.p, nb, total);
+    FK_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_apply<TI>, dim3((unsigned)nb), dim3(FK_SCAN_BLOCK), 0, s, in, n, part.p, out);
+    FK_LAUNCH_CHECK();
+}
+
+void exclusive_scan_u32_to_u64(hipStream_t s, const uint32_t *in, uint64_t *out, size_t n, uint64_t *total,
+                               DevBuf<uint64_t> &tmp) {
+    exclusive_scan_impl<uint32_t>(s, in, out, n, total, tmp);
+}
+void exclusive_scan_u64(hipStream_t s, const uint64_t *in, uint64_t *out, size_t n, uint64_t *total,
+                        DevBuf<uint64_t> &tmp) {
+    exclusive_scan_impl<uint64_t>(s, in, out, n, total, tmp);
+}
+
+// ---------------------------------------------------------------------------
+// Brute-force exact kNN.
+// Phase 1: keys[q][i] = (dist(q, row i), i) for a batch of queries; a block
+// stages 64 data rows in LDS and sweeps the batch's queries 4 at a time.
+// Phase 2: one block per query finds the K-th smallest 64-bit key by an
+// 8-pass byte radix select, gathers the K keys <= it and sorts them.
+// ---------------------------------------------------------------------------
+#define FK_BF_ROWS 64
+
+__global__ __launch_bounds__(256) void k_bf_keys(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
+                                                 uint32_t n_used, const float *__restrict__ qv,
+                                                 uint32_t q_stride, uint32_t n_q,
+                                                 const uint32_t *__restrict__ self_ids,
+                                                 uint64_t *__restrict__ keys) {
+    extern __shared__ __align__(16) float s_rows[];
+    const uint32_t ls = dpad + 4;
+    const uint32_t r0 = blockIdx.x * FK_BF_ROWS;
+    const uint32_t nr = (n_used - r0) < FK_BF_ROWS ? (n_used - r0) : FK_BF_ROWS;
+    const uint32_t v4 = dpad >> 2;
+    for (uint32_t i = threadIdx.x; i < nr * v4; i += blockDim.x) {
+        const uint32_t r = i / v4, c = i - r * v4;
+        *reinterpret_cast<float4 *>(s_rows + r * ls + c * 4) =
+            *reinterpret_cast<const float4 *>(rows + (size_t)(r0 + r) * stride + c * 4);
+    }
+    __syncthreads();
+    const uint32_t r = threadIdx.x & 63u;
+    for (uint32_t q = threadIdx.x >> 6; q < n_q; q += 4) {
+        if (r < nr) {
+            const uint32_t id = r0 + r;
+            uint64_t k;
+            if (self_ids && self_ids[q] == id) k = FK_KEY_EMPTY;
+            else k = make_key(l2_canon(qv + (size_t)q * q_stride, s_rows + r * ls, dpad), id);
+            keys[(size_t)q * n_used + id] = k;
+        }
+    }
+}
+
+#define FK_SEL_THREADS 1024
+#define FK_SEL_MAXK 2048
+
+__global__ __launch_bounds__(FK_SEL_THREADS) void k_topk_select(const uint64_t *__restrict__ keys, uint32_t n,
+                                                                uint32_t K, uint32_t *__restrict__ out_ids,
+                                                                float *__restrict__ out_d) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint64_t s_sel[FK_SEL_MAXK];
+    __shared__ uint32_t s_cnt;
+    __shared__ uint64_t s_prefix, s_mask;
+    __shared__ uint32_t s_need;
+    const uint64_t *kq = keys + (size_t)blockIdx.x * n;
+    if (threadIdx.x == 0) { s_prefix = 0; s_mask = 0; s_need = K; s_cnt = 0; }
+    __syncthreads();
+    for (int byte = 7; byte >= 0; --byte) {
+        for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+        __syncthreads();
+        const uint64_t prefix = s_prefix, mask = s_mask;
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const uint64_t k = kq[i];
+            if ((k & mask) == prefix) atomicAdd(&hist[(uint32_t)(k >> (8 * byte)) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t need = s_need, b = 0;
+            for (; b < 256; ++b) {
+                if (hist[b] >= need) break;
+                need -= hist[b];
+            }
+            if (b == 256) b = 255;  // fewer than K keys: keep everything
+            s_prefix = prefix | ((uint64_t)b << (8 * byte));
+            s_mask = mask | (0xFFull << (8 * byte));
+            s_need = need;
+        }
+        __syncthreads();
+    }
+    const uint64_t kth = s_prefix;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint64_t k = kq[i];
+        if (k <= kth && k != FK_KEY_EMPTY) {
+            const uint32_t p = atomicAdd(&s_cnt, 1u);
+            if (p < FK_SEL_MAXK) s_sel[p] = k;
+        }
+    }
+    __syncthreads();
+    const uint32_t cnt = s_cnt < FK_SEL_MAXK ? s_cnt : FK_SEL_MAXK;
+    const uint32_t np = next_pow2(cnt < 2 ? 2 : cnt);
+    for (uint32_t i = cnt + threadIdx.x; i < np; i += blockDim.x) s_sel[i] = FK_KEY_EMPTY;
+    __syncthreads();
+    block_bitonic_sort(s_sel, np);
+    for (uint32_t i = threadIdx.x; i < K; i += blockDim.x) {
+        const uint64_t k = i < cnt ? s_sel[i] : FK_KEY_EMPTY;
+        out_ids[(size_t)blockIdx.x * K + i] = key_id(k);
+        if (out_d) out_d[(size_t)blockIdx.x * K + i] = key_dist(k);
+    }
+}
+
+void bruteforce_knn(hipStream_t s, const Dataset &ds, const float *d_queries, uint32_t q_stride, uint32_t n_q,
+                    uint32_t n_used, uint32_t K, const uint32_t *d_self, uint32_t *d_out_ids, float *d_out_dists) {
+    FK_CHECK_ARG(K >= 1 && K <= FK_SEL_MAXK, "K must be in [1, 2048]");
+    FK_CHECK_ARG(n_used >= 1, "empty dataset");
+    const size_t lds = (size_t)FK_BF_ROWS * (ds.dpad + 4) * sizeof(float);
+    FK_CHECK_ARG(lds <= 64 * 1024, "dimension too large for the brute-force kernel (d <= 252)");
+    // Batch queries so the key matrix stays within ~2 GiB.
+    size_t qb = ((size_t)2 << 30) / ((size_t)n_used * 8);
+    if (qb < 1) qb = 1;
+    if (qb > n_q) qb = n_q;
+    DevBuf<uint64_t> keys(qb * (size_t)n_used);
+    for (uint32_t q0 = 0; q0 < n_q; q0 += (uint32_t)qb) {
+        const uint32_t nq = (uint32_t)((n_q - q0) < qb ? (n_q - q0) : qb);
+        const unsigned gx = (n_used + FK_BF_ROWS - 1) / FK_BF_ROWS;
+        hipLaunchKernelGGL(k_bf_keys, dim3(gx), dim3(256), lds, s, ds.rows.p, ds.stride, ds.dpad, n_used,
+                           d_queries + (size_t)q0 * q_stride, q_stride, nq, d_self ? d_self + q0 : nullptr,
+                           keys.p);
+        FK_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_topk_select, dim3(nq), dim3(FK_SEL_THREADS), 0, s, keys.p, n_used, K,
+                           d_out_ids + (size_t)q0 * K, d_out_dists ? d_out_dists + (size_t)q0 * K : nullptr);
+        FK_LAUNCH_CHECK();
+    }
+    FK_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace fk

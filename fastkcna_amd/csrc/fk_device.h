@@ -1,0 +1,114 @@
+// Device-side building blocks shared by every kernel of the engine.
+//
+// Pool entries are packed 64-bit keys: the float32 squared distance's bit
+// pattern in the high word and the node id in the low word.  Squared L2 is
+// never negative, so the unsigned order of the key is the (dist, id) order
+// that the engine uses everywhere as its canonical tie order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define FK_WAVE 64
+#define FK_KEY_EMPTY 0xFFFFFFFFFFFFFFFFull
+
+__device__ __forceinline__ uint32_t key_id(uint64_t k) { return (uint32_t)k; }
+__device__ __forceinline__ float key_dist(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
+__device__ __forceinline__ uint64_t make_key(float d, uint32_t id) {
+    return ((uint64_t)__float_as_uint(d) << 32) | (uint64_t)id;
+}
+
+// One 8-float step of the canonical squared-L2: lane k of the reference's
+// 256-bit accumulator gets fma(a[k]-b[k], a[k]-b[k], acc[k]).  g++ -O3
+// -march=native contracts the sub/mul/add macro of src/metric.cpp:37-42 into
+// exactly this fused form.
+__device__ __forceinline__ void l2_step8(const float *__restrict__ a, const float *__restrict__ b,
+                                         float acc[8]) {
+    const float4 a0 = *reinterpret_cast<const float4 *>(a);
+    const float4 a1 = *reinterpret_cast<const float4 *>(a + 4);
+    const float4 b0 = *reinterpret_cast<const float4 *>(b);
+    const float4 b1 = *reinterpret_cast<const float4 *>(b + 4);
+    float t;
+    t = a0.x - b0.x; acc[0] = __builtin_fmaf(t, t, acc[0]);
+    t = a0.y - b0.y; acc[1] = __builtin_fmaf(t, t, acc[1]);
+    t = a0.z - b0.z; acc[2] = __builtin_fmaf(t, t, acc[2]);
+    t = a0.w - b0.w; acc[3] = __builtin_fmaf(t, t, acc[3]);
+    t = a1.x - b1.x; acc[4] = __builtin_fmaf(t, t, acc[4]);
+    t = a1.y - b1.y; acc[5] = __builtin_fmaf(t, t, acc[5]);
+    t = a1.z - b1.z; acc[6] = __builtin_fmaf(t, t, acc[6]);
+    t = a1.w - b1.w; acc[7] = __builtin_fmaf(t, t, acc[7]);
+}
+
+// Canonical squared L2 of two zero-padded rows of dpad floats (dpad % 8 == 0,
+// 16-byte aligned), in the exact order of float_l2sqr_avx
+// (src/metric.cpp:45-78): the dpad % 32 tail blocks first (offsets +16, +8,
+// +0 past the last full 32-block), then every 8-block from 0 upward, each
+// into 8 lane accumulators; then the lanes summed left to right.  The result
+// is bit-identical to the reference's distance and symmetric in (a, b).
+__device__ __forceinline__ float l2_canon(const float *__restrict__ a, const float *__restrict__ b,
+                                          uint32_t dpad) {
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const uint32_t dr = dpad & 31u;
+    const uint32_t dd = dpad - dr;
+    if (dr == 24u) {
+        l2_step8(a + dd + 16, b + dd + 16, acc);
+        l2_step8(a + dd + 8, b + dd + 8, acc);
+        l2_step8(a + dd, b + dd, acc);
+    } else if (dr == 16u) {
+        l2_step8(a + dd + 8, b + dd + 8, acc);
+        l2_step8(a + dd, b + dd, acc);
+    } else if (dr == 8u) {
+        l2_step8(a + dd, b + dd, acc);
+    }
+    for (uint32_t i = 0; i < dd; i += 8) l2_step8(a + i, b + i, acc);
+    float r = acc[0] + acc[1];
+    r += acc[2];
+    r += acc[3];
+    r += acc[4];
+    r += acc[5];
+    r += acc[6];
+    r += acc[7];
+    return r;
+}
+
+// Occlusion rules of the three pruner families; dqt is the kept neighbour's
+// distance to the pool owner, djk the kept neighbour to candidate distance,
+// pd the candidate's distance to the owner.
+//   NSG / HNSW layer: src/kgraph.cpp:1836     djk < p.dist
+//   tau-MNG:          src/kgraph.cpp:2859     djk < p.dist - 3*tau
+//   alpha-PG:         src/kgraph.cpp:3184-3187
+//        cos = (dqt + djk - p.dist) / 2 / sqrt(dqt * djk); djk < p.dist && cos < thr
+enum { FK_RULE_NSG = 0, FK_RULE_TAU = 1, FK_RULE_ALPHA = 2 };
+
+template <int RULE>
+__device__ __forceinline__ bool occludes(float djk, float pd, float dqt, float tau3, float cos_thr) {
+    if (RULE == FK_RULE_NSG) return djk < pd;
+    if (RULE == FK_RULE_TAU) return djk < pd - tau3;
+    float num = dqt + djk;
+    num = num - pd;
+    num = num / 2.0f;
+    const float den = __builtin_sqrtf(dqt * djk);
+    const float c = num / den;
+    return (djk < pd) && (c < cos_thr);
+}
+
+// Wave-wide helpers (wave64).
+__device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (FK_WAVE - 1); }
+
+// Counter-based RNG: splitmix64 finaliser over a 64-bit counter.
+__device__ __host__ __forceinline__ uint64_t fk_mix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__device__ __host__ __forceinline__ uint32_t fk_rand32(uint64_t seed, uint64_t stream, uint64_t k) {
+    return (uint32_t)(fk_mix64(seed * 0xD1B54A32D192ED03ull ^ fk_mix64(stream * 0x8CB92BA72F3D8DD7ull + k)) >> 32);
+}

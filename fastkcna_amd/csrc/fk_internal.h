@@ -1,0 +1,135 @@
+// Host-side internals of the engine: the handle, device buffers, error
+// plumbing and the kernel launch wrappers shared by the C-ABI translation
+// units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fk_api.h"
+
+namespace fk {
+
+// Thrown inside the library, converted to an fk_status at the C boundary.
+struct Error : std::runtime_error {
+    fk_status code;
+    Error(fk_status c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define FK_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw ::fk::Error(e_ == hipErrorOutOfMemory ? FK_E_NOMEM : FK_E_DEVICE,       \
+                              std::string(#call) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+#define FK_CHECK_ARG(cond, msg)                                                           \
+    do {                                                                                  \
+        if (!(cond)) throw ::fk::Error(FK_E_INVALID, msg);                                \
+    } while (0)
+
+#define FK_LAUNCH_CHECK() FK_HIP(hipGetLastError())
+
+// Owning device allocation; freed on scope exit.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        FK_HIP(hipMalloc((void **)&p, count * sizeof(T)));
+        n = count;
+    }
+    // Grow-only reallocation (contents discarded).
+    void ensure(size_t count) {
+        if (count > n || p == nullptr) alloc(count);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+struct Dataset {
+    DevBuf<float> rows;   // n_total x stride floats, zero padded
+    uint32_t n_total = 0;
+    uint32_t d = 0;
+    uint32_t dpad = 0;    // d rounded up to a multiple of 8
+    uint32_t stride = 0;  // floats between rows (multiple of 8)
+};
+
+}  // namespace fk
+
+struct fk_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    fk::Dataset ds;
+    fk::DevBuf<unsigned long long> counters;  // small scratch for kernel counters
+};
+
+namespace fk {
+
+// Launch wrappers implemented in the kernel translation units.  All launch on
+// the handle's stream and take device pointers.
+void launch_pair_distances(hipStream_t s, const Dataset &ds, const uint32_t *a, const uint32_t *b,
+                           size_t n, float *out);
+
+struct PruneLaunch {
+    const uint64_t *cand = nullptr;   // per-task sorted candidate keys
+    const uint32_t *cand_len = nullptr;
+    uint64_t cand_stride = 0;         // fixed stride, used when cand_off is null
+    const uint64_t *cand_off = nullptr;  // optional CSR offsets per slot
+    const uint32_t *qid = nullptr;    // owner node per task (nullptr: task index)
+    const uint32_t *task_list = nullptr;  // optional indirection: task -> pool slot
+    uint32_t n_tasks = 0;
+    uint32_t oracle_n = 0;            // ids >= oracle_n are skipped
+    uint32_t maxc = 0;                // examined positions < min(len, maxc)
+    uint32_t R = 0;
+    float tau3 = 0.f, cos_thr = 0.f;
+    int rule = 0;
+    uint32_t *out_ids = nullptr;
+    float *out_d = nullptr;
+    uint64_t out_stride = 0;
+    uint32_t *out_len = nullptr;
+    unsigned long long *cc = nullptr;  // accumulated distance count
+};
+void launch_prune(hipStream_t s, const Dataset &ds, const PruneLaunch &a);
+
+void launch_join_distances(hipStream_t s, const Dataset &ds, const uint32_t *ids,
+                           const uint64_t *list_off, const uint32_t *n_new, const uint64_t *pair_off,
+                           uint32_t n_nodes, float *out);
+
+// Exclusive prefix sums over device arrays (out may alias in).  `total`
+// (device pointer, may be nullptr) receives the sum; `tmp` is grown as needed
+// and must stay alive until the stream has run the scan.
+void exclusive_scan_u32_to_u64(hipStream_t s, const uint32_t *in, uint64_t *out, size_t n,
+                               uint64_t *total, DevBuf<uint64_t> &tmp);
+void exclusive_scan_u64(hipStream_t s, const uint64_t *in, uint64_t *out, size_t n, uint64_t *total,
+                        DevBuf<uint64_t> &tmp);
+
+// Brute-force exact kNN of device queries against rows [0, n_used).
+void bruteforce_knn(hipStream_t s, const Dataset &ds, const float *d_queries, uint32_t q_stride,
+                    uint32_t n_q, uint32_t n_used, uint32_t K, const uint32_t *d_self,
+                    uint32_t *d_out_ids, float *d_out_dists);
+
+}  // namespace fk

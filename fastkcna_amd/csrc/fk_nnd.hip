@@ -1,0 +1,984 @@
+// NN-descent on the GPU: random init, the local join as a distance block per
+// node with threshold-filtered candidate buckets, a per-node merge of the
+// buckets into the sorted pools, the new/old/reverse resampling step, the
+// stop rule over sampled ground truth, and the BridgeView (P2K) rounds.
+//
+// Reference: KGraphConstructor (src/kgraph.cpp:924-1459), BridgeView
+// (src/kgraph.cpp:2478-2663).
+//
+// Concurrency model.  The reference inserts every join pair into two pools
+// under per-node spinlocks while other threads keep joining.  Here a join
+// never touches a pool: each pair that passes the target's threshold (the
+// pool's L-th distance at iteration start, the reference's `radius` filter)
+// becomes a bucket entry for the target, and one workgroup per target then
+// merges its bucket into its pool.  The merged pool is the L best distinct
+// ids of (pool U candidates) in (dist, id) order, which is what the
+// reference's sorted insert converges to up to the order of equal distances,
+// and it is deterministic.
+#include <math.h>
+
+#include <algorithm>
+#include <chrono>
+
+#include "fk_build.h"
+#include "fk_device.h"
+#include "fk_sort.h"
+
+namespace fk {
+
+double now_sec() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void BuildState::sync() { FK_HIP(hipStreamSynchronize(s)); }
+
+// ---------------------------------------------------------------------------
+// Block-wide exclusive scan of n u32 values in LDS (in place); returns total.
+// All threads must call.  s_part needs blockDim.x entries.
+// ---------------------------------------------------------------------------
+__device__ uint32_t block_scan_inplace(uint32_t *v, uint32_t n, uint32_t *s_part) {
+    const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
+    const uint32_t b0 = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < per; ++k)
+        if (b0 + k < n) sum += v[b0 + k];
+    s_part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t o = 1; o < blockDim.x; o <<= 1) {
+        const uint32_t t = threadIdx.x >= o ? s_part[threadIdx.x - o] : 0u;
+        __syncthreads();
+        s_part[threadIdx.x] += t;
+        __syncthreads();
+    }
+    const uint32_t total = s_part[blockDim.x - 1];
+    uint32_t run = s_part[threadIdx.x] - sum;
+    for (uint32_t k = 0; k < per; ++k)
+        if (b0 + k < n) {
+            const uint32_t x = v[b0 + k];
+            v[b0 + k] = run;
+            run += x;
+        }
+    __syncthreads();
+    return total;
+}
+
+// ---------------------------------------------------------------------------
+// Merge of candidate keys into a pool held in LDS.
+//   s_p / s_pf : pool keys and flags, *s_np entries, sorted ascending
+//   s_c        : nc candidate keys sorted ascending; s_keep[k] = 1 marks the
+//                candidates to insert (distinct and absent from the pool)
+// Keeps the first `lim` entries of the union; inserted entries get flag
+// `newf`.  All threads must call.
+// ---------------------------------------------------------------------------
+#define FK_MRG_CH 1024
+static_assert(FK_MRG_CH <= FK_POOL_MAX, "merge chunk must fit the pool-sized scratch");
+
+__device__ void block_merge(uint64_t *s_p, uint8_t *s_pf, uint32_t *s_np, const uint64_t *s_c, uint32_t *s_keep,
+                            uint32_t nc, uint32_t lim, uint8_t newf, uint64_t *s_c2, uint64_t *s_o, uint8_t *s_of,
+                            uint32_t *s_part) {
+    const uint32_t np = *s_np;
+    // keep flags are parked in s_of while s_keep turns into positions
+    for (uint32_t k = threadIdx.x; k < nc; k += blockDim.x) s_of[k] = (uint8_t)(s_keep[k] ? 1 : 0);
+    __syncthreads();
+    const uint32_t nk = block_scan_inplace(s_keep, nc, s_part);
+    for (uint32_t k = threadIdx.x; k < nc; k += blockDim.x)
+        if (s_of[k]) s_c2[s_keep[k]] = s_c[k];
+    __syncthreads();
+    // rank-merge pool and kept candidates into s_o / s_of; no key occurs in
+    // both lists, so the two rank computations never collide
+    for (uint32_t j = threadIdx.x; j < np; j += blockDim.x) {
+        const uint32_t pos = j + lower_bound_cnt(s_c2, nk, s_p[j]);
+        if (pos < lim) { s_o[pos] = s_p[j]; s_of[pos] = s_pf[j]; }
+    }
+    for (uint32_t k = threadIdx.x; k < nk; k += blockDim.x) {
+        const uint32_t pos = k + lower_bound_cnt(s_p, np, s_c2[k]);
+        if (pos < lim) { s_o[pos] = s_c2[k]; s_of[pos] = newf; }
+    }
+    __syncthreads();
+    const uint32_t nn = (np + nk) < lim ? (np + nk) : lim;
+    for (uint32_t j = threadIdx.x; j < nn; j += blockDim.x) { s_p[j] = s_o[j]; s_pf[j] = s_of[j]; }
+    __syncthreads();
+    if (threadIdx.x == 0) *s_np = nn;
+    __syncthreads();
+}
+
+__device__ __forceinline__ bool sorted_contains(const uint64_t *a, uint32_t n, uint64_t x) {
+    const uint32_t i = lower_bound_cnt(a, n, x);
+    return i < n && a[i] == x;
+}
+
+// ---------------------------------------------------------------------------
+// Random init (src/kgraph.cpp:1001-1044).  GenRandom (:82-110): `size`
+// distinct values in [0, N) -- draw in [0, N - size), sort, make strictly
+// increasing, rotate by a random offset.  One 64-thread block per node.
+// ---------------------------------------------------------------------------
+__device__ void gen_random_block(uint32_t *a, uint32_t size, uint32_t N, uint64_t seed, uint64_t stream) {
+    if (size >= N) {
+        for (uint32_t i = threadIdx.x; i < size; i += blockDim.x) a[i] = i % N;
+        __syncthreads();
+        return;
+    }
+    const uint32_t np = next_pow2(size);
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x)
+        a[i] = i < size ? fk_rand32(seed, stream, i) % (N - size) : 0xFFFFFFFFu;
+    __syncthreads();
+    block_bitonic_sort(a, np);
+    if (threadIdx.x == 0) {
+        for (uint32_t i = 1; i < size; ++i)
+            if (a[i] <= a[i - 1]) a[i] = a[i - 1] + 1;
+    }
+    __syncthreads();
+    const uint64_t off = fk_rand32(seed, stream, size) % N;
+    for (uint32_t i = threadIdx.x; i < size; i += blockDim.x) a[i] = (uint32_t)(((uint64_t)a[i] + off) % N);
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(64) void k_nnd_init(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
+                                                 uint32_t N, uint32_t S, uint32_t cap, uint32_t Lp, uint64_t seed,
+                                                 uint64_t *__restrict__ pool, uint8_t *__restrict__ pflg,
+                                                 uint32_t *__restrict__ cnt, uint32_t *__restrict__ Marr,
+                                                 float *__restrict__ thr, uint32_t *__restrict__ lists) {
+    __shared__ uint32_t s_a[512];
+    __shared__ uint64_t s_k[256];
+    const uint32_t n = blockIdx.x;
+    // nn_new: 2S random ids (may include n itself, as in the reference)
+    gen_random_block(s_a, 2 * S, N, seed, 2ull * n);
+    for (uint32_t i = threadIdx.x; i < 2 * S; i += blockDim.x) lists[(uint64_t)n * 2 * S + i] = s_a[i];
+    __syncthreads();
+    // pool: S random ids != n from S + 1 draws
+    gen_random_block(s_a, S + 1, N, seed, 2ull * n + 1);
+    if (threadIdx.x == 0) {
+        uint32_t i = 0;
+        for (uint32_t l = 0; l < S; ++l) {
+            if (s_a[i] == n) ++i;
+            s_a[256 + l] = s_a[i++];
+        }
+    }
+    __syncthreads();
+    const uint32_t np = next_pow2(S < 2 ? 2 : S);
+    for (uint32_t l = threadIdx.x; l < np; l += blockDim.x) {
+        if (l < S) {
+            const uint32_t id = s_a[256 + l];
+            s_k[l] = make_key(l2_canon(rows + (size_t)id * stride, rows + (size_t)n * stride, dpad), id);
+        } else {
+            s_k[l] = FK_KEY_EMPTY;
+        }
+    }
+    __syncthreads();
+    block_bitonic_sort(s_k, np);
+    uint64_t *pr = pool + (uint64_t)n * cap;
+    uint8_t *pf = pflg + (uint64_t)n * cap;
+    for (uint32_t l = threadIdx.x; l < S; l += blockDim.x) {
+        pr[l] = s_k[l];
+        pf[l] = FK_FLG_NEW | FK_FLG_ISNEW;
+    }
+    if (threadIdx.x == 0) {
+        cnt[n] = S;
+        Marr[n] = S;
+        thr[n] = S >= Lp ? key_dist(s_k[Lp - 1]) : INFINITY;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pair counts per node: C(nn, 2) + nn * no.
+// ---------------------------------------------------------------------------
+__global__ void k_pair_counts(const uint64_t *__restrict__ list_off, const uint32_t *__restrict__ n_new, uint32_t N,
+                              uint32_t *__restrict__ out) {
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const uint32_t T = (uint32_t)(list_off[n + 1] - list_off[n]);
+    const uint32_t nn = n_new[n];
+    out[n] = nn * (nn ? nn - 1 : 0) / 2 + nn * (T - nn);
+}
+
+// ---------------------------------------------------------------------------
+// Local join, pass 1: pair distances (kept for pass 2), per-target candidate
+// counts and one bucket reservation per (join, target).
+// src/kgraph.cpp:970-987 enumerates new x new (i < j) and new x old; a pair
+// joins j into pool(i) and i into pool(j).  Pairs of equal ids add nothing.
+// ---------------------------------------------------------------------------
+#define FK_JOIN_LDS_ROWS_BYTES (40 * 1024)
+
+__global__ __launch_bounds__(256) void k_join_count(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
+                                                    const uint32_t *__restrict__ lists,
+                                                    const uint64_t *__restrict__ list_off,
+                                                    const uint32_t *__restrict__ n_new,
+                                                    const uint64_t *__restrict__ pair_off,
+                                                    const float *__restrict__ thr, float *__restrict__ pdist,
+                                                    uint32_t *__restrict__ resv, uint32_t *__restrict__ bcnt) {
+    extern __shared__ __align__(16) float s_rows[];
+    __shared__ float s_thr[FK_LIST_MAX];
+    __shared__ uint32_t s_c[FK_LIST_MAX];
+    const uint32_t node = blockIdx.x;
+    const uint64_t lo = list_off[node];
+    const uint32_t T = (uint32_t)(list_off[node + 1] - lo);
+    const uint32_t nn = n_new[node];
+    const uint32_t *lst = lists + lo;
+    float *o = pdist + pair_off[node];
+    const uint32_t ls = dpad + 4;
+    const bool in_lds = (size_t)T * ls * sizeof(float) <= FK_JOIN_LDS_ROWS_BYTES;
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
+        s_thr[t] = thr[lst[t]];
+        s_c[t] = 0;
+    }
+    if (in_lds) {
+        const uint32_t v4 = dpad >> 2;
+        for (uint32_t i = threadIdx.x; i < T * v4; i += blockDim.x) {
+            const uint32_t r = i / v4, c = i - r * v4;
+            *reinterpret_cast<float4 *>(s_rows + r * ls + c * 4) =
+                *reinterpret_cast<const float4 *>(rows + (size_t)lst[r] * stride + c * 4);
+        }
+    }
+    __syncthreads();
+    const uint64_t cells = (uint64_t)nn * T;
+    for (uint64_t c = threadIdx.x; c < cells; c += blockDim.x) {
+        const uint32_t x = (uint32_t)(c / T), y = (uint32_t)(c - (uint64_t)x * T);
+        if (y <= x) continue;
+        const uint32_t ix = lst[x], iy = lst[y];
+        const float *ra = in_lds ? s_rows + x * ls : rows + (size_t)ix * stride;
+        const float *rb = in_lds ? s_rows + y * ls : rows + (size_t)iy * stride;
+        const float dd = l2_canon(ra, rb, dpad);
+        const uint64_t rs = (uint64_t)x * (T - 1) - (uint64_t)x * (x - 1) / 2;
+        o[rs + (y - x - 1)] = dd;
+        if (ix != iy) {
+            if (dd <= s_thr[x]) atomicAdd(&s_c[x], 1u);
+            if (dd <= s_thr[y]) atomicAdd(&s_c[y], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
+        const uint32_t c = s_c[t];
+        resv[lo + t] = c ? atomicAdd(&bcnt[lst[t]], c) : 0u;
+    }
+}
+
+// Pass 2: write the candidates into their reserved bucket slots.
+__global__ __launch_bounds__(256) void k_join_fill(const uint32_t *__restrict__ lists,
+                                                   const uint64_t *__restrict__ list_off,
+                                                   const uint32_t *__restrict__ n_new,
+                                                   const uint64_t *__restrict__ pair_off,
+                                                   const float *__restrict__ thr, const float *__restrict__ pdist,
+                                                   const uint32_t *__restrict__ resv,
+                                                   const uint64_t *__restrict__ boff, uint64_t *__restrict__ bkey,
+                                                   uint32_t *__restrict__ bsrc) {
+    __shared__ float s_thr[FK_LIST_MAX];
+    __shared__ uint32_t s_c[FK_LIST_MAX];
+    __shared__ uint64_t s_base[FK_LIST_MAX];
+    const uint32_t node = blockIdx.x;
+    const uint64_t lo = list_off[node];
+    const uint32_t T = (uint32_t)(list_off[node + 1] - lo);
+    const uint32_t nn = n_new[node];
+    const uint32_t *lst = lists + lo;
+    const float *o = pdist + pair_off[node];
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
+        const uint32_t id = lst[t];
+        s_thr[t] = thr[id];
+        s_c[t] = 0;
+        s_base[t] = boff[id] + resv[lo + t];
+    }
+    __syncthreads();
+    const uint64_t cells = (uint64_t)nn * T;
+    for (uint64_t c = threadIdx.x; c < cells; c += blockDim.x) {
+        const uint32_t x = (uint32_t)(c / T), y = (uint32_t)(c - (uint64_t)x * T);
+        if (y <= x) continue;
+        const uint32_t ix = lst[x], iy = lst[y];
+        if (ix == iy) continue;
+        const uint64_t rs = (uint64_t)x * (T - 1) - (uint64_t)x * (x - 1) / 2;
+        const float dd = o[rs + (y - x - 1)];
+        if (dd <= s_thr[x]) {
+            const uint64_t sl = s_base[x] + atomicAdd(&s_c[x], 1u);
+            bkey[sl] = make_key(dd, iy);
+            bsrc[sl] = node;
+        }
+        if (dd <= s_thr[y]) {
+            const uint64_t sl = s_base[y] + atomicAdd(&s_c[y], 1u);
+            bkey[sl] = make_key(dd, ix);
+            bsrc[sl] = node;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Bucket merge, one workgroup per target node.  Candidates are sorted by
+// (key, source join); the first of equal keys is the one kept.  A kept
+// candidate that ranks before the iteration-start K-th entry marks its source
+// join `found` (src/kgraph.cpp:1111-1114: an insert at position < K).
+// ---------------------------------------------------------------------------
+struct MergeShared {
+    uint64_t p[FK_POOL_MAX];
+    uint64_t o[FK_POOL_MAX > FK_MRG_CH ? FK_POOL_MAX : FK_MRG_CH];
+    uint64_t c[FK_MRG_CH];
+    uint64_t c2[FK_MRG_CH];
+    uint32_t src[FK_MRG_CH];
+    uint32_t keep[FK_MRG_CH];
+    uint32_t part[256];
+    uint8_t pf[FK_POOL_MAX];
+    uint8_t of[FK_POOL_MAX];
+    uint32_t np;
+};
+
+__device__ __forceinline__ bool key_src_less(uint64_t ka, uint32_t sa, uint64_t kb, uint32_t sb) {
+    return ka < kb || (ka == kb && sa < sb);
+}
+
+__device__ void block_sort_key_src(uint64_t *k, uint32_t *v, uint32_t n) {
+    for (uint32_t s = 2; s <= n; s <<= 1) {
+        for (uint32_t j = s >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const bool up = (i & s) == 0;
+                    const bool gt = key_src_less(k[ixj], v[ixj], k[i], v[i]);
+                    if (gt == up) {
+                        const uint64_t tk = k[i]; k[i] = k[ixj]; k[ixj] = tk;
+                        const uint32_t tv = v[i]; v[i] = v[ixj]; v[ixj] = tv;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, uint32_t K, uint64_t *__restrict__ pool,
+                                                   uint8_t *__restrict__ pflg, uint32_t *__restrict__ cnt,
+                                                   float *__restrict__ thr, const uint32_t *__restrict__ bcnt,
+                                                   const uint64_t *__restrict__ boff,
+                                                   const uint64_t *__restrict__ bkey,
+                                                   const uint32_t *__restrict__ bsrc, uint8_t *__restrict__ found) {
+    __shared__ MergeShared sh;
+    const uint32_t i = blockIdx.x;
+    const uint32_t nb = bcnt[i];
+    if (nb == 0) return;
+    uint64_t *pr = pool + (uint64_t)i * cap;
+    uint8_t *pf = pflg + (uint64_t)i * cap;
+    const uint32_t np0 = cnt[i];
+    for (uint32_t j = threadIdx.x; j < np0; j += blockDim.x) { sh.p[j] = pr[j]; sh.pf[j] = pf[j]; }
+    if (threadIdx.x == 0) sh.np = np0;
+    __syncthreads();
+    const uint64_t kth = np0 >= K ? sh.p[K - 1] : FK_KEY_EMPTY;
+    const uint64_t base = boff[i];
+    for (uint32_t c0 = 0; c0 < nb; c0 += FK_MRG_CH) {
+        const uint32_t m = (nb - c0) < FK_MRG_CH ? (nb - c0) : FK_MRG_CH;
+        const uint32_t m2 = next_pow2(m < 2 ? 2 : m);
+        for (uint32_t k = threadIdx.x; k < m2; k += blockDim.x) {
+            sh.c[k] = k < m ? bkey[base + c0 + k] : FK_KEY_EMPTY;
+            sh.src[k] = k < m ? bsrc[base + c0 + k] : 0xFFFFFFFFu;
+        }
+        __syncthreads();
+        block_sort_key_src(sh.c, sh.src, m2);
+        const uint32_t np = sh.np;
+        for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
+            const uint64_t key = sh.c[k];
+            const bool first = (k == 0) || (sh.c[k - 1] != key);
+            const bool keep = first && !sorted_contains(sh.p, np, key);
+            sh.keep[k] = keep ? 1u : 0u;
+            if (keep && key < kth) found[sh.src[k]] = 1;
+        }
+        __syncthreads();
+        block_merge(sh.p, sh.pf, &sh.np, sh.c, sh.keep, m, Lp, (uint8_t)(FK_FLG_NEW | FK_FLG_ISNEW), sh.c2, sh.o,
+                    sh.of, sh.part);
+    }
+    const uint32_t np = sh.np;
+    for (uint32_t j = threadIdx.x; j < np; j += blockDim.x) { pr[j] = sh.p[j]; pf[j] = sh.pf[j]; }
+    if (threadIdx.x == 0) {
+        cnt[i] = np;
+        thr[i] = np >= Lp ? key_dist(sh.p[Lp - 1]) : INFINITY;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stop-rule statistics (src/kgraph.cpp:1371-1394).
+// delta = mean over nodes of (#flagged among the first K entries) / K;
+// recall = mean over controls of |pool[0..min(L,K)) ∩ gt[0..K)| / K.
+// ---------------------------------------------------------------------------
+__global__ void k_delta(const uint8_t *__restrict__ pflg, const uint32_t *__restrict__ cnt, uint32_t N, uint32_t cap,
+                        uint32_t K, unsigned long long *__restrict__ acc) {
+    const uint32_t n = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    unsigned long long c = 0;
+    if (n < N) {
+        const uint32_t lim = cnt[n] < K ? cnt[n] : K;
+        const uint8_t *pf = pflg + (uint64_t)n * cap;
+        for (uint32_t l = lane_id(); l < lim; l += 64) c += (pf[l] & FK_FLG_NEW) ? 1 : 0;
+    }
+    c = wave_sum(c);
+    if (lane_id() == 0 && c) atomicAdd(acc, c);
+}
+
+__global__ __launch_bounds__(256) void k_recall(const uint64_t *__restrict__ pool, const uint32_t *__restrict__ cnt,
+                                                uint32_t cap, const uint32_t *__restrict__ ctrl_ids,
+                                                const uint32_t *__restrict__ gt, uint32_t gt_stride, uint32_t K,
+                                                unsigned long long *__restrict__ acc) {
+    __shared__ uint32_t s_gt[2 * FK_POOL_MAX];
+    __shared__ uint32_t s_found;
+    const uint32_t c = blockIdx.x;
+    const uint32_t np2 = next_pow2(K < 2 ? 2 : K);
+    for (uint32_t k = threadIdx.x; k < np2; k += blockDim.x) s_gt[k] = k < K ? gt[(uint64_t)c * gt_stride + k] : 0xFFFFFFFFu;
+    if (threadIdx.x == 0) s_found = 0;
+    __syncthreads();
+    block_bitonic_sort(s_gt, np2);
+    const uint32_t id = ctrl_ids[c];
+    const uint32_t lim = cnt[id] < K ? cnt[id] : K;
+    const uint64_t *pr = pool + (uint64_t)id * cap;
+    uint32_t f = 0;
+    for (uint32_t l = threadIdx.x; l < lim; l += blockDim.x) {
+        const uint32_t x = key_id(pr[l]);
+        const uint32_t pos = lower_bound_cnt(s_gt, K, x);
+        if (pos < K && s_gt[pos] == x) ++f;
+    }
+    if (f) atomicAdd(&s_found, f);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(acc, (unsigned long long)s_found);
+}
+
+// ---------------------------------------------------------------------------
+// update() (src/kgraph.cpp:1118-1202).
+// ---------------------------------------------------------------------------
+// M growth and radiusM.
+__global__ void k_upd_m(const uint64_t *__restrict__ pool, const uint8_t *__restrict__ pflg,
+                        const uint32_t *__restrict__ cnt, const uint8_t *__restrict__ found, uint32_t N, uint32_t cap,
+                        uint32_t S, uint32_t *__restrict__ Marr, float *__restrict__ radiusM) {
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    uint32_t M = Marr[n];
+    const uint32_t L = cnt[n];
+    if (found[n]) {
+        const uint32_t maxl = (M + S) < L ? (M + S) : L;
+        const uint8_t *pf = pflg + (uint64_t)n * cap;
+        uint32_t c = 0, l = 0;
+        while (l < maxl && c < S) {
+            if (pf[l] & FK_FLG_NEW) ++c;
+            ++l;
+        }
+        M = l;
+    }
+    if (M > L) M = L;
+    if (M == 0) M = L < 1 ? 0 : 1;
+    Marr[n] = M;
+    radiusM[n] = M ? key_dist(pool[(uint64_t)n * cap + M - 1]) : INFINITY;
+}
+
+// Forward new/old counts and reverse-push counts.
+__global__ void k_upd_count(const uint64_t *__restrict__ pool, const uint8_t *__restrict__ pflg, uint32_t N,
+                            uint32_t cap, const uint32_t *__restrict__ Marr, const float *__restrict__ radiusM,
+                            uint32_t *__restrict__ fcnt_new, uint32_t *__restrict__ fcnt_old,
+                            uint32_t *__restrict__ rcnt_new, uint32_t *__restrict__ rcnt_old) {
+    const uint32_t n = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (n >= N) return;
+    const uint32_t M = Marr[n];
+    const uint64_t *pr = pool + (uint64_t)n * cap;
+    const uint8_t *pf = pflg + (uint64_t)n * cap;
+    uint32_t fn = 0;
+    for (uint32_t l = lane_id(); l < M; l += 64) {
+        const uint64_t e = pr[l];
+        const uint32_t o = key_id(e);
+        const bool f = (pf[l] & FK_FLG_NEW) != 0;
+        fn += f ? 1u : 0u;
+        if (key_dist(e) > radiusM[o]) atomicAdd(f ? &rcnt_new[o] : &rcnt_old[o], 1u);
+    }
+    fn = wave_sum(fn);
+    if (lane_id() == 0) { fcnt_new[n] = fn; fcnt_old[n] = M - fn; }
+}
+
+__global__ void k_list_sizes(const uint32_t *__restrict__ fcnt_new, const uint32_t *__restrict__ fcnt_old,
+                             const uint32_t *__restrict__ rcnt_new, const uint32_t *__restrict__ rcnt_old, uint32_t N,
+                             uint32_t R, uint32_t *__restrict__ n_new, uint32_t *__restrict__ lsize) {
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const uint32_t rn = (R && rcnt_new[n] > R) ? R : rcnt_new[n];
+    const uint32_t ro = (R && rcnt_old[n] > R) ? R : rcnt_old[n];
+    n_new[n] = fcnt_new[n] + rn;
+    lsize[n] = fcnt_new[n] + rn + fcnt_old[n] + ro;
+}
+
+// Forward lists, reverse bucket fill, flag clearing.
+__global__ void k_upd_fill(const uint64_t *__restrict__ pool, uint8_t *__restrict__ pflg, uint32_t N, uint32_t cap,
+                           const uint32_t *__restrict__ Marr, const float *__restrict__ radiusM,
+                           const uint64_t *__restrict__ list_off, const uint32_t *__restrict__ n_new,
+                           const uint32_t *__restrict__ fcnt_new, uint32_t *__restrict__ lists,
+                           const uint64_t *__restrict__ roff_new, const uint64_t *__restrict__ roff_old,
+                           uint32_t *__restrict__ rcur_new, uint32_t *__restrict__ rcur_old,
+                           uint32_t *__restrict__ rbuf_new, uint32_t *__restrict__ rbuf_old) {
+    const uint32_t n = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (n >= N) return;
+    const uint32_t lane = lane_id();
+    const uint32_t M = Marr[n];
+    const uint64_t *pr = pool + (uint64_t)n * cap;
+    uint8_t *pf = pflg + (uint64_t)n * cap;
+    uint32_t *ln = lists + list_off[n];
+    uint32_t *lo = ln + n_new[n];
+    uint32_t bn = 0, bo = 0;
+    for (uint32_t l0 = 0; l0 < M; l0 += 64) {
+        const uint32_t l = l0 + lane;
+        const bool v = l < M;
+        uint64_t e = 0;
+        uint8_t fl = 0;
+        if (v) { e = pr[l]; fl = pf[l]; }
+        const bool f = v && (fl & FK_FLG_NEW);
+        const uint64_t mn = wave_ballot(f), mo = wave_ballot(v && !f);
+        const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        if (f) ln[bn + __popcll(mn & below)] = key_id(e);
+        if (v && !f) lo[bo + __popcll(mo & below)] = key_id(e);
+        bn += __popcll(mn);
+        bo += __popcll(mo);
+        if (v) {
+            const uint32_t o = key_id(e);
+            if (key_dist(e) > radiusM[o]) {
+                if (f) rbuf_new[roff_new[o] + atomicAdd(&rcur_new[o], 1u)] = n;
+                else rbuf_old[roff_old[o] + atomicAdd(&rcur_old[o], 1u)] = n;
+            }
+            if (f) pf[l] = fl & (uint8_t)~FK_FLG_NEW;
+        }
+    }
+    (void)fcnt_new;
+}
+
+// Reverse-list cap: when more than R nodes pushed, keep a uniform random R
+// subset (the reference shuffles with rand() and truncates, :1189-1200); the
+// subset is the R smallest counter-based hashes, so it is reproducible.
+#define FK_SEL_CH 2048
+__global__ __launch_bounds__(256) void k_rev_select(uint32_t N, uint32_t R, uint64_t seed, uint32_t it,
+                                                    const uint32_t *__restrict__ rcnt_new,
+                                                    const uint32_t *__restrict__ rcnt_old,
+                                                    const uint64_t *__restrict__ roff_new,
+                                                    const uint64_t *__restrict__ roff_old,
+                                                    const uint32_t *__restrict__ rbuf_new,
+                                                    const uint32_t *__restrict__ rbuf_old,
+                                                    const uint64_t *__restrict__ list_off,
+                                                    const uint32_t *__restrict__ n_new,
+                                                    const uint32_t *__restrict__ fcnt_new,
+                                                    const uint32_t *__restrict__ fcnt_old, uint32_t *__restrict__ lists) {
+    __shared__ uint64_t s_k[FK_SEL_CH];
+    const uint32_t o = blockIdx.x;
+    for (int which = 0; which < 2; ++which) {
+        const uint32_t c = which == 0 ? rcnt_new[o] : rcnt_old[o];
+        const uint32_t *src = which == 0 ? rbuf_new + roff_new[o] : rbuf_old + roff_old[o];
+        uint32_t *dst = lists + list_off[o] + (which == 0 ? fcnt_new[o] : n_new[o] + fcnt_old[o]);
+        if (c == 0) continue;
+        if (R == 0 || c <= R) {
+            for (uint32_t k = threadIdx.x; k < c; k += blockDim.x) dst[k] = src[k];
+            continue;
+        }
+        // running best-R selection over chunks
+        uint32_t have = 0;
+        const uint64_t stream = ((uint64_t)it << 33) | ((uint64_t)which << 32) | o;
+        for (uint32_t c0 = 0; c0 < c; c0 += (FK_SEL_CH - R)) {
+            const uint32_t m = (c - c0) < (FK_SEL_CH - R) ? (c - c0) : (FK_SEL_CH - R);
+            const uint32_t tot = have + m;
+            const uint32_t np2 = next_pow2(tot < 2 ? 2 : tot);
+            for (uint32_t k = threadIdx.x; k < np2; k += blockDim.x) {
+                if (k >= have) {
+                    if (k < tot) {
+                        const uint32_t s = src[c0 + (k - have)];
+                        s_k[k] = ((uint64_t)fk_rand32(seed, stream, s) << 32) | s;
+                    } else {
+                        s_k[k] = FK_KEY_EMPTY;
+                    }
+                }
+            }
+            __syncthreads();
+            block_bitonic_sort(s_k, np2);
+            have = tot < R ? tot : R;
+            __syncthreads();
+        }
+        for (uint32_t k = threadIdx.x; k < R; k += blockDim.x) dst[k] = (uint32_t)s_k[k];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host phases.
+// ---------------------------------------------------------------------------
+static void zero_u32(BuildState &b, DevBuf<uint32_t> &x, size_t n) { FK_HIP(hipMemsetAsync(x.p, 0, n * 4, b.s)); }
+
+void nnd_alloc(BuildState &b) {
+    const size_t N = b.N;
+    b.pool.alloc(N * b.cap);
+    b.pflg.alloc(N * b.cap);
+    b.cnt.alloc(N);
+    b.thr.alloc(N);
+    b.M.alloc(N);
+    b.radiusM.alloc(N);
+    b.found.alloc(N);
+    b.list_off.alloc(N + 1);
+    b.n_new.alloc(N);
+    b.pair_off.alloc(N + 1);
+    b.pair_cnt.alloc(N);
+    b.bcnt.alloc(N);
+    b.boff.alloc(N + 1);
+    b.fcnt_new.alloc(N);
+    b.fcnt_old.alloc(N);
+    b.rcnt_new.alloc(N);
+    b.rcnt_old.alloc(N);
+    b.roff_new.alloc(N + 1);
+    b.roff_old.alloc(N + 1);
+    b.acc.alloc(8);
+    b.scal.alloc(16);
+}
+
+// GenerateControl (src/kgraph.cpp:363-387): `controls` distinct random nodes
+// and their exact MAXL nearest neighbours (self excluded, ties by id).
+void nnd_controls(BuildState &b) {
+    const double t0 = now_sec();
+    b.n_ctrl = (b.P.flags & FK_FLAG_NO_CONTROLS) ? 0 : b.P.controls;
+    if (b.n_ctrl == 0) return;
+    std::vector<uint32_t> idx(b.N);
+    for (uint32_t i = 0; i < b.N; ++i) idx[i] = i;
+    // partial Fisher-Yates with the build's host stream
+    for (uint32_t i = 0; i < b.n_ctrl; ++i) {
+        const uint32_t j = i + (uint32_t)(b.next_rand() % (b.N - i));
+        std::swap(idx[i], idx[j]);
+    }
+    std::vector<uint32_t> ids(idx.begin(), idx.begin() + b.n_ctrl);
+    b.ctrl_ids.alloc(b.n_ctrl);
+    b.ctrl_gt.alloc((size_t)b.n_ctrl * b.P.MAXL);
+    FK_HIP(hipMemcpyAsync(b.ctrl_ids.p, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, b.s));
+    // control rows as queries
+    DevBuf<float> q((size_t)b.n_ctrl * b.ds->dpad);
+    for (uint32_t c = 0; c < b.n_ctrl; ++c)
+        FK_HIP(hipMemcpyAsync(q.p + (size_t)c * b.ds->dpad, b.ds->rows.p + (size_t)ids[c] * b.ds->stride,
+                              (size_t)b.ds->dpad * 4, hipMemcpyDeviceToDevice, b.s));
+    const uint32_t K = b.P.MAXL < b.N - 1 ? b.P.MAXL : b.N - 1;
+    if (K < b.P.MAXL) FK_HIP(hipMemsetAsync(b.ctrl_gt.p, 0xFF, b.ctrl_gt.bytes(), b.s));
+    DevBuf<uint32_t> gt((size_t)b.n_ctrl * K);
+    bruteforce_knn(b.s, *b.ds, q.p, b.ds->dpad, b.n_ctrl, b.N, K, b.ctrl_ids.p, gt.p, nullptr);
+    FK_HIP(hipMemcpy2DAsync(b.ctrl_gt.p, (size_t)b.P.MAXL * 4, gt.p, (size_t)K * 4, (size_t)K * 4, b.n_ctrl,
+                            hipMemcpyDeviceToDevice, b.s));
+    b.sync();
+    b.st->d_ctrl += (uint64_t)b.n_ctrl * (b.N - 1);
+    b.st->t_ctrl += now_sec() - t0;
+}
+
+float nnd_evaluate(BuildState &b, uint32_t K) {
+    if (b.n_ctrl == 0) return 0.f;
+    FK_CHECK_ARG(K <= FK_POOL_MAX * 2, "K too large for recall evaluation");
+    FK_HIP(hipMemsetAsync(b.acc.p, 0, 8, b.s));
+    hipLaunchKernelGGL(k_recall, dim3(b.n_ctrl), dim3(256), 0, b.s, b.pool.p, b.cnt.p, b.cap, b.ctrl_ids.p,
+                       b.ctrl_gt.p, b.P.MAXL, K, b.acc.p);
+    FK_LAUNCH_CHECK();
+    unsigned long long f = 0;
+    FK_HIP(hipMemcpyAsync(&f, b.acc.p, 8, hipMemcpyDeviceToHost, b.s));
+    b.sync();
+    // mean over controls of found/K (EvaluateRecall, src/kgraph.cpp:113-169)
+    return (float)((double)f / K / b.n_ctrl);
+}
+
+// Exclusive scan of n counts into out[0..n], out[n] = total; returns total.
+static uint64_t scan_to_host(BuildState &b, const uint32_t *in, uint64_t *out, size_t n) {
+    exclusive_scan_u32_to_u64(b.s, in, out, n, out + n, b.scan_tmp);
+    uint64_t tot = 0;
+    FK_HIP(hipMemcpyAsync(&tot, out + n, 8, hipMemcpyDeviceToHost, b.s));
+    b.sync();
+    return tot;
+}
+
+static void nnd_join(BuildState &b) {
+    const uint32_t N = b.N;
+    const unsigned g256 = (N + 255) / 256;
+    hipLaunchKernelGGL(k_pair_counts, dim3(g256), dim3(256), 0, b.s, b.list_off.p, b.n_new.p, N, b.pair_cnt.p);
+    FK_LAUNCH_CHECK();
+    const uint64_t pairs = scan_to_host(b, b.pair_cnt.p, b.pair_off.p, N);
+    b.st->d_join += pairs;
+    b.pdist.ensure(pairs);
+    zero_u32(b, b.bcnt, N);
+    FK_HIP(hipMemsetAsync(b.found.p, 0, N, b.s));
+    uint64_t nlist = 0;
+    FK_HIP(hipMemcpyAsync(&nlist, b.list_off.p + N, 8, hipMemcpyDeviceToHost, b.s));
+    b.sync();
+    b.resv.ensure(nlist);
+    const size_t lds = FK_JOIN_LDS_ROWS_BYTES;
+    hipLaunchKernelGGL(k_join_count, dim3(N), dim3(256), lds, b.s, b.ds->rows.p, b.ds->stride, b.ds->dpad,
+                       b.lists.p, b.list_off.p, b.n_new.p, b.pair_off.p, b.thr.p, b.pdist.p, b.resv.p, b.bcnt.p);
+    FK_LAUNCH_CHECK();
+    const uint64_t cands = scan_to_host(b, b.bcnt.p, b.boff.p, N);
+    b.bkey.ensure(cands);
+    b.bsrc.ensure(cands);
+    hipLaunchKernelGGL(k_join_fill, dim3(N), dim3(256), 0, b.s, b.lists.p, b.list_off.p, b.n_new.p, b.pair_off.p,
+                       b.thr.p, b.pdist.p, b.resv.p, b.boff.p, b.bkey.p, b.bsrc.p);
+    FK_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_nnd_merge, dim3(N), dim3(256), 0, b.s, b.cap, b.P.L, b.P.K, b.pool.p, b.pflg.p, b.cnt.p,
+                       b.thr.p, b.bcnt.p, b.boff.p, b.bkey.p, b.bsrc.p, b.found.p);
+    FK_LAUNCH_CHECK();
+    if (b.verbose) fprintf(stderr, "[fk] join: pairs %llu candidates %llu\n", (unsigned long long)pairs,
+                           (unsigned long long)cands);
+}
+
+static void nnd_update(BuildState &b, uint32_t it) {
+    const uint32_t N = b.N;
+    const unsigned g256 = (N + 255) / 256;
+    const unsigned gw = (N + 3) / 4;  // 4 waves per 256-thread block
+    hipLaunchKernelGGL(k_upd_m, dim3(g256), dim3(256), 0, b.s, b.pool.p, b.pflg.p, b.cnt.p, b.found.p, N, b.cap,
+                       b.P.S, b.M.p, b.radiusM.p);
+    FK_LAUNCH_CHECK();
+    zero_u32(b, b.rcnt_new, N);
+    zero_u32(b, b.rcnt_old, N);
+    hipLaunchKernelGGL(k_upd_count, dim3(gw), dim3(256), 0, b.s, b.pool.p, b.pflg.p, N, b.cap, b.M.p, b.radiusM.p,
+                       b.fcnt_new.p, b.fcnt_old.p, b.rcnt_new.p, b.rcnt_old.p);
+    FK_LAUNCH_CHECK();
+    const uint64_t tn = scan_to_host(b, b.rcnt_new.p, b.roff_new.p, N);
+    const uint64_t to = scan_to_host(b, b.rcnt_old.p, b.roff_old.p, N);
+    b.rbuf_new.ensure(tn);
+    b.rbuf_old.ensure(to);
+    hipLaunchKernelGGL(k_list_sizes, dim3(g256), dim3(256), 0, b.s, b.fcnt_new.p, b.fcnt_old.p, b.rcnt_new.p,
+                       b.rcnt_old.p, N, b.P.R, b.n_new.p, b.pair_cnt.p);
+    FK_LAUNCH_CHECK();
+    const uint64_t nlist = scan_to_host(b, b.pair_cnt.p, b.list_off.p, N);
+    b.lists.ensure(nlist);
+    // reverse cursors reuse the bucket and pair counters
+    zero_u32(b, b.bcnt, N);
+    zero_u32(b, b.pair_cnt, N);
+    hipLaunchKernelGGL(k_upd_fill, dim3(gw), dim3(256), 0, b.s, b.pool.p, b.pflg.p, N, b.cap, b.M.p, b.radiusM.p,
+                       b.list_off.p, b.n_new.p, b.fcnt_new.p, b.lists.p, b.roff_new.p, b.roff_old.p, b.bcnt.p,
+                       b.pair_cnt.p, b.rbuf_new.p, b.rbuf_old.p);
+    FK_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_rev_select, dim3(N), dim3(256), 0, b.s, N, b.P.R, (uint64_t)b.P.seed, it, b.rcnt_new.p,
+                       b.rcnt_old.p, b.roff_new.p, b.roff_old.p, b.rbuf_new.p, b.rbuf_old.p, b.list_off.p,
+                       b.n_new.p, b.fcnt_new.p, b.fcnt_old.p, b.lists.p);
+    FK_LAUNCH_CHECK();
+    b.sync();
+}
+
+void nnd_run(BuildState &b) {
+    const double t0 = now_sec();
+    const uint32_t N = b.N;
+    const BuildParams &P = b.P;
+    FK_CHECK_ARG(P.S >= 1 && P.S <= 128, "S must be in [1, 128]");
+    FK_CHECK_ARG(P.S <= P.L, "S must not exceed L");
+    FK_CHECK_ARG(P.MAXL <= FK_POOL_MAX, "max(L, K, search_L, search_K) above 1024 is not supported");
+    nnd_alloc(b);
+    // init (iteration 1 joins 2S random ids per node, old lists empty)
+    b.lists.ensure((size_t)N * 2 * P.S);
+    {
+        std::vector<uint64_t> lo((size_t)N + 1);
+        for (size_t i = 0; i <= N; ++i) lo[i] = i * 2 * P.S;
+        FK_HIP(hipMemcpyAsync(b.list_off.p, lo.data(), lo.size() * 8, hipMemcpyHostToDevice, b.s));
+        std::vector<uint32_t> nn(N, 2 * P.S);
+        FK_HIP(hipMemcpyAsync(b.n_new.p, nn.data(), (size_t)N * 4, hipMemcpyHostToDevice, b.s));
+        b.sync();
+    }
+    hipLaunchKernelGGL(k_nnd_init, dim3(N), dim3(64), 0, b.s, b.ds->rows.p, b.ds->stride, b.ds->dpad, N, P.S, b.cap,
+                       P.L, (uint64_t)P.seed, b.pool.p, b.pflg.p, b.cnt.p, b.M.p, b.thr.p, b.lists.p);
+    FK_LAUNCH_CHECK();
+    b.st->d_init += (uint64_t)N * P.S;
+    const double tk0 = now_sec();
+    for (uint32_t it = 0; it < P.iterations; ++it) {
+        b.st->nnd_iterations++;
+        nnd_join(b);
+        // statistics
+        FK_HIP(hipMemsetAsync(b.acc.p, 0, 8, b.s));
+        hipLaunchKernelGGL(k_delta, dim3((N + 3) / 4), dim3(256), 0, b.s, b.pflg.p, b.cnt.p, N, b.cap, P.K, b.acc.p);
+        FK_LAUNCH_CHECK();
+        unsigned long long fl = 0;
+        FK_HIP(hipMemcpyAsync(&fl, b.acc.p, 8, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+        const float delta = (float)((double)fl / P.K / N);
+        const float recall = nnd_evaluate(b, P.K);
+        b.st->nnd_delta = delta;
+        b.st->nnd_recall = recall;
+        if (b.st->n_kgraph < FK_MAX_LOOPS + 64) b.st->kgraph_time[b.st->n_kgraph++] = now_sec() - tk0;
+        if (b.st->n_recall < FK_MAX_LOOPS + 64) b.st->recall[b.st->n_recall++] = recall;
+        if (b.verbose)
+            fprintf(stderr, "[fk] iteration: %u recall: %g delta: %g time: %g\n", it + 1, recall, delta,
+                    now_sec() - tk0);
+        if (delta <= P.delta) { b.st->nnd_stop = 1; break; }
+        if (b.n_ctrl && recall >= P.recall) { b.st->nnd_stop = 2; break; }
+        if (it + 1 >= P.iterations) break;
+        nnd_update(b, it);
+    }
+    b.sync();
+    b.st->t_nnd += now_sec() - t0;
+}
+
+// ---------------------------------------------------------------------------
+// BridgeView (P2K batched search, src/kgraph.cpp:2478-2663), pull form: node
+// i expands the flagged entries q of its window [0, min(window, L_i)) and
+// joins q's out-list into its own pool, which is what the reference's
+// reverse lists rnn_new deliver to i.  A pair (q -> t) is skipped when t == i
+// or when both the pool entry q and the out-list entry t are old (:2640-2642).
+// ---------------------------------------------------------------------------
+__global__ void k_bv_init(uint8_t *__restrict__ pflg, const uint32_t *__restrict__ cnt, uint32_t N, uint32_t cap,
+                          uint32_t cur_loop, uint32_t pre_L, uint32_t cur_L) {
+    const uint32_t n = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (n >= N) return;
+    uint8_t *pf = pflg + (uint64_t)n * cap;
+    const uint32_t L = cnt[n];
+    for (uint32_t j = lane_id(); j < L; j += 64) {
+        const uint8_t f = pf[j];
+        uint8_t isnew = 0;
+        if (cur_loop != 0 && j >= pre_L && j < cur_L) isnew = (f & FK_FLG_NEW) ? FK_FLG_ISNEW : 0;
+        pf[j] = (uint8_t)(FK_FLG_NEW | isnew);
+    }
+}
+
+__global__ void k_bv_count(const uint8_t *__restrict__ pflg, const uint32_t *__restrict__ cnt, uint32_t N,
+                           uint32_t cap, uint32_t W, uint32_t S, unsigned long long *__restrict__ acc) {
+    const uint32_t n = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    uint32_t f = 0;
+    if (n < N) {
+        const uint8_t *pf = pflg + (uint64_t)n * cap;
+        const uint32_t lim = cnt[n] < W ? cnt[n] : W;
+        for (uint32_t j = lane_id(); j < lim; j += 64) f += (pf[j] & FK_FLG_NEW) ? 1u : 0u;
+    }
+    f = wave_sum(f);
+    if (lane_id() == 0 && n < N) {
+        const uint32_t capped = f < S ? f : S;
+        atomicAdd(acc, (unsigned long long)capped);
+        atomicAdd(acc + 1, (unsigned long long)f);
+    }
+}
+
+struct BvShared {
+    MergeShared m;
+    uint32_t sel[FK_POOL_MAX];
+    uint8_t sel_new[FK_POOL_MAX];
+    uint32_t nsel;
+    uint32_t nc;
+    float thr;
+};
+
+__global__ __launch_bounds__(256) void k_bv_round(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
+                                                  uint32_t N, uint32_t cap, uint32_t W, uint32_t S_cap,
+                                                  uint64_t *__restrict__ pool, uint8_t *__restrict__ pflg,
+                                                  uint32_t *__restrict__ cnt, const uint32_t *__restrict__ nsg_id,
+                                                  const uint32_t *__restrict__ nsg_len,
+                                                  const uint8_t *__restrict__ nsg_new, uint32_t ncap,
+                                                  unsigned long long *__restrict__ cc_acc) {
+    __shared__ BvShared sh;
+    extern __shared__ __align__(16) float s_row[];
+    const uint32_t i = blockIdx.x;
+    uint64_t *pr = pool + (uint64_t)i * cap;
+    uint8_t *pf = pflg + (uint64_t)i * cap;
+    const uint32_t np0 = cnt[i];
+    const uint32_t lim = np0 < W ? np0 : W;
+    for (uint32_t j = threadIdx.x; j < np0; j += blockDim.x) { sh.m.p[j] = pr[j]; sh.m.pf[j] = pf[j]; }
+    __syncthreads();
+    // select flagged window entries (first S_cap in pool order), clear them
+    if (threadIdx.x == 0) {
+        uint32_t ns = 0;
+        for (uint32_t j = 0; j < lim && ns < S_cap; ++j) {
+            const uint8_t f = sh.m.pf[j];
+            if (f & FK_FLG_NEW) {
+                sh.sel[ns] = key_id(sh.m.p[j]);
+                sh.sel_new[ns] = (f & FK_FLG_ISNEW) ? 1 : 0;
+                ++ns;
+                sh.m.pf[j] = f & (uint8_t)~FK_FLG_NEW;
+            }
+        }
+        sh.nsel = ns;
+        sh.m.np = np0;
+        sh.nc = 0;
+        sh.thr = np0 >= cap ? key_dist(sh.m.p[cap - 1]) : INFINITY;
+    }
+    for (uint32_t k = threadIdx.x; k < (dpad >> 2); k += blockDim.x)
+        reinterpret_cast<float4 *>(s_row)[k] = reinterpret_cast<const float4 *>(rows + (size_t)i * stride)[k];
+    __syncthreads();
+    if (sh.nsel == 0) return;
+    const float thr = sh.thr;
+    unsigned long long cc = 0;
+    const uint32_t nsel = sh.nsel;
+    for (uint32_t s = 0; s < nsel; ++s) {
+        const uint32_t q = sh.sel[s];
+        const bool qnew = sh.sel_new[s] != 0;
+        const uint32_t deg = nsg_len[q];
+        for (uint32_t m0 = 0; m0 < deg; m0 += blockDim.x) {
+            const uint32_t m = m0 + threadIdx.x;
+            bool emit = false;
+            uint64_t key = 0;
+            if (m < deg) {
+                const uint32_t t = nsg_id[(uint64_t)q * ncap + m];
+                const bool tnew = nsg_new[(uint64_t)q * ncap + m] != 0;
+                if (t != i && (qnew || tnew)) {
+                    ++cc;
+                    const float dd = l2_canon(s_row, rows + (size_t)t * stride, dpad);
+                    if (dd <= thr) { emit = true; key = make_key(dd, t); }
+                }
+            }
+            if (emit) {
+                const uint32_t slot = atomicAdd(&sh.nc, 1u);
+                sh.m.c[slot] = key;
+            }
+            __syncthreads();
+            // flush when the next batch might not fit, or at the very end
+            const bool last = (s + 1 == nsel) && (m0 + blockDim.x >= deg);
+            const uint32_t nc = sh.nc;
+            __syncthreads();  // every thread has read nc before it may change
+            if (last || nc + blockDim.x > FK_MRG_CH) {
+                if (nc > 0) {
+                    const uint32_t m2 = next_pow2(nc < 2 ? 2 : nc);
+                    for (uint32_t k = nc + threadIdx.x; k < m2; k += blockDim.x) sh.m.c[k] = FK_KEY_EMPTY;
+                    __syncthreads();
+                    block_bitonic_sort(sh.m.c, m2);
+                    const uint32_t npp = sh.m.np;
+                    for (uint32_t k = threadIdx.x; k < nc; k += blockDim.x) {
+                        const uint64_t key2 = sh.m.c[k];
+                        const bool first = (k == 0) || (sh.m.c[k - 1] != key2);
+                        sh.m.keep[k] = (first && !sorted_contains(sh.m.p, npp, key2)) ? 1u : 0u;
+                    }
+                    __syncthreads();
+                    block_merge(sh.m.p, sh.m.pf, &sh.m.np, sh.m.c, sh.m.keep, nc, cap,
+                                (uint8_t)(FK_FLG_NEW | FK_FLG_ISNEW), sh.m.c2, sh.m.o, sh.m.of, sh.m.part);
+                }
+                if (threadIdx.x == 0) sh.nc = 0;
+                __syncthreads();
+            }
+        }
+    }
+    const uint32_t np = sh.m.np;
+    for (uint32_t j = threadIdx.x; j < np; j += blockDim.x) { pr[j] = sh.m.p[j]; pf[j] = sh.m.pf[j]; }
+    if (threadIdx.x == 0) cnt[i] = np;
+    cc = wave_sum(cc);
+    if (lane_id() == 0 && cc) atomicAdd(cc_acc, cc);
+}
+
+void bridgeview(BuildState &b) {
+    const double t0 = now_sec();
+    const uint32_t N = b.N;
+    const BuildParams &P = b.P;
+    const uint32_t cur_loop = b.loopcount - 1;
+    const uint32_t cur_L = std::min(P.search_L + cur_loop * P.step, P.MAXL);
+    const uint32_t pre_L = cur_loop ? std::min(P.search_L + (cur_loop - 1) * P.step, P.MAXL) : 0;
+    const unsigned gw = (N + 3) / 4;
+    hipLaunchKernelGGL(k_bv_init, dim3(gw), dim3(256), 0, b.s, b.pflg.p, b.cnt.p, N, b.cap, cur_loop, pre_L, cur_L);
+    FK_LAUNCH_CHECK();
+    const uint32_t W = std::min(P.search_L + (b.loopcount - 1) * P.step, P.MAXL);
+    const uint32_t S = P.massq_S;
+    uint64_t cc_total = 0;
+    uint32_t rounds = 0;
+    const size_t lds_row = (size_t)b.ds->dpad * sizeof(float);
+    while (true) {
+        FK_HIP(hipMemsetAsync(b.acc.p, 0, 16, b.s));
+        hipLaunchKernelGGL(k_bv_count, dim3(gw), dim3(256), 0, b.s, b.pflg.p, b.cnt.p, N, b.cap, W, S, b.acc.p);
+        FK_LAUNCH_CHECK();
+        unsigned long long cnts[2] = {0, 0};
+        FK_HIP(hipMemcpyAsync(cnts, b.acc.p, 16, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+        // delta = cnt / nd_ / S_ in float (src/kgraph.cpp:2492); below 0.5 the
+        // round also expands every remaining flagged entry and is the last.
+        const float delta = (float)cnts[0] / (float)N / (float)S;
+        const bool last = delta < 0.5f;
+        const uint32_t scap = last ? 0xFFFFFFFFu : S;
+        FK_HIP(hipMemsetAsync(b.acc.p, 0, 8, b.s));
+        hipLaunchKernelGGL(k_bv_round, dim3(N), dim3(256), lds_row, b.s, b.ds->rows.p, b.ds->stride, b.ds->dpad, N,
+                           b.cap, W, scap, b.pool.p, b.pflg.p, b.cnt.p, b.nsg_id.p, b.nsg_len.p, b.nsg_new.p, b.ncap,
+                           b.acc.p);
+        FK_LAUNCH_CHECK();
+        unsigned long long cc = 0;
+        FK_HIP(hipMemcpyAsync(&cc, b.acc.p, 8, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+        cc_total += cc;
+        ++rounds;
+        if (b.verbose)
+            fprintf(stderr, "[fk] bridgeview round %u: expansions %llu (all %llu) dists %llu%s\n", rounds,
+                    cnts[0], cnts[1], cc, last ? " (last)" : "");
+        if (last) break;
+    }
+    const uint32_t li = b.loopcount - 1;
+    if (li < FK_MAX_LOOPS) {
+        b.st->search_cc[li] = (float)((double)cc_total / ((double)N * (double)(N - 1) / 2.0));
+        b.st->bv_rounds[li] = rounds;
+    }
+    b.st->d_bv += cc_total;
+    b.st->t_bv += now_sec() - t0;
+}
+
+}  // namespace fk

@@ -1,0 +1,470 @@
+// Graph search and connectivity:
+//   * beam search over the out-lists (get_neighbors(const float*, ...),
+//     src/kgraph.cpp:1560-1623), one wavefront per query; used for the entry
+//     point (ComputeEp, :1528-1536) and, batched, for query evaluation
+//   * the centroid (MatrixOracle::Calcenter, src/kgraph-data.h:413-428)
+//   * tree_grow connectivity (src/kgraph.cpp:2145-2270): the reference's DFS
+//     only decides which nodes are reached, so it is a BFS here; nodes left
+//     unreached are attached in ascending id order exactly as findroot does
+//     (first reached pool neighbour with degree < nsg_R, else a random
+//     reached node), each followed by a BFS from the attached node.
+#include <math.h>
+
+#include <algorithm>
+
+#include "fk_api_impl.h"
+#include "fk_build.h"
+#include "fk_device.h"
+#include "fk_sort.h"
+
+namespace fk {
+
+// ---------------------------------------------------------------------------
+// Centroid: per-block column sums in double, then an ordered sum of the block
+// partials (deterministic).  The reference accumulates in float in row order.
+// ---------------------------------------------------------------------------
+#define FK_CEN_ROWS 1024
+__global__ void k_colsum(const float *__restrict__ rows, uint32_t stride, uint32_t d, uint32_t n,
+                         double *__restrict__ part) {
+    const uint32_t r0 = blockIdx.x * FK_CEN_ROWS;
+    const uint32_t r1 = (r0 + FK_CEN_ROWS) < n ? (r0 + FK_CEN_ROWS) : n;
+    for (uint32_t c = threadIdx.x; c < d; c += blockDim.x) {
+        double s = 0.0;
+        for (uint32_t r = r0; r < r1; ++r) s += (double)rows[(size_t)r * stride + c];
+        part[(size_t)blockIdx.x * d + c] = s;
+    }
+}
+
+__global__ void k_colsum_final(const double *__restrict__ part, uint32_t nb, uint32_t d, uint32_t n,
+                               float *__restrict__ center) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d) return;
+    double s = 0.0;
+    for (uint32_t b = 0; b < nb; ++b) s += part[(size_t)b * d + c];
+    center[c] = (float)(s / (double)n);
+}
+
+// ---------------------------------------------------------------------------
+// Beam search, one 64-lane workgroup per query.
+// init_mode 0: start from the first L entries of pool(init_node)
+//              (get_neighbors, :1566-1571), random fill up to L
+// init_mode 1: start from ep and ep's out-list, random fill up to L
+// The visited set is a per-query bitmap of N bits.
+// ---------------------------------------------------------------------------
+#define FK_BEAM_MAX 1024
+
+struct BeamArgs {
+    const float *rows;
+    uint32_t stride, dpad, N;
+    const float *queries;      // n_q x dpad
+    uint32_t n_q;
+    uint32_t L, K;
+    int init_mode;
+    uint32_t init_node;        // mode 0 (per launch)
+    const uint64_t *pool;      // mode 0
+    const uint32_t *pool_cnt;
+    uint32_t pool_cap;
+    uint32_t ep;               // mode 1
+    // graph: either fixed-stride lists or CSR
+    const uint32_t *g_ids;
+    const uint32_t *g_len;     // fixed stride mode
+    uint32_t g_cap;
+    const uint64_t *g_off;     // CSR mode when non-null
+    uint32_t *visited;         // n_q x ((N + 31) / 32) words, zeroed
+    uint64_t seed;
+    uint32_t *out_ids;         // n_q x K
+    float *out_d;
+    unsigned long long *n_dist;
+};
+
+__device__ __forceinline__ bool test_and_set(uint32_t *bm, uint32_t id) {
+    const uint32_t bit = 1u << (id & 31u);
+    return (atomicOr(&bm[id >> 5], bit) & bit) == 0;
+}
+
+__global__ __launch_bounds__(64) void k_beam(BeamArgs a) {
+    __shared__ uint64_t s_key[FK_BEAM_MAX + 64];
+    __shared__ uint8_t s_flag[FK_BEAM_MAX + 64];
+    __shared__ uint64_t s_o[FK_BEAM_MAX + 64];
+    __shared__ uint8_t s_of[FK_BEAM_MAX + 64];
+    __shared__ uint32_t s_init[FK_BEAM_MAX];
+    __shared__ __align__(16) float s_q[1024];
+    __shared__ uint32_t s_n;
+    const uint32_t q = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    const float *qv = a.queries + (size_t)q * a.dpad;
+    for (uint32_t k = lane; k < a.dpad; k += 64) s_q[k] = qv[k];
+    uint32_t *bm = a.visited + (size_t)q * ((a.N + 31) / 32);
+    const uint32_t L = a.L;
+    // initial ids
+    if (lane == 0) {
+        uint32_t n = 0;
+        if (a.init_mode == 0) {
+            const uint32_t pc = a.pool_cnt[a.init_node];
+            const uint64_t *pr = a.pool + (uint64_t)a.init_node * a.pool_cap;
+            for (uint32_t i = 0; i < pc && n < L; ++i) {
+                const uint32_t id = key_id(pr[i]);
+                if (id < a.N && test_and_set(bm, id)) s_init[n++] = id;
+            }
+        } else {
+            if (test_and_set(bm, a.ep)) s_init[n++] = a.ep;
+            const uint64_t o = a.g_off ? a.g_off[a.ep] : (uint64_t)a.ep * a.g_cap;
+            const uint32_t deg = a.g_off ? (uint32_t)(a.g_off[a.ep + 1] - o) : a.g_len[a.ep];
+            for (uint32_t m = 0; m < deg && n < L; ++m) {
+                const uint32_t id = a.g_ids[o + m];
+                if (id < a.N && test_and_set(bm, id)) s_init[n++] = id;
+            }
+        }
+        uint64_t k = 0;
+        const uint32_t want = L < a.N ? L : a.N;
+        while (n < want) {
+            const uint32_t id = fk_rand32(a.seed, q, k++) % a.N;
+            if (test_and_set(bm, id)) s_init[n++] = id;
+        }
+        s_n = n;
+    }
+    __syncthreads();
+    uint32_t Lc = s_n;
+    unsigned long long nd = 0;
+    const uint32_t np2 = next_pow2(Lc < 2 ? 2 : Lc);
+    for (uint32_t k = lane; k < np2; k += 64) {
+        if (k < Lc) {
+            const uint32_t id = s_init[k];
+            s_key[k] = make_key(l2_canon(s_q, a.rows + (size_t)id * a.stride, a.dpad), id);
+            ++nd;
+        } else {
+            s_key[k] = FK_KEY_EMPTY;
+        }
+    }
+    __syncthreads();
+    block_bitonic_sort(s_key, np2);
+    for (uint32_t k = lane; k < Lc; k += 64) s_flag[k] = 1;
+    __syncthreads();
+    uint32_t k = 0;
+    while (k < Lc) {
+        uint32_t nk = Lc;
+        if (s_flag[k]) {
+            __syncthreads();
+            if (lane == 0) s_flag[k] = 0;
+            const uint32_t n = key_id(s_key[k]);
+            const uint64_t o = a.g_off ? a.g_off[n] : (uint64_t)n * a.g_cap;
+            const uint32_t deg = a.g_off ? (uint32_t)(a.g_off[n + 1] - o) : a.g_len[n];
+            for (uint32_t m0 = 0; m0 < deg; m0 += 64) {
+                const uint32_t m = m0 + lane;
+                uint64_t cand = FK_KEY_EMPTY;
+                const float worst = key_dist(s_key[Lc - 1]);
+                if (m < deg) {
+                    const uint32_t id = a.g_ids[o + m];
+                    if (id < a.N && test_and_set(bm, id)) {
+                        ++nd;
+                        const float dd = l2_canon(s_q, a.rows + (size_t)id * a.stride, a.dpad);
+                        if (dd < worst) cand = make_key(dd, id);  // `dist >= pool[L-1].dist` skips
+                    }
+                }
+                const uint64_t sorted = wave_sort64(cand);
+                const uint64_t mv = wave_ballot(sorted != FK_KEY_EMPTY);
+                const uint32_t nc = (uint32_t)__popcll(mv);
+                if (nc == 0) continue;
+                // rank-merge the sorted candidates (lanes 0..nc-1) into the pool
+                __syncthreads();
+                const uint32_t lim = (Lc + nc) < L ? (Lc + nc) : L;
+                if (lane < nc) s_o[FK_BEAM_MAX + lane] = sorted;
+                __syncthreads();
+                const uint64_t *cs = s_o + FK_BEAM_MAX;
+                for (uint32_t j = lane; j < Lc; j += 64) {
+                    const uint32_t pos = j + lower_bound_cnt(cs, nc, s_key[j]);
+                    if (pos < lim) { s_o[pos] = s_key[j]; s_of[pos] = s_flag[j]; }
+                }
+                uint32_t minpos = 0xFFFFFFFFu;
+                if (lane < nc) {
+                    const uint32_t pos = lane + lower_bound_cnt(s_key, Lc, sorted);
+                    if (pos < lim) { s_o[pos] = sorted; s_of[pos] = 1; minpos = pos; }
+                }
+                // smallest insertion position (lane 0 holds the smallest key)
+                const uint32_t r0 = __shfl(minpos, 0, 64);
+                __syncthreads();
+                for (uint32_t j = lane; j < lim; j += 64) { s_key[j] = s_o[j]; s_flag[j] = s_of[j]; }
+                __syncthreads();
+                Lc = lim;
+                if (r0 < nk) nk = r0;
+            }
+        }
+        if (nk <= k) k = nk; else ++k;
+        __syncthreads();
+    }
+    const uint32_t kk = a.K;
+    for (uint32_t j = lane; j < kk; j += 64) {
+        const uint64_t key = j < Lc ? s_key[j] : FK_KEY_EMPTY;
+        a.out_ids[(size_t)q * kk + j] = key_id(key);
+        if (a.out_d) a.out_d[(size_t)q * kk + j] = key_dist(key);
+    }
+    if (a.n_dist) {
+        nd = wave_sum(nd);
+        if (lane == 0) atomicAdd(a.n_dist, nd);
+    }
+}
+
+static void launch_beam(hipStream_t s, const BeamArgs &a) {
+    FK_CHECK_ARG(a.L >= 1 && a.L <= FK_BEAM_MAX, "search_L must be in [1, 1024]");
+    FK_CHECK_ARG(a.dpad <= 1024, "dimension above 1024 is not supported by the beam search");
+    FK_CHECK_ARG(a.K <= a.L, "K must not exceed search_L");
+    hipLaunchKernelGGL(k_beam, dim3(a.n_q), dim3(64), 0, s, a);
+    FK_LAUNCH_CHECK();
+}
+
+// ComputeEp (src/kgraph.cpp:1528-1536).
+void compute_ep(BuildState &b) {
+    const double t0 = now_sec();
+    const Dataset &ds = *b.ds;
+    DevBuf<float> center(ds.dpad);
+    FK_HIP(hipMemsetAsync(center.p, 0, center.bytes(), b.s));
+    {
+        const uint32_t nb = (ds.n_total + FK_CEN_ROWS - 1) / FK_CEN_ROWS;
+        DevBuf<double> part((size_t)nb * ds.d);
+        hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, b.s, ds.rows.p, ds.stride, ds.d, ds.n_total, part.p);
+        FK_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_colsum_final, dim3((ds.d + 255) / 256), dim3(256), 0, b.s, part.p, nb, ds.d,
+                           ds.n_total, center.p);
+        FK_LAUNCH_CHECK();
+        b.sync();
+    }
+    DevBuf<uint32_t> vis((b.N + 31) / 32);
+    FK_HIP(hipMemsetAsync(vis.p, 0, vis.bytes(), b.s));
+    DevBuf<uint32_t> oid(1);
+    BeamArgs a{};
+    a.rows = ds.rows.p;
+    a.stride = ds.stride;
+    a.dpad = ds.dpad;
+    a.N = b.N;
+    a.queries = center.p;
+    a.n_q = 1;
+    a.L = b.P.search_L;
+    a.K = 1;
+    a.init_mode = 0;
+    a.init_node = (uint32_t)(b.next_rand() % b.N);
+    a.pool = b.pool.p;
+    a.pool_cnt = b.cnt.p;
+    a.pool_cap = b.cap;
+    a.g_ids = b.nsg_id.p;
+    a.g_len = b.nsg_len.p;
+    a.g_cap = b.ncap;
+    a.visited = vis.p;
+    a.seed = b.next_rand();
+    a.out_ids = oid.p;
+    a.n_dist = b.acc.p + 4;
+    FK_HIP(hipMemsetAsync(b.acc.p + 4, 0, 8, b.s));
+    launch_beam(b.s, a);
+    unsigned long long nd = 0;
+    FK_HIP(hipMemcpyAsync(&b.ep, oid.p, 4, hipMemcpyDeviceToHost, b.s));
+    FK_HIP(hipMemcpyAsync(&nd, b.acc.p + 4, 8, hipMemcpyDeviceToHost, b.s));
+    b.sync();
+    b.st->d_ep += nd;
+    b.st->t_ep += now_sec() - t0;
+    if (b.verbose) fprintf(stderr, "[fk] ep: %u\n", b.ep);
+}
+
+// ---------------------------------------------------------------------------
+// BFS over the out-lists.
+// ---------------------------------------------------------------------------
+__global__ void k_bfs_expand(const uint32_t *__restrict__ nsg_id, const uint32_t *__restrict__ nsg_len,
+                             uint32_t ncap, const uint32_t *__restrict__ front, uint32_t nf,
+                             uint32_t *__restrict__ vis, uint32_t *__restrict__ next, uint32_t *__restrict__ nnext) {
+    const uint32_t w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (w >= nf) return;
+    const uint32_t f = front[w];
+    const uint32_t deg = nsg_len[f];
+    for (uint32_t k = lane_id(); k < deg; k += 64) {
+        const uint32_t t = nsg_id[(uint64_t)f * ncap + k];
+        if (atomicExch(&vis[t], 1u) == 0u) next[atomicAdd(nnext, 1u)] = t;
+    }
+}
+
+__global__ void k_set_edges(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
+                            const uint32_t *__restrict__ src, const uint32_t *__restrict__ slot,
+                            const uint32_t *__restrict__ dst, uint32_t n, uint32_t ncap, uint32_t *__restrict__ nsg_id,
+                            float *__restrict__ nsg_d) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t p = (uint64_t)src[i] * ncap + slot[i];
+    nsg_id[p] = dst[i];
+    nsg_d[p] = l2_canon(rows + (size_t)src[i] * stride, rows + (size_t)dst[i] * stride, dpad);
+}
+
+struct Bfs {
+    BuildState &b;
+    DevBuf<uint32_t> vis, f0, f1, cnt;
+    std::vector<uint8_t> hvis;
+    std::vector<uint32_t> hbuf;
+    explicit Bfs(BuildState &bb) : b(bb), vis(bb.N), f0(bb.N), f1(bb.N), cnt(1), hvis(bb.N, 0), hbuf(bb.N) {
+        FK_HIP(hipMemsetAsync(vis.p, 0, vis.bytes(), b.s));
+    }
+    // Marks everything reachable from `root` (which must be unreached);
+    // returns the number of nodes newly reached.
+    uint32_t run(uint32_t root) {
+        const uint32_t one = 1;
+        FK_HIP(hipMemcpyAsync(vis.p + root, &one, 4, hipMemcpyHostToDevice, b.s));
+        FK_HIP(hipMemcpyAsync(f0.p, &root, 4, hipMemcpyHostToDevice, b.s));
+        b.sync();
+        hvis[root] = 1;
+        uint32_t nf = 1, total = 1;
+        uint32_t *cur = f0.p, *nxt = f1.p;
+        while (nf) {
+            FK_HIP(hipMemsetAsync(cnt.p, 0, 4, b.s));
+            hipLaunchKernelGGL(k_bfs_expand, dim3((nf + 3) / 4), dim3(256), 0, b.s, b.nsg_id.p, b.nsg_len.p, b.ncap,
+                               cur, nf, vis.p, nxt, cnt.p);
+            FK_LAUNCH_CHECK();
+            uint32_t nn = 0;
+            FK_HIP(hipMemcpyAsync(&nn, cnt.p, 4, hipMemcpyDeviceToHost, b.s));
+            b.sync();
+            if (nn) {
+                FK_HIP(hipMemcpyAsync(hbuf.data(), nxt, (size_t)nn * 4, hipMemcpyDeviceToHost, b.s));
+                b.sync();
+                for (uint32_t i = 0; i < nn; ++i) hvis[hbuf[i]] = 1;
+            }
+            total += nn;
+            nf = nn;
+            std::swap(cur, nxt);
+        }
+        return total;
+    }
+};
+
+void tree_grow(BuildState &b) {
+    const double t0 = now_sec();
+    const uint32_t N = b.N;
+    Bfs bfs(b);
+    uint32_t reached = bfs.run(b.ep);
+    std::vector<uint32_t> deg(N);
+    FK_HIP(hipMemcpyAsync(deg.data(), b.nsg_len.p, (size_t)N * 4, hipMemcpyDeviceToHost, b.s));
+    b.sync();
+    std::vector<uint32_t> e_src, e_slot, e_dst;
+    std::vector<uint64_t> prow(b.cap);
+    uint32_t start = 0;
+    while (reached < N) {
+        uint32_t id = N;
+        for (uint32_t i = start; i < N; ++i)
+            if (!bfs.hvis[i]) { id = i; start = i; break; }
+        if (id == N) break;
+        // findroot: first reached pool neighbour with room (:2201-2210)
+        uint32_t pc = 0;
+        FK_HIP(hipMemcpyAsync(&pc, b.cnt.p + id, 4, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+        if (pc) {
+            FK_HIP(hipMemcpyAsync(prow.data(), b.pool.p + (uint64_t)id * b.cap, (size_t)pc * 8,
+                                  hipMemcpyDeviceToHost, b.s));
+            b.sync();
+        }
+        uint32_t root = N;
+        for (uint32_t i = 0; i < pc; ++i) {
+            const uint32_t p = (uint32_t)prow[i];
+            if (p < N && bfs.hvis[p] && deg[p] < b.P.nsg_R) { root = p; break; }
+        }
+        if (root == N) {
+            // random reached node whose degree is below nsg_R * factor (:2212-2228)
+            uint64_t tries = 0;
+            while (true) {
+                ++tries;
+                const uint64_t factor = tries / N + 1;
+                const uint32_t rid = (uint32_t)(b.next_rand() % N);
+                if (bfs.hvis[rid] && deg[rid] < b.ncap && deg[rid] < b.P.nsg_R * factor) { root = rid; break; }
+                if (tries > 64ull * N) throw Error(FK_E_INTERNAL, "tree_grow: no node can take an edge");
+            }
+        }
+        e_src.push_back(root);
+        e_slot.push_back(deg[root]);
+        e_dst.push_back(id);
+        // the new edge is needed by the BFS below
+        const uint32_t nd = deg[root] + 1;
+        FK_HIP(hipMemcpyAsync(b.nsg_id.p + (uint64_t)root * b.ncap + deg[root], &id, 4, hipMemcpyHostToDevice, b.s));
+        FK_HIP(hipMemcpyAsync(b.nsg_len.p + root, &nd, 4, hipMemcpyHostToDevice, b.s));
+        b.sync();
+        deg[root] = nd;
+        reached += bfs.run(id);
+    }
+    if (!e_src.empty()) {
+        const uint32_t n = (uint32_t)e_src.size();
+        DevBuf<uint32_t> s(n), sl(n), d(n);
+        FK_HIP(hipMemcpyAsync(s.p, e_src.data(), (size_t)n * 4, hipMemcpyHostToDevice, b.s));
+        FK_HIP(hipMemcpyAsync(sl.p, e_slot.data(), (size_t)n * 4, hipMemcpyHostToDevice, b.s));
+        FK_HIP(hipMemcpyAsync(d.p, e_dst.data(), (size_t)n * 4, hipMemcpyHostToDevice, b.s));
+        hipLaunchKernelGGL(k_set_edges, dim3((n + 255) / 256), dim3(256), 0, b.s, b.ds->rows.p, b.ds->stride,
+                           b.ds->dpad, s.p, sl.p, d.p, n, b.ncap, b.nsg_id.p, b.nsg_d.p);
+        FK_LAUNCH_CHECK();
+        b.sync();
+        b.st->d_conn += n;
+    }
+    b.st->n_attached += (uint32_t)e_src.size();
+    uint32_t mx = b.width;
+    for (uint32_t i = 0; i < N; ++i) mx = std::max(mx, deg[i]);
+    b.width = mx;
+    const double dt = now_sec() - t0;
+    const uint32_t li = b.loopcount;
+    if (li < FK_MAX_LOOPS) b.st->tree_grow_time[li] = dt;
+    b.st->t_conn += dt;
+    if (b.verbose) fprintf(stderr, "[fk] tree_grow: attached %zu (%.3f s)\n", e_src.size(), dt);
+}
+
+}  // namespace fk
+
+using namespace fk;
+
+extern "C" fk_status fk_search(fk_handle *h, const uint64_t *offsets, const uint32_t *ids, uint32_t n, uint32_t ep,
+                               const float *queries, uint32_t n_q, uint32_t search_L, uint32_t K, uint32_t *out_ids,
+                               uint64_t *n_dist) {
+    if (!h) return FK_E_INVALID;
+    return guard(h, [&] {
+        require_dataset(h);
+        FK_CHECK_ARG(offsets && queries && out_ids, "null pointer");
+        FK_CHECK_ARG(n >= 1 && n <= h->ds.n_total && ep < n, "graph size / entry point out of range");
+        FK_CHECK_ARG(search_L >= K && K >= 1, "need 1 <= K <= search_L");
+        const uint64_t ne = offsets[n];
+        for (uint64_t i = 0; i < ne; ++i) FK_CHECK_ARG(ids[i] < n, "graph id out of range");
+        if (n_dist) *n_dist = 0;
+        if (n_q == 0) return;
+        select_device(h);
+        const Dataset &ds = h->ds;
+        DevBuf<uint64_t> doff((size_t)n + 1);
+        DevBuf<uint32_t> dids(ne);
+        FK_HIP(hipMemcpyAsync(doff.p, offsets, ((size_t)n + 1) * 8, hipMemcpyHostToDevice, h->stream));
+        if (ne) FK_HIP(hipMemcpyAsync(dids.p, ids, ne * 4, hipMemcpyHostToDevice, h->stream));
+        const uint32_t words = (n + 31) / 32;
+        uint32_t qb = (uint32_t)std::min<uint64_t>(n_q, std::max<uint64_t>(1, (512ull << 20) / ((uint64_t)words * 4)));
+        DevBuf<uint32_t> vis((size_t)qb * words);
+        DevBuf<float> dq((size_t)qb * ds.dpad);
+        DevBuf<uint32_t> dout((size_t)qb * K);
+        DevBuf<unsigned long long> cnt(1);
+        FK_HIP(hipMemsetAsync(cnt.p, 0, 8, h->stream));
+        for (uint32_t q0 = 0; q0 < n_q; q0 += qb) {
+            const uint32_t nq = std::min(qb, n_q - q0);
+            FK_HIP(hipMemsetAsync(vis.p, 0, (size_t)nq * words * 4, h->stream));
+            if (ds.dpad != ds.d) FK_HIP(hipMemsetAsync(dq.p, 0, dq.bytes(), h->stream));
+            FK_HIP(hipMemcpy2DAsync(dq.p, (size_t)ds.dpad * 4, queries + (size_t)q0 * ds.d, (size_t)ds.d * 4,
+                                    (size_t)ds.d * 4, nq, hipMemcpyHostToDevice, h->stream));
+            BeamArgs a{};
+            a.rows = ds.rows.p;
+            a.stride = ds.stride;
+            a.dpad = ds.dpad;
+            a.N = n;
+            a.queries = dq.p;
+            a.n_q = nq;
+            a.L = search_L;
+            a.K = K;
+            a.init_mode = 1;
+            a.ep = ep;
+            a.g_ids = dids.p;
+            a.g_off = doff.p;
+            a.visited = vis.p;
+            a.seed = 0x9E3779B97F4A7C15ull ^ q0;
+            a.out_ids = dout.p;
+            a.n_dist = cnt.p;
+            launch_beam(h->stream, a);
+            FK_HIP(hipMemcpyAsync(out_ids + (size_t)q0 * K, dout.p, (size_t)nq * K * 4, hipMemcpyDeviceToHost,
+                                  h->stream));
+            FK_HIP(hipStreamSynchronize(h->stream));
+        }
+        unsigned long long c = 0;
+        FK_HIP(hipMemcpyAsync(&c, cnt.p, 8, hipMemcpyDeviceToHost, h->stream));
+        FK_HIP(hipStreamSynchronize(h->stream));
+        if (n_dist) *n_dist = c;
+    });
+}

@@ -1,0 +1,156 @@
+"""NumPy-facing wrapper of one fk_handle (one GPU, one HIP stream, one
+resident dataset).  Every method calls the C ABI; inputs and outputs are host
+NumPy arrays."""
+import ctypes
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import FkError, FkGraph, FkParams, FkStats, ptr
+
+
+def _as(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class Device:
+    def __init__(self, device=0):
+        self.lib = _ffi.load()
+        h = ctypes.c_void_p()
+        st = self.lib.fk_open(device, ctypes.byref(h))
+        if st != _ffi.FK_OK:
+            raise FkError(st, f"fk_open({device}) failed (no gfx950 device?)")
+        self.h = h
+        self.n_total = 0
+        self.d = 0
+
+    def close(self):
+        if self.h:
+            self.lib.fk_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, what):
+        if st != _ffi.FK_OK:
+            msg = self.lib.fk_last_error(self.h)
+            raise FkError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    # -- dataset -----------------------------------------------------------
+    def upload(self, rows):
+        rows = _as(rows, np.float32)
+        assert rows.ndim == 2
+        self._check(
+            self.lib.fk_dataset_upload(self.h, ptr(rows), rows.strides[0], rows.shape[0], rows.shape[1]),
+            "fk_dataset_upload",
+        )
+        self.n_total, self.d = rows.shape
+
+    # -- unit entry points -------------------------------------------------
+    def pair_distances(self, a, b):
+        a = _as(a, np.uint32)
+        b = _as(b, np.uint32)
+        out = np.empty(len(a), np.float32)
+        self._check(self.lib.fk_pair_distances(self.h, ptr(a), ptr(b), len(a), ptr(out)), "fk_pair_distances")
+        return out
+
+    def join_distances(self, ids, list_off, n_new):
+        ids = _as(ids, np.uint32)
+        list_off = _as(list_off, np.uint64)
+        n_new = _as(n_new, np.uint32)
+        nn = n_new.astype(np.uint64)
+        T = np.diff(list_off)
+        total = int(np.sum(nn * (nn - (nn > 0)) // 2 + nn * (T - nn)))
+        out = np.empty(max(total, 1), np.float32)
+        npairs = ctypes.c_uint64()
+        self._check(
+            self.lib.fk_join_distances(self.h, ptr(ids), ptr(list_off), ptr(n_new), len(n_new), ptr(out),
+                                       ctypes.byref(npairs)),
+            "fk_join_distances",
+        )
+        assert npairs.value == total
+        return out[:total]
+
+    def prune(self, pg_type, pools, pool_len, q, oracle_size, params, reversed_=False):
+        pools = _as(pools, np.uint64)
+        pool_len = _as(pool_len, np.uint32)
+        n_tasks, stride = pools.shape
+        q = None if q is None else _as(q, np.uint32)
+        R = params.nsg_R
+        out_ids = np.zeros((n_tasks, R), np.uint32)
+        out_d = np.zeros((n_tasks, R), np.float32)
+        out_len = np.zeros(n_tasks, np.uint32)
+        cc = ctypes.c_uint64()
+        p = _ffi.params_struct(params)
+        self._check(
+            self.lib.fk_prune(self.h, pg_type, ptr(pools), ptr(pool_len), stride, ptr(q), n_tasks, oracle_size,
+                              ctypes.byref(p), 1 if reversed_ else 0, ptr(out_ids), ptr(out_d), R, ptr(out_len),
+                              ctypes.byref(cc)),
+            "fk_prune",
+        )
+        return out_ids, out_d, out_len, cc.value
+
+    def knn(self, queries, K, n_used=None, self_ids=None):
+        queries = _as(queries, np.float32)
+        n_q = queries.shape[0]
+        n_used = self.n_total if n_used is None else n_used
+        self_ids = None if self_ids is None else _as(self_ids, np.uint32)
+        ids = np.empty((n_q, K), np.uint32)
+        d = np.empty((n_q, K), np.float32)
+        self._check(
+            self.lib.fk_knn_bruteforce(self.h, ptr(queries), n_q, n_used, K, ptr(self_ids), ptr(ids), ptr(d)),
+            "fk_knn_bruteforce",
+        )
+        return ids, d
+
+    def nndescent(self, params, n_used=None):
+        n_used = self.n_total if n_used is None else n_used
+        L = min(params.L, n_used - 1)
+        pools = np.zeros((n_used, L), np.uint64)
+        plen = np.zeros(n_used, np.uint32)
+        st = FkStats()
+        p = _ffi.params_struct(params)
+        self._check(
+            self.lib.fk_nndescent(self.h, n_used, ctypes.byref(p), ptr(pools), ptr(plen), L, ctypes.byref(st)),
+            "fk_nndescent",
+        )
+        return pools, plen, st.to_dict()
+
+    def build(self, pg_type, params, n_used=None):
+        """Full build; returns (offsets, ids, ep, width, stats dict)."""
+        n_used = self.n_total if n_used is None else n_used
+        g = FkGraph()
+        p = _ffi.params_struct(params)
+        self._check(self.lib.fk_build(self.h, pg_type, n_used, ctypes.byref(p), ctypes.byref(g)), "fk_build")
+        try:
+            n = g.n
+            off = np.ctypeslib.as_array(g.offsets, shape=(n + 1,)).copy()
+            ne = int(off[-1])
+            ids = np.ctypeslib.as_array(g.ids, shape=(max(ne, 1),))[:ne].copy()
+            return off, ids, int(g.ep), int(g.width), g.stats.to_dict()
+        finally:
+            self.lib.fk_graph_free(ctypes.byref(g))
+
+    def search(self, offsets, ids, ep, queries, search_L, K):
+        offsets = _as(offsets, np.uint64)
+        ids = _as(ids, np.uint32)
+        queries = _as(queries, np.float32)
+        n = len(offsets) - 1
+        out = np.empty((queries.shape[0], K), np.uint32)
+        nd = ctypes.c_uint64()
+        self._check(
+            self.lib.fk_search(self.h, ptr(offsets), ptr(ids), n, ep, ptr(queries), queries.shape[0], search_L, K,
+                               ptr(out), ctypes.byref(nd)),
+            "fk_search",
+        )
+        return out, nd.value

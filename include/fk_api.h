@@ -1,0 +1,184 @@
+/*
+ * fk_api.h -- C ABI of the MI355X (gfx950) graph-construction engine.
+ *
+ * Plain C, no C++ or torch types: pointers, sizes and a status code.  One
+ * handle owns one HIP stream and one resident dataset on one GPU.  Calls on a
+ * handle are serialised and a handle is not thread-safe; separate handles are
+ * independent.  Every entry point returns an fk_status and never throws; the
+ * text of the last failure is kept per handle (fk_last_error).
+ *
+ * Reference interfaces replaced (paths relative to the FastKCNA sources):
+ *   fk_build            KGraphImpl::buildFastNSG / buildFastTauMNG /
+ *                       buildFastAlphaPG / buildFastHNSW
+ *                       (src/kgraph.h:230-233, src/kgraph.cpp:3720-3966)
+ *   fk_dataset_upload   MatrixOracle<float, metric::l2sqr> construction over a
+ *                       Matrix<float> (src/kgraph-data.h:349-372); the oracle's
+ *                       size()/set_size() prefix is fk_build's n_used
+ *   fk_pair_distances   IndexOracle::operator()(i, j)
+ *                       (src/kgraph.h:72, src/kgraph-data.h:392-395 ->
+ *                       src/metric.cpp:45-78)
+ *   fk_join_distances   the local join's distance calls, KGraphConstructor::join
+ *                       + Nhood::join (src/kgraph.cpp:970-987, 1097-1117)
+ *   fk_prune            nsg_sync_prune / taumg_sync_prune / alphapg_sync_prune
+ *                       and their *_reversed_prune variants
+ *                       (src/kgraph.cpp:1809-1851, 2012-2048, 2833-2876,
+ *                       3038-3083, 3157-3203, 3374-3422)
+ *   fk_nndescent        the KGraphConstructor constructor's NN-descent loop
+ *                       (src/kgraph.cpp:1281-1459)
+ *   fk_knn_bruteforce   LinearSearch / GenerateControl (src/kgraph.cpp:309-387)
+ *   fk_search           get_neighbors(const float*, ...) beam search
+ *                       (src/kgraph.cpp:1560-1623), batched over queries
+ */
+#ifndef FK_API_H
+#define FK_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    FK_OK = 0,
+    FK_E_INVALID = 1,   /* bad argument; maps to kgraph::invalid_argument   */
+    FK_E_NOMEM = 2,     /* device or host allocation failed                */
+    FK_E_DEVICE = 3,    /* HIP runtime / launch failure                    */
+    FK_E_IO = 4,        /* file error; maps to kgraph::io_error            */
+    FK_E_INTERNAL = 5,  /* invariant broken inside the library             */
+    FK_E_NODEVICE = 6   /* no usable gfx950 device                         */
+} fk_status;
+
+/* Proximity-graph family; values equal the reference's PG_type codes
+ * (src/kgraph.cpp:3741, 3816, 3874, 3931). */
+typedef enum {
+    FK_PG_NSG = 1,
+    FK_PG_HNSW_LAYER = 2,
+    FK_PG_TAUMNG = 3,
+    FK_PG_ALPHAPG = 4
+} fk_pg_type;
+
+/* Flags (fk_params.flags). */
+#define FK_FLAG_VERBOSE 0x1u     /* per-iteration log lines on stderr            */
+#define FK_FLAG_NO_CONTROLS 0x2u /* skip the sampled ground truth (recall = 0)   */
+
+/* KGraph::IndexParams fields used on the build path (src/kgraph.h:116-170),
+ * one for one, plus flags. */
+typedef struct {
+    uint32_t K, L, S, R, iterations, controls, seed;
+    float delta, recall;
+    uint32_t nsg_R, search_L, search_K, step, massq_S, loop_i;
+    float tau, angle;
+    uint32_t flags;
+} fk_params;
+
+#define FK_MAX_LOOPS 16
+
+/* Per-build statistics.  Distance-evaluation counters follow the
+ * reference's counting points: d_join = n_comps (src/kgraph.cpp:1108),
+ * d_prune = prune_cc * N(N-1)/2 (:2115), d_bv = search_cc * N(N-1)/2 (:2503);
+ * the others are not counted by the reference.  Times are seconds. */
+typedef struct {
+    uint64_t d_init, d_ctrl, d_join, d_prune, d_prune_rev, d_inter, d_ep, d_bv, d_conn;
+    double t_total, t_ctrl, t_nnd, t_prune, t_inter, t_ep, t_conn, t_bv, t_upload;
+    uint32_t nnd_iterations;      /* NN-descent joins run                     */
+    uint32_t nnd_stop;            /* 0 iterations, 1 delta, 2 recall           */
+    float nnd_recall, nnd_delta;  /* last NN-descent stop-rule values         */
+    uint32_t n_loops;             /* prune loops run (loop_i + 1)             */
+    uint32_t n_kgraph, n_recall;  /* entries used in kgraph_time / recall     */
+    double kgraph_time[FK_MAX_LOOPS + 64];
+    float recall[FK_MAX_LOOPS + 64];
+    double prune_time[FK_MAX_LOOPS], inter_time[FK_MAX_LOOPS], tree_grow_time[FK_MAX_LOOPS];
+    double buildnsg_time[FK_MAX_LOOPS], search_time[FK_MAX_LOOPS], con_time[FK_MAX_LOOPS];
+    float prune_cc[FK_MAX_LOOPS], search_cc[FK_MAX_LOOPS];
+    uint32_t aod;                 /* average out-degree, integer as reference */
+    uint32_t max_degree, min_degree;
+    uint32_t n_attached;          /* edges added by the connectivity pass     */
+    uint32_t bv_rounds[FK_MAX_LOOPS];
+} fk_stats;
+
+/* Output graph in CSR form, allocated by the library (fk_graph_free). */
+typedef struct {
+    uint32_t n, ep, width;
+    uint64_t *offsets; /* n + 1 entries */
+    uint32_t *ids;     /* offsets[n] entries, each node's list in reference order */
+    fk_stats stats;
+} fk_graph;
+
+typedef struct fk_handle fk_handle;
+
+/* Open device `device` (HIP ordinal).  Fails with FK_E_NODEVICE when it is
+ * not present. */
+fk_status fk_open(int device, fk_handle **h);
+void fk_close(fk_handle *h);
+const char *fk_last_error(const fk_handle *h);
+/* Library version string (KGraph::version, src/kgraph.h:266). */
+const char *fk_version(void);
+
+/* Copy n_total rows of d float32 values (host memory, row_stride_bytes apart)
+ * to the device.  Replaces any previously resident dataset. */
+fk_status fk_dataset_upload(fk_handle *h, const float *rows, size_t row_stride_bytes,
+                            uint32_t n_total, uint32_t d);
+
+/* Full FastKCNA build over the first n_used resident rows (the oracle's
+ * size(); Calcenter still averages all n_total rows, src/kgraph-data.h:413-428).
+ * Runs sampled ground truth, NN-descent, loop_i+1 prune loops with reverse
+ * edges, entry point and connectivity, and loop_i BridgeView passes.  For
+ * FK_PG_HNSW_LAYER the connectivity pass is skipped (src/kgraph.cpp:2326-2337).
+ * The graph is returned in *out (free with fk_graph_free). */
+fk_status fk_build(fk_handle *h, fk_pg_type type, uint32_t n_used, const fk_params *p,
+                   fk_graph *out);
+void fk_graph_free(fk_graph *g);
+
+/* Squared L2 between resident rows a[k] and b[k], k < n, in the reference's
+ * float_l2sqr_avx summation order (bit-identical results). */
+fk_status fk_pair_distances(fk_handle *h, const uint32_t *a, const uint32_t *b, size_t n,
+                            float *out);
+
+/* Local-join distances.  For node t < n_nodes, its sample list is
+ * ids[list_off[t] .. list_off[t+1]), whose first n_new[t] entries are "new" and
+ * the rest "old".  Pairs are (x, y) with x < n_new[t] and x < y, enumerated
+ * row-major in x, then y; out[pair_off[t] + k] receives the k-th pair's
+ * distance, pair_off being the exclusive prefix sum of
+ * n_new*(n_new-1)/2 + n_new*n_old.  *n_pairs receives the total. */
+fk_status fk_join_distances(fk_handle *h, const uint32_t *ids, const uint64_t *list_off,
+                            const uint32_t *n_new, uint32_t n_nodes, float *out,
+                            uint64_t *n_pairs);
+
+/* Occlusion prune of n_tasks candidate pools.  Pool t holds pool_len[t]
+ * packed entries (float distance bits << 32 | id), sorted ascending, at
+ * pools[t * pool_stride].  q[t] is the pool's owner node.  reversed != 0 runs
+ * the *_reversed_prune form (no search_K cap, no id < size() guard).
+ * Output: out_ids/out_dists[t * out_stride + k] for k < out_len[t] (at most
+ * nsg_R, out_stride >= nsg_R), *cc = distances computed.  Rows are the resident
+ * dataset; oracle_size is the reference's oracle.size(). */
+fk_status fk_prune(fk_handle *h, fk_pg_type type, const uint64_t *pools, const uint32_t *pool_len,
+                   uint32_t pool_stride, const uint32_t *q, uint32_t n_tasks, uint32_t oracle_size,
+                   const fk_params *p, int reversed, uint32_t *out_ids, float *out_dists,
+                   uint32_t out_stride, uint32_t *out_len, uint64_t *cc);
+
+/* NN-descent only (the KGraphConstructor loop) over the first n_used rows.
+ * pools receives n_used * pool_stride packed entries (pool_stride >= L), each
+ * node's first pool_len[i] sorted ascending; stats may be NULL. */
+fk_status fk_nndescent(fk_handle *h, uint32_t n_used, const fk_params *p, uint64_t *pools,
+                       uint32_t *pool_len, uint32_t pool_stride, fk_stats *stats);
+
+/* Exact K nearest rows (among the first n_used) of n_q query vectors given
+ * in host memory (n_q x d, contiguous), ties by ascending id; exclude_self
+ * != 0 treats query i as row self_ids[i] and skips it (LinearSearch). */
+fk_status fk_knn_bruteforce(fk_handle *h, const float *queries, uint32_t n_q, uint32_t n_used,
+                            uint32_t K, const uint32_t *self_ids, uint32_t *out_ids,
+                            float *out_dists);
+
+/* Beam search of n_q host queries over a CSR graph (the layout fk_build
+ * returns) from entry point ep with beam width search_L; writes the best K
+ * ids per query. */
+fk_status fk_search(fk_handle *h, const uint64_t *offsets, const uint32_t *ids, uint32_t n,
+                    uint32_t ep, const float *queries, uint32_t n_q, uint32_t search_L,
+                    uint32_t K, uint32_t *out_ids, uint64_t *n_dist);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FK_API_H */

@@ -1,0 +1,217 @@
+"""GPU parity through the C ABI against the CPU oracle, at sizes the oracle
+finishes in seconds.
+
+Bars: distances bit-exact against the reference order (and within 1e-6
+relative of FP64 on float data); prune ids, kept distances and distance
+counts bit-exact; exact kNN ids bit-exact; NN-descent / full builds by
+recall against brute force (no worse than the oracle, with a stated margin
+for the sampling RNG) and graph invariants.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from fastkcna_amd import FK_PG_ALPHAPG, FK_PG_HNSW_LAYER, FK_PG_NSG, FK_PG_TAUMNG, IndexParams
+from fastkcna_amd.datasets import float_mixture, integer_mixture, unit_mixture
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def datasets():
+    return {
+        "int128": integer_mixture(4000, 128, 32, seed=11),
+        "float960": float_mixture(600, 960, 8, seed=12),
+        "unit96": unit_mixture(3000, 96, 16, seed=13, sigma=0.25),
+        "odd100": np.random.default_rng(14).normal(size=(1500, 100)).astype(np.float32),
+    }
+
+
+@pytest.mark.parametrize("name", ["int128", "float960", "unit96", "odd100"])
+def test_pair_distances_bitexact(device, name):
+    x = datasets()[name]
+    say("pair", name, x.shape)
+    device.upload(x)
+    rng = np.random.default_rng(1)
+    a = rng.integers(0, len(x), 20000)
+    b = rng.integers(0, len(x), 20000)
+    g = device.pair_distances(a, b)
+    c, c64 = oracle.pair_distances(x, a, b, with_f64=True)
+    assert np.array_equal(g.view(np.uint32), c.view(np.uint32))
+    nz = c64 > 0
+    assert np.max(np.abs(g[nz] - c64[nz]) / c64[nz]) < 1e-6
+    if name == "int128":
+        assert np.array_equal(g.astype(np.float64), c64)
+
+
+@pytest.mark.parametrize("name", ["int128", "unit96", "float960"])
+def test_join_distances_bitexact(device, name):
+    x = datasets()[name]
+    say("join", name)
+    device.upload(x)
+    rng = np.random.default_rng(2)
+    n_nodes = 300
+    sizes = rng.integers(0, 140, n_nodes)
+    sizes[:3] = [0, 1, 200]
+    nn = np.minimum(rng.integers(0, 60, n_nodes), sizes)
+    off = np.zeros(n_nodes + 1, np.uint64)
+    off[1:] = np.cumsum(sizes)
+    ids = rng.integers(0, len(x), int(off[-1])).astype(np.uint32)
+    g = device.join_distances(ids, off, nn)
+    a, b = [], []
+    for t in range(n_nodes):
+        lst = ids[off[t]:off[t + 1]]
+        T = len(lst)
+        for i in range(nn[t]):
+            for j in range(i + 1, T):
+                a.append(lst[i])
+                b.append(lst[j])
+    c = oracle.pair_distances(x, np.array(a), np.array(b))
+    assert np.array_equal(g.view(np.uint32), c.view(np.uint32))
+
+
+def make_pools(x, n_tasks, L, rng, q=None):
+    n = len(x)
+    q = rng.choice(n, n_tasks, replace=False).astype(np.uint32) if q is None else q
+    pools = np.zeros((n_tasks, L), np.uint64)
+    plen = np.zeros(n_tasks, np.uint32)
+    for t, qq in enumerate(q):
+        cand = rng.choice(n, L, replace=False)
+        d = oracle.pair_distances(x, np.full(L, qq), cand)
+        order = np.lexsort((cand, d))
+        pools[t] = oracle.pack(d[order], cand[order])
+        plen[t] = rng.integers(1, L + 1)
+    return pools, plen, q
+
+
+@pytest.mark.parametrize("pg", [FK_PG_NSG, FK_PG_TAUMNG, FK_PG_ALPHAPG])
+@pytest.mark.parametrize("reversed_", [False, True])
+@pytest.mark.parametrize("name", ["int128", "unit96", "float960"])
+def test_prune_bitexact(device, pg, reversed_, name):
+    x = datasets()[name]
+    say("prune", pg, reversed_, name)
+    device.upload(x)
+    rng = np.random.default_rng(3)
+    tau = {"int128": 300.0, "unit96": 0.05, "float960": 0.5}[name]
+    p = IndexParams(nsg_R=24, search_K=90, tau=tau, angle=63)
+    pools, plen, q = make_pools(x, 400, 120, rng)
+    # a self entry at the front of some pools
+    for t in range(0, 400, 7):
+        pools[t, 1:] = pools[t, :-1]
+        pools[t, 0] = oracle.pack(np.float32([0.0]), [q[t]])[0]
+    osz = len(x) - 50
+    g = device.prune(pg, pools, plen, q, osz, p, reversed_=reversed_)
+    c = oracle.prune(pg, x, pools, plen, q, osz, p, reversed_=reversed_)
+    assert np.array_equal(g[2], c[2])
+    for t in range(400):
+        k = c[2][t]
+        assert np.array_equal(g[0][t, :k], c[0][t, :k]), t
+        assert np.array_equal(g[1][t, :k].view(np.uint32), c[1][t, :k].view(np.uint32)), t
+    assert g[3] == c[3]
+
+
+def test_prune_alpha_degenerate_cases(device):
+    """Zero distances (duplicate rows) hit the 0/0 and x/0 paths of the
+    alpha rule's cosine; GPU and oracle must agree on every one."""
+    rng = np.random.default_rng(4)
+    base = rng.integers(0, 4, size=(60, 8)).astype(np.float32)
+    x = np.concatenate([base, base[:30]])  # duplicates
+    device.upload(x)
+    p = IndexParams(nsg_R=20, search_K=100, angle=60)
+    pools, plen, q = make_pools(x, 80, 60, rng)
+    g = device.prune(FK_PG_ALPHAPG, pools, plen, q, len(x), p)
+    c = oracle.prune(FK_PG_ALPHAPG, x, pools, plen, q, len(x), p)
+    assert np.array_equal(g[2], c[2])
+    for t in range(80):
+        assert np.array_equal(g[0][t, :c[2][t]], c[0][t, :c[2][t]])
+
+
+@pytest.mark.parametrize("name", ["int128", "unit96"])
+def test_knn_bruteforce_matches_oracle(device, name):
+    x = datasets()[name]
+    say("knn", name)
+    device.upload(x)
+    rng = np.random.default_rng(5)
+    sel = rng.choice(len(x), 37, replace=False).astype(np.uint32)
+    gi, gd = device.knn(x[sel], 100, self_ids=sel)
+    ci, cd = oracle.knn(x, x[sel], 100, self_ids=sel)
+    assert np.array_equal(gi, ci)
+    assert np.array_equal(gd.view(np.uint32), cd.view(np.uint32))
+
+
+def knn_recall(pool_ids, gt, K):
+    return float(np.mean([len(set(pool_ids[i, :K].tolist()) & set(gt[i, :K].tolist())) / K
+                          for i in range(len(gt))]))
+
+
+def test_nndescent_recall_not_worse_than_oracle(device):
+    x = datasets()["int128"]
+    say("nndescent int128")
+    device.upload(x)
+    p = IndexParams(K=30, L=40, S=10, R=50, iterations=6, controls=50)
+    gp, gl, gst = device.nndescent(p)
+    cp, cl, cst = oracle.nndescent(x, p, threads=8)
+    gt, _ = oracle.knn(x, x, 30, self_ids=np.arange(len(x)))
+    _, gids = oracle.unpack(gp)
+    _, cids = oracle.unpack(cp)
+    rg = knn_recall(gids, gt, 30)
+    rc = knn_recall(cids, gt, 30)
+    say("recall gpu", rg, "oracle", rc, "joins", gst["d_join"], cst["d_join"])
+    assert rg >= rc - 0.01
+    gd, _ = oracle.unpack(gp)
+    for i in range(len(x)):
+        row = gids[i, :gl[i]]
+        assert i not in row
+        assert len(set(row.tolist())) == len(row)
+        assert np.all(np.diff(gd[i, :gl[i]]) >= 0)
+        # pool distances are the exact distances of their ids
+    sel = np.arange(0, len(x), 97)
+    d_check = oracle.pair_distances(x, np.repeat(sel, 5), gids[sel, :5].ravel())
+    assert np.array_equal(d_check.view(np.uint32), gd[sel, :5].ravel().view(np.uint32))
+
+
+def reachable(off, ids, ep):
+    n = len(off) - 1
+    seen = np.zeros(n, bool)
+    seen[ep] = True
+    st = [ep]
+    while st:
+        u = st.pop()
+        for v in ids[off[u]:off[u + 1]]:
+            if not seen[v]:
+                seen[v] = True
+                st.append(int(v))
+    return seen
+
+
+@pytest.mark.parametrize("pg", [FK_PG_NSG, FK_PG_TAUMNG, FK_PG_ALPHAPG, FK_PG_HNSW_LAYER])
+def test_full_build_invariants_and_recall(device, pg):
+    x = datasets()["int128"]
+    say("build", pg)
+    device.upload(x)
+    p = IndexParams(K=60, L=60, S=10, R=60, iterations=5, nsg_R=24, search_L=40, search_K=60, tau=20.0, angle=60)
+    off, ids, ep, width, st = device.build(pg, p)
+    n = len(x)
+    for i in range(n):
+        nb = ids[off[i]:off[i + 1]]
+        assert i not in nb and len(set(nb.tolist())) == len(nb)
+    assert np.diff(off).max() <= width
+    if pg != FK_PG_HNSW_LAYER:
+        assert reachable(off, ids, ep).all()
+    # query recall@10 with the same evaluator for GPU and oracle graphs
+    rng = np.random.default_rng(6)
+    qs = (x[rng.choice(n, 200, replace=False)] + rng.normal(0, 3, size=(200, 128))).astype(np.float32)
+    gt, _ = oracle.knn(x, qs, 10)
+    res, _ = device.search(off, ids, ep, qs, 40, 10)
+    rg = np.mean([len(set(res[i]) & set(gt[i])) / 10 for i in range(200)])
+    coff, cids, cep, cw, cst, _ = oracle.build(pg, x, p, threads=8)
+    resc, _ = device.search(coff, cids, cep, qs, 40, 10)
+    rc = np.mean([len(set(resc[i]) & set(gt[i])) / 10 for i in range(200)])
+    say("query recall@10 gpu", rg, "oracle", rc)
+    assert rg >= rc - 0.02

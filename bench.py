@@ -1,0 +1,172 @@
+"""Benchmark: FastKCNA NSG build of 1,000,000 x 128 float32 points on one MI355X.
+
+One step = one full graph build (sampled ground truth, NN-descent K=L=500 S=12
+R=100 I=6, three NSG prune loops with reverse edges, entry point,
+connectivity and two BridgeView passes) over the resident dataset, ending
+with the graph in host memory.  The data is the integer-valued mixture of
+1,024 Gaussians (seed 1) with 10,000 held-out queries; query recall@10 at
+search_L = 80 is evaluated after the timed region.
+
+Prints one JSON line (metric, value = distance evaluations per second of the
+whole build, ms_per_step = build milliseconds, ...), with a `roofline` object
+for the dominant kernel (timed with HIP events on the library's stream) and a
+`cpu_baseline` object (the CPU oracle, the reference's algorithm restated in
+C++/OpenMP, timed on this host on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "build sec @ recall@10 parity (1M×128 NSG); dist-evals/s; chosen vs plain FP32"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def recall_at_10(res, gt):
+    return float(np.mean([len(set(res[i].tolist()) & set(gt[i].tolist())) / 10.0 for i in range(len(gt))]))
+
+
+def total_dists(st):
+    return sum(int(st[k]) for k in ("d_init", "d_ctrl", "d_join", "d_prune", "d_prune_rev", "d_inter", "d_ep",
+                                    "d_bv", "d_conn"))
+
+
+def roofline(st, families):
+    """Roofline of the dominant kernel family: algorithmic HBM bytes per
+    launch over its mean launch time (HIP events on the library's stream)."""
+    ms = st["prof_ms"]
+    i = int(np.argmax(ms[:len(families)]))
+    if ms[i] <= 0:
+        return None
+    launches = max(int(st["prof_launches"][i]), 1)
+    gbs = st["prof_bytes"][i] / (ms[i] / 1000.0) / 1e9
+    return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+            "traffic": None, "kernel": families[i], "launches": launches,
+            "bytes_per_launch": st["prof_bytes"][i] / launches, "ms_per_launch": ms[i] / launches,
+            "share_of_build": ms[i] / 1000.0 / max(st["t_total"], 1e-9)}
+
+
+def cpu_baseline(n_sample, threads):
+    """The CPU oracle on a bounded sample of the same workload."""
+    import oracle
+    from fastkcna_amd import FK_PG_NSG
+    from fastkcna_amd.datasets import integer_mixture, params_c2
+    from fastkcna_amd.kgraph import IndexParams
+    x = integer_mixture(n_sample, 128, 1024, seed=1)
+    p = params_c2(IndexParams)
+    t0 = time.perf_counter()
+    off, ids, ep, width, st, _ = oracle.build(FK_PG_NSG, x, p, threads=threads)
+    dt = time.perf_counter() - t0
+    nd = total_dists(st)
+    return {"value": nd / dt, "unit": "dist-evals/s", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle (C++/OpenMP restatement of the reference) NSG build of the first {n_sample:,} "
+                      f"points of the same mixture, same parameters; {dt:.1f} s, {nd:,} distance evaluations",
+            "build_sec": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--queries", type=int, default=10_000)
+    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--verbose", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the build does not shard "
+              "across GPUs in this version)", file=sys.stderr)
+        sys.exit(2)
+
+    from fastkcna_amd import FK_PG_NSG, Device, _ffi
+    from fastkcna_amd.datasets import integer_mixture, params_c2
+    from fastkcna_amd.kgraph import IndexParams
+
+    t = time.perf_counter()
+    x = integer_mixture(args.n + args.queries, 128, 1024, seed=1)
+    base, queries = x[:args.n], x[args.n:]
+    log(f"data {base.shape} + {queries.shape} in {time.perf_counter() - t:.1f}s")
+    p = params_c2(IndexParams)
+    p.flags = _ffi.FK_FLAG_PROFILE | (_ffi.FK_FLAG_VERBOSE if args.verbose else 0)
+
+    dev = Device(0)
+    t = time.perf_counter()
+    dev.upload(base)
+    log(f"upload {time.perf_counter() - t:.2f}s")
+
+    for w in range(args.warmup):
+        t = time.perf_counter()
+        dev.build(FK_PG_NSG, p)
+        log(f"warmup {w} {time.perf_counter() - t:.2f}s")
+
+    times, stats, graph = [], [], None
+    for s in range(args.steps):
+        t = time.perf_counter()
+        off, ids, ep, width, st = dev.build(FK_PG_NSG, p)
+        dt = time.perf_counter() - t
+        times.append(dt)
+        stats.append(st)
+        graph = (off, ids, ep)
+        log(f"step {s}: build {dt:.3f}s, dist-evals {total_dists(st):,}, aod {st['aod']}")
+
+    off, ids, ep = graph
+    t = time.perf_counter()
+    gt, _ = dev.knn(queries, 10)
+    res, _ = dev.search(off, ids, ep, queries, 80, 10)
+    rec = recall_at_10(res, gt)
+    log(f"recall@10 (search_L 80) {rec:.4f}; evaluation {time.perf_counter() - t:.1f}s")
+
+    st = stats[-1]
+    ms = 1000.0 * float(np.mean(times))
+    nd = total_dists(st)
+    value = nd / (ms / 1000.0)
+    roof = roofline(st, _ffi.PROF_FAMILIES)
+    log("kernel families (ms, GB/s): " + ", ".join(
+        f"{f}={st['prof_ms'][i]:.1f}/{st['prof_bytes'][i] / max(st['prof_ms'][i], 1e-9) / 1e6:.0f}"
+        for i, f in enumerate(_ffi.PROF_FAMILIES) if st['prof_launches'][i]))
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(args.cpu_sample, threads)
+        log(f"cpu baseline {cpu['value']:.3e} dist-evals/s on {threads} threads ({cpu['build_sec']:.1f}s)")
+        cpu.pop("build_sec")
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "dist-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"C2: NSG build, {args.n:,}x128 float32 integer-valued mixture of 1,024 "
+                               "Gaussians (seed 1), K=L=500 S=12 R=100 I=6 nsg_R=50 search_L=80 search_K=500",
+                   "build_sec": ms / 1000.0, "recall_at_10": rec, "search_L": 80, "queries": args.queries,
+                   "dist_evals": nd, "aod": int(st["aod"]), "nnd_iterations": int(st["nnd_iterations"]),
+                   "phase_sec": {k: round(float(st[k]), 4) for k in ("t_ctrl", "t_nnd", "t_prune", "t_inter",
+                                                                    "t_ep", "t_conn", "t_bv")}},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    dev.close()
+
+
+if __name__ == "__main__":
+    main()

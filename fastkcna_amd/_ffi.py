@@ -16,7 +16,11 @@ FK_OK, FK_E_INVALID, FK_E_NOMEM, FK_E_DEVICE, FK_E_IO, FK_E_INTERNAL, FK_E_NODEV
 FK_PG_NSG, FK_PG_HNSW_LAYER, FK_PG_TAUMNG, FK_PG_ALPHAPG = 1, 2, 3, 4
 FK_FLAG_VERBOSE = 0x1
 FK_FLAG_NO_CONTROLS = 0x2
+FK_FLAG_PROFILE = 0x4
 FK_MAX_LOOPS = 16
+FK_PROF_N = 16
+PROF_FAMILIES = ["controls", "init", "join", "join_fill", "merge", "update", "prune", "prune_rev", "inter", "bv",
+                 "ep", "conn"]
 
 STATUS_NAMES = {
     FK_OK: "FK_OK",
@@ -58,6 +62,7 @@ class FkStats(ctypes.Structure):
         ("prune_cc", f32 * FK_MAX_LOOPS), ("search_cc", f32 * FK_MAX_LOOPS),
         ("aod", u32), ("max_degree", u32), ("min_degree", u32), ("n_attached", u32),
         ("bv_rounds", u32 * FK_MAX_LOOPS),
+        ("prof_ms", f64 * FK_PROF_N), ("prof_bytes", f64 * FK_PROF_N), ("prof_launches", u32 * FK_PROF_N),
     ]
 
     def to_dict(self):

@@ -43,6 +43,7 @@ struct BuildState {
     BuildParams P{};
     fk_stats *st = nullptr;
     bool verbose = false;
+    double t0 = 0;         // build start (seconds, steady clock)
 
     DevBuf<uint64_t> pool;
     DevBuf<uint8_t> pflg;
@@ -88,6 +89,22 @@ struct BuildState {
     // scratch
     DevBuf<uint64_t> scal;      // 16 u64
     DevBuf<uint64_t> scan_tmp;  // prefix-scan partials
+
+    // kernel-family timing (FK_FLAG_PROFILE)
+    bool prof = false;
+    struct ProfRec {
+        hipEvent_t a, b;
+        int fam;
+        double bytes;
+    };
+    std::vector<hipEvent_t> ev_free;
+    std::vector<ProfRec> ev_pending;
+    hipEvent_t ev_get();
+    // Returns a start event recorded on the build stream (nullptr when off).
+    hipEvent_t prof_begin();
+    void prof_end(hipEvent_t start, int fam, double bytes);
+    void prof_flush();
+    ~BuildState();
 
     uint64_t next_rand() {
         host_rng += 0x9E3779B97F4A7C15ull;

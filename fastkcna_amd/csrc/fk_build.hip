@@ -218,10 +218,17 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
     a.out_stride = b.ncap;
     a.out_len = b.nsg_len.p;
     a.cc = b.acc.p;
+    a.examined = b.acc.p + 1;
+    hipEvent_t pe = b.prof_begin();
     launch_prune(b.s, *b.ds, a);
-    unsigned long long cc = 0;
-    FK_HIP(hipMemcpyAsync(&cc, b.acc.p, 8, hipMemcpyDeviceToHost, b.s));
+    unsigned long long cc2[2] = {0, 0};
+    FK_HIP(hipMemcpyAsync(cc2, b.acc.p, 16, hipMemcpyDeviceToHost, b.s));
     b.sync();
+    const unsigned long long cc = cc2[0];
+    // every examined candidate's row and key read once, kept ids and
+    // distances written
+    b.prof_end(pe, FK_PROF_PRUNE, (double)cc2[1] * (4.0 * b.ds->d + 8.0) + 8.0 * (double)N * P.nsg_R);
+    b.prof_flush();
     b.width = P.nsg_R;
     b.st->d_prune += cc;
     const double t1 = now_sec();
@@ -232,6 +239,7 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
     b.st->t_prune += t1 - t0;
     // 2. reverse edges
     FK_HIP(hipMemsetAsync(b.bcnt.p, 0, (size_t)N * 4, b.s));
+    pe = b.prof_begin();
     hipLaunchKernelGGL(k_indeg, dim3(gw), dim3(256), 0, b.s, b.nsg_id.p, b.nsg_len.p, N, b.ncap, b.bcnt.p);
     FK_LAUNCH_CHECK();
     exclusive_scan_u32_to_u64(b.s, b.bcnt.p, b.boff.p, N, b.boff.p + N, b.scan_tmp);
@@ -253,6 +261,10 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
     uint32_t hc[2] = {0, 0};
     FK_HIP(hipMemcpyAsync(hc, counters.p, 8, hipMemcpyDeviceToHost, b.s));
     b.sync();
+    // out-lists read twice (count, fill) and rewritten, in-edges written and
+    // read once as 8-byte keys
+    b.prof_end(pe, FK_PROF_INTER, 3.0 * 8.0 * (double)n_in + 16.0 * (double)n_in);
+    b.prof_flush();
     if (hc[1]) throw Error(FK_E_INTERNAL, "in-degree above the reverse-edge merge capacity (" +
                                               std::to_string(hc[1]) + " > 4096)");
     // 3. reversed prune of over-full lists (no search_K cap, no size guard)
@@ -266,10 +278,16 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
         r.n_tasks = hc[0];
         r.oracle_n = 0xFFFFFFFFu;
         r.maxc = 0xFFFFFFFFu;
+        r.examined = b.acc.p + 1;
+        FK_HIP(hipMemsetAsync(b.acc.p + 1, 0, 8, b.s));
+        hipEvent_t pr = b.prof_begin();
         launch_prune(b.s, *b.ds, r);
-        unsigned long long rc = 0;
-        FK_HIP(hipMemcpyAsync(&rc, b.acc.p, 8, hipMemcpyDeviceToHost, b.s));
+        unsigned long long rcc[2] = {0, 0};
+        FK_HIP(hipMemcpyAsync(rcc, b.acc.p, 16, hipMemcpyDeviceToHost, b.s));
         b.sync();
+        const unsigned long long rc = rcc[0];
+        b.prof_end(pr, FK_PROF_PRUNE_REV, (double)rcc[1] * (4.0 * b.ds->d + 8.0) + 8.0 * (double)hc[0] * P.nsg_R);
+        b.prof_flush();
         b.st->d_prune_rev += rc;
     }
     const double t2 = now_sec();
@@ -326,7 +344,10 @@ void build_loops(BuildState &b, fk_pg_type type) {
         if (type != FK_PG_HNSW_LAYER) tree_grow(b);
         update_nhoods(b);
         b.sync();
-        if (li < FK_MAX_LOOPS) b.st->buildnsg_time[li] = now_sec() - tb;
+        if (li < FK_MAX_LOOPS) {
+            b.st->buildnsg_time[li] = now_sec() - tb;
+            b.st->con_time[li] = now_sec() - b.t0;
+        }
         b.loopcount++;
         b.st->n_loops = iter;
         if (iter == iters) {
@@ -338,6 +359,7 @@ void build_loops(BuildState &b, fk_pg_type type) {
         if (li < FK_MAX_LOOPS) b.st->search_time[li] = now_sec() - ts;
         const float rec = nnd_evaluate(b, b.P.MAXL);
         if (b.st->n_recall < FK_MAX_LOOPS + 64) b.st->recall[b.st->n_recall++] = rec;
+        if (b.st->n_kgraph < FK_MAX_LOOPS + 64) b.st->kgraph_time[b.st->n_kgraph++] = now_sec() - b.t0;
         if (b.verbose) fprintf(stderr, "[fk] recall after BridgeView: %g\n", rec);
     }
 }
@@ -388,6 +410,8 @@ fk_status fk_build(fk_handle *h, fk_pg_type type, uint32_t n_used, const fk_para
         b.st = &out->stats;
         b.verbose = (p->flags & FK_FLAG_VERBOSE) != 0;
         b.host_rng = ((uint64_t)b.P.seed << 32) ^ 0x5DEECE66Dull;
+        b.prof = (p->flags & FK_FLAG_PROFILE) != 0;
+        b.t0 = t0;
         nnd_controls(b);
         nnd_run(b);
         // buildFast*: evaluate() at MAXL, then the constructor's time
@@ -440,6 +464,8 @@ fk_status fk_nndescent(fk_handle *h, uint32_t n_used, const fk_params *p, uint64
         memset(b.st, 0, sizeof(*b.st));
         b.verbose = (p->flags & FK_FLAG_VERBOSE) != 0;
         b.host_rng = ((uint64_t)b.P.seed << 32) ^ 0x5DEECE66Dull;
+        b.prof = (p->flags & FK_FLAG_PROFILE) != 0;
+        b.t0 = t0;
         nnd_controls(b);
         nnd_run(b);
         FK_HIP(hipMemcpyAsync(pool_len, b.cnt.p, (size_t)b.N * 4, hipMemcpyDeviceToHost, b.s));

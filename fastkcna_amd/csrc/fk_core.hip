@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ rows, u
     }
     const uint32_t limit = len < a.maxc ? len : a.maxc;
     uint32_t pos = start + 1;
-    unsigned long long cc = 0;
+    unsigned long long cc = 0, exam = 0;
     bool stop = nout >= a.R;
     while (!stop && pos < limit) {
         const uint32_t nb = (limit - pos) < 64u ? (limit - pos) : 64u;
@@ -171,7 +171,10 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ rows, u
                 }
             }
         }
-        if (lane <= stop_lane) cc += mycc;
+        if (lane <= stop_lane) {
+            cc += mycc;
+            exam += skip ? 0u : 1u;
+        }
         pos += nb;
     }
     __builtin_amdgcn_wave_barrier();
@@ -183,6 +186,10 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ rows, u
     if (a.cc) {
         cc = wave_sum(cc);
         if (lane == 0 && cc) atomicAdd(a.cc, cc);
+    }
+    if (a.examined) {
+        exam = wave_sum(exam);
+        if (lane == 0 && exam) atomicAdd(a.examined, exam);
     }
 }
 

@@ -112,6 +112,7 @@ struct PruneLaunch {
     uint64_t out_stride = 0;
     uint32_t *out_len = nullptr;
     unsigned long long *cc = nullptr;  // accumulated distance count
+    unsigned long long *examined = nullptr;  // accumulated examined candidates
 };
 void launch_prune(hipStream_t s, const Dataset &ds, const PruneLaunch &a);
 

@@ -32,6 +32,56 @@ double now_sec() {
 
 void BuildState::sync() { FK_HIP(hipStreamSynchronize(s)); }
 
+hipEvent_t BuildState::ev_get() {
+    if (!ev_free.empty()) {
+        hipEvent_t e = ev_free.back();
+        ev_free.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    FK_HIP(hipEventCreate(&e));
+    return e;
+}
+
+hipEvent_t BuildState::prof_begin() {
+    if (!prof) return nullptr;
+    hipEvent_t e = ev_get();
+    FK_HIP(hipEventRecord(e, s));
+    return e;
+}
+
+void BuildState::prof_end(hipEvent_t start, int fam, double bytes) {
+    if (!prof || !start) return;
+    hipEvent_t e = ev_get();
+    FK_HIP(hipEventRecord(e, s));
+    ev_pending.push_back({start, e, fam, bytes});
+}
+
+void BuildState::prof_flush() {
+    for (auto &r : ev_pending) {
+        FK_HIP(hipEventSynchronize(r.b));
+        float ms = 0.f;
+        FK_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        if (st && r.fam >= 0 && r.fam < FK_PROF_N) {
+            st->prof_ms[r.fam] += ms;
+            st->prof_bytes[r.fam] += r.bytes;
+            st->prof_launches[r.fam] += 1;
+        }
+        ev_free.push_back(r.a);
+        ev_free.push_back(r.b);
+    }
+    ev_pending.clear();
+}
+
+BuildState::~BuildState() {
+    if (s) (void)hipStreamSynchronize(s);
+    for (auto &r : ev_pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto e : ev_free) (void)hipEventDestroy(e);
+}
+
 // ---------------------------------------------------------------------------
 // Block-wide exclusive scan of n u32 values in LDS (in place); returns total.
 // All threads must call.  s_part needs blockDim.x entries.
@@ -586,6 +636,13 @@ __global__ __launch_bounds__(256) void k_rev_select(uint32_t N, uint32_t R, uint
     }
 }
 
+__global__ void k_sum_u32(const uint32_t *__restrict__ v, uint32_t n, unsigned long long *__restrict__ acc) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long x = i < n ? v[i] : 0u;
+    x = wave_sum(x);
+    if (lane_id() == 0 && x) atomicAdd(acc, x);
+}
+
 // ---------------------------------------------------------------------------
 // Host phases.
 // ---------------------------------------------------------------------------
@@ -641,7 +698,10 @@ void nnd_controls(BuildState &b) {
     const uint32_t K = b.P.MAXL < b.N - 1 ? b.P.MAXL : b.N - 1;
     if (K < b.P.MAXL) FK_HIP(hipMemsetAsync(b.ctrl_gt.p, 0xFF, b.ctrl_gt.bytes(), b.s));
     DevBuf<uint32_t> gt((size_t)b.n_ctrl * K);
+    hipEvent_t pe = b.prof_begin();
     bruteforce_knn(b.s, *b.ds, q.p, b.ds->dpad, b.n_ctrl, b.N, K, b.ctrl_ids.p, gt.p, nullptr);
+    // algorithmic bytes: every row read once, the K ids written per control
+    b.prof_end(pe, FK_PROF_CONTROLS, (double)b.N * 4.0 * b.ds->d + (double)b.n_ctrl * K * 4.0);
     FK_HIP(hipMemcpy2DAsync(b.ctrl_gt.p, (size_t)b.P.MAXL * 4, gt.p, (size_t)K * 4, (size_t)K * 4, b.n_ctrl,
                             hipMemcpyDeviceToDevice, b.s));
     b.sync();
@@ -687,18 +747,39 @@ static void nnd_join(BuildState &b) {
     b.sync();
     b.resv.ensure(nlist);
     const size_t lds = FK_JOIN_LDS_ROWS_BYTES;
+    const double d4 = 4.0 * b.ds->d;
+    hipEvent_t pe = b.prof_begin();
     hipLaunchKernelGGL(k_join_count, dim3(N), dim3(256), lds, b.s, b.ds->rows.p, b.ds->stride, b.ds->dpad,
                        b.lists.p, b.list_off.p, b.n_new.p, b.pair_off.p, b.thr.p, b.pdist.p, b.resv.p, b.bcnt.p);
     FK_LAUNCH_CHECK();
+    // sampled rows and ids gathered once per join, one float per pair written
+    b.prof_end(pe, FK_PROF_JOIN, (double)nlist * (d4 + 4.0) + 4.0 * (double)pairs);
     const uint64_t cands = scan_to_host(b, b.bcnt.p, b.boff.p, N);
     b.bkey.ensure(cands);
     b.bsrc.ensure(cands);
+    double pool_entries = 0;
+    if (b.prof) {
+        FK_HIP(hipMemsetAsync(b.acc.p + 6, 0, 8, b.s));
+        hipLaunchKernelGGL(k_sum_u32, dim3((N + 255) / 256), dim3(256), 0, b.s, b.cnt.p, N, b.acc.p + 6);
+        unsigned long long pe_cnt = 0;
+        FK_HIP(hipMemcpyAsync(&pe_cnt, b.acc.p + 6, 8, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+        pool_entries = (double)pe_cnt;
+    }
+    pe = b.prof_begin();
     hipLaunchKernelGGL(k_join_fill, dim3(N), dim3(256), 0, b.s, b.lists.p, b.list_off.p, b.n_new.p, b.pair_off.p,
                        b.thr.p, b.pdist.p, b.resv.p, b.boff.p, b.bkey.p, b.bsrc.p);
     FK_LAUNCH_CHECK();
+    // pair distances and list ids read, (key, source) bucket entries written
+    b.prof_end(pe, FK_PROF_JOIN_FILL, 4.0 * (double)pairs + 4.0 * (double)nlist + 12.0 * (double)cands);
+    pe = b.prof_begin();
     hipLaunchKernelGGL(k_nnd_merge, dim3(N), dim3(256), 0, b.s, b.cap, b.P.L, b.P.K, b.pool.p, b.pflg.p, b.cnt.p,
                        b.thr.p, b.bcnt.p, b.boff.p, b.bkey.p, b.bsrc.p, b.found.p);
     FK_LAUNCH_CHECK();
+    // bucket entries read once; each pool read and rewritten (8-byte key +
+    // 1-byte flags per entry), before and after sizes bounded by L
+    b.prof_end(pe, FK_PROF_MERGE, 12.0 * (double)cands + 18.0 * pool_entries);
+    b.prof_flush();
     if (b.verbose) fprintf(stderr, "[fk] join: pairs %llu candidates %llu\n", (unsigned long long)pairs,
                            (unsigned long long)cands);
 }
@@ -707,6 +788,7 @@ static void nnd_update(BuildState &b, uint32_t it) {
     const uint32_t N = b.N;
     const unsigned g256 = (N + 255) / 256;
     const unsigned gw = (N + 3) / 4;  // 4 waves per 256-thread block
+    hipEvent_t pe = b.prof_begin();
     hipLaunchKernelGGL(k_upd_m, dim3(g256), dim3(256), 0, b.s, b.pool.p, b.pflg.p, b.cnt.p, b.found.p, N, b.cap,
                        b.P.S, b.M.p, b.radiusM.p);
     FK_LAUNCH_CHECK();
@@ -735,7 +817,11 @@ static void nnd_update(BuildState &b, uint32_t it) {
                        b.rcnt_old.p, b.roff_new.p, b.roff_old.p, b.rbuf_new.p, b.rbuf_old.p, b.list_off.p,
                        b.n_new.p, b.fcnt_new.p, b.fcnt_old.p, b.lists.p);
     FK_LAUNCH_CHECK();
+    // the first M entries of every pool read twice (keys + flags), lists and
+    // reverse buckets written once
+    b.prof_end(pe, FK_PROF_UPDATE, 18.0 * (double)nlist + 8.0 * (double)(tn + to) + 4.0 * (double)nlist);
     b.sync();
+    b.prof_flush();
 }
 
 void nnd_run(BuildState &b) {
@@ -756,9 +842,11 @@ void nnd_run(BuildState &b) {
         FK_HIP(hipMemcpyAsync(b.n_new.p, nn.data(), (size_t)N * 4, hipMemcpyHostToDevice, b.s));
         b.sync();
     }
+    hipEvent_t pe0 = b.prof_begin();
     hipLaunchKernelGGL(k_nnd_init, dim3(N), dim3(64), 0, b.s, b.ds->rows.p, b.ds->stride, b.ds->dpad, N, P.S, b.cap,
                        P.L, (uint64_t)P.seed, b.pool.p, b.pflg.p, b.cnt.p, b.M.p, b.thr.p, b.lists.p);
     FK_LAUNCH_CHECK();
+    b.prof_end(pe0, FK_PROF_INIT, (double)N * ((P.S + 1) * (4.0 * b.ds->d) + P.S * 9.0 + 2 * P.S * 4.0));
     b.st->d_init += (uint64_t)N * P.S;
     const double tk0 = now_sec();
     for (uint32_t it = 0; it < P.iterations; ++it) {
@@ -775,7 +863,7 @@ void nnd_run(BuildState &b) {
         const float recall = nnd_evaluate(b, P.K);
         b.st->nnd_delta = delta;
         b.st->nnd_recall = recall;
-        if (b.st->n_kgraph < FK_MAX_LOOPS + 64) b.st->kgraph_time[b.st->n_kgraph++] = now_sec() - tk0;
+        if (b.st->n_kgraph < FK_MAX_LOOPS + 64) b.st->kgraph_time[b.st->n_kgraph++] = now_sec() - b.t0;
         if (b.st->n_recall < FK_MAX_LOOPS + 64) b.st->recall[b.st->n_recall++] = recall;
         if (b.verbose)
             fprintf(stderr, "[fk] iteration: %u recall: %g delta: %g time: %g\n", it + 1, recall, delta,
@@ -957,7 +1045,17 @@ void bridgeview(BuildState &b) {
         const float delta = (float)cnts[0] / (float)N / (float)S;
         const bool last = delta < 0.5f;
         const uint32_t scap = last ? 0xFFFFFFFFu : S;
+        double pool_entries = 0;
+        if (b.prof) {
+            FK_HIP(hipMemsetAsync(b.acc.p + 6, 0, 8, b.s));
+            hipLaunchKernelGGL(k_sum_u32, dim3((N + 255) / 256), dim3(256), 0, b.s, b.cnt.p, N, b.acc.p + 6);
+            unsigned long long pc = 0;
+            FK_HIP(hipMemcpyAsync(&pc, b.acc.p + 6, 8, hipMemcpyDeviceToHost, b.s));
+            b.sync();
+            pool_entries = (double)pc;
+        }
         FK_HIP(hipMemsetAsync(b.acc.p, 0, 8, b.s));
+        hipEvent_t pe = b.prof_begin();
         hipLaunchKernelGGL(k_bv_round, dim3(N), dim3(256), lds_row, b.s, b.ds->rows.p, b.ds->stride, b.ds->dpad, N,
                            b.cap, W, scap, b.pool.p, b.pflg.p, b.cnt.p, b.nsg_id.p, b.nsg_len.p, b.nsg_new.p, b.ncap,
                            b.acc.p);
@@ -965,6 +1063,10 @@ void bridgeview(BuildState &b) {
         unsigned long long cc = 0;
         FK_HIP(hipMemcpyAsync(&cc, b.acc.p, 8, hipMemcpyDeviceToHost, b.s));
         b.sync();
+        // one candidate row (+ its id) per distance; each pool read and
+        // rewritten once (8-byte key + 1-byte flags)
+        b.prof_end(pe, FK_PROF_BV, (double)cc * (4.0 * b.ds->d + 4.0) + 18.0 * pool_entries);
+        b.prof_flush();
         cc_total += cc;
         ++rounds;
         if (b.verbose)

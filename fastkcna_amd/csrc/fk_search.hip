@@ -218,6 +218,7 @@ void compute_ep(BuildState &b) {
     const Dataset &ds = *b.ds;
     DevBuf<float> center(ds.dpad);
     FK_HIP(hipMemsetAsync(center.p, 0, center.bytes(), b.s));
+    hipEvent_t pe = b.prof_begin();
     {
         const uint32_t nb = (ds.n_total + FK_CEN_ROWS - 1) / FK_CEN_ROWS;
         DevBuf<double> part((size_t)nb * ds.d);
@@ -258,6 +259,9 @@ void compute_ep(BuildState &b) {
     FK_HIP(hipMemcpyAsync(&b.ep, oid.p, 4, hipMemcpyDeviceToHost, b.s));
     FK_HIP(hipMemcpyAsync(&nd, b.acc.p + 4, 8, hipMemcpyDeviceToHost, b.s));
     b.sync();
+    // the centroid reads every resident row once; the search one row per distance
+    b.prof_end(pe, FK_PROF_EP, (double)ds.n_total * 4.0 * ds.d + (double)nd * 4.0 * ds.d);
+    b.prof_flush();
     b.st->d_ep += nd;
     b.st->t_ep += now_sec() - t0;
     if (b.verbose) fprintf(stderr, "[fk] ep: %u\n", b.ep);
@@ -310,12 +314,15 @@ struct Bfs {
         uint32_t *cur = f0.p, *nxt = f1.p;
         while (nf) {
             FK_HIP(hipMemsetAsync(cnt.p, 0, 4, b.s));
+            hipEvent_t pe = b.prof_begin();
             hipLaunchKernelGGL(k_bfs_expand, dim3((nf + 3) / 4), dim3(256), 0, b.s, b.nsg_id.p, b.nsg_len.p, b.ncap,
                                cur, nf, vis.p, nxt, cnt.p);
             FK_LAUNCH_CHECK();
             uint32_t nn = 0;
             FK_HIP(hipMemcpyAsync(&nn, cnt.p, 4, hipMemcpyDeviceToHost, b.s));
             b.sync();
+            // frontier out-lists read (ids + length), reached flags written
+            b.prof_end(pe, FK_PROF_CONN, (double)nf * (4.0 + 4.0 * b.P.nsg_R) + 8.0 * nn);
             if (nn) {
                 FK_HIP(hipMemcpyAsync(hbuf.data(), nxt, (size_t)nn * 4, hipMemcpyDeviceToHost, b.s));
                 b.sync();
@@ -331,6 +338,12 @@ struct Bfs {
 
 void tree_grow(BuildState &b) {
     const double t0 = now_sec();
+    struct Flush {
+        BuildState &b;
+        ~Flush() {
+            try { b.prof_flush(); } catch (...) {}
+        }
+    } flush{b};
     const uint32_t N = b.N;
     Bfs bfs(b);
     uint32_t reached = bfs.run(b.ep);

@@ -61,6 +61,24 @@ typedef enum {
 /* Flags (fk_params.flags). */
 #define FK_FLAG_VERBOSE 0x1u     /* per-iteration log lines on stderr            */
 #define FK_FLAG_NO_CONTROLS 0x2u /* skip the sampled ground truth (recall = 0)   */
+#define FK_FLAG_PROFILE 0x4u     /* time kernel families with HIP events         */
+
+/* Kernel families timed under FK_FLAG_PROFILE (fk_stats.prof_*). */
+enum {
+    FK_PROF_CONTROLS = 0, /* brute-force keys + top-k select (GenerateControl) */
+    FK_PROF_INIT = 1,     /* random init                                        */
+    FK_PROF_JOIN = 2,     /* local-join distance block + bucket reservation     */
+    FK_PROF_JOIN_FILL = 3,/* bucket fill                                        */
+    FK_PROF_MERGE = 4,    /* per-node bucket merge into the pools               */
+    FK_PROF_UPDATE = 5,   /* resampling (new / old / reverse lists)             */
+    FK_PROF_PRUNE = 6,    /* occlusion prune of the pools                       */
+    FK_PROF_PRUNE_REV = 7,/* reversed prune of over-full out-lists              */
+    FK_PROF_INTER = 8,    /* reverse-edge gather and merge                      */
+    FK_PROF_BV = 9,       /* BridgeView rounds                                  */
+    FK_PROF_EP = 10,      /* entry-point beam search + centroid                 */
+    FK_PROF_CONN = 11,    /* BFS connectivity                                   */
+    FK_PROF_N = 16
+};
 
 /* KGraph::IndexParams fields used on the build path (src/kgraph.h:116-170),
  * one for one, plus flags. */
@@ -95,6 +113,11 @@ typedef struct {
     uint32_t max_degree, min_degree;
     uint32_t n_attached;          /* edges added by the connectivity pass     */
     uint32_t bv_rounds[FK_MAX_LOOPS];
+    /* FK_FLAG_PROFILE: per kernel family, summed kernel milliseconds (HIP
+     * events on the build's stream), algorithmic HBM bytes and launches. */
+    double prof_ms[FK_PROF_N];
+    double prof_bytes[FK_PROF_N];
+    uint32_t prof_launches[FK_PROF_N];
 } fk_stats;
 
 /* Output graph in CSR form, allocated by the library (fk_graph_free). */

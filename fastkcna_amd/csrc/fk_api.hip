@@ -3,6 +3,7 @@
 // NN-descent live in fk_build.hip, the query search in fk_search.hip.
 #include <math.h>
 
+#include <algorithm>
 #include <new>
 
 #include "fk_api_impl.h"
@@ -28,6 +29,43 @@ int rule_of(fk_pg_type t) {
 }
 
 void select_device(fk_handle *h) { FK_HIP(hipSetDevice(h->device)); }
+
+// Host rows (any stride) -> device rows of dpad floats, zero padded, through
+// two pinned staging buffers so packing and PCIe transfer overlap.
+void upload_rows(fk_handle *h, float *dst, uint32_t dpad, const float *src, size_t src_stride_bytes, uint32_t n,
+                 uint32_t d) {
+    const size_t row_bytes = (size_t)dpad * sizeof(float);
+    const size_t chunk_rows = std::max<size_t>(1, ((size_t)32 << 20) / row_bytes);
+    float *stage[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    struct Cleanup {
+        float **s;
+        hipEvent_t *e;
+        ~Cleanup() {
+            for (int i = 0; i < 2; ++i) {
+                if (e[i]) { (void)hipEventSynchronize(e[i]); (void)hipEventDestroy(e[i]); }
+                if (s[i]) (void)hipHostFree(s[i]);
+            }
+        }
+    } cleanup{stage, done};
+    for (int i = 0; i < 2; ++i) {
+        FK_HIP(hipHostMalloc((void **)&stage[i], chunk_rows * row_bytes, hipHostMallocDefault));
+        FK_HIP(hipEventCreate(&done[i]));
+    }
+    int k = 0;
+    for (size_t r0 = 0; r0 < n; r0 += chunk_rows, k ^= 1) {
+        const size_t m = std::min<size_t>(chunk_rows, n - r0);
+        FK_HIP(hipEventSynchronize(done[k]));  // the buffer's previous copy has finished
+        for (size_t r = 0; r < m; ++r) {
+            float *o = stage[k] + r * dpad;
+            memcpy(o, (const char *)src + (r0 + r) * src_stride_bytes, (size_t)d * sizeof(float));
+            if (dpad != d) memset(o + d, 0, (size_t)(dpad - d) * sizeof(float));
+        }
+        FK_HIP(hipMemcpyAsync(dst + r0 * dpad, stage[k], m * row_bytes, hipMemcpyHostToDevice, h->stream));
+        FK_HIP(hipEventRecord(done[k], h->stream));
+    }
+    FK_HIP(hipStreamSynchronize(h->stream));
+}
 
 void require_dataset(fk_handle *h) {
     if (h->ds.n_total == 0) throw Error(FK_E_INVALID, "no dataset uploaded (fk_dataset_upload)");
@@ -89,10 +127,7 @@ fk_status fk_dataset_upload(fk_handle *h, const float *rows, size_t row_stride_b
         ds.n_total = 0;
         const uint32_t dpad = (d + 7u) & ~7u;
         ds.rows.alloc((size_t)n_total * dpad);
-        if (dpad != d) FK_HIP(hipMemsetAsync(ds.rows.p, 0, ds.rows.bytes(), h->stream));
-        FK_HIP(hipMemcpy2DAsync(ds.rows.p, (size_t)dpad * sizeof(float), rows, row_stride_bytes,
-                                (size_t)d * sizeof(float), n_total, hipMemcpyHostToDevice, h->stream));
-        FK_HIP(hipStreamSynchronize(h->stream));
+        upload_rows(h, ds.rows.p, dpad, rows, row_stride_bytes, n_total, d);
         ds.n_total = n_total;
         ds.d = d;
         ds.dpad = dpad;
@@ -234,9 +269,7 @@ fk_status fk_knn_bruteforce(fk_handle *h, const float *queries, uint32_t n_q, ui
         select_device(h);
         const Dataset &ds = h->ds;
         DevBuf<float> dq((size_t)n_q * ds.dpad);
-        if (ds.dpad != ds.d) FK_HIP(hipMemsetAsync(dq.p, 0, dq.bytes(), h->stream));
-        FK_HIP(hipMemcpy2DAsync(dq.p, (size_t)ds.dpad * 4, queries, (size_t)ds.d * 4, (size_t)ds.d * 4, n_q,
-                                hipMemcpyHostToDevice, h->stream));
+        upload_rows(h, dq.p, ds.dpad, queries, (size_t)ds.d * 4, n_q, ds.d);
         DevBuf<uint32_t> dself, dout((size_t)n_q * K);
         DevBuf<float> dd((size_t)n_q * K);
         if (self_ids) {

@@ -177,14 +177,29 @@ __global__ void k_update_nhoods(const uint32_t *__restrict__ nsg_id, const uint3
     if (lane_id() == 0) old_len[n] = deg;
 }
 
-// CalculateNSG (src/kgraph.cpp:2273-2293): degree statistics.
-__global__ void k_degree_stats(const uint32_t *__restrict__ nsg_len, uint32_t N, unsigned long long *__restrict__ acc) {
-    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    const uint32_t d = nsg_len[n];
-    atomicAdd(acc, (unsigned long long)d);
-    atomicMax(acc + 1, (unsigned long long)d);
-    atomicMin(acc + 2, (unsigned long long)d);
+// CalculateNSG (src/kgraph.cpp:2273-2293): degree statistics; one atomic
+// per block.
+__global__ __launch_bounds__(1024) void k_degree_stats(const uint32_t *__restrict__ nsg_len, uint32_t N,
+                                                       unsigned long long *__restrict__ acc) {
+    __shared__ uint32_t s_mx, s_mn;
+    if (threadIdx.x == 0) { s_mx = 0; s_mn = 0xFFFFFFFFu; }
+    __syncthreads();
+    unsigned long long sum = 0;
+    uint32_t mx = 0, mn = 0xFFFFFFFFu;
+    for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+        const uint32_t d = nsg_len[n];
+        sum += d;
+        mx = d > mx ? d : mx;
+        mn = d < mn ? d : mn;
+    }
+    atomicMax(&s_mx, mx);
+    atomicMin(&s_mn, mn);
+    sum = block_sum(sum);
+    if (threadIdx.x == 0) {
+        atomicAdd(acc, sum);
+        atomicMax(acc + 1, (unsigned long long)s_mx);
+        atomicMin(acc + 2, (unsigned long long)s_mn);
+    }
 }
 
 __global__ void k_pack_csr(const uint32_t *__restrict__ nsg_id, const uint32_t *__restrict__ nsg_len, uint32_t N,
@@ -308,7 +323,8 @@ static void update_nhoods(BuildState &b) {
 static void calculate_nsg(BuildState &b) {
     unsigned long long init[3] = {0, 0, ~0ull};
     FK_HIP(hipMemcpyAsync(b.acc.p, init, 24, hipMemcpyHostToDevice, b.s));
-    hipLaunchKernelGGL(k_degree_stats, dim3((b.N + 255) / 256), dim3(256), 0, b.s, b.nsg_len.p, b.N, b.acc.p);
+    const unsigned g = std::min<unsigned>(1024u, (b.N + 1023) / 1024);
+    hipLaunchKernelGGL(k_degree_stats, dim3(g), dim3(1024), 0, b.s, b.nsg_len.p, b.N, b.acc.p);
     FK_LAUNCH_CHECK();
     unsigned long long r[3];
     FK_HIP(hipMemcpyAsync(r, b.acc.p, 24, hipMemcpyDeviceToHost, b.s));

@@ -95,7 +95,29 @@ void launch_join_distances(hipStream_t s, const Dataset &ds, const uint32_t *ids
 #define FK_PRUNE_WPB 4
 #define FK_PRUNE_RMAX 256
 
-template <int RULE>
+// DP > 0: rows of exactly DP floats, each lane's candidate row held in
+// registers for all of its occlusion checks; DP == 0: any width, read from
+// memory per check.
+template <int DP>
+struct CandRow {
+    float4 v[DP / 4];
+    __device__ __forceinline__ void load(const float *p) {
+#pragma unroll
+        for (int k = 0; k < DP / 4; ++k) v[k] = reinterpret_cast<const float4 *>(p)[k];
+    }
+    __device__ __forceinline__ float dist(const float *m, const float *, uint32_t) const {
+        return l2_canon_reg<DP>(v, m);
+    }
+};
+template <>
+struct CandRow<0> {
+    __device__ __forceinline__ void load(const float *) {}
+    __device__ __forceinline__ float dist(const float *m, const float *p, uint32_t dpad) const {
+        return l2_canon(m, p, dpad);
+    }
+};
+
+template <int RULE, int DP>
 __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
                                                PruneLaunch a) {
     __shared__ uint32_t s_oid[FK_PRUNE_WPB][FK_PRUNE_RMAX];
@@ -137,6 +159,8 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ rows, u
         const float pd = key_dist(key);
         const bool skip = !valid || pid >= a.oracle_n;
         const float *prow = rows + (size_t)(skip ? 0u : pid) * stride;
+        CandRow<DP> crow;
+        crow.load(prow);
         bool occl = false;
         uint32_t mycc = 0;
         __builtin_amdgcn_wave_barrier();
@@ -145,7 +169,7 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ rows, u
                 const uint32_t tid = so[t];
                 if (tid == pid) { occl = true; break; }
                 ++mycc;
-                const float djk = l2_canon(rows + (size_t)tid * stride, prow, dpad);
+                const float djk = crow.dist(rows + (size_t)tid * stride, prow, dpad);
                 if (occludes<RULE>(djk, pd, sd[t], a.tau3, a.cos_thr)) { occl = true; break; }
             }
         }
@@ -166,7 +190,7 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ rows, u
                     occl = true;
                 } else {
                     ++mycc;
-                    const float djk = l2_canon(rows + (size_t)aid * stride, prow, dpad);
+                    const float djk = crow.dist(rows + (size_t)aid * stride, prow, dpad);
                     if (occludes<RULE>(djk, pd, ad, a.tau3, a.cos_thr)) occl = true;
                 }
             }
@@ -193,23 +217,26 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ rows, u
     }
 }
 
+template <int RULE>
+static void launch_prune_rule(hipStream_t s, const Dataset &ds, const PruneLaunch &a, unsigned grid) {
+    const dim3 blk(64 * FK_PRUNE_WPB);
+    switch (ds.dpad) {
+        case 32: hipLaunchKernelGGL((k_prune<RULE, 32>), dim3(grid), blk, 0, s, ds.rows.p, ds.stride, ds.dpad, a); break;
+        case 64: hipLaunchKernelGGL((k_prune<RULE, 64>), dim3(grid), blk, 0, s, ds.rows.p, ds.stride, ds.dpad, a); break;
+        case 96: hipLaunchKernelGGL((k_prune<RULE, 96>), dim3(grid), blk, 0, s, ds.rows.p, ds.stride, ds.dpad, a); break;
+        case 128: hipLaunchKernelGGL((k_prune<RULE, 128>), dim3(grid), blk, 0, s, ds.rows.p, ds.stride, ds.dpad, a); break;
+        default: hipLaunchKernelGGL((k_prune<RULE, 0>), dim3(grid), blk, 0, s, ds.rows.p, ds.stride, ds.dpad, a); break;
+    }
+}
+
 void launch_prune(hipStream_t s, const Dataset &ds, const PruneLaunch &a) {
     if (a.n_tasks == 0) return;
     FK_CHECK_ARG(a.R <= FK_PRUNE_RMAX, "nsg_R above the supported maximum (256)");
     const unsigned grid = (a.n_tasks + FK_PRUNE_WPB - 1) / FK_PRUNE_WPB;
     switch (a.rule) {
-        case FK_RULE_NSG:
-            hipLaunchKernelGGL(k_prune<FK_RULE_NSG>, dim3(grid), dim3(64 * FK_PRUNE_WPB), 0, s, ds.rows.p,
-                               ds.stride, ds.dpad, a);
-            break;
-        case FK_RULE_TAU:
-            hipLaunchKernelGGL(k_prune<FK_RULE_TAU>, dim3(grid), dim3(64 * FK_PRUNE_WPB), 0, s, ds.rows.p,
-                               ds.stride, ds.dpad, a);
-            break;
-        default:
-            hipLaunchKernelGGL(k_prune<FK_RULE_ALPHA>, dim3(grid), dim3(64 * FK_PRUNE_WPB), 0, s, ds.rows.p,
-                               ds.stride, ds.dpad, a);
-            break;
+        case FK_RULE_NSG: launch_prune_rule<FK_RULE_NSG>(s, ds, a, grid); break;
+        case FK_RULE_TAU: launch_prune_rule<FK_RULE_TAU>(s, ds, a, grid); break;
+        default: launch_prune_rule<FK_RULE_ALPHA>(s, ds, a, grid); break;
     }
     FK_LAUNCH_CHECK();
 }

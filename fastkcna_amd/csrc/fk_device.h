@@ -70,6 +70,51 @@ __device__ __forceinline__ float l2_canon(const float *__restrict__ a, const flo
     return r;
 }
 
+// Canonical squared L2 with one operand held in registers (DP floats,
+// compile-time): identical order and result to l2_canon.
+template <int DP>
+__device__ __forceinline__ void l2_step8_reg(const float4 &a0, const float4 &a1, const float *__restrict__ b,
+                                             float acc[8]) {
+    const float4 b0 = *reinterpret_cast<const float4 *>(b);
+    const float4 b1 = *reinterpret_cast<const float4 *>(b + 4);
+    float t;
+    t = a0.x - b0.x; acc[0] = __builtin_fmaf(t, t, acc[0]);
+    t = a0.y - b0.y; acc[1] = __builtin_fmaf(t, t, acc[1]);
+    t = a0.z - b0.z; acc[2] = __builtin_fmaf(t, t, acc[2]);
+    t = a0.w - b0.w; acc[3] = __builtin_fmaf(t, t, acc[3]);
+    t = a1.x - b1.x; acc[4] = __builtin_fmaf(t, t, acc[4]);
+    t = a1.y - b1.y; acc[5] = __builtin_fmaf(t, t, acc[5]);
+    t = a1.z - b1.z; acc[6] = __builtin_fmaf(t, t, acc[6]);
+    t = a1.w - b1.w; acc[7] = __builtin_fmaf(t, t, acc[7]);
+}
+
+template <int DP>
+__device__ __forceinline__ float l2_canon_reg(const float4 (&a)[DP / 4], const float *__restrict__ b) {
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    constexpr int dr = DP & 31;
+    constexpr int dd = DP - dr;
+    if (dr == 24) {
+        l2_step8_reg<DP>(a[(dd + 16) / 4], a[(dd + 16) / 4 + 1], b + dd + 16, acc);
+        l2_step8_reg<DP>(a[(dd + 8) / 4], a[(dd + 8) / 4 + 1], b + dd + 8, acc);
+        l2_step8_reg<DP>(a[dd / 4], a[dd / 4 + 1], b + dd, acc);
+    } else if (dr == 16) {
+        l2_step8_reg<DP>(a[(dd + 8) / 4], a[(dd + 8) / 4 + 1], b + dd + 8, acc);
+        l2_step8_reg<DP>(a[dd / 4], a[dd / 4 + 1], b + dd, acc);
+    } else if (dr == 8) {
+        l2_step8_reg<DP>(a[dd / 4], a[dd / 4 + 1], b + dd, acc);
+    }
+#pragma unroll
+    for (int i = 0; i < dd; i += 8) l2_step8_reg<DP>(a[i / 4], a[i / 4 + 1], b + i, acc);
+    float r = acc[0] + acc[1];
+    r += acc[2];
+    r += acc[3];
+    r += acc[4];
+    r += acc[5];
+    r += acc[6];
+    r += acc[7];
+    return r;
+}
+
 // Occlusion rules of the three pruner families; dqt is the kept neighbour's
 // distance to the pool owner, djk the kept neighbour to candidate distance,
 // pd the candidate's distance to the owner.
@@ -101,6 +146,22 @@ __device__ __forceinline__ T wave_sum(T v) {
 }
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (FK_WAVE - 1); }
+
+// Block-wide sum (blockDim.x a multiple of 64, at most 1024); the result is
+// valid in thread 0.  Every thread of the block must call it.
+template <typename T>
+__device__ __forceinline__ T block_sum(T v) {
+    __shared__ T s_part[16];
+    v = wave_sum(v);
+    const uint32_t w = threadIdx.x >> 6;
+    if (lane_id() == 0) s_part[w] = v;
+    __syncthreads();
+    T r = 0;
+    if (threadIdx.x == 0)
+        for (uint32_t i = 0; i < (blockDim.x >> 6); ++i) r += s_part[i];
+    __syncthreads();
+    return r;
+}
 
 // Counter-based RNG: splitmix64 finaliser over a 64-bit counter.
 __device__ __host__ __forceinline__ uint64_t fk_mix64(uint64_t x) {

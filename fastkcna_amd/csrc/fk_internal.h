@@ -69,6 +69,28 @@ struct DevBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
+// Pinned (page-locked) host buffer, grow-only.
+template <typename T>
+struct PinnedBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    void ensure(size_t count) {
+        if (count <= n && p) return;
+        release();
+        FK_HIP(hipHostMalloc((void **)&p, (count ? count : 1) * sizeof(T), hipHostMallocDefault));
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~PinnedBuf() { release(); }
+};
+
 struct Dataset {
     DevBuf<float> rows;   // n_total x stride floats, zero padded
     uint32_t n_total = 0;
@@ -85,6 +107,7 @@ struct fk_handle {
     std::string last_error;
     fk::Dataset ds;
     fk::DevBuf<unsigned long long> counters;  // small scratch for kernel counters
+    fk::PinnedBuf<uint32_t> host_rows;         // connectivity pass: pool rows of attached ids
 };
 
 namespace fk {

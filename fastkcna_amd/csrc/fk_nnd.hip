@@ -359,7 +359,6 @@ struct MergeShared {
     uint64_t o[FK_POOL_MAX > FK_MRG_CH ? FK_POOL_MAX : FK_MRG_CH];
     uint64_t c[FK_MRG_CH];
     uint64_t c2[FK_MRG_CH];
-    uint32_t src[FK_MRG_CH];
     uint32_t keep[FK_MRG_CH];
     uint32_t part[256];
     uint8_t pf[FK_POOL_MAX];
@@ -397,6 +396,7 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
                                                    const uint64_t *__restrict__ bkey,
                                                    const uint32_t *__restrict__ bsrc, uint8_t *__restrict__ found) {
     __shared__ MergeShared sh;
+    __shared__ uint32_t s_src[FK_MRG_CH];
     const uint32_t i = blockIdx.x;
     const uint32_t nb = bcnt[i];
     if (nb == 0) return;
@@ -413,17 +413,17 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
         const uint32_t m2 = next_pow2(m < 2 ? 2 : m);
         for (uint32_t k = threadIdx.x; k < m2; k += blockDim.x) {
             sh.c[k] = k < m ? bkey[base + c0 + k] : FK_KEY_EMPTY;
-            sh.src[k] = k < m ? bsrc[base + c0 + k] : 0xFFFFFFFFu;
+            s_src[k] = k < m ? bsrc[base + c0 + k] : 0xFFFFFFFFu;
         }
         __syncthreads();
-        block_sort_key_src(sh.c, sh.src, m2);
+        block_sort_key_src(sh.c, s_src, m2);
         const uint32_t np = sh.np;
         for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
             const uint64_t key = sh.c[k];
             const bool first = (k == 0) || (sh.c[k - 1] != key);
             const bool keep = first && !sorted_contains(sh.p, np, key);
             sh.keep[k] = keep ? 1u : 0u;
-            if (keep && key < kth) found[sh.src[k]] = 1;
+            if (keep && key < kth) found[s_src[k]] = 1;
         }
         __syncthreads();
         block_merge(sh.p, sh.pf, &sh.np, sh.c, sh.keep, m, Lp, (uint8_t)(FK_FLG_NEW | FK_FLG_ISNEW), sh.c2, sh.o,
@@ -442,17 +442,19 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
 // delta = mean over nodes of (#flagged among the first K entries) / K;
 // recall = mean over controls of |pool[0..min(L,K)) ∩ gt[0..K)| / K.
 // ---------------------------------------------------------------------------
-__global__ void k_delta(const uint8_t *__restrict__ pflg, const uint32_t *__restrict__ cnt, uint32_t N, uint32_t cap,
-                        uint32_t K, unsigned long long *__restrict__ acc) {
-    const uint32_t n = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+// One wavefront per node, grid-stride; one atomic per block.
+__global__ __launch_bounds__(1024) void k_delta(const uint8_t *__restrict__ pflg, const uint32_t *__restrict__ cnt,
+                                                uint32_t N, uint32_t cap, uint32_t K,
+                                                unsigned long long *__restrict__ acc) {
+    const uint32_t wpb = blockDim.x >> 6;
     unsigned long long c = 0;
-    if (n < N) {
+    for (uint32_t n = blockIdx.x * wpb + (threadIdx.x >> 6); n < N; n += gridDim.x * wpb) {
         const uint32_t lim = cnt[n] < K ? cnt[n] : K;
         const uint8_t *pf = pflg + (uint64_t)n * cap;
         for (uint32_t l = lane_id(); l < lim; l += 64) c += (pf[l] & FK_FLG_NEW) ? 1 : 0;
     }
-    c = wave_sum(c);
-    if (lane_id() == 0 && c) atomicAdd(acc, c);
+    c = block_sum(c);
+    if (threadIdx.x == 0 && c) atomicAdd(acc, c);
 }
 
 __global__ __launch_bounds__(256) void k_recall(const uint64_t *__restrict__ pool, const uint32_t *__restrict__ cnt,
@@ -636,11 +638,17 @@ __global__ __launch_bounds__(256) void k_rev_select(uint32_t N, uint32_t R, uint
     }
 }
 
-__global__ void k_sum_u32(const uint32_t *__restrict__ v, uint32_t n, unsigned long long *__restrict__ acc) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long x = i < n ? v[i] : 0u;
-    x = wave_sum(x);
-    if (lane_id() == 0 && x) atomicAdd(acc, x);
+__global__ __launch_bounds__(1024) void k_sum_u32(const uint32_t *__restrict__ v, uint32_t n,
+                                                  unsigned long long *__restrict__ acc) {
+    unsigned long long x = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x += v[i];
+    x = block_sum(x);
+    if (threadIdx.x == 0 && x) atomicAdd(acc, x);
+}
+
+static unsigned reduce_grid(uint32_t n_waves) {
+    const unsigned g = (n_waves + 15) / 16;
+    return g < 2048 ? (g ? g : 1) : 2048;
 }
 
 // ---------------------------------------------------------------------------
@@ -760,7 +768,7 @@ static void nnd_join(BuildState &b) {
     double pool_entries = 0;
     if (b.prof) {
         FK_HIP(hipMemsetAsync(b.acc.p + 6, 0, 8, b.s));
-        hipLaunchKernelGGL(k_sum_u32, dim3((N + 255) / 256), dim3(256), 0, b.s, b.cnt.p, N, b.acc.p + 6);
+        hipLaunchKernelGGL(k_sum_u32, dim3(reduce_grid(N / 64 + 1)), dim3(1024), 0, b.s, b.cnt.p, N, b.acc.p + 6);
         unsigned long long pe_cnt = 0;
         FK_HIP(hipMemcpyAsync(&pe_cnt, b.acc.p + 6, 8, hipMemcpyDeviceToHost, b.s));
         b.sync();
@@ -854,7 +862,7 @@ void nnd_run(BuildState &b) {
         nnd_join(b);
         // statistics
         FK_HIP(hipMemsetAsync(b.acc.p, 0, 8, b.s));
-        hipLaunchKernelGGL(k_delta, dim3((N + 3) / 4), dim3(256), 0, b.s, b.pflg.p, b.cnt.p, N, b.cap, P.K, b.acc.p);
+        hipLaunchKernelGGL(k_delta, dim3(reduce_grid(N)), dim3(1024), 0, b.s, b.pflg.p, b.cnt.p, N, b.cap, P.K, b.acc.p);
         FK_LAUNCH_CHECK();
         unsigned long long fl = 0;
         FK_HIP(hipMemcpyAsync(&fl, b.acc.p, 8, hipMemcpyDeviceToHost, b.s));
@@ -898,31 +906,63 @@ __global__ void k_bv_init(uint8_t *__restrict__ pflg, const uint32_t *__restrict
     }
 }
 
-__global__ void k_bv_count(const uint8_t *__restrict__ pflg, const uint32_t *__restrict__ cnt, uint32_t N,
-                           uint32_t cap, uint32_t W, uint32_t S, unsigned long long *__restrict__ acc) {
-    const uint32_t n = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    uint32_t f = 0;
-    if (n < N) {
+__global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ pflg, const uint32_t *__restrict__ cnt,
+                                                   uint32_t N, uint32_t cap, uint32_t W, uint32_t S,
+                                                   unsigned long long *__restrict__ acc) {
+    const uint32_t wpb = blockDim.x >> 6;
+    unsigned long long capped = 0, all = 0;
+    for (uint32_t n = blockIdx.x * wpb + (threadIdx.x >> 6); n < N; n += gridDim.x * wpb) {
         const uint8_t *pf = pflg + (uint64_t)n * cap;
         const uint32_t lim = cnt[n] < W ? cnt[n] : W;
+        uint32_t f = 0;
         for (uint32_t j = lane_id(); j < lim; j += 64) f += (pf[j] & FK_FLG_NEW) ? 1u : 0u;
+        f = wave_sum(f);
+        if (lane_id() == 0) {
+            capped += f < S ? f : S;
+            all += f;
+        }
     }
-    f = wave_sum(f);
-    if (lane_id() == 0 && n < N) {
-        const uint32_t capped = f < S ? f : S;
-        atomicAdd(acc, (unsigned long long)capped);
-        atomicAdd(acc + 1, (unsigned long long)f);
+    capped = block_sum(capped);
+    all = block_sum(all);
+    if (threadIdx.x == 0) {
+        if (capped) atomicAdd(acc, capped);
+        if (all) atomicAdd(acc + 1, all);
     }
 }
 
+#define FK_BV_HASH 4096
+#define FK_BV_HASH_FILL (FK_BV_HASH * 3 / 4)
+
 struct BvShared {
+    alignas(16) float row[1024];      // the owner's row (float4 reads)
     MergeShared m;
     uint32_t sel[FK_POOL_MAX];
+    uint32_t sdeg[FK_POOL_MAX + 1];  // exclusive prefix of the selected out-degrees
     uint8_t sel_new[FK_POOL_MAX];
+    uint32_t table[FK_BV_HASH];       // ids already in the pool or seen this round
     uint32_t nsel;
     uint32_t nc;
+    uint32_t fill;
     float thr;
 };
+
+// Inserts id into the round's id set; true when it was not there.  When the
+// table is three-quarters full every id counts as new (the merge still
+// removes duplicates), so the set only ever saves work.
+__device__ __forceinline__ bool bv_set_insert(uint32_t *table, uint32_t *fill, uint32_t id) {
+    if (*(volatile uint32_t *)fill >= FK_BV_HASH_FILL) return true;
+    uint32_t h = (id * 2654435761u) & (FK_BV_HASH - 1);
+    for (uint32_t probe = 0; probe < 64; ++probe) {
+        const uint32_t old = atomicCAS(&table[h], 0xFFFFFFFFu, id);
+        if (old == 0xFFFFFFFFu) {
+            atomicAdd(fill, 1u);
+            return true;
+        }
+        if (old == id) return false;
+        h = (h + 1) & (FK_BV_HASH - 1);
+    }
+    return true;
+}
 
 __global__ __launch_bounds__(256) void k_bv_round(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
                                                   uint32_t N, uint32_t cap, uint32_t W, uint32_t S_cap,
@@ -932,90 +972,102 @@ __global__ __launch_bounds__(256) void k_bv_round(const float *__restrict__ rows
                                                   const uint8_t *__restrict__ nsg_new, uint32_t ncap,
                                                   unsigned long long *__restrict__ cc_acc) {
     __shared__ BvShared sh;
-    extern __shared__ __align__(16) float s_row[];
     const uint32_t i = blockIdx.x;
     uint64_t *pr = pool + (uint64_t)i * cap;
     uint8_t *pf = pflg + (uint64_t)i * cap;
     const uint32_t np0 = cnt[i];
     const uint32_t lim = np0 < W ? np0 : W;
     for (uint32_t j = threadIdx.x; j < np0; j += blockDim.x) { sh.m.p[j] = pr[j]; sh.m.pf[j] = pf[j]; }
+    for (uint32_t j = threadIdx.x; j < FK_BV_HASH; j += blockDim.x) sh.table[j] = 0xFFFFFFFFu;
     __syncthreads();
     // select flagged window entries (first S_cap in pool order), clear them
-    if (threadIdx.x == 0) {
-        uint32_t ns = 0;
-        for (uint32_t j = 0; j < lim && ns < S_cap; ++j) {
-            const uint8_t f = sh.m.pf[j];
-            if (f & FK_FLG_NEW) {
-                sh.sel[ns] = key_id(sh.m.p[j]);
-                sh.sel_new[ns] = (f & FK_FLG_ISNEW) ? 1 : 0;
-                ++ns;
-                sh.m.pf[j] = f & (uint8_t)~FK_FLG_NEW;
-            }
+    for (uint32_t j = threadIdx.x; j < lim; j += blockDim.x) sh.m.keep[j] = (sh.m.pf[j] & FK_FLG_NEW) ? 1u : 0u;
+    __syncthreads();
+    const uint32_t nflag = block_scan_inplace(sh.m.keep, lim, sh.m.part);
+    const uint32_t ns = nflag < S_cap ? nflag : S_cap;
+    for (uint32_t j = threadIdx.x; j < lim; j += blockDim.x) {
+        const uint8_t f = sh.m.pf[j];
+        const uint32_t r = sh.m.keep[j];
+        if ((f & FK_FLG_NEW) && r < ns) {
+            const uint32_t q = key_id(sh.m.p[j]);
+            sh.sel[r] = q;
+            sh.sel_new[r] = (f & FK_FLG_ISNEW) ? 1 : 0;
+            sh.sdeg[r] = nsg_len[q];
+            sh.m.pf[j] = f & (uint8_t)~FK_FLG_NEW;
         }
+    }
+    __syncthreads();
+    const uint32_t tot = block_scan_inplace(sh.sdeg, ns, sh.m.part);
+    if (threadIdx.x == 0) {
+        sh.sdeg[ns] = tot;
         sh.nsel = ns;
         sh.m.np = np0;
         sh.nc = 0;
+        sh.fill = 0;
         sh.thr = np0 >= cap ? key_dist(sh.m.p[cap - 1]) : INFINITY;
     }
     for (uint32_t k = threadIdx.x; k < (dpad >> 2); k += blockDim.x)
-        reinterpret_cast<float4 *>(s_row)[k] = reinterpret_cast<const float4 *>(rows + (size_t)i * stride)[k];
+        reinterpret_cast<float4 *>(sh.row)[k] = reinterpret_cast<const float4 *>(rows + (size_t)i * stride)[k];
     __syncthreads();
-    if (sh.nsel == 0) return;
-    const float thr = sh.thr;
-    unsigned long long cc = 0;
     const uint32_t nsel = sh.nsel;
-    for (uint32_t s = 0; s < nsel; ++s) {
-        const uint32_t q = sh.sel[s];
-        const bool qnew = sh.sel_new[s] != 0;
-        const uint32_t deg = nsg_len[q];
-        for (uint32_t m0 = 0; m0 < deg; m0 += blockDim.x) {
-            const uint32_t m = m0 + threadIdx.x;
-            bool emit = false;
-            uint64_t key = 0;
-            if (m < deg) {
-                const uint32_t t = nsg_id[(uint64_t)q * ncap + m];
-                const bool tnew = nsg_new[(uint64_t)q * ncap + m] != 0;
-                if (t != i && (qnew || tnew)) {
-                    ++cc;
-                    const float dd = l2_canon(s_row, rows + (size_t)t * stride, dpad);
-                    if (dd <= thr) { emit = true; key = make_key(dd, t); }
+    if (nsel == 0) return;
+    // ids already in the pool need no distance
+    for (uint32_t j = threadIdx.x; j < np0; j += blockDim.x) bv_set_insert(sh.table, &sh.fill, key_id(sh.m.p[j]));
+    bv_set_insert(sh.table, &sh.fill, i);
+    __syncthreads();
+    const float thr = sh.thr;
+    const uint32_t total = sh.sdeg[nsel];
+    unsigned long long cc = 0;
+    for (uint32_t f0 = 0; f0 < total; f0 += blockDim.x) {
+        const uint32_t f = f0 + threadIdx.x;
+        if (f < total) {
+            // entry s with sdeg[s] <= f < sdeg[s + 1]
+            uint32_t lo = 0, hi = nsel;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (sh.sdeg[mid] <= f) lo = mid; else hi = mid;
+            }
+            const uint32_t q = sh.sel[lo];
+            const uint32_t m = f - sh.sdeg[lo];
+            const uint32_t t = nsg_id[(uint64_t)q * ncap + m];
+            const bool tnew = nsg_new[(uint64_t)q * ncap + m] != 0;
+            if (t != i && (sh.sel_new[lo] || tnew)) {
+                ++cc;  // the reference evaluates every such pair (search_cc)
+                if (bv_set_insert(sh.table, &sh.fill, t)) {
+                    const float dd = l2_canon(sh.row, rows + (size_t)t * stride, dpad);
+                    if (dd <= thr) sh.m.c[atomicAdd(&sh.nc, 1u)] = make_key(dd, t);
                 }
             }
-            if (emit) {
-                const uint32_t slot = atomicAdd(&sh.nc, 1u);
-                sh.m.c[slot] = key;
-            }
-            __syncthreads();
-            // flush when the next batch might not fit, or at the very end
-            const bool last = (s + 1 == nsel) && (m0 + blockDim.x >= deg);
-            const uint32_t nc = sh.nc;
-            __syncthreads();  // every thread has read nc before it may change
-            if (last || nc + blockDim.x > FK_MRG_CH) {
-                if (nc > 0) {
-                    const uint32_t m2 = next_pow2(nc < 2 ? 2 : nc);
-                    for (uint32_t k = nc + threadIdx.x; k < m2; k += blockDim.x) sh.m.c[k] = FK_KEY_EMPTY;
-                    __syncthreads();
-                    block_bitonic_sort(sh.m.c, m2);
-                    const uint32_t npp = sh.m.np;
-                    for (uint32_t k = threadIdx.x; k < nc; k += blockDim.x) {
-                        const uint64_t key2 = sh.m.c[k];
-                        const bool first = (k == 0) || (sh.m.c[k - 1] != key2);
-                        sh.m.keep[k] = (first && !sorted_contains(sh.m.p, npp, key2)) ? 1u : 0u;
-                    }
-                    __syncthreads();
-                    block_merge(sh.m.p, sh.m.pf, &sh.m.np, sh.m.c, sh.m.keep, nc, cap,
-                                (uint8_t)(FK_FLG_NEW | FK_FLG_ISNEW), sh.m.c2, sh.m.o, sh.m.of, sh.m.part);
-                }
-                if (threadIdx.x == 0) sh.nc = 0;
+        }
+        __syncthreads();
+        const bool last = f0 + blockDim.x >= total;
+        const uint32_t nc = sh.nc;
+        __syncthreads();  // every thread has read nc before it may change
+        if (last || nc + blockDim.x > FK_MRG_CH) {
+            if (nc > 0) {
+                const uint32_t m2 = next_pow2(nc < 2 ? 2 : nc);
+                for (uint32_t k = nc + threadIdx.x; k < m2; k += blockDim.x) sh.m.c[k] = FK_KEY_EMPTY;
                 __syncthreads();
+                block_bitonic_sort(sh.m.c, m2);
+                const uint32_t npp = sh.m.np;
+                for (uint32_t k = threadIdx.x; k < nc; k += blockDim.x) {
+                    const uint64_t key2 = sh.m.c[k];
+                    const bool first = (k == 0) || (sh.m.c[k - 1] != key2);
+                    sh.m.keep[k] = (first && !sorted_contains(sh.m.p, npp, key2)) ? 1u : 0u;
+                }
+                __syncthreads();
+                block_merge(sh.m.p, sh.m.pf, &sh.m.np, sh.m.c, sh.m.keep, nc, cap,
+                            (uint8_t)(FK_FLG_NEW | FK_FLG_ISNEW), sh.m.c2, sh.m.o, sh.m.of, sh.m.part);
             }
+            if (threadIdx.x == 0) sh.nc = 0;
+            __syncthreads();
         }
     }
     const uint32_t np = sh.m.np;
     for (uint32_t j = threadIdx.x; j < np; j += blockDim.x) { pr[j] = sh.m.p[j]; pf[j] = sh.m.pf[j]; }
     if (threadIdx.x == 0) cnt[i] = np;
-    cc = wave_sum(cc);
-    if (lane_id() == 0 && cc) atomicAdd(cc_acc, cc);
+    cc = block_sum(cc);
+    if (threadIdx.x == 0 && cc) atomicAdd(cc_acc, cc);
 }
 
 void bridgeview(BuildState &b) {
@@ -1032,10 +1084,10 @@ void bridgeview(BuildState &b) {
     const uint32_t S = P.massq_S;
     uint64_t cc_total = 0;
     uint32_t rounds = 0;
-    const size_t lds_row = (size_t)b.ds->dpad * sizeof(float);
     while (true) {
         FK_HIP(hipMemsetAsync(b.acc.p, 0, 16, b.s));
-        hipLaunchKernelGGL(k_bv_count, dim3(gw), dim3(256), 0, b.s, b.pflg.p, b.cnt.p, N, b.cap, W, S, b.acc.p);
+        hipLaunchKernelGGL(k_bv_count, dim3(reduce_grid(N)), dim3(1024), 0, b.s, b.pflg.p, b.cnt.p, N, b.cap, W, S,
+                           b.acc.p);
         FK_LAUNCH_CHECK();
         unsigned long long cnts[2] = {0, 0};
         FK_HIP(hipMemcpyAsync(cnts, b.acc.p, 16, hipMemcpyDeviceToHost, b.s));
@@ -1048,7 +1100,7 @@ void bridgeview(BuildState &b) {
         double pool_entries = 0;
         if (b.prof) {
             FK_HIP(hipMemsetAsync(b.acc.p + 6, 0, 8, b.s));
-            hipLaunchKernelGGL(k_sum_u32, dim3((N + 255) / 256), dim3(256), 0, b.s, b.cnt.p, N, b.acc.p + 6);
+            hipLaunchKernelGGL(k_sum_u32, dim3(reduce_grid(N / 64 + 1)), dim3(1024), 0, b.s, b.cnt.p, N, b.acc.p + 6);
             unsigned long long pc = 0;
             FK_HIP(hipMemcpyAsync(&pc, b.acc.p + 6, 8, hipMemcpyDeviceToHost, b.s));
             b.sync();
@@ -1056,7 +1108,7 @@ void bridgeview(BuildState &b) {
         }
         FK_HIP(hipMemsetAsync(b.acc.p, 0, 8, b.s));
         hipEvent_t pe = b.prof_begin();
-        hipLaunchKernelGGL(k_bv_round, dim3(N), dim3(256), lds_row, b.s, b.ds->rows.p, b.ds->stride, b.ds->dpad, N,
+        hipLaunchKernelGGL(k_bv_round, dim3(N), dim3(256), 0, b.s, b.ds->rows.p, b.ds->stride, b.ds->dpad, N,
                            b.cap, W, scap, b.pool.p, b.pflg.p, b.cnt.p, b.nsg_id.p, b.nsg_len.p, b.nsg_new.p, b.ncap,
                            b.acc.p);
         FK_LAUNCH_CHECK();

@@ -268,145 +268,180 @@ void compute_ep(BuildState &b) {
 }
 
 // ---------------------------------------------------------------------------
-// BFS over the out-lists.
+// Connectivity (tree_grow, src/kgraph.cpp:2145-2270).
+//
+// The reference sweeps from ep, then repeatedly takes the smallest unreached
+// id u, links it from a root (the first reached pool entry of u with degree
+// < nsg_R, else a random reached node with degree < nsg_R * factor) and
+// sweeps again from u.  A sweep never leaves through a reached node (every
+// out-edge of a reached node was followed when it was reached), so u's sweep
+// reaches exactly the unreached nodes v whose smallest unreached reacher is
+// u.  Hence one min-label propagation over the unreached subgraph, label(v) =
+// min { w unreached : w reaches v }, replaces all sweeps: the attached ids are
+// the v with label(v) == v, taken in ascending order, and "v was reached
+// before u is attached" is "v reached from ep, or label(v) < u".  Only the
+// root choices (a sequential walk over degrees) run on the host.
 // ---------------------------------------------------------------------------
+#define FK_REACHED 0xFFFFFFFFu
+
 __global__ void k_bfs_expand(const uint32_t *__restrict__ nsg_id, const uint32_t *__restrict__ nsg_len,
                              uint32_t ncap, const uint32_t *__restrict__ front, uint32_t nf,
-                             uint32_t *__restrict__ vis, uint32_t *__restrict__ next, uint32_t *__restrict__ nnext) {
+                             uint32_t *__restrict__ label, uint32_t *__restrict__ next, uint32_t *__restrict__ nnext) {
     const uint32_t w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (w >= nf) return;
     const uint32_t f = front[w];
     const uint32_t deg = nsg_len[f];
     for (uint32_t k = lane_id(); k < deg; k += 64) {
         const uint32_t t = nsg_id[(uint64_t)f * ncap + k];
-        if (atomicExch(&vis[t], 1u) == 0u) next[atomicAdd(nnext, 1u)] = t;
+        if (atomicExch(&label[t], FK_REACHED) != FK_REACHED) next[atomicAdd(nnext, 1u)] = t;
     }
+}
+
+__global__ void k_label_init(uint32_t *__restrict__ label, uint32_t N) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < N) label[v] = v;
+}
+
+// One wavefront per node: push the node's label along its out-edges to
+// unreached targets (atomicMin); *changed is set when any label dropped.
+__global__ void k_label_prop(const uint32_t *__restrict__ nsg_id, const uint32_t *__restrict__ nsg_len, uint32_t N,
+                             uint32_t ncap, uint32_t *__restrict__ label, uint32_t *__restrict__ changed) {
+    const uint32_t u = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (u >= N) return;
+    const uint32_t lu = label[u];
+    if (lu == FK_REACHED) return;
+    const uint32_t deg = nsg_len[u];
+    bool ch = false;
+    for (uint32_t k = lane_id(); k < deg; k += 64) {
+        const uint32_t t = nsg_id[(uint64_t)u * ncap + k];
+        const uint32_t lt = label[t];
+        if (lt != FK_REACHED && lt > lu) {
+            if (atomicMin(&label[t], lu) > lu) ch = true;
+        }
+    }
+    if (wave_ballot(ch) && lane_id() == 0) *changed = 1u;
+}
+
+// Pool ids (first cnt entries) of a list of nodes, for the root choices.
+__global__ void k_gather_pool_ids(const uint64_t *__restrict__ pool, const uint32_t *__restrict__ cnt, uint32_t cap,
+                                  const uint32_t *__restrict__ nodes, uint32_t n, uint32_t *__restrict__ out) {
+    const uint32_t w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (w >= n) return;
+    const uint32_t v = nodes[w];
+    const uint32_t c = cnt[v];
+    uint32_t *o = out + (uint64_t)w * (cap + 1);
+    if (lane_id() == 0) o[0] = c;
+    for (uint32_t k = lane_id(); k < c; k += 64) o[1 + k] = key_id(pool[(uint64_t)v * cap + k]);
 }
 
 __global__ void k_set_edges(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
                             const uint32_t *__restrict__ src, const uint32_t *__restrict__ slot,
                             const uint32_t *__restrict__ dst, uint32_t n, uint32_t ncap, uint32_t *__restrict__ nsg_id,
-                            float *__restrict__ nsg_d) {
+                            float *__restrict__ nsg_d, uint32_t *__restrict__ nsg_len) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint64_t p = (uint64_t)src[i] * ncap + slot[i];
     nsg_id[p] = dst[i];
     nsg_d[p] = l2_canon(rows + (size_t)src[i] * stride, rows + (size_t)dst[i] * stride, dpad);
+    atomicMax(&nsg_len[src[i]], slot[i] + 1);
 }
-
-struct Bfs {
-    BuildState &b;
-    DevBuf<uint32_t> vis, f0, f1, cnt;
-    std::vector<uint8_t> hvis;
-    std::vector<uint32_t> hbuf;
-    explicit Bfs(BuildState &bb) : b(bb), vis(bb.N), f0(bb.N), f1(bb.N), cnt(1), hvis(bb.N, 0), hbuf(bb.N) {
-        FK_HIP(hipMemsetAsync(vis.p, 0, vis.bytes(), b.s));
-    }
-    // Marks everything reachable from `root` (which must be unreached);
-    // returns the number of nodes newly reached.
-    uint32_t run(uint32_t root) {
-        const uint32_t one = 1;
-        FK_HIP(hipMemcpyAsync(vis.p + root, &one, 4, hipMemcpyHostToDevice, b.s));
-        FK_HIP(hipMemcpyAsync(f0.p, &root, 4, hipMemcpyHostToDevice, b.s));
-        b.sync();
-        hvis[root] = 1;
-        uint32_t nf = 1, total = 1;
-        uint32_t *cur = f0.p, *nxt = f1.p;
-        while (nf) {
-            FK_HIP(hipMemsetAsync(cnt.p, 0, 4, b.s));
-            hipEvent_t pe = b.prof_begin();
-            hipLaunchKernelGGL(k_bfs_expand, dim3((nf + 3) / 4), dim3(256), 0, b.s, b.nsg_id.p, b.nsg_len.p, b.ncap,
-                               cur, nf, vis.p, nxt, cnt.p);
-            FK_LAUNCH_CHECK();
-            uint32_t nn = 0;
-            FK_HIP(hipMemcpyAsync(&nn, cnt.p, 4, hipMemcpyDeviceToHost, b.s));
-            b.sync();
-            // frontier out-lists read (ids + length), reached flags written
-            b.prof_end(pe, FK_PROF_CONN, (double)nf * (4.0 + 4.0 * b.P.nsg_R) + 8.0 * nn);
-            if (nn) {
-                FK_HIP(hipMemcpyAsync(hbuf.data(), nxt, (size_t)nn * 4, hipMemcpyDeviceToHost, b.s));
-                b.sync();
-                for (uint32_t i = 0; i < nn; ++i) hvis[hbuf[i]] = 1;
-            }
-            total += nn;
-            nf = nn;
-            std::swap(cur, nxt);
-        }
-        return total;
-    }
-};
 
 void tree_grow(BuildState &b) {
     const double t0 = now_sec();
-    struct Flush {
-        BuildState &b;
-        ~Flush() {
-            try { b.prof_flush(); } catch (...) {}
-        }
-    } flush{b};
     const uint32_t N = b.N;
-    Bfs bfs(b);
-    uint32_t reached = bfs.run(b.ep);
-    std::vector<uint32_t> deg(N);
+    hipEvent_t pe = b.prof_begin();
+    DevBuf<uint32_t> label(N), f0(N), f1(N), cnt(1);
+    // 1. nodes reachable from ep are marked FK_REACHED (frontier BFS)
+    hipLaunchKernelGGL(k_label_init, dim3((N + 255) / 256), dim3(256), 0, b.s, label.p, N);
+    FK_LAUNCH_CHECK();
+    const uint32_t reached_mark = FK_REACHED;
+    FK_HIP(hipMemcpyAsync(label.p + b.ep, &reached_mark, 4, hipMemcpyHostToDevice, b.s));
+    FK_HIP(hipMemcpyAsync(f0.p, &b.ep, 4, hipMemcpyHostToDevice, b.s));
+    uint32_t nf = 1, levels = 0;
+    uint32_t *cur = f0.p, *nxt = f1.p;
+    while (nf) {
+        FK_HIP(hipMemsetAsync(cnt.p, 0, 4, b.s));
+        hipLaunchKernelGGL(k_bfs_expand, dim3((nf + 3) / 4), dim3(256), 0, b.s, b.nsg_id.p, b.nsg_len.p, b.ncap, cur,
+                           nf, label.p, nxt, cnt.p);
+        FK_LAUNCH_CHECK();
+        FK_HIP(hipMemcpyAsync(&nf, cnt.p, 4, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+        std::swap(cur, nxt);
+        ++levels;
+    }
+    // 2. min-label propagation over the unreached subgraph
+    uint32_t rounds = 0;
+    while (true) {
+        FK_HIP(hipMemsetAsync(cnt.p, 0, 4, b.s));
+        hipLaunchKernelGGL(k_label_prop, dim3((N + 3) / 4), dim3(256), 0, b.s, b.nsg_id.p, b.nsg_len.p, N, b.ncap,
+                           label.p, cnt.p);
+        FK_LAUNCH_CHECK();
+        uint32_t ch = 0;
+        FK_HIP(hipMemcpyAsync(&ch, cnt.p, 4, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+        ++rounds;
+        if (!ch) break;
+    }
+    std::vector<uint32_t> lab(N), deg(N);
+    FK_HIP(hipMemcpyAsync(lab.data(), label.p, (size_t)N * 4, hipMemcpyDeviceToHost, b.s));
     FK_HIP(hipMemcpyAsync(deg.data(), b.nsg_len.p, (size_t)N * 4, hipMemcpyDeviceToHost, b.s));
     b.sync();
+    // every round reads each unreached node's out-list and its targets' labels
+    b.prof_end(pe, FK_PROF_CONN, (double)(rounds + levels) * ((double)N * 8.0 + 8.0 * (double)N * b.P.nsg_R));
+    b.prof_flush();
+    std::vector<uint32_t> roots_of;  // attached ids, ascending
+    for (uint32_t v = 0; v < N; ++v)
+        if (lab[v] == v) roots_of.push_back(v);
+    const uint32_t na = (uint32_t)roots_of.size();
     std::vector<uint32_t> e_src, e_slot, e_dst;
-    std::vector<uint64_t> prow(b.cap);
-    uint32_t start = 0;
-    while (reached < N) {
-        uint32_t id = N;
-        for (uint32_t i = start; i < N; ++i)
-            if (!bfs.hvis[i]) { id = i; start = i; break; }
-        if (id == N) break;
-        // findroot: first reached pool neighbour with room (:2201-2210)
-        uint32_t pc = 0;
-        FK_HIP(hipMemcpyAsync(&pc, b.cnt.p + id, 4, hipMemcpyDeviceToHost, b.s));
+    if (na) {
+        // 3. pool rows of the attached ids, one gather
+        const uint32_t rs = b.cap + 1;
+        DevBuf<uint32_t> dnodes(na), drows((size_t)na * rs);
+        PinnedBuf<uint32_t> &rows = b.h->host_rows;
+        rows.ensure((size_t)na * rs);
+        FK_HIP(hipMemcpyAsync(dnodes.p, roots_of.data(), (size_t)na * 4, hipMemcpyHostToDevice, b.s));
+        hipLaunchKernelGGL(k_gather_pool_ids, dim3((na + 3) / 4), dim3(256), 0, b.s, b.pool.p, b.cnt.p, b.cap, dnodes.p,
+                           na, drows.p);
+        FK_LAUNCH_CHECK();
+        FK_HIP(hipMemcpyAsync(rows.p, drows.p, (size_t)na * rs * 4, hipMemcpyDeviceToHost, b.s));
         b.sync();
-        if (pc) {
-            FK_HIP(hipMemcpyAsync(prow.data(), b.pool.p + (uint64_t)id * b.cap, (size_t)pc * 8,
-                                  hipMemcpyDeviceToHost, b.s));
-            b.sync();
-        }
-        uint32_t root = N;
-        for (uint32_t i = 0; i < pc; ++i) {
-            const uint32_t p = (uint32_t)prow[i];
-            if (p < N && bfs.hvis[p] && deg[p] < b.P.nsg_R) { root = p; break; }
-        }
-        if (root == N) {
-            // random reached node whose degree is below nsg_R * factor (:2212-2228)
-            uint64_t tries = 0;
-            while (true) {
-                ++tries;
-                const uint64_t factor = tries / N + 1;
-                const uint32_t rid = (uint32_t)(b.next_rand() % N);
-                if (bfs.hvis[rid] && deg[rid] < b.ncap && deg[rid] < b.P.nsg_R * factor) { root = rid; break; }
-                if (tries > 64ull * N) throw Error(FK_E_INTERNAL, "tree_grow: no node can take an edge");
+        // 4. root choices in ascending order of the attached ids
+        for (uint32_t k = 0; k < na; ++k) {
+            const uint32_t u = roots_of[k];
+            auto reached = [&](uint32_t v) { return lab[v] == FK_REACHED || lab[v] < u; };
+            const uint32_t *prow = rows.p + (size_t)k * rs;
+            uint32_t root = N;
+            for (uint32_t i = 0; i < prow[0]; ++i) {
+                const uint32_t p = prow[1 + i];
+                if (p < N && reached(p) && deg[p] < b.P.nsg_R) { root = p; break; }
             }
+            if (root == N) {
+                uint64_t tries = 0;
+                while (true) {
+                    ++tries;
+                    const uint64_t factor = tries / N + 1;
+                    const uint32_t rid = (uint32_t)(b.next_rand() % N);
+                    if (reached(rid) && deg[rid] < b.ncap && deg[rid] < b.P.nsg_R * factor) { root = rid; break; }
+                    if (tries > 64ull * N) throw Error(FK_E_INTERNAL, "tree_grow: no node can take an edge");
+                }
+            }
+            e_src.push_back(root);
+            e_slot.push_back(deg[root]);
+            e_dst.push_back(u);
+            deg[root]++;
         }
-        e_src.push_back(root);
-        e_slot.push_back(deg[root]);
-        e_dst.push_back(id);
-        // the new edge is needed by the BFS below
-        const uint32_t nd = deg[root] + 1;
-        FK_HIP(hipMemcpyAsync(b.nsg_id.p + (uint64_t)root * b.ncap + deg[root], &id, 4, hipMemcpyHostToDevice, b.s));
-        FK_HIP(hipMemcpyAsync(b.nsg_len.p + root, &nd, 4, hipMemcpyHostToDevice, b.s));
-        b.sync();
-        deg[root] = nd;
-        reached += bfs.run(id);
-    }
-    if (!e_src.empty()) {
-        const uint32_t n = (uint32_t)e_src.size();
-        DevBuf<uint32_t> s(n), sl(n), d(n);
-        FK_HIP(hipMemcpyAsync(s.p, e_src.data(), (size_t)n * 4, hipMemcpyHostToDevice, b.s));
-        FK_HIP(hipMemcpyAsync(sl.p, e_slot.data(), (size_t)n * 4, hipMemcpyHostToDevice, b.s));
-        FK_HIP(hipMemcpyAsync(d.p, e_dst.data(), (size_t)n * 4, hipMemcpyHostToDevice, b.s));
-        hipLaunchKernelGGL(k_set_edges, dim3((n + 255) / 256), dim3(256), 0, b.s, b.ds->rows.p, b.ds->stride,
-                           b.ds->dpad, s.p, sl.p, d.p, n, b.ncap, b.nsg_id.p, b.nsg_d.p);
+        DevBuf<uint32_t> s(na), sl(na), d(na);
+        FK_HIP(hipMemcpyAsync(s.p, e_src.data(), (size_t)na * 4, hipMemcpyHostToDevice, b.s));
+        FK_HIP(hipMemcpyAsync(sl.p, e_slot.data(), (size_t)na * 4, hipMemcpyHostToDevice, b.s));
+        FK_HIP(hipMemcpyAsync(d.p, e_dst.data(), (size_t)na * 4, hipMemcpyHostToDevice, b.s));
+        hipLaunchKernelGGL(k_set_edges, dim3((na + 255) / 256), dim3(256), 0, b.s, b.ds->rows.p, b.ds->stride,
+                           b.ds->dpad, s.p, sl.p, d.p, na, b.ncap, b.nsg_id.p, b.nsg_d.p, b.nsg_len.p);
         FK_LAUNCH_CHECK();
         b.sync();
-        b.st->d_conn += n;
+        b.st->d_conn += na;
     }
-    b.st->n_attached += (uint32_t)e_src.size();
+    b.st->n_attached += na;
     uint32_t mx = b.width;
     for (uint32_t i = 0; i < N; ++i) mx = std::max(mx, deg[i]);
     b.width = mx;
@@ -414,7 +449,9 @@ void tree_grow(BuildState &b) {
     const uint32_t li = b.loopcount;
     if (li < FK_MAX_LOOPS) b.st->tree_grow_time[li] = dt;
     b.st->t_conn += dt;
-    if (b.verbose) fprintf(stderr, "[fk] tree_grow: attached %zu (%.3f s)\n", e_src.size(), dt);
+    if (b.verbose)
+        fprintf(stderr, "[fk] tree_grow: attached %u (%.3f s; %u BFS levels, %u label rounds)\n", na, dt, levels,
+                rounds);
 }
 
 }  // namespace fk
@@ -450,9 +487,7 @@ extern "C" fk_status fk_search(fk_handle *h, const uint64_t *offsets, const uint
         for (uint32_t q0 = 0; q0 < n_q; q0 += qb) {
             const uint32_t nq = std::min(qb, n_q - q0);
             FK_HIP(hipMemsetAsync(vis.p, 0, (size_t)nq * words * 4, h->stream));
-            if (ds.dpad != ds.d) FK_HIP(hipMemsetAsync(dq.p, 0, dq.bytes(), h->stream));
-            FK_HIP(hipMemcpy2DAsync(dq.p, (size_t)ds.dpad * 4, queries + (size_t)q0 * ds.d, (size_t)ds.d * 4,
-                                    (size_t)ds.d * 4, nq, hipMemcpyHostToDevice, h->stream));
+            upload_rows(h, dq.p, ds.dpad, queries + (size_t)q0 * ds.d, (size_t)ds.d * 4, nq, ds.d);
             BeamArgs a{};
             a.rows = ds.rows.p;
             a.stride = ds.stride;

@@ -56,8 +56,26 @@ def roofline(st, families):
             "share_of_build": ms[i] / 1000.0 / max(st["t_total"], 1e-9)}
 
 
-def cpu_baseline(n_sample, threads):
-    """The CPU oracle on a bounded sample of the same workload."""
+def copy_probe(mib):
+    """One device-to-device copy of `mib` MiB (HIP runtime copy kernel): the
+    reference point that scales FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    n = mib << 20
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy probe: {mib} MiB device-to-device")
+
+
+def cpu_baseline(n_sample, threads, dev, queries):
+    """The CPU oracle on a bounded sample of the same workload; the GPU builds
+    the same sample and both graphs are searched by the same evaluator."""
     import oracle
     from fastkcna_amd import FK_PG_NSG
     from fastkcna_amd.datasets import integer_mixture, params_c2
@@ -68,10 +86,23 @@ def cpu_baseline(n_sample, threads):
     off, ids, ep, width, st, _ = oracle.build(FK_PG_NSG, x, p, threads=threads)
     dt = time.perf_counter() - t0
     nd = total_dists(st)
+    # recall parity on the sample: one evaluator, one ground truth
+    dev.upload(x)
+    t1 = time.perf_counter()
+    goff, gids, gep, _, gst = dev.build(FK_PG_NSG, p)
+    gdt = time.perf_counter() - t1
+    gt, _ = dev.knn(queries, 10)
+    rc = recall_at_10(dev.search(off, ids, ep, queries, 80, 10)[0], gt)
+    rg = recall_at_10(dev.search(goff, gids, gep, queries, 80, 10)[0], gt)
+    kc, kg = float(st["recall"][st["n_recall"] - 1]), float(gst["recall"][gst["n_recall"] - 1])
     return {"value": nd / dt, "unit": "dist-evals/s", "cores": threads, "kind": "port",
             "sample": f"CPU oracle (C++/OpenMP restatement of the reference) NSG build of the first {n_sample:,} "
                       f"points of the same mixture, same parameters; {dt:.1f} s, {nd:,} distance evaluations",
-            "build_sec": dt}
+            "build_sec": dt,
+            "sample_parity": {"cpu_build_sec": round(dt, 3), "gpu_build_sec": round(gdt, 4),
+                              "speedup": round(dt / gdt, 1),
+                              "cpu_recall_at_10": rc, "gpu_recall_at_10": rg,
+                              "cpu_knn_recall_at_maxl": kc, "gpu_knn_recall_at_maxl": kg}}
 
 
 def main():
@@ -84,6 +115,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, help="MiB of a device-to-device copy to run first")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the build does not shard "
@@ -102,6 +134,8 @@ def main():
     p.flags = _ffi.FK_FLAG_PROFILE | (_ffi.FK_FLAG_VERBOSE if args.verbose else 0)
 
     dev = Device(0)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
     t = time.perf_counter()
     dev.upload(base)
     log(f"upload {time.perf_counter() - t:.2f}s")
@@ -139,8 +173,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(args.cpu_sample, threads)
-        log(f"cpu baseline {cpu['value']:.3e} dist-evals/s on {threads} threads ({cpu['build_sec']:.1f}s)")
+        cpu = cpu_baseline(args.cpu_sample, threads, dev, queries[:2000])
+        log(f"cpu baseline {cpu['value']:.3e} dist-evals/s on {threads} threads ({cpu['build_sec']:.1f}s); "
+            f"sample parity {cpu['sample_parity']}")
         cpu.pop("build_sec")
     out = {
         "metric": METRIC,

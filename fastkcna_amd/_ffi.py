@@ -20,7 +20,7 @@ FK_FLAG_PROFILE = 0x4
 FK_MAX_LOOPS = 16
 FK_PROF_N = 16
 PROF_FAMILIES = ["controls", "init", "join", "join_fill", "merge", "update", "prune", "prune_rev", "inter", "bv",
-                 "ep", "conn"]
+                 "ep", "conn", "order"]
 
 STATUS_NAMES = {
     FK_OK: "FK_OK",

@@ -11,6 +11,7 @@
 //   nsg_new   u8 [N][ncap]  isnew bit of each out-list entry
 //   old_id    u32[N][ncap]  previous loop's out-list (UpdateNhoods)
 #pragma once
+#include "fk_device.h"
 #include "fk_internal.h"
 
 namespace fk {
@@ -86,6 +87,13 @@ struct BuildState {
     std::vector<float> center;  // Calcenter over all resident rows (cached)
     uint64_t host_rng = 0;      // the reference's global rand() stream
 
+    // locality order (fk_order.hip): node at each position of the order
+    DevBuf<uint32_t> order;     // N
+    bool has_order = false;
+    DevBuf<uint32_t> ord_lab, ord_val;
+    DevBuf<uint64_t> ord_key;
+    DevBuf<uint8_t> ord_tmp;
+
     // scratch
     DevBuf<uint64_t> scal;      // 16 u64
     DevBuf<uint64_t> scan_tmp;  // prefix-scan partials
@@ -124,6 +132,21 @@ float nnd_evaluate(BuildState &b, uint32_t K);  // recall over controls at K
 
 // Pool merge and BridgeView (fk_nnd.hip)
 void bridgeview(BuildState &b);
+
+// Locality order of the nodes from the current pools (fk_order.hip)
+void locality_order(BuildState &b);
+
+// Node of this block for one-block-per-node kernels: with an order, XCD x
+// walks the contiguous run x of the order; without one, block = node.
+// Launch with node_grid(b) blocks; blocks past N return.
+__device__ __forceinline__ uint32_t node_of_block(const uint32_t *order, uint32_t N) {
+    if (!order) return blockIdx.x;
+    const uint32_t pos = xcd_position((N + FK_NUM_XCD - 1) / FK_NUM_XCD);
+    return pos < N ? order[pos] : 0xFFFFFFFFu;
+}
+inline unsigned node_grid(const BuildState &b) {
+    return b.has_order ? (b.N + FK_NUM_XCD - 1) / FK_NUM_XCD * FK_NUM_XCD : b.N;
+}
 
 // Loops (fk_build.hip)
 void build_loops(BuildState &b, fk_pg_type type);

@@ -234,6 +234,10 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
     a.out_len = b.nsg_len.p;
     a.cc = b.acc.p;
     a.examined = b.acc.p + 1;
+    if (b.has_order) {
+        a.task_list = b.order.p;
+        a.xcd_order = true;
+    }
     hipEvent_t pe = b.prof_begin();
     launch_prune(b.s, *b.ds, a);
     unsigned long long cc2[2] = {0, 0};
@@ -349,6 +353,7 @@ void build_loops(BuildState &b, fk_pg_type type) {
     b.old_len.alloc(N);
     FK_HIP(hipMemsetAsync(b.old_len.p, 0, (size_t)N * 4, b.s));
     b.loopcount = 0;
+    locality_order(b);
     const uint32_t iters = b.P.loop_i + 1;
     const double s0 = b.st->t_total;  // build start offset is tracked by the caller
     (void)s0;
@@ -372,6 +377,7 @@ void build_loops(BuildState &b, fk_pg_type type) {
         }
         const double ts = now_sec();
         bridgeview(b);
+        locality_order(b);
         if (li < FK_MAX_LOOPS) b.st->search_time[li] = now_sec() - ts;
         const float rec = nnd_evaluate(b, b.P.MAXL);
         if (b.st->n_recall < FK_MAX_LOOPS + 64) b.st->recall[b.st->n_recall++] = rec;

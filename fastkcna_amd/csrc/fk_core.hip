@@ -124,7 +124,8 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ rows, u
     __shared__ float s_od[FK_PRUNE_WPB][FK_PRUNE_RMAX];
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x >> 6;
-    const uint32_t task = blockIdx.x * FK_PRUNE_WPB + w;
+    const uint32_t vb = a.xcd_order ? xcd_position((gridDim.x + FK_NUM_XCD - 1) / FK_NUM_XCD) : blockIdx.x;
+    const uint32_t task = vb * FK_PRUNE_WPB + w;
     if (task >= a.n_tasks) return;
     const uint32_t slot = a.task_list ? a.task_list[task] : task;
     const uint64_t *c = a.cand + (a.cand_off ? a.cand_off[slot] : (uint64_t)slot * a.cand_stride);
@@ -232,7 +233,8 @@ static void launch_prune_rule(hipStream_t s, const Dataset &ds, const PruneLaunc
 void launch_prune(hipStream_t s, const Dataset &ds, const PruneLaunch &a) {
     if (a.n_tasks == 0) return;
     FK_CHECK_ARG(a.R <= FK_PRUNE_RMAX, "nsg_R above the supported maximum (256)");
-    const unsigned grid = (a.n_tasks + FK_PRUNE_WPB - 1) / FK_PRUNE_WPB;
+    unsigned grid = (a.n_tasks + FK_PRUNE_WPB - 1) / FK_PRUNE_WPB;
+    if (a.xcd_order) grid = (grid + FK_NUM_XCD - 1) / FK_NUM_XCD * FK_NUM_XCD;
     switch (a.rule) {
         case FK_RULE_NSG: launch_prune_rule<FK_RULE_NSG>(s, ds, a, grid); break;
         case FK_RULE_TAU: launch_prune_rule<FK_RULE_TAU>(s, ds, a, grid); break;

@@ -163,6 +163,16 @@ __device__ __forceinline__ T block_sum(T v) {
     return r;
 }
 
+// XCD-aware block order.  Workgroups are dispatched to the 8 XCDs round
+// robin (block b -> XCD b % 8), each XCD with its own L2; launching
+// 8 * per_xcd blocks and giving XCD x the contiguous run
+// [x * per_xcd, (x + 1) * per_xcd) of a locality order keeps neighbouring
+// work on one L2.  Returns the position in that order.
+#define FK_NUM_XCD 8
+__device__ __forceinline__ uint32_t xcd_position(uint32_t per_xcd) {
+    return (blockIdx.x % FK_NUM_XCD) * per_xcd + blockIdx.x / FK_NUM_XCD;
+}
+
 // Counter-based RNG: splitmix64 finaliser over a 64-bit counter.
 __device__ __host__ __forceinline__ uint64_t fk_mix64(uint64_t x) {
     x += 0x9E3779B97F4A7C15ull;

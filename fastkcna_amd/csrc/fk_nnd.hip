@@ -255,11 +255,13 @@ __global__ __launch_bounds__(256) void k_join_count(const float *__restrict__ ro
                                                     const uint32_t *__restrict__ n_new,
                                                     const uint64_t *__restrict__ pair_off,
                                                     const float *__restrict__ thr, float *__restrict__ pdist,
-                                                    uint32_t *__restrict__ resv, uint32_t *__restrict__ bcnt) {
+                                                    uint32_t *__restrict__ resv, uint32_t *__restrict__ bcnt,
+                                                    uint32_t N, const uint32_t *__restrict__ order) {
     extern __shared__ __align__(16) float s_rows[];
     __shared__ float s_thr[FK_LIST_MAX];
     __shared__ uint32_t s_c[FK_LIST_MAX];
-    const uint32_t node = blockIdx.x;
+    const uint32_t node = node_of_block(order, N);
+    if (node >= N) return;
     const uint64_t lo = list_off[node];
     const uint32_t T = (uint32_t)(list_off[node + 1] - lo);
     const uint32_t nn = n_new[node];
@@ -757,8 +759,9 @@ static void nnd_join(BuildState &b) {
     const size_t lds = FK_JOIN_LDS_ROWS_BYTES;
     const double d4 = 4.0 * b.ds->d;
     hipEvent_t pe = b.prof_begin();
-    hipLaunchKernelGGL(k_join_count, dim3(N), dim3(256), lds, b.s, b.ds->rows.p, b.ds->stride, b.ds->dpad,
-                       b.lists.p, b.list_off.p, b.n_new.p, b.pair_off.p, b.thr.p, b.pdist.p, b.resv.p, b.bcnt.p);
+    hipLaunchKernelGGL(k_join_count, dim3(node_grid(b)), dim3(256), lds, b.s, b.ds->rows.p, b.ds->stride,
+                       b.ds->dpad, b.lists.p, b.list_off.p, b.n_new.p, b.pair_off.p, b.thr.p, b.pdist.p, b.resv.p,
+                       b.bcnt.p, N, b.has_order ? b.order.p : nullptr);
     FK_LAUNCH_CHECK();
     // sampled rows and ids gathered once per join, one float per pair written
     b.prof_end(pe, FK_PROF_JOIN, (double)nlist * (d4 + 4.0) + 4.0 * (double)pairs);
@@ -879,6 +882,9 @@ void nnd_run(BuildState &b) {
         if (delta <= P.delta) { b.st->nnd_stop = 1; break; }
         if (b.n_ctrl && recall >= P.recall) { b.st->nnd_stop = 2; break; }
         if (it + 1 >= P.iterations) break;
+        // from the third join on, the pools' heads are close enough for a
+        // useful locality order
+        if (it >= 1) locality_order(b);
         nnd_update(b, it);
     }
     b.sync();
@@ -970,9 +976,11 @@ __global__ __launch_bounds__(256) void k_bv_round(const float *__restrict__ rows
                                                   uint32_t *__restrict__ cnt, const uint32_t *__restrict__ nsg_id,
                                                   const uint32_t *__restrict__ nsg_len,
                                                   const uint8_t *__restrict__ nsg_new, uint32_t ncap,
-                                                  unsigned long long *__restrict__ cc_acc) {
+                                                  unsigned long long *__restrict__ cc_acc,
+                                                  const uint32_t *__restrict__ order) {
     __shared__ BvShared sh;
-    const uint32_t i = blockIdx.x;
+    const uint32_t i = node_of_block(order, N);
+    if (i >= N) return;
     uint64_t *pr = pool + (uint64_t)i * cap;
     uint8_t *pf = pflg + (uint64_t)i * cap;
     const uint32_t np0 = cnt[i];
@@ -1108,9 +1116,9 @@ void bridgeview(BuildState &b) {
         }
         FK_HIP(hipMemsetAsync(b.acc.p, 0, 8, b.s));
         hipEvent_t pe = b.prof_begin();
-        hipLaunchKernelGGL(k_bv_round, dim3(N), dim3(256), 0, b.s, b.ds->rows.p, b.ds->stride, b.ds->dpad, N,
-                           b.cap, W, scap, b.pool.p, b.pflg.p, b.cnt.p, b.nsg_id.p, b.nsg_len.p, b.nsg_new.p, b.ncap,
-                           b.acc.p);
+        hipLaunchKernelGGL(k_bv_round, dim3(node_grid(b)), dim3(256), 0, b.s, b.ds->rows.p, b.ds->stride,
+                           b.ds->dpad, N, b.cap, W, scap, b.pool.p, b.pflg.p, b.cnt.p, b.nsg_id.p, b.nsg_len.p,
+                           b.nsg_new.p, b.ncap, b.acc.p, b.has_order ? b.order.p : nullptr);
         FK_LAUNCH_CHECK();
         unsigned long long cc = 0;
         FK_HIP(hipMemcpyAsync(&cc, b.acc.p, 8, hipMemcpyDeviceToHost, b.s));

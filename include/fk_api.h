@@ -77,6 +77,7 @@ enum {
     FK_PROF_BV = 9,       /* BridgeView rounds                                  */
     FK_PROF_EP = 10,      /* entry-point beam search + centroid                 */
     FK_PROF_CONN = 11,    /* BFS connectivity                                   */
+    FK_PROF_ORDER = 12,   /* locality order (label hops + key sort)             */
     FK_PROF_N = 16
 };
 

@@ -108,6 +108,7 @@ void fk_close(fk_handle *h) {
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     h->ds.rows.release();
+    h->ds.chains.release();
     h->counters.release();
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -124,6 +125,7 @@ fk_status fk_dataset_upload(fk_handle *h, const float *rows, size_t row_stride_b
         select_device(h);
         Dataset &ds = h->ds;
         ds.rows.release();
+        ds.chains.release();
         ds.n_total = 0;
         const uint32_t dpad = (d + 7u) & ~7u;
         ds.rows.alloc((size_t)n_total * dpad);
@@ -132,6 +134,8 @@ fk_status fk_dataset_upload(fk_handle *h, const float *rows, size_t row_stride_b
         ds.d = d;
         ds.dpad = dpad;
         ds.stride = dpad;
+        build_chains(h->stream, ds);
+        FK_HIP(hipStreamSynchronize(h->stream));
     });
 }
 

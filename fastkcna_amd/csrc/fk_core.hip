@@ -95,34 +95,19 @@ void launch_join_distances(hipStream_t s, const Dataset &ds, const uint32_t *ids
 #define FK_PRUNE_WPB 4
 #define FK_PRUNE_RMAX 256
 
-// DP > 0: rows of exactly DP floats, each lane's candidate row held in
-// registers for all of its occlusion checks; DP == 0: any width, read from
-// memory per check.
-template <int DP>
-struct CandRow {
-    float4 v[DP / 4];
-    __device__ __forceinline__ void load(const float *p) {
-#pragma unroll
-        for (int k = 0; k < DP / 4; ++k) v[k] = reinterpret_cast<const float4 *>(p)[k];
-    }
-    __device__ __forceinline__ float dist(const float *m, const float *, uint32_t) const {
-        return l2_canon_reg<DP>(v, m);
-    }
-};
-template <>
-struct CandRow<0> {
-    __device__ __forceinline__ void load(const float *) {}
-    __device__ __forceinline__ float dist(const float *m, const float *p, uint32_t dpad) const {
-        return l2_canon(m, p, dpad);
-    }
-};
-
-template <int RULE, int DP>
-__global__ __launch_bounds__(256) void k_prune(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
-                                               PruneLaunch a) {
+// One wavefront per pool; candidates go in batches of 8, one per octet
+// (8 lanes), each octet holding its candidate's chains in registers and
+// computing each occlusion distance as one octet distance.  Within a batch
+// the accepts are resolved in candidate order with a ballot: the first
+// octet that no kept neighbour occludes is accepted, the later ones are
+// checked against it, and so on, which is the reference's sequential loop.
+template <int RULE, int NB>
+__global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains, uint32_t dpad, PruneLaunch a) {
     __shared__ uint32_t s_oid[FK_PRUNE_WPB][FK_PRUNE_RMAX];
     __shared__ float s_od[FK_PRUNE_WPB][FK_PRUNE_RMAX];
     const uint32_t lane = lane_id();
+    const uint32_t g = lane >> 3, k = lane & 7u;
+    const uint32_t nb = NB ? (uint32_t)NB : (dpad >> 3);
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t vb = a.xcd_order ? xcd_position((gridDim.x + FK_NUM_XCD - 1) / FK_NUM_XCD) : blockIdx.x;
     const uint32_t task = vb * FK_PRUNE_WPB + w;
@@ -136,6 +121,7 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ rows, u
     uint32_t *olen = a.out_len + slot;
     uint32_t *so = s_oid[w];
     float *sd = s_od[w];
+    const float *lane_chains = chains + k * nb;
 
     uint32_t start = 0;
     if (len > 0 && key_id(c[0]) == q) start = 1;
@@ -153,59 +139,59 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ rows, u
     unsigned long long cc = 0, exam = 0;
     bool stop = nout >= a.R;
     while (!stop && pos < limit) {
-        const uint32_t nb = (limit - pos) < 64u ? (limit - pos) : 64u;
-        const bool valid = lane < nb;
-        const uint64_t key = valid ? c[pos + lane] : FK_KEY_EMPTY;
+        const uint32_t nbat = (limit - pos) < 8u ? (limit - pos) : 8u;
+        const bool valid = g < nbat;
+        const uint64_t key = valid ? c[pos + g] : FK_KEY_EMPTY;
         const uint32_t pid = key_id(key);
         const float pd = key_dist(key);
         const bool skip = !valid || pid >= a.oracle_n;
-        const float *prow = rows + (size_t)(skip ? 0u : pid) * stride;
-        CandRow<DP> crow;
-        crow.load(prow);
+        OctChain<NB> crow;
+        crow.load(lane_chains + (size_t)(skip ? 0u : pid) * dpad, nb);
         bool occl = false;
         uint32_t mycc = 0;
         __builtin_amdgcn_wave_barrier();
         if (!skip) {
+            // octet-uniform: every lane of an octet sees the same pid, tid, djk
             for (uint32_t t = 0; t < nout; ++t) {
                 const uint32_t tid = so[t];
                 if (tid == pid) { occl = true; break; }
                 ++mycc;
-                const float djk = crow.dist(rows + (size_t)tid * stride, prow, dpad);
+                const float djk = crow.dist(lane_chains + (size_t)tid * dpad, nb);
                 if (occludes<RULE>(djk, pd, sd[t], a.tau3, a.cos_thr)) { occl = true; break; }
             }
         }
         uint32_t done = 0;
-        uint32_t stop_lane = 63;
+        uint32_t stop_g = 7;
         while (true) {
-            const uint64_t m = wave_ballot(!skip && !occl && lane >= done);
+            const uint64_t m = wave_ballot(!skip && !occl && g >= done);
             if (m == 0) break;
-            const uint32_t al = (uint32_t)__builtin_ctzll(m);
-            const uint32_t aid = __shfl(pid, al, 64);
-            const float ad = __shfl(pd, al, 64);
+            const uint32_t al = (uint32_t)__builtin_ctzll(m) >> 3;
+            const uint32_t aid = __shfl(pid, (int)(al * 8), 64);
+            const float ad = __shfl(pd, (int)(al * 8), 64);
             if (lane == 0) { so[nout] = aid; sd[nout] = ad; }
             ++nout;
             done = al + 1;
-            if (nout >= a.R) { stop = true; stop_lane = al; break; }
-            if (!skip && !occl && lane > al) {
+            if (nout >= a.R) { stop = true; stop_g = al; break; }
+            if (!skip && !occl && g > al) {
                 if (aid == pid) {
                     occl = true;
                 } else {
                     ++mycc;
-                    const float djk = crow.dist(rows + (size_t)aid * stride, prow, dpad);
+                    const float djk = crow.dist(lane_chains + (size_t)aid * dpad, nb);
                     if (occludes<RULE>(djk, pd, ad, a.tau3, a.cos_thr)) occl = true;
                 }
             }
         }
-        if (lane <= stop_lane) {
+        if (k == 0 && g <= stop_g) {
             cc += mycc;
             exam += skip ? 0u : 1u;
         }
-        pos += nb;
+        pos += nbat;
     }
     __builtin_amdgcn_wave_barrier();
-    for (uint32_t k = lane; k < nout; k += 64) {
-        oid[k] = so[k];
-        if (od) od[k] = sd[k];
+    for (uint32_t t = lane; t < nout; t += 64) {
+        oid[t] = so[t];
+        if (od) od[t] = sd[t];
     }
     if (lane == 0) *olen = nout;
     if (a.cc) {
@@ -221,13 +207,38 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ rows, u
 template <int RULE>
 static void launch_prune_rule(hipStream_t s, const Dataset &ds, const PruneLaunch &a, unsigned grid) {
     const dim3 blk(64 * FK_PRUNE_WPB);
+    const float *ch = ds.chains.p;
     switch (ds.dpad) {
-        case 32: hipLaunchKernelGGL((k_prune<RULE, 32>), dim3(grid), blk, 0, s, ds.rows.p, ds.stride, ds.dpad, a); break;
-        case 64: hipLaunchKernelGGL((k_prune<RULE, 64>), dim3(grid), blk, 0, s, ds.rows.p, ds.stride, ds.dpad, a); break;
-        case 96: hipLaunchKernelGGL((k_prune<RULE, 96>), dim3(grid), blk, 0, s, ds.rows.p, ds.stride, ds.dpad, a); break;
-        case 128: hipLaunchKernelGGL((k_prune<RULE, 128>), dim3(grid), blk, 0, s, ds.rows.p, ds.stride, ds.dpad, a); break;
-        default: hipLaunchKernelGGL((k_prune<RULE, 0>), dim3(grid), blk, 0, s, ds.rows.p, ds.stride, ds.dpad, a); break;
+        case 32: hipLaunchKernelGGL((k_prune<RULE, 4>), dim3(grid), blk, 0, s, ch, ds.dpad, a); break;
+        case 64: hipLaunchKernelGGL((k_prune<RULE, 8>), dim3(grid), blk, 0, s, ch, ds.dpad, a); break;
+        case 96: hipLaunchKernelGGL((k_prune<RULE, 12>), dim3(grid), blk, 0, s, ch, ds.dpad, a); break;
+        case 128: hipLaunchKernelGGL((k_prune<RULE, 16>), dim3(grid), blk, 0, s, ch, ds.dpad, a); break;
+        case 256: hipLaunchKernelGGL((k_prune<RULE, 32>), dim3(grid), blk, 0, s, ch, ds.dpad, a); break;
+        default: hipLaunchKernelGGL((k_prune<RULE, 0>), dim3(grid), blk, 0, s, ch, ds.dpad, a); break;
     }
+}
+
+// Chain-major copy of the rows (layout of OctChain).
+__global__ void k_chain_layout(const float *__restrict__ rows, uint32_t stride, uint64_t n, uint32_t dpad,
+                               float *__restrict__ chains) {
+    const uint32_t nb = dpad >> 3;
+    const uint64_t total = n * dpad;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = e / dpad;
+        const uint32_t x = (uint32_t)(e - i * dpad);  // position in the chain-major row
+        const uint32_t kk = x / nb, j = x - kk * nb;
+        chains[e] = rows[i * stride + chain_block(j, dpad) + kk];
+    }
+}
+
+void build_chains(hipStream_t s, Dataset &ds) {
+    ds.chains.alloc((size_t)ds.n_total * ds.dpad);
+    const uint64_t total = (uint64_t)ds.n_total * ds.dpad;
+    const unsigned grid = (unsigned)std::min<uint64_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_chain_layout, dim3(grid), dim3(256), 0, s, ds.rows.p, ds.stride, (uint64_t)ds.n_total,
+                       ds.dpad, ds.chains.p);
+    FK_LAUNCH_CHECK();
 }
 
 void launch_prune(hipStream_t s, const Dataset &ds, const PruneLaunch &a) {

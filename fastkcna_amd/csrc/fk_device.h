@@ -115,6 +115,71 @@ __device__ __forceinline__ float l2_canon_reg(const float4 (&a)[DP / 4], const f
     return r;
 }
 
+// Octet (8-lane) form of the canonical squared L2.
+// The 8 lane accumulators of float_l2sqr_avx are independent fma chains over
+// the 8-blocks in canonical order (the dpad % 32 tail blocks from the last
+// down, then the full blocks ascending); lane k of an octet runs chain k.
+// The chain-major copy of the dataset keeps, per row, each chain's
+// nb = dpad / 8 elements contiguous in that order:
+//     chains[i * dpad + k * nb + j] = row_i[B_j + k],  B_j = j-th block.
+// An octet then reads a row as 8 coalesced runs, and the result, summed
+// lane 0 to lane 7 left to right, is bit-identical to l2_canon.
+__device__ __forceinline__ uint32_t chain_block(uint32_t j, uint32_t dpad) {
+    const uint32_t dr = dpad & 31u, dd = dpad - dr, ntail = dr >> 3;
+    return j < ntail ? dd + 8u * (ntail - 1u - j) : 8u * (j - ntail);
+}
+
+// Ordered sum of the 8 accumulators of this lane's octet, in every lane of
+// the octet.  The 8 lanes must be active together.
+__device__ __forceinline__ float octet_sum(float acc) {
+    const int base = (int)(threadIdx.x & 63u & ~7u);
+    float r = __shfl(acc, base, 64) + __shfl(acc, base + 1, 64);
+    r += __shfl(acc, base + 2, 64);
+    r += __shfl(acc, base + 3, 64);
+    r += __shfl(acc, base + 4, 64);
+    r += __shfl(acc, base + 5, 64);
+    r += __shfl(acc, base + 6, 64);
+    r += __shfl(acc, base + 7, 64);
+    return r;
+}
+
+// One lane's chain of one row, in registers (NB = dpad / 8, a multiple of
+// 4) or read from memory per distance (NB == 0).
+template <int NB>
+struct OctChain {
+    float4 v[NB / 4];
+    __device__ __forceinline__ void load(const float *__restrict__ c, uint32_t) {
+#pragma unroll
+        for (int q = 0; q < NB / 4; ++q) v[q] = reinterpret_cast<const float4 *>(c)[q];
+    }
+    // distance to the row whose chain (same lane) starts at o
+    __device__ __forceinline__ float dist(const float *__restrict__ o, uint32_t) const {
+        float acc = 0.f, t;
+#pragma unroll
+        for (int q = 0; q < NB / 4; ++q) {
+            const float4 b = reinterpret_cast<const float4 *>(o)[q];
+            t = v[q].x - b.x; acc = __builtin_fmaf(t, t, acc);
+            t = v[q].y - b.y; acc = __builtin_fmaf(t, t, acc);
+            t = v[q].z - b.z; acc = __builtin_fmaf(t, t, acc);
+            t = v[q].w - b.w; acc = __builtin_fmaf(t, t, acc);
+        }
+        return octet_sum(acc);
+    }
+};
+template <>
+struct OctChain<0> {
+    const float *p;
+    __device__ __forceinline__ void load(const float *__restrict__ c, uint32_t) { p = c; }
+    __device__ __forceinline__ float dist(const float *__restrict__ o, uint32_t nb) const {
+        float acc = 0.f;
+        for (uint32_t j = 0; j < nb; ++j) {
+            const float t = p[j] - o[j];
+            acc = __builtin_fmaf(t, t, acc);
+        }
+        return octet_sum(acc);
+    }
+};
+
 // Occlusion rules of the three pruner families; dqt is the kept neighbour's
 // distance to the pool owner, djk the kept neighbour to candidate distance,
 // pd the candidate's distance to the owner.

@@ -93,6 +93,7 @@ struct PinnedBuf {
 
 struct Dataset {
     DevBuf<float> rows;   // n_total x stride floats, zero padded
+    DevBuf<float> chains; // n_total x dpad floats, chain-major copy (octet distances)
     uint32_t n_total = 0;
     uint32_t d = 0;
     uint32_t dpad = 0;    // d rounded up to a multiple of 8
@@ -139,6 +140,7 @@ struct PruneLaunch {
     unsigned long long *examined = nullptr;  // accumulated examined candidates
 };
 void launch_prune(hipStream_t s, const Dataset &ds, const PruneLaunch &a);
+void build_chains(hipStream_t s, Dataset &ds);  // fills ds.chains from ds.rows
 
 void launch_join_distances(hipStream_t s, const Dataset &ds, const uint32_t *ids,
                            const uint64_t *list_off, const uint32_t *n_new, const uint64_t *pair_off,

@@ -351,44 +351,26 @@ __global__ __launch_bounds__(256) void k_join_fill(const uint32_t *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Bucket merge, one workgroup per target node.  Candidates are sorted by
-// (key, source join); the first of equal keys is the one kept.  A kept
-// candidate that ranks before the iteration-start K-th entry marks its source
-// join `found` (src/kgraph.cpp:1111-1114: an insert at position < K).
+// Bucket merge, one workgroup per target node.  Of equal candidate keys the
+// one of the smallest source join is the one kept.  A kept candidate that
+// ranks before the iteration-start K-th entry marks its source join `found`
+// (src/kgraph.cpp:1111-1114: an insert at position < K).
 // ---------------------------------------------------------------------------
-struct MergeShared {
-    uint64_t p[FK_POOL_MAX];
-    uint64_t o[FK_POOL_MAX > FK_MRG_CH ? FK_POOL_MAX : FK_MRG_CH];
-    uint64_t c[FK_MRG_CH];
-    uint64_t c2[FK_MRG_CH];
-    uint32_t keep[FK_MRG_CH];
-    uint32_t part[256];
-    uint8_t pf[FK_POOL_MAX];
-    uint8_t of[FK_POOL_MAX];
-    uint32_t np;
-};
 
-__device__ __forceinline__ bool key_src_less(uint64_t ka, uint32_t sa, uint64_t kb, uint32_t sb) {
-    return ka < kb || (ka == kb && sa < sb);
-}
+// Merge v2: the chunk's candidates are de-duplicated by id in an LDS hash
+// set before sorting (equal ids carry equal keys: the distance to the
+// target is symmetric and computed in one canonical order), keeping the
+// smallest source join per id; ids already in the pool, and keys that could
+// neither be inserted nor mark a join, are dropped first.  Only the distinct
+// new candidates are sorted and rank-merged.  Same result as sorting every
+// candidate by (key, source) and keeping the first of equal keys.
+#define FK_MRG_HASH (2 * FK_MRG_CH)
 
-__device__ void block_sort_key_src(uint64_t *k, uint32_t *v, uint32_t n) {
-    for (uint32_t s = 2; s <= n; s <<= 1) {
-        for (uint32_t j = s >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-                const uint32_t ixj = i ^ j;
-                if (ixj > i) {
-                    const bool up = (i & s) == 0;
-                    const bool gt = key_src_less(k[ixj], v[ixj], k[i], v[i]);
-                    if (gt == up) {
-                        const uint64_t tk = k[i]; k[i] = k[ixj]; k[ixj] = tk;
-                        const uint32_t tv = v[i]; v[i] = v[ixj]; v[ixj] = tv;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
+__device__ __forceinline__ uint32_t mrg_hash(uint32_t id) { return (id * 2654435761u) >> 21; }  // 11 bits
+static_assert(FK_MRG_HASH == 2048, "mrg_hash yields 11 bits");
+
+size_t nnd_merge_lds(uint32_t cap) {
+    return (size_t)cap * 16 + (size_t)FK_MRG_CH * 8 + (size_t)FK_MRG_HASH * 8 + (size_t)cap * 2;
 }
 
 __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, uint32_t K, uint64_t *__restrict__ pool,
@@ -397,45 +379,90 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
                                                    const uint64_t *__restrict__ boff,
                                                    const uint64_t *__restrict__ bkey,
                                                    const uint32_t *__restrict__ bsrc, uint8_t *__restrict__ found) {
-    __shared__ MergeShared sh;
-    __shared__ uint32_t s_src[FK_MRG_CH];
+    extern __shared__ __align__(16) uint8_t smem[];
+    uint64_t *s_p = reinterpret_cast<uint64_t *>(smem);
+    uint64_t *s_o = s_p + cap;
+    uint64_t *s_u = s_o + cap;
+    uint32_t *s_tid = reinterpret_cast<uint32_t *>(s_u + FK_MRG_CH);
+    uint32_t *s_tsrc = s_tid + FK_MRG_HASH;
+    uint8_t *s_pf = reinterpret_cast<uint8_t *>(s_tsrc + FK_MRG_HASH);
+    uint8_t *s_of = s_pf + cap;
+    __shared__ uint32_t s_np, s_nu;
     const uint32_t i = blockIdx.x;
     const uint32_t nb = bcnt[i];
     if (nb == 0) return;
     uint64_t *pr = pool + (uint64_t)i * cap;
     uint8_t *pf = pflg + (uint64_t)i * cap;
     const uint32_t np0 = cnt[i];
-    for (uint32_t j = threadIdx.x; j < np0; j += blockDim.x) { sh.p[j] = pr[j]; sh.pf[j] = pf[j]; }
-    if (threadIdx.x == 0) sh.np = np0;
+    for (uint32_t j = threadIdx.x; j < np0; j += blockDim.x) { s_p[j] = pr[j]; s_pf[j] = pf[j]; }
+    if (threadIdx.x == 0) s_np = np0;
     __syncthreads();
-    const uint64_t kth = np0 >= K ? sh.p[K - 1] : FK_KEY_EMPTY;
+    const uint64_t kth = np0 >= K ? s_p[K - 1] : FK_KEY_EMPTY;
     const uint64_t base = boff[i];
     for (uint32_t c0 = 0; c0 < nb; c0 += FK_MRG_CH) {
         const uint32_t m = (nb - c0) < FK_MRG_CH ? (nb - c0) : FK_MRG_CH;
-        const uint32_t m2 = next_pow2(m < 2 ? 2 : m);
-        for (uint32_t k = threadIdx.x; k < m2; k += blockDim.x) {
-            sh.c[k] = k < m ? bkey[base + c0 + k] : FK_KEY_EMPTY;
-            s_src[k] = k < m ? bsrc[base + c0 + k] : 0xFFFFFFFFu;
+        for (uint32_t j = threadIdx.x; j < FK_MRG_HASH; j += blockDim.x) {
+            s_tid[j] = 0xFFFFFFFFu;
+            s_tsrc[j] = 0xFFFFFFFFu;
         }
+        if (threadIdx.x == 0) s_nu = 0;
         __syncthreads();
-        block_sort_key_src(sh.c, s_src, m2);
-        const uint32_t np = sh.np;
+        const uint32_t np = s_np;
+        const uint64_t lth = np >= Lp ? s_p[Lp - 1] : FK_KEY_EMPTY;
         for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
-            const uint64_t key = sh.c[k];
-            const bool first = (k == 0) || (sh.c[k - 1] != key);
-            const bool keep = first && !sorted_contains(sh.p, np, key);
-            sh.keep[k] = keep ? 1u : 0u;
-            if (keep && key < kth) found[s_src[k]] = 1;
+            const uint64_t key = bkey[base + c0 + k];
+            if (key >= kth && key >= lth) continue;
+            if (sorted_contains(s_p, np, key)) continue;
+            const uint32_t id = key_id(key);
+            uint32_t h = mrg_hash(id);
+            while (true) {
+                const uint32_t old = atomicCAS(&s_tid[h], 0xFFFFFFFFu, id);
+                if (old == 0xFFFFFFFFu) {
+                    s_u[atomicAdd(&s_nu, 1u)] = key;
+                    break;
+                }
+                if (old == id) break;
+                h = (h + 1) & (FK_MRG_HASH - 1);
+            }
+            atomicMin(&s_tsrc[h], bsrc[base + c0 + k]);
         }
         __syncthreads();
-        block_merge(sh.p, sh.pf, &sh.np, sh.c, sh.keep, m, Lp, (uint8_t)(FK_FLG_NEW | FK_FLG_ISNEW), sh.c2, sh.o,
-                    sh.of, sh.part);
+        const uint32_t nu = s_nu;
+        if (nu > 0) {
+            const uint32_t m2 = next_pow2(nu < 2 ? 2 : nu);
+            for (uint32_t k = nu + threadIdx.x; k < m2; k += blockDim.x) s_u[k] = FK_KEY_EMPTY;
+            __syncthreads();
+            block_bitonic_sort(s_u, m2);
+            for (uint32_t k = threadIdx.x; k < nu; k += blockDim.x) {
+                const uint64_t key = s_u[k];
+                if (key < kth) {
+                    const uint32_t id = key_id(key);
+                    uint32_t h = mrg_hash(id);
+                    while (s_tid[h] != id) h = (h + 1) & (FK_MRG_HASH - 1);
+                    found[s_tsrc[h]] = 1;
+                }
+            }
+            // rank merge; no key is in both lists
+            for (uint32_t j = threadIdx.x; j < np; j += blockDim.x) {
+                const uint32_t pos = j + lower_bound_cnt(s_u, nu, s_p[j]);
+                if (pos < Lp) { s_o[pos] = s_p[j]; s_of[pos] = s_pf[j]; }
+            }
+            for (uint32_t k = threadIdx.x; k < nu; k += blockDim.x) {
+                const uint32_t pos = k + lower_bound_cnt(s_p, np, s_u[k]);
+                if (pos < Lp) { s_o[pos] = s_u[k]; s_of[pos] = (uint8_t)(FK_FLG_NEW | FK_FLG_ISNEW); }
+            }
+            __syncthreads();
+            const uint32_t nn = (np + nu) < Lp ? (np + nu) : Lp;
+            for (uint32_t j = threadIdx.x; j < nn; j += blockDim.x) { s_p[j] = s_o[j]; s_pf[j] = s_of[j]; }
+            if (threadIdx.x == 0) s_np = nn;
+        }
+        __syncthreads();
     }
-    const uint32_t np = sh.np;
-    for (uint32_t j = threadIdx.x; j < np; j += blockDim.x) { pr[j] = sh.p[j]; pf[j] = sh.pf[j]; }
+    const uint32_t np = s_np;
+    for (uint32_t j = threadIdx.x; j < np; j += blockDim.x) { pr[j] = s_p[j]; pf[j] = s_pf[j]; }
     if (threadIdx.x == 0) {
         cnt[i] = np;
-        thr[i] = np >= Lp ? key_dist(sh.p[Lp - 1]) : INFINITY;
+        thr[i] = np >= Lp ? key_dist(s_p[Lp - 1]) : INFINITY;
     }
 }
 
@@ -784,7 +811,8 @@ static void nnd_join(BuildState &b) {
     // pair distances and list ids read, (key, source) bucket entries written
     b.prof_end(pe, FK_PROF_JOIN_FILL, 4.0 * (double)pairs + 4.0 * (double)nlist + 12.0 * (double)cands);
     pe = b.prof_begin();
-    hipLaunchKernelGGL(k_nnd_merge, dim3(N), dim3(256), 0, b.s, b.cap, b.P.L, b.P.K, b.pool.p, b.pflg.p, b.cnt.p,
+    hipLaunchKernelGGL(k_nnd_merge, dim3(N), dim3(256), nnd_merge_lds(b.cap), b.s, b.cap, b.P.L, b.P.K, b.pool.p,
+                       b.pflg.p, b.cnt.p,
                        b.thr.p, b.bcnt.p, b.boff.p, b.bkey.p, b.bsrc.p, b.found.p);
     FK_LAUNCH_CHECK();
     // bucket entries read once; each pool read and rewritten (8-byte key +
@@ -936,20 +964,40 @@ __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ p
     }
 }
 
-#define FK_BV_HASH 4096
+#define FK_BV_HASH 2048
 #define FK_BV_HASH_FILL (FK_BV_HASH * 3 / 4)
+#define FK_BV_CH 512     // candidates gathered between two pool merges
+#define FK_BV_THREADS 256
 
-struct BvShared {
-    alignas(16) float row[1024];      // the owner's row (float4 reads)
-    MergeShared m;
-    uint32_t sel[FK_POOL_MAX];
-    uint32_t sdeg[FK_POOL_MAX + 1];  // exclusive prefix of the selected out-degrees
-    uint8_t sel_new[FK_POOL_MAX];
-    uint32_t table[FK_BV_HASH];       // ids already in the pool or seen this round
-    uint32_t nsel;
-    uint32_t nc;
-    uint32_t fill;
-    float thr;
+// Dynamic LDS of a BridgeView round, sized by the pool capacity:
+//   u64 p[cap], o[cap], c[CH], c2[CH]
+//   u32 keep[kc], part[256], sel[cap], sdeg[cap + 1], todo[256], table[HASH]
+//   u8  pf[cap], of[kc], sel_new[cap]           (kc = max(cap, CH))
+struct BvLds {
+    uint64_t *p, *o, *c, *c2;
+    uint32_t *keep, *part, *sel, *sdeg, *todo, *table;
+    uint8_t *pf, *of, *sel_new;
+    __device__ __host__ static size_t bytes(uint32_t cap) {
+        const size_t kc = cap > FK_BV_CH ? cap : FK_BV_CH;
+        return 8 * (2 * (size_t)cap + 2 * FK_BV_CH) +
+               4 * (kc + FK_BV_THREADS + cap + cap + 1 + FK_BV_THREADS + FK_BV_HASH) + (cap + kc + cap);
+    }
+    __device__ void carve(uint8_t *base, uint32_t cap) {
+        const uint32_t kc = cap > FK_BV_CH ? cap : FK_BV_CH;
+        p = reinterpret_cast<uint64_t *>(base);
+        o = p + cap;
+        c = o + cap;
+        c2 = c + FK_BV_CH;
+        keep = reinterpret_cast<uint32_t *>(c2 + FK_BV_CH);
+        part = keep + kc;
+        sel = part + FK_BV_THREADS;
+        sdeg = sel + cap;
+        todo = sdeg + cap + 1;
+        table = todo + FK_BV_THREADS;
+        pf = reinterpret_cast<uint8_t *>(table + FK_BV_HASH);
+        of = pf + cap;
+        sel_new = of + kc;
+    }
 };
 
 // Inserts id into the round's id set; true when it was not there.  When the
@@ -970,66 +1018,79 @@ __device__ __forceinline__ bool bv_set_insert(uint32_t *table, uint32_t *fill, u
     return true;
 }
 
-__global__ __launch_bounds__(256) void k_bv_round(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
-                                                  uint32_t N, uint32_t cap, uint32_t W, uint32_t S_cap,
-                                                  uint64_t *__restrict__ pool, uint8_t *__restrict__ pflg,
-                                                  uint32_t *__restrict__ cnt, const uint32_t *__restrict__ nsg_id,
-                                                  const uint32_t *__restrict__ nsg_len,
-                                                  const uint8_t *__restrict__ nsg_new, uint32_t ncap,
-                                                  unsigned long long *__restrict__ cc_acc,
-                                                  const uint32_t *__restrict__ order) {
-    __shared__ BvShared sh;
+template <int NB>
+__global__ __launch_bounds__(FK_BV_THREADS) void k_bv_round(const float *__restrict__ chains, uint32_t dpad,
+                                                            uint32_t N, uint32_t cap, uint32_t W, uint32_t S_cap,
+                                                            uint64_t *__restrict__ pool, uint8_t *__restrict__ pflg,
+                                                            uint32_t *__restrict__ cnt,
+                                                            const uint32_t *__restrict__ nsg_id,
+                                                            const uint32_t *__restrict__ nsg_len,
+                                                            const uint8_t *__restrict__ nsg_new, uint32_t ncap,
+                                                            unsigned long long *__restrict__ cc_acc,
+                                                            const uint32_t *__restrict__ order) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    BvLds sh;
+    sh.carve(smem, cap);
+    __shared__ uint32_t s_np, s_nsel, s_nc, s_fill, s_ntodo;
+    __shared__ float s_thr;
     const uint32_t i = node_of_block(order, N);
     if (i >= N) return;
     uint64_t *pr = pool + (uint64_t)i * cap;
     uint8_t *pf = pflg + (uint64_t)i * cap;
     const uint32_t np0 = cnt[i];
     const uint32_t lim = np0 < W ? np0 : W;
-    for (uint32_t j = threadIdx.x; j < np0; j += blockDim.x) { sh.m.p[j] = pr[j]; sh.m.pf[j] = pf[j]; }
+    for (uint32_t j = threadIdx.x; j < np0; j += blockDim.x) { sh.p[j] = pr[j]; sh.pf[j] = pf[j]; }
     for (uint32_t j = threadIdx.x; j < FK_BV_HASH; j += blockDim.x) sh.table[j] = 0xFFFFFFFFu;
     __syncthreads();
     // select flagged window entries (first S_cap in pool order), clear them
-    for (uint32_t j = threadIdx.x; j < lim; j += blockDim.x) sh.m.keep[j] = (sh.m.pf[j] & FK_FLG_NEW) ? 1u : 0u;
+    for (uint32_t j = threadIdx.x; j < lim; j += blockDim.x) sh.keep[j] = (sh.pf[j] & FK_FLG_NEW) ? 1u : 0u;
     __syncthreads();
-    const uint32_t nflag = block_scan_inplace(sh.m.keep, lim, sh.m.part);
+    const uint32_t nflag = block_scan_inplace(sh.keep, lim, sh.part);
     const uint32_t ns = nflag < S_cap ? nflag : S_cap;
     for (uint32_t j = threadIdx.x; j < lim; j += blockDim.x) {
-        const uint8_t f = sh.m.pf[j];
-        const uint32_t r = sh.m.keep[j];
+        const uint8_t f = sh.pf[j];
+        const uint32_t r = sh.keep[j];
         if ((f & FK_FLG_NEW) && r < ns) {
-            const uint32_t q = key_id(sh.m.p[j]);
+            const uint32_t q = key_id(sh.p[j]);
             sh.sel[r] = q;
             sh.sel_new[r] = (f & FK_FLG_ISNEW) ? 1 : 0;
             sh.sdeg[r] = nsg_len[q];
-            sh.m.pf[j] = f & (uint8_t)~FK_FLG_NEW;
+            sh.pf[j] = f & (uint8_t)~FK_FLG_NEW;
         }
     }
     __syncthreads();
-    const uint32_t tot = block_scan_inplace(sh.sdeg, ns, sh.m.part);
+    const uint32_t tot = block_scan_inplace(sh.sdeg, ns, sh.part);
     if (threadIdx.x == 0) {
         sh.sdeg[ns] = tot;
-        sh.nsel = ns;
-        sh.m.np = np0;
-        sh.nc = 0;
-        sh.fill = 0;
-        sh.thr = np0 >= cap ? key_dist(sh.m.p[cap - 1]) : INFINITY;
+        s_nsel = ns;
+        s_np = np0;
+        s_nc = 0;
+        s_fill = 0;
+        s_ntodo = 0;
+        s_thr = np0 >= cap ? key_dist(sh.p[cap - 1]) : INFINITY;
     }
-    for (uint32_t k = threadIdx.x; k < (dpad >> 2); k += blockDim.x)
-        reinterpret_cast<float4 *>(sh.row)[k] = reinterpret_cast<const float4 *>(rows + (size_t)i * stride)[k];
     __syncthreads();
-    const uint32_t nsel = sh.nsel;
-    if (nsel == 0) return;
+    const uint32_t nsel = s_nsel;
+    if (nsel == 0) return;  // nothing selected, nothing changed
     // ids already in the pool need no distance
-    for (uint32_t j = threadIdx.x; j < np0; j += blockDim.x) bv_set_insert(sh.table, &sh.fill, key_id(sh.m.p[j]));
-    bv_set_insert(sh.table, &sh.fill, i);
+    for (uint32_t j = threadIdx.x; j < np0; j += blockDim.x) bv_set_insert(sh.table, &s_fill, key_id(sh.p[j]));
+    bv_set_insert(sh.table, &s_fill, i);
     __syncthreads();
-    const float thr = sh.thr;
+    const float thr = s_thr;
     const uint32_t total = sh.sdeg[nsel];
+    // distances: one octet per fresh id, lane k holding chain k of the
+    // owner's row
+    const uint32_t oc = threadIdx.x >> 3, k = threadIdx.x & 7u;
+    const uint32_t nb = NB ? (uint32_t)NB : (dpad >> 3);
+    const float *lane_chains = chains + k * nb;
+    OctChain<NB> mine;
+    mine.load(lane_chains + (size_t)i * dpad, nb);
     unsigned long long cc = 0;
     for (uint32_t f0 = 0; f0 < total; f0 += blockDim.x) {
+        // one thread per (entry, neighbour) pair: entry s with
+        // sdeg[s] <= f < sdeg[s + 1]
         const uint32_t f = f0 + threadIdx.x;
         if (f < total) {
-            // entry s with sdeg[s] <= f < sdeg[s + 1]
             uint32_t lo = 0, hi = nsel;
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
@@ -1041,41 +1102,65 @@ __global__ __launch_bounds__(256) void k_bv_round(const float *__restrict__ rows
             const bool tnew = nsg_new[(uint64_t)q * ncap + m] != 0;
             if (t != i && (sh.sel_new[lo] || tnew)) {
                 ++cc;  // the reference evaluates every such pair (search_cc)
-                if (bv_set_insert(sh.table, &sh.fill, t)) {
-                    const float dd = l2_canon(sh.row, rows + (size_t)t * stride, dpad);
-                    if (dd <= thr) sh.m.c[atomicAdd(&sh.nc, 1u)] = make_key(dd, t);
-                }
+                if (bv_set_insert(sh.table, &s_fill, t)) sh.todo[atomicAdd(&s_ntodo, 1u)] = t;
             }
         }
         __syncthreads();
+        const uint32_t nt = s_ntodo;
+        for (uint32_t e = oc; e < nt; e += (FK_BV_THREADS >> 3)) {  // octet-uniform
+            const uint32_t t = sh.todo[e];
+            const float dd = mine.dist(lane_chains + (size_t)t * dpad, nb);
+            if (k == 0 && dd <= thr) sh.c[atomicAdd(&s_nc, 1u)] = make_key(dd, t);
+        }
+        __syncthreads();
         const bool last = f0 + blockDim.x >= total;
-        const uint32_t nc = sh.nc;
+        const uint32_t nc = s_nc;
+        if (threadIdx.x == 0) s_ntodo = 0;  // every thread has read it
         __syncthreads();  // every thread has read nc before it may change
-        if (last || nc + blockDim.x > FK_MRG_CH) {
+        if (last || nc + blockDim.x > FK_BV_CH) {
             if (nc > 0) {
                 const uint32_t m2 = next_pow2(nc < 2 ? 2 : nc);
-                for (uint32_t k = nc + threadIdx.x; k < m2; k += blockDim.x) sh.m.c[k] = FK_KEY_EMPTY;
+                for (uint32_t x = nc + threadIdx.x; x < m2; x += blockDim.x) sh.c[x] = FK_KEY_EMPTY;
                 __syncthreads();
-                block_bitonic_sort(sh.m.c, m2);
-                const uint32_t npp = sh.m.np;
-                for (uint32_t k = threadIdx.x; k < nc; k += blockDim.x) {
-                    const uint64_t key2 = sh.m.c[k];
-                    const bool first = (k == 0) || (sh.m.c[k - 1] != key2);
-                    sh.m.keep[k] = (first && !sorted_contains(sh.m.p, npp, key2)) ? 1u : 0u;
+                block_bitonic_sort(sh.c, m2);
+                const uint32_t npp = s_np;
+                for (uint32_t x = threadIdx.x; x < nc; x += blockDim.x) {
+                    const uint64_t key2 = sh.c[x];
+                    const bool first = (x == 0) || (sh.c[x - 1] != key2);
+                    sh.keep[x] = (first && !sorted_contains(sh.p, npp, key2)) ? 1u : 0u;
                 }
                 __syncthreads();
-                block_merge(sh.m.p, sh.m.pf, &sh.m.np, sh.m.c, sh.m.keep, nc, cap,
-                            (uint8_t)(FK_FLG_NEW | FK_FLG_ISNEW), sh.m.c2, sh.m.o, sh.m.of, sh.m.part);
+                block_merge(sh.p, sh.pf, &s_np, sh.c, sh.keep, nc, cap, (uint8_t)(FK_FLG_NEW | FK_FLG_ISNEW), sh.c2,
+                            sh.o, sh.of, sh.part);
             }
-            if (threadIdx.x == 0) sh.nc = 0;
+            if (threadIdx.x == 0) s_nc = 0;
             __syncthreads();
         }
     }
-    const uint32_t np = sh.m.np;
-    for (uint32_t j = threadIdx.x; j < np; j += blockDim.x) { pr[j] = sh.m.p[j]; pf[j] = sh.m.pf[j]; }
+    const uint32_t np = s_np;
+    for (uint32_t j = threadIdx.x; j < np; j += blockDim.x) { pr[j] = sh.p[j]; pf[j] = sh.pf[j]; }
     if (threadIdx.x == 0) cnt[i] = np;
     cc = block_sum(cc);
     if (threadIdx.x == 0 && cc) atomicAdd(cc_acc, cc);
+}
+
+static void launch_bv_round(BuildState &b, uint32_t W, uint32_t scap) {
+    const uint32_t *ord = b.has_order ? b.order.p : nullptr;
+    const size_t lds = BvLds::bytes(b.cap);
+#define FK_BV_LAUNCH(NBV)                                                                                       \
+    hipLaunchKernelGGL(k_bv_round<NBV>, dim3(node_grid(b)), dim3(FK_BV_THREADS), lds, b.s, b.ds->chains.p,     \
+                       b.ds->dpad, b.N,                                                                         \
+                       b.cap, W, scap, b.pool.p, b.pflg.p, b.cnt.p, b.nsg_id.p, b.nsg_len.p, b.nsg_new.p, b.ncap, \
+                       b.acc.p, ord)
+    switch (b.ds->dpad) {
+        case 32: FK_BV_LAUNCH(4); break;
+        case 64: FK_BV_LAUNCH(8); break;
+        case 96: FK_BV_LAUNCH(12); break;
+        case 128: FK_BV_LAUNCH(16); break;
+        case 256: FK_BV_LAUNCH(32); break;
+        default: FK_BV_LAUNCH(0); break;
+    }
+#undef FK_BV_LAUNCH
 }
 
 void bridgeview(BuildState &b) {
@@ -1116,9 +1201,7 @@ void bridgeview(BuildState &b) {
         }
         FK_HIP(hipMemsetAsync(b.acc.p, 0, 8, b.s));
         hipEvent_t pe = b.prof_begin();
-        hipLaunchKernelGGL(k_bv_round, dim3(node_grid(b)), dim3(256), 0, b.s, b.ds->rows.p, b.ds->stride,
-                           b.ds->dpad, N, b.cap, W, scap, b.pool.p, b.pflg.p, b.cnt.p, b.nsg_id.p, b.nsg_len.p,
-                           b.nsg_new.p, b.ncap, b.acc.p, b.has_order ? b.order.p : nullptr);
+        launch_bv_round(b, W, scap);
         FK_LAUNCH_CHECK();
         unsigned long long cc = 0;
         FK_HIP(hipMemcpyAsync(&cc, b.acc.p, 8, hipMemcpyDeviceToHost, b.s));

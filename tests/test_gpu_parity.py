@@ -30,10 +30,13 @@ def datasets():
         "float960": float_mixture(600, 960, 8, seed=12),
         "unit96": unit_mixture(3000, 96, 16, seed=13, sigma=0.25),
         "odd100": np.random.default_rng(14).normal(size=(1500, 100)).astype(np.float32),
+        # dpad % 32 of 24 and 16: the canonical order's tail blocks come first
+        "odd20": np.random.default_rng(15).normal(size=(1500, 20)).astype(np.float32),
+        "odd44": integer_mixture(2500, 44, 16, seed=16),
     }
 
 
-@pytest.mark.parametrize("name", ["int128", "float960", "unit96", "odd100"])
+@pytest.mark.parametrize("name", ["int128", "float960", "unit96", "odd100", "odd20", "odd44"])
 def test_pair_distances_bitexact(device, name):
     x = datasets()[name]
     say("pair", name, x.shape)
@@ -92,13 +95,13 @@ def make_pools(x, n_tasks, L, rng, q=None):
 
 @pytest.mark.parametrize("pg", [FK_PG_NSG, FK_PG_TAUMNG, FK_PG_ALPHAPG])
 @pytest.mark.parametrize("reversed_", [False, True])
-@pytest.mark.parametrize("name", ["int128", "unit96", "float960"])
+@pytest.mark.parametrize("name", ["int128", "unit96", "float960", "odd100", "odd20", "odd44"])
 def test_prune_bitexact(device, pg, reversed_, name):
     x = datasets()[name]
     say("prune", pg, reversed_, name)
     device.upload(x)
     rng = np.random.default_rng(3)
-    tau = {"int128": 300.0, "unit96": 0.05, "float960": 0.5}[name]
+    tau = {"int128": 300.0, "unit96": 0.05, "float960": 0.5, "odd100": 1.0, "odd20": 0.2, "odd44": 100.0}[name]
     p = IndexParams(nsg_R=24, search_K=90, tau=tau, angle=63)
     pools, plen, q = make_pools(x, 400, 120, rng)
     # a self entry at the front of some pools
@@ -215,3 +218,29 @@ def test_full_build_invariants_and_recall(device, pg):
     rc = np.mean([len(set(resc[i]) & set(gt[i])) / 10 for i in range(200)])
     say("query recall@10 gpu", rg, "oracle", rc)
     assert rg >= rc - 0.02
+
+
+@pytest.mark.parametrize("name", ["odd44", "odd20"])
+def test_full_build_odd_widths(device, name):
+    """Rows whose padded width is not a multiple of 32 (octet distances
+    with the canonical tail blocks, generic chain path)."""
+    x = datasets()[name]
+    say("build odd", name, x.shape)
+    device.upload(x)
+    n, d = x.shape
+    p = IndexParams(K=40, L=40, S=10, R=40, iterations=5, nsg_R=16, search_L=40, search_K=40)
+    off, ids, ep, width, st = device.build(FK_PG_NSG, p)
+    for i in range(n):
+        nb = ids[off[i]:off[i + 1]]
+        assert i not in nb and len(set(nb.tolist())) == len(nb)
+    assert reachable(off, ids, ep).all()
+    rng = np.random.default_rng(7)
+    qs = x[rng.choice(n, 100, replace=False)] + rng.normal(0, 0.05, size=(100, d)).astype(np.float32)
+    gt, _ = oracle.knn(x, qs, 10)
+    res, _ = device.search(off, ids, ep, qs, 40, 10)
+    rg = np.mean([len(set(res[i]) & set(gt[i])) / 10 for i in range(100)])
+    coff, cids, cep, _, _, _ = oracle.build(FK_PG_NSG, x, p, threads=8)
+    resc, _ = device.search(coff, cids, cep, qs, 40, 10)
+    rc = np.mean([len(set(resc[i]) & set(gt[i])) / 10 for i in range(100)])
+    say("query recall@10 gpu", rg, "oracle", rc)
+    assert rg >= rc - 0.03

@@ -41,6 +41,25 @@ def total_dists(st):
                                     "d_bv", "d_conn"))
 
 
+# kernel of each timed family (the family's dominant launch)
+FAMILY_KERNEL = {"bv": "k_bv_round", "prune": "k_prune", "join": "k_join_count", "join_fill": "k_join_fill",
+                 "merge": "k_nnd_merge", "controls": "k_topk_select"}
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "c2_nsg_build_hbm_traffic.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by
+    a device copy of known size) from the committed rocprofv3 counter summary
+    of this workload; None when the summary has no row for it."""
+    if kernel is None or not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    for line in open(TRAFFIC_SUMMARY):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0].strip("`").split("<")[0].endswith(kernel):
+            return float(cells[4]) * 1e9
+    return None
+
+
 def roofline(st, families):
     """Roofline of the dominant kernel family: algorithmic HBM bytes per
     launch over its mean launch time (HIP events on the library's stream)."""
@@ -50,10 +69,12 @@ def roofline(st, families):
         return None
     launches = max(int(st["prof_launches"][i]), 1)
     gbs = st["prof_bytes"][i] / (ms[i] / 1000.0) / 1e9
+    kernel = FAMILY_KERNEL.get(families[i])
     return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-            "traffic": None, "kernel": families[i], "launches": launches,
-            "bytes_per_launch": st["prof_bytes"][i] / launches, "ms_per_launch": ms[i] / launches,
-            "share_of_build": ms[i] / 1000.0 / max(st["t_total"], 1e-9)}
+            "traffic": measured_traffic(kernel), "kernel": kernel or families[i], "family": families[i],
+            "launches": launches, "bytes_per_launch": st["prof_bytes"][i] / launches,
+            "ms_per_launch": ms[i] / launches, "share_of_build": ms[i] / 1000.0 / max(st["t_total"], 1e-9),
+            "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT)}
 
 
 def copy_probe(mib):

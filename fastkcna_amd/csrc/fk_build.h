@@ -111,6 +111,8 @@ struct BuildState {
     // Returns a start event recorded on the build stream (nullptr when off).
     hipEvent_t prof_begin();
     void prof_end(hipEvent_t start, int fam, double bytes);
+    void prof_set_last_bytes(double bytes);  // bytes of the last prof_end, known later
+    DevBuf<uint32_t> bv_mark;  // profiling: nodes whose out-list a round reads
     void prof_flush();
     ~BuildState();
 

@@ -57,6 +57,10 @@ void BuildState::prof_end(hipEvent_t start, int fam, double bytes) {
     ev_pending.push_back({start, e, fam, bytes});
 }
 
+void BuildState::prof_set_last_bytes(double bytes) {
+    if (!ev_pending.empty()) ev_pending.back().bytes = bytes;
+}
+
 void BuildState::prof_flush() {
     for (auto &r : ev_pending) {
         FK_HIP(hipEventSynchronize(r.b));
@@ -675,6 +679,20 @@ __global__ __launch_bounds__(1024) void k_sum_u32(const uint32_t *__restrict__ v
     if (threadIdx.x == 0 && x) atomicAdd(acc, x);
 }
 
+// Out-degree sum of the marked nodes; clears the marks.
+__global__ __launch_bounds__(1024) void k_marked_degree(uint32_t *__restrict__ mark,
+                                                        const uint32_t *__restrict__ nsg_len, uint32_t n,
+                                                        unsigned long long *__restrict__ acc) {
+    unsigned long long x = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (mark[i]) {
+            x += nsg_len[i];
+            mark[i] = 0;
+        }
+    x = block_sum(x);
+    if (threadIdx.x == 0 && x) atomicAdd(acc, x);
+}
+
 static unsigned reduce_grid(uint32_t n_waves) {
     const unsigned g = (n_waves + 15) / 16;
     return g < 2048 ? (g ? g : 1) : 2048;
@@ -1027,7 +1045,8 @@ __global__ __launch_bounds__(FK_BV_THREADS) void k_bv_round(const float *__restr
                                                             const uint32_t *__restrict__ nsg_len,
                                                             const uint8_t *__restrict__ nsg_new, uint32_t ncap,
                                                             unsigned long long *__restrict__ cc_acc,
-                                                            const uint32_t *__restrict__ order) {
+                                                            const uint32_t *__restrict__ order,
+                                                            uint32_t *__restrict__ mark) {
     extern __shared__ __align__(16) uint8_t smem[];
     BvLds sh;
     sh.carve(smem, cap);
@@ -1056,6 +1075,7 @@ __global__ __launch_bounds__(FK_BV_THREADS) void k_bv_round(const float *__restr
             sh.sel_new[r] = (f & FK_FLG_ISNEW) ? 1 : 0;
             sh.sdeg[r] = nsg_len[q];
             sh.pf[j] = f & (uint8_t)~FK_FLG_NEW;
+            if (mark) mark[q] = 1u;  // profiling: q's out-list is read this round
         }
     }
     __syncthreads();
@@ -1147,11 +1167,12 @@ __global__ __launch_bounds__(FK_BV_THREADS) void k_bv_round(const float *__restr
 static void launch_bv_round(BuildState &b, uint32_t W, uint32_t scap) {
     const uint32_t *ord = b.has_order ? b.order.p : nullptr;
     const size_t lds = BvLds::bytes(b.cap);
+    uint32_t *mark = b.prof ? b.bv_mark.p : nullptr;
 #define FK_BV_LAUNCH(NBV)                                                                                       \
     hipLaunchKernelGGL(k_bv_round<NBV>, dim3(node_grid(b)), dim3(FK_BV_THREADS), lds, b.s, b.ds->chains.p,     \
                        b.ds->dpad, b.N,                                                                         \
                        b.cap, W, scap, b.pool.p, b.pflg.p, b.cnt.p, b.nsg_id.p, b.nsg_len.p, b.nsg_new.p, b.ncap, \
-                       b.acc.p, ord)
+                       b.acc.p, ord, mark)
     switch (b.ds->dpad) {
         case 32: FK_BV_LAUNCH(4); break;
         case 64: FK_BV_LAUNCH(8); break;
@@ -1190,25 +1211,34 @@ void bridgeview(BuildState &b) {
         const float delta = (float)cnts[0] / (float)N / (float)S;
         const bool last = delta < 0.5f;
         const uint32_t scap = last ? 0xFFFFFFFFu : S;
-        double pool_entries = 0;
-        if (b.prof) {
-            FK_HIP(hipMemsetAsync(b.acc.p + 6, 0, 8, b.s));
-            hipLaunchKernelGGL(k_sum_u32, dim3(reduce_grid(N / 64 + 1)), dim3(1024), 0, b.s, b.cnt.p, N, b.acc.p + 6);
-            unsigned long long pc = 0;
-            FK_HIP(hipMemcpyAsync(&pc, b.acc.p + 6, 8, hipMemcpyDeviceToHost, b.s));
-            b.sync();
-            pool_entries = (double)pc;
-        }
         FK_HIP(hipMemsetAsync(b.acc.p, 0, 8, b.s));
+        if (b.prof && b.bv_mark.n < N) {
+            b.bv_mark.alloc(N);
+            FK_HIP(hipMemsetAsync(b.bv_mark.p, 0, (size_t)N * 4, b.s));
+        }
         hipEvent_t pe = b.prof_begin();
         launch_bv_round(b, W, scap);
         FK_LAUNCH_CHECK();
+        b.prof_end(pe, FK_PROF_BV, 0.0);
         unsigned long long cc = 0;
         FK_HIP(hipMemcpyAsync(&cc, b.acc.p, 8, hipMemcpyDeviceToHost, b.s));
+        if (b.prof) {
+            // minimal traffic of a round: the rows (+ ids) of every expansion
+            // (|rnn_q|) and of every out-list entry of an expanded q
+            // (|nsg_q|), each read once, plus one read and one write of the
+            // packed pools (8 bytes per slot, MAXL + 1 slots)
+            FK_HIP(hipMemsetAsync(b.acc.p + 7, 0, 8, b.s));
+            hipLaunchKernelGGL(k_marked_degree, dim3(reduce_grid(N / 64 + 1)), dim3(1024), 0, b.s, b.bv_mark.p,
+                               b.nsg_len.p, N, b.acc.p + 7);
+            FK_LAUNCH_CHECK();
+            unsigned long long deg = 0;
+            FK_HIP(hipMemcpyAsync(&deg, b.acc.p + 7, 8, hipMemcpyDeviceToHost, b.s));
+            b.sync();
+            const double expansions = (double)(last ? cnts[1] : cnts[0]);
+            b.prof_set_last_bytes((expansions + (double)deg) * (4.0 * b.ds->d + 4.0) +
+                                  16.0 * (double)N * (double)(P.MAXL + 1));
+        }
         b.sync();
-        // one candidate row (+ its id) per distance; each pool read and
-        // rewritten once (8-byte key + 1-byte flags)
-        b.prof_end(pe, FK_PROF_BV, (double)cc * (4.0 * b.ds->d + 4.0) + 18.0 * pool_entries);
         b.prof_flush();
         cc_total += cc;
         ++rounds;

@@ -244,3 +244,47 @@ def test_full_build_odd_widths(device, name):
     rc = np.mean([len(set(resc[i]) & set(gt[i])) / 10 for i in range(100)])
     say("query recall@10 gpu", rg, "oracle", rc)
     assert rg >= rc - 0.03
+
+
+_C2_CACHE = {}
+
+
+def _c2_dumped_pool(device, p, n_dump=10_000):
+    if "pool" not in _C2_CACHE:
+        x = integer_mixture(1_000_000, 128, 1024, seed=1)
+        device.upload(x)
+        pools, plen, _ = device.nndescent(p)
+        _C2_CACHE["pool"] = (x, np.ascontiguousarray(pools[:n_dump]), plen[:n_dump].copy())
+        del pools
+    return _C2_CACHE["pool"]
+
+
+@pytest.mark.parametrize("pg", [FK_PG_NSG, FK_PG_ALPHAPG])
+def test_prune_bitexact_c2_dumped_pool(device, pg):
+    """The 1M x 128 NSG workload (1,024-centre integer mixture, seed 1,
+    K=L=500 S=12 R=100 I=6, nsg_R=50, search_K=500, angle 60): the pools of
+    nodes 0..9999 right after NN-descent are pruned on the GPU and by the
+    oracle; ids, kept distances and distance counts must agree bit for bit.
+    The pool keys themselves are checked against exact distances."""
+    from fastkcna_amd.datasets import params_c2
+    p = params_c2(IndexParams)
+    say("c2 dumped pool", pg)
+    x, pools, plen = _c2_dumped_pool(device, p)
+    device.upload(x)
+    n_dump = len(plen)
+    assert (plen == 500).all()
+    d, ids = oracle.unpack(pools)
+    assert (pools[:, 1:] > pools[:, :-1]).all()  # strictly (dist, id) ordered
+    rows = np.repeat(np.arange(0, n_dump, 10), 500)
+    exact = oracle.pair_distances(x, rows, ids[::10].ravel())
+    assert np.array_equal(exact.view(np.uint32), d[::10].ravel().view(np.uint32))
+    q = np.arange(n_dump, dtype=np.uint32)
+    g = device.prune(pg, pools, plen, q, len(x), p)
+    c = oracle.prune(pg, x, pools, plen, q, len(x), p)
+    assert np.array_equal(g[2], c[2])
+    for t in range(n_dump):
+        k = c[2][t]
+        assert np.array_equal(g[0][t, :k], c[0][t, :k]), t
+        assert np.array_equal(g[1][t, :k].view(np.uint32), c[1][t, :k].view(np.uint32)), t
+    assert g[3] == c[3]
+    say("c2 dumped pool: kept", int(c[2].sum()), "distances", c[3])

@@ -101,13 +101,103 @@ void launch_join_distances(hipStream_t s, const Dataset &ds, const uint32_t *ids
 // the accepts are resolved in candidate order with a ballot: the first
 // octet that no kept neighbour occludes is accepted, the later ones are
 // checked against it, and so on, which is the reference's sequential loop.
+//
+// Latency: the keys of batch b + 2 and the rows of batch b + 1 are loaded
+// while batch b is decided, and the chains of the first PruneMC kept
+// neighbours (which decide most occlusions) are kept in LDS, so a batch
+// rarely waits on a dependent global load.
+template <int NB>
+struct PruneMC {
+    static constexpr uint32_t value = NB == 0 ? 0u : ((4096u / (NB * 32u)) < 8u ? (4096u / (NB * 32u)) : 8u);
+};
+
+__host__ __device__ inline uint32_t prune_wave_bytes(uint32_t mc, uint32_t dpad, uint32_t R) {
+    return (mc * dpad * 4u + R * 8u + 15u) & ~15u;
+}
+
 template <int RULE, int NB>
-__global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains, uint32_t dpad, PruneLaunch a) {
-    __shared__ uint32_t s_oid[FK_PRUNE_WPB][FK_PRUNE_RMAX];
-    __shared__ float s_od[FK_PRUNE_WPB][FK_PRUNE_RMAX];
+struct PruneWave {
+    uint32_t oracle_n, R;
+    float tau3, cos_thr;
+    const float *lane_chains;  // chain k of row 0
+    float *lane_cache;         // chain k of cached kept neighbour 0
+    uint32_t *so;
+    float *sd;
+    uint32_t dpad, nb, g, k, lane, limit;
+    uint32_t nout;
+    bool stop;
+    unsigned long long cc, exam;
+
+    __device__ uint64_t key_at(const uint64_t *c, uint32_t idx) const {
+        return idx < limit ? c[idx] : FK_KEY_EMPTY;
+    }
+    __device__ const float *row_of(uint64_t key) const {
+        const uint32_t id = key_id(key);
+        return lane_chains + (size_t)((key != FK_KEY_EMPTY && id < oracle_n) ? id : 0u) * dpad;
+    }
+    __device__ float member_dist(const OctChain<NB> &crow, uint32_t t, uint32_t tid) const {
+        if (t < PruneMC<NB>::value) return crow.dist(lane_cache + (size_t)t * dpad, nb);
+        return crow.dist(lane_chains + (size_t)tid * dpad, nb);
+    }
+    // Decides the batch at pos whose key (this octet's) and chain are given.
+    __device__ void batch(uint32_t pos, uint64_t key, const OctChain<NB> &crow) {
+        const uint32_t nbat = (limit - pos) < 8u ? (limit - pos) : 8u;
+        const bool valid = g < nbat;
+        const uint32_t pid = key_id(key);
+        const float pd = key_dist(key);
+        const bool skip = !valid || pid >= oracle_n;
+        bool occl = false;
+        uint32_t mycc = 0;
+        __builtin_amdgcn_wave_barrier();
+        if (!skip) {
+            // octet-uniform: every lane of an octet sees the same pid, tid, djk
+            for (uint32_t t = 0; t < nout; ++t) {
+                const uint32_t tid = so[t];
+                if (tid == pid) { occl = true; break; }
+                ++mycc;
+                const float djk = member_dist(crow, t, tid);
+                if (occludes<RULE>(djk, pd, sd[t], tau3, cos_thr)) { occl = true; break; }
+            }
+        }
+        uint32_t done = 0;
+        uint32_t stop_g = 7;
+        while (true) {
+            const uint64_t m = wave_ballot(!skip && !occl && g >= done);
+            if (m == 0) break;
+            const uint32_t al = (uint32_t)__builtin_ctzll(m) >> 3;
+            const uint32_t aid = __shfl(pid, (int)(al * 8), 64);
+            const float ad = __shfl(pd, (int)(al * 8), 64);
+            const uint32_t slot = nout;
+            if (lane == 0) { so[slot] = aid; sd[slot] = ad; }
+            if (slot < PruneMC<NB>::value && g == al) crow.store(lane_cache + (size_t)slot * dpad, nb);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            ++nout;
+            done = al + 1;
+            if (nout >= R) { stop = true; stop_g = al; break; }
+            if (!skip && !occl && g > al) {
+                if (aid == pid) {
+                    occl = true;
+                } else {
+                    ++mycc;
+                    const float djk = member_dist(crow, slot, aid);
+                    if (occludes<RULE>(djk, pd, ad, tau3, cos_thr)) occl = true;
+                }
+            }
+        }
+        if (k == 0 && g <= stop_g) {
+            cc += mycc;
+            exam += skip ? 0u : 1u;
+        }
+    }
+};
+
+template <int RULE, int NB>
+__global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains, uint32_t dpad, PruneLaunch a,
+                                               uint32_t wave_bytes) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    constexpr uint32_t MC = PruneMC<NB>::value;
     const uint32_t lane = lane_id();
-    const uint32_t g = lane >> 3, k = lane & 7u;
-    const uint32_t nb = NB ? (uint32_t)NB : (dpad >> 3);
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t vb = a.xcd_order ? xcd_position((gridDim.x + FK_NUM_XCD - 1) / FK_NUM_XCD) : blockIdx.x;
     const uint32_t task = vb * FK_PRUNE_WPB + w;
@@ -119,9 +209,24 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains,
     uint32_t *oid = a.out_ids + (uint64_t)slot * a.out_stride;
     float *od = a.out_d ? a.out_d + (uint64_t)slot * a.out_stride : nullptr;
     uint32_t *olen = a.out_len + slot;
-    uint32_t *so = s_oid[w];
-    float *sd = s_od[w];
-    const float *lane_chains = chains + k * nb;
+    uint8_t *wb = smem + (size_t)w * wave_bytes;
+
+    PruneWave<RULE, NB> pw;
+    pw.oracle_n = a.oracle_n;
+    pw.R = a.R;
+    pw.tau3 = a.tau3;
+    pw.cos_thr = a.cos_thr;
+    pw.g = lane >> 3;
+    pw.k = lane & 7u;
+    pw.lane = lane;
+    pw.dpad = dpad;
+    pw.nb = NB ? (uint32_t)NB : (dpad >> 3);
+    pw.lane_chains = chains + pw.k * pw.nb;
+    pw.lane_cache = reinterpret_cast<float *>(wb) + pw.k * pw.nb;
+    pw.so = reinterpret_cast<uint32_t *>(wb + (size_t)MC * dpad * 4u);
+    pw.sd = reinterpret_cast<float *>(pw.so + a.R);
+    pw.cc = 0;
+    pw.exam = 0;
 
     uint32_t start = 0;
     if (len > 0 && key_id(c[0]) == q) start = 1;
@@ -129,92 +234,72 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains,
         if (lane == 0) *olen = 0;
         return;
     }
-    uint32_t nout = 1;
+    pw.limit = len < a.maxc ? len : a.maxc;
     {
+        // the first candidate is always kept
         const uint64_t k0 = c[start];
-        if (lane == 0) { so[0] = key_id(k0); sd[0] = key_dist(k0); }
-    }
-    const uint32_t limit = len < a.maxc ? len : a.maxc;
-    uint32_t pos = start + 1;
-    unsigned long long cc = 0, exam = 0;
-    bool stop = nout >= a.R;
-    while (!stop && pos < limit) {
-        const uint32_t nbat = (limit - pos) < 8u ? (limit - pos) : 8u;
-        const bool valid = g < nbat;
-        const uint64_t key = valid ? c[pos + g] : FK_KEY_EMPTY;
-        const uint32_t pid = key_id(key);
-        const float pd = key_dist(key);
-        const bool skip = !valid || pid >= a.oracle_n;
-        OctChain<NB> crow;
-        crow.load(lane_chains + (size_t)(skip ? 0u : pid) * dpad, nb);
-        bool occl = false;
-        uint32_t mycc = 0;
+        if (lane == 0) { pw.so[0] = key_id(k0); pw.sd[0] = key_dist(k0); }
+        if (MC > 0 && pw.g == 0) {
+            OctChain<NB> t0;
+            t0.load(pw.lane_chains + (size_t)key_id(k0) * dpad, pw.nb);
+            t0.store(pw.lane_cache, pw.nb);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        if (!skip) {
-            // octet-uniform: every lane of an octet sees the same pid, tid, djk
-            for (uint32_t t = 0; t < nout; ++t) {
-                const uint32_t tid = so[t];
-                if (tid == pid) { occl = true; break; }
-                ++mycc;
-                const float djk = crow.dist(lane_chains + (size_t)tid * dpad, nb);
-                if (occludes<RULE>(djk, pd, sd[t], a.tau3, a.cos_thr)) { occl = true; break; }
-            }
-        }
-        uint32_t done = 0;
-        uint32_t stop_g = 7;
-        while (true) {
-            const uint64_t m = wave_ballot(!skip && !occl && g >= done);
-            if (m == 0) break;
-            const uint32_t al = (uint32_t)__builtin_ctzll(m) >> 3;
-            const uint32_t aid = __shfl(pid, (int)(al * 8), 64);
-            const float ad = __shfl(pd, (int)(al * 8), 64);
-            if (lane == 0) { so[nout] = aid; sd[nout] = ad; }
-            ++nout;
-            done = al + 1;
-            if (nout >= a.R) { stop = true; stop_g = al; break; }
-            if (!skip && !occl && g > al) {
-                if (aid == pid) {
-                    occl = true;
-                } else {
-                    ++mycc;
-                    const float djk = crow.dist(lane_chains + (size_t)aid * dpad, nb);
-                    if (occludes<RULE>(djk, pd, ad, a.tau3, a.cos_thr)) occl = true;
-                }
-            }
-        }
-        if (k == 0 && g <= stop_g) {
-            cc += mycc;
-            exam += skip ? 0u : 1u;
-        }
-        pos += nbat;
+    }
+    pw.nout = 1;
+    pw.stop = pw.nout >= a.R;
+    const uint32_t p0 = start + 1;
+    // software pipeline, unrolled by two so the prefetched chain is never
+    // copied (a copy would wait for its load)
+    // invariant at the top: ka, kb = this octet's keys of batches 2j, 2j + 1;
+    // ra = chain of batch 2j (loaded or in flight)
+    uint64_t ka = pw.key_at(c, p0 + pw.g), kb = pw.key_at(c, p0 + 8 + pw.g);
+    OctChain<NB> ra, rb;
+    ra.load(pw.row_of(ka), pw.nb);
+    for (uint32_t pos = p0; !pw.stop && pos < pw.limit; pos += 16) {
+        rb.load(pw.row_of(kb), pw.nb);
+        const uint64_t kc = pw.key_at(c, pos + 16 + pw.g);
+        pw.batch(pos, ka, ra);
+        if (pw.stop || pos + 8 >= pw.limit) break;
+        ra.load(pw.row_of(kc), pw.nb);
+        const uint64_t kd = pw.key_at(c, pos + 24 + pw.g);
+        pw.batch(pos + 8, kb, rb);
+        ka = kc;
+        kb = kd;
     }
     __builtin_amdgcn_wave_barrier();
-    for (uint32_t t = lane; t < nout; t += 64) {
-        oid[t] = so[t];
-        if (od) od[t] = sd[t];
+    for (uint32_t t = lane; t < pw.nout; t += 64) {
+        oid[t] = pw.so[t];
+        if (od) od[t] = pw.sd[t];
     }
-    if (lane == 0) *olen = nout;
+    if (lane == 0) *olen = pw.nout;
     if (a.cc) {
-        cc = wave_sum(cc);
-        if (lane == 0 && cc) atomicAdd(a.cc, cc);
+        const unsigned long long s = wave_sum(pw.cc);
+        if (lane == 0 && s) atomicAdd(a.cc, s);
     }
     if (a.examined) {
-        exam = wave_sum(exam);
-        if (lane == 0 && exam) atomicAdd(a.examined, exam);
+        const unsigned long long s = wave_sum(pw.exam);
+        if (lane == 0 && s) atomicAdd(a.examined, s);
     }
+}
+
+template <int RULE, int NB>
+static void launch_prune_nb(hipStream_t s, const Dataset &ds, const PruneLaunch &a, unsigned grid) {
+    const uint32_t wave_bytes = prune_wave_bytes(PruneMC<NB>::value, ds.dpad, a.R);
+    hipLaunchKernelGGL((k_prune<RULE, NB>), dim3(grid), dim3(64 * FK_PRUNE_WPB), (size_t)wave_bytes * FK_PRUNE_WPB,
+                       s, ds.chains.p, ds.dpad, a, wave_bytes);
 }
 
 template <int RULE>
 static void launch_prune_rule(hipStream_t s, const Dataset &ds, const PruneLaunch &a, unsigned grid) {
-    const dim3 blk(64 * FK_PRUNE_WPB);
-    const float *ch = ds.chains.p;
     switch (ds.dpad) {
-        case 32: hipLaunchKernelGGL((k_prune<RULE, 4>), dim3(grid), blk, 0, s, ch, ds.dpad, a); break;
-        case 64: hipLaunchKernelGGL((k_prune<RULE, 8>), dim3(grid), blk, 0, s, ch, ds.dpad, a); break;
-        case 96: hipLaunchKernelGGL((k_prune<RULE, 12>), dim3(grid), blk, 0, s, ch, ds.dpad, a); break;
-        case 128: hipLaunchKernelGGL((k_prune<RULE, 16>), dim3(grid), blk, 0, s, ch, ds.dpad, a); break;
-        case 256: hipLaunchKernelGGL((k_prune<RULE, 32>), dim3(grid), blk, 0, s, ch, ds.dpad, a); break;
-        default: hipLaunchKernelGGL((k_prune<RULE, 0>), dim3(grid), blk, 0, s, ch, ds.dpad, a); break;
+        case 32: launch_prune_nb<RULE, 4>(s, ds, a, grid); break;
+        case 64: launch_prune_nb<RULE, 8>(s, ds, a, grid); break;
+        case 96: launch_prune_nb<RULE, 12>(s, ds, a, grid); break;
+        case 128: launch_prune_nb<RULE, 16>(s, ds, a, grid); break;
+        case 256: launch_prune_nb<RULE, 32>(s, ds, a, grid); break;
+        default: launch_prune_nb<RULE, 0>(s, ds, a, grid); break;
     }
 }
 

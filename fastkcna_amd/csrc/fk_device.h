@@ -152,6 +152,10 @@ struct OctChain {
 #pragma unroll
         for (int q = 0; q < NB / 4; ++q) v[q] = reinterpret_cast<const float4 *>(c)[q];
     }
+    __device__ __forceinline__ void store(float *__restrict__ c, uint32_t) const {
+#pragma unroll
+        for (int q = 0; q < NB / 4; ++q) reinterpret_cast<float4 *>(c)[q] = v[q];
+    }
     // distance to the row whose chain (same lane) starts at o
     __device__ __forceinline__ float dist(const float *__restrict__ o, uint32_t) const {
         float acc = 0.f, t;
@@ -170,6 +174,7 @@ template <>
 struct OctChain<0> {
     const float *p;
     __device__ __forceinline__ void load(const float *__restrict__ c, uint32_t) { p = c; }
+    __device__ __forceinline__ void store(float *__restrict__, uint32_t) const {}
     __device__ __forceinline__ float dist(const float *__restrict__ o, uint32_t nb) const {
         float acc = 0.f;
         for (uint32_t j = 0; j < nb; ++j) {

@@ -253,6 +253,15 @@ __global__ void k_pair_counts(const uint64_t *__restrict__ list_off, const uint3
 // ---------------------------------------------------------------------------
 #define FK_JOIN_LDS_ROWS_BYTES (40 * 1024)
 
+// Dynamic LDS of the two passes: per-list-slot threshold, count and id (and
+// bucket base in pass 2), sized by the longest list (L + 2R); pass 1 also
+// stages the sampled rows when they fit.
+size_t join_count_lds(uint32_t tcap) { return FK_JOIN_LDS_ROWS_BYTES + (size_t)tcap * 12; }
+size_t join_fill_lds(uint32_t tcap) { return (size_t)tcap * 20; }
+
+// One wavefront per row x of the new list, lanes over y > x: the x-side
+// candidate count of a 64-wide step is one ballot, the y-side counts go to
+// distinct slots.
 __global__ __launch_bounds__(256) void k_join_count(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
                                                     const uint32_t *__restrict__ lists,
                                                     const uint64_t *__restrict__ list_off,
@@ -260,10 +269,11 @@ __global__ __launch_bounds__(256) void k_join_count(const float *__restrict__ ro
                                                     const uint64_t *__restrict__ pair_off,
                                                     const float *__restrict__ thr, float *__restrict__ pdist,
                                                     uint32_t *__restrict__ resv, uint32_t *__restrict__ bcnt,
-                                                    uint32_t N, const uint32_t *__restrict__ order) {
+                                                    uint32_t N, const uint32_t *__restrict__ order, uint32_t tcap) {
     extern __shared__ __align__(16) float s_rows[];
-    __shared__ float s_thr[FK_LIST_MAX];
-    __shared__ uint32_t s_c[FK_LIST_MAX];
+    float *s_thr = s_rows + FK_JOIN_LDS_ROWS_BYTES / 4;
+    uint32_t *s_c = reinterpret_cast<uint32_t *>(s_thr + tcap);
+    uint32_t *s_id = s_c + tcap;
     const uint32_t node = node_of_block(order, N);
     if (node >= N) return;
     const uint64_t lo = list_off[node];
@@ -274,7 +284,9 @@ __global__ __launch_bounds__(256) void k_join_count(const float *__restrict__ ro
     const uint32_t ls = dpad + 4;
     const bool in_lds = (size_t)T * ls * sizeof(float) <= FK_JOIN_LDS_ROWS_BYTES;
     for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
-        s_thr[t] = thr[lst[t]];
+        const uint32_t id = lst[t];
+        s_id[t] = id;
+        s_thr[t] = thr[id];
         s_c[t] = 0;
     }
     if (in_lds) {
@@ -286,29 +298,39 @@ __global__ __launch_bounds__(256) void k_join_count(const float *__restrict__ ro
         }
     }
     __syncthreads();
-    const uint64_t cells = (uint64_t)nn * T;
-    for (uint64_t c = threadIdx.x; c < cells; c += blockDim.x) {
-        const uint32_t x = (uint32_t)(c / T), y = (uint32_t)(c - (uint64_t)x * T);
-        if (y <= x) continue;
-        const uint32_t ix = lst[x], iy = lst[y];
+    const uint32_t lane = lane_id(), nw = blockDim.x >> 6;
+    for (uint32_t x = threadIdx.x >> 6; x < nn; x += nw) {
+        const uint32_t ix = s_id[x];
+        const float tx = s_thr[x];
         const float *ra = in_lds ? s_rows + x * ls : rows + (size_t)ix * stride;
-        const float *rb = in_lds ? s_rows + y * ls : rows + (size_t)iy * stride;
-        const float dd = l2_canon(ra, rb, dpad);
-        const uint64_t rs = (uint64_t)x * (T - 1) - (uint64_t)x * (x - 1) / 2;
-        o[rs + (y - x - 1)] = dd;
-        if (ix != iy) {
-            if (dd <= s_thr[x]) atomicAdd(&s_c[x], 1u);
-            if (dd <= s_thr[y]) atomicAdd(&s_c[y], 1u);
+        float *ox = o + ((uint64_t)x * (T - 1) - (uint64_t)x * (x - 1) / 2) - (x + 1);  // ox[y], y > x
+        uint32_t cx = 0;
+        for (uint32_t y0 = x + 1; y0 < T; y0 += 64) {
+            const uint32_t y = y0 + lane;
+            bool ax = false;
+            if (y < T) {
+                const uint32_t iy = s_id[y];
+                const float *rb = in_lds ? s_rows + y * ls : rows + (size_t)iy * stride;
+                const float dd = l2_canon(ra, rb, dpad);
+                ox[y] = dd;
+                if (ix != iy) {
+                    ax = dd <= tx;
+                    if (dd <= s_thr[y]) atomicAdd(&s_c[y], 1u);
+                }
+            }
+            cx += (uint32_t)__builtin_popcountll(wave_ballot(ax));
         }
+        if (lane == 0 && cx) atomicAdd(&s_c[x], cx);
     }
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
         const uint32_t c = s_c[t];
-        resv[lo + t] = c ? atomicAdd(&bcnt[lst[t]], c) : 0u;
+        resv[lo + t] = c ? atomicAdd(&bcnt[s_id[t]], c) : 0u;
     }
 }
 
-// Pass 2: write the candidates into their reserved bucket slots.
+// Pass 2: write the candidates into their reserved bucket slots (same walk
+// and conditions as pass 1, so the per-target counts agree).
 __global__ __launch_bounds__(256) void k_join_fill(const uint32_t *__restrict__ lists,
                                                    const uint64_t *__restrict__ list_off,
                                                    const uint32_t *__restrict__ n_new,
@@ -316,11 +338,14 @@ __global__ __launch_bounds__(256) void k_join_fill(const uint32_t *__restrict__ 
                                                    const float *__restrict__ thr, const float *__restrict__ pdist,
                                                    const uint32_t *__restrict__ resv,
                                                    const uint64_t *__restrict__ boff, uint64_t *__restrict__ bkey,
-                                                   uint32_t *__restrict__ bsrc) {
-    __shared__ float s_thr[FK_LIST_MAX];
-    __shared__ uint32_t s_c[FK_LIST_MAX];
-    __shared__ uint64_t s_base[FK_LIST_MAX];
-    const uint32_t node = blockIdx.x;
+                                                   uint32_t *__restrict__ bsrc, uint32_t N,
+                                                   const uint32_t *__restrict__ order, uint32_t tcap) {
+    extern __shared__ __align__(16) uint64_t s_base[];
+    float *s_thr = reinterpret_cast<float *>(s_base + tcap);
+    uint32_t *s_c = reinterpret_cast<uint32_t *>(s_thr + tcap);
+    uint32_t *s_id = s_c + tcap;
+    const uint32_t node = node_of_block(order, N);
+    if (node >= N) return;
     const uint64_t lo = list_off[node];
     const uint32_t T = (uint32_t)(list_off[node + 1] - lo);
     const uint32_t nn = n_new[node];
@@ -328,28 +353,47 @@ __global__ __launch_bounds__(256) void k_join_fill(const uint32_t *__restrict__ 
     const float *o = pdist + pair_off[node];
     for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
         const uint32_t id = lst[t];
+        s_id[t] = id;
         s_thr[t] = thr[id];
         s_c[t] = 0;
         s_base[t] = boff[id] + resv[lo + t];
     }
     __syncthreads();
-    const uint64_t cells = (uint64_t)nn * T;
-    for (uint64_t c = threadIdx.x; c < cells; c += blockDim.x) {
-        const uint32_t x = (uint32_t)(c / T), y = (uint32_t)(c - (uint64_t)x * T);
-        if (y <= x) continue;
-        const uint32_t ix = lst[x], iy = lst[y];
-        if (ix == iy) continue;
-        const uint64_t rs = (uint64_t)x * (T - 1) - (uint64_t)x * (x - 1) / 2;
-        const float dd = o[rs + (y - x - 1)];
-        if (dd <= s_thr[x]) {
-            const uint64_t sl = s_base[x] + atomicAdd(&s_c[x], 1u);
-            bkey[sl] = make_key(dd, iy);
-            bsrc[sl] = node;
-        }
-        if (dd <= s_thr[y]) {
-            const uint64_t sl = s_base[y] + atomicAdd(&s_c[y], 1u);
-            bkey[sl] = make_key(dd, ix);
-            bsrc[sl] = node;
+    const uint32_t lane = lane_id(), nw = blockDim.x >> 6;
+    for (uint32_t x = threadIdx.x >> 6; x < nn; x += nw) {
+        const uint32_t ix = s_id[x];
+        const float tx = s_thr[x];
+        const uint64_t bx = s_base[x];
+        const float *ox = o + ((uint64_t)x * (T - 1) - (uint64_t)x * (x - 1) / 2) - (x + 1);
+        for (uint32_t y0 = x + 1; y0 < T; y0 += 64) {
+            const uint32_t y = y0 + lane;
+            bool ax = false, ay = false;
+            float dd = 0.f;
+            uint32_t iy = 0;
+            if (y < T) {
+                iy = s_id[y];
+                if (ix != iy) {
+                    dd = ox[y];
+                    ax = dd <= tx;
+                    ay = dd <= s_thr[y];
+                }
+            }
+            const uint64_t m = wave_ballot(ax);
+            uint32_t base = 0;
+            if (lane == 0 && m) base = atomicAdd(&s_c[x], (uint32_t)__builtin_popcountll(m));
+            base = __shfl(base, 0, 64);
+            if (ax) {
+                const uint32_t rank =
+                    __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                const uint64_t sl = bx + base + rank;
+                bkey[sl] = make_key(dd, iy);
+                bsrc[sl] = node;
+            }
+            if (ay) {
+                const uint64_t sl = s_base[y] + atomicAdd(&s_c[y], 1u);
+                bkey[sl] = make_key(dd, ix);
+                bsrc[sl] = node;
+            }
         }
     }
 }
@@ -801,12 +845,14 @@ static void nnd_join(BuildState &b) {
     FK_HIP(hipMemcpyAsync(&nlist, b.list_off.p + N, 8, hipMemcpyDeviceToHost, b.s));
     b.sync();
     b.resv.ensure(nlist);
-    const size_t lds = FK_JOIN_LDS_ROWS_BYTES;
+    const uint32_t tcap = std::max(b.P.L + 2 * b.P.R, 2 * b.P.S);  // longest sample list
+    const size_t lds = join_count_lds(tcap);
+    const uint32_t *ord = b.has_order ? b.order.p : nullptr;
     const double d4 = 4.0 * b.ds->d;
     hipEvent_t pe = b.prof_begin();
     hipLaunchKernelGGL(k_join_count, dim3(node_grid(b)), dim3(256), lds, b.s, b.ds->rows.p, b.ds->stride,
                        b.ds->dpad, b.lists.p, b.list_off.p, b.n_new.p, b.pair_off.p, b.thr.p, b.pdist.p, b.resv.p,
-                       b.bcnt.p, N, b.has_order ? b.order.p : nullptr);
+                       b.bcnt.p, N, ord, tcap);
     FK_LAUNCH_CHECK();
     // sampled rows and ids gathered once per join, one float per pair written
     b.prof_end(pe, FK_PROF_JOIN, (double)nlist * (d4 + 4.0) + 4.0 * (double)pairs);
@@ -823,8 +869,9 @@ static void nnd_join(BuildState &b) {
         pool_entries = (double)pe_cnt;
     }
     pe = b.prof_begin();
-    hipLaunchKernelGGL(k_join_fill, dim3(N), dim3(256), 0, b.s, b.lists.p, b.list_off.p, b.n_new.p, b.pair_off.p,
-                       b.thr.p, b.pdist.p, b.resv.p, b.boff.p, b.bkey.p, b.bsrc.p);
+    hipLaunchKernelGGL(k_join_fill, dim3(node_grid(b)), dim3(256), join_fill_lds(tcap), b.s, b.lists.p, b.list_off.p,
+                       b.n_new.p, b.pair_off.p, b.thr.p, b.pdist.p, b.resv.p, b.boff.p, b.bkey.p, b.bsrc.p, N, ord,
+                       tcap);
     FK_LAUNCH_CHECK();
     // pair distances and list ids read, (key, source) bucket entries written
     b.prof_end(pe, FK_PROF_JOIN_FILL, 4.0 * (double)pairs + 4.0 * (double)nlist + 12.0 * (double)cands);

@@ -167,9 +167,11 @@ fk_status fk_join_distances(fk_handle *h, const uint32_t *ids, const uint64_t *l
         const uint64_t n_ids = list_off[n_nodes];
         FK_CHECK_ARG(n_ids == 0 || ids, "null ids");
         std::vector<uint64_t> pair_off(n_nodes + 1, 0);
+        uint64_t tmax = 0;
         for (uint32_t t = 0; t < n_nodes; ++t) {
             FK_CHECK_ARG(list_off[t + 1] >= list_off[t], "list offsets must be non-decreasing");
             const uint64_t T = list_off[t + 1] - list_off[t];
+            tmax = std::max(tmax, T);
             const uint64_t nn = n_new[t];
             FK_CHECK_ARG(nn <= T, "n_new larger than the list");
             pair_off[t + 1] = pair_off[t] + nn * (nn - (nn ? 1 : 0)) / 2 + nn * (T - nn);
@@ -187,7 +189,8 @@ fk_status fk_join_distances(fk_handle *h, const uint32_t *ids, const uint64_t *l
         FK_HIP(hipMemcpyAsync(dlo.p, list_off, (size_t)(n_nodes + 1) * 8, hipMemcpyHostToDevice, h->stream));
         FK_HIP(hipMemcpyAsync(dpo.p, pair_off.data(), (size_t)(n_nodes + 1) * 8, hipMemcpyHostToDevice,
                               h->stream));
-        launch_join_distances(h->stream, h->ds, dids.p, dlo.p, dnn.p, dpo.p, n_nodes, dout.p);
+        FK_CHECK_ARG(tmax <= 16384, "sample list longer than 16384");
+        launch_join_distances(h->stream, h->ds, dids.p, dlo.p, dnn.p, dpo.p, n_nodes, (uint32_t)tmax, dout.p);
         FK_HIP(hipMemcpyAsync(out, dout.p, pair_off[n_nodes] * 4, hipMemcpyDeviceToHost, h->stream));
         FK_HIP(hipStreamSynchronize(h->stream));
     });

@@ -2,6 +2,7 @@
 // pruner, prefix scans and the brute-force exact kNN.
 #include "fk_device.h"
 #include "fk_internal.h"
+#include "fk_join.h"
 #include "fk_sort.h"
 
 namespace fk {
@@ -28,13 +29,10 @@ void launch_pair_distances(hipStream_t s, const Dataset &ds, const uint32_t *a, 
 }
 
 // ---------------------------------------------------------------------------
-// Local-join distance block.  One workgroup per node; the node's sample rows
-// are staged in LDS (row stride dpad + 4 floats, so lanes reading the same
-// offset of different rows hit different banks) when they fit, and read from
-// HBM/L2 otherwise.  Pair (x, y), x < n_new, x < y < T, goes to
+// Local-join distance block (the NN-descent join's join_block), one
+// workgroup per node.  Pair (x, y), x < n_new, x < y < T, goes to
 // out[pair_off + rowstart(x) + y - x - 1], rowstart(x) = x(T-1) - x(x-1)/2.
 // ---------------------------------------------------------------------------
-#define FK_JOIN_LDS_BYTES (48 * 1024)
 
 __global__ __launch_bounds__(256) void k_join_distances(const float *__restrict__ rows, uint32_t stride,
                                                         uint32_t dpad, const uint32_t *__restrict__ ids,
@@ -43,39 +41,26 @@ __global__ __launch_bounds__(256) void k_join_distances(const float *__restrict_
                                                         const uint64_t *__restrict__ pair_off,
                                                         float *__restrict__ out) {
     extern __shared__ __align__(16) float s_rows[];
+    uint32_t *s_id = reinterpret_cast<uint32_t *>(s_rows + FK_JOIN_LDS_ROWS_BYTES / 4);
     const uint32_t node = blockIdx.x;
     const uint64_t lo = list_off[node];
     const uint32_t T = (uint32_t)(list_off[node + 1] - lo);
     const uint32_t nn = n_new[node];
-    const uint32_t *lst = ids + lo;
     float *o = out + pair_off[node];
-    const uint32_t ls = dpad + 4;
-    const bool in_lds = (size_t)T * ls * sizeof(float) <= FK_JOIN_LDS_BYTES;
-    if (in_lds) {
-        const uint32_t v4 = dpad >> 2;  // float4 per row
-        for (uint32_t i = threadIdx.x; i < T * v4; i += blockDim.x) {
-            const uint32_t r = i / v4, c = i - r * v4;
-            const float4 v = *reinterpret_cast<const float4 *>(rows + (size_t)lst[r] * stride + c * 4);
-            *reinterpret_cast<float4 *>(s_rows + r * ls + c * 4) = v;
-        }
-        __syncthreads();
-    }
-    const uint64_t cells = (uint64_t)nn * T;
-    for (uint64_t c = threadIdx.x; c < cells; c += blockDim.x) {
-        const uint32_t x = (uint32_t)(c / T), y = (uint32_t)(c - (uint64_t)x * T);
-        if (y <= x) continue;
-        const float *ra = in_lds ? s_rows + x * ls : rows + (size_t)lst[x] * stride;
-        const float *rb = in_lds ? s_rows + y * ls : rows + (size_t)lst[y] * stride;
-        const uint64_t rs = (uint64_t)x * (T - 1) - (uint64_t)x * (x - 1) / 2;
-        o[rs + (y - x - 1)] = l2_canon(ra, rb, dpad);
-    }
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) s_id[t] = ids[lo + t];
+    __syncthreads();
+    join_block(rows, stride, dpad, s_id, T, nn, s_rows,
+               [&](uint32_t x, uint32_t y, float dd) { o[join_pair_index(x, y, T)] = dd; });
 }
 
 void launch_join_distances(hipStream_t s, const Dataset &ds, const uint32_t *ids, const uint64_t *list_off,
-                           const uint32_t *n_new, const uint64_t *pair_off, uint32_t n_nodes, float *out) {
+                           const uint32_t *n_new, const uint64_t *pair_off, uint32_t n_nodes, uint32_t tmax,
+                           float *out) {
     if (n_nodes == 0) return;
-    hipLaunchKernelGGL(k_join_distances, dim3(n_nodes), dim3(256), FK_JOIN_LDS_BYTES, s, ds.rows.p, ds.stride,
-                       ds.dpad, ids, list_off, n_new, pair_off, out);
+    FK_CHECK_ARG(2ull * (ds.dpad + 4) * 4 <= FK_JOIN_LDS_ROWS_BYTES, "rows too wide for the join block");
+    FK_CHECK_ARG(tmax <= 16384, "sample list longer than 16384");
+    hipLaunchKernelGGL(k_join_distances, dim3(n_nodes), dim3(256), FK_JOIN_LDS_ROWS_BYTES + (size_t)tmax * 4, s,
+                       ds.rows.p, ds.stride, ds.dpad, ids, list_off, n_new, pair_off, out);
     FK_LAUNCH_CHECK();
 }
 
@@ -86,11 +71,9 @@ void launch_join_distances(hipStream_t s, const Dataset &ds, const uint32_t *ids
 // pool[start] is always kept (start skips a leading self entry); every later
 // candidate at position < min(len, maxc) is examined in order until nsg_R are
 // kept; it is dropped if an already kept entry has its id or occludes it.
-// The wave takes 64 candidates at a time, one per lane: every lane checks its
-// candidate against the kept list (in kept order, stopping at the first
-// occluder, as the reference does), then the lowest surviving lane is
-// accepted and only the lanes above it check the new member, repeatedly.
-// That reproduces the reference's decisions and its distance count exactly.
+// Each candidate checks the kept list in kept order and stops at the first
+// occluder, as the reference does, so the decisions and the distance count
+// are the reference's exactly.
 // ---------------------------------------------------------------------------
 #define FK_PRUNE_WPB 4
 #define FK_PRUNE_RMAX 256
@@ -193,7 +176,7 @@ struct PruneWave {
 };
 
 template <int RULE, int NB>
-__global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains, uint32_t dpad, PruneLaunch a,
+__global__ __launch_bounds__(256, 5) void k_prune(const float *__restrict__ chains, uint32_t dpad, PruneLaunch a,
                                                uint32_t wave_bytes) {
     extern __shared__ __align__(16) uint8_t smem[];
     constexpr uint32_t MC = PruneMC<NB>::value;

@@ -144,7 +144,7 @@ void build_chains(hipStream_t s, Dataset &ds);  // fills ds.chains from ds.rows
 
 void launch_join_distances(hipStream_t s, const Dataset &ds, const uint32_t *ids,
                            const uint64_t *list_off, const uint32_t *n_new, const uint64_t *pair_off,
-                           uint32_t n_nodes, float *out);
+                           uint32_t n_nodes, uint32_t tmax, float *out);
 
 // Exclusive prefix sums over device arrays (out may alias in).  `total`
 // (device pointer, may be nullptr) receives the sum; `tmp` is grown as needed

@@ -22,6 +22,7 @@
 
 #include "fk_build.h"
 #include "fk_device.h"
+#include "fk_join.h"
 #include "fk_sort.h"
 
 namespace fk {
@@ -251,17 +252,12 @@ __global__ void k_pair_counts(const uint64_t *__restrict__ list_off, const uint3
 // src/kgraph.cpp:970-987 enumerates new x new (i < j) and new x old; a pair
 // joins j into pool(i) and i into pool(j).  Pairs of equal ids add nothing.
 // ---------------------------------------------------------------------------
-#define FK_JOIN_LDS_ROWS_BYTES (40 * 1024)
-
 // Dynamic LDS of the two passes: per-list-slot threshold, count and id (and
 // bucket base in pass 2), sized by the longest list (L + 2R); pass 1 also
 // stages the sampled rows when they fit.
 size_t join_count_lds(uint32_t tcap) { return FK_JOIN_LDS_ROWS_BYTES + (size_t)tcap * 12; }
 size_t join_fill_lds(uint32_t tcap) { return (size_t)tcap * 20; }
 
-// One wavefront per row x of the new list, lanes over y > x: the x-side
-// candidate count of a 64-wide step is one ballot, the y-side counts go to
-// distinct slots.
 __global__ __launch_bounds__(256) void k_join_count(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
                                                     const uint32_t *__restrict__ lists,
                                                     const uint64_t *__restrict__ list_off,
@@ -281,47 +277,20 @@ __global__ __launch_bounds__(256) void k_join_count(const float *__restrict__ ro
     const uint32_t nn = n_new[node];
     const uint32_t *lst = lists + lo;
     float *o = pdist + pair_off[node];
-    const uint32_t ls = dpad + 4;
-    const bool in_lds = (size_t)T * ls * sizeof(float) <= FK_JOIN_LDS_ROWS_BYTES;
     for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
         const uint32_t id = lst[t];
         s_id[t] = id;
         s_thr[t] = thr[id];
         s_c[t] = 0;
     }
-    if (in_lds) {
-        const uint32_t v4 = dpad >> 2;
-        for (uint32_t i = threadIdx.x; i < T * v4; i += blockDim.x) {
-            const uint32_t r = i / v4, c = i - r * v4;
-            *reinterpret_cast<float4 *>(s_rows + r * ls + c * 4) =
-                *reinterpret_cast<const float4 *>(rows + (size_t)lst[r] * stride + c * 4);
-        }
-    }
     __syncthreads();
-    const uint32_t lane = lane_id(), nw = blockDim.x >> 6;
-    for (uint32_t x = threadIdx.x >> 6; x < nn; x += nw) {
-        const uint32_t ix = s_id[x];
-        const float tx = s_thr[x];
-        const float *ra = in_lds ? s_rows + x * ls : rows + (size_t)ix * stride;
-        float *ox = o + ((uint64_t)x * (T - 1) - (uint64_t)x * (x - 1) / 2) - (x + 1);  // ox[y], y > x
-        uint32_t cx = 0;
-        for (uint32_t y0 = x + 1; y0 < T; y0 += 64) {
-            const uint32_t y = y0 + lane;
-            bool ax = false;
-            if (y < T) {
-                const uint32_t iy = s_id[y];
-                const float *rb = in_lds ? s_rows + y * ls : rows + (size_t)iy * stride;
-                const float dd = l2_canon(ra, rb, dpad);
-                ox[y] = dd;
-                if (ix != iy) {
-                    ax = dd <= tx;
-                    if (dd <= s_thr[y]) atomicAdd(&s_c[y], 1u);
-                }
-            }
-            cx += (uint32_t)__builtin_popcountll(wave_ballot(ax));
+    join_block(rows, stride, dpad, s_id, T, nn, s_rows, [&](uint32_t x, uint32_t y, float dd) {
+        o[join_pair_index(x, y, T)] = dd;
+        if (s_id[x] != s_id[y]) {
+            if (dd <= s_thr[x]) atomicAdd(&s_c[x], 1u);
+            if (dd <= s_thr[y]) atomicAdd(&s_c[y], 1u);
         }
-        if (lane == 0 && cx) atomicAdd(&s_c[x], cx);
-    }
+    });
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
         const uint32_t c = s_c[t];
@@ -359,41 +328,22 @@ __global__ __launch_bounds__(256) void k_join_fill(const uint32_t *__restrict__ 
         s_base[t] = boff[id] + resv[lo + t];
     }
     __syncthreads();
-    const uint32_t lane = lane_id(), nw = blockDim.x >> 6;
-    for (uint32_t x = threadIdx.x >> 6; x < nn; x += nw) {
-        const uint32_t ix = s_id[x];
-        const float tx = s_thr[x];
-        const uint64_t bx = s_base[x];
-        const float *ox = o + ((uint64_t)x * (T - 1) - (uint64_t)x * (x - 1) / 2) - (x + 1);
-        for (uint32_t y0 = x + 1; y0 < T; y0 += 64) {
-            const uint32_t y = y0 + lane;
-            bool ax = false, ay = false;
-            float dd = 0.f;
-            uint32_t iy = 0;
-            if (y < T) {
-                iy = s_id[y];
-                if (ix != iy) {
-                    dd = ox[y];
-                    ax = dd <= tx;
-                    ay = dd <= s_thr[y];
-                }
-            }
-            const uint64_t m = wave_ballot(ax);
-            uint32_t base = 0;
-            if (lane == 0 && m) base = atomicAdd(&s_c[x], (uint32_t)__builtin_popcountll(m));
-            base = __shfl(base, 0, 64);
-            if (ax) {
-                const uint32_t rank =
-                    __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                const uint64_t sl = bx + base + rank;
-                bkey[sl] = make_key(dd, iy);
-                bsrc[sl] = node;
-            }
-            if (ay) {
-                const uint64_t sl = s_base[y] + atomicAdd(&s_c[y], 1u);
-                bkey[sl] = make_key(dd, ix);
-                bsrc[sl] = node;
-            }
+    const uint32_t cells = nn * T;
+    for (uint32_t c = threadIdx.x; c < cells; c += blockDim.x) {
+        const uint32_t x = c / T, y = c - x * T;
+        if (y <= x) continue;
+        const uint32_t ix = s_id[x], iy = s_id[y];
+        if (ix == iy) continue;
+        const float dd = o[join_pair_index(x, y, T)];
+        if (dd <= s_thr[x]) {
+            const uint64_t sl = s_base[x] + atomicAdd(&s_c[x], 1u);
+            bkey[sl] = make_key(dd, iy);
+            bsrc[sl] = node;
+        }
+        if (dd <= s_thr[y]) {
+            const uint64_t sl = s_base[y] + atomicAdd(&s_c[y], 1u);
+            bkey[sl] = make_key(dd, ix);
+            bsrc[sl] = node;
         }
     }
 }
@@ -846,6 +796,28 @@ static void nnd_join(BuildState &b) {
     b.sync();
     b.resv.ensure(nlist);
     const uint32_t tcap = std::max(b.P.L + 2 * b.P.R, 2 * b.P.S);  // longest sample list
+    FK_CHECK_ARG(2ull * (b.ds->dpad + 4) * 4 <= FK_JOIN_LDS_ROWS_BYTES, "rows too wide for the join block");
+    if (b.verbose) {
+        // list-size profile: which blocks stage their rows in LDS
+        std::vector<uint64_t> lo((size_t)N + 1);
+        std::vector<uint32_t> nnv(N);
+        FK_HIP(hipMemcpyAsync(lo.data(), b.list_off.p, lo.size() * 8, hipMemcpyDeviceToHost, b.s));
+        FK_HIP(hipMemcpyAsync(nnv.data(), b.n_new.p, (size_t)N * 4, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+        const uint32_t fit = FK_JOIN_LDS_ROWS_BYTES / ((b.ds->dpad + 4) * 4);
+        uint64_t tmax = 0, nfit = 0, pfit = 0, ptot = 0, nsum = 0;
+        for (uint32_t i = 0; i < N; ++i) {
+            const uint64_t T = lo[i + 1] - lo[i], nn = nnv[i];
+            const uint64_t pr = nn * (nn ? nn - 1 : 0) / 2 + nn * (T - nn);
+            tmax = std::max(tmax, T);
+            nsum += nn;
+            ptot += pr;
+            if (T <= fit) { ++nfit; pfit += pr; }
+        }
+        fprintf(stderr, "[fk] join lists: mean T %.1f, mean new %.1f, max T %llu; %.1f%% of nodes (%.1f%% of pairs) "
+                        "stage rows in LDS (T <= %u)\n", (double)lo[N] / N, (double)nsum / N,
+                (unsigned long long)tmax, 100.0 * nfit / N, 100.0 * pfit / std::max<uint64_t>(ptot, 1), fit);
+    }
     const size_t lds = join_count_lds(tcap);
     const uint32_t *ord = b.has_order ? b.order.p : nullptr;
     const double d4 = 4.0 * b.ds->d;

@@ -460,6 +460,57 @@ __global__ __launch_bounds__(256) void k_bf_keys(const float *__restrict__ rows,
     }
 }
 
+// Wide rows (the 64-row LDS tile of k_bf_keys does not fit): one octet per
+// row, rows read once per group of FK_BF_QG queries from the chain-major
+// copy, the group's queries staged in LDS in chain-major form.
+#define FK_BF_QG 8
+#define FK_BF_WROWS 256
+
+__global__ __launch_bounds__(256) void k_bf_keys_wide(const float *__restrict__ chains, uint32_t dpad,
+                                                      uint32_t n_used, const float *__restrict__ qv,
+                                                      uint32_t q_stride, uint32_t n_q,
+                                                      const uint32_t *__restrict__ self_ids,
+                                                      uint64_t *__restrict__ keys) {
+    extern __shared__ __align__(16) float s_q[];  // FK_BF_QG x dpad, chain-major
+    const uint32_t oc = threadIdx.x >> 3, k = threadIdx.x & 7u, nb = dpad >> 3;
+    const uint32_t r0 = blockIdx.x * FK_BF_WROWS;
+    for (uint32_t q0 = 0; q0 < n_q; q0 += FK_BF_QG) {
+        const uint32_t nq = (n_q - q0) < FK_BF_QG ? (n_q - q0) : FK_BF_QG;
+        __syncthreads();  // the previous group is done with s_q
+        for (uint32_t e = threadIdx.x; e < nq * dpad; e += blockDim.x) {
+            const uint32_t qq = e / dpad, x = e - qq * dpad;
+            const uint32_t kk = x / nb, j = x - kk * nb;
+            s_q[e] = qv[(size_t)(q0 + qq) * q_stride + chain_block(j, dpad) + kk];
+        }
+        __syncthreads();
+        for (uint32_t rr = oc; rr < FK_BF_WROWS; rr += (blockDim.x >> 3)) {
+            const uint32_t id = r0 + rr;
+            if (id >= n_used) break;  // octet-uniform
+            const float *rc = chains + (size_t)id * dpad + k * nb;
+            float acc[FK_BF_QG];
+#pragma unroll
+            for (uint32_t qq = 0; qq < FK_BF_QG; ++qq) acc[qq] = 0.f;
+            for (uint32_t j = 0; j < nb; ++j) {
+                const float r = rc[j];
+#pragma unroll
+                for (uint32_t qq = 0; qq < FK_BF_QG; ++qq) {
+                    const float t = s_q[qq * dpad + k * nb + j] - r;
+                    acc[qq] = __builtin_fmaf(t, t, acc[qq]);
+                }
+            }
+#pragma unroll
+            for (uint32_t qq = 0; qq < FK_BF_QG; ++qq) {
+                const float dd = octet_sum(acc[qq]);
+                if (qq < nq && k == 0) {
+                    const uint32_t q = q0 + qq;
+                    keys[(size_t)q * n_used + id] =
+                        (self_ids && self_ids[q] == id) ? FK_KEY_EMPTY : make_key(dd, id);
+                }
+            }
+        }
+    }
+}
+
 #define FK_SEL_THREADS 1024
 #define FK_SEL_MAXK 2048
 
@@ -522,7 +573,9 @@ void bruteforce_knn(hipStream_t s, const Dataset &ds, const float *d_queries, ui
     FK_CHECK_ARG(K >= 1 && K <= FK_SEL_MAXK, "K must be in [1, 2048]");
     FK_CHECK_ARG(n_used >= 1, "empty dataset");
     const size_t lds = (size_t)FK_BF_ROWS * (ds.dpad + 4) * sizeof(float);
-    FK_CHECK_ARG(lds <= 64 * 1024, "dimension too large for the brute-force kernel (d <= 252)");
+    const bool wide = lds > 64 * 1024;
+    const size_t lds_wide = (size_t)FK_BF_QG * ds.dpad * sizeof(float);
+    FK_CHECK_ARG(!wide || lds_wide <= 64 * 1024, "dimension too large for the brute-force kernel (d <= 2048)");
     // Batch queries so the key matrix stays within ~2 GiB.
     size_t qb = ((size_t)2 << 30) / ((size_t)n_used * 8);
     if (qb < 1) qb = 1;
@@ -530,10 +583,17 @@ void bruteforce_knn(hipStream_t s, const Dataset &ds, const float *d_queries, ui
     DevBuf<uint64_t> keys(qb * (size_t)n_used);
     for (uint32_t q0 = 0; q0 < n_q; q0 += (uint32_t)qb) {
         const uint32_t nq = (uint32_t)((n_q - q0) < qb ? (n_q - q0) : qb);
-        const unsigned gx = (n_used + FK_BF_ROWS - 1) / FK_BF_ROWS;
-        hipLaunchKernelGGL(k_bf_keys, dim3(gx), dim3(256), lds, s, ds.rows.p, ds.stride, ds.dpad, n_used,
-                           d_queries + (size_t)q0 * q_stride, q_stride, nq, d_self ? d_self + q0 : nullptr,
-                           keys.p);
+        if (wide) {
+            const unsigned gx = (n_used + FK_BF_WROWS - 1) / FK_BF_WROWS;
+            hipLaunchKernelGGL(k_bf_keys_wide, dim3(gx), dim3(256), lds_wide, s, ds.chains.p, ds.dpad, n_used,
+                               d_queries + (size_t)q0 * q_stride, q_stride, nq, d_self ? d_self + q0 : nullptr,
+                               keys.p);
+        } else {
+            const unsigned gx = (n_used + FK_BF_ROWS - 1) / FK_BF_ROWS;
+            hipLaunchKernelGGL(k_bf_keys, dim3(gx), dim3(256), lds, s, ds.rows.p, ds.stride, ds.dpad, n_used,
+                               d_queries + (size_t)q0 * q_stride, q_stride, nq, d_self ? d_self + q0 : nullptr,
+                               keys.p);
+        }
         FK_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_topk_select, dim3(nq), dim3(FK_SEL_THREADS), 0, s, keys.p, n_used, K,
                            d_out_ids + (size_t)q0 * K, d_out_dists ? d_out_dists + (size_t)q0 * K : nullptr);

@@ -135,7 +135,7 @@ def test_prune_alpha_degenerate_cases(device):
         assert np.array_equal(g[0][t, :c[2][t]], c[0][t, :c[2][t]])
 
 
-@pytest.mark.parametrize("name", ["int128", "unit96"])
+@pytest.mark.parametrize("name", ["int128", "unit96", "float960", "odd20"])
 def test_knn_bruteforce_matches_oracle(device, name):
     x = datasets()[name]
     say("knn", name)
@@ -220,10 +220,11 @@ def test_full_build_invariants_and_recall(device, pg):
     assert rg >= rc - 0.02
 
 
-@pytest.mark.parametrize("name", ["odd44", "odd20"])
+@pytest.mark.parametrize("name", ["odd44", "odd20", "float960"])
 def test_full_build_odd_widths(device, name):
     """Rows whose padded width is not a multiple of 32 (octet distances
-    with the canonical tail blocks, generic chain path)."""
+    with the canonical tail blocks, generic chain path) and wide rows (d =
+    960: tiled join block, wide brute-force controls)."""
     x = datasets()[name]
     say("build odd", name, x.shape)
     device.upload(x)

@@ -1001,197 +1001,266 @@ __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ p
     }
 }
 
-#define FK_BV_HASH 2048
-#define FK_BV_HASH_FILL (FK_BV_HASH * 3 / 4)
-#define FK_BV_CH 512     // candidates gathered between two pool merges
-#define FK_BV_THREADS 256
+// One wavefront per node, FK_BVW_WAVES nodes per workgroup and no block
+// barriers: a node's round is a chain of dependent gathers (pool -> the
+// selected entries' out-lists -> their rows), so many nodes in flight per
+// CU is what hides the latency.  Per wave, in LDS:
+//   u64 p[cap]   the pool (sorted keys), merged in place
+//   u64 c[CH]    candidate keys (distance <= the pool's last at round start)
+//   u32 sel[W], sdeg[W + 1], todo[64];  u8 pf[cap], seln[W]
+// Candidates already in the pool or repeated are dropped at the merge
+// (equal ids carry equal keys: the distance is symmetric and computed in one
+// canonical order), so no id set is kept; every pair the reference evaluates
+// (search_cc) is evaluated.
+#define FK_BVW_WAVES 4
+#define FK_BVW_CH 256
 
-// Dynamic LDS of a BridgeView round, sized by the pool capacity:
-//   u64 p[cap], o[cap], c[CH], c2[CH]
-//   u32 keep[kc], part[256], sel[cap], sdeg[cap + 1], todo[256], table[HASH]
-//   u8  pf[cap], of[kc], sel_new[cap]           (kc = max(cap, CH))
-struct BvLds {
-    uint64_t *p, *o, *c, *c2;
-    uint32_t *keep, *part, *sel, *sdeg, *todo, *table;
-    uint8_t *pf, *of, *sel_new;
-    __device__ __host__ static size_t bytes(uint32_t cap) {
-        const size_t kc = cap > FK_BV_CH ? cap : FK_BV_CH;
-        return 8 * (2 * (size_t)cap + 2 * FK_BV_CH) +
-               4 * (kc + FK_BV_THREADS + cap + cap + 1 + FK_BV_THREADS + FK_BV_HASH) + (cap + kc + cap);
-    }
-    __device__ void carve(uint8_t *base, uint32_t cap) {
-        const uint32_t kc = cap > FK_BV_CH ? cap : FK_BV_CH;
-        p = reinterpret_cast<uint64_t *>(base);
-        o = p + cap;
-        c = o + cap;
-        c2 = c + FK_BV_CH;
-        keep = reinterpret_cast<uint32_t *>(c2 + FK_BV_CH);
-        part = keep + kc;
-        sel = part + FK_BV_THREADS;
-        sdeg = sel + cap;
-        todo = sdeg + cap + 1;
-        table = todo + FK_BV_THREADS;
-        pf = reinterpret_cast<uint8_t *>(table + FK_BV_HASH);
-        of = pf + cap;
-        sel_new = of + kc;
-    }
-};
+__host__ __device__ inline size_t bvw_wave_bytes(uint32_t cap, uint32_t W) {
+    const size_t b = 8 * ((size_t)cap + FK_BVW_CH) + 4 * ((size_t)W + (W + 1) + 64) + (size_t)cap + W;
+    return (b + 15) & ~(size_t)15;
+}
 
-// Inserts id into the round's id set; true when it was not there.  When the
-// table is three-quarters full every id counts as new (the merge still
-// removes duplicates), so the set only ever saves work.
-__device__ __forceinline__ bool bv_set_insert(uint32_t *table, uint32_t *fill, uint32_t id) {
-    if (*(volatile uint32_t *)fill >= FK_BV_HASH_FILL) return true;
-    uint32_t h = (id * 2654435761u) & (FK_BV_HASH - 1);
-    for (uint32_t probe = 0; probe < 64; ++probe) {
-        const uint32_t old = atomicCAS(&table[h], 0xFFFFFFFFu, id);
-        if (old == 0xFFFFFFFFu) {
-            atomicAdd(fill, 1u);
-            return true;
+// Exclusive prefix sum of v over the wave; the wave total in every lane.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t &total) {
+    const uint32_t lane = lane_id();
+    uint32_t x = v;
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - v;
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sorts c[0, n) (n a power of two) ascending; one wave.
+__device__ void wave_bitonic_sort(uint64_t *c, uint32_t n) {
+    const uint32_t lane = lane_id();
+    for (uint32_t k = 2; k <= n; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = lane; i < n; i += 64) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const uint64_t a = c[i], b2 = c[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((a > b2) == up) { c[i] = b2; c[ixj] = a; }
+                }
+            }
+            wave_lds_sync();
         }
-        if (old == id) return false;
-        h = (h + 1) & (FK_BV_HASH - 1);
     }
-    return true;
+}
+
+// Merges the nc candidate keys of c into the pool p / pf (np entries, at
+// most cap kept) in place and returns the new size.  One wave.
+__device__ uint32_t bvw_merge(uint64_t *p, uint8_t *pf, uint32_t np, uint64_t *c, uint32_t nc, uint32_t cap,
+                              uint32_t &inserted) {
+    const uint32_t lane = lane_id();
+    const uint32_t n2 = next_pow2(nc < 2 ? 2 : nc);
+    for (uint32_t x = nc + lane; x < n2; x += 64) c[x] = FK_KEY_EMPTY;
+    wave_lds_sync();
+    wave_bitonic_sort(c, n2);
+    // keep the first of equal keys that is not in the pool; compacted in
+    // place (chunks in order, every lane reads before any lane writes)
+    uint32_t nk = 0;
+    for (uint32_t x0 = 0; x0 < nc; x0 += 64) {
+        const uint32_t x = x0 + lane;
+        uint64_t key = FK_KEY_EMPTY;
+        bool keep = false;
+        if (x < nc) {
+            key = c[x];
+            keep = (x == 0 || c[x - 1] != key) && !sorted_contains(p, np, key);
+        }
+        const uint64_t m = wave_ballot(keep);
+        wave_lds_sync();
+        if (keep) c[nk + lanes_below(m)] = key;
+        nk += (uint32_t)__builtin_popcountll(m);
+        wave_lds_sync();
+    }
+    inserted = nk;
+    if (nk == 0) return np;
+    // candidate positions against the pool before it moves
+    uint32_t cpos[FK_BVW_CH / 64];
+#pragma unroll
+    for (uint32_t r = 0; r < FK_BVW_CH / 64; ++r) {
+        const uint32_t x = r * 64 + lane;
+        cpos[r] = x < nk ? x + lower_bound_cnt(p, np, c[x]) : 0xFFFFFFFFu;
+    }
+    // pool entries move up to j + (#candidates below), last chunk first,
+    // each chunk read before it is written
+    for (uint32_t ch = (np + 63) / 64; ch-- > 0;) {
+        const uint32_t j = ch * 64 + lane;
+        uint64_t key = 0;
+        uint8_t f = 0;
+        uint32_t pos = 0xFFFFFFFFu;
+        if (j < np) {
+            key = p[j];
+            f = pf[j];
+            pos = j + lower_bound_cnt(c, nk, key);
+        }
+        wave_lds_sync();
+        if (pos < cap) { p[pos] = key; pf[pos] = f; }
+        wave_lds_sync();
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < FK_BVW_CH / 64; ++r) {
+        const uint32_t x = r * 64 + lane;
+        if (cpos[r] < cap) { p[cpos[r]] = c[x]; pf[cpos[r]] = (uint8_t)(FK_FLG_NEW | FK_FLG_ISNEW); }
+    }
+    wave_lds_sync();
+    return (np + nk) < cap ? (np + nk) : cap;
 }
 
 template <int NB>
-__global__ __launch_bounds__(FK_BV_THREADS) void k_bv_round(const float *__restrict__ chains, uint32_t dpad,
-                                                            uint32_t N, uint32_t cap, uint32_t W, uint32_t S_cap,
-                                                            uint64_t *__restrict__ pool, uint8_t *__restrict__ pflg,
-                                                            uint32_t *__restrict__ cnt,
-                                                            const uint32_t *__restrict__ nsg_id,
-                                                            const uint32_t *__restrict__ nsg_len,
-                                                            const uint8_t *__restrict__ nsg_new, uint32_t ncap,
-                                                            unsigned long long *__restrict__ cc_acc,
-                                                            const uint32_t *__restrict__ order,
-                                                            uint32_t *__restrict__ mark) {
+__global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
+    const float *__restrict__ chains, uint32_t dpad, uint32_t N, uint32_t cap, uint32_t W, uint32_t S_cap,
+    uint64_t *__restrict__ pool, uint8_t *__restrict__ pflg, uint32_t *__restrict__ cnt,
+    const uint32_t *__restrict__ nsg_id, const uint32_t *__restrict__ nsg_len, const uint8_t *__restrict__ nsg_new,
+    uint32_t ncap, unsigned long long *__restrict__ cc_acc, const uint32_t *__restrict__ order,
+    uint32_t *__restrict__ mark, uint32_t wave_bytes, uint32_t wpb) {
     extern __shared__ __align__(16) uint8_t smem[];
-    BvLds sh;
-    sh.carve(smem, cap);
-    __shared__ uint32_t s_np, s_nsel, s_nc, s_fill, s_ntodo;
-    __shared__ float s_thr;
-    const uint32_t i = node_of_block(order, N);
-    if (i >= N) return;
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint32_t per_xcd = ((N + wpb - 1) / wpb + FK_NUM_XCD - 1) / FK_NUM_XCD;
+    const uint32_t pos = (order ? xcd_position(per_xcd) : blockIdx.x) * wpb + w;
+    if (pos >= N) return;
+    const uint32_t i = order ? order[pos] : pos;
+    uint8_t *wb = smem + (size_t)w * wave_bytes;
+    uint64_t *p = reinterpret_cast<uint64_t *>(wb);
+    uint64_t *c = p + cap;
+    uint32_t *sel = reinterpret_cast<uint32_t *>(c + FK_BVW_CH);
+    uint32_t *sdeg = sel + W;
+    uint32_t *todo = sdeg + W + 1;
+    uint8_t *pf = reinterpret_cast<uint8_t *>(todo + 64);
+    uint8_t *seln = pf + cap;
+
     uint64_t *pr = pool + (uint64_t)i * cap;
-    uint8_t *pf = pflg + (uint64_t)i * cap;
+    uint8_t *pfr = pflg + (uint64_t)i * cap;
     const uint32_t np0 = cnt[i];
     const uint32_t lim = np0 < W ? np0 : W;
-    for (uint32_t j = threadIdx.x; j < np0; j += blockDim.x) { sh.p[j] = pr[j]; sh.pf[j] = pf[j]; }
-    for (uint32_t j = threadIdx.x; j < FK_BV_HASH; j += blockDim.x) sh.table[j] = 0xFFFFFFFFu;
-    __syncthreads();
-    // select flagged window entries (first S_cap in pool order), clear them
-    for (uint32_t j = threadIdx.x; j < lim; j += blockDim.x) sh.keep[j] = (sh.pf[j] & FK_FLG_NEW) ? 1u : 0u;
-    __syncthreads();
-    const uint32_t nflag = block_scan_inplace(sh.keep, lim, sh.part);
-    const uint32_t ns = nflag < S_cap ? nflag : S_cap;
-    for (uint32_t j = threadIdx.x; j < lim; j += blockDim.x) {
-        const uint8_t f = sh.pf[j];
-        const uint32_t r = sh.keep[j];
-        if ((f & FK_FLG_NEW) && r < ns) {
-            const uint32_t q = key_id(sh.p[j]);
-            sh.sel[r] = q;
-            sh.sel_new[r] = (f & FK_FLG_ISNEW) ? 1 : 0;
-            sh.sdeg[r] = nsg_len[q];
-            sh.pf[j] = f & (uint8_t)~FK_FLG_NEW;
+    for (uint32_t j = lane; j < np0; j += 64) { p[j] = pr[j]; pf[j] = pfr[j]; }
+    wave_lds_sync();
+    // select the first S_cap flagged window entries, clear their flags
+    uint32_t ns = 0;
+    for (uint32_t j0 = 0; j0 < lim && ns < S_cap; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        const uint8_t f = j < lim ? pf[j] : (uint8_t)0;
+        const bool flagged = (f & FK_FLG_NEW) != 0;
+        const uint64_t m = wave_ballot(flagged);
+        const uint32_t r = ns + lanes_below(m);
+        if (flagged && r < S_cap) {
+            const uint32_t q = key_id(p[j]);
+            sel[r] = q;
+            seln[r] = (f & FK_FLG_ISNEW) ? 1 : 0;
+            sdeg[r] = nsg_len[q];
+            pf[j] = f & (uint8_t)~FK_FLG_NEW;
             if (mark) mark[q] = 1u;  // profiling: q's out-list is read this round
         }
+        const uint32_t add = (uint32_t)__builtin_popcountll(m);
+        ns = (ns + add) < S_cap ? (ns + add) : S_cap;
     }
-    __syncthreads();
-    const uint32_t tot = block_scan_inplace(sh.sdeg, ns, sh.part);
-    if (threadIdx.x == 0) {
-        sh.sdeg[ns] = tot;
-        s_nsel = ns;
-        s_np = np0;
-        s_nc = 0;
-        s_fill = 0;
-        s_ntodo = 0;
-        s_thr = np0 >= cap ? key_dist(sh.p[cap - 1]) : INFINITY;
+    if (ns == 0) return;  // nothing selected, nothing changed
+    wave_lds_sync();
+    // exclusive prefix of the selected out-degrees
+    uint32_t run = 0;
+    for (uint32_t s0 = 0; s0 < ns; s0 += 64) {
+        const uint32_t x = s0 + lane;
+        const uint32_t v = x < ns ? sdeg[x] : 0u;
+        uint32_t tot;
+        const uint32_t ex = wave_excl_scan(v, tot);
+        wave_lds_sync();
+        if (x < ns) sdeg[x] = run + ex;
+        run += tot;
     }
-    __syncthreads();
-    const uint32_t nsel = s_nsel;
-    if (nsel == 0) return;  // nothing selected, nothing changed
-    // ids already in the pool need no distance
-    for (uint32_t j = threadIdx.x; j < np0; j += blockDim.x) bv_set_insert(sh.table, &s_fill, key_id(sh.p[j]));
-    bv_set_insert(sh.table, &s_fill, i);
-    __syncthreads();
-    const float thr = s_thr;
-    const uint32_t total = sh.sdeg[nsel];
-    // distances: one octet per fresh id, lane k holding chain k of the
-    // owner's row
-    const uint32_t oc = threadIdx.x >> 3, k = threadIdx.x & 7u;
+    if (lane == 0) sdeg[ns] = run;
+    wave_lds_sync();
+    const uint32_t total = run;
+    const float thr = np0 >= cap ? key_dist(p[cap - 1]) : INFINITY;
+    const uint32_t g = lane >> 3, k = lane & 7u;
     const uint32_t nb = NB ? (uint32_t)NB : (dpad >> 3);
     const float *lane_chains = chains + k * nb;
     OctChain<NB> mine;
     mine.load(lane_chains + (size_t)i * dpad, nb);
+    uint32_t np = np0, nc = 0;
     unsigned long long cc = 0;
-    for (uint32_t f0 = 0; f0 < total; f0 += blockDim.x) {
-        // one thread per (entry, neighbour) pair: entry s with
+    bool merged = false;
+    for (uint32_t f0 = 0; f0 < total; f0 += 64) {
+        // one lane per (entry, neighbour) pair: entry s with
         // sdeg[s] <= f < sdeg[s + 1]
-        const uint32_t f = f0 + threadIdx.x;
+        const uint32_t f = f0 + lane;
+        bool use = false;
+        uint32_t t = 0;
         if (f < total) {
-            uint32_t lo = 0, hi = nsel;
+            uint32_t lo = 0, hi = ns;
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (sh.sdeg[mid] <= f) lo = mid; else hi = mid;
+                if (sdeg[mid] <= f) lo = mid; else hi = mid;
             }
-            const uint32_t q = sh.sel[lo];
-            const uint32_t m = f - sh.sdeg[lo];
-            const uint32_t t = nsg_id[(uint64_t)q * ncap + m];
+            const uint32_t q = sel[lo];
+            const uint32_t m = f - sdeg[lo];
+            t = nsg_id[(uint64_t)q * ncap + m];
             const bool tnew = nsg_new[(uint64_t)q * ncap + m] != 0;
-            if (t != i && (sh.sel_new[lo] || tnew)) {
-                ++cc;  // the reference evaluates every such pair (search_cc)
-                if (bv_set_insert(sh.table, &s_fill, t)) sh.todo[atomicAdd(&s_ntodo, 1u)] = t;
-            }
+            use = t != i && (seln[lo] || tnew);
         }
-        __syncthreads();
-        const uint32_t nt = s_ntodo;
-        for (uint32_t e = oc; e < nt; e += (FK_BV_THREADS >> 3)) {  // octet-uniform
-            const uint32_t t = sh.todo[e];
-            const float dd = mine.dist(lane_chains + (size_t)t * dpad, nb);
-            if (k == 0 && dd <= thr) sh.c[atomicAdd(&s_nc, 1u)] = make_key(dd, t);
-        }
-        __syncthreads();
-        const bool last = f0 + blockDim.x >= total;
-        const uint32_t nc = s_nc;
-        if (threadIdx.x == 0) s_ntodo = 0;  // every thread has read it
-        __syncthreads();  // every thread has read nc before it may change
-        if (last || nc + blockDim.x > FK_BV_CH) {
-            if (nc > 0) {
-                const uint32_t m2 = next_pow2(nc < 2 ? 2 : nc);
-                for (uint32_t x = nc + threadIdx.x; x < m2; x += blockDim.x) sh.c[x] = FK_KEY_EMPTY;
-                __syncthreads();
-                block_bitonic_sort(sh.c, m2);
-                const uint32_t npp = s_np;
-                for (uint32_t x = threadIdx.x; x < nc; x += blockDim.x) {
-                    const uint64_t key2 = sh.c[x];
-                    const bool first = (x == 0) || (sh.c[x - 1] != key2);
-                    sh.keep[x] = (first && !sorted_contains(sh.p, npp, key2)) ? 1u : 0u;
-                }
-                __syncthreads();
-                block_merge(sh.p, sh.pf, &s_np, sh.c, sh.keep, nc, cap, (uint8_t)(FK_FLG_NEW | FK_FLG_ISNEW), sh.c2,
-                            sh.o, sh.of, sh.part);
+        const uint64_t mu = wave_ballot(use);
+        const uint32_t nt = (uint32_t)__builtin_popcountll(mu);
+        cc += nt;  // the reference evaluates every such pair (search_cc)
+        if (use) todo[lanes_below(mu)] = t;
+        wave_lds_sync();
+        // one octet per distance
+        for (uint32_t e0 = 0; e0 < nt; e0 += 8) {
+            const uint32_t e = e0 + g;
+            bool ok = false;
+            uint32_t te = 0;
+            float dd = 0.f;
+            if (e < nt) {  // octet-uniform
+                te = todo[e];
+                dd = mine.dist(lane_chains + (size_t)te * dpad, nb);
+                ok = k == 0 && dd <= thr;
             }
-            if (threadIdx.x == 0) s_nc = 0;
-            __syncthreads();
+            const uint64_t mc = wave_ballot(ok);
+            if (ok) c[nc + lanes_below(mc)] = make_key(dd, te);
+            nc += (uint32_t)__builtin_popcountll(mc);
+        }
+        wave_lds_sync();
+        if (nc > FK_BVW_CH - 64 || (f0 + 64 >= total && nc > 0)) {
+            uint32_t ins = 0;
+            np = bvw_merge(p, pf, np, c, nc, cap, ins);
+            merged = merged || ins > 0;
+            nc = 0;
         }
     }
-    const uint32_t np = s_np;
-    for (uint32_t j = threadIdx.x; j < np; j += blockDim.x) { pr[j] = sh.p[j]; pf[j] = sh.pf[j]; }
-    if (threadIdx.x == 0) cnt[i] = np;
-    cc = block_sum(cc);
-    if (threadIdx.x == 0 && cc) atomicAdd(cc_acc, cc);
+    if (merged) {
+        for (uint32_t j = lane; j < np; j += 64) { pr[j] = p[j]; pfr[j] = pf[j]; }
+        if (lane == 0) cnt[i] = np;
+    } else {
+        for (uint32_t j = lane; j < lim; j += 64) pfr[j] = pf[j];  // only flags changed
+    }
+    if (lane == 0 && cc) atomicAdd(cc_acc, cc);  // the same count in every lane
 }
 
 static void launch_bv_round(BuildState &b, uint32_t W, uint32_t scap) {
     const uint32_t *ord = b.has_order ? b.order.p : nullptr;
-    const size_t lds = BvLds::bytes(b.cap);
+    const uint32_t wave_bytes = (uint32_t)bvw_wave_bytes(b.cap, W);
+    uint32_t wpb = (64u * 1024u) / wave_bytes;
+    wpb = wpb < FK_BVW_WAVES ? wpb : FK_BVW_WAVES;
+    FK_CHECK_ARG(wpb >= 1, "pool too large for the BridgeView kernel");
+    const size_t lds = (size_t)wave_bytes * wpb;
+    const uint32_t nblk = (b.N + wpb - 1) / wpb;
+    const unsigned grid = ord ? (nblk + FK_NUM_XCD - 1) / FK_NUM_XCD * FK_NUM_XCD : nblk;
     uint32_t *mark = b.prof ? b.bv_mark.p : nullptr;
-#define FK_BV_LAUNCH(NBV)                                                                                       \
-    hipLaunchKernelGGL(k_bv_round<NBV>, dim3(node_grid(b)), dim3(FK_BV_THREADS), lds, b.s, b.ds->chains.p,     \
-                       b.ds->dpad, b.N,                                                                         \
-                       b.cap, W, scap, b.pool.p, b.pflg.p, b.cnt.p, b.nsg_id.p, b.nsg_len.p, b.nsg_new.p, b.ncap, \
-                       b.acc.p, ord, mark)
+#define FK_BV_LAUNCH(NBV)                                                                                        \
+    hipLaunchKernelGGL(k_bv_round<NBV>, dim3(grid), dim3(64 * wpb), lds, b.s, b.ds->chains.p,                   \
+                       b.ds->dpad, b.N, b.cap, W, scap, b.pool.p, b.pflg.p, b.cnt.p, b.nsg_id.p, b.nsg_len.p,   \
+                       b.nsg_new.p, b.ncap, b.acc.p, ord, mark, wave_bytes, wpb)
     switch (b.ds->dpad) {
         case 32: FK_BV_LAUNCH(4); break;
         case 64: FK_BV_LAUNCH(8); break;

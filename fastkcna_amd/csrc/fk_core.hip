@@ -94,8 +94,10 @@ struct PruneMC {
     static constexpr uint32_t value = NB == 0 ? 0u : ((4096u / (NB * 32u)) < 8u ? (4096u / (NB * 32u)) : 8u);
 };
 
+// LDS per wave: the cached kept chains (mc x dpad), one staging row
+// (dpad), kept ids and distances (R each).
 __host__ __device__ inline uint32_t prune_wave_bytes(uint32_t mc, uint32_t dpad, uint32_t R) {
-    return (mc * dpad * 4u + R * 8u + 15u) & ~15u;
+    return ((mc + 1u) * dpad * 4u + R * 8u + 15u) & ~15u;
 }
 
 template <int RULE, int NB>
@@ -104,6 +106,7 @@ struct PruneWave {
     float tau3, cos_thr;
     const float *lane_chains;  // chain k of row 0
     float *lane_cache;         // chain k of cached kept neighbour 0
+    float *lane_stage;         // chain k of the staging row
     uint32_t *so;
     float *sd;
     uint32_t dpad, nb, g, k, lane, limit;
@@ -132,14 +135,69 @@ struct PruneWave {
         bool occl = false;
         uint32_t mycc = 0;
         __builtin_amdgcn_wave_barrier();
+        const uint32_t n0 = nout;
+        const uint32_t nA = n0 < PruneMC<NB>::value ? n0 : PruneMC<NB>::value;
         if (!skip) {
-            // octet-uniform: every lane of an octet sees the same pid, tid, djk
-            for (uint32_t t = 0; t < nout; ++t) {
+            // the cached kept neighbours decide most candidates; octet-
+            // uniform: every lane of an octet sees the same pid, tid, djk
+            for (uint32_t t = 0; t < nA; ++t) {
                 const uint32_t tid = so[t];
                 if (tid == pid) { occl = true; break; }
                 ++mycc;
                 const float djk = member_dist(crow, t, tid);
                 if (occludes<RULE>(djk, pd, sd[t], tau3, cos_thr)) { occl = true; break; }
+            }
+        }
+        if (n0 > nA) {
+            // the rest, for the survivors one at a time, eight kept
+            // neighbours per step (one per octet) so their rows load
+            // together; the first occluder in kept order ends the check,
+            // and the distances counted are those the sequential loop makes
+            uint64_t surv = wave_ballot(!skip && !occl && k == 0);
+            while (surv) {
+                const uint32_t sl = (uint32_t)__builtin_ctzll(surv);
+                surv &= surv - 1;
+                const uint32_t sg = sl >> 3;
+                const uint32_t spid = __shfl(pid, (int)sl, 64);
+                const float spd = __shfl(pd, (int)sl, 64);
+                const float *cp;
+                if constexpr (NB > 0) {
+                    if (g == sg) crow.store(lane_stage, nb);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    cp = lane_stage;
+                } else {
+                    cp = lane_chains + (size_t)spid * dpad;
+                }
+                uint32_t first = 0xFFFFFFFFu;
+                bool dup = false;
+                for (uint32_t t0 = nA; t0 < n0; t0 += 8) {
+                    const uint32_t t = t0 + g;
+                    bool hit = false, isdup = false;
+                    if (t < n0) {  // octet-uniform
+                        const uint32_t tid = so[t];
+                        if (tid == spid) {
+                            hit = isdup = true;
+                        } else {
+                            const float djk = octet_dist2<NB>(cp, lane_chains + (size_t)tid * dpad, nb);
+                            hit = occludes<RULE>(djk, spd, sd[t], tau3, cos_thr);
+                        }
+                    }
+                    const uint64_t mh = wave_ballot(hit && k == 0);
+                    if (mh) {
+                        const uint32_t fo = (uint32_t)__builtin_ctzll(mh) >> 3;
+                        first = t0 + fo;
+                        dup = __shfl(isdup ? 1 : 0, (int)(fo * 8), 64) != 0;
+                        break;
+                    }
+                }
+                const uint32_t add = first == 0xFFFFFFFFu ? n0 - nA : first - nA + (dup ? 0u : 1u);
+                if (g == sg) {
+                    mycc += add;
+                    occl = first != 0xFFFFFFFFu;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();  // the staging row is free again
             }
         }
         uint32_t done = 0;
@@ -176,7 +234,7 @@ struct PruneWave {
 };
 
 template <int RULE, int NB>
-__global__ __launch_bounds__(256, 5) void k_prune(const float *__restrict__ chains, uint32_t dpad, PruneLaunch a,
+__global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains, uint32_t dpad, PruneLaunch a,
                                                uint32_t wave_bytes) {
     extern __shared__ __align__(16) uint8_t smem[];
     constexpr uint32_t MC = PruneMC<NB>::value;
@@ -206,7 +264,8 @@ __global__ __launch_bounds__(256, 5) void k_prune(const float *__restrict__ chai
     pw.nb = NB ? (uint32_t)NB : (dpad >> 3);
     pw.lane_chains = chains + pw.k * pw.nb;
     pw.lane_cache = reinterpret_cast<float *>(wb) + pw.k * pw.nb;
-    pw.so = reinterpret_cast<uint32_t *>(wb + (size_t)MC * dpad * 4u);
+    pw.lane_stage = reinterpret_cast<float *>(wb) + (size_t)MC * dpad + pw.k * pw.nb;
+    pw.so = reinterpret_cast<uint32_t *>(wb + (size_t)(MC + 1) * dpad * 4u);
     pw.sd = reinterpret_cast<float *>(pw.so + a.R);
     pw.cc = 0;
     pw.exam = 0;

@@ -185,6 +185,30 @@ struct OctChain<0> {
     }
 };
 
+// Octet distance between two chain-major rows given by pointers (this
+// lane's chain of each); a is the minuend, as in OctChain::dist.
+template <int NB>
+__device__ __forceinline__ float octet_dist2(const float *__restrict__ a, const float *__restrict__ b, uint32_t nb) {
+    float acc = 0.f, t;
+    if constexpr (NB > 0) {
+#pragma unroll
+        for (int q = 0; q < NB / 4; ++q) {
+            const float4 x = reinterpret_cast<const float4 *>(a)[q];
+            const float4 y = reinterpret_cast<const float4 *>(b)[q];
+            t = x.x - y.x; acc = __builtin_fmaf(t, t, acc);
+            t = x.y - y.y; acc = __builtin_fmaf(t, t, acc);
+            t = x.z - y.z; acc = __builtin_fmaf(t, t, acc);
+            t = x.w - y.w; acc = __builtin_fmaf(t, t, acc);
+        }
+    } else {
+        for (uint32_t j = 0; j < nb; ++j) {
+            t = a[j] - b[j];
+            acc = __builtin_fmaf(t, t, acc);
+        }
+    }
+    return octet_sum(acc);
+}
+
 // Occlusion rules of the three pruner families; dqt is the kept neighbour's
 // distance to the pool owner, djk the kept neighbour to candidate distance,
 // pd the candidate's distance to the owner.

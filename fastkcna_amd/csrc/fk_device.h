@@ -156,6 +156,18 @@ struct OctChain {
 #pragma unroll
         for (int q = 0; q < NB / 4; ++q) reinterpret_cast<float4 *>(c)[q] = v[q];
     }
+    // distance to another row held in registers
+    __device__ __forceinline__ float dist_reg(const OctChain &o, uint32_t) const {
+        float acc = 0.f, t;
+#pragma unroll
+        for (int q = 0; q < NB / 4; ++q) {
+            t = v[q].x - o.v[q].x; acc = __builtin_fmaf(t, t, acc);
+            t = v[q].y - o.v[q].y; acc = __builtin_fmaf(t, t, acc);
+            t = v[q].z - o.v[q].z; acc = __builtin_fmaf(t, t, acc);
+            t = v[q].w - o.v[q].w; acc = __builtin_fmaf(t, t, acc);
+        }
+        return octet_sum(acc);
+    }
     // distance to the row whose chain (same lane) starts at o
     __device__ __forceinline__ float dist(const float *__restrict__ o, uint32_t) const {
         float acc = 0.f, t;
@@ -175,6 +187,7 @@ struct OctChain<0> {
     const float *p;
     __device__ __forceinline__ void load(const float *__restrict__ c, uint32_t) { p = c; }
     __device__ __forceinline__ void store(float *__restrict__, uint32_t) const {}
+    __device__ __forceinline__ float dist_reg(const OctChain &o, uint32_t nb) const { return dist(o.p, nb); }
     __device__ __forceinline__ float dist(const float *__restrict__ o, uint32_t nb) const {
         float acc = 0.f;
         for (uint32_t j = 0; j < nb; ++j) {

@@ -87,76 +87,6 @@ BuildState::~BuildState() {
     for (auto e : ev_free) (void)hipEventDestroy(e);
 }
 
-// ---------------------------------------------------------------------------
-// Block-wide exclusive scan of n u32 values in LDS (in place); returns total.
-// All threads must call.  s_part needs blockDim.x entries.
-// ---------------------------------------------------------------------------
-__device__ uint32_t block_scan_inplace(uint32_t *v, uint32_t n, uint32_t *s_part) {
-    const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
-    const uint32_t b0 = threadIdx.x * per;
-    uint32_t sum = 0;
-    for (uint32_t k = 0; k < per; ++k)
-        if (b0 + k < n) sum += v[b0 + k];
-    s_part[threadIdx.x] = sum;
-    __syncthreads();
-    for (uint32_t o = 1; o < blockDim.x; o <<= 1) {
-        const uint32_t t = threadIdx.x >= o ? s_part[threadIdx.x - o] : 0u;
-        __syncthreads();
-        s_part[threadIdx.x] += t;
-        __syncthreads();
-    }
-    const uint32_t total = s_part[blockDim.x - 1];
-    uint32_t run = s_part[threadIdx.x] - sum;
-    for (uint32_t k = 0; k < per; ++k)
-        if (b0 + k < n) {
-            const uint32_t x = v[b0 + k];
-            v[b0 + k] = run;
-            run += x;
-        }
-    __syncthreads();
-    return total;
-}
-
-// ---------------------------------------------------------------------------
-// Merge of candidate keys into a pool held in LDS.
-//   s_p / s_pf : pool keys and flags, *s_np entries, sorted ascending
-//   s_c        : nc candidate keys sorted ascending; s_keep[k] = 1 marks the
-//                candidates to insert (distinct and absent from the pool)
-// Keeps the first `lim` entries of the union; inserted entries get flag
-// `newf`.  All threads must call.
-// ---------------------------------------------------------------------------
-#define FK_MRG_CH 1024
-static_assert(FK_MRG_CH <= FK_POOL_MAX, "merge chunk must fit the pool-sized scratch");
-
-__device__ void block_merge(uint64_t *s_p, uint8_t *s_pf, uint32_t *s_np, const uint64_t *s_c, uint32_t *s_keep,
-                            uint32_t nc, uint32_t lim, uint8_t newf, uint64_t *s_c2, uint64_t *s_o, uint8_t *s_of,
-                            uint32_t *s_part) {
-    const uint32_t np = *s_np;
-    // keep flags are parked in s_of while s_keep turns into positions
-    for (uint32_t k = threadIdx.x; k < nc; k += blockDim.x) s_of[k] = (uint8_t)(s_keep[k] ? 1 : 0);
-    __syncthreads();
-    const uint32_t nk = block_scan_inplace(s_keep, nc, s_part);
-    for (uint32_t k = threadIdx.x; k < nc; k += blockDim.x)
-        if (s_of[k]) s_c2[s_keep[k]] = s_c[k];
-    __syncthreads();
-    // rank-merge pool and kept candidates into s_o / s_of; no key occurs in
-    // both lists, so the two rank computations never collide
-    for (uint32_t j = threadIdx.x; j < np; j += blockDim.x) {
-        const uint32_t pos = j + lower_bound_cnt(s_c2, nk, s_p[j]);
-        if (pos < lim) { s_o[pos] = s_p[j]; s_of[pos] = s_pf[j]; }
-    }
-    for (uint32_t k = threadIdx.x; k < nk; k += blockDim.x) {
-        const uint32_t pos = k + lower_bound_cnt(s_p, np, s_c2[k]);
-        if (pos < lim) { s_o[pos] = s_c2[k]; s_of[pos] = newf; }
-    }
-    __syncthreads();
-    const uint32_t nn = (np + nk) < lim ? (np + nk) : lim;
-    for (uint32_t j = threadIdx.x; j < nn; j += blockDim.x) { s_p[j] = s_o[j]; s_pf[j] = s_of[j]; }
-    __syncthreads();
-    if (threadIdx.x == 0) *s_np = nn;
-    __syncthreads();
-}
-
 __device__ __forceinline__ bool sorted_contains(const uint64_t *a, uint32_t n, uint64_t x) {
     const uint32_t i = lower_bound_cnt(a, n, x);
     return i < n && a[i] == x;
@@ -355,20 +285,23 @@ __global__ __launch_bounds__(256) void k_join_fill(const uint32_t *__restrict__ 
 // (src/kgraph.cpp:1111-1114: an insert at position < K).
 // ---------------------------------------------------------------------------
 
-// Merge v2: the chunk's candidates are de-duplicated by id in an LDS hash
-// set before sorting (equal ids carry equal keys: the distance to the
-// target is symmetric and computed in one canonical order), keeping the
-// smallest source join per id; ids already in the pool, and keys that could
-// neither be inserted nor mark a join, are dropped first.  Only the distinct
-// new candidates are sorted and rank-merged.  Same result as sorting every
-// candidate by (key, source) and keeping the first of equal keys.
-#define FK_MRG_HASH (2 * FK_MRG_CH)
+// The chunk's candidates are de-duplicated by id in an LDS hash set before
+// sorting (equal ids carry equal keys: the distance to the target is
+// symmetric and computed in one canonical order), keeping the smallest
+// source join per id; ids already in the pool, and keys that could neither
+// be inserted nor mark a join, are dropped first.  Only the distinct new
+// candidates are sorted, then merged into the pool in place.  Same result as
+// sorting every candidate by (key, source) and keeping the first of equal
+// keys.  LDS: pool keys and flags, FK_NM_CH candidate keys and a 2 x
+// FK_NM_CH-slot id set (16.8 KB at MAXL 500: eight workgroups per CU).
+#define FK_NM_CH 512
+#define FK_NM_HASH (2 * FK_NM_CH)
 
-__device__ __forceinline__ uint32_t mrg_hash(uint32_t id) { return (id * 2654435761u) >> 21; }  // 11 bits
-static_assert(FK_MRG_HASH == 2048, "mrg_hash yields 11 bits");
+__device__ __forceinline__ uint32_t nm_hash(uint32_t id) { return (id * 2654435761u) >> 22; }  // 10 bits
+static_assert(FK_NM_HASH == 1024, "nm_hash yields 10 bits");
 
 size_t nnd_merge_lds(uint32_t cap) {
-    return (size_t)cap * 16 + (size_t)FK_MRG_CH * 8 + (size_t)FK_MRG_HASH * 8 + (size_t)cap * 2;
+    return (size_t)cap * 8 + (size_t)FK_NM_CH * 8 + (size_t)FK_NM_HASH * 8 + (size_t)cap;
 }
 
 __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, uint32_t K, uint64_t *__restrict__ pool,
@@ -379,12 +312,10 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
                                                    const uint32_t *__restrict__ bsrc, uint8_t *__restrict__ found) {
     extern __shared__ __align__(16) uint8_t smem[];
     uint64_t *s_p = reinterpret_cast<uint64_t *>(smem);
-    uint64_t *s_o = s_p + cap;
-    uint64_t *s_u = s_o + cap;
-    uint32_t *s_tid = reinterpret_cast<uint32_t *>(s_u + FK_MRG_CH);
-    uint32_t *s_tsrc = s_tid + FK_MRG_HASH;
-    uint8_t *s_pf = reinterpret_cast<uint8_t *>(s_tsrc + FK_MRG_HASH);
-    uint8_t *s_of = s_pf + cap;
+    uint64_t *s_u = s_p + cap;
+    uint32_t *s_tid = reinterpret_cast<uint32_t *>(s_u + FK_NM_CH);
+    uint32_t *s_tsrc = s_tid + FK_NM_HASH;
+    uint8_t *s_pf = reinterpret_cast<uint8_t *>(s_tsrc + FK_NM_HASH);
     __shared__ uint32_t s_np, s_nu;
     const uint32_t i = blockIdx.x;
     const uint32_t nb = bcnt[i];
@@ -397,9 +328,10 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
     __syncthreads();
     const uint64_t kth = np0 >= K ? s_p[K - 1] : FK_KEY_EMPTY;
     const uint64_t base = boff[i];
-    for (uint32_t c0 = 0; c0 < nb; c0 += FK_MRG_CH) {
-        const uint32_t m = (nb - c0) < FK_MRG_CH ? (nb - c0) : FK_MRG_CH;
-        for (uint32_t j = threadIdx.x; j < FK_MRG_HASH; j += blockDim.x) {
+    uint32_t first_change = 0xFFFFFFFFu;  // smallest pool position rewritten (thread 0's copy)
+    for (uint32_t c0 = 0; c0 < nb; c0 += FK_NM_CH) {
+        const uint32_t m = (nb - c0) < FK_NM_CH ? (nb - c0) : FK_NM_CH;
+        for (uint32_t j = threadIdx.x; j < FK_NM_HASH; j += blockDim.x) {
             s_tid[j] = 0xFFFFFFFFu;
             s_tsrc[j] = 0xFFFFFFFFu;
         }
@@ -412,7 +344,7 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
             if (key >= kth && key >= lth) continue;
             if (sorted_contains(s_p, np, key)) continue;
             const uint32_t id = key_id(key);
-            uint32_t h = mrg_hash(id);
+            uint32_t h = nm_hash(id);
             while (true) {
                 const uint32_t old = atomicCAS(&s_tid[h], 0xFFFFFFFFu, id);
                 if (old == 0xFFFFFFFFu) {
@@ -420,7 +352,7 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
                     break;
                 }
                 if (old == id) break;
-                h = (h + 1) & (FK_MRG_HASH - 1);
+                h = (h + 1) & (FK_NM_HASH - 1);
             }
             atomicMin(&s_tsrc[h], bsrc[base + c0 + k]);
         }
@@ -431,33 +363,62 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
             for (uint32_t k = nu + threadIdx.x; k < m2; k += blockDim.x) s_u[k] = FK_KEY_EMPTY;
             __syncthreads();
             block_bitonic_sort(s_u, m2);
-            for (uint32_t k = threadIdx.x; k < nu; k += blockDim.x) {
-                const uint64_t key = s_u[k];
-                if (key < kth) {
-                    const uint32_t id = key_id(key);
-                    uint32_t h = mrg_hash(id);
-                    while (s_tid[h] != id) h = (h + 1) & (FK_MRG_HASH - 1);
-                    found[s_tsrc[h]] = 1;
+            // found marks, and each candidate's merged position against the
+            // pool before it moves (at most FK_NM_CH / 256 per thread)
+            uint32_t upos[FK_NM_CH / 256];
+#pragma unroll
+            for (uint32_t r = 0; r < FK_NM_CH / 256; ++r) {
+                const uint32_t k = r * 256 + threadIdx.x;
+                upos[r] = 0xFFFFFFFFu;
+                if (k < nu) {
+                    const uint64_t key = s_u[k];
+                    if (key < kth) {
+                        const uint32_t id = key_id(key);
+                        uint32_t h = nm_hash(id);
+                        while (s_tid[h] != id) h = (h + 1) & (FK_NM_HASH - 1);
+                        found[s_tsrc[h]] = 1;
+                    }
+                    upos[r] = k + lower_bound_cnt(s_p, np, key);
                 }
             }
-            // rank merge; no key is in both lists
-            for (uint32_t j = threadIdx.x; j < np; j += blockDim.x) {
-                const uint32_t pos = j + lower_bound_cnt(s_u, nu, s_p[j]);
-                if (pos < Lp) { s_o[pos] = s_p[j]; s_of[pos] = s_pf[j]; }
+            // pool entries move up to j + (#candidates below), last chunk
+            // first, each chunk read before it is written; then the
+            // candidates fill the remaining slots
+            for (uint32_t ch = (np + 255) / 256; ch-- > 0;) {
+                const uint32_t j = ch * 256 + threadIdx.x;
+                uint64_t key = 0;
+                uint8_t f = 0;
+                uint32_t pos = 0xFFFFFFFFu;
+                if (j < np) {
+                    key = s_p[j];
+                    f = s_pf[j];
+                    pos = j + lower_bound_cnt(s_u, nu, key);
+                }
+                __syncthreads();
+                if (pos < Lp) { s_p[pos] = key; s_pf[pos] = f; }
+                __syncthreads();
             }
-            for (uint32_t k = threadIdx.x; k < nu; k += blockDim.x) {
-                const uint32_t pos = k + lower_bound_cnt(s_p, np, s_u[k]);
-                if (pos < Lp) { s_o[pos] = s_u[k]; s_of[pos] = (uint8_t)(FK_FLG_NEW | FK_FLG_ISNEW); }
+#pragma unroll
+            for (uint32_t r = 0; r < FK_NM_CH / 256; ++r)
+                if (upos[r] < Lp) {
+                    s_p[upos[r]] = s_u[r * 256 + threadIdx.x];
+                    s_pf[upos[r]] = (uint8_t)(FK_FLG_NEW | FK_FLG_ISNEW);
+                }
+            if (threadIdx.x == 0) {
+                const uint32_t u0 = upos[0];  // the smallest candidate's position
+                first_change = u0 < first_change ? u0 : first_change;
+                s_np = (np + nu) < Lp ? (np + nu) : Lp;
             }
-            __syncthreads();
-            const uint32_t nn = (np + nu) < Lp ? (np + nu) : Lp;
-            for (uint32_t j = threadIdx.x; j < nn; j += blockDim.x) { s_p[j] = s_o[j]; s_pf[j] = s_of[j]; }
-            if (threadIdx.x == 0) s_np = nn;
         }
         __syncthreads();
     }
+    // entries below the first inserted position are unchanged
+    __shared__ uint32_t s_first;
+    if (threadIdx.x == 0) s_first = first_change;
+    __syncthreads();
     const uint32_t np = s_np;
-    for (uint32_t j = threadIdx.x; j < np; j += blockDim.x) { pr[j] = s_p[j]; pf[j] = s_pf[j]; }
+    const uint32_t st = s_first < np ? s_first : np;
+    for (uint32_t j = st + threadIdx.x; j < np; j += blockDim.x) { pr[j] = s_p[j]; pf[j] = s_pf[j]; }
     if (threadIdx.x == 0) {
         cnt[i] = np;
         thr[i] = np >= Lp ? key_dist(s_p[Lp - 1]) : INFINITY;
@@ -1014,6 +975,7 @@ __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ p
 // (search_cc) is evaluated.
 #define FK_BVW_WAVES 4
 #define FK_BVW_CH 256
+#define FK_BVW_INFLIGHT 1  // rows per octet in flight in the distance step
 
 __host__ __device__ inline size_t bvw_wave_bytes(uint32_t cap, uint32_t W) {
     const size_t b = 8 * ((size_t)cap + FK_BVW_CH) + 4 * ((size_t)W + (W + 1) + 64) + (size_t)cap + W;
@@ -1215,20 +1177,25 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
         cc += nt;  // the reference evaluates every such pair (search_cc)
         if (use) todo[lanes_below(mu)] = t;
         wave_lds_sync();
-        // one octet per distance
-        for (uint32_t e0 = 0; e0 < nt; e0 += 8) {
-            const uint32_t e = e0 + g;
-            bool ok = false;
-            uint32_t te = 0;
-            float dd = 0.f;
-            if (e < nt) {  // octet-uniform
-                te = todo[e];
-                dd = mine.dist(lane_chains + (size_t)te * dpad, nb);
-                ok = k == 0 && dd <= thr;
+        // one octet per distance, FK_BVW_INFLIGHT rows per octet in flight
+        // (slots past nt compute a discarded distance to row 0)
+        for (uint32_t e0 = 0; e0 < nt; e0 += 8 * FK_BVW_INFLIGHT) {
+            OctChain<NB> rr[FK_BVW_INFLIGHT];
+            uint32_t te[FK_BVW_INFLIGHT];
+#pragma unroll
+            for (uint32_t u = 0; u < FK_BVW_INFLIGHT; ++u) {
+                const uint32_t e = e0 + u * 8 + g;
+                te[u] = e < nt ? todo[e] : 0u;
+                rr[u].load(lane_chains + (size_t)te[u] * dpad, nb);
             }
-            const uint64_t mc = wave_ballot(ok);
-            if (ok) c[nc + lanes_below(mc)] = make_key(dd, te);
-            nc += (uint32_t)__builtin_popcountll(mc);
+#pragma unroll
+            for (uint32_t u = 0; u < FK_BVW_INFLIGHT; ++u) {
+                const float dd = mine.dist_reg(rr[u], nb);
+                const bool ok = (e0 + u * 8 + g) < nt && k == 0 && dd <= thr;
+                const uint64_t mc = wave_ballot(ok);
+                if (ok) c[nc + lanes_below(mc)] = make_key(dd, te[u]);
+                nc += (uint32_t)__builtin_popcountll(mc);
+            }
         }
         wave_lds_sync();
         if (nc > FK_BVW_CH - 64 || (f0 + 64 >= total && nc > 0)) {

@@ -126,6 +126,37 @@ def cpu_baseline(n_sample, threads, dev, queries):
                               "cpu_knn_recall_at_maxl": kc, "gpu_knn_recall_at_maxl": kg}}
 
 
+def chosen_leg(dev, p, args, plain_graph, gt, queries, plain_sec):
+    """The same build with the local join in Gram form on the MFMA units
+    (FK_FLAG_GRAM): same steps and warm-up, its graph compared with the plain
+    path's and searched by the same evaluator."""
+    from fastkcna_amd import FK_PG_NSG, _ffi
+    import copy
+    q = copy.copy(p)
+    q.flags = p.flags | _ffi.FK_FLAG_GRAM
+    for _ in range(args.warmup):
+        dev.build(FK_PG_NSG, q)
+    times, st, g = [], None, None
+    for s in range(args.steps):
+        t = time.perf_counter()
+        off, ids, ep, width, st = dev.build(FK_PG_NSG, q)
+        times.append(time.perf_counter() - t)
+        g = (off, ids, ep)
+        log(f"chosen step {s}: build {times[-1]:.3f}s")
+    sec = float(np.mean(times))
+    same = bool(np.array_equal(g[0], plain_graph[0]) and np.array_equal(g[1], plain_graph[1])
+                and g[2] == plain_graph[2])
+    rec = recall_at_10(dev.search(g[0], g[1], g[2], queries, 80, 10)[0], gt)
+    join_ms = st["prof_ms"][_ffi.PROF_FAMILIES.index("join")]
+    out = {"path": "MFMA Gram local join (FK_FLAG_GRAM), FP32 recheck; everything else as the plain path",
+           "gram_mode": _ffi.GRAM_MODE_NAMES[int(st["gram_mode"])], "build_sec": sec,
+           "dist_evals_per_sec": total_dists(st) / sec, "speedup_vs_plain": plain_sec / sec,
+           "recall_at_10": rec, "graph_identical_to_plain": same, "join_ms": join_ms,
+           "gram_pairs": int(st["d_gram"]), "fp32_rechecks": int(st["d_recheck"])}
+    log(f"chosen: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +168,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="MiB of a device-to-device copy to run first")
+    ap.add_argument("--no-chosen", action="store_true", help="skip the chosen (MFMA Gram join) leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the build does not shard "
@@ -183,6 +215,10 @@ def main():
     rec = recall_at_10(res, gt)
     log(f"recall@10 (search_L 80) {rec:.4f}; evaluation {time.perf_counter() - t:.1f}s")
 
+    chosen = None
+    if not args.no_chosen:
+        chosen = chosen_leg(dev, p, args, graph, gt, queries, float(np.mean(times)))
+
     st = stats[-1]
     ms = 1000.0 * float(np.mean(times))
     nd = total_dists(st)
@@ -218,6 +254,7 @@ def main():
                    "phase_sec": {k: round(float(st[k]), 4) for k in ("t_ctrl", "t_nnd", "t_prune", "t_inter",
                                                                     "t_ep", "t_conn", "t_bv")}},
         "roofline": roof,
+        "chosen": chosen,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

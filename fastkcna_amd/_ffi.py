@@ -17,6 +17,11 @@ FK_PG_NSG, FK_PG_HNSW_LAYER, FK_PG_TAUMNG, FK_PG_ALPHAPG = 1, 2, 3, 4
 FK_FLAG_VERBOSE = 0x1
 FK_FLAG_NO_CONTROLS = 0x2
 FK_FLAG_PROFILE = 0x4
+FK_FLAG_GRAM = 0x8
+FK_FLAG_GRAM_BF16 = 0x10
+FK_FLAG_GRAM_F16 = 0x20
+FK_GRAM_NONE, FK_GRAM_EXACT, FK_GRAM_BF16, FK_GRAM_F16 = range(4)
+GRAM_MODE_NAMES = {FK_GRAM_NONE: "none", FK_GRAM_EXACT: "exact", FK_GRAM_BF16: "bf16", FK_GRAM_F16: "f16"}
 FK_MAX_LOOPS = 16
 FK_PROF_N = 16
 PROF_FAMILIES = ["controls", "init", "join", "join_fill", "merge", "update", "prune", "prune_rev", "inter", "bv",
@@ -63,6 +68,7 @@ class FkStats(ctypes.Structure):
         ("aod", u32), ("max_degree", u32), ("min_degree", u32), ("n_attached", u32),
         ("bv_rounds", u32 * FK_MAX_LOOPS),
         ("prof_ms", f64 * FK_PROF_N), ("prof_bytes", f64 * FK_PROF_N), ("prof_launches", u32 * FK_PROF_N),
+        ("gram_mode", u32), ("reserved0", u32), ("d_gram", u64), ("d_recheck", u64),
     ]
 
     def to_dict(self):
@@ -92,6 +98,7 @@ _PROTOS = {
     "fk_build": (ctypes.c_int, [_P, ctypes.c_int, u32, ctypes.POINTER(FkParams), ctypes.POINTER(FkGraph)]),
     "fk_graph_free": (None, [ctypes.POINTER(FkGraph)]),
     "fk_pair_distances": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, _P]),
+    "fk_gram_pair_distances": (ctypes.c_int, [_P, u32, _P, _P, ctypes.c_size_t, _P, _P, _P, ctypes.POINTER(u32)]),
     "fk_join_distances": (ctypes.c_int, [_P, _P, _P, _P, u32, _P, ctypes.POINTER(u64)]),
     "fk_prune": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, u32, _P, u32, u32, ctypes.POINTER(FkParams),
                                 ctypes.c_int, _P, _P, u32, _P, ctypes.POINTER(u64)]),

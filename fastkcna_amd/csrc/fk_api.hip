@@ -126,6 +126,9 @@ fk_status fk_dataset_upload(fk_handle *h, const float *rows, size_t row_stride_b
         Dataset &ds = h->ds;
         ds.rows.release();
         ds.chains.release();
+        ds.gram_nrm.release();
+        ds.gram_fmt = FK_GRAM_NONE;
+        ds.gram_req = 0;
         ds.n_total = 0;
         const uint32_t dpad = (d + 7u) & ~7u;
         ds.rows.alloc((size_t)n_total * dpad);
@@ -154,6 +157,31 @@ fk_status fk_pair_distances(fk_handle *h, const uint32_t *a, const uint32_t *b, 
         FK_HIP(hipMemcpyAsync(db.p, b, n * 4, hipMemcpyHostToDevice, h->stream));
         launch_pair_distances(h->stream, h->ds, da.p, db.p, n, dout.p);
         FK_HIP(hipMemcpyAsync(out, dout.p, n * 4, hipMemcpyDeviceToHost, h->stream));
+        FK_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+fk_status fk_gram_pair_distances(fk_handle *h, uint32_t flags, const uint32_t *a, const uint32_t *b, size_t n,
+                                 float *out, float *lo, float *hi, uint32_t *mode) {
+    if (!h) return FK_E_INVALID;
+    return guard(h, [&] {
+        require_dataset(h);
+        FK_CHECK_ARG(n == 0 || (a && b && out && lo && hi), "null pointer");
+        FK_CHECK_ARG(!((flags & FK_FLAG_GRAM_BF16) && (flags & FK_FLAG_GRAM_F16)), "conflicting Gram formats");
+        for (size_t i = 0; i < n; ++i)
+            FK_CHECK_ARG(a[i] < h->ds.n_total && b[i] < h->ds.n_total, "row id out of range");
+        select_device(h);
+        gram_prepare(h->stream, h->ds, flags);
+        if (mode) *mode = (uint32_t)h->ds.gram_fmt;
+        if (n == 0) return;
+        DevBuf<uint32_t> da(n), db(n);
+        DevBuf<float> dout(3 * n);
+        FK_HIP(hipMemcpyAsync(da.p, a, n * 4, hipMemcpyHostToDevice, h->stream));
+        FK_HIP(hipMemcpyAsync(db.p, b, n * 4, hipMemcpyHostToDevice, h->stream));
+        launch_gram_pairs(h->stream, h->ds, da.p, db.p, n, dout.p, dout.p + n, dout.p + 2 * n);
+        FK_HIP(hipMemcpyAsync(out, dout.p, n * 4, hipMemcpyDeviceToHost, h->stream));
+        FK_HIP(hipMemcpyAsync(lo, dout.p + n, n * 4, hipMemcpyDeviceToHost, h->stream));
+        FK_HIP(hipMemcpyAsync(hi, dout.p + 2 * n, n * 4, hipMemcpyDeviceToHost, h->stream));
         FK_HIP(hipStreamSynchronize(h->stream));
     });
 }

@@ -37,7 +37,7 @@ BuildParams shrink_params(const fk_params &p, uint32_t N);
 struct BuildState {
     fk_handle *h = nullptr;
     hipStream_t s = nullptr;
-    const Dataset *ds = nullptr;
+    Dataset *ds = nullptr;
     uint32_t N = 0;        // oracle.size()
     uint32_t cap = 0;      // pool capacity (MAXL)
     uint32_t ncap = 0;     // out-list capacity
@@ -70,6 +70,7 @@ struct BuildState {
     DevBuf<uint32_t> fcnt_new, fcnt_old, rcnt_new, rcnt_old;  // N each
     DevBuf<uint64_t> roff_new, roff_old;                       // N + 1
     DevBuf<uint32_t> rbuf_new, rbuf_old;
+    uint32_t tmax = 0;          // longest sample list of the current iteration
 
     // sampled ground truth (GenerateControl)
     uint32_t n_ctrl = 0;
@@ -131,6 +132,10 @@ void nnd_alloc(BuildState &b);
 void nnd_controls(BuildState &b);
 void nnd_run(BuildState &b);            // init + iterations (constructor)
 float nnd_evaluate(BuildState &b, uint32_t K);  // recall over controls at K
+
+// Pass 1 of the local join in Gram form (fk_gram.hip); rechecked pairs are
+// added to *n_recheck (device counter)
+void launch_join_gram(BuildState &b, uint32_t tcap, unsigned long long *n_recheck);
 
 // Pool merge and BridgeView (fk_nnd.hip)
 void bridgeview(BuildState &b);

@@ -1,0 +1,401 @@
+// Local join in Gram form on the MFMA units, with an FP32 recheck.
+//
+// The local join (src/kgraph.cpp:970-987) needs, per node, the distance of
+// every (new, new) and (new, old) pair of its sampled list.  Here a node's
+// pairs are 16 x 16 tiles of the Gram matrix G = X Y^T, computed by
+// v_mfma_f32_16x16x16 on 16-bit operands (BF16, or FP16 for unit-normalised
+// rows) with FP32 accumulation, and
+//     d~(x, y) = |x~|^2 + |y~|^2 - 2 G(x, y)
+// with FP32 norms of the rounded rows.  Pools only ever hold exact FP32
+// distances (the canonical order of src/metric.cpp:45-78):
+//   * exact form -- integer-valued rows with |x_i| <= 256 and small norms
+//     (the integer mixtures): the 16-bit operands are the values themselves
+//     and every sum is an integer below 2^24, so d~ IS the canonical
+//     distance, bit for bit, and no recheck is needed;
+//   * bounded forms -- every pair whose canonical distance could be at or
+//     below either endpoint's pool threshold (the join's filter) is
+//     recomputed in FP32 from the staged rows; the others can neither enter
+//     a pool nor mark a join, so the pools come out identical to the plain
+//     FP32 join's.  The bound is rigorous:
+//       |x~ - x| <= u |x| per row (u = 2^-8 BF16, 2^-11 FP16, round to
+//       nearest), so the exact distance a of x, y and a~ of x~, y~ satisfy
+//       |a - a~| <= e = u (|x| + |y|) (+ an FP16 underflow term);
+//       the FP32 evaluation of d~ is within f = f_rel (|x~|^2 + |y~|^2) of
+//       a~^2; and the canonical FP32 distance c is within a factor
+//       (1 +- gam) of a^2.  A pair is skipped only when
+//       (sqrt(max(d~ - f, 0)) - e)^2 (1 - gam) > max(thr_x, thr_y),
+//     which implies c > both thresholds.
+#include <math.h>
+
+#include <algorithm>
+
+#include "fk_build.h"
+#include "fk_device.h"
+#include "fk_internal.h"
+#include "fk_join.h"
+
+namespace fk {
+
+typedef short fk_s16x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 fk_f16x4 __attribute__((ext_vector_type(4)));
+typedef float fk_f32x4 __attribute__((ext_vector_type(4)));
+
+// BF16 bits of x, rounded to nearest even (finite inputs).
+__device__ __forceinline__ uint32_t bf16_bits(float x) {
+    const uint32_t b = __float_as_uint(x);
+    return (b + 0x7FFFu + ((b >> 16) & 1u)) >> 16;
+}
+
+template <int FMT>
+__device__ __forceinline__ float quantize(float x) {
+    if constexpr (FMT == FK_GRAM_F16) return (float)(_Float16)x;
+    else return __uint_as_float(bf16_bits(x) << 16);
+}
+
+// One K-step of a 16 x 16 x 16 tile: a, b are this lane's 4 consecutive
+// values of its A row and B row (already scaled).
+template <int FMT>
+__device__ __forceinline__ fk_f32x4 mfma_step(float4 a, float4 b, fk_f32x4 acc) {
+    if constexpr (FMT == FK_GRAM_F16) {
+        fk_f16x4 ha = {(_Float16)a.x, (_Float16)a.y, (_Float16)a.z, (_Float16)a.w};
+        fk_f16x4 hb = {(_Float16)b.x, (_Float16)b.y, (_Float16)b.z, (_Float16)b.w};
+        return __builtin_amdgcn_mfma_f32_16x16x16f16(ha, hb, acc, 0, 0, 0);
+    } else {
+        fk_s16x4 sa = {(short)bf16_bits(a.x), (short)bf16_bits(a.y), (short)bf16_bits(a.z), (short)bf16_bits(a.w)};
+        fk_s16x4 sb = {(short)bf16_bits(b.x), (short)bf16_bits(b.y), (short)bf16_bits(b.z), (short)bf16_bits(b.w)};
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(sa, sb, acc, 0, 0, 0);
+    }
+}
+
+// 16 x 16 Gram tile.  Lane l supplies row (l & 15) of the A block (arow) and
+// of the B block (brow); it reads the values at k = 16 s + 4 (l >> 4) .. +3
+// for every step s, zero past dpad.  On return lane l holds
+// G[4 (l >> 4) + e][l & 15], e = 0..3 (the MFMA output layout), in the
+// scaled units (scale^2 G).  Every lane of the wave must call it.
+template <int FMT>
+__device__ __forceinline__ fk_f32x4 gram_tile(const float *__restrict__ arow, const float *__restrict__ brow,
+                                              uint32_t dpad, float scale) {
+    fk_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    const uint32_t kq = (lane_id() >> 4) * 4u;
+    const uint32_t steps = (dpad + 15u) >> 4;
+    for (uint32_t s = 0; s < steps; ++s) {
+        const uint32_t k = s * 16u + kq;
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+        if (k < dpad) {
+            a = *reinterpret_cast<const float4 *>(arow + k);
+            b = *reinterpret_cast<const float4 *>(brow + k);
+        }
+        if constexpr (FMT == FK_GRAM_F16) {
+            a.x *= scale; a.y *= scale; a.z *= scale; a.w *= scale;
+            b.x *= scale; b.y *= scale; b.z *= scale; b.w *= scale;
+        }
+        acc = mfma_step<FMT>(a, b, acc);
+    }
+    return acc;
+}
+
+// Error model of one prepared dataset (see the file comment).
+struct GramBound {
+    int exact;        // d~ is the canonical distance
+    float inv_s2;     // 1 / scale^2 (exact power of two)
+    float u;          // relative rounding of a row, with slack
+    float e_abs;      // absolute rounding of a row (FP16 underflow), unscaled
+    float f_rel;      // FP32 evaluation of d~, relative to |x~|^2 + |y~|^2
+    float gam;        // canonical FP32 distance, relative to a^2
+};
+
+// Lower bound on the canonical distance given d~ (0 when nothing is known).
+__device__ __forceinline__ float gram_lower(float dt, float2 nx, float2 ny, const GramBound &g) {
+    const float f = g.f_rel * (nx.x + ny.x);
+    const float e = g.u * (nx.y + ny.y) + g.e_abs;
+    const float lo = __builtin_sqrtf(fmaxf(dt - f, 0.f)) - e;
+    return lo > 0.f ? lo * lo * (1.f - g.gam) : 0.f;
+}
+__device__ __forceinline__ float gram_upper(float dt, float2 nx, float2 ny, const GramBound &g) {
+    const float f = g.f_rel * (nx.x + ny.x);
+    const float e = g.u * (nx.y + ny.y) + g.e_abs;
+    const float hi = __builtin_sqrtf(dt + f) + e;
+    return hi * hi * (1.f + g.gam);
+}
+
+// ---------------------------------------------------------------------------
+// Dataset statistics and per-row norms.
+// ---------------------------------------------------------------------------
+// out[0] AND of "row is integer-valued", out[1] max |x_i| (float bits),
+// out[2] OR of "row has a negative value", out[3] max |x|^2, out[4] min
+// |x|^2 (float bits, rounded outward).
+__global__ __launch_bounds__(256) void k_gram_stats(const float *__restrict__ rows, uint32_t stride, uint32_t d,
+                                                    uint32_t n, uint32_t *__restrict__ out) {
+    uint32_t all_int = 1, neg = 0;
+    float mabs = 0.f, nmax = 0.f, nmin = INFINITY;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float *r = rows + (size_t)i * stride;
+        double s = 0.0;
+        for (uint32_t k = 0; k < d; ++k) {
+            const float v = r[k];
+            all_int &= (v == rintf(v)) ? 1u : 0u;
+            neg |= v < 0.f ? 1u : 0u;
+            mabs = fmaxf(mabs, fabsf(v));
+            s += (double)v * (double)v;
+        }
+        nmax = fmaxf(nmax, (float)(s * (1.0 + 1e-7)));
+        nmin = fminf(nmin, (float)(s * (1.0 - 1e-7)));
+    }
+    atomicAnd(&out[0], all_int);
+    atomicMax(&out[1], __float_as_uint(mabs));
+    atomicOr(&out[2], neg);
+    atomicMax(&out[3], __float_as_uint(nmax));
+    atomicMin(&out[4], __float_as_uint(nmin));
+}
+
+// nrm[i] = (|x~|^2 in FP32 (unscaled), |x| rounded up).
+template <int FMT>
+__global__ __launch_bounds__(256) void k_gram_norms(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
+                                                    uint32_t n, float scale, float inv_s2, float2 *__restrict__ nrm) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *r = rows + (size_t)i * stride;
+    float q2 = 0.f;
+    double s = 0.0;
+    for (uint32_t k = 0; k < dpad; ++k) {
+        const float v = r[k];
+        const float q = quantize<FMT>(FMT == FK_GRAM_F16 ? v * scale : v);
+        q2 = q2 + q * q;
+        s += (double)v * (double)v;
+    }
+    nrm[i] = make_float2(q2 * inv_s2, (float)(sqrt(s) * (1.0 + 1e-6)));
+}
+
+static GramBound gram_bound(const Dataset &ds) {
+    GramBound g{};
+    g.exact = ds.gram_fmt == FK_GRAM_EXACT;
+    g.inv_s2 = 1.f / (ds.gram_scale * ds.gram_scale);
+    if (g.exact) return g;
+    const double dp = ds.dpad;
+    const double u = ds.gram_fmt == FK_GRAM_F16 ? ldexp(1.0, -11) : ldexp(1.0, -8);
+    g.u = (float)(u * 1.001);
+    // FP16: values below the smallest normal are rounded with absolute
+    // error <= 2^-25 (scaled units) per element
+    g.e_abs = ds.gram_fmt == FK_GRAM_F16 ? (float)(2.0 * sqrt(dp) * ldexp(1.0, -25) / ds.gram_scale) : 0.f;
+    // norms: dpad products and sums; Gram: dpad + 16 accumulations; two
+    // final operations
+    g.f_rel = (float)((2.0 * dp + 16.0) * ldexp(1.0, -24) * 1.01 + ldexp(1.0, -21));
+    // canonical distance: subtraction, square and fma per element, dpad / 8
+    // terms per lane and 7 lane sums
+    g.gam = (float)((dp + 16.0) * ldexp(1.0, -23));
+    return g;
+}
+
+void gram_prepare(hipStream_t s, Dataset &ds, uint32_t flags) {
+    const uint32_t want = flags & (FK_FLAG_GRAM_BF16 | FK_FLAG_GRAM_F16);
+    if (ds.gram_fmt != FK_GRAM_NONE && ds.gram_req == want) return;
+    DevBuf<uint32_t> st(8);
+    const uint32_t init[5] = {1u, 0u, 0u, 0u, 0x7F800000u};
+    FK_HIP(hipMemcpyAsync(st.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+    const unsigned grid = std::min<unsigned>(2048, (ds.n_total + 255) / 256);
+    hipLaunchKernelGGL(k_gram_stats, dim3(grid), dim3(256), 0, s, ds.rows.p, ds.stride, ds.d, ds.n_total, st.p);
+    FK_LAUNCH_CHECK();
+    uint32_t h[5];
+    FK_HIP(hipMemcpyAsync(h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    FK_HIP(hipStreamSynchronize(s));
+    float mabs, nmax, nmin;
+    memcpy(&mabs, &h[1], 4);
+    memcpy(&nmax, &h[3], 4);
+    memcpy(&nmin, &h[4], 4);
+    int fmt;
+    if (want & FK_FLAG_GRAM_F16) fmt = FK_GRAM_F16;
+    else if (want & FK_FLAG_GRAM_BF16) fmt = FK_GRAM_BF16;
+    else if (h[0] && mabs <= 256.f && ((!h[2] && nmax < 8388608.f) || nmax < 4194304.f)) fmt = FK_GRAM_EXACT;
+    else if (nmin >= 0.998f && nmax <= 1.002f) fmt = FK_GRAM_F16;  // unit-normalised
+    else fmt = FK_GRAM_BF16;
+    float scale = 1.f;
+    if (fmt == FK_GRAM_F16) {
+        FK_CHECK_ARG(mabs > 0.f && std::isfinite(mabs), "FP16 Gram form needs finite, non-zero rows");
+        // largest power of two keeping every scaled value <= 2^14
+        scale = ldexpf(1.f, 14 - ilogbf(mabs) - 1);
+    }
+    ds.gram_nrm.alloc(ds.n_total);
+    const unsigned g2 = (ds.n_total + 255) / 256;
+    const float inv_s2 = 1.f / (scale * scale);
+    if (fmt == FK_GRAM_F16)
+        hipLaunchKernelGGL(k_gram_norms<FK_GRAM_F16>, dim3(g2), dim3(256), 0, s, ds.rows.p, ds.stride, ds.dpad,
+                           ds.n_total, scale, inv_s2, ds.gram_nrm.p);
+    else
+        hipLaunchKernelGGL(k_gram_norms<FK_GRAM_BF16>, dim3(g2), dim3(256), 0, s, ds.rows.p, ds.stride, ds.dpad,
+                           ds.n_total, scale, inv_s2, ds.gram_nrm.p);
+    FK_LAUNCH_CHECK();
+    FK_HIP(hipStreamSynchronize(s));
+    ds.gram_fmt = fmt;
+    ds.gram_scale = scale;
+    ds.gram_req = want;
+}
+
+// ---------------------------------------------------------------------------
+// Pass 1 of the local join in Gram form: same outputs as k_join_count (pair
+// distances for pass 2, per-slot candidate counts, bucket reservations).
+// One workgroup (4 waves) per node; the sampled rows are staged in LDS in
+// FP32 as in the plain join (tiles of x rows resident, y rows streamed when
+// the list is longer than the staging area); each wave takes 16 x 16 pair
+// tiles round robin.
+// ---------------------------------------------------------------------------
+template <int FMT>
+__global__ __launch_bounds__(256) void k_join_gram(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
+                                                   const float2 *__restrict__ nrm, float scale, GramBound gb,
+                                                   const uint32_t *__restrict__ lists,
+                                                   const uint64_t *__restrict__ list_off,
+                                                   const uint32_t *__restrict__ n_new,
+                                                   const uint64_t *__restrict__ pair_off,
+                                                   const float *__restrict__ thr, float *__restrict__ pdist,
+                                                   uint32_t *__restrict__ resv, uint32_t *__restrict__ bcnt,
+                                                   uint32_t N, const uint32_t *__restrict__ order, uint32_t tcap,
+                                                   unsigned long long *__restrict__ n_recheck) {
+    extern __shared__ __align__(16) float s_rows[];
+    float2 *s_n = reinterpret_cast<float2 *>(s_rows + FK_JOIN_LDS_ROWS_BYTES / 4);
+    float *s_thr = reinterpret_cast<float *>(s_n + tcap);
+    uint32_t *s_c = reinterpret_cast<uint32_t *>(s_thr + tcap);
+    uint32_t *s_id = s_c + tcap;
+    const uint32_t node = node_of_block(order, N);
+    if (node >= N) return;
+    const uint64_t lo = list_off[node];
+    const uint32_t T = (uint32_t)(list_off[node + 1] - lo);
+    const uint32_t nn = n_new[node];
+    const uint32_t *lst = lists + lo;
+    float *o = pdist + pair_off[node];
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
+        const uint32_t id = lst[t];
+        s_id[t] = id;
+        s_thr[t] = thr[id];
+        s_n[t] = nrm[id];
+        s_c[t] = 0;
+    }
+    __syncthreads();
+    const uint32_t ls = dpad + 4;
+    const uint32_t cap = FK_JOIN_LDS_ROWS_BYTES / (ls * 4);
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    uint32_t rechecks = 0;
+    // x chunk [x0, x0 + nx) at s_rows, y chunk [y0, y0 + ny) at s_y
+    auto tiles = [&](uint32_t x0, uint32_t nx, uint32_t y0, uint32_t ny, const float *s_y) {
+        const uint32_t ntx = (nx + 15) >> 4, nty = (ny + 15) >> 4;
+        for (uint32_t t = wave; t < ntx * nty; t += 4) {
+            const uint32_t tx = t / nty, ty = t - tx * nty;
+            // no pair of the tile has y > x
+            if (y0 + ty * 16 + std::min(16u, ny - ty * 16) - 1 <= x0 + tx * 16) continue;
+            const uint32_t ra = std::min(tx * 16 + (lane & 15), nx - 1);
+            const uint32_t rb = std::min(ty * 16 + (lane & 15), ny - 1);
+            const fk_f32x4 g = gram_tile<FMT>(s_rows + ra * ls, s_y + rb * ls, dpad, scale);
+            const uint32_t yi = ty * 16 + (lane & 15);
+#pragma unroll
+            for (uint32_t e = 0; e < 4; ++e) {
+                const uint32_t xi = tx * 16 + (lane >> 4) * 4 + e;
+                const uint32_t x = x0 + xi, y = y0 + yi;
+                if (xi >= nx || yi >= ny || y <= x) continue;
+                const float2 nx2 = s_n[x], ny2 = s_n[y];
+                const float dt = (nx2.x + ny2.x) - 2.f * (g[e] * gb.inv_s2);
+                float dd = dt;
+                if (!gb.exact && gram_lower(dt, nx2, ny2, gb) <= fmaxf(s_thr[x], s_thr[y])) {
+                    dd = l2_canon(s_rows + xi * ls, s_y + yi * ls, dpad);
+                    ++rechecks;
+                }
+                o[join_pair_index(x, y, T)] = dd;
+                if (s_id[x] != s_id[y]) {
+                    if (dd <= s_thr[x]) atomicAdd(&s_c[x], 1u);
+                    if (dd <= s_thr[y]) atomicAdd(&s_c[y], 1u);
+                }
+            }
+        }
+    };
+    if (T <= cap) {
+        join_stage_rows(rows, stride, dpad, s_id, 0, T, s_rows);
+        __syncthreads();
+        tiles(0, nn, 0, T, s_rows);
+    } else {
+        const uint32_t bx = nn < cap / 2 ? nn : cap / 2, by = cap - bx;
+        float *s_y = s_rows + bx * ls;
+        for (uint32_t x0 = 0; x0 < nn; x0 += bx) {
+            const uint32_t nx = (nn - x0) < bx ? (nn - x0) : bx;
+            __syncthreads();  // the previous tiles are done with the x rows
+            join_stage_rows(rows, stride, dpad, s_id, x0, nx, s_rows);
+            for (uint32_t y0 = x0 + 1; y0 < T; y0 += by) {
+                const uint32_t ny = (T - y0) < by ? (T - y0) : by;
+                if (y0 != x0 + 1) __syncthreads();  // the previous tile is done with the y rows
+                join_stage_rows(rows, stride, dpad, s_id, y0, ny, s_y);
+                __syncthreads();
+                tiles(x0, nx, y0, ny, s_y);
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
+        const uint32_t c = s_c[t];
+        resv[lo + t] = c ? atomicAdd(&bcnt[s_id[t]], c) : 0u;
+    }
+    if (!gb.exact) {
+        const unsigned long long r = block_sum((unsigned long long)rechecks);
+        if (threadIdx.x == 0 && r) atomicAdd(n_recheck, r);
+    }
+}
+
+size_t join_gram_lds(uint32_t tcap) { return FK_JOIN_LDS_ROWS_BYTES + (size_t)tcap * 20; }
+
+void launch_join_gram(BuildState &b, uint32_t tcap, unsigned long long *n_recheck) {
+    const Dataset &ds = *b.ds;
+    const GramBound gb = gram_bound(ds);
+    const uint32_t *ord = b.has_order ? b.order.p : nullptr;
+    if (ds.gram_fmt == FK_GRAM_F16)
+        hipLaunchKernelGGL(k_join_gram<FK_GRAM_F16>, dim3(node_grid(b)), dim3(256), join_gram_lds(tcap), b.s,
+                           ds.rows.p, ds.stride, ds.dpad, ds.gram_nrm.p, ds.gram_scale, gb, b.lists.p, b.list_off.p,
+                           b.n_new.p, b.pair_off.p, b.thr.p, b.pdist.p, b.resv.p, b.bcnt.p, b.N, ord, tcap,
+                           n_recheck);
+    else
+        hipLaunchKernelGGL(k_join_gram<FK_GRAM_BF16>, dim3(node_grid(b)), dim3(256), join_gram_lds(tcap), b.s,
+                           ds.rows.p, ds.stride, ds.dpad, ds.gram_nrm.p, ds.gram_scale, gb, b.lists.p, b.list_off.p,
+                           b.n_new.p, b.pair_off.p, b.thr.p, b.pdist.p, b.resv.p, b.bcnt.p, b.N, ord, tcap,
+                           n_recheck);
+    FK_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Gram-form distances of arbitrary pairs with their bounds (the unit entry
+// point): one wavefront per 16 pairs, whose rows form the A and B blocks;
+// the pair distances are the tile's diagonal.
+// ---------------------------------------------------------------------------
+template <int FMT>
+__global__ __launch_bounds__(256) void k_gram_pairs(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
+                                                    const float2 *__restrict__ nrm, float scale, GramBound gb,
+                                                    const uint32_t *__restrict__ a, const uint32_t *__restrict__ b,
+                                                    size_t n, float *__restrict__ out, float *__restrict__ lo,
+                                                    float *__restrict__ hi) {
+    const size_t base = ((size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16;
+    if (base >= n) return;  // whole wave
+    const uint32_t lane = lane_id();
+    const size_t pr = std::min<size_t>(base + (lane & 15), n - 1);
+    const uint32_t ia = a[pr], ib = b[pr];
+    const fk_f32x4 g = gram_tile<FMT>(rows + (size_t)ia * stride, rows + (size_t)ib * stride, dpad, scale);
+    // lane l holds rows 4 (l >> 4) .. +3 of column l & 15: the diagonal
+    // element of column j sits in the lane group j >> 2, entry j & 3
+    const uint32_t j = lane & 15;
+    if ((j >> 2) != (lane >> 4)) return;
+    const size_t p = base + j;
+    if (p >= n) return;
+    const float2 nx2 = nrm[ia], ny2 = nrm[ib];
+    const float dt = (nx2.x + ny2.x) - 2.f * (g[j & 3] * gb.inv_s2);
+    out[p] = dt;
+    lo[p] = gb.exact ? dt : gram_lower(dt, nx2, ny2, gb);
+    hi[p] = gb.exact ? dt : gram_upper(dt, nx2, ny2, gb);
+}
+
+void launch_gram_pairs(hipStream_t s, const Dataset &ds, const uint32_t *a, const uint32_t *b, size_t n,
+                       float *out, float *lo, float *hi) {
+    if (n == 0) return;
+    const GramBound gb = gram_bound(ds);
+    const unsigned grid = (unsigned)((n + 63) / 64);  // 4 waves x 16 pairs
+    if (ds.gram_fmt == FK_GRAM_F16)
+        hipLaunchKernelGGL(k_gram_pairs<FK_GRAM_F16>, dim3(grid), dim3(256), 0, s, ds.rows.p, ds.stride, ds.dpad,
+                           ds.gram_nrm.p, ds.gram_scale, gb, a, b, n, out, lo, hi);
+    else
+        hipLaunchKernelGGL(k_gram_pairs<FK_GRAM_BF16>, dim3(grid), dim3(256), 0, s, ds.rows.p, ds.stride, ds.dpad,
+                           ds.gram_nrm.p, ds.gram_scale, gb, a, b, n, out, lo, hi);
+    FK_LAUNCH_CHECK();
+}
+
+}  // namespace fk

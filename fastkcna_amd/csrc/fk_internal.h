@@ -98,6 +98,12 @@ struct Dataset {
     uint32_t d = 0;
     uint32_t dpad = 0;    // d rounded up to a multiple of 8
     uint32_t stride = 0;  // floats between rows (multiple of 8)
+    // Gram-form join (fk_gram.hip): per-row (|x~|^2, |x|), operand format
+    // (FK_GRAM_*), FP16 prescale and the FK_FLAG_GRAM_* request it was made for
+    DevBuf<float2> gram_nrm;
+    int gram_fmt = 0;
+    float gram_scale = 1.f;
+    uint32_t gram_req = 0;
 };
 
 }  // namespace fk
@@ -145,6 +151,12 @@ void build_chains(hipStream_t s, Dataset &ds);  // fills ds.chains from ds.rows
 void launch_join_distances(hipStream_t s, const Dataset &ds, const uint32_t *ids,
                            const uint64_t *list_off, const uint32_t *n_new, const uint64_t *pair_off,
                            uint32_t n_nodes, uint32_t tmax, float *out);
+
+// Gram-form distances (fk_gram.hip): prepare the dataset for the flags'
+// operand format (cached), and pair distances with their bounds.
+void gram_prepare(hipStream_t s, Dataset &ds, uint32_t flags);
+void launch_gram_pairs(hipStream_t s, const Dataset &ds, const uint32_t *a, const uint32_t *b, size_t n,
+                       float *out, float *lo, float *hi);
 
 // Exclusive prefix sums over device arrays (out may alias in).  `total`
 // (device pointer, may be nullptr) receives the sum; `tmp` is grown as needed

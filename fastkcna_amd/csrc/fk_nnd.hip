@@ -279,29 +279,36 @@ __global__ __launch_bounds__(256) void k_join_fill(const uint32_t *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Bucket merge, one workgroup per target node.  Of equal candidate keys the
-// one of the smallest source join is the one kept.  A kept candidate that
-// ranks before the iteration-start K-th entry marks its source join `found`
-// (src/kgraph.cpp:1111-1114: an insert at position < K).
+// Bucket merge, one workgroup per target node.  The merged pool is the best
+// Lp distinct ids of pool ∪ candidates.  Every candidate id that ends up in
+// it without having been in the iteration-start pool records the smallest
+// source join among all candidates with its key, and if it ranks before the
+// iteration-start K-th entry it marks that join `found`
+// (src/kgraph.cpp:1111-1114: an insert at position < K).  Both are
+// independent of the order of the bucket, so the merge is deterministic.
 // ---------------------------------------------------------------------------
 
-// The chunk's candidates are de-duplicated by id in an LDS hash set before
-// sorting (equal ids carry equal keys: the distance to the target is
-// symmetric and computed in one canonical order), keeping the smallest
-// source join per id; ids already in the pool, and keys that could neither
-// be inserted nor mark a join, are dropped first.  Only the distinct new
-// candidates are sorted, then merged into the pool in place.  Same result as
-// sorting every candidate by (key, source) and keeping the first of equal
-// keys.  LDS: pool keys and flags, FK_NM_CH candidate keys and a 2 x
-// FK_NM_CH-slot id set (16.8 KB at MAXL 500: eight workgroups per CU).
+// The bucket is taken in chunks.  A chunk's candidates are de-duplicated by
+// id in an LDS hash set before sorting (equal ids carry equal keys: the
+// distance to the target is symmetric and computed in one canonical order),
+// keeping the smallest source join per id; candidates that cannot enter the
+// pool are dropped first, and candidates already in the pool only lower the
+// recorded source of an entry inserted by an earlier chunk.  A key that
+// cannot enter the pool at its chunk never can later (the pool only
+// improves), so each inserted entry's recorded source is the minimum over
+// the whole bucket.  Only the distinct new candidates are sorted, then
+// merged into the pool in place.  LDS: pool keys, flags and sources,
+// FK_NM_CH candidate keys and a 2 x FK_NM_CH-slot id set (18.8 KB at
+// MAXL 500: eight workgroups per CU).
 #define FK_NM_CH 512
 #define FK_NM_HASH (2 * FK_NM_CH)
+#define FK_NM_NOSRC 0xFFFFFFFFu
 
 __device__ __forceinline__ uint32_t nm_hash(uint32_t id) { return (id * 2654435761u) >> 22; }  // 10 bits
 static_assert(FK_NM_HASH == 1024, "nm_hash yields 10 bits");
 
 size_t nnd_merge_lds(uint32_t cap) {
-    return (size_t)cap * 8 + (size_t)FK_NM_CH * 8 + (size_t)FK_NM_HASH * 8 + (size_t)cap;
+    return (size_t)cap * 8 + (size_t)FK_NM_CH * 8 + (size_t)FK_NM_HASH * 8 + (size_t)cap * 4 + (size_t)cap;
 }
 
 __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, uint32_t K, uint64_t *__restrict__ pool,
@@ -315,7 +322,8 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
     uint64_t *s_u = s_p + cap;
     uint32_t *s_tid = reinterpret_cast<uint32_t *>(s_u + FK_NM_CH);
     uint32_t *s_tsrc = s_tid + FK_NM_HASH;
-    uint8_t *s_pf = reinterpret_cast<uint8_t *>(s_tsrc + FK_NM_HASH);
+    uint32_t *s_ps = s_tsrc + FK_NM_HASH;  // source of entries inserted by this merge
+    uint8_t *s_pf = reinterpret_cast<uint8_t *>(s_ps + cap);
     __shared__ uint32_t s_np, s_nu;
     const uint32_t i = blockIdx.x;
     const uint32_t nb = bcnt[i];
@@ -323,7 +331,11 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
     uint64_t *pr = pool + (uint64_t)i * cap;
     uint8_t *pf = pflg + (uint64_t)i * cap;
     const uint32_t np0 = cnt[i];
-    for (uint32_t j = threadIdx.x; j < np0; j += blockDim.x) { s_p[j] = pr[j]; s_pf[j] = pf[j]; }
+    for (uint32_t j = threadIdx.x; j < np0; j += blockDim.x) {
+        s_p[j] = pr[j];
+        s_pf[j] = pf[j];
+        s_ps[j] = FK_NM_NOSRC;
+    }
     if (threadIdx.x == 0) s_np = np0;
     __syncthreads();
     const uint64_t kth = np0 >= K ? s_p[K - 1] : FK_KEY_EMPTY;
@@ -333,7 +345,7 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
         const uint32_t m = (nb - c0) < FK_NM_CH ? (nb - c0) : FK_NM_CH;
         for (uint32_t j = threadIdx.x; j < FK_NM_HASH; j += blockDim.x) {
             s_tid[j] = 0xFFFFFFFFu;
-            s_tsrc[j] = 0xFFFFFFFFu;
+            s_tsrc[j] = FK_NM_NOSRC;
         }
         if (threadIdx.x == 0) s_nu = 0;
         __syncthreads();
@@ -341,8 +353,13 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
         const uint64_t lth = np >= Lp ? s_p[Lp - 1] : FK_KEY_EMPTY;
         for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
             const uint64_t key = bkey[base + c0 + k];
-            if (key >= kth && key >= lth) continue;
-            if (sorted_contains(s_p, np, key)) continue;
+            if (key >= lth) continue;
+            const uint32_t src = bsrc[base + c0 + k];
+            const uint32_t at = lower_bound_cnt(s_p, np, key);
+            if (at < np && s_p[at] == key) {
+                if (s_ps[at] != FK_NM_NOSRC) atomicMin(&s_ps[at], src);
+                continue;
+            }
             const uint32_t id = key_id(key);
             uint32_t h = nm_hash(id);
             while (true) {
@@ -354,7 +371,7 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
                 if (old == id) break;
                 h = (h + 1) & (FK_NM_HASH - 1);
             }
-            atomicMin(&s_tsrc[h], bsrc[base + c0 + k]);
+            atomicMin(&s_tsrc[h], src);
         }
         __syncthreads();
         const uint32_t nu = s_nu;
@@ -363,22 +380,23 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
             for (uint32_t k = nu + threadIdx.x; k < m2; k += blockDim.x) s_u[k] = FK_KEY_EMPTY;
             __syncthreads();
             block_bitonic_sort(s_u, m2);
-            // found marks, and each candidate's merged position against the
-            // pool before it moves (at most FK_NM_CH / 256 per thread)
-            uint32_t upos[FK_NM_CH / 256];
+            // each candidate's merged position against the pool before it
+            // moves, and its smallest source (at most FK_NM_CH / 256 per thread)
+            uint32_t upos[FK_NM_CH / 256], usrc[FK_NM_CH / 256];
 #pragma unroll
             for (uint32_t r = 0; r < FK_NM_CH / 256; ++r) {
                 const uint32_t k = r * 256 + threadIdx.x;
                 upos[r] = 0xFFFFFFFFu;
+                usrc[r] = FK_NM_NOSRC;
                 if (k < nu) {
                     const uint64_t key = s_u[k];
-                    if (key < kth) {
+                    upos[r] = k + lower_bound_cnt(s_p, np, key);
+                    if (upos[r] < Lp) {
                         const uint32_t id = key_id(key);
                         uint32_t h = nm_hash(id);
                         while (s_tid[h] != id) h = (h + 1) & (FK_NM_HASH - 1);
-                        found[s_tsrc[h]] = 1;
+                        usrc[r] = s_tsrc[h];
                     }
-                    upos[r] = k + lower_bound_cnt(s_p, np, key);
                 }
             }
             // pool entries move up to j + (#candidates below), last chunk
@@ -388,14 +406,16 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
                 const uint32_t j = ch * 256 + threadIdx.x;
                 uint64_t key = 0;
                 uint8_t f = 0;
+                uint32_t ps = FK_NM_NOSRC;
                 uint32_t pos = 0xFFFFFFFFu;
                 if (j < np) {
                     key = s_p[j];
                     f = s_pf[j];
+                    ps = s_ps[j];
                     pos = j + lower_bound_cnt(s_u, nu, key);
                 }
                 __syncthreads();
-                if (pos < Lp) { s_p[pos] = key; s_pf[pos] = f; }
+                if (pos < Lp) { s_p[pos] = key; s_pf[pos] = f; s_ps[pos] = ps; }
                 __syncthreads();
             }
 #pragma unroll
@@ -403,6 +423,7 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
                 if (upos[r] < Lp) {
                     s_p[upos[r]] = s_u[r * 256 + threadIdx.x];
                     s_pf[upos[r]] = (uint8_t)(FK_FLG_NEW | FK_FLG_ISNEW);
+                    s_ps[upos[r]] = usrc[r];
                 }
             if (threadIdx.x == 0) {
                 const uint32_t u0 = upos[0];  // the smallest candidate's position
@@ -418,7 +439,11 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
     __syncthreads();
     const uint32_t np = s_np;
     const uint32_t st = s_first < np ? s_first : np;
-    for (uint32_t j = st + threadIdx.x; j < np; j += blockDim.x) { pr[j] = s_p[j]; pf[j] = s_pf[j]; }
+    for (uint32_t j = st + threadIdx.x; j < np; j += blockDim.x) {
+        pr[j] = s_p[j];
+        pf[j] = s_pf[j];
+        if (s_ps[j] != FK_NM_NOSRC && s_p[j] < kth) found[s_ps[j]] = 1;
+    }
     if (threadIdx.x == 0) {
         cnt[i] = np;
         thr[i] = np >= Lp ? key_dist(s_p[Lp - 1]) : INFINITY;
@@ -522,13 +547,19 @@ __global__ void k_upd_count(const uint64_t *__restrict__ pool, const uint8_t *__
 
 __global__ void k_list_sizes(const uint32_t *__restrict__ fcnt_new, const uint32_t *__restrict__ fcnt_old,
                              const uint32_t *__restrict__ rcnt_new, const uint32_t *__restrict__ rcnt_old, uint32_t N,
-                             uint32_t R, uint32_t *__restrict__ n_new, uint32_t *__restrict__ lsize) {
+                             uint32_t R, uint32_t *__restrict__ n_new, uint32_t *__restrict__ lsize,
+                             uint32_t *__restrict__ lmax) {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    const uint32_t rn = (R && rcnt_new[n] > R) ? R : rcnt_new[n];
-    const uint32_t ro = (R && rcnt_old[n] > R) ? R : rcnt_old[n];
-    n_new[n] = fcnt_new[n] + rn;
-    lsize[n] = fcnt_new[n] + rn + fcnt_old[n] + ro;
+    uint32_t t = 0;
+    if (n < N) {
+        const uint32_t rn = (R && rcnt_new[n] > R) ? R : rcnt_new[n];
+        const uint32_t ro = (R && rcnt_old[n] > R) ? R : rcnt_old[n];
+        n_new[n] = fcnt_new[n] + rn;
+        t = fcnt_new[n] + rn + fcnt_old[n] + ro;
+        lsize[n] = t;
+    }
+    for (int o = 32; o > 0; o >>= 1) t = max(t, (uint32_t)__shfl_xor((int)t, o, 64));
+    if (lane_id() == 0 && t) atomicMax(lmax, t);
 }
 
 // Forward lists, reverse bucket fill, flag clearing.
@@ -756,7 +787,8 @@ static void nnd_join(BuildState &b) {
     FK_HIP(hipMemcpyAsync(&nlist, b.list_off.p + N, 8, hipMemcpyDeviceToHost, b.s));
     b.sync();
     b.resv.ensure(nlist);
-    const uint32_t tcap = std::max(b.P.L + 2 * b.P.R, 2 * b.P.S);  // longest sample list
+    const uint32_t tcap = std::max(b.tmax, 1u);  // longest sample list of this iteration
+    FK_CHECK_ARG(tcap <= FK_LIST_MAX, "sample list longer than the join supports");
     FK_CHECK_ARG(2ull * (b.ds->dpad + 4) * 4 <= FK_JOIN_LDS_ROWS_BYTES, "rows too wide for the join block");
     if (b.verbose) {
         // list-size profile: which blocks stage their rows in LDS
@@ -782,14 +814,27 @@ static void nnd_join(BuildState &b) {
     const size_t lds = join_count_lds(tcap);
     const uint32_t *ord = b.has_order ? b.order.p : nullptr;
     const double d4 = 4.0 * b.ds->d;
+    const bool gram = (b.P.flags & FK_FLAG_GRAM) != 0;
+    if (gram) FK_HIP(hipMemsetAsync(b.scal.p, 0, 8, b.s));
     hipEvent_t pe = b.prof_begin();
-    hipLaunchKernelGGL(k_join_count, dim3(node_grid(b)), dim3(256), lds, b.s, b.ds->rows.p, b.ds->stride,
-                       b.ds->dpad, b.lists.p, b.list_off.p, b.n_new.p, b.pair_off.p, b.thr.p, b.pdist.p, b.resv.p,
-                       b.bcnt.p, N, ord, tcap);
-    FK_LAUNCH_CHECK();
+    if (gram) {
+        launch_join_gram(b, tcap, (unsigned long long *)b.scal.p);
+    } else {
+        hipLaunchKernelGGL(k_join_count, dim3(node_grid(b)), dim3(256), lds, b.s, b.ds->rows.p, b.ds->stride,
+                           b.ds->dpad, b.lists.p, b.list_off.p, b.n_new.p, b.pair_off.p, b.thr.p, b.pdist.p,
+                           b.resv.p, b.bcnt.p, N, ord, tcap);
+        FK_LAUNCH_CHECK();
+    }
     // sampled rows and ids gathered once per join, one float per pair written
     b.prof_end(pe, FK_PROF_JOIN, (double)nlist * (d4 + 4.0) + 4.0 * (double)pairs);
     const uint64_t cands = scan_to_host(b, b.bcnt.p, b.boff.p, N);
+    if (gram) {
+        uint64_t rc = 0;
+        FK_HIP(hipMemcpyAsync(&rc, b.scal.p, 8, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+        b.st->d_gram += pairs;
+        b.st->d_recheck += rc;
+    }
     b.bkey.ensure(cands);
     b.bsrc.ensure(cands);
     double pool_entries = 0;
@@ -838,10 +883,15 @@ static void nnd_update(BuildState &b, uint32_t it) {
     const uint64_t to = scan_to_host(b, b.rcnt_old.p, b.roff_old.p, N);
     b.rbuf_new.ensure(tn);
     b.rbuf_old.ensure(to);
+    uint32_t *lmax = reinterpret_cast<uint32_t *>(b.scal.p + 1);
+    FK_HIP(hipMemsetAsync(lmax, 0, 4, b.s));
     hipLaunchKernelGGL(k_list_sizes, dim3(g256), dim3(256), 0, b.s, b.fcnt_new.p, b.fcnt_old.p, b.rcnt_new.p,
-                       b.rcnt_old.p, N, b.P.R, b.n_new.p, b.pair_cnt.p);
+                       b.rcnt_old.p, N, b.P.R, b.n_new.p, b.pair_cnt.p, lmax);
     FK_LAUNCH_CHECK();
-    const uint64_t nlist = scan_to_host(b, b.pair_cnt.p, b.list_off.p, N);
+    uint32_t tmax = 0;
+    FK_HIP(hipMemcpyAsync(&tmax, lmax, 4, hipMemcpyDeviceToHost, b.s));
+    const uint64_t nlist = scan_to_host(b, b.pair_cnt.p, b.list_off.p, N);  // synchronises
+    b.tmax = tmax;
     b.lists.ensure(nlist);
     // reverse cursors reuse the bucket and pair counters
     zero_u32(b, b.bcnt, N);
@@ -869,8 +919,13 @@ void nnd_run(BuildState &b) {
     FK_CHECK_ARG(P.S <= P.L, "S must not exceed L");
     FK_CHECK_ARG(P.MAXL <= FK_POOL_MAX, "max(L, K, search_L, search_K) above 1024 is not supported");
     nnd_alloc(b);
+    if (P.flags & FK_FLAG_GRAM) {
+        gram_prepare(b.s, *b.ds, P.flags);
+        b.st->gram_mode = (uint32_t)b.ds->gram_fmt;
+    }
     // init (iteration 1 joins 2S random ids per node, old lists empty)
     b.lists.ensure((size_t)N * 2 * P.S);
+    b.tmax = 2 * P.S;
     {
         std::vector<uint64_t> lo((size_t)N + 1);
         for (size_t i = 0; i <= N; ++i) lo[i] = i * 2 * P.S;

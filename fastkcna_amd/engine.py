@@ -64,6 +64,18 @@ class Device:
         self._check(self.lib.fk_pair_distances(self.h, ptr(a), ptr(b), len(a), ptr(out)), "fk_pair_distances")
         return out
 
+    def gram_pair_distances(self, a, b, flags=0):
+        """MFMA Gram-form distances of resident row pairs: (dist, lo, hi,
+        mode), lo <= canonical FP32 distance <= hi."""
+        a = _as(a, np.uint32)
+        b = _as(b, np.uint32)
+        n = len(a)
+        out, lo, hi = (np.empty(n, np.float32) for _ in range(3))
+        mode = ctypes.c_uint32()
+        self._check(self.lib.fk_gram_pair_distances(self.h, flags, ptr(a), ptr(b), n, ptr(out), ptr(lo), ptr(hi),
+                                                    ctypes.byref(mode)), "fk_gram_pair_distances")
+        return out, lo, hi, mode.value
+
     def join_distances(self, ids, list_off, n_new):
         ids = _as(ids, np.uint32)
         list_off = _as(list_off, np.uint64)

@@ -62,6 +62,22 @@ typedef enum {
 #define FK_FLAG_VERBOSE 0x1u     /* per-iteration log lines on stderr            */
 #define FK_FLAG_NO_CONTROLS 0x2u /* skip the sampled ground truth (recall = 0)   */
 #define FK_FLAG_PROFILE 0x4u     /* time kernel families with HIP events         */
+/* Local join in Gram form on the MFMA units (16-bit operands, FP32
+ * accumulation and norms), every pair that could enter a pool recomputed in
+ * exact FP32: the pools equal the plain FP32 join's.  The operand format is
+ * picked from the data (fk_stats.gram_mode) unless forced. */
+#define FK_FLAG_GRAM 0x8u
+#define FK_FLAG_GRAM_BF16 0x10u  /* force the bounded BF16 form                  */
+#define FK_FLAG_GRAM_F16 0x20u   /* force the bounded FP16 form                  */
+
+/* Gram operand formats (fk_stats.gram_mode, fk_gram_pair_distances). */
+enum {
+    FK_GRAM_NONE = 0,
+    FK_GRAM_EXACT = 1, /* integer rows, |x_i| <= 256, small norms: BF16 operands are
+                          exact and the Gram distance is the FP32 distance */
+    FK_GRAM_BF16 = 2,  /* BF16 operands, FP32 recheck inside a rigorous bound   */
+    FK_GRAM_F16 = 3    /* FP16 operands (power-of-two prescaled), same recheck   */
+};
 
 /* Kernel families timed under FK_FLAG_PROFILE (fk_stats.prof_*). */
 enum {
@@ -119,6 +135,11 @@ typedef struct {
     double prof_ms[FK_PROF_N];
     double prof_bytes[FK_PROF_N];
     uint32_t prof_launches[FK_PROF_N];
+    /* FK_FLAG_GRAM: operand format, join pairs computed on the MFMA units,
+     * and of those the pairs recomputed in FP32 (bounded forms only). */
+    uint32_t gram_mode;
+    uint32_t reserved0;
+    uint64_t d_gram, d_recheck;
 } fk_stats;
 
 /* Output graph in CSR form, allocated by the library (fk_graph_free). */
@@ -158,6 +179,15 @@ void fk_graph_free(fk_graph *g);
  * float_l2sqr_avx summation order (bit-identical results). */
 fk_status fk_pair_distances(fk_handle *h, const uint32_t *a, const uint32_t *b, size_t n,
                             float *out);
+
+/* Gram-form (MFMA) squared L2 of resident rows a[k], b[k], k < n, in the
+ * operand format a build with these flags (FK_FLAG_GRAM_*) would use; *mode
+ * receives it.  lo[k] <= (canonical FP32 distance) <= hi[k] is guaranteed;
+ * in FK_GRAM_EXACT mode out = lo = hi = the canonical distance.  Test and
+ * measurement entry point of the FK_FLAG_GRAM join; no reference
+ * counterpart (the reference has only the FP32 distance, src/metric.cpp:45-78). */
+fk_status fk_gram_pair_distances(fk_handle *h, uint32_t flags, const uint32_t *a, const uint32_t *b, size_t n,
+                                 float *out, float *lo, float *hi, uint32_t *mode);
 
 /* Local-join distances.  For node t < n_nodes, its sample list is
  * ids[list_off[t] .. list_off[t+1]), whose first n_new[t] entries are "new" and

@@ -1,0 +1,157 @@
+"""The Gram-form (MFMA) local join against the plain FP32 join.
+
+Bars: on integer data the Gram distance is the canonical FP32 distance bit for
+bit (exact form); on float data every Gram distance comes with bounds that
+contain the canonical distance, and its error against FP64 is reported; the
+NN-descent pools and the whole NSG build with FK_FLAG_GRAM are bit-identical
+to the plain FP32 path's, in every operand format (the recheck makes the
+candidate sets equal, so pools hold only exact FP32 distances).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from fastkcna_amd import FK_PG_NSG, FK_PG_TAUMNG, IndexParams, _ffi
+from fastkcna_amd.datasets import float_mixture, integer_mixture, unit_mixture
+
+pytestmark = pytest.mark.gpu
+
+GRAM = _ffi.FK_FLAG_GRAM
+BF16 = _ffi.FK_FLAG_GRAM_BF16
+F16 = _ffi.FK_FLAG_GRAM_F16
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def data(name):
+    if name == "int128":
+        return integer_mixture(3000, 128, 16, seed=21)
+    if name == "float960":
+        return float_mixture(1200, 960, 8, seed=22)
+    if name == "unit96":
+        return unit_mixture(3000, 96, 16, seed=23, sigma=0.25)
+    if name == "odd100":
+        return np.random.default_rng(24).normal(size=(2000, 100)).astype(np.float32)
+    if name == "odd20":
+        return (np.random.default_rng(25).normal(size=(2000, 20)) * 1e3).astype(np.float32)
+    raise KeyError(name)
+
+
+def pairs_with_near_duplicates(x, n, rng):
+    a = rng.integers(0, len(x), n)
+    b = rng.integers(0, len(x), n)
+    b[: n // 8] = a[: n // 8]  # identical rows: the worst cancellation
+    return a, b
+
+
+@pytest.mark.parametrize("name", ["int128", "odd20"])
+def test_gram_exact_form_is_canonical(device, name):
+    x = data(name)
+    if name == "odd20":  # integer-valued but signed: exact form needs |x|^2 < 2^22
+        x = np.rint(x / 64.0).astype(np.float32)
+    say("gram exact", name, x.shape)
+    device.upload(x)
+    rng = np.random.default_rng(3)
+    a, b = pairs_with_near_duplicates(x, 20000, rng)
+    g, lo, hi, mode = device.gram_pair_distances(a, b)
+    assert mode == _ffi.FK_GRAM_EXACT
+    c, c64 = oracle.pair_distances(x, a, b, with_f64=True)
+    assert np.array_equal(g.view(np.uint32), c.view(np.uint32))
+    assert np.array_equal(lo, g) and np.array_equal(hi, g)
+    assert np.array_equal(g.astype(np.float64), c64)
+
+
+@pytest.mark.parametrize("name,flags,mode,max_rel", [
+    ("float960", 0, _ffi.FK_GRAM_BF16, 2e-1),
+    ("unit96", 0, _ffi.FK_GRAM_F16, 2e-2),
+    ("odd100", 0, _ffi.FK_GRAM_BF16, 1e-1),
+    ("float960", F16, _ffi.FK_GRAM_F16, 2e-2),
+    ("int128", BF16, _ffi.FK_GRAM_BF16, 1e-1),
+    ("unit96", BF16, _ffi.FK_GRAM_BF16, 1e-1),
+])
+def test_gram_bounds_contain_canonical(device, name, flags, mode, max_rel):
+    x = data(name)
+    say("gram bounds", name, flags)
+    device.upload(x)
+    rng = np.random.default_rng(4)
+    a, b = pairs_with_near_duplicates(x, 40000, rng)
+    g, lo, hi, m = device.gram_pair_distances(a, b, flags)
+    assert m == mode
+    c, c64 = oracle.pair_distances(x, a, b, with_f64=True)
+    assert np.all(lo <= c) and np.all(c <= hi)
+    assert np.all(lo <= hi)
+    # error of the Gram form against FP64, over pairs of distinct rows
+    nz = c64 > 0
+    rel = np.abs(g[nz].astype(np.float64) - c64[nz]) / c64[nz]
+    say(f"  {name} mode {_ffi.GRAM_MODE_NAMES[m]}: rel err vs FP64 median {np.median(rel):.2e} "
+        f"p99 {np.quantile(rel, 0.99):.2e} max {rel.max():.2e}")
+    assert rel.max() < max_rel
+
+
+def nnd_params(**kw):
+    p = IndexParams(K=40, L=40, S=10, R=50, iterations=5, nsg_R=16, search_L=40, search_K=40)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+@pytest.mark.parametrize("name,flags", [
+    ("int128", GRAM),           # exact form
+    ("int128", GRAM | BF16),    # bounded BF16 on the same data
+    ("float960", GRAM),         # bounded BF16
+    ("unit96", GRAM),           # bounded FP16
+    ("odd100", GRAM | F16),     # bounded FP16, dpad % 16 == 8
+])
+def test_gram_nndescent_pools_identical(device, name, flags):
+    x = data(name)
+    say("gram nnd", name, flags)
+    device.upload(x)
+    p = nnd_params()
+    pools0, len0, st0 = device.nndescent(p)
+    pools0b, len0b, _ = device.nndescent(p)
+    assert np.array_equal(pools0, pools0b) and np.array_equal(len0, len0b), "plain NN-descent is not repeatable"
+    p.flags = flags
+    pools1, len1, st1 = device.nndescent(p)
+    assert np.array_equal(len0, len1)
+    assert np.array_equal(pools0, pools1)
+    assert st1["d_gram"] == st0["d_join"] == st1["d_join"]
+    if flags == GRAM and name == "int128":
+        assert st1["gram_mode"] == _ffi.FK_GRAM_EXACT and st1["d_recheck"] == 0
+    else:
+        assert 0 < st1["d_recheck"] <= st1["d_gram"]
+    say(f"  {name}: mode {_ffi.GRAM_MODE_NAMES[st1['gram_mode']]}, {st1['d_gram']} Gram pairs, "
+        f"{st1['d_recheck']} rechecked ({st1['d_recheck'] / max(st1['d_gram'], 1):.3f})")
+
+
+def test_gram_join_tiled_lists_identical(device):
+    """Long sample lists (L + 2R above the LDS staging area) take the tiled
+    path of the Gram join; the pools must still equal the plain join's."""
+    x = data("int128")
+    device.upload(x)
+    p = nnd_params(K=100, L=100, S=20, R=150, iterations=4)
+    pools0, len0, _ = device.nndescent(p)
+    p.flags = GRAM | BF16
+    pools1, len1, st1 = device.nndescent(p)
+    assert np.array_equal(len0, len1) and np.array_equal(pools0, pools1)
+    p.flags = GRAM
+    pools2, len2, _ = device.nndescent(p)
+    assert np.array_equal(len0, len2) and np.array_equal(pools0, pools2)
+
+
+@pytest.mark.parametrize("name,pg", [("int128", FK_PG_NSG), ("float960", FK_PG_TAUMNG)])
+def test_gram_full_build_identical(device, name, pg):
+    x = data(name)
+    say("gram build", name, pg)
+    device.upload(x)
+    p = nnd_params(tau=0.01)
+    off0, ids0, ep0, w0, st0 = device.build(pg, p)
+    p.flags = GRAM
+    off1, ids1, ep1, w1, st1 = device.build(pg, p)
+    assert ep0 == ep1 and w0 == w1
+    assert np.array_equal(off0, off1) and np.array_equal(ids0, ids1)
+    assert st1["d_gram"] > 0

@@ -126,14 +126,13 @@ def cpu_baseline(n_sample, threads, dev, queries):
                               "cpu_knn_recall_at_maxl": kc, "gpu_knn_recall_at_maxl": kg}}
 
 
-def chosen_leg(dev, p, args, plain_graph, gt, queries, plain_sec):
-    """The same build with the local join in Gram form on the MFMA units
-    (FK_FLAG_GRAM): same steps and warm-up, its graph compared with the plain
-    path's and searched by the same evaluator."""
+def variant_leg(dev, p, args, extra_flags, plain_graph, gt, queries, plain_sec, label):
+    """The same build with extra flags: same steps and warm-up, its graph
+    compared with the plain path's and searched by the same evaluator."""
     from fastkcna_amd import FK_PG_NSG, _ffi
     import copy
     q = copy.copy(p)
-    q.flags = p.flags | _ffi.FK_FLAG_GRAM
+    q.flags = p.flags | extra_flags
     for _ in range(args.warmup):
         dev.build(FK_PG_NSG, q)
     times, st, g = [], None, None
@@ -142,18 +141,39 @@ def chosen_leg(dev, p, args, plain_graph, gt, queries, plain_sec):
         off, ids, ep, width, st = dev.build(FK_PG_NSG, q)
         times.append(time.perf_counter() - t)
         g = (off, ids, ep)
-        log(f"chosen step {s}: build {times[-1]:.3f}s")
+        log(f"{label} step {s}: build {times[-1]:.3f}s")
     sec = float(np.mean(times))
     same = bool(np.array_equal(g[0], plain_graph[0]) and np.array_equal(g[1], plain_graph[1])
                 and g[2] == plain_graph[2])
     rec = recall_at_10(dev.search(g[0], g[1], g[2], queries, 80, 10)[0], gt)
-    join_ms = st["prof_ms"][_ffi.PROF_FAMILIES.index("join")]
+    out = {"build_sec": sec, "dist_evals_per_sec": total_dists(st) / sec, "speedup_vs_plain": plain_sec / sec,
+           "recall_at_10": rec, "graph_identical_to_plain": same,
+           "join_ms": st["prof_ms"][_ffi.PROF_FAMILIES.index("join")],
+           "knn_recall_at_maxl": float(st["recall"][st["n_recall"] - 1])}
+    return out, st
+
+
+def chosen_leg(dev, p, args, plain_graph, gt, queries, plain_sec):
+    """The chosen path: the local join in Gram form on the MFMA units."""
+    from fastkcna_amd import _ffi
+    out, st = variant_leg(dev, p, args, _ffi.FK_FLAG_GRAM, plain_graph, gt, queries, plain_sec, "chosen")
     out = {"path": "MFMA Gram local join (FK_FLAG_GRAM), FP32 recheck; everything else as the plain path",
-           "gram_mode": _ffi.GRAM_MODE_NAMES[int(st["gram_mode"])], "build_sec": sec,
-           "dist_evals_per_sec": total_dists(st) / sec, "speedup_vs_plain": plain_sec / sec,
-           "recall_at_10": rec, "graph_identical_to_plain": same, "join_ms": join_ms,
+           "gram_mode": _ffi.GRAM_MODE_NAMES[int(st["gram_mode"])], **out,
            "gram_pairs": int(st["d_gram"]), "fp32_rechecks": int(st["d_recheck"])}
     log(f"chosen: {out}")
+    return out
+
+
+def screen_leg(dev, p, args, plain_graph, gt, queries, plain_sec):
+    """The plain join with the learned distance screen (FK_FLAG_SCREEN)."""
+    from fastkcna_amd import _ffi
+    out, st = variant_leg(dev, p, args, _ffi.FK_FLAG_SCREEN, plain_graph, gt, queries, plain_sec, "screen")
+    out = {"path": "plain FP32 join with the learned distance screen (FK_FLAG_SCREEN)", **out,
+           "screened_pairs": int(st["d_screened"]), "skipped_pairs": int(st["d_screen_skip"]),
+           "est_lost_candidate_frac": 64.0 * st["screen_missed"] / max(st["screen_cands"], 1),
+           "screen_sec": float(st["t_screen"]), "train_pairs": int(st["screen_train_pairs"]),
+           "train_loss": float(st["screen_loss"]), "last_margin": float(st["screen_margin"])}
+    log(f"screen: {out}")
     return out
 
 
@@ -169,6 +189,7 @@ def main():
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="MiB of a device-to-device copy to run first")
     ap.add_argument("--no-chosen", action="store_true", help="skip the chosen (MFMA Gram join) leg")
+    ap.add_argument("--no-screen", action="store_true", help="skip the learned-screen leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the build does not shard "
@@ -215,9 +236,11 @@ def main():
     rec = recall_at_10(res, gt)
     log(f"recall@10 (search_L 80) {rec:.4f}; evaluation {time.perf_counter() - t:.1f}s")
 
-    chosen = None
+    chosen = screen = None
     if not args.no_chosen:
         chosen = chosen_leg(dev, p, args, graph, gt, queries, float(np.mean(times)))
+    if not args.no_screen:
+        screen = screen_leg(dev, p, args, graph, gt, queries, float(np.mean(times)))
 
     st = stats[-1]
     ms = 1000.0 * float(np.mean(times))
@@ -255,6 +278,7 @@ def main():
                                                                     "t_ep", "t_conn", "t_bv")}},
         "roofline": roof,
         "chosen": chosen,
+        "learned_screen": screen,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -20,6 +20,8 @@ FK_FLAG_PROFILE = 0x4
 FK_FLAG_GRAM = 0x8
 FK_FLAG_GRAM_BF16 = 0x10
 FK_FLAG_GRAM_F16 = 0x20
+FK_FLAG_SCREEN = 0x40
+FK_FLAG_SCREEN_NOTRAIN = 0x80
 FK_GRAM_NONE, FK_GRAM_EXACT, FK_GRAM_BF16, FK_GRAM_F16 = range(4)
 GRAM_MODE_NAMES = {FK_GRAM_NONE: "none", FK_GRAM_EXACT: "exact", FK_GRAM_BF16: "bf16", FK_GRAM_F16: "f16"}
 FK_MAX_LOOPS = 16
@@ -69,6 +71,9 @@ class FkStats(ctypes.Structure):
         ("bv_rounds", u32 * FK_MAX_LOOPS),
         ("prof_ms", f64 * FK_PROF_N), ("prof_bytes", f64 * FK_PROF_N), ("prof_launches", u32 * FK_PROF_N),
         ("gram_mode", u32), ("reserved0", u32), ("d_gram", u64), ("d_recheck", u64),
+        ("d_screened", u64), ("d_screen_skip", u64), ("screen_audit", u64), ("screen_missed", u64),
+        ("screen_margin", f32), ("screen_loss", f32), ("screen_train_pairs", u32), ("reserved1", u32),
+        ("t_screen", f64), ("screen_calib_pairs", u64), ("screen_cands", u64),
     ]
 
     def to_dict(self):
@@ -99,6 +104,9 @@ _PROTOS = {
     "fk_graph_free": (None, [ctypes.POINTER(FkGraph)]),
     "fk_pair_distances": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, _P]),
     "fk_gram_pair_distances": (ctypes.c_int, [_P, u32, _P, _P, ctypes.c_size_t, _P, _P, _P, ctypes.POINTER(u32)]),
+    "fk_screen_param_count": (ctypes.c_size_t, [u32]),
+    "fk_screen_grad": (ctypes.c_int, [_P, _P, _P, _P, _P, u32, f32, _P, ctypes.POINTER(f64)]),
+    "fk_screen_predict": (ctypes.c_int, [_P, _P, _P, _P, u32, f32, _P]),
     "fk_join_distances": (ctypes.c_int, [_P, _P, _P, _P, u32, _P, ctypes.POINTER(u64)]),
     "fk_prune": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, u32, _P, u32, u32, ctypes.POINTER(FkParams),
                                 ctypes.c_int, _P, _P, u32, _P, ctypes.POINTER(u64)]),

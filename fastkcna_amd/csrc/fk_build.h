@@ -34,6 +34,18 @@ struct BuildParams {
 
 BuildParams shrink_params(const fk_params &p, uint32_t N);
 
+// Learned distance screen of one build (fk_screen.hip).
+struct ScreenState {
+    bool ready = false;
+    DevBuf<float> w, g, m, v;             // FP32 weights, gradient, Adam moments
+    DevBuf<float> wi;                     // inference weights (BF16-rounded MLP)
+    DevBuf<uint16_t> z;                   // BF16 projections, N x 32
+    DevBuf<unsigned long long> cnt;       // screened, skipped, audited, missed
+    DevBuf<float> rho;                    // recalibration residuals
+    DevBuf<uint32_t> rcnt;
+    float mu = 1.f, margin = 0.f;
+};
+
 struct BuildState {
     fk_handle *h = nullptr;
     hipStream_t s = nullptr;
@@ -71,6 +83,7 @@ struct BuildState {
     DevBuf<uint64_t> roff_new, roff_old;                       // N + 1
     DevBuf<uint32_t> rbuf_new, rbuf_old;
     uint32_t tmax = 0;          // longest sample list of the current iteration
+    ScreenState scr;
 
     // sampled ground truth (GenerateControl)
     uint32_t n_ctrl = 0;
@@ -137,6 +150,15 @@ float nnd_evaluate(BuildState &b, uint32_t K);  // recall over controls at K
 // added to *n_recheck (device counter)
 void launch_join_gram(BuildState &b, uint32_t tcap, unsigned long long *n_recheck);
 
+// Learned distance screen (fk_screen.hip): training on the first join's
+// pairs, the screened join pass 1, and the per-iteration counters.
+void screen_train(BuildState &b);
+void screen_calibrate(BuildState &b);
+void launch_join_screen(BuildState &b, uint32_t tcap);
+size_t join_screen_lds(uint32_t tcap);
+#define FK_JOIN_SCREEN_LDS_MAX (64 * 1024)  // longer lists join unscreened
+void screen_collect(BuildState &b);
+
 // Pool merge and BridgeView (fk_nnd.hip)
 void bridgeview(BuildState &b);
 
@@ -160,6 +182,7 @@ void build_loops(BuildState &b, fk_pg_type type);
 
 // Search / graph utilities (fk_search.hip)
 void compute_ep(BuildState &b);
+void column_mean(hipStream_t s, const Dataset &ds, float *center);
 void tree_grow(BuildState &b);
 
 double now_sec();

@@ -398,6 +398,8 @@ static void check_params(const fk_params *p) {
     FK_CHECK_ARG((uint64_t)p->L + 2ull * p->R <= FK_LIST_MAX, "L + 2R must not exceed 2048");
     FK_CHECK_ARG(p->massq_S >= 1, "massq_S must be positive");
     FK_CHECK_ARG(!((p->flags & FK_FLAG_GRAM_BF16) && (p->flags & FK_FLAG_GRAM_F16)), "conflicting Gram formats");
+    FK_CHECK_ARG(!((p->flags & FK_FLAG_GRAM) && (p->flags & FK_FLAG_SCREEN)),
+                 "FK_FLAG_SCREEN and FK_FLAG_GRAM cannot be combined");
 }
 
 static void stats_to_out(const BuildState &b, fk_graph *out, std::vector<uint64_t> &off, uint32_t *ids) {

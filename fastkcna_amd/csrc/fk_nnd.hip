@@ -816,9 +816,13 @@ static void nnd_join(BuildState &b) {
     const double d4 = 4.0 * b.ds->d;
     const bool gram = (b.P.flags & FK_FLAG_GRAM) != 0;
     if (gram) FK_HIP(hipMemsetAsync(b.scal.p, 0, 8, b.s));
+    const bool screen = b.scr.ready && join_screen_lds(tcap) <= FK_JOIN_SCREEN_LDS_MAX;
+    if (screen) screen_calibrate(b);
     hipEvent_t pe = b.prof_begin();
     if (gram) {
         launch_join_gram(b, tcap, (unsigned long long *)b.scal.p);
+    } else if (screen) {
+        launch_join_screen(b, tcap);
     } else {
         hipLaunchKernelGGL(k_join_count, dim3(node_grid(b)), dim3(256), lds, b.s, b.ds->rows.p, b.ds->stride,
                            b.ds->dpad, b.lists.p, b.list_off.p, b.n_new.p, b.pair_off.p, b.thr.p, b.pdist.p,
@@ -828,6 +832,10 @@ static void nnd_join(BuildState &b) {
     // sampled rows and ids gathered once per join, one float per pair written
     b.prof_end(pe, FK_PROF_JOIN, (double)nlist * (d4 + 4.0) + 4.0 * (double)pairs);
     const uint64_t cands = scan_to_host(b, b.bcnt.p, b.boff.p, N);
+    if (screen) {
+        screen_collect(b);
+        b.st->screen_cands += cands;
+    }
     if (gram) {
         uint64_t rc = 0;
         FK_HIP(hipMemcpyAsync(&rc, b.scal.p, 8, hipMemcpyDeviceToHost, b.s));
@@ -944,6 +952,8 @@ void nnd_run(BuildState &b) {
     for (uint32_t it = 0; it < P.iterations; ++it) {
         b.st->nnd_iterations++;
         nnd_join(b);
+        // the screen learns from the first join's exact pair distances
+        if (it == 0 && (P.flags & FK_FLAG_SCREEN) && P.iterations > 1) screen_train(b);
         // statistics
         FK_HIP(hipMemsetAsync(b.acc.p, 0, 8, b.s));
         hipLaunchKernelGGL(k_delta, dim3(reduce_grid(N)), dim3(1024), 0, b.s, b.pflg.p, b.cnt.p, N, b.cap, P.K, b.acc.p);

@@ -212,6 +212,19 @@ static void launch_beam(hipStream_t s, const BeamArgs &a) {
     FK_LAUNCH_CHECK();
 }
 
+// Column means of all resident rows into center[0, d) (the caller zeroes the
+// padding); synchronises the stream.
+void column_mean(hipStream_t s, const Dataset &ds, float *center) {
+    const uint32_t nb = (ds.n_total + FK_CEN_ROWS - 1) / FK_CEN_ROWS;
+    DevBuf<double> part((size_t)nb * ds.d);
+    hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, s, ds.rows.p, ds.stride, ds.d, ds.n_total, part.p);
+    FK_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_colsum_final, dim3((ds.d + 255) / 256), dim3(256), 0, s, part.p, nb, ds.d, ds.n_total,
+                       center);
+    FK_LAUNCH_CHECK();
+    FK_HIP(hipStreamSynchronize(s));
+}
+
 // ComputeEp (src/kgraph.cpp:1528-1536).
 void compute_ep(BuildState &b) {
     const double t0 = now_sec();
@@ -219,16 +232,7 @@ void compute_ep(BuildState &b) {
     DevBuf<float> center(ds.dpad);
     FK_HIP(hipMemsetAsync(center.p, 0, center.bytes(), b.s));
     hipEvent_t pe = b.prof_begin();
-    {
-        const uint32_t nb = (ds.n_total + FK_CEN_ROWS - 1) / FK_CEN_ROWS;
-        DevBuf<double> part((size_t)nb * ds.d);
-        hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, b.s, ds.rows.p, ds.stride, ds.d, ds.n_total, part.p);
-        FK_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_colsum_final, dim3((ds.d + 255) / 256), dim3(256), 0, b.s, part.p, nb, ds.d,
-                           ds.n_total, center.p);
-        FK_LAUNCH_CHECK();
-        b.sync();
-    }
+    column_mean(b.s, ds, center.p);
     DevBuf<uint32_t> vis((b.N + 31) / 32);
     FK_HIP(hipMemsetAsync(vis.p, 0, vis.bytes(), b.s));
     DevBuf<uint32_t> oid(1);

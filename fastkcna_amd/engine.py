@@ -76,6 +76,27 @@ class Device:
                                                     ctypes.byref(mode)), "fk_gram_pair_distances")
         return out, lo, hi, mode.value
 
+    def screen_param_count(self):
+        return int(self.lib.fk_screen_param_count(self.d))
+
+    def screen_grad(self, weights, a, b, dist, inv_mu):
+        """Gradient and loss of one training batch of the learned screen."""
+        w = _as(weights, np.float32)
+        a, b, dist = _as(a, np.uint32), _as(b, np.uint32), _as(dist, np.float32)
+        g = np.empty_like(w)
+        loss = ctypes.c_double()
+        self._check(self.lib.fk_screen_grad(self.h, ptr(w), ptr(a), ptr(b), ptr(dist), len(a), inv_mu, ptr(g),
+                                            ctypes.byref(loss)), "fk_screen_grad")
+        return g, loss.value
+
+    def screen_predict(self, weights, a, b, inv_mu):
+        w = _as(weights, np.float32)
+        a, b = _as(a, np.uint32), _as(b, np.uint32)
+        out = np.empty(len(a), np.float32)
+        self._check(self.lib.fk_screen_predict(self.h, ptr(w), ptr(a), ptr(b), len(a), inv_mu, ptr(out)),
+                    "fk_screen_predict")
+        return out
+
     def join_distances(self, ids, list_off, n_new):
         ids = _as(ids, np.uint32)
         list_off = _as(list_off, np.uint64)

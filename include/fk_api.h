@@ -70,6 +70,17 @@ typedef enum {
 #define FK_FLAG_GRAM_BF16 0x10u  /* force the bounded BF16 form                  */
 #define FK_FLAG_GRAM_F16 0x20u   /* force the bounded FP16 form                  */
 
+/* Learned distance screen in the local join (NN-descent iterations 2+): a
+ * d -> 32 projection and a 32-64-1 MLP, trained during the build on the first
+ * iteration's exact pair distances, skip the exact distance of pairs it
+ * predicts cannot enter either pool (recalibrated before each join on a
+ * sample of its own pairs to lose about 1e-3 of the pairs a threshold would
+ * admit; audited in fk_stats).  NOTRAIN keeps the
+ * initial weights: the plain random-projection estimate, calibrated the same
+ * way.  Not combinable with FK_FLAG_GRAM. */
+#define FK_FLAG_SCREEN 0x40u
+#define FK_FLAG_SCREEN_NOTRAIN 0x80u
+
 /* Gram operand formats (fk_stats.gram_mode, fk_gram_pair_distances). */
 enum {
     FK_GRAM_NONE = 0,
@@ -140,6 +151,17 @@ typedef struct {
     uint32_t gram_mode;
     uint32_t reserved0;
     uint64_t d_gram, d_recheck;
+    /* FK_FLAG_SCREEN: join pairs the screen evaluated, pairs it skipped, the
+     * skipped pairs audited with the exact distance, and the pool candidates
+     * those audited pairs would have given (each counted per endpoint). */
+    uint64_t d_screened, d_screen_skip, screen_audit, screen_missed;
+    float screen_margin, screen_loss;  /* calibrated relative margin, final training loss */
+    uint32_t screen_train_pairs, reserved1;
+    double t_screen;                   /* training, projection and calibration seconds */
+    uint64_t screen_calib_pairs;       /* admitted pairs in the per-join calibration samples */
+    uint64_t screen_cands;             /* pool candidates of the screened joins; one skipped
+                                          pair in 64 is audited, so about 64 * screen_missed
+                                          admitted candidates were lost to the screen */
 } fk_stats;
 
 /* Output graph in CSR form, allocated by the library (fk_graph_free). */
@@ -188,6 +210,19 @@ fk_status fk_pair_distances(fk_handle *h, const uint32_t *a, const uint32_t *b, 
  * counterpart (the reference has only the FP32 distance, src/metric.cpp:45-78). */
 fk_status fk_gram_pair_distances(fk_handle *h, uint32_t flags, const uint32_t *a, const uint32_t *b, size_t n,
                                  float *out, float *lo, float *hi, uint32_t *mode);
+
+/* Learned screen, test entry points (no reference counterpart).  Weights are
+ * fk_screen_param_count(d) floats: P^T [dpad][32] | W1 [64][32] | b1 [64] |
+ * w2 [64] | b2, dpad = d rounded up to 8.  fk_screen_grad runs the forward and
+ * backward kernels of one training batch (n <= 65536 pairs of resident rows,
+ * targets dist, inv_mu = 1 / distance scale) and returns the gradient of
+ * loss = mean ((y - t) / t)^2, t = max(dist * inv_mu, 0.01), and the loss;
+ * fk_screen_predict returns y (the predicted distance times inv_mu) in FP32. */
+size_t fk_screen_param_count(uint32_t d);
+fk_status fk_screen_grad(fk_handle *h, const float *weights, const uint32_t *a, const uint32_t *b,
+                         const float *dist, uint32_t n, float inv_mu, float *grad, double *loss);
+fk_status fk_screen_predict(fk_handle *h, const float *weights, const uint32_t *a, const uint32_t *b,
+                            uint32_t n, float inv_mu, float *out);
 
 /* Local-join distances.  For node t < n_nodes, its sample list is
  * ids[list_off[t] .. list_off[t+1]), whose first n_new[t] entries are "new" and

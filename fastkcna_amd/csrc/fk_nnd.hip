@@ -1030,7 +1030,10 @@ __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ p
 // One wavefront per node, FK_BVW_WAVES nodes per workgroup and no block
 // barriers: a node's round is a chain of dependent gathers (pool -> the
 // selected entries' out-lists -> their rows), so many nodes in flight per
-// CU is what hides the latency.  Per wave, in LDS:
+// CU is what hides the latency: the out-list entries of up to 256 pairs are
+// read with independent loads, and only the search window of the pool is
+// read up front (the rest only when candidates are merged into it, which a
+// round with few or no candidates never does).  Per wave, in LDS:
 //   u64 p[cap]   the pool (sorted keys), merged in place
 //   u64 c[CH]    candidate keys (distance <= the pool's last at round start)
 //   u32 sel[W], sdeg[W + 1], todo[64];  u8 pf[cap], seln[W]
@@ -1040,7 +1043,12 @@ __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ p
 // (search_cc) is evaluated.
 #define FK_BVW_WAVES 4
 #define FK_BVW_CH 256
-#define FK_BVW_INFLIGHT 1  // rows per octet in flight in the distance step
+#ifndef FK_BVW_INFLIGHT
+#define FK_BVW_INFLIGHT 2  // rows per octet in flight in the distance step
+#endif
+#ifndef FK_BVW_PREFETCH
+#define FK_BVW_PREFETCH 4  // chunks of 64 out-list entries read at once
+#endif
 
 __host__ __device__ inline size_t bvw_wave_bytes(uint32_t cap, uint32_t W) {
     const size_t b = 8 * ((size_t)cap + FK_BVW_CH) + 4 * ((size_t)W + (W + 1) + 64) + (size_t)cap + W;
@@ -1173,7 +1181,9 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
     uint8_t *pfr = pflg + (uint64_t)i * cap;
     const uint32_t np0 = cnt[i];
     const uint32_t lim = np0 < W ? np0 : W;
-    for (uint32_t j = lane; j < np0; j += 64) { p[j] = pr[j]; pf[j] = pfr[j]; }
+    // the window only; the rest of the pool is read when there is something
+    // to merge into it
+    for (uint32_t j = lane; j < lim; j += 64) { p[j] = pr[j]; pf[j] = pfr[j]; }
     wave_lds_sync();
     // select the first S_cap flagged window entries, clear their flags
     uint32_t ns = 0;
@@ -1210,7 +1220,7 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
     if (lane == 0) sdeg[ns] = run;
     wave_lds_sync();
     const uint32_t total = run;
-    const float thr = np0 >= cap ? key_dist(p[cap - 1]) : INFINITY;
+    const float thr = np0 >= cap ? key_dist(pr[cap - 1]) : INFINITY;
     const uint32_t g = lane >> 3, k = lane & 7u;
     const uint32_t nb = NB ? (uint32_t)NB : (dpad >> 3);
     const float *lane_chains = chains + k * nb;
@@ -1218,56 +1228,73 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
     mine.load(lane_chains + (size_t)i * dpad, nb);
     uint32_t np = np0, nc = 0;
     unsigned long long cc = 0;
-    bool merged = false;
-    for (uint32_t f0 = 0; f0 < total; f0 += 64) {
-        // one lane per (entry, neighbour) pair: entry s with
+    bool merged = false, loaded = false;
+    for (uint32_t s0 = 0; s0 < total; s0 += 64 * FK_BVW_PREFETCH) {
+        // the out-list entries of FK_BVW_PREFETCH chunks of 64 (entry,
+        // neighbour) pairs are read at once; pair f belongs to entry s with
         // sdeg[s] <= f < sdeg[s + 1]
-        const uint32_t f = f0 + lane;
-        bool use = false;
-        uint32_t t = 0;
-        if (f < total) {
-            uint32_t lo = 0, hi = ns;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (sdeg[mid] <= f) lo = mid; else hi = mid;
-            }
-            const uint32_t q = sel[lo];
-            const uint32_t m = f - sdeg[lo];
-            t = nsg_id[(uint64_t)q * ncap + m];
-            const bool tnew = nsg_new[(uint64_t)q * ncap + m] != 0;
-            use = t != i && (seln[lo] || tnew);
-        }
-        const uint64_t mu = wave_ballot(use);
-        const uint32_t nt = (uint32_t)__builtin_popcountll(mu);
-        cc += nt;  // the reference evaluates every such pair (search_cc)
-        if (use) todo[lanes_below(mu)] = t;
-        wave_lds_sync();
-        // one octet per distance, FK_BVW_INFLIGHT rows per octet in flight
-        // (slots past nt compute a discarded distance to row 0)
-        for (uint32_t e0 = 0; e0 < nt; e0 += 8 * FK_BVW_INFLIGHT) {
-            OctChain<NB> rr[FK_BVW_INFLIGHT];
-            uint32_t te[FK_BVW_INFLIGHT];
+        uint32_t tr[FK_BVW_PREFETCH];
+        uint8_t nr[FK_BVW_PREFETCH], sr[FK_BVW_PREFETCH];
 #pragma unroll
-            for (uint32_t u = 0; u < FK_BVW_INFLIGHT; ++u) {
-                const uint32_t e = e0 + u * 8 + g;
-                te[u] = e < nt ? todo[e] : 0u;
-                rr[u].load(lane_chains + (size_t)te[u] * dpad, nb);
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < FK_BVW_INFLIGHT; ++u) {
-                const float dd = mine.dist_reg(rr[u], nb);
-                const bool ok = (e0 + u * 8 + g) < nt && k == 0 && dd <= thr;
-                const uint64_t mc = wave_ballot(ok);
-                if (ok) c[nc + lanes_below(mc)] = make_key(dd, te[u]);
-                nc += (uint32_t)__builtin_popcountll(mc);
+        for (uint32_t r = 0; r < FK_BVW_PREFETCH; ++r) {
+            const uint32_t f = s0 + r * 64 + lane;
+            tr[r] = i;  // "t == i": unused
+            nr[r] = sr[r] = 0;
+            if (f < total) {
+                uint32_t lo = 0, hi = ns;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (sdeg[mid] <= f) lo = mid; else hi = mid;
+                }
+                const uint64_t at = (uint64_t)sel[lo] * ncap + (f - sdeg[lo]);
+                tr[r] = nsg_id[at];
+                nr[r] = nsg_new[at];
+                sr[r] = seln[lo];
             }
         }
-        wave_lds_sync();
-        if (nc > FK_BVW_CH - 64 || (f0 + 64 >= total && nc > 0)) {
-            uint32_t ins = 0;
-            np = bvw_merge(p, pf, np, c, nc, cap, ins);
-            merged = merged || ins > 0;
-            nc = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < FK_BVW_PREFETCH; ++r) {
+            const uint32_t f0 = s0 + r * 64;
+            if (f0 >= total) break;
+            const uint32_t t = tr[r];
+            const bool use = t != i && (sr[r] || nr[r]);
+            const uint64_t mu = wave_ballot(use);
+            const uint32_t nt = (uint32_t)__builtin_popcountll(mu);
+            cc += nt;  // the reference evaluates every such pair (search_cc)
+            if (use) todo[lanes_below(mu)] = t;
+            wave_lds_sync();
+            // one octet per distance, FK_BVW_INFLIGHT rows per octet in
+            // flight (slots past nt compute a discarded distance to row 0)
+            for (uint32_t e0 = 0; e0 < nt; e0 += 8 * FK_BVW_INFLIGHT) {
+                OctChain<NB> rr[FK_BVW_INFLIGHT];
+                uint32_t te[FK_BVW_INFLIGHT];
+#pragma unroll
+                for (uint32_t u = 0; u < FK_BVW_INFLIGHT; ++u) {
+                    const uint32_t e = e0 + u * 8 + g;
+                    te[u] = e < nt ? todo[e] : 0u;
+                    rr[u].load(lane_chains + (size_t)te[u] * dpad, nb);
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < FK_BVW_INFLIGHT; ++u) {
+                    const float dd = mine.dist_reg(rr[u], nb);
+                    const bool ok = (e0 + u * 8 + g) < nt && k == 0 && dd <= thr;
+                    const uint64_t mc = wave_ballot(ok);
+                    if (ok) c[nc + lanes_below(mc)] = make_key(dd, te[u]);
+                    nc += (uint32_t)__builtin_popcountll(mc);
+                }
+            }
+            wave_lds_sync();
+            if (nc > FK_BVW_CH - 64 || (f0 + 64 >= total && nc > 0)) {
+                if (!loaded) {
+                    for (uint32_t j = lim + lane; j < np0; j += 64) { p[j] = pr[j]; pf[j] = pfr[j]; }
+                    wave_lds_sync();
+                    loaded = true;
+                }
+                uint32_t ins = 0;
+                np = bvw_merge(p, pf, np, c, nc, cap, ins);
+                merged = merged || ins > 0;
+                nc = 0;
+            }
         }
     }
     if (merged) {

@@ -262,6 +262,11 @@ fk_status fk_prune(fk_handle *h, fk_pg_type type, const uint64_t *pools, const u
         if (q) FK_HIP(hipMemcpyAsync(dq.p, q, (size_t)n_tasks * 4, hipMemcpyHostToDevice, h->stream));
         FK_HIP(hipMemsetAsync(h->counters.p, 0, 8, h->stream));
         PruneLaunch a;
+        if (p->flags & FK_FLAG_GRAM) {
+            // exact-form rows: occlusion distances from dot products
+            gram_prepare(h->stream, h->ds, p->flags);
+            if (h->ds.gram_fmt == FK_GRAM_EXACT) a.nrm = h->ds.gram_nrm.p;
+        }
         a.cand = dpool.p;
         a.cand_len = dlen.p;
         a.cand_stride = pool_stride;

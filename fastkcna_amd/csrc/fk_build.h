@@ -173,6 +173,11 @@ __device__ __forceinline__ uint32_t node_of_block(const uint32_t *order, uint32_
     const uint32_t pos = xcd_position((N + FK_NUM_XCD - 1) / FK_NUM_XCD);
     return pos < N ? order[pos] : 0xFFFFFFFFu;
 }
+// Per-row norms when the build may take distances in the exact form
+// (FK_FLAG_GRAM on exact-form rows, fk_gram.hip), else nullptr.
+inline const float2 *exact_norms(const BuildState &b) {
+    return ((b.P.flags & FK_FLAG_GRAM) && b.ds->gram_fmt == FK_GRAM_EXACT) ? b.ds->gram_nrm.p : nullptr;
+}
 inline unsigned node_grid(const BuildState &b) {
     return b.has_order ? (b.N + FK_NUM_XCD - 1) / FK_NUM_XCD * FK_NUM_XCD : b.N;
 }

@@ -218,6 +218,7 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
     // 1. prune every pool
     FK_HIP(hipMemsetAsync(b.acc.p, 0, 16, b.s));
     PruneLaunch a;
+    a.nrm = exact_norms(b);
     a.cand = b.pool.p;
     a.cand_len = b.cnt.p;
     a.cand_stride = b.cap;

@@ -95,12 +95,16 @@ struct PruneMC {
 };
 
 // LDS per wave: the cached kept chains (mc x dpad), one staging row
-// (dpad), kept ids and distances (R each).
+// (dpad), kept ids, distances and norms (R each).
+//
+// DOT: the rows are exact-form rows (fk_gram.hip) and every occlusion
+// distance is taken as |a|^2 + |b|^2 - 2 a.b (exact_dist), bit-identical to
+// the canonical distance at half the arithmetic.
 __host__ __device__ inline uint32_t prune_wave_bytes(uint32_t mc, uint32_t dpad, uint32_t R) {
-    return ((mc + 1u) * dpad * 4u + R * 8u + 15u) & ~15u;
+    return ((mc + 1u) * dpad * 4u + R * 12u + 15u) & ~15u;
 }
 
-template <int RULE, int NB>
+template <int RULE, int NB, bool DOT>
 struct PruneWave {
     uint32_t oracle_n, R;
     float tau3, cos_thr;
@@ -108,7 +112,8 @@ struct PruneWave {
     float *lane_cache;         // chain k of cached kept neighbour 0
     float *lane_stage;         // chain k of the staging row
     uint32_t *so;
-    float *sd;
+    float *sd, *sn;
+    const float2 *nrm;  // DOT: per-row norms
     uint32_t dpad, nb, g, k, lane, limit;
     uint32_t nout;
     bool stop;
@@ -121,12 +126,19 @@ struct PruneWave {
         const uint32_t id = key_id(key);
         return lane_chains + (size_t)((key != FK_KEY_EMPTY && id < oracle_n) ? id : 0u) * dpad;
     }
-    __device__ float member_dist(const OctChain<NB> &crow, uint32_t t, uint32_t tid) const {
-        if (t < PruneMC<NB>::value) return crow.dist(lane_cache + (size_t)t * dpad, nb);
-        return crow.dist(lane_chains + (size_t)tid * dpad, nb);
+    __device__ float norm_of(uint64_t key) const {
+        if constexpr (!DOT) return 0.f;
+        const uint32_t id = key_id(key);
+        return nrm[(key != FK_KEY_EMPTY && id < oracle_n) ? id : 0u].x;
     }
-    // Decides the batch at pos whose key (this octet's) and chain are given.
-    __device__ void batch(uint32_t pos, uint64_t key, const OctChain<NB> &crow) {
+    __device__ float member_dist(const OctChain<NB> &crow, float cn, uint32_t t, uint32_t tid) const {
+        const float *o = t < PruneMC<NB>::value ? lane_cache + (size_t)t * dpad : lane_chains + (size_t)tid * dpad;
+        if constexpr (DOT) return exact_dist(cn, sn[t], crow.dot(o, nb));
+        else return crow.dist(o, nb);
+    }
+    // Decides the batch at pos whose key (this octet's), chain and norm
+    // (DOT) are given.
+    __device__ void batch(uint32_t pos, uint64_t key, const OctChain<NB> &crow, float cn) {
         const uint32_t nbat = (limit - pos) < 8u ? (limit - pos) : 8u;
         const bool valid = g < nbat;
         const uint32_t pid = key_id(key);
@@ -144,7 +156,7 @@ struct PruneWave {
                 const uint32_t tid = so[t];
                 if (tid == pid) { occl = true; break; }
                 ++mycc;
-                const float djk = member_dist(crow, t, tid);
+                const float djk = member_dist(crow, cn, t, tid);
                 if (occludes<RULE>(djk, pd, sd[t], tau3, cos_thr)) { occl = true; break; }
             }
         }
@@ -160,6 +172,7 @@ struct PruneWave {
                 const uint32_t sg = sl >> 3;
                 const uint32_t spid = __shfl(pid, (int)sl, 64);
                 const float spd = __shfl(pd, (int)sl, 64);
+                const float spn = DOT ? __shfl(cn, (int)sl, 64) : 0.f;
                 const float *cp;
                 if constexpr (NB > 0) {
                     if (g == sg) crow.store(lane_stage, nb);
@@ -179,7 +192,9 @@ struct PruneWave {
                         if (tid == spid) {
                             hit = isdup = true;
                         } else {
-                            const float djk = octet_dist2<NB>(cp, lane_chains + (size_t)tid * dpad, nb);
+                            const float djk =
+                                DOT ? exact_dist(spn, sn[t], octet_dot2<NB>(cp, lane_chains + (size_t)tid * dpad, nb))
+                                    : octet_dist2<NB>(cp, lane_chains + (size_t)tid * dpad, nb);
                             hit = occludes<RULE>(djk, spd, sd[t], tau3, cos_thr);
                         }
                     }
@@ -208,8 +223,13 @@ struct PruneWave {
             const uint32_t al = (uint32_t)__builtin_ctzll(m) >> 3;
             const uint32_t aid = __shfl(pid, (int)(al * 8), 64);
             const float ad = __shfl(pd, (int)(al * 8), 64);
+            const float an = DOT ? __shfl(cn, (int)(al * 8), 64) : 0.f;
             const uint32_t slot = nout;
-            if (lane == 0) { so[slot] = aid; sd[slot] = ad; }
+            if (lane == 0) {
+                so[slot] = aid;
+                sd[slot] = ad;
+                if (DOT) sn[slot] = an;
+            }
             if (slot < PruneMC<NB>::value && g == al) crow.store(lane_cache + (size_t)slot * dpad, nb);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -221,7 +241,7 @@ struct PruneWave {
                     occl = true;
                 } else {
                     ++mycc;
-                    const float djk = member_dist(crow, slot, aid);
+                    const float djk = member_dist(crow, cn, slot, aid);
                     if (occludes<RULE>(djk, pd, ad, tau3, cos_thr)) occl = true;
                 }
             }
@@ -233,7 +253,7 @@ struct PruneWave {
     }
 };
 
-template <int RULE, int NB>
+template <int RULE, int NB, bool DOT>
 __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains, uint32_t dpad, PruneLaunch a,
                                                uint32_t wave_bytes) {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -252,7 +272,7 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains,
     uint32_t *olen = a.out_len + slot;
     uint8_t *wb = smem + (size_t)w * wave_bytes;
 
-    PruneWave<RULE, NB> pw;
+    PruneWave<RULE, NB, DOT> pw;
     pw.oracle_n = a.oracle_n;
     pw.R = a.R;
     pw.tau3 = a.tau3;
@@ -267,6 +287,8 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains,
     pw.lane_stage = reinterpret_cast<float *>(wb) + (size_t)MC * dpad + pw.k * pw.nb;
     pw.so = reinterpret_cast<uint32_t *>(wb + (size_t)(MC + 1) * dpad * 4u);
     pw.sd = reinterpret_cast<float *>(pw.so + a.R);
+    pw.sn = pw.sd + a.R;
+    pw.nrm = a.nrm;
     pw.cc = 0;
     pw.exam = 0;
 
@@ -280,7 +302,11 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains,
     {
         // the first candidate is always kept
         const uint64_t k0 = c[start];
-        if (lane == 0) { pw.so[0] = key_id(k0); pw.sd[0] = key_dist(k0); }
+        if (lane == 0) {
+            pw.so[0] = key_id(k0);
+            pw.sd[0] = key_dist(k0);
+            if (DOT) pw.sn[0] = a.nrm[key_id(k0)].x;  // kept whatever its id, like its row
+        }
         if (MC > 0 && pw.g == 0) {
             OctChain<NB> t0;
             t0.load(pw.lane_chains + (size_t)key_id(k0) * dpad, pw.nb);
@@ -299,14 +325,17 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains,
     uint64_t ka = pw.key_at(c, p0 + pw.g), kb = pw.key_at(c, p0 + 8 + pw.g);
     OctChain<NB> ra, rb;
     ra.load(pw.row_of(ka), pw.nb);
+    float na = pw.norm_of(ka), nbn = 0.f;
     for (uint32_t pos = p0; !pw.stop && pos < pw.limit; pos += 16) {
         rb.load(pw.row_of(kb), pw.nb);
+        nbn = pw.norm_of(kb);
         const uint64_t kc = pw.key_at(c, pos + 16 + pw.g);
-        pw.batch(pos, ka, ra);
+        pw.batch(pos, ka, ra, na);
         if (pw.stop || pos + 8 >= pw.limit) break;
         ra.load(pw.row_of(kc), pw.nb);
+        na = pw.norm_of(kc);
         const uint64_t kd = pw.key_at(c, pos + 24 + pw.g);
-        pw.batch(pos + 8, kb, rb);
+        pw.batch(pos + 8, kb, rb, nbn);
         ka = kc;
         kb = kd;
     }
@@ -329,8 +358,15 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains,
 template <int RULE, int NB>
 static void launch_prune_nb(hipStream_t s, const Dataset &ds, const PruneLaunch &a, unsigned grid) {
     const uint32_t wave_bytes = prune_wave_bytes(PruneMC<NB>::value, ds.dpad, a.R);
-    hipLaunchKernelGGL((k_prune<RULE, NB>), dim3(grid), dim3(64 * FK_PRUNE_WPB), (size_t)wave_bytes * FK_PRUNE_WPB,
-                       s, ds.chains.p, ds.dpad, a, wave_bytes);
+    if constexpr (NB == 12 || NB == 16 || NB == 32) {
+        if (a.nrm) {
+            hipLaunchKernelGGL((k_prune<RULE, NB, true>), dim3(grid), dim3(64 * FK_PRUNE_WPB),
+                               (size_t)wave_bytes * FK_PRUNE_WPB, s, ds.chains.p, ds.dpad, a, wave_bytes);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_prune<RULE, NB, false>), dim3(grid), dim3(64 * FK_PRUNE_WPB),
+                       (size_t)wave_bytes * FK_PRUNE_WPB, s, ds.chains.p, ds.dpad, a, wave_bytes);
 }
 
 template <int RULE>

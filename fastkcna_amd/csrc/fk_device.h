@@ -181,6 +181,30 @@ struct OctChain {
         }
         return octet_sum(acc);
     }
+    // dot products (the exact form: see exact_dist)
+    __device__ __forceinline__ float dot_reg(const OctChain &o, uint32_t) const {
+        float acc = 0.f;
+#pragma unroll
+        for (int q = 0; q < NB / 4; ++q) {
+            acc = __builtin_fmaf(v[q].x, o.v[q].x, acc);
+            acc = __builtin_fmaf(v[q].y, o.v[q].y, acc);
+            acc = __builtin_fmaf(v[q].z, o.v[q].z, acc);
+            acc = __builtin_fmaf(v[q].w, o.v[q].w, acc);
+        }
+        return octet_sum(acc);
+    }
+    __device__ __forceinline__ float dot(const float *__restrict__ o, uint32_t) const {
+        float acc = 0.f;
+#pragma unroll
+        for (int q = 0; q < NB / 4; ++q) {
+            const float4 b = reinterpret_cast<const float4 *>(o)[q];
+            acc = __builtin_fmaf(v[q].x, b.x, acc);
+            acc = __builtin_fmaf(v[q].y, b.y, acc);
+            acc = __builtin_fmaf(v[q].z, b.z, acc);
+            acc = __builtin_fmaf(v[q].w, b.w, acc);
+        }
+        return octet_sum(acc);
+    }
 };
 template <>
 struct OctChain<0> {
@@ -196,7 +220,20 @@ struct OctChain<0> {
         }
         return octet_sum(acc);
     }
+    __device__ __forceinline__ float dot_reg(const OctChain &o, uint32_t nb) const { return dot(o.p, nb); }
+    __device__ __forceinline__ float dot(const float *__restrict__ o, uint32_t nb) const {
+        float acc = 0.f;
+        for (uint32_t j = 0; j < nb; ++j) acc = __builtin_fmaf(p[j], o[j], acc);
+        return octet_sum(acc);
+    }
 };
+
+// Exact form of the squared L2 for rows that the Gram preparation marked
+// exact (integer values, |x_i| <= 256, small norms): every product, partial
+// sum and the result are integers below 2^24, so |a|^2 + |b|^2 - 2 a.b is
+// the canonical distance bit for bit, in any summation order, at one fused
+// multiply-add per element instead of a subtract and a multiply-add.
+__device__ __forceinline__ float exact_dist(float na, float nb, float dot) { return (na + nb) - 2.f * dot; }
 
 // Octet distance between two chain-major rows given by pointers (this
 // lane's chain of each); a is the minuend, as in OctChain::dist.
@@ -218,6 +255,25 @@ __device__ __forceinline__ float octet_dist2(const float *__restrict__ a, const 
             t = a[j] - b[j];
             acc = __builtin_fmaf(t, t, acc);
         }
+    }
+    return octet_sum(acc);
+}
+
+template <int NB>
+__device__ __forceinline__ float octet_dot2(const float *__restrict__ a, const float *__restrict__ b, uint32_t nb) {
+    float acc = 0.f;
+    if constexpr (NB > 0) {
+#pragma unroll
+        for (int q = 0; q < NB / 4; ++q) {
+            const float4 x = reinterpret_cast<const float4 *>(a)[q];
+            const float4 y = reinterpret_cast<const float4 *>(b)[q];
+            acc = __builtin_fmaf(x.x, y.x, acc);
+            acc = __builtin_fmaf(x.y, y.y, acc);
+            acc = __builtin_fmaf(x.z, y.z, acc);
+            acc = __builtin_fmaf(x.w, y.w, acc);
+        }
+    } else {
+        for (uint32_t j = 0; j < nb; ++j) acc = __builtin_fmaf(a[j], b[j], acc);
     }
     return octet_sum(acc);
 }

@@ -144,6 +144,7 @@ struct PruneLaunch {
     uint32_t *out_len = nullptr;
     unsigned long long *cc = nullptr;  // accumulated distance count
     unsigned long long *examined = nullptr;  // accumulated examined candidates
+    const float2 *nrm = nullptr;      // exact-form rows: norms for dot-product distances
 };
 void launch_prune(hipStream_t s, const Dataset &ds, const PruneLaunch &a);
 void build_chains(hipStream_t s, Dataset &ds);  // fills ds.chains from ds.rows

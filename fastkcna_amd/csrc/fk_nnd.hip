@@ -1155,9 +1155,11 @@ __device__ uint32_t bvw_merge(uint64_t *p, uint8_t *pf, uint32_t np, uint64_t *c
     return (np + nk) < cap ? (np + nk) : cap;
 }
 
-template <int NB>
+// DOT: exact-form rows (fk_gram.hip), distances as |a|^2 + |b|^2 - 2 a.b.
+template <int NB, bool DOT>
 __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
-    const float *__restrict__ chains, uint32_t dpad, uint32_t N, uint32_t cap, uint32_t W, uint32_t S_cap,
+    const float *__restrict__ chains, const float2 *__restrict__ nrm, uint32_t dpad, uint32_t N, uint32_t cap,
+    uint32_t W, uint32_t S_cap,
     uint64_t *__restrict__ pool, uint8_t *__restrict__ pflg, uint32_t *__restrict__ cnt,
     const uint32_t *__restrict__ nsg_id, const uint32_t *__restrict__ nsg_len, const uint8_t *__restrict__ nsg_new,
     uint32_t ncap, unsigned long long *__restrict__ cc_acc, const uint32_t *__restrict__ order,
@@ -1226,6 +1228,7 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
     const float *lane_chains = chains + k * nb;
     OctChain<NB> mine;
     mine.load(lane_chains + (size_t)i * dpad, nb);
+    const float ni = DOT ? nrm[i].x : 0.f;
     uint32_t np = np0, nc = 0;
     unsigned long long cc = 0;
     bool merged = false, loaded = false;
@@ -1268,15 +1271,17 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
             for (uint32_t e0 = 0; e0 < nt; e0 += 8 * FK_BVW_INFLIGHT) {
                 OctChain<NB> rr[FK_BVW_INFLIGHT];
                 uint32_t te[FK_BVW_INFLIGHT];
+                float tn[FK_BVW_INFLIGHT];
 #pragma unroll
                 for (uint32_t u = 0; u < FK_BVW_INFLIGHT; ++u) {
                     const uint32_t e = e0 + u * 8 + g;
                     te[u] = e < nt ? todo[e] : 0u;
                     rr[u].load(lane_chains + (size_t)te[u] * dpad, nb);
+                    tn[u] = DOT ? nrm[te[u]].x : 0.f;
                 }
 #pragma unroll
                 for (uint32_t u = 0; u < FK_BVW_INFLIGHT; ++u) {
-                    const float dd = mine.dist_reg(rr[u], nb);
+                    const float dd = DOT ? exact_dist(ni, tn[u], mine.dot_reg(rr[u], nb)) : mine.dist_reg(rr[u], nb);
                     const bool ok = (e0 + u * 8 + g) < nt && k == 0 && dd <= thr;
                     const uint64_t mc = wave_ballot(ok);
                     if (ok) c[nc + lanes_below(mc)] = make_key(dd, te[u]);
@@ -1316,17 +1321,18 @@ static void launch_bv_round(BuildState &b, uint32_t W, uint32_t scap) {
     const uint32_t nblk = (b.N + wpb - 1) / wpb;
     const unsigned grid = ord ? (nblk + FK_NUM_XCD - 1) / FK_NUM_XCD * FK_NUM_XCD : nblk;
     uint32_t *mark = b.prof ? b.bv_mark.p : nullptr;
-#define FK_BV_LAUNCH(NBV)                                                                                        \
-    hipLaunchKernelGGL(k_bv_round<NBV>, dim3(grid), dim3(64 * wpb), lds, b.s, b.ds->chains.p,                   \
+    const float2 *nrm = exact_norms(b);
+#define FK_BV_LAUNCH(NBV, DOT)                                                                                   \
+    hipLaunchKernelGGL((k_bv_round<NBV, DOT>), dim3(grid), dim3(64 * wpb), lds, b.s, b.ds->chains.p, nrm,       \
                        b.ds->dpad, b.N, b.cap, W, scap, b.pool.p, b.pflg.p, b.cnt.p, b.nsg_id.p, b.nsg_len.p,   \
                        b.nsg_new.p, b.ncap, b.acc.p, ord, mark, wave_bytes, wpb)
     switch (b.ds->dpad) {
-        case 32: FK_BV_LAUNCH(4); break;
-        case 64: FK_BV_LAUNCH(8); break;
-        case 96: FK_BV_LAUNCH(12); break;
-        case 128: FK_BV_LAUNCH(16); break;
-        case 256: FK_BV_LAUNCH(32); break;
-        default: FK_BV_LAUNCH(0); break;
+        case 32: FK_BV_LAUNCH(4, false); break;
+        case 64: FK_BV_LAUNCH(8, false); break;
+        case 96: if (nrm) FK_BV_LAUNCH(12, true); else FK_BV_LAUNCH(12, false); break;
+        case 128: if (nrm) FK_BV_LAUNCH(16, true); else FK_BV_LAUNCH(16, false); break;
+        case 256: if (nrm) FK_BV_LAUNCH(32, true); else FK_BV_LAUNCH(32, false); break;
+        default: FK_BV_LAUNCH(0, false); break;
     }
 #undef FK_BV_LAUNCH
 }

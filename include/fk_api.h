@@ -65,7 +65,10 @@ typedef enum {
 /* Local join in Gram form on the MFMA units (16-bit operands, FP32
  * accumulation and norms), every pair that could enter a pool recomputed in
  * exact FP32: the pools equal the plain FP32 join's.  The operand format is
- * picked from the data (fk_stats.gram_mode) unless forced. */
+ * picked from the data (fk_stats.gram_mode) unless forced.  On exact-form
+ * rows (FK_GRAM_EXACT) the prunes and BridgeView also take their distances
+ * as |a|^2 + |b|^2 - 2 a.b, which is the FP32 distance bit for bit there
+ * (fk_prune honours the flag too). */
 #define FK_FLAG_GRAM 0x8u
 #define FK_FLAG_GRAM_BF16 0x10u  /* force the bounded BF16 form                  */
 #define FK_FLAG_GRAM_F16 0x20u   /* force the bounded FP16 form                  */

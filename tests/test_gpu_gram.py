@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle
-from fastkcna_amd import FK_PG_NSG, FK_PG_TAUMNG, IndexParams, _ffi
+from fastkcna_amd import FK_PG_ALPHAPG, FK_PG_HNSW_LAYER, FK_PG_NSG, FK_PG_TAUMNG, IndexParams, _ffi
 from fastkcna_amd.datasets import float_mixture, integer_mixture, unit_mixture
 
 pytestmark = pytest.mark.gpu
@@ -155,3 +155,51 @@ def test_gram_full_build_identical(device, name, pg):
     assert ep0 == ep1 and w0 == w1
     assert np.array_equal(off0, off1) and np.array_equal(ids0, ids1)
     assert st1["d_gram"] > 0
+
+
+@pytest.mark.parametrize("pg", [FK_PG_NSG, FK_PG_TAUMNG, FK_PG_ALPHAPG])
+@pytest.mark.parametrize("reversed_", [False, True])
+def test_exact_form_prune_bitexact(device, pg, reversed_):
+    """With FK_FLAG_GRAM on exact-form rows the pruner takes its occlusion
+    distances as |a|^2 + |b|^2 - 2 a.b; ids, kept distances and distance
+    counts must equal the CPU oracle's (canonical FP32 distances)."""
+    from test_gpu_parity import make_pools
+    x = integer_mixture(4000, 128, 32, seed=11)
+    # duplicate rows: zero distances take the 0/0 and x/0 paths of the alpha rule
+    x[100:140] = x[200:240]
+    say("exact prune", pg, reversed_)
+    device.upload(x)
+    rng = np.random.default_rng(3)
+    p = IndexParams(nsg_R=24, search_K=90, tau=300.0, angle=63)
+    pools, plen, q = make_pools(x, 400, 120, rng)
+    osz = len(x) - 50
+    c = oracle.prune(pg, x, pools, plen, q, osz, p, reversed_=reversed_)
+    g0 = device.prune(pg, pools, plen, q, osz, p, reversed_=reversed_)
+    p.flags = GRAM
+    g = device.prune(pg, pools, plen, q, osz, p, reversed_=reversed_)
+    bad0 = [t for t in range(400) if g0[2][t] != c[2][t] or not np.array_equal(g0[0][t, :c[2][t]], c[0][t, :c[2][t]])]
+    bad = [t for t in range(400) if g[2][t] != c[2][t] or not np.array_equal(g[0][t, :c[2][t]], c[0][t, :c[2][t]])]
+    say(f"  differing tasks: plain {len(bad0)} {bad0[:5]}, exact form {len(bad)} {bad[:5]}")
+    for t in bad[:2]:
+        say("   task", t, "q", q[t], "len", plen[t], "oracle", c[0][t, :c[2][t]].tolist(), "gpu", g[0][t, :g[2][t]].tolist())
+    assert np.array_equal(g[2], c[2])
+    for t in range(400):
+        k = c[2][t]
+        assert np.array_equal(g[0][t, :k], c[0][t, :k]), t
+        assert np.array_equal(g[1][t, :k].view(np.uint32), c[1][t, :k].view(np.uint32)), t
+    assert g[3] == c[3]
+
+
+@pytest.mark.parametrize("pg", [FK_PG_ALPHAPG, FK_PG_HNSW_LAYER])
+def test_gram_full_build_identical_exact_form(device, pg):
+    """Exact form end to end (join, prunes, BridgeView): the same graph as the
+    plain FP32 build."""
+    x = data("int128")
+    device.upload(x)
+    p = nnd_params(angle=60)
+    off0, ids0, ep0, w0, _ = device.build(pg, p)
+    p.flags = GRAM
+    off1, ids1, ep1, w1, st1 = device.build(pg, p)
+    assert st1["gram_mode"] == _ffi.FK_GRAM_EXACT
+    assert ep0 == ep1 and w0 == w1
+    assert np.array_equal(off0, off1) and np.array_equal(ids0, ids1)

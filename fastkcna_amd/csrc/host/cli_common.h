@@ -2,7 +2,8 @@
 // fasthnsw.  Same flags, defaults, positional arguments and files as the
 // reference tools (src/fastnsg.cpp:40-209, src/fasthnsw.cpp:46-358), parsed
 // without Boost: --name value, --name=value, -X value, -Xvalue; positional
-// DATA and OUTPUT.  GPU-only additions: --device N (HIP ordinal, default 0).
+// DATA and OUTPUT.  GPU-only additions: --device N (HIP ordinal, default 0),
+// --gram (FK_FLAG_GRAM) and --screen (FK_FLAG_SCREEN).
 #pragma once
 #include <stdio.h>
 #include <stdlib.h>
@@ -20,6 +21,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "fk_api.h"
 #include "kgraph-data.h"
 #include "kgraph.h"
 
@@ -99,6 +101,10 @@ inline int run_cli(int argc, char **argv, Mode mode) {
         {{"--synthetic"}, false, "generate synthetic data, for performance evaluation only", setter(synthetic)},
         {{"--l2norm"}, true, "l2-normalize data", [&](const std::string &) { l2norm = true; }},
         {{"--device"}, false, "GPU ordinal (HIP)", setter(params.device)},
+        {{"--gram"}, true, "local join in Gram form on the MFMA units (same graph)",
+         [&](const std::string &) { params.gpu_flags |= FK_FLAG_GRAM; }},
+        {{"--screen"}, true, "learned distance screen in the local join",
+         [&](const std::string &) { params.gpu_flags |= FK_FLAG_SCREEN; }},
     };
     auto find = [&](const std::string &name) -> Option * {
         for (auto &o : opts)

@@ -58,6 +58,8 @@ fk_params to_fk(KGraph::IndexParams const &p) {
     f.search_K = p.search_K; f.step = p.step; f.massq_S = p.massq_S; f.loop_i = p.loop_i; f.tau = p.tau;
     f.angle = p.angle;
     f.flags = verbosity > 1 ? FK_FLAG_VERBOSE : 0;
+    f.flags |= p.gpu_flags & (FK_FLAG_GRAM | FK_FLAG_GRAM_BF16 | FK_FLAG_GRAM_F16 | FK_FLAG_SCREEN |
+                              FK_FLAG_SCREEN_NOTRAIN);
     return f;
 }
 

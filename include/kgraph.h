@@ -115,6 +115,9 @@ public:
         unsigned P;
         /// GPU ordinal used by the build (not in the reference; default 0).
         int device;
+        /// GPU-only build options (not in the reference; default 0): the
+        /// FK_FLAG_GRAM* and FK_FLAG_SCREEN* bits of fk_api.h.
+        unsigned gpu_flags;
         IndexParams()
             : nsg_R(default_nsgr), search_L(default_searchl), search_K(default_searchk), step(default_step),
               tau(default_tau), angle(default_angle), nt(0), massq_S(default_massqS), loop_i(default_loop_i),
@@ -122,7 +125,7 @@ public:
               controls(default_controls), seed(default_seed), nthreads(default_nthreads), delta(default_delta),
               recall(default_recall), prune(default_prune), reverse(default_reverse), num0(default_num0),
               incr_r(default_incr_r), sl(0), iter(default_iter), qid(0), Lcon(0), sidx(0), B(0), P(default_P),
-              device(0) {}
+              device(0), gpu_flags(0) {}
     };
 
     struct SearchParams {

@@ -151,3 +151,18 @@ def test_fasthnsw_writes_hnswlib_index(tmp_path):
         hits += len(set(lab[found].tolist()) & set(gt[qi].tolist()))
     assert hits / 500 > 0.8, hits / 500
     assert "fasthnsw indexing time" in res.read_text()
+
+
+def test_fastnsg_gram_flag_writes_the_same_index(tmp_path):
+    """--gram (local join on the MFMA units, exact form on integer rows) must
+    write the byte-identical NSG index of the plain build."""
+    x = integer_mixture(3000, 128, 16, seed=23, sigma=40.0)
+    data = tmp_path / "b.lshkit"
+    write_lshkit(data, x)
+    outs = []
+    for extra in ([], ["--gram"]):
+        out = tmp_path / f"g{len(outs)}.nsg"
+        run("fastnsg", data, out, "-K", 40, "-L", 40, "-S", 10, "-R", 50, "-I", 5, "--nsg_R", 16, "--search_L",
+            40, "--search_K", 40, *extra)
+        outs.append(out.read_bytes())
+    assert outs[0] == outs[1]

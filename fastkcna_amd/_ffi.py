@@ -102,6 +102,7 @@ _PROTOS = {
     "fk_dataset_upload": (ctypes.c_int, [_P, _P, ctypes.c_size_t, u32, u32]),
     "fk_build": (ctypes.c_int, [_P, ctypes.c_int, u32, ctypes.POINTER(FkParams), ctypes.POINTER(FkGraph)]),
     "fk_graph_free": (None, [ctypes.POINTER(FkGraph)]),
+    "fk_build_batch": (ctypes.c_int, [_P, ctypes.c_int, u32, _P, ctypes.POINTER(FkParams), ctypes.POINTER(FkGraph)]),
     "fk_pair_distances": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, _P]),
     "fk_gram_pair_distances": (ctypes.c_int, [_P, u32, _P, _P, ctypes.c_size_t, _P, _P, _P, ctypes.POINTER(u32)]),
     "fk_screen_param_count": (ctypes.c_size_t, [u32]),

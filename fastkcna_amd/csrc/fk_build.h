@@ -14,6 +14,8 @@
 #include "fk_device.h"
 #include "fk_internal.h"
 
+#include <algorithm>
+
 namespace fk {
 
 #define FK_FLG_NEW 0x1u
@@ -99,7 +101,29 @@ struct BuildState {
     uint32_t width = 0;
     uint32_t loopcount = 0;
     std::vector<float> center;  // Calcenter over all resident rows (cached)
-    uint64_t host_rng = 0;      // the reference's global rand() stream
+
+    // Independent sets built together (fk_build_batch): set s is the nodes
+    // [set_lo[s], set_lo[s + 1]); every per-set decision (random draws, stop
+    // rule, BridgeView rounds, entry point, connectivity) is made per set,
+    // so each set's graph is the one a build of its rows alone gives.
+    // fk_build is the case n_sets == 1 (set_of empty).
+    uint32_t n_sets = 1;
+    std::vector<uint32_t> set_lo;       // host, n_sets + 1
+    DevBuf<uint32_t> d_set_lo;          // device copy (n_sets > 1)
+    DevBuf<uint32_t> set_of;            // node -> set (n_sets > 1)
+    std::vector<uint64_t> set_rng;      // per set: the reference's global rand() stream
+    std::vector<uint32_t> set_ep;       // per set: entry point (node id)
+    std::vector<uint32_t> set_width;    // per set: width (max out-degree seen)
+    std::vector<uint8_t> set_active;    // per set: NN-descent still iterating
+    DevBuf<uint8_t> d_set_active;
+    std::vector<uint32_t> ctrl_lo;      // controls of set s: [ctrl_lo[s], ctrl_lo[s + 1])
+    DevBuf<unsigned long long> set_acc; // per-set reduction scratch (2 per set)
+    uint32_t set_n(uint32_t s) const { return set_lo[s + 1] - set_lo[s]; }
+    uint32_t set_of_node(uint32_t v) const {  // host lookup
+        return (uint32_t)(std::upper_bound(set_lo.begin(), set_lo.end(), v) - set_lo.begin()) - 1;
+    }
+    const uint32_t *dev_set_lo() const { return n_sets > 1 ? d_set_lo.p : nullptr; }
+    const uint32_t *dev_set_of() const { return n_sets > 1 ? set_of.p : nullptr; }
 
     // locality order (fk_order.hip): node at each position of the order
     DevBuf<uint32_t> order;     // N
@@ -130,13 +154,17 @@ struct BuildState {
     void prof_flush();
     ~BuildState();
 
-    uint64_t next_rand() {
-        host_rng += 0x9E3779B97F4A7C15ull;
-        uint64_t x = host_rng;
+    uint64_t next_rand(uint32_t set = 0) {
+        uint64_t &r = set_rng[set];
+        r += 0x9E3779B97F4A7C15ull;
+        uint64_t x = r;
         x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
         x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
         return x ^ (x >> 31);
     }
+    // Sets up the set bookkeeping (n_sets >= 1; lo has n_sets + 1 entries
+    // and covers [0, N)).
+    void init_sets(const std::vector<uint32_t> &lo);
     void sync();
 };
 
@@ -144,7 +172,8 @@ struct BuildState {
 void nnd_alloc(BuildState &b);
 void nnd_controls(BuildState &b);
 void nnd_run(BuildState &b);            // init + iterations (constructor)
-float nnd_evaluate(BuildState &b, uint32_t K);  // recall over controls at K
+// recall over controls at K (all controls; per set into *per_set if given)
+float nnd_evaluate(BuildState &b, uint32_t K, std::vector<float> *per_set = nullptr);
 
 // Pass 1 of the local join in Gram form (fk_gram.hip); rechecked pairs are
 // added to *n_recheck (device counter)
@@ -187,7 +216,7 @@ void build_loops(BuildState &b, fk_pg_type type);
 
 // Search / graph utilities (fk_search.hip)
 void compute_ep(BuildState &b);
-void column_mean(hipStream_t s, const Dataset &ds, float *center);
+void column_mean(hipStream_t s, const Dataset &ds, float *center, uint32_t r0 = 0, uint32_t n = 0);
 void tree_grow(BuildState &b);
 
 double now_sec();

@@ -250,6 +250,7 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
     b.prof_end(pe, FK_PROF_PRUNE, (double)cc2[1] * (4.0 * b.ds->d + 8.0) + 8.0 * (double)N * P.nsg_R);
     b.prof_flush();
     b.width = P.nsg_R;
+    b.set_width.assign(b.n_sets, P.nsg_R);
     b.st->d_prune += cc;
     const double t1 = now_sec();
     if (li < FK_MAX_LOOPS) {
@@ -435,7 +436,7 @@ fk_status fk_build(fk_handle *h, fk_pg_type type, uint32_t n_used, const fk_para
         b.cap = b.P.MAXL;
         b.st = &out->stats;
         b.verbose = (p->flags & FK_FLAG_VERBOSE) != 0;
-        b.host_rng = ((uint64_t)b.P.seed << 32) ^ 0x5DEECE66Dull;
+        b.init_sets({0u, n_used});
         b.prof = (p->flags & FK_FLAG_PROFILE) != 0;
         b.t0 = t0;
         nnd_controls(b);
@@ -466,6 +467,96 @@ fk_status fk_build(fk_handle *h, fk_pg_type type, uint32_t n_used, const fk_para
     });
 }
 
+fk_status fk_build_batch(fk_handle *h, fk_pg_type type, uint32_t n_sets, const uint64_t *set_off,
+                         const fk_params *p, fk_graph *out) {
+    if (!h || !out || !set_off || n_sets == 0) return FK_E_INVALID;
+    for (uint32_t k = 0; k < n_sets; ++k) memset(&out[k], 0, sizeof(out[k]));
+    return guard(h, [&] {
+        require_dataset(h);
+        check_params(p);
+        FK_CHECK_ARG(type == FK_PG_NSG || type == FK_PG_HNSW_LAYER || type == FK_PG_TAUMNG || type == FK_PG_ALPHAPG,
+                     "unknown graph type");
+        FK_CHECK_ARG(!(p->flags & FK_FLAG_SCREEN), "the learned screen is not supported in batch builds");
+        FK_CHECK_ARG(set_off[0] == 0 && set_off[n_sets] <= h->ds.n_total, "sets must lie in the resident rows");
+        // every set keeps the unshrunk parameters (src/kgraph.cpp:1288-1334
+        // shrinks them only for sets no larger than K, L, S, controls,
+        // search_L or search_K)
+        const uint64_t need = std::max({(uint64_t)p->K, (uint64_t)p->L, (uint64_t)p->S, (uint64_t)p->controls,
+                                        (uint64_t)p->search_L, (uint64_t)p->search_K}) + 1;
+        std::vector<uint32_t> lo(n_sets + 1);
+        for (uint32_t k = 0; k < n_sets; ++k) {
+            FK_CHECK_ARG(set_off[k + 1] >= set_off[k] + need,
+                         "every set needs more rows than max(K, L, S, controls, search_L, search_K)");
+            lo[k] = (uint32_t)set_off[k];
+        }
+        lo[n_sets] = (uint32_t)set_off[n_sets];
+        select_device(h);
+        const double t0 = now_sec();
+        fk_stats st;
+        memset(&st, 0, sizeof(st));
+        BuildState b;
+        b.h = h;
+        b.s = h->stream;
+        b.ds = &h->ds;
+        b.N = lo[n_sets];
+        b.P = shrink_params(*p, b.N);
+        b.cap = b.P.MAXL;
+        b.st = &st;
+        b.verbose = (p->flags & FK_FLAG_VERBOSE) != 0;
+        b.prof = (p->flags & FK_FLAG_PROFILE) != 0;
+        b.t0 = t0;
+        b.init_sets(lo);
+        nnd_controls(b);
+        nnd_run(b);
+        const float rec = nnd_evaluate(b, b.P.MAXL);
+        if (st.n_recall < FK_MAX_LOOPS + 64) st.recall[st.n_recall++] = rec;
+        if (st.n_kgraph < FK_MAX_LOOPS + 64) st.kgraph_time[st.n_kgraph++] = now_sec() - t0;
+        build_loops(b, type);
+        // one CSR for all sets, then per set with ids relative to the set
+        std::vector<uint32_t> len(b.N);
+        FK_HIP(hipMemcpyAsync(len.data(), b.nsg_len.p, (size_t)b.N * 4, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+        std::vector<uint64_t> off((size_t)b.N + 1, 0);
+        for (uint32_t i = 0; i < b.N; ++i) off[i + 1] = off[i] + len[i];
+        DevBuf<uint64_t> doff(b.N + 1);
+        DevBuf<uint32_t> dids(off[b.N]);
+        FK_HIP(hipMemcpyAsync(doff.p, off.data(), off.size() * 8, hipMemcpyHostToDevice, b.s));
+        hipLaunchKernelGGL(k_pack_csr, dim3((b.N + 3) / 4), dim3(256), 0, b.s, b.nsg_id.p, b.nsg_len.p, b.N, b.ncap,
+                           doff.p, dids.p);
+        FK_LAUNCH_CHECK();
+        std::vector<uint32_t> ids(std::max<uint64_t>(off[b.N], 1));
+        FK_HIP(hipMemcpyAsync(ids.data(), dids.p, off[b.N] * 4, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+        st.t_total = now_sec() - t0;
+        std::vector<fk_graph> res(n_sets);
+        struct Free {
+            std::vector<fk_graph> &r;
+            bool keep = false;
+            ~Free() {
+                if (!keep)
+                    for (auto &g : r) { free(g.offsets); free(g.ids); }
+            }
+        } guard_res{res};
+        for (uint32_t k = 0; k < n_sets; ++k) {
+            fk_graph &g = res[k];
+            memset(&g, 0, sizeof(g));
+            const uint32_t a = lo[k], n = lo[k + 1] - a;
+            const uint64_t e0 = off[a], e1 = off[lo[k + 1]];
+            g.n = n;
+            g.ep = b.set_ep[k] - a;
+            g.width = b.set_width[k];
+            g.offsets = (uint64_t *)malloc(((size_t)n + 1) * sizeof(uint64_t));
+            g.ids = (uint32_t *)malloc(std::max<uint64_t>(e1 - e0, 1) * 4);
+            if (!g.offsets || !g.ids) throw std::bad_alloc();
+            for (uint32_t i = 0; i <= n; ++i) g.offsets[i] = off[a + i] - e0;
+            for (uint64_t e = e0; e < e1; ++e) g.ids[e - e0] = ids[e] - a;
+            g.stats = st;
+        }
+        guard_res.keep = true;
+        for (uint32_t k = 0; k < n_sets; ++k) out[k] = res[k];
+    });
+}
+
 fk_status fk_nndescent(fk_handle *h, uint32_t n_used, const fk_params *p, uint64_t *pools, uint32_t *pool_len,
                        uint32_t pool_stride, fk_stats *stats) {
     if (!h) return FK_E_INVALID;
@@ -489,7 +580,7 @@ fk_status fk_nndescent(fk_handle *h, uint32_t n_used, const fk_params *p, uint64
         b.st = stats ? stats : &local;
         memset(b.st, 0, sizeof(*b.st));
         b.verbose = (p->flags & FK_FLAG_VERBOSE) != 0;
-        b.host_rng = ((uint64_t)b.P.seed << 32) ^ 0x5DEECE66Dull;
+        b.init_sets({0u, n_used});
         b.prof = (p->flags & FK_FLAG_PROFILE) != 0;
         b.t0 = t0;
         nnd_controls(b);

@@ -663,8 +663,14 @@ __global__ __launch_bounds__(FK_SEL_THREADS) void k_topk_select(const uint64_t *
     }
 }
 
+__global__ void k_add_u32(uint32_t *__restrict__ v, size_t n, uint32_t add) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] += add;
+}
+
 void bruteforce_knn(hipStream_t s, const Dataset &ds, const float *d_queries, uint32_t q_stride, uint32_t n_q,
-                    uint32_t n_used, uint32_t K, const uint32_t *d_self, uint32_t *d_out_ids, float *d_out_dists) {
+                    uint32_t n_used, uint32_t K, const uint32_t *d_self, uint32_t *d_out_ids, float *d_out_dists,
+                    uint32_t base) {
     FK_CHECK_ARG(K >= 1 && K <= FK_SEL_MAXK, "K must be in [1, 2048]");
     FK_CHECK_ARG(n_used >= 1, "empty dataset");
     const size_t lds = (size_t)FK_BF_ROWS * (ds.dpad + 4) * sizeof(float);
@@ -680,18 +686,25 @@ void bruteforce_knn(hipStream_t s, const Dataset &ds, const float *d_queries, ui
         const uint32_t nq = (uint32_t)((n_q - q0) < qb ? (n_q - q0) : qb);
         if (wide) {
             const unsigned gx = (n_used + FK_BF_WROWS - 1) / FK_BF_WROWS;
-            hipLaunchKernelGGL(k_bf_keys_wide, dim3(gx), dim3(256), lds_wide, s, ds.chains.p, ds.dpad, n_used,
+            hipLaunchKernelGGL(k_bf_keys_wide, dim3(gx), dim3(256), lds_wide, s, ds.chains.p + (size_t)base * ds.dpad,
+                               ds.dpad, n_used,
                                d_queries + (size_t)q0 * q_stride, q_stride, nq, d_self ? d_self + q0 : nullptr,
                                keys.p);
         } else {
             const unsigned gx = (n_used + FK_BF_ROWS - 1) / FK_BF_ROWS;
-            hipLaunchKernelGGL(k_bf_keys, dim3(gx), dim3(256), lds, s, ds.rows.p, ds.stride, ds.dpad, n_used,
+            hipLaunchKernelGGL(k_bf_keys, dim3(gx), dim3(256), lds, s, ds.rows.p + (size_t)base * ds.stride, ds.stride,
+                               ds.dpad, n_used,
                                d_queries + (size_t)q0 * q_stride, q_stride, nq, d_self ? d_self + q0 : nullptr,
                                keys.p);
         }
         FK_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_topk_select, dim3(nq), dim3(FK_SEL_THREADS), 0, s, keys.p, n_used, K,
                            d_out_ids + (size_t)q0 * K, d_out_dists ? d_out_dists + (size_t)q0 * K : nullptr);
+        FK_LAUNCH_CHECK();
+    }
+    if (base) {
+        const size_t n = (size_t)n_q * K;
+        hipLaunchKernelGGL(k_add_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_out_ids, n, base);
         FK_LAUNCH_CHECK();
     }
     FK_HIP(hipStreamSynchronize(s));

@@ -167,9 +167,10 @@ void exclusive_scan_u32_to_u64(hipStream_t s, const uint32_t *in, uint64_t *out,
 void exclusive_scan_u64(hipStream_t s, const uint64_t *in, uint64_t *out, size_t n, uint64_t *total,
                         DevBuf<uint64_t> &tmp);
 
-// Brute-force exact kNN of device queries against rows [0, n_used).
+// Brute-force exact kNN of device queries against rows [base, base + n_used);
+// d_self holds ids relative to base, the output ids are absolute.
 void bruteforce_knn(hipStream_t s, const Dataset &ds, const float *d_queries, uint32_t q_stride,
                     uint32_t n_q, uint32_t n_used, uint32_t K, const uint32_t *d_self,
-                    uint32_t *d_out_ids, float *d_out_dists);
+                    uint32_t *d_out_ids, float *d_out_dists, uint32_t base = 0);
 
 }  // namespace fk

@@ -33,6 +33,29 @@ double now_sec() {
 
 void BuildState::sync() { FK_HIP(hipStreamSynchronize(s)); }
 
+void BuildState::init_sets(const std::vector<uint32_t> &lo) {
+    FK_CHECK_ARG(lo.size() >= 2 && lo.front() == 0 && lo.back() == N, "sets must cover the nodes");
+    n_sets = (uint32_t)lo.size() - 1;
+    set_lo = lo;
+    set_rng.assign(n_sets, ((uint64_t)P.seed << 32) ^ 0x5DEECE66Dull);
+    set_ep.assign(n_sets, 0);
+    set_width.assign(n_sets, 0);
+    set_active.assign(n_sets, 1);
+    set_acc.alloc(2 * (size_t)n_sets);
+    if (n_sets > 1) {
+        std::vector<uint32_t> of(N);
+        for (uint32_t k = 0; k < n_sets; ++k)
+            for (uint32_t v = lo[k]; v < lo[k + 1]; ++v) of[v] = k;
+        d_set_lo.alloc(n_sets + 1);
+        set_of.alloc(N);
+        d_set_active.alloc(n_sets);
+        FK_HIP(hipMemcpyAsync(d_set_lo.p, lo.data(), lo.size() * 4, hipMemcpyHostToDevice, s));
+        FK_HIP(hipMemcpyAsync(set_of.p, of.data(), (size_t)N * 4, hipMemcpyHostToDevice, s));
+        FK_HIP(hipMemcpyAsync(d_set_active.p, set_active.data(), n_sets, hipMemcpyHostToDevice, s));
+        sync();
+    }
+}
+
 hipEvent_t BuildState::ev_get() {
     if (!ev_free.empty()) {
         hipEvent_t e = ev_free.back();
@@ -118,25 +141,36 @@ __device__ void gen_random_block(uint32_t *a, uint32_t size, uint32_t N, uint64_
     __syncthreads();
 }
 
+// With sets, node n of set s draws from the set's own ids, with the streams
+// of its id within the set (a build of the set's rows alone draws the same).
 __global__ __launch_bounds__(64) void k_nnd_init(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
                                                  uint32_t N, uint32_t S, uint32_t cap, uint32_t Lp, uint64_t seed,
+                                                 const uint32_t *__restrict__ set_lo,
+                                                 const uint32_t *__restrict__ set_of,
                                                  uint64_t *__restrict__ pool, uint8_t *__restrict__ pflg,
                                                  uint32_t *__restrict__ cnt, uint32_t *__restrict__ Marr,
                                                  float *__restrict__ thr, uint32_t *__restrict__ lists) {
     __shared__ uint32_t s_a[512];
     __shared__ uint64_t s_k[256];
     const uint32_t n = blockIdx.x;
+    uint32_t lo = 0, Ns = N;
+    if (set_of) {
+        const uint32_t k = set_of[n];
+        lo = set_lo[k];
+        Ns = set_lo[k + 1] - lo;
+    }
+    const uint32_t ln = n - lo;
     // nn_new: 2S random ids (may include n itself, as in the reference)
-    gen_random_block(s_a, 2 * S, N, seed, 2ull * n);
-    for (uint32_t i = threadIdx.x; i < 2 * S; i += blockDim.x) lists[(uint64_t)n * 2 * S + i] = s_a[i];
+    gen_random_block(s_a, 2 * S, Ns, seed, 2ull * ln);
+    for (uint32_t i = threadIdx.x; i < 2 * S; i += blockDim.x) lists[(uint64_t)n * 2 * S + i] = lo + s_a[i];
     __syncthreads();
     // pool: S random ids != n from S + 1 draws
-    gen_random_block(s_a, S + 1, N, seed, 2ull * n + 1);
+    gen_random_block(s_a, S + 1, Ns, seed, 2ull * ln + 1);
     if (threadIdx.x == 0) {
         uint32_t i = 0;
         for (uint32_t l = 0; l < S; ++l) {
-            if (s_a[i] == n) ++i;
-            s_a[256 + l] = s_a[i++];
+            if (s_a[i] == ln) ++i;
+            s_a[256 + l] = lo + s_a[i++];
         }
     }
     __syncthreads();
@@ -456,16 +490,26 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
 // recall = mean over controls of |pool[0..min(L,K)) ∩ gt[0..K)| / K.
 // ---------------------------------------------------------------------------
 // One wavefront per node, grid-stride; one atomic per block.
+// With sets, acc[set] per set (one atomic per node).
 __global__ __launch_bounds__(1024) void k_delta(const uint8_t *__restrict__ pflg, const uint32_t *__restrict__ cnt,
                                                 uint32_t N, uint32_t cap, uint32_t K,
+                                                const uint32_t *__restrict__ set_of,
                                                 unsigned long long *__restrict__ acc) {
     const uint32_t wpb = blockDim.x >> 6;
     unsigned long long c = 0;
     for (uint32_t n = blockIdx.x * wpb + (threadIdx.x >> 6); n < N; n += gridDim.x * wpb) {
         const uint32_t lim = cnt[n] < K ? cnt[n] : K;
         const uint8_t *pf = pflg + (uint64_t)n * cap;
-        for (uint32_t l = lane_id(); l < lim; l += 64) c += (pf[l] & FK_FLG_NEW) ? 1 : 0;
+        unsigned long long cn = 0;
+        for (uint32_t l = lane_id(); l < lim; l += 64) cn += (pf[l] & FK_FLG_NEW) ? 1 : 0;
+        if (set_of) {
+            cn = wave_sum(cn);
+            if (lane_id() == 0 && cn) atomicAdd(&acc[set_of[n]], cn);
+        } else {
+            c += cn;
+        }
     }
+    if (set_of) return;  // uniform
     c = block_sum(c);
     if (threadIdx.x == 0 && c) atomicAdd(acc, c);
 }
@@ -473,7 +517,7 @@ __global__ __launch_bounds__(1024) void k_delta(const uint8_t *__restrict__ pflg
 __global__ __launch_bounds__(256) void k_recall(const uint64_t *__restrict__ pool, const uint32_t *__restrict__ cnt,
                                                 uint32_t cap, const uint32_t *__restrict__ ctrl_ids,
                                                 const uint32_t *__restrict__ gt, uint32_t gt_stride, uint32_t K,
-                                                unsigned long long *__restrict__ acc) {
+                                                uint32_t *__restrict__ found_out) {
     __shared__ uint32_t s_gt[2 * FK_POOL_MAX];
     __shared__ uint32_t s_found;
     const uint32_t c = blockIdx.x;
@@ -493,18 +537,26 @@ __global__ __launch_bounds__(256) void k_recall(const uint64_t *__restrict__ poo
     }
     if (f) atomicAdd(&s_found, f);
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(acc, (unsigned long long)s_found);
+    if (threadIdx.x == 0) found_out[c] = s_found;
 }
 
 // ---------------------------------------------------------------------------
 // update() (src/kgraph.cpp:1118-1202).
 // ---------------------------------------------------------------------------
 // M growth and radiusM.
+// A node of a set that has stopped gets M = 0: it samples nothing and joins
+// nothing from then on.
 __global__ void k_upd_m(const uint64_t *__restrict__ pool, const uint8_t *__restrict__ pflg,
                         const uint32_t *__restrict__ cnt, const uint8_t *__restrict__ found, uint32_t N, uint32_t cap,
-                        uint32_t S, uint32_t *__restrict__ Marr, float *__restrict__ radiusM) {
+                        uint32_t S, const uint32_t *__restrict__ set_of, const uint8_t *__restrict__ set_active,
+                        uint32_t *__restrict__ Marr, float *__restrict__ radiusM) {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
+    if (set_of && !set_active[set_of[n]]) {
+        Marr[n] = 0;
+        radiusM[n] = INFINITY;
+        return;
+    }
     uint32_t M = Marr[n];
     const uint32_t L = cnt[n];
     if (found[n]) {
@@ -618,9 +670,15 @@ __global__ __launch_bounds__(256) void k_rev_select(uint32_t N, uint32_t R, uint
                                                     const uint64_t *__restrict__ list_off,
                                                     const uint32_t *__restrict__ n_new,
                                                     const uint32_t *__restrict__ fcnt_new,
-                                                    const uint32_t *__restrict__ fcnt_old, uint32_t *__restrict__ lists) {
+                                                    const uint32_t *__restrict__ fcnt_old,
+                                                    const uint32_t *__restrict__ set_lo,
+                                                    const uint32_t *__restrict__ set_of,
+                                                    uint32_t *__restrict__ lists) {
     __shared__ uint64_t s_k[FK_SEL_CH];
     const uint32_t o = blockIdx.x;
+    // the hashes take ids within the node's set (a build of the set alone
+    // selects the same entries)
+    const uint32_t lo = set_of ? set_lo[set_of[o]] : 0u;
     for (int which = 0; which < 2; ++which) {
         const uint32_t c = which == 0 ? rcnt_new[o] : rcnt_old[o];
         const uint32_t *src = which == 0 ? rbuf_new + roff_new[o] : rbuf_old + roff_old[o];
@@ -632,7 +690,7 @@ __global__ __launch_bounds__(256) void k_rev_select(uint32_t N, uint32_t R, uint
         }
         // running best-R selection over chunks
         uint32_t have = 0;
-        const uint64_t stream = ((uint64_t)it << 33) | ((uint64_t)which << 32) | o;
+        const uint64_t stream = ((uint64_t)it << 33) | ((uint64_t)which << 32) | (o - lo);
         for (uint32_t c0 = 0; c0 < c; c0 += (FK_SEL_CH - R)) {
             const uint32_t m = (c - c0) < (FK_SEL_CH - R) ? (c - c0) : (FK_SEL_CH - R);
             const uint32_t tot = have + m;
@@ -641,7 +699,7 @@ __global__ __launch_bounds__(256) void k_rev_select(uint32_t N, uint32_t R, uint
                 if (k >= have) {
                     if (k < tot) {
                         const uint32_t s = src[c0 + (k - have)];
-                        s_k[k] = ((uint64_t)fk_rand32(seed, stream, s) << 32) | s;
+                        s_k[k] = ((uint64_t)fk_rand32(seed, stream, s - lo) << 32) | s;
                     } else {
                         s_k[k] = FK_KEY_EMPTY;
                     }
@@ -715,53 +773,74 @@ void nnd_alloc(BuildState &b) {
 }
 
 // GenerateControl (src/kgraph.cpp:363-387): `controls` distinct random nodes
-// and their exact MAXL nearest neighbours (self excluded, ties by id).
+// and their exact MAXL nearest neighbours (self excluded, ties by id); per
+// set, within the set, with the set's random stream.
 void nnd_controls(BuildState &b) {
     const double t0 = now_sec();
-    b.n_ctrl = (b.P.flags & FK_FLAG_NO_CONTROLS) ? 0 : b.P.controls;
+    const uint32_t nc = (b.P.flags & FK_FLAG_NO_CONTROLS) ? 0 : b.P.controls;
+    b.n_ctrl = nc * b.n_sets;
+    b.ctrl_lo.assign(b.n_sets + 1, 0);
+    for (uint32_t k = 0; k <= b.n_sets; ++k) b.ctrl_lo[k] = k * nc;
     if (b.n_ctrl == 0) return;
-    std::vector<uint32_t> idx(b.N);
-    for (uint32_t i = 0; i < b.N; ++i) idx[i] = i;
-    // partial Fisher-Yates with the build's host stream
-    for (uint32_t i = 0; i < b.n_ctrl; ++i) {
-        const uint32_t j = i + (uint32_t)(b.next_rand() % (b.N - i));
-        std::swap(idx[i], idx[j]);
-    }
-    std::vector<uint32_t> ids(idx.begin(), idx.begin() + b.n_ctrl);
     b.ctrl_ids.alloc(b.n_ctrl);
     b.ctrl_gt.alloc((size_t)b.n_ctrl * b.P.MAXL);
-    FK_HIP(hipMemcpyAsync(b.ctrl_ids.p, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, b.s));
-    // control rows as queries
-    DevBuf<float> q((size_t)b.n_ctrl * b.ds->dpad);
-    for (uint32_t c = 0; c < b.n_ctrl; ++c)
-        FK_HIP(hipMemcpyAsync(q.p + (size_t)c * b.ds->dpad, b.ds->rows.p + (size_t)ids[c] * b.ds->stride,
-                              (size_t)b.ds->dpad * 4, hipMemcpyDeviceToDevice, b.s));
-    const uint32_t K = b.P.MAXL < b.N - 1 ? b.P.MAXL : b.N - 1;
-    if (K < b.P.MAXL) FK_HIP(hipMemsetAsync(b.ctrl_gt.p, 0xFF, b.ctrl_gt.bytes(), b.s));
-    DevBuf<uint32_t> gt((size_t)b.n_ctrl * K);
-    hipEvent_t pe = b.prof_begin();
-    bruteforce_knn(b.s, *b.ds, q.p, b.ds->dpad, b.n_ctrl, b.N, K, b.ctrl_ids.p, gt.p, nullptr);
-    // algorithmic bytes: every row read once, the K ids written per control
-    b.prof_end(pe, FK_PROF_CONTROLS, (double)b.N * 4.0 * b.ds->d + (double)b.n_ctrl * K * 4.0);
-    FK_HIP(hipMemcpy2DAsync(b.ctrl_gt.p, (size_t)b.P.MAXL * 4, gt.p, (size_t)K * 4, (size_t)K * 4, b.n_ctrl,
-                            hipMemcpyDeviceToDevice, b.s));
-    b.sync();
-    b.st->d_ctrl += (uint64_t)b.n_ctrl * (b.N - 1);
+    DevBuf<float> q((size_t)nc * b.ds->dpad);
+    DevBuf<uint32_t> dself(nc);
+    for (uint32_t k = 0; k < b.n_sets; ++k) {
+        const uint32_t lo = b.set_lo[k], Ns = b.set_n(k);
+        std::vector<uint32_t> idx(Ns);
+        for (uint32_t i = 0; i < Ns; ++i) idx[i] = i;
+        // partial Fisher-Yates with the set's host stream
+        for (uint32_t i = 0; i < nc; ++i) {
+            const uint32_t j = i + (uint32_t)(b.next_rand(k) % (Ns - i));
+            std::swap(idx[i], idx[j]);
+        }
+        std::vector<uint32_t> local(idx.begin(), idx.begin() + nc), ids(nc);
+        for (uint32_t c = 0; c < nc; ++c) ids[c] = lo + local[c];
+        FK_HIP(hipMemcpyAsync(b.ctrl_ids.p + b.ctrl_lo[k], ids.data(), (size_t)nc * 4, hipMemcpyHostToDevice, b.s));
+        FK_HIP(hipMemcpyAsync(dself.p, local.data(), (size_t)nc * 4, hipMemcpyHostToDevice, b.s));
+        // control rows as queries
+        for (uint32_t c = 0; c < nc; ++c)
+            FK_HIP(hipMemcpyAsync(q.p + (size_t)c * b.ds->dpad, b.ds->rows.p + (size_t)ids[c] * b.ds->stride,
+                                  (size_t)b.ds->dpad * 4, hipMemcpyDeviceToDevice, b.s));
+        const uint32_t K = b.P.MAXL < Ns - 1 ? b.P.MAXL : Ns - 1;
+        uint32_t *gt_out = b.ctrl_gt.p + (size_t)b.ctrl_lo[k] * b.P.MAXL;
+        if (K < b.P.MAXL) FK_HIP(hipMemsetAsync(gt_out, 0xFF, (size_t)nc * b.P.MAXL * 4, b.s));
+        DevBuf<uint32_t> gt((size_t)nc * K);
+        hipEvent_t pe = b.prof_begin();
+        bruteforce_knn(b.s, *b.ds, q.p, b.ds->dpad, nc, Ns, K, dself.p, gt.p, nullptr, lo);
+        // algorithmic bytes: every row read once, the K ids written per control
+        b.prof_end(pe, FK_PROF_CONTROLS, (double)Ns * 4.0 * b.ds->d + (double)nc * K * 4.0);
+        FK_HIP(hipMemcpy2DAsync(gt_out, (size_t)b.P.MAXL * 4, gt.p, (size_t)K * 4, (size_t)K * 4, nc,
+                                hipMemcpyDeviceToDevice, b.s));
+        b.sync();
+        b.st->d_ctrl += (uint64_t)nc * (Ns - 1);
+    }
     b.st->t_ctrl += now_sec() - t0;
 }
 
-float nnd_evaluate(BuildState &b, uint32_t K) {
+float nnd_evaluate(BuildState &b, uint32_t K, std::vector<float> *per_set) {
+    if (per_set) per_set->assign(b.n_sets, 0.f);
     if (b.n_ctrl == 0) return 0.f;
     FK_CHECK_ARG(K <= FK_POOL_MAX * 2, "K too large for recall evaluation");
-    FK_HIP(hipMemsetAsync(b.acc.p, 0, 8, b.s));
+    DevBuf<uint32_t> found(b.n_ctrl);
     hipLaunchKernelGGL(k_recall, dim3(b.n_ctrl), dim3(256), 0, b.s, b.pool.p, b.cnt.p, b.cap, b.ctrl_ids.p,
-                       b.ctrl_gt.p, b.P.MAXL, K, b.acc.p);
+                       b.ctrl_gt.p, b.P.MAXL, K, found.p);
     FK_LAUNCH_CHECK();
-    unsigned long long f = 0;
-    FK_HIP(hipMemcpyAsync(&f, b.acc.p, 8, hipMemcpyDeviceToHost, b.s));
+    std::vector<uint32_t> f(b.n_ctrl);
+    FK_HIP(hipMemcpyAsync(f.data(), found.p, (size_t)b.n_ctrl * 4, hipMemcpyDeviceToHost, b.s));
     b.sync();
-    // mean over controls of found/K (EvaluateRecall, src/kgraph.cpp:113-169)
-    return (float)((double)f / K / b.n_ctrl);
+    // mean over controls of found/K (EvaluateRecall, src/kgraph.cpp:113-169),
+    // per set and over all controls
+    unsigned long long all = 0;
+    for (uint32_t k = 0; k < b.n_sets; ++k) {
+        unsigned long long fs = 0;
+        for (uint32_t c = b.ctrl_lo[k]; c < b.ctrl_lo[k + 1]; ++c) fs += f[c];
+        all += fs;
+        const uint32_t ns = b.ctrl_lo[k + 1] - b.ctrl_lo[k];
+        if (per_set && ns) (*per_set)[k] = (float)((double)fs / K / ns);
+    }
+    return (float)((double)all / K / b.n_ctrl);
 }
 
 // Exclusive scan of n counts into out[0..n], out[n] = total; returns total.
@@ -880,7 +959,7 @@ static void nnd_update(BuildState &b, uint32_t it) {
     const unsigned gw = (N + 3) / 4;  // 4 waves per 256-thread block
     hipEvent_t pe = b.prof_begin();
     hipLaunchKernelGGL(k_upd_m, dim3(g256), dim3(256), 0, b.s, b.pool.p, b.pflg.p, b.cnt.p, b.found.p, N, b.cap,
-                       b.P.S, b.M.p, b.radiusM.p);
+                       b.P.S, b.dev_set_of(), b.n_sets > 1 ? b.d_set_active.p : nullptr, b.M.p, b.radiusM.p);
     FK_LAUNCH_CHECK();
     zero_u32(b, b.rcnt_new, N);
     zero_u32(b, b.rcnt_old, N);
@@ -910,7 +989,7 @@ static void nnd_update(BuildState &b, uint32_t it) {
     FK_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_rev_select, dim3(N), dim3(256), 0, b.s, N, b.P.R, (uint64_t)b.P.seed, it, b.rcnt_new.p,
                        b.rcnt_old.p, b.roff_new.p, b.roff_old.p, b.rbuf_new.p, b.rbuf_old.p, b.list_off.p,
-                       b.n_new.p, b.fcnt_new.p, b.fcnt_old.p, b.lists.p);
+                       b.n_new.p, b.fcnt_new.p, b.fcnt_old.p, b.dev_set_lo(), b.dev_set_of(), b.lists.p);
     FK_LAUNCH_CHECK();
     // the first M entries of every pool read twice (keys + flags), lists and
     // reverse buckets written once
@@ -944,7 +1023,8 @@ void nnd_run(BuildState &b) {
     }
     hipEvent_t pe0 = b.prof_begin();
     hipLaunchKernelGGL(k_nnd_init, dim3(N), dim3(64), 0, b.s, b.ds->rows.p, b.ds->stride, b.ds->dpad, N, P.S, b.cap,
-                       P.L, (uint64_t)P.seed, b.pool.p, b.pflg.p, b.cnt.p, b.M.p, b.thr.p, b.lists.p);
+                       P.L, (uint64_t)P.seed, b.dev_set_lo(), b.dev_set_of(), b.pool.p, b.pflg.p, b.cnt.p, b.M.p,
+                       b.thr.p, b.lists.p);
     FK_LAUNCH_CHECK();
     b.prof_end(pe0, FK_PROF_INIT, (double)N * ((P.S + 1) * (4.0 * b.ds->d) + P.S * 9.0 + 2 * P.S * 4.0));
     b.st->d_init += (uint64_t)N * P.S;
@@ -954,25 +1034,47 @@ void nnd_run(BuildState &b) {
         nnd_join(b);
         // the screen learns from the first join's exact pair distances
         if (it == 0 && (P.flags & FK_FLAG_SCREEN) && P.iterations > 1) screen_train(b);
-        // statistics
-        FK_HIP(hipMemsetAsync(b.acc.p, 0, 8, b.s));
-        hipLaunchKernelGGL(k_delta, dim3(reduce_grid(N)), dim3(1024), 0, b.s, b.pflg.p, b.cnt.p, N, b.cap, P.K, b.acc.p);
+        // statistics and the stop rule, per set (EvaluateDelta / EvaluateRecall)
+        FK_HIP(hipMemsetAsync(b.set_acc.p, 0, 8 * (size_t)b.n_sets, b.s));
+        hipLaunchKernelGGL(k_delta, dim3(reduce_grid(N)), dim3(1024), 0, b.s, b.pflg.p, b.cnt.p, N, b.cap, P.K,
+                           b.dev_set_of(), b.set_acc.p);
         FK_LAUNCH_CHECK();
-        unsigned long long fl = 0;
-        FK_HIP(hipMemcpyAsync(&fl, b.acc.p, 8, hipMemcpyDeviceToHost, b.s));
+        std::vector<unsigned long long> fl(b.n_sets);
+        FK_HIP(hipMemcpyAsync(fl.data(), b.set_acc.p, 8 * (size_t)b.n_sets, hipMemcpyDeviceToHost, b.s));
         b.sync();
-        const float delta = (float)((double)fl / P.K / N);
-        const float recall = nnd_evaluate(b, P.K);
+        std::vector<float> rec_set;
+        const float recall = nnd_evaluate(b, P.K, &rec_set);
+        unsigned long long fl_all = 0;
+        uint64_t n_all = 0;
+        uint32_t n_active = 0;
+        float delta = 0.f;
+        for (uint32_t k = 0; k < b.n_sets; ++k) {
+            if (!b.set_active[k]) continue;
+            fl_all += fl[k];
+            n_all += b.set_n(k);
+            const float dk = (float)((double)fl[k] / P.K / b.set_n(k));
+            uint32_t stop = 0;
+            if (dk <= P.delta) stop = 1;
+            else if (b.ctrl_lo[k + 1] > b.ctrl_lo[k] && rec_set[k] >= P.recall) stop = 2;
+            if (b.n_sets == 1) delta = dk;
+            if (stop || it + 1 >= P.iterations) {
+                b.set_active[k] = 0;
+                if (b.n_sets == 1) b.st->nnd_stop = stop;
+            } else {
+                ++n_active;
+            }
+        }
+        if (b.n_sets > 1) delta = (float)((double)fl_all / P.K / (double)n_all);
         b.st->nnd_delta = delta;
         b.st->nnd_recall = recall;
         if (b.st->n_kgraph < FK_MAX_LOOPS + 64) b.st->kgraph_time[b.st->n_kgraph++] = now_sec() - b.t0;
         if (b.st->n_recall < FK_MAX_LOOPS + 64) b.st->recall[b.st->n_recall++] = recall;
         if (b.verbose)
-            fprintf(stderr, "[fk] iteration: %u recall: %g delta: %g time: %g\n", it + 1, recall, delta,
-                    now_sec() - tk0);
-        if (delta <= P.delta) { b.st->nnd_stop = 1; break; }
-        if (b.n_ctrl && recall >= P.recall) { b.st->nnd_stop = 2; break; }
-        if (it + 1 >= P.iterations) break;
+            fprintf(stderr, "[fk] iteration: %u recall: %g delta: %g time: %g (%u of %u sets go on)\n", it + 1,
+                    recall, delta, now_sec() - tk0, n_active, b.n_sets);
+        if (n_active == 0) break;
+        if (b.n_sets > 1)
+            FK_HIP(hipMemcpyAsync(b.d_set_active.p, b.set_active.data(), b.n_sets, hipMemcpyHostToDevice, b.s));
         // from the third join on, the pools' heads are close enough for a
         // useful locality order
         if (it >= 1) locality_order(b);
@@ -1003,8 +1105,11 @@ __global__ void k_bv_init(uint8_t *__restrict__ pflg, const uint32_t *__restrict
     }
 }
 
+// Flagged window entries, capped at S per node and all; with sets, into
+// acc[2 set] and acc[2 set + 1] (one atomic pair per node).
 __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ pflg, const uint32_t *__restrict__ cnt,
                                                    uint32_t N, uint32_t cap, uint32_t W, uint32_t S,
+                                                   const uint32_t *__restrict__ set_of,
                                                    unsigned long long *__restrict__ acc) {
     const uint32_t wpb = blockDim.x >> 6;
     unsigned long long capped = 0, all = 0;
@@ -1015,10 +1120,19 @@ __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ p
         for (uint32_t j = lane_id(); j < lim; j += 64) f += (pf[j] & FK_FLG_NEW) ? 1u : 0u;
         f = wave_sum(f);
         if (lane_id() == 0) {
-            capped += f < S ? f : S;
-            all += f;
+            if (set_of) {
+                if (f) {
+                    const uint32_t k = set_of[n];
+                    atomicAdd(&acc[2 * k], (unsigned long long)(f < S ? f : S));
+                    atomicAdd(&acc[2 * k + 1], (unsigned long long)f);
+                }
+            } else {
+                capped += f < S ? f : S;
+                all += f;
+            }
         }
     }
+    if (set_of) return;  // uniform
     capped = block_sum(capped);
     all = block_sum(all);
     if (threadIdx.x == 0) {
@@ -1159,7 +1273,7 @@ __device__ uint32_t bvw_merge(uint64_t *p, uint8_t *pf, uint32_t np, uint64_t *c
 template <int NB, bool DOT>
 __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
     const float *__restrict__ chains, const float2 *__restrict__ nrm, uint32_t dpad, uint32_t N, uint32_t cap,
-    uint32_t W, uint32_t S_cap,
+    uint32_t W, uint32_t S_cap_all, const uint32_t *__restrict__ set_of, const uint32_t *__restrict__ set_scap,
     uint64_t *__restrict__ pool, uint8_t *__restrict__ pflg, uint32_t *__restrict__ cnt,
     const uint32_t *__restrict__ nsg_id, const uint32_t *__restrict__ nsg_len, const uint8_t *__restrict__ nsg_new,
     uint32_t ncap, unsigned long long *__restrict__ cc_acc, const uint32_t *__restrict__ order,
@@ -1170,6 +1284,8 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
     const uint32_t pos = (order ? xcd_position(per_xcd) : blockIdx.x) * wpb + w;
     if (pos >= N) return;
     const uint32_t i = order ? order[pos] : pos;
+    // a set whose rounds are over expands nothing (S_cap 0)
+    const uint32_t S_cap = set_of ? set_scap[set_of[i]] : S_cap_all;
     uint8_t *wb = smem + (size_t)w * wave_bytes;
     uint64_t *p = reinterpret_cast<uint64_t *>(wb);
     uint64_t *c = p + cap;
@@ -1311,7 +1427,7 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
     if (lane == 0 && cc) atomicAdd(cc_acc, cc);  // the same count in every lane
 }
 
-static void launch_bv_round(BuildState &b, uint32_t W, uint32_t scap) {
+static void launch_bv_round(BuildState &b, uint32_t W, uint32_t scap, const uint32_t *set_scap) {
     const uint32_t *ord = b.has_order ? b.order.p : nullptr;
     const uint32_t wave_bytes = (uint32_t)bvw_wave_bytes(b.cap, W);
     uint32_t wpb = (64u * 1024u) / wave_bytes;
@@ -1324,7 +1440,8 @@ static void launch_bv_round(BuildState &b, uint32_t W, uint32_t scap) {
     const float2 *nrm = exact_norms(b);
 #define FK_BV_LAUNCH(NBV, DOT)                                                                                   \
     hipLaunchKernelGGL((k_bv_round<NBV, DOT>), dim3(grid), dim3(64 * wpb), lds, b.s, b.ds->chains.p, nrm,       \
-                       b.ds->dpad, b.N, b.cap, W, scap, b.pool.p, b.pflg.p, b.cnt.p, b.nsg_id.p, b.nsg_len.p,   \
+                       b.ds->dpad, b.N, b.cap, W, scap, b.dev_set_of(), set_scap, b.pool.p, b.pflg.p, b.cnt.p,  \
+                       b.nsg_id.p, b.nsg_len.p,                                                                 \
                        b.nsg_new.p, b.ncap, b.acc.p, ord, mark, wave_bytes, wpb)
     switch (b.ds->dpad) {
         case 32: FK_BV_LAUNCH(4, false); break;
@@ -1351,26 +1468,42 @@ void bridgeview(BuildState &b) {
     const uint32_t S = P.massq_S;
     uint64_t cc_total = 0;
     uint32_t rounds = 0;
+    // per set: the rounds run until the set's own last round
+    std::vector<uint8_t> done(b.n_sets, 0);
+    std::vector<uint32_t> scap_set(b.n_sets);
+    DevBuf<uint32_t> d_scap(b.n_sets);
     while (true) {
-        FK_HIP(hipMemsetAsync(b.acc.p, 0, 16, b.s));
+        FK_HIP(hipMemsetAsync(b.set_acc.p, 0, 16 * (size_t)b.n_sets, b.s));
         hipLaunchKernelGGL(k_bv_count, dim3(reduce_grid(N)), dim3(1024), 0, b.s, b.pflg.p, b.cnt.p, N, b.cap, W, S,
-                           b.acc.p);
+                           b.dev_set_of(), b.set_acc.p);
         FK_LAUNCH_CHECK();
-        unsigned long long cnts[2] = {0, 0};
-        FK_HIP(hipMemcpyAsync(cnts, b.acc.p, 16, hipMemcpyDeviceToHost, b.s));
+        std::vector<unsigned long long> sc(2 * (size_t)b.n_sets);
+        FK_HIP(hipMemcpyAsync(sc.data(), b.set_acc.p, 16 * (size_t)b.n_sets, hipMemcpyDeviceToHost, b.s));
         b.sync();
         // delta = cnt / nd_ / S_ in float (src/kgraph.cpp:2492); below 0.5 the
         // round also expands every remaining flagged entry and is the last.
-        const float delta = (float)cnts[0] / (float)N / (float)S;
-        const bool last = delta < 0.5f;
-        const uint32_t scap = last ? 0xFFFFFFFFu : S;
+        unsigned long long cnts[2] = {0, 0};
+        bool last = true;
+        for (uint32_t k = 0; k < b.n_sets; ++k) {
+            if (done[k]) { scap_set[k] = 0; continue; }
+            const float delta = (float)sc[2 * k] / (float)b.set_n(k) / (float)S;
+            const bool lk = delta < 0.5f;
+            scap_set[k] = lk ? 0xFFFFFFFFu : S;
+            if (lk) done[k] = 1;
+            else last = false;
+            cnts[0] += sc[2 * k];
+            cnts[1] += sc[2 * k + 1];
+        }
+        const uint32_t scap = scap_set[0];
+        if (b.n_sets > 1)
+            FK_HIP(hipMemcpyAsync(d_scap.p, scap_set.data(), (size_t)b.n_sets * 4, hipMemcpyHostToDevice, b.s));
         FK_HIP(hipMemsetAsync(b.acc.p, 0, 8, b.s));
         if (b.prof && b.bv_mark.n < N) {
             b.bv_mark.alloc(N);
             FK_HIP(hipMemsetAsync(b.bv_mark.p, 0, (size_t)N * 4, b.s));
         }
         hipEvent_t pe = b.prof_begin();
-        launch_bv_round(b, W, scap);
+        launch_bv_round(b, W, scap, b.n_sets > 1 ? d_scap.p : nullptr);
         FK_LAUNCH_CHECK();
         b.prof_end(pe, FK_PROF_BV, 0.0);
         unsigned long long cc = 0;

@@ -70,8 +70,13 @@ struct BeamArgs {
     const uint32_t *g_len;     // fixed stride mode
     uint32_t g_cap;
     const uint64_t *g_off;     // CSR mode when non-null
-    uint32_t *visited;         // n_q x ((N + 31) / 32) words, zeroed
+    uint32_t *visited;         // n_q x ((N + 31) / 32) words, zeroed (or per-query ranges below)
     uint64_t seed;
+    // per-query id ranges (sets): query q searches ids [q_lo[q], q_lo[q] + q_n[q]),
+    // starts from pool(q_init[q]), fills with stream 0 of q_seed[q] (what a
+    // build of the set alone does), and owns visited words from q_vis[q]
+    const uint32_t *q_lo, *q_n, *q_init;
+    const uint64_t *q_seed, *q_vis;
     uint32_t *out_ids;         // n_q x K
     float *out_d;
     unsigned long long *n_dist;
@@ -94,32 +99,37 @@ __global__ __launch_bounds__(64) void k_beam(BeamArgs a) {
     const uint32_t lane = threadIdx.x;
     const float *qv = a.queries + (size_t)q * a.dpad;
     for (uint32_t k = lane; k < a.dpad; k += 64) s_q[k] = qv[k];
-    uint32_t *bm = a.visited + (size_t)q * ((a.N + 31) / 32);
+    const uint32_t lo = a.q_lo ? a.q_lo[q] : 0u, Nq = a.q_lo ? a.q_n[q] : a.N;
+    // the visited bitmap indexes ids relative to lo
+    uint32_t *bm = a.visited + (a.q_vis ? a.q_vis[q] : (size_t)q * ((a.N + 31) / 32));
     const uint32_t L = a.L;
     // initial ids
     if (lane == 0) {
         uint32_t n = 0;
         if (a.init_mode == 0) {
-            const uint32_t pc = a.pool_cnt[a.init_node];
-            const uint64_t *pr = a.pool + (uint64_t)a.init_node * a.pool_cap;
+            const uint32_t init = a.q_init ? a.q_init[q] : a.init_node;
+            const uint32_t pc = a.pool_cnt[init];
+            const uint64_t *pr = a.pool + (uint64_t)init * a.pool_cap;
             for (uint32_t i = 0; i < pc && n < L; ++i) {
                 const uint32_t id = key_id(pr[i]);
-                if (id < a.N && test_and_set(bm, id)) s_init[n++] = id;
+                if (id < a.N && test_and_set(bm, id - lo)) s_init[n++] = id;
             }
         } else {
-            if (test_and_set(bm, a.ep)) s_init[n++] = a.ep;
+            if (test_and_set(bm, a.ep - lo)) s_init[n++] = a.ep;
             const uint64_t o = a.g_off ? a.g_off[a.ep] : (uint64_t)a.ep * a.g_cap;
             const uint32_t deg = a.g_off ? (uint32_t)(a.g_off[a.ep + 1] - o) : a.g_len[a.ep];
             for (uint32_t m = 0; m < deg && n < L; ++m) {
                 const uint32_t id = a.g_ids[o + m];
-                if (id < a.N && test_and_set(bm, id)) s_init[n++] = id;
+                if (id < a.N && test_and_set(bm, id - lo)) s_init[n++] = id;
             }
         }
         uint64_t k = 0;
-        const uint32_t want = L < a.N ? L : a.N;
+        const uint32_t want = L < Nq ? L : Nq;
+        const uint64_t seed = a.q_seed ? a.q_seed[q] : a.seed;
+        const uint64_t stream = a.q_seed ? 0u : q;
         while (n < want) {
-            const uint32_t id = fk_rand32(a.seed, q, k++) % a.N;
-            if (test_and_set(bm, id)) s_init[n++] = id;
+            const uint32_t id = lo + fk_rand32(seed, stream, k++) % Nq;
+            if (test_and_set(bm, id - lo)) s_init[n++] = id;
         }
         s_n = n;
     }
@@ -155,7 +165,7 @@ __global__ __launch_bounds__(64) void k_beam(BeamArgs a) {
                 const float worst = key_dist(s_key[Lc - 1]);
                 if (m < deg) {
                     const uint32_t id = a.g_ids[o + m];
-                    if (id < a.N && test_and_set(bm, id)) {
+                    if (id < a.N && test_and_set(bm, id - lo)) {
                         ++nd;
                         const float dd = l2_canon(s_q, a.rows + (size_t)id * a.stride, a.dpad);
                         if (dd < worst) cand = make_key(dd, id);  // `dist >= pool[L-1].dist` skips
@@ -212,41 +222,65 @@ static void launch_beam(hipStream_t s, const BeamArgs &a) {
     FK_LAUNCH_CHECK();
 }
 
-// Column means of all resident rows into center[0, d) (the caller zeroes the
-// padding); synchronises the stream.
-void column_mean(hipStream_t s, const Dataset &ds, float *center) {
-    const uint32_t nb = (ds.n_total + FK_CEN_ROWS - 1) / FK_CEN_ROWS;
+// Column means of the resident rows [r0, r0 + n) into center[0, d) (the
+// caller zeroes the padding); synchronises the stream.
+void column_mean(hipStream_t s, const Dataset &ds, float *center, uint32_t r0, uint32_t n) {
+    if (n == 0) n = ds.n_total - r0;
+    const uint32_t nb = (n + FK_CEN_ROWS - 1) / FK_CEN_ROWS;
     DevBuf<double> part((size_t)nb * ds.d);
-    hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, s, ds.rows.p, ds.stride, ds.d, ds.n_total, part.p);
+    hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, s, ds.rows.p + (size_t)r0 * ds.stride, ds.stride, ds.d, n,
+                       part.p);
     FK_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_colsum_final, dim3((ds.d + 255) / 256), dim3(256), 0, s, part.p, nb, ds.d, ds.n_total,
-                       center);
+    hipLaunchKernelGGL(k_colsum_final, dim3((ds.d + 255) / 256), dim3(256), 0, s, part.p, nb, ds.d, n, center);
     FK_LAUNCH_CHECK();
     FK_HIP(hipStreamSynchronize(s));
 }
 
-// ComputeEp (src/kgraph.cpp:1528-1536).
+// ComputeEp (src/kgraph.cpp:1528-1536), per set: the centroid of the set's
+// rows (all resident rows for a single build: Calcenter averages the whole
+// matrix, not the oracle's prefix), then a beam search from a random node's
+// pool, one query per set.
 void compute_ep(BuildState &b) {
     const double t0 = now_sec();
     const Dataset &ds = *b.ds;
-    DevBuf<float> center(ds.dpad);
+    const uint32_t ns = b.n_sets;
+    DevBuf<float> center((size_t)ns * ds.dpad);
     FK_HIP(hipMemsetAsync(center.p, 0, center.bytes(), b.s));
     hipEvent_t pe = b.prof_begin();
-    column_mean(b.s, ds, center.p);
-    DevBuf<uint32_t> vis((b.N + 31) / 32);
+    std::vector<uint32_t> q_lo(ns), q_n(ns), q_init(ns);
+    std::vector<uint64_t> q_seed(ns), q_vis(ns);
+    uint64_t words = 0;
+    double center_rows = 0;
+    for (uint32_t k = 0; k < ns; ++k) {
+        const uint32_t lo = b.set_lo[k], n = b.set_n(k);
+        if (ns == 1) column_mean(b.s, ds, center.p, 0, ds.n_total);
+        else column_mean(b.s, ds, center.p + (size_t)k * ds.dpad, lo, n);
+        center_rows += ns == 1 ? ds.n_total : n;
+        q_lo[k] = lo;
+        q_n[k] = n;
+        q_init[k] = lo + (uint32_t)(b.next_rand(k) % n);
+        q_seed[k] = b.next_rand(k);
+        q_vis[k] = words;
+        words += (n + 31) / 32;
+    }
+    DevBuf<uint32_t> vis(words), oid(ns), dq(3 * (size_t)ns);
+    DevBuf<uint64_t> dq64(2 * (size_t)ns);
     FK_HIP(hipMemsetAsync(vis.p, 0, vis.bytes(), b.s));
-    DevBuf<uint32_t> oid(1);
+    FK_HIP(hipMemcpyAsync(dq.p, q_lo.data(), (size_t)ns * 4, hipMemcpyHostToDevice, b.s));
+    FK_HIP(hipMemcpyAsync(dq.p + ns, q_n.data(), (size_t)ns * 4, hipMemcpyHostToDevice, b.s));
+    FK_HIP(hipMemcpyAsync(dq.p + 2 * ns, q_init.data(), (size_t)ns * 4, hipMemcpyHostToDevice, b.s));
+    FK_HIP(hipMemcpyAsync(dq64.p, q_seed.data(), (size_t)ns * 8, hipMemcpyHostToDevice, b.s));
+    FK_HIP(hipMemcpyAsync(dq64.p + ns, q_vis.data(), (size_t)ns * 8, hipMemcpyHostToDevice, b.s));
     BeamArgs a{};
     a.rows = ds.rows.p;
     a.stride = ds.stride;
     a.dpad = ds.dpad;
     a.N = b.N;
     a.queries = center.p;
-    a.n_q = 1;
+    a.n_q = ns;
     a.L = b.P.search_L;
     a.K = 1;
     a.init_mode = 0;
-    a.init_node = (uint32_t)(b.next_rand() % b.N);
     a.pool = b.pool.p;
     a.pool_cnt = b.cnt.p;
     a.pool_cap = b.cap;
@@ -254,17 +288,22 @@ void compute_ep(BuildState &b) {
     a.g_len = b.nsg_len.p;
     a.g_cap = b.ncap;
     a.visited = vis.p;
-    a.seed = b.next_rand();
+    a.q_lo = dq.p;
+    a.q_n = dq.p + ns;
+    a.q_init = dq.p + 2 * ns;
+    a.q_seed = dq64.p;
+    a.q_vis = dq64.p + ns;
     a.out_ids = oid.p;
     a.n_dist = b.acc.p + 4;
     FK_HIP(hipMemsetAsync(b.acc.p + 4, 0, 8, b.s));
     launch_beam(b.s, a);
     unsigned long long nd = 0;
-    FK_HIP(hipMemcpyAsync(&b.ep, oid.p, 4, hipMemcpyDeviceToHost, b.s));
+    FK_HIP(hipMemcpyAsync(b.set_ep.data(), oid.p, (size_t)ns * 4, hipMemcpyDeviceToHost, b.s));
     FK_HIP(hipMemcpyAsync(&nd, b.acc.p + 4, 8, hipMemcpyDeviceToHost, b.s));
     b.sync();
-    // the centroid reads every resident row once; the search one row per distance
-    b.prof_end(pe, FK_PROF_EP, (double)ds.n_total * 4.0 * ds.d + (double)nd * 4.0 * ds.d);
+    b.ep = b.set_ep[0];
+    // the centroids read every row once; the search one row per distance
+    b.prof_end(pe, FK_PROF_EP, center_rows * 4.0 * ds.d + (double)nd * 4.0 * ds.d);
     b.prof_flush();
     b.st->d_ep += nd;
     b.st->t_ep += now_sec() - t0;
@@ -287,6 +326,11 @@ void compute_ep(BuildState &b) {
 // root choices (a sequential walk over degrees) run on the host.
 // ---------------------------------------------------------------------------
 #define FK_REACHED 0xFFFFFFFFu
+
+__global__ void k_mark_reached(const uint32_t *__restrict__ ids, uint32_t n, uint32_t *__restrict__ label) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) label[ids[i]] = FK_REACHED;
+}
 
 __global__ void k_bfs_expand(const uint32_t *__restrict__ nsg_id, const uint32_t *__restrict__ nsg_len,
                              uint32_t ncap, const uint32_t *__restrict__ front, uint32_t nf,
@@ -358,10 +402,17 @@ void tree_grow(BuildState &b) {
     // 1. nodes reachable from ep are marked FK_REACHED (frontier BFS)
     hipLaunchKernelGGL(k_label_init, dim3((N + 255) / 256), dim3(256), 0, b.s, label.p, N);
     FK_LAUNCH_CHECK();
-    const uint32_t reached_mark = FK_REACHED;
-    FK_HIP(hipMemcpyAsync(label.p + b.ep, &reached_mark, 4, hipMemcpyHostToDevice, b.s));
-    FK_HIP(hipMemcpyAsync(f0.p, &b.ep, 4, hipMemcpyHostToDevice, b.s));
-    uint32_t nf = 1, levels = 0;
+    // every set's entry point is a source (edges never leave a set)
+    {
+        std::vector<uint32_t> marks(b.n_sets, FK_REACHED);
+        DevBuf<uint32_t> eps(b.n_sets);
+        FK_HIP(hipMemcpyAsync(eps.p, b.set_ep.data(), (size_t)b.n_sets * 4, hipMemcpyHostToDevice, b.s));
+        FK_HIP(hipMemcpyAsync(f0.p, b.set_ep.data(), (size_t)b.n_sets * 4, hipMemcpyHostToDevice, b.s));
+        hipLaunchKernelGGL(k_mark_reached, dim3((b.n_sets + 255) / 256), dim3(256), 0, b.s, eps.p, b.n_sets, label.p);
+        FK_LAUNCH_CHECK();
+        b.sync();
+    }
+    uint32_t nf = b.n_sets, levels = 0;
     uint32_t *cur = f0.p, *nxt = f1.p;
     while (nf) {
         FK_HIP(hipMemsetAsync(cnt.p, 0, 4, b.s));
@@ -421,13 +472,15 @@ void tree_grow(BuildState &b) {
                 if (p < N && reached(p) && deg[p] < b.P.nsg_R) { root = p; break; }
             }
             if (root == N) {
+                // a random reached node of u's set, with the set's stream
+                const uint32_t k = b.set_of_node(u), lo = b.set_lo[k], ns = b.set_n(k);
                 uint64_t tries = 0;
                 while (true) {
                     ++tries;
-                    const uint64_t factor = tries / N + 1;
-                    const uint32_t rid = (uint32_t)(b.next_rand() % N);
+                    const uint64_t factor = tries / ns + 1;
+                    const uint32_t rid = lo + (uint32_t)(b.next_rand(k) % ns);
                     if (reached(rid) && deg[rid] < b.ncap && deg[rid] < b.P.nsg_R * factor) { root = rid; break; }
-                    if (tries > 64ull * N) throw Error(FK_E_INTERNAL, "tree_grow: no node can take an edge");
+                    if (tries > 64ull * ns) throw Error(FK_E_INTERNAL, "tree_grow: no node can take an edge");
                 }
             }
             e_src.push_back(root);
@@ -449,6 +502,8 @@ void tree_grow(BuildState &b) {
     uint32_t mx = b.width;
     for (uint32_t i = 0; i < N; ++i) mx = std::max(mx, deg[i]);
     b.width = mx;
+    for (uint32_t k = 0; k < b.n_sets; ++k)
+        for (uint32_t i = b.set_lo[k]; i < b.set_lo[k + 1]; ++i) b.set_width[k] = std::max(b.set_width[k], deg[i]);
     const double dt = now_sec() - t0;
     const uint32_t li = b.loopcount;
     if (li < FK_MAX_LOOPS) b.st->tree_grow_time[li] = dt;

@@ -174,6 +174,28 @@ class Device:
         finally:
             self.lib.fk_graph_free(ctypes.byref(g))
 
+    def build_batch(self, pg_type, params, set_off):
+        """Batched build of the resident row ranges [set_off[s], set_off[s+1]);
+        returns one (offsets, ids, ep, width) per set (ids relative to the
+        set) and the batch's statistics."""
+        set_off = _as(set_off, np.uint64)
+        n = len(set_off) - 1
+        gs = (FkGraph * n)()
+        p = _ffi.params_struct(params)
+        self._check(self.lib.fk_build_batch(self.h, pg_type, n, ptr(set_off), ctypes.byref(p), gs), "fk_build_batch")
+        out, st = [], None
+        try:
+            for g in gs:
+                off = np.ctypeslib.as_array(g.offsets, shape=(g.n + 1,)).copy()
+                ne = int(off[-1])
+                ids = np.ctypeslib.as_array(g.ids, shape=(max(ne, 1),))[:ne].copy()
+                out.append((off, ids, int(g.ep), int(g.width)))
+            st = gs[0].stats.to_dict()
+        finally:
+            for g in gs:
+                self.lib.fk_graph_free(ctypes.byref(g))
+        return out, st
+
     def search(self, offsets, ids, ep, queries, search_L, K):
         offsets = _as(offsets, np.uint64)
         ids = _as(ids, np.uint32)

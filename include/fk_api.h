@@ -200,6 +200,22 @@ fk_status fk_build(fk_handle *h, fk_pg_type type, uint32_t n_used, const fk_para
                    fk_graph *out);
 void fk_graph_free(fk_graph *g);
 
+/* Batched build of n_sets independent datasets in one pass (the reference
+ * builds one dataset per KGraph::buildFast* call, src/kgraph.cpp:3720-3966;
+ * this entry point has no counterpart and is for callers with many datasets).
+ * Set s is the resident rows [set_off[s], set_off[s + 1]) (set_off[0] = 0,
+ * ascending, set_off[n_sets] <= n_total); every kernel launch covers all
+ * sets, and every per-set decision (random draws, controls and stop rule,
+ * BridgeView rounds, entry point, connectivity) is made per set, so out[s]
+ * is the graph fk_build gives for set s's rows uploaded alone, with ids
+ * relative to the set.  Every set must have more rows than max(K, L, S,
+ * controls, search_L, search_K); rows of different widths are uploaded
+ * zero-padded to the widest (distances are unchanged when every width is a
+ * multiple of 32).  out[s].stats holds the whole batch's statistics; free
+ * each out[s] with fk_graph_free.  Not combinable with FK_FLAG_SCREEN. */
+fk_status fk_build_batch(fk_handle *h, fk_pg_type type, uint32_t n_sets, const uint64_t *set_off,
+                         const fk_params *p, fk_graph *out);
+
 /* Squared L2 between resident rows a[k] and b[k], k < n, in the reference's
  * float_l2sqr_avx summation order (bit-identical results). */
 fk_status fk_pair_distances(fk_handle *h, const uint32_t *a, const uint32_t *b, size_t n,

@@ -177,6 +177,43 @@ def screen_leg(dev, p, args, plain_graph, gt, queries, plain_sec):
     return out
 
 
+def batching_leg(dev, n_sets):
+    """The first n_sets datasets of the batching workload (n log-uniform in
+    [2,000, 200,000], d in {96, 128, 256}, unit-normalised 16-centre
+    mixtures; alpha-PG, K=L=100 S=10 R=50 I=6 nsg_R=32), grouped by width:
+    one fk_build_batch per width against one fk_build per dataset (uploads
+    outside both timings); the graphs are compared."""
+    from fastkcna_amd import FK_PG_ALPHAPG
+    from fastkcna_amd.datasets import c4_sets, params_c4
+    from fastkcna_amd.kgraph import IndexParams
+    t = time.perf_counter()
+    sets = [x for x, _ in c4_sets(n_sets=n_sets, n_queries=0)]
+    log(f"batching: {n_sets} sets, {sum(len(x) for x in sets):,} rows generated in {time.perf_counter() - t:.1f}s")
+    p = params_c4(IndexParams)
+    t_batch = t_single = 0.0
+    same = True
+    for d in sorted({x.shape[1] for x in sets}):
+        group = [x for x in sets if x.shape[1] == d]
+        rows = np.concatenate(group)
+        off = np.zeros(len(group) + 1, np.uint64)
+        off[1:] = np.cumsum([len(x) for x in group])
+        dev.upload(rows)
+        dev.build_batch(FK_PG_ALPHAPG, p, off)  # warm-up
+        t = time.perf_counter()
+        got, _ = dev.build_batch(FK_PG_ALPHAPG, p, off)
+        t_batch += time.perf_counter() - t
+        for k, x in enumerate(group):
+            dev.upload(x)
+            t = time.perf_counter()
+            o, i, ep, w, _ = dev.build(FK_PG_ALPHAPG, p)
+            t_single += time.perf_counter() - t
+            same = same and np.array_equal(o, got[k][0]) and np.array_equal(i, got[k][1]) and ep == got[k][2]
+    out = {"sets": n_sets, "rows": int(sum(len(x) for x in sets)), "batched_sec": t_batch,
+           "one_by_one_sec": t_single, "speedup": t_single / t_batch, "graphs_identical": bool(same)}
+    log(f"batching: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -190,6 +227,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, help="MiB of a device-to-device copy to run first")
     ap.add_argument("--no-chosen", action="store_true", help="skip the chosen (MFMA Gram join) leg")
     ap.add_argument("--no-screen", action="store_true", help="skip the learned-screen leg")
+    ap.add_argument("--batch-sets", type=int, default=0,
+                    help="also time the batching workload's first N datasets batched vs one by one")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the build does not shard "
@@ -250,6 +289,7 @@ def main():
     log("kernel families (ms, GB/s): " + ", ".join(
         f"{f}={st['prof_ms'][i]:.1f}/{st['prof_bytes'][i] / max(st['prof_ms'][i], 1e-9) / 1e6:.0f}"
         for i, f in enumerate(_ffi.PROF_FAMILIES) if st['prof_launches'][i]))
+    batching = batching_leg(dev, args.batch_sets) if args.batch_sets else None
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
@@ -279,6 +319,7 @@ def main():
         "roofline": roof,
         "chosen": chosen,
         "learned_screen": screen,
+        "batching": batching,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -95,3 +95,9 @@ def params_c2(IndexParams):
 
 def params_c3(IndexParams):
     return IndexParams(K=500, L=500, S=12, R=100, iterations=6, nsg_R=50, search_L=80, search_K=500, tau=10)
+
+
+def params_c4(IndexParams):
+    # "alpha = 1.2" read as the reference's --angle in degrees (its only
+    # alpha-PG parameter); search_K = K as for the other configurations
+    return IndexParams(K=100, L=100, S=10, R=50, iterations=6, nsg_R=32, search_L=40, search_K=100, angle=1.2)

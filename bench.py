@@ -149,7 +149,9 @@ def variant_leg(dev, p, args, extra_flags, plain_graph, gt, queries, plain_sec, 
     out = {"build_sec": sec, "dist_evals_per_sec": total_dists(st) / sec, "speedup_vs_plain": plain_sec / sec,
            "recall_at_10": rec, "graph_identical_to_plain": same,
            "join_ms": st["prof_ms"][_ffi.PROF_FAMILIES.index("join")],
-           "knn_recall_at_maxl": float(st["recall"][st["n_recall"] - 1])}
+           "knn_recall_at_maxl": float(st["recall"][st["n_recall"] - 1]),
+           "families_ms": {f: round(st["prof_ms"][i], 1) for i, f in enumerate(_ffi.PROF_FAMILIES)
+                           if st["prof_launches"][i]}}
     return out, st
 
 

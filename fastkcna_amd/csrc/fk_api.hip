@@ -127,6 +127,7 @@ fk_status fk_dataset_upload(fk_handle *h, const float *rows, size_t row_stride_b
         ds.rows.release();
         ds.chains.release();
         ds.gram_nrm.release();
+        ds.chains8.release();
         ds.gram_fmt = FK_GRAM_NONE;
         ds.gram_req = 0;
         ds.n_total = 0;
@@ -265,7 +266,10 @@ fk_status fk_prune(fk_handle *h, fk_pg_type type, const uint64_t *pools, const u
         if (p->flags & FK_FLAG_GRAM) {
             // exact-form rows: occlusion distances from dot products
             gram_prepare(h->stream, h->ds, p->flags);
-            if (h->ds.gram_fmt == FK_GRAM_EXACT) a.nrm = h->ds.gram_nrm.p;
+            if (h->ds.gram_fmt == FK_GRAM_EXACT) {
+                a.nrm = h->ds.gram_nrm.p;
+                a.chains8 = h->ds.chains8.p;
+            }
         }
         a.cand = dpool.p;
         a.cand_len = dlen.p;

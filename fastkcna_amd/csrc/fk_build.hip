@@ -219,6 +219,7 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
     FK_HIP(hipMemsetAsync(b.acc.p, 0, 16, b.s));
     PruneLaunch a;
     a.nrm = exact_norms(b);
+    a.chains8 = a.nrm ? b.ds->chains8.p : nullptr;
     a.cand = b.pool.p;
     a.cand_len = b.cnt.p;
     a.cand_stride = b.cap;

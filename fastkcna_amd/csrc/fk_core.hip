@@ -97,20 +97,24 @@ struct PruneMC {
 // LDS per wave: the cached kept chains (mc x dpad), one staging row
 // (dpad), kept ids, distances and norms (R each).
 //
-// DOT: the rows are exact-form rows (fk_gram.hip) and every occlusion
-// distance is taken as |a|^2 + |b|^2 - 2 a.b (exact_dist), bit-identical to
-// the canonical distance at half the arithmetic.
-__host__ __device__ inline uint32_t prune_wave_bytes(uint32_t mc, uint32_t dpad, uint32_t R) {
-    return ((mc + 1u) * dpad * 4u + R * 12u + 15u) & ~15u;
+// MODE (RowKind): 0 canonical FP32; 1 exact-form rows (fk_gram.hip), every
+// occlusion distance taken as |a|^2 + |b|^2 - 2 a.b (exact_dist),
+// bit-identical to the canonical distance at half the arithmetic; 2 the same
+// from byte rows (a quarter of the bytes, v_dot4_u32_u8).
+__host__ __device__ inline uint32_t prune_wave_bytes(uint32_t mc, uint32_t dpad, uint32_t R, uint32_t elem) {
+    return ((mc + 1u) * dpad * elem + R * 12u + 15u) & ~15u;
 }
 
-template <int RULE, int NB, bool DOT>
+template <int RULE, int NB, int MODE>
 struct PruneWave {
+    using E = typename RowKind<NB, MODE>::E;
+    using Chain = typename RowKind<NB, MODE>::Chain;
+    static constexpr bool DOT = RowKind<NB, MODE>::dot;
     uint32_t oracle_n, R;
     float tau3, cos_thr;
-    const float *lane_chains;  // chain k of row 0
-    float *lane_cache;         // chain k of cached kept neighbour 0
-    float *lane_stage;         // chain k of the staging row
+    const E *lane_chains;      // chain k of row 0
+    E *lane_cache;             // chain k of cached kept neighbour 0
+    E *lane_stage;             // chain k of the staging row
     uint32_t *so;
     float *sd, *sn;
     const float2 *nrm;  // DOT: per-row norms
@@ -122,7 +126,7 @@ struct PruneWave {
     __device__ uint64_t key_at(const uint64_t *c, uint32_t idx) const {
         return idx < limit ? c[idx] : FK_KEY_EMPTY;
     }
-    __device__ const float *row_of(uint64_t key) const {
+    __device__ const E *row_of(uint64_t key) const {
         const uint32_t id = key_id(key);
         return lane_chains + (size_t)((key != FK_KEY_EMPTY && id < oracle_n) ? id : 0u) * dpad;
     }
@@ -131,14 +135,14 @@ struct PruneWave {
         const uint32_t id = key_id(key);
         return nrm[(key != FK_KEY_EMPTY && id < oracle_n) ? id : 0u].x;
     }
-    __device__ float member_dist(const OctChain<NB> &crow, float cn, uint32_t t, uint32_t tid) const {
-        const float *o = t < PruneMC<NB>::value ? lane_cache + (size_t)t * dpad : lane_chains + (size_t)tid * dpad;
+    __device__ float member_dist(const Chain &crow, float cn, uint32_t t, uint32_t tid) const {
+        const E *o = t < PruneMC<NB>::value ? lane_cache + (size_t)t * dpad : lane_chains + (size_t)tid * dpad;
         if constexpr (DOT) return exact_dist(cn, sn[t], crow.dot(o, nb));
         else return crow.dist(o, nb);
     }
     // Decides the batch at pos whose key (this octet's), chain and norm
     // (DOT) are given.
-    __device__ void batch(uint32_t pos, uint64_t key, const OctChain<NB> &crow, float cn) {
+    __device__ void batch(uint32_t pos, uint64_t key, const Chain &crow, float cn) {
         const uint32_t nbat = (limit - pos) < 8u ? (limit - pos) : 8u;
         const bool valid = g < nbat;
         const uint32_t pid = key_id(key);
@@ -173,7 +177,7 @@ struct PruneWave {
                 const uint32_t spid = __shfl(pid, (int)sl, 64);
                 const float spd = __shfl(pd, (int)sl, 64);
                 const float spn = DOT ? __shfl(cn, (int)sl, 64) : 0.f;
-                const float *cp;
+                const E *cp;
                 if constexpr (NB > 0) {
                     if (g == sg) crow.store(lane_stage, nb);
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -193,8 +197,8 @@ struct PruneWave {
                             hit = isdup = true;
                         } else {
                             const float djk =
-                                DOT ? exact_dist(spn, sn[t], octet_dot2<NB>(cp, lane_chains + (size_t)tid * dpad, nb))
-                                    : octet_dist2<NB>(cp, lane_chains + (size_t)tid * dpad, nb);
+                                DOT ? exact_dist(spn, sn[t], chain_dot2<NB>(cp, lane_chains + (size_t)tid * dpad, nb))
+                                    : chain_dist2<NB>(cp, lane_chains + (size_t)tid * dpad, nb);
                             hit = occludes<RULE>(djk, spd, sd[t], tau3, cos_thr);
                         }
                     }
@@ -253,9 +257,14 @@ struct PruneWave {
     }
 };
 
-template <int RULE, int NB, bool DOT>
-__global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains, uint32_t dpad, PruneLaunch a,
+template <int RULE, int NB, int MODE>
+__global__ __launch_bounds__(256) void k_prune(const void *__restrict__ chains_v, uint32_t dpad, PruneLaunch a,
                                                uint32_t wave_bytes) {
+    using PW = PruneWave<RULE, NB, MODE>;
+    using E = typename PW::E;
+    using Chain = typename PW::Chain;
+    constexpr bool DOT = PW::DOT;
+    const E *chains = reinterpret_cast<const E *>(chains_v);
     extern __shared__ __align__(16) uint8_t smem[];
     constexpr uint32_t MC = PruneMC<NB>::value;
     const uint32_t lane = lane_id();
@@ -272,7 +281,7 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains,
     uint32_t *olen = a.out_len + slot;
     uint8_t *wb = smem + (size_t)w * wave_bytes;
 
-    PruneWave<RULE, NB, DOT> pw;
+    PW pw;
     pw.oracle_n = a.oracle_n;
     pw.R = a.R;
     pw.tau3 = a.tau3;
@@ -283,9 +292,9 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains,
     pw.dpad = dpad;
     pw.nb = NB ? (uint32_t)NB : (dpad >> 3);
     pw.lane_chains = chains + pw.k * pw.nb;
-    pw.lane_cache = reinterpret_cast<float *>(wb) + pw.k * pw.nb;
-    pw.lane_stage = reinterpret_cast<float *>(wb) + (size_t)MC * dpad + pw.k * pw.nb;
-    pw.so = reinterpret_cast<uint32_t *>(wb + (size_t)(MC + 1) * dpad * 4u);
+    pw.lane_cache = reinterpret_cast<E *>(wb) + pw.k * pw.nb;
+    pw.lane_stage = reinterpret_cast<E *>(wb) + (size_t)MC * dpad + pw.k * pw.nb;
+    pw.so = reinterpret_cast<uint32_t *>(wb + (size_t)(MC + 1) * dpad * sizeof(E));
     pw.sd = reinterpret_cast<float *>(pw.so + a.R);
     pw.sn = pw.sd + a.R;
     pw.nrm = a.nrm;
@@ -308,7 +317,7 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains,
             if (DOT) pw.sn[0] = a.nrm[key_id(k0)].x;  // kept whatever its id, like its row
         }
         if (MC > 0 && pw.g == 0) {
-            OctChain<NB> t0;
+            Chain t0;
             t0.load(pw.lane_chains + (size_t)key_id(k0) * dpad, pw.nb);
             t0.store(pw.lane_cache, pw.nb);
         }
@@ -323,7 +332,7 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains,
     // invariant at the top: ka, kb = this octet's keys of batches 2j, 2j + 1;
     // ra = chain of batch 2j (loaded or in flight)
     uint64_t ka = pw.key_at(c, p0 + pw.g), kb = pw.key_at(c, p0 + 8 + pw.g);
-    OctChain<NB> ra, rb;
+    Chain ra, rb;
     ra.load(pw.row_of(ka), pw.nb);
     float na = pw.norm_of(ka), nbn = 0.f;
     for (uint32_t pos = p0; !pw.stop && pos < pw.limit; pos += 16) {
@@ -357,16 +366,25 @@ __global__ __launch_bounds__(256) void k_prune(const float *__restrict__ chains,
 
 template <int RULE, int NB>
 static void launch_prune_nb(hipStream_t s, const Dataset &ds, const PruneLaunch &a, unsigned grid) {
-    const uint32_t wave_bytes = prune_wave_bytes(PruneMC<NB>::value, ds.dpad, a.R);
-    if constexpr (NB == 12 || NB == 16 || NB == 32) {
-        if (a.nrm) {
-            hipLaunchKernelGGL((k_prune<RULE, NB, true>), dim3(grid), dim3(64 * FK_PRUNE_WPB),
-                               (size_t)wave_bytes * FK_PRUNE_WPB, s, ds.chains.p, ds.dpad, a, wave_bytes);
+    if constexpr (NB == 16 || NB == 32) {
+        if (a.nrm && a.chains8) {
+            const uint32_t wb = prune_wave_bytes(PruneMC<NB>::value, ds.dpad, a.R, 1);
+            hipLaunchKernelGGL((k_prune<RULE, NB, 2>), dim3(grid), dim3(64 * FK_PRUNE_WPB),
+                               (size_t)wb * FK_PRUNE_WPB, s, (const void *)a.chains8, ds.dpad, a, wb);
             return;
         }
     }
-    hipLaunchKernelGGL((k_prune<RULE, NB, false>), dim3(grid), dim3(64 * FK_PRUNE_WPB),
-                       (size_t)wave_bytes * FK_PRUNE_WPB, s, ds.chains.p, ds.dpad, a, wave_bytes);
+    const uint32_t wave_bytes = prune_wave_bytes(PruneMC<NB>::value, ds.dpad, a.R, 4);
+    if constexpr (NB == 12 || NB == 16 || NB == 32) {
+        if (a.nrm) {
+            hipLaunchKernelGGL((k_prune<RULE, NB, 1>), dim3(grid), dim3(64 * FK_PRUNE_WPB),
+                               (size_t)wave_bytes * FK_PRUNE_WPB, s, (const void *)ds.chains.p, ds.dpad, a,
+                               wave_bytes);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_prune<RULE, NB, 0>), dim3(grid), dim3(64 * FK_PRUNE_WPB),
+                       (size_t)wave_bytes * FK_PRUNE_WPB, s, (const void *)ds.chains.p, ds.dpad, a, wave_bytes);
 }
 
 template <int RULE>

@@ -143,6 +143,16 @@ __device__ __forceinline__ float octet_sum(float acc) {
     return r;
 }
 
+// Sum over the 8 lanes of this lane's octet of 32-bit integers (exact),
+// as a float (every sum here is below 2^24).
+__device__ __forceinline__ float octet_sum_u32(uint32_t acc) {
+    const int base = (int)(threadIdx.x & 63u & ~7u);
+    uint32_t r = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r += (uint32_t)__shfl((int)acc, base + j, 64);
+    return (float)r;
+}
+
 // One lane's chain of one row, in registers (NB = dpad / 8, a multiple of
 // 4) or read from memory per distance (NB == 0).
 template <int NB>
@@ -228,6 +238,44 @@ struct OctChain<0> {
     }
 };
 
+// Byte form of a lane's chain (exact-form rows whose values are integers in
+// [0, 255]): NB bytes, read as 16-byte vectors; dot products with
+// v_dot4_u32_u8, exact in 32-bit integers.
+template <int NB>
+struct OctChainU8 {
+    static_assert(NB % 16 == 0, "byte chains are read 16 bytes at a time");
+    uint4 v[NB / 16];
+    __device__ __forceinline__ void load(const uint8_t *__restrict__ c, uint32_t) {
+#pragma unroll
+        for (int q = 0; q < NB / 16; ++q) v[q] = reinterpret_cast<const uint4 *>(c)[q];
+    }
+    __device__ __forceinline__ void store(uint8_t *__restrict__ c, uint32_t) const {
+#pragma unroll
+        for (int q = 0; q < NB / 16; ++q) reinterpret_cast<uint4 *>(c)[q] = v[q];
+    }
+    __device__ __forceinline__ static uint32_t dot4x4(const uint4 &a, const uint4 &b, uint32_t acc) {
+        acc = __builtin_amdgcn_udot4(a.x, b.x, acc, false);
+        acc = __builtin_amdgcn_udot4(a.y, b.y, acc, false);
+        acc = __builtin_amdgcn_udot4(a.z, b.z, acc, false);
+        return __builtin_amdgcn_udot4(a.w, b.w, acc, false);
+    }
+    __device__ __forceinline__ float dot_reg(const OctChainU8 &o, uint32_t) const {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int q = 0; q < NB / 16; ++q) acc = dot4x4(v[q], o.v[q], acc);
+        return octet_sum_u32(acc);
+    }
+    __device__ __forceinline__ float dot(const uint8_t *__restrict__ o, uint32_t) const {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int q = 0; q < NB / 16; ++q) acc = dot4x4(v[q], reinterpret_cast<const uint4 *>(o)[q], acc);
+        return octet_sum_u32(acc);
+    }
+    // exact squared distance is only taken in the dot form
+    __device__ __forceinline__ float dist(const uint8_t *__restrict__, uint32_t) const { return 0.f; }
+    __device__ __forceinline__ float dist_reg(const OctChainU8 &, uint32_t) const { return 0.f; }
+};
+
 // Exact form of the squared L2 for rows that the Gram preparation marked
 // exact (integer values, |x_i| <= 256, small norms): every product, partial
 // sum and the result are integers below 2^24, so |a|^2 + |b|^2 - 2 a.b is
@@ -277,6 +325,50 @@ __device__ __forceinline__ float octet_dot2(const float *__restrict__ a, const f
     }
     return octet_sum(acc);
 }
+
+__device__ __forceinline__ float octet_dot2(const uint8_t *__restrict__ a, const uint8_t *__restrict__ b,
+                                            uint32_t nb) {
+    uint32_t acc = 0;
+    for (uint32_t j = 0; j < nb; j += 16)
+        acc = OctChainU8<16>::dot4x4(*reinterpret_cast<const uint4 *>(a + j), *reinterpret_cast<const uint4 *>(b + j),
+                                     acc);
+    return octet_sum_u32(acc);
+}
+
+// Octet distance / dot of two chains given by pointers, for either row type.
+template <int NB>
+__device__ __forceinline__ float chain_dot2(const float *__restrict__ a, const float *__restrict__ b, uint32_t nb) {
+    return octet_dot2<NB>(a, b, nb);
+}
+template <int NB>
+__device__ __forceinline__ float chain_dot2(const uint8_t *__restrict__ a, const uint8_t *__restrict__ b,
+                                            uint32_t nb) {
+    return octet_dot2(a, b, nb);
+}
+template <int NB>
+__device__ __forceinline__ float chain_dist2(const float *__restrict__ a, const float *__restrict__ b, uint32_t nb) {
+    return octet_dist2<NB>(a, b, nb);
+}
+template <int NB>
+__device__ __forceinline__ float chain_dist2(const uint8_t *__restrict__, const uint8_t *__restrict__, uint32_t) {
+    return 0.f;  // byte rows only take the dot form
+}
+
+// Row kinds of the octet kernels: MODE 0 FP32 rows, canonical distance;
+// MODE 1 FP32 exact-form rows, dot form; MODE 2 byte exact-form rows, dot
+// form (NB a multiple of 16).
+template <int NB, int MODE>
+struct RowKind {
+    using E = float;
+    using Chain = OctChain<NB>;
+    static constexpr bool dot = MODE != 0;
+};
+template <int NB>
+struct RowKind<NB, 2> {
+    using E = uint8_t;
+    using Chain = OctChainU8<NB>;
+    static constexpr bool dot = true;
+};
 
 // Occlusion rules of the three pruner families; dqt is the kept neighbour's
 // distance to the pool owner, djk the kept neighbour to candidate distance,

@@ -166,6 +166,11 @@ __global__ __launch_bounds__(256) void k_gram_norms(const float *__restrict__ ro
     nrm[i] = make_float2(q2 * inv_s2, (float)(sqrt(s) * (1.0 + 1e-6)));
 }
 
+__global__ void k_to_bytes(const float *__restrict__ src, size_t n, uint8_t *__restrict__ dst) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = (uint8_t)src[i];
+}
+
 static GramBound gram_bound(const Dataset &ds) {
     GramBound g{};
     g.exact = ds.gram_fmt == FK_GRAM_EXACT;
@@ -224,6 +229,16 @@ void gram_prepare(hipStream_t s, Dataset &ds, uint32_t flags) {
         hipLaunchKernelGGL(k_gram_norms<FK_GRAM_BF16>, dim3(g2), dim3(256), 0, s, ds.rows.p, ds.stride, ds.dpad,
                            ds.n_total, scale, inv_s2, ds.gram_nrm.p);
     FK_LAUNCH_CHECK();
+    // exact form with values in [0, 255]: a chain-major byte copy for the
+    // octet kernels (dot products in 32-bit integers, exact)
+    ds.chains8.release();
+    if (fmt == FK_GRAM_EXACT && !h[2] && mabs <= 255.f) {
+        ds.chains8.alloc((size_t)ds.n_total * ds.dpad);
+        const size_t total = (size_t)ds.n_total * ds.dpad;
+        hipLaunchKernelGGL(k_to_bytes, dim3((unsigned)std::min<size_t>((total + 255) / 256, 65536)), dim3(256), 0, s,
+                           ds.chains.p, total, ds.chains8.p);
+        FK_LAUNCH_CHECK();
+    }
     FK_HIP(hipStreamSynchronize(s));
     ds.gram_fmt = fmt;
     ds.gram_scale = scale;

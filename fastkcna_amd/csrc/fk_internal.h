@@ -101,6 +101,7 @@ struct Dataset {
     // Gram-form join (fk_gram.hip): per-row (|x~|^2, |x|), operand format
     // (FK_GRAM_*), FP16 prescale and the FK_FLAG_GRAM_* request it was made for
     DevBuf<float2> gram_nrm;
+    DevBuf<uint8_t> chains8;  // exact form with values in [0, 255]: chain-major byte copy
     int gram_fmt = 0;
     float gram_scale = 1.f;
     uint32_t gram_req = 0;
@@ -145,6 +146,7 @@ struct PruneLaunch {
     unsigned long long *cc = nullptr;  // accumulated distance count
     unsigned long long *examined = nullptr;  // accumulated examined candidates
     const float2 *nrm = nullptr;      // exact-form rows: norms for dot-product distances
+    const uint8_t *chains8 = nullptr; // exact-form byte rows (chain-major), with nrm
 };
 void launch_prune(hipStream_t s, const Dataset &ds, const PruneLaunch &a);
 void build_chains(hipStream_t s, Dataset &ds);  // fills ds.chains from ds.rows

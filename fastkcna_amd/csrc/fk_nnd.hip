@@ -1269,15 +1269,20 @@ __device__ uint32_t bvw_merge(uint64_t *p, uint8_t *pf, uint32_t np, uint64_t *c
     return (np + nk) < cap ? (np + nk) : cap;
 }
 
-// DOT: exact-form rows (fk_gram.hip), distances as |a|^2 + |b|^2 - 2 a.b.
-template <int NB, bool DOT>
+// MODE (RowKind): 0 canonical FP32; 1 exact-form rows (fk_gram.hip),
+// distances as |a|^2 + |b|^2 - 2 a.b; 2 the same from byte rows.
+template <int NB, int MODE>
 __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
-    const float *__restrict__ chains, const float2 *__restrict__ nrm, uint32_t dpad, uint32_t N, uint32_t cap,
+    const void *__restrict__ chains_v, const float2 *__restrict__ nrm, uint32_t dpad, uint32_t N, uint32_t cap,
     uint32_t W, uint32_t S_cap_all, const uint32_t *__restrict__ set_of, const uint32_t *__restrict__ set_scap,
     uint64_t *__restrict__ pool, uint8_t *__restrict__ pflg, uint32_t *__restrict__ cnt,
     const uint32_t *__restrict__ nsg_id, const uint32_t *__restrict__ nsg_len, const uint8_t *__restrict__ nsg_new,
     uint32_t ncap, unsigned long long *__restrict__ cc_acc, const uint32_t *__restrict__ order,
     uint32_t *__restrict__ mark, uint32_t wave_bytes, uint32_t wpb) {
+    using E = typename RowKind<NB, MODE>::E;
+    using Chain = typename RowKind<NB, MODE>::Chain;
+    constexpr bool DOT = RowKind<NB, MODE>::dot;
+    const E *chains = reinterpret_cast<const E *>(chains_v);
     extern __shared__ __align__(16) uint8_t smem[];
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint32_t per_xcd = ((N + wpb - 1) / wpb + FK_NUM_XCD - 1) / FK_NUM_XCD;
@@ -1341,8 +1346,8 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
     const float thr = np0 >= cap ? key_dist(pr[cap - 1]) : INFINITY;
     const uint32_t g = lane >> 3, k = lane & 7u;
     const uint32_t nb = NB ? (uint32_t)NB : (dpad >> 3);
-    const float *lane_chains = chains + k * nb;
-    OctChain<NB> mine;
+    const E *lane_chains = chains + k * nb;
+    Chain mine;
     mine.load(lane_chains + (size_t)i * dpad, nb);
     const float ni = DOT ? nrm[i].x : 0.f;
     uint32_t np = np0, nc = 0;
@@ -1385,7 +1390,7 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
             // one octet per distance, FK_BVW_INFLIGHT rows per octet in
             // flight (slots past nt compute a discarded distance to row 0)
             for (uint32_t e0 = 0; e0 < nt; e0 += 8 * FK_BVW_INFLIGHT) {
-                OctChain<NB> rr[FK_BVW_INFLIGHT];
+                Chain rr[FK_BVW_INFLIGHT];
                 uint32_t te[FK_BVW_INFLIGHT];
                 float tn[FK_BVW_INFLIGHT];
 #pragma unroll
@@ -1438,18 +1443,20 @@ static void launch_bv_round(BuildState &b, uint32_t W, uint32_t scap, const uint
     const unsigned grid = ord ? (nblk + FK_NUM_XCD - 1) / FK_NUM_XCD * FK_NUM_XCD : nblk;
     uint32_t *mark = b.prof ? b.bv_mark.p : nullptr;
     const float2 *nrm = exact_norms(b);
-#define FK_BV_LAUNCH(NBV, DOT)                                                                                   \
-    hipLaunchKernelGGL((k_bv_round<NBV, DOT>), dim3(grid), dim3(64 * wpb), lds, b.s, b.ds->chains.p, nrm,       \
+    const uint8_t *c8 = nrm && b.ds->chains8.p ? b.ds->chains8.p : nullptr;
+#define FK_BV_LAUNCH(NBV, MODE)                                                                                  \
+    hipLaunchKernelGGL((k_bv_round<NBV, MODE>), dim3(grid), dim3(64 * wpb), lds, b.s,                          \
+                       MODE == 2 ? (const void *)c8 : (const void *)b.ds->chains.p, nrm,                        \
                        b.ds->dpad, b.N, b.cap, W, scap, b.dev_set_of(), set_scap, b.pool.p, b.pflg.p, b.cnt.p,  \
                        b.nsg_id.p, b.nsg_len.p,                                                                 \
                        b.nsg_new.p, b.ncap, b.acc.p, ord, mark, wave_bytes, wpb)
     switch (b.ds->dpad) {
-        case 32: FK_BV_LAUNCH(4, false); break;
-        case 64: FK_BV_LAUNCH(8, false); break;
-        case 96: if (nrm) FK_BV_LAUNCH(12, true); else FK_BV_LAUNCH(12, false); break;
-        case 128: if (nrm) FK_BV_LAUNCH(16, true); else FK_BV_LAUNCH(16, false); break;
-        case 256: if (nrm) FK_BV_LAUNCH(32, true); else FK_BV_LAUNCH(32, false); break;
-        default: FK_BV_LAUNCH(0, false); break;
+        case 32: FK_BV_LAUNCH(4, 0); break;
+        case 64: FK_BV_LAUNCH(8, 0); break;
+        case 96: if (nrm) FK_BV_LAUNCH(12, 1); else FK_BV_LAUNCH(12, 0); break;
+        case 128: if (c8) FK_BV_LAUNCH(16, 2); else if (nrm) FK_BV_LAUNCH(16, 1); else FK_BV_LAUNCH(16, 0); break;
+        case 256: if (c8) FK_BV_LAUNCH(32, 2); else if (nrm) FK_BV_LAUNCH(32, 1); else FK_BV_LAUNCH(32, 0); break;
+        default: FK_BV_LAUNCH(0, 0); break;
     }
 #undef FK_BV_LAUNCH
 }

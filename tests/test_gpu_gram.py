@@ -157,17 +157,30 @@ def test_gram_full_build_identical(device, name, pg):
     assert st1["d_gram"] > 0
 
 
+def exact_rows(kind):
+    """Exact-form rows of the three octet row kinds: bytes (values in
+    [0, 255], d = 128: byte chains), signed (d = 128: FP32 dot form) and
+    d = 96 (FP32 dot form)."""
+    if kind == "u8_128":
+        return integer_mixture(4000, 128, 32, seed=11)
+    if kind == "s8_128":
+        return integer_mixture(4000, 128, 32, seed=12) - 128.0
+    return integer_mixture(4000, 96, 32, seed=13)
+
+
+@pytest.mark.parametrize("kind", ["u8_128", "s8_128", "u8_96"])
 @pytest.mark.parametrize("pg", [FK_PG_NSG, FK_PG_TAUMNG, FK_PG_ALPHAPG])
 @pytest.mark.parametrize("reversed_", [False, True])
-def test_exact_form_prune_bitexact(device, pg, reversed_):
+def test_exact_form_prune_bitexact(device, pg, reversed_, kind):
     """With FK_FLAG_GRAM on exact-form rows the pruner takes its occlusion
-    distances as |a|^2 + |b|^2 - 2 a.b; ids, kept distances and distance
-    counts must equal the CPU oracle's (canonical FP32 distances)."""
+    distances as |a|^2 + |b|^2 - 2 a.b (from byte rows when the values are
+    in [0, 255] and d = 128); ids, kept distances and distance counts must
+    equal the CPU oracle's (canonical FP32 distances)."""
     from test_gpu_parity import make_pools
-    x = integer_mixture(4000, 128, 32, seed=11)
+    x = exact_rows(kind)
     # duplicate rows: zero distances take the 0/0 and x/0 paths of the alpha rule
     x[100:140] = x[200:240]
-    say("exact prune", pg, reversed_)
+    say("exact prune", pg, reversed_, kind)
     device.upload(x)
     rng = np.random.default_rng(3)
     p = IndexParams(nsg_R=24, search_K=90, tau=300.0, angle=63)
@@ -190,11 +203,12 @@ def test_exact_form_prune_bitexact(device, pg, reversed_):
     assert g[3] == c[3]
 
 
+@pytest.mark.parametrize("kind", ["u8_128", "s8_128", "u8_96"])
 @pytest.mark.parametrize("pg", [FK_PG_ALPHAPG, FK_PG_HNSW_LAYER])
-def test_gram_full_build_identical_exact_form(device, pg):
-    """Exact form end to end (join, prunes, BridgeView): the same graph as the
-    plain FP32 build."""
-    x = data("int128")
+def test_gram_full_build_identical_exact_form(device, pg, kind):
+    """Exact form end to end (join, prunes, BridgeView, every octet row
+    kind): the same graph as the plain FP32 build."""
+    x = exact_rows(kind)[:3000]
     device.upload(x)
     p = nnd_params(angle=60)
     off0, ids0, ep0, w0, _ = device.build(pg, p)

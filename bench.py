@@ -14,6 +14,7 @@ for the dominant kernel (timed with HIP events on the library's stream) and a
 C++/OpenMP, timed on this host on a bounded sample).
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -34,6 +35,16 @@ def log(*a):
 
 def recall_at_10(res, gt):
     return float(np.mean([len(set(res[i].tolist()) & set(gt[i].tolist())) / 10.0 for i in range(len(gt))]))
+
+
+def graph_digest(off, ids, ep):
+    """SHA-1 of the CSR graph (offsets, ids, entry point): equal digests mean
+    equal graphs, whatever kernels built them."""
+    h = hashlib.sha1()
+    h.update(np.ascontiguousarray(off, dtype=np.uint64).tobytes())
+    h.update(np.ascontiguousarray(ids, dtype=np.uint32).tobytes())
+    h.update(np.uint64(ep).tobytes())
+    return h.hexdigest()
 
 
 def total_dists(st):
@@ -271,6 +282,8 @@ def main():
         log(f"step {s}: build {dt:.3f}s, dist-evals {total_dists(st):,}, aod {st['aod']}")
 
     off, ids, ep = graph
+    digest = graph_digest(off, ids, ep)
+    log(f"graph digest {digest}")
     t = time.perf_counter()
     gt, _ = dev.knn(queries, 10)
     res, _ = dev.search(off, ids, ep, queries, 80, 10)
@@ -316,6 +329,7 @@ def main():
                                "Gaussians (seed 1), K=L=500 S=12 R=100 I=6 nsg_R=50 search_L=80 search_K=500",
                    "build_sec": ms / 1000.0, "recall_at_10": rec, "search_L": 80, "queries": args.queries,
                    "dist_evals": nd, "aod": int(st["aod"]), "nnd_iterations": int(st["nnd_iterations"]),
+                   "graph_digest": digest,
                    "phase_sec": {k: round(float(st[k]), 4) for k in ("t_ctrl", "t_nnd", "t_prune", "t_inter",
                                                                     "t_ep", "t_conn", "t_bv")}},
         "roofline": roof,

@@ -93,7 +93,7 @@ fk_status fk_open(int device, fk_handle **out) {
     fk_status st = guard(h, [&] {
         select_device(h);
         FK_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-        h->counters.alloc(64);
+        h->counters.ensure(2);
     });
     if (st != FK_OK) {
         delete h;
@@ -109,7 +109,7 @@ void fk_close(fk_handle *h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     h->ds.rows.release();
     h->ds.chains.release();
-    h->counters.release();
+    h->counters.buf.release();
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -261,7 +261,7 @@ fk_status fk_prune(fk_handle *h, fk_pg_type type, const uint64_t *pools, const u
         FK_HIP(hipMemcpyAsync(dpool.p, pools, pool_n * 8, hipMemcpyHostToDevice, h->stream));
         FK_HIP(hipMemcpyAsync(dlen.p, pool_len, (size_t)n_tasks * 4, hipMemcpyHostToDevice, h->stream));
         if (q) FK_HIP(hipMemcpyAsync(dq.p, q, (size_t)n_tasks * 4, hipMemcpyHostToDevice, h->stream));
-        FK_HIP(hipMemsetAsync(h->counters.p, 0, 8, h->stream));
+        h->counters.clear(h->stream, 0);
         PruneLaunch a;
         if (p->flags & FK_FLAG_GRAM) {
             // exact-form rows: occlusion distances from dot products
@@ -286,7 +286,7 @@ fk_status fk_prune(fk_handle *h, fk_pg_type type, const uint64_t *pools, const u
         a.out_d = dod.p;
         a.out_stride = out_stride;
         a.out_len = dolen.p;
-        a.cc = h->counters.p;
+        a.cc = h->counters.at(0);
         launch_prune(h->stream, h->ds, a);
         FK_HIP(hipMemcpyAsync(out_ids, doid.p, (size_t)n_tasks * out_stride * 4, hipMemcpyDeviceToHost,
                               h->stream));
@@ -294,9 +294,7 @@ fk_status fk_prune(fk_handle *h, fk_pg_type type, const uint64_t *pools, const u
             FK_HIP(hipMemcpyAsync(out_dists, dod.p, (size_t)n_tasks * out_stride * 4, hipMemcpyDeviceToHost,
                                   h->stream));
         FK_HIP(hipMemcpyAsync(out_len, dolen.p, (size_t)n_tasks * 4, hipMemcpyDeviceToHost, h->stream));
-        unsigned long long c = 0;
-        FK_HIP(hipMemcpyAsync(&c, h->counters.p, 8, hipMemcpyDeviceToHost, h->stream));
-        FK_HIP(hipStreamSynchronize(h->stream));
+        const unsigned long long c = h->counters.read1(h->stream, 0);
         if (cc) *cc = c;
     });
 }

@@ -23,6 +23,13 @@ namespace fk {
 #define FK_POOL_MAX 1024   // largest supported MAXL
 #define FK_LIST_MAX 2048   // largest supported NN-descent sample list
 #define FK_NSG_SLACK 64    // out-list room for connectivity edges
+// tally counters of a build (BuildState::tally)
+#define FK_T_PRUNE_CC 0    // prune: distances
+#define FK_T_PRUNE_EXAM 1  // prune: examined candidates
+#define FK_T_BV_CC 2       // BridgeView round: distances
+#define FK_T_SEARCH 3      // entry-point search: distances
+#define FK_T_RECHECK 4     // Gram join: FP32 rechecks
+#define FK_T_SLOTS 5
 
 struct BuildParams {
     // effective (shrunk) parameters, src/kgraph.cpp:1288-1334
@@ -92,6 +99,8 @@ struct BuildState {
     DevBuf<uint32_t> ctrl_ids;   // n_ctrl
     DevBuf<uint32_t> ctrl_gt;    // n_ctrl x MAXL ids
     DevBuf<unsigned long long> acc;  // reduction scratch
+    // per-wave counters of the big launches (slots FK_T_*)
+    Tally tally;
 
     // pruned graph
     DevBuf<uint32_t> nsg_id, nsg_len, old_id, old_len;

@@ -216,7 +216,7 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
     const BuildParams &P = b.P;
     const unsigned gw = (N + 3) / 4;
     // 1. prune every pool
-    FK_HIP(hipMemsetAsync(b.acc.p, 0, 16, b.s));
+    b.tally.clear(b.s, FK_T_PRUNE_CC, 2);
     PruneLaunch a;
     a.nrm = exact_norms(b);
     a.chains8 = a.nrm ? b.ds->chains8.p : nullptr;
@@ -234,8 +234,8 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
     a.out_d = b.nsg_d.p;
     a.out_stride = b.ncap;
     a.out_len = b.nsg_len.p;
-    a.cc = b.acc.p;
-    a.examined = b.acc.p + 1;
+    a.cc = b.tally.at(FK_T_PRUNE_CC);
+    a.examined = b.tally.at(FK_T_PRUNE_EXAM);
     if (b.has_order) {
         a.task_list = b.order.p;
         a.xcd_order = true;
@@ -243,8 +243,7 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
     hipEvent_t pe = b.prof_begin();
     launch_prune(b.s, *b.ds, a);
     unsigned long long cc2[2] = {0, 0};
-    FK_HIP(hipMemcpyAsync(cc2, b.acc.p, 16, hipMemcpyDeviceToHost, b.s));
-    b.sync();
+    b.tally.read(b.s, FK_T_PRUNE_CC, 2, cc2);
     const unsigned long long cc = cc2[0];
     // every examined candidate's row and key read once, kept ids and
     // distances written
@@ -291,7 +290,7 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
                                               std::to_string(hc[1]) + " > 4096)");
     // 3. reversed prune of over-full lists (no search_K cap, no size guard)
     if (hc[0]) {
-        FK_HIP(hipMemsetAsync(b.acc.p, 0, 8, b.s));
+        b.tally.clear(b.s, FK_T_PRUNE_CC, 2);
         PruneLaunch r = a;
         r.cand = rp_key.p;
         r.cand_off = rp_off.p;
@@ -300,13 +299,11 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
         r.n_tasks = hc[0];
         r.oracle_n = 0xFFFFFFFFu;
         r.maxc = 0xFFFFFFFFu;
-        r.examined = b.acc.p + 1;
-        FK_HIP(hipMemsetAsync(b.acc.p + 1, 0, 8, b.s));
+        r.examined = b.tally.at(FK_T_PRUNE_EXAM);
         hipEvent_t pr = b.prof_begin();
         launch_prune(b.s, *b.ds, r);
         unsigned long long rcc[2] = {0, 0};
-        FK_HIP(hipMemcpyAsync(rcc, b.acc.p, 16, hipMemcpyDeviceToHost, b.s));
-        b.sync();
+        b.tally.read(b.s, FK_T_PRUNE_CC, 2, rcc);
         const unsigned long long rc = rcc[0];
         b.prof_end(pr, FK_PROF_PRUNE_REV, (double)rcc[1] * (4.0 * b.ds->d + 8.0) + 8.0 * (double)hc[0] * P.nsg_R);
         b.prof_flush();

@@ -356,11 +356,11 @@ __global__ __launch_bounds__(256) void k_prune(const void *__restrict__ chains_v
     if (lane == 0) *olen = pw.nout;
     if (a.cc) {
         const unsigned long long s = wave_sum(pw.cc);
-        if (lane == 0 && s) atomicAdd(a.cc, s);
+        if (lane == 0 && s) tally_add(a.cc, s);
     }
     if (a.examined) {
         const unsigned long long s = wave_sum(pw.exam);
-        if (lane == 0 && s) atomicAdd(a.examined, s);
+        if (lane == 0 && s) tally_add(a.examined, s);
     }
 }
 

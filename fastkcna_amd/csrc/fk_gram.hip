@@ -346,7 +346,7 @@ __global__ __launch_bounds__(256) void k_join_gram(const float *__restrict__ row
     }
     if (!gb.exact) {
         const unsigned long long r = block_sum((unsigned long long)rechecks);
-        if (threadIdx.x == 0 && r) atomicAdd(n_recheck, r);
+        if (threadIdx.x == 0 && r) tally_add(n_recheck, r);
     }
 }
 

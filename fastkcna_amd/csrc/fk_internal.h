@@ -69,6 +69,47 @@ struct DevBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
+// Striped device counters.  Atomics to one address are performed one at a
+// time, and a launch whose million waves each add a count to the same word
+// stalls the memory pipeline of every CU behind that word.  A tally counter
+// is FK_TALLY_STRIPES words 128 bytes apart: a workgroup adds to the stripe
+// of its index, and the host sums the stripes after the launch.
+#define FK_TALLY_STRIPES 64
+#define FK_TALLY_STRIDE 16  // words per stripe (128 bytes)
+#define FK_TALLY_WORDS (FK_TALLY_STRIPES * FK_TALLY_STRIDE)
+
+__device__ __forceinline__ void tally_add(unsigned long long *t, unsigned long long v) {
+    atomicAdd(t + (blockIdx.x & (FK_TALLY_STRIPES - 1)) * FK_TALLY_STRIDE, v);
+}
+
+// Host side: `slots` counters of FK_TALLY_WORDS words each.
+struct Tally {
+    DevBuf<unsigned long long> buf;
+    std::vector<unsigned long long> host;
+    void ensure(size_t slots) { buf.ensure(slots * FK_TALLY_WORDS); }
+    unsigned long long *at(size_t k) const { return buf.p + k * FK_TALLY_WORDS; }
+    void clear(hipStream_t s, size_t k, size_t n = 1) {
+        FK_HIP(hipMemsetAsync(at(k), 0, n * FK_TALLY_WORDS * sizeof(unsigned long long), s));
+    }
+    // Sums of counters [k, k + n) into out[0, n); synchronizes the stream.
+    void read(hipStream_t s, size_t k, size_t n, unsigned long long *out) {
+        host.resize(n * FK_TALLY_WORDS);
+        FK_HIP(hipMemcpyAsync(host.data(), at(k), n * FK_TALLY_WORDS * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, s));
+        FK_HIP(hipStreamSynchronize(s));
+        for (size_t c = 0; c < n; ++c) {
+            unsigned long long v = 0;
+            for (size_t j = 0; j < FK_TALLY_STRIPES; ++j) v += host[c * FK_TALLY_WORDS + j * FK_TALLY_STRIDE];
+            out[c] = v;
+        }
+    }
+    unsigned long long read1(hipStream_t s, size_t k) {
+        unsigned long long v = 0;
+        read(s, k, 1, &v);
+        return v;
+    }
+};
+
 // Pinned (page-locked) host buffer, grow-only.
 template <typename T>
 struct PinnedBuf {
@@ -114,7 +155,7 @@ struct fk_handle {
     hipStream_t stream = nullptr;
     std::string last_error;
     fk::Dataset ds;
-    fk::DevBuf<unsigned long long> counters;  // small scratch for kernel counters
+    fk::Tally counters;  // kernel counters (tally slots)
     fk::PinnedBuf<uint32_t> host_rows;         // connectivity pass: pool rows of attached ids
 };
 
@@ -143,8 +184,8 @@ struct PruneLaunch {
     float *out_d = nullptr;
     uint64_t out_stride = 0;
     uint32_t *out_len = nullptr;
-    unsigned long long *cc = nullptr;  // accumulated distance count
-    unsigned long long *examined = nullptr;  // accumulated examined candidates
+    unsigned long long *cc = nullptr;  // accumulated distance count (tally counter)
+    unsigned long long *examined = nullptr;  // accumulated examined candidates (tally counter)
     const float2 *nrm = nullptr;      // exact-form rows: norms for dot-product distances
     const uint8_t *chains8 = nullptr; // exact-form byte rows (chain-major), with nrm
 };

@@ -769,6 +769,7 @@ void nnd_alloc(BuildState &b) {
     b.roff_new.alloc(N + 1);
     b.roff_old.alloc(N + 1);
     b.acc.alloc(8);
+    b.tally.ensure(FK_T_SLOTS);
     b.scal.alloc(16);
 }
 
@@ -894,12 +895,12 @@ static void nnd_join(BuildState &b) {
     const uint32_t *ord = b.has_order ? b.order.p : nullptr;
     const double d4 = 4.0 * b.ds->d;
     const bool gram = (b.P.flags & FK_FLAG_GRAM) != 0;
-    if (gram) FK_HIP(hipMemsetAsync(b.scal.p, 0, 8, b.s));
+    if (gram) b.tally.clear(b.s, FK_T_RECHECK);
     const bool screen = b.scr.ready && join_screen_lds(tcap) <= FK_JOIN_SCREEN_LDS_MAX;
     if (screen) screen_calibrate(b);
     hipEvent_t pe = b.prof_begin();
     if (gram) {
-        launch_join_gram(b, tcap, (unsigned long long *)b.scal.p);
+        launch_join_gram(b, tcap, b.tally.at(FK_T_RECHECK));
     } else if (screen) {
         launch_join_screen(b, tcap);
     } else {
@@ -916,9 +917,7 @@ static void nnd_join(BuildState &b) {
         b.st->screen_cands += cands;
     }
     if (gram) {
-        uint64_t rc = 0;
-        FK_HIP(hipMemcpyAsync(&rc, b.scal.p, 8, hipMemcpyDeviceToHost, b.s));
-        b.sync();
+        const uint64_t rc = b.tally.read1(b.s, FK_T_RECHECK);
         b.st->d_gram += pairs;
         b.st->d_recheck += rc;
     }
@@ -1150,7 +1149,7 @@ __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ p
 // round with few or no candidates never does).  Per wave, in LDS:
 //   u64 p[cap]   the pool (sorted keys), merged in place
 //   u64 c[CH]    candidate keys (distance <= the pool's last at round start)
-//   u32 sel[W], sdeg[W + 1], todo[64];  u8 pf[cap], seln[W]
+//   u32 sel[W], sdeg[W + 1], todo[FK_BVW_TODO];  u8 pf[cap], seln[W]
 // Candidates already in the pool or repeated are dropped at the merge
 // (equal ids carry equal keys: the distance is symmetric and computed in one
 // canonical order), so no id set is kept; every pair the reference evaluates
@@ -1160,12 +1159,20 @@ __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ p
 #ifndef FK_BVW_INFLIGHT
 #define FK_BVW_INFLIGHT 2  // rows per octet in flight in the distance step
 #endif
+#ifndef FK_BVW_INFLIGHT8
+#define FK_BVW_INFLIGHT8 4  // the same for byte rows (16 bytes per lane and row)
+#endif
+#ifndef FK_BVW_WPE
+#define FK_BVW_WPE 4       // waves per SIMD the FP32 row kinds are compiled for
+#endif
+#define FK_BVW_TODO 320    // one read of 64 FK_BVW_PREFETCH pairs + a remainder below 64
 #ifndef FK_BVW_PREFETCH
 #define FK_BVW_PREFETCH 4  // chunks of 64 out-list entries read at once
 #endif
+#define FK_BVW_LOADS 8     // chunks of 64 pool entries read at once
 
 __host__ __device__ inline size_t bvw_wave_bytes(uint32_t cap, uint32_t W) {
-    const size_t b = 8 * ((size_t)cap + FK_BVW_CH) + 4 * ((size_t)W + (W + 1) + 64) + (size_t)cap + W;
+    const size_t b = 8 * ((size_t)cap + FK_BVW_CH) + 4 * ((size_t)W + (W + 1) + FK_BVW_TODO) + (size_t)cap + W;
     return (b + 15) & ~(size_t)15;
 }
 
@@ -1211,8 +1218,10 @@ __device__ void wave_bitonic_sort(uint64_t *c, uint32_t n) {
 
 // Merges the nc candidate keys of c into the pool p / pf (np entries, at
 // most cap kept) in place and returns the new size.  One wave.
+// `first` receives the smallest pool position written (cap when none); the
+// entries below it are unchanged.
 __device__ uint32_t bvw_merge(uint64_t *p, uint8_t *pf, uint32_t np, uint64_t *c, uint32_t nc, uint32_t cap,
-                              uint32_t &inserted) {
+                              uint32_t &inserted, uint32_t &first) {
     const uint32_t lane = lane_id();
     const uint32_t n2 = next_pow2(nc < 2 ? 2 : nc);
     for (uint32_t x = nc + lane; x < n2; x += 64) c[x] = FK_KEY_EMPTY;
@@ -1236,6 +1245,7 @@ __device__ uint32_t bvw_merge(uint64_t *p, uint8_t *pf, uint32_t np, uint64_t *c
         wave_lds_sync();
     }
     inserted = nk;
+    first = cap;
     if (nk == 0) return np;
     // candidate positions against the pool before it moves
     uint32_t cpos[FK_BVW_CH / 64];
@@ -1244,9 +1254,12 @@ __device__ uint32_t bvw_merge(uint64_t *p, uint8_t *pf, uint32_t np, uint64_t *c
         const uint32_t x = r * 64 + lane;
         cpos[r] = x < nk ? x + lower_bound_cnt(p, np, c[x]) : 0xFFFFFFFFu;
     }
+    // the smallest candidate (lane 0's first) lands lowest; nothing below it moves
+    first = (uint32_t)__shfl((int)cpos[0], 0, 64);
+    if (first > cap) first = cap;
     // pool entries move up to j + (#candidates below), last chunk first,
     // each chunk read before it is written
-    for (uint32_t ch = (np + 63) / 64; ch-- > 0;) {
+    for (uint32_t ch = (np + 63) / 64; ch-- > first / 64;) {
         const uint32_t j = ch * 64 + lane;
         uint64_t key = 0;
         uint8_t f = 0;
@@ -1269,10 +1282,91 @@ __device__ uint32_t bvw_merge(uint64_t *p, uint8_t *pf, uint32_t np, uint64_t *c
     return (np + nk) < cap ? (np + nk) : cap;
 }
 
+// The same merge for at most 64 candidates with a short chain of dependent
+// LDS accesses (what bounds a wave that merges a few candidates): the
+// candidates are sorted in registers (one per lane), one search of the pool
+// per candidate serves both the de-duplication and its final position, and
+// the pool entries that move are all read, with their new positions, before
+// any is written (FK_BVW_MOVE chunks of 64 at a time, highest first).
+#define FK_BVW_SMALL 64
+#define FK_BVW_MOVE 4
+__device__ uint32_t bvw_merge_small(uint64_t *p, uint8_t *pf, uint32_t np, uint64_t *c, uint32_t nc, uint32_t cap,
+                                    uint32_t &inserted, uint32_t &first) {
+    const uint32_t lane = lane_id();
+    uint64_t key = lane < nc ? c[lane] : FK_KEY_EMPTY;
+    key = wave_sort64(key);
+    const uint64_t prev = __shfl_up(key, 1, 64);
+    const uint32_t pos = key != FK_KEY_EMPTY ? lower_bound_cnt(p, np, key) : np;
+    const bool keep = key != FK_KEY_EMPTY && (lane == 0 || prev != key) && !(pos < np && p[pos] == key);
+    const uint64_t m = wave_ballot(keep);
+    const uint32_t nk = (uint32_t)__builtin_popcountll(m);
+    inserted = nk;
+    first = cap;
+    if (nk == 0) return np;
+    const uint32_t cpos = lanes_below(m) + pos;  // final position of a kept candidate
+    first = (uint32_t)__shfl((int)cpos, (int)__builtin_ctzll(m), 64);
+    if (first > cap) first = cap;
+    wave_lds_sync();  // every lane has read its c[lane]
+    if (keep) c[lanes_below(m)] = key;
+    wave_lds_sync();
+    const uint32_t ch_lo = first / 64, ch_hi = (np + 63) / 64;
+    for (uint32_t g1 = ch_hi; g1 > ch_lo;) {
+        const uint32_t g0 = (g1 - ch_lo) > FK_BVW_MOVE ? g1 - FK_BVW_MOVE : ch_lo;
+        uint64_t kk[FK_BVW_MOVE];
+        uint8_t ff[FK_BVW_MOVE];
+        uint32_t pp[FK_BVW_MOVE];
+#pragma unroll
+        for (uint32_t r = 0; r < FK_BVW_MOVE; ++r) {
+            const uint32_t j = (g0 + r) * 64 + lane;
+            pp[r] = 0xFFFFFFFFu;
+            if (g0 + r < g1 && j < np) {
+                kk[r] = p[j];
+                ff[r] = pf[j];
+                pp[r] = j + lower_bound_cnt(c, nk, kk[r]);
+            }
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (uint32_t r = 0; r < FK_BVW_MOVE; ++r)
+            if (pp[r] < cap) { p[pp[r]] = kk[r]; pf[pp[r]] = ff[r]; }
+        wave_lds_sync();
+        g1 = g0;
+    }
+    if (keep && cpos < cap) { p[cpos] = key; pf[cpos] = (uint8_t)(FK_FLG_NEW | FK_FLG_ISNEW); }
+    wave_lds_sync();
+    return (np + nk) < cap ? (np + nk) : cap;
+}
+
+// Copies the pool entries [a, b) of one node (keys and flag bytes) from
+// global memory to LDS.  Every load of a batch of 64 FK_BVW_LOADS entries is
+// issued before any LDS store, unconditionally (lanes past b read entry
+// b - 1 again) and into whole registers, so a batch costs one memory round
+// trip.  One wave; a < b or nothing happens.
+__device__ __forceinline__ void bvw_load_pool(uint64_t *p, uint8_t *pf, const uint64_t *__restrict__ pr,
+                                              const uint8_t *__restrict__ pfr, uint32_t a, uint32_t b) {
+    const uint32_t lane = lane_id();
+    for (uint32_t j0 = a; j0 < b; j0 += 64 * FK_BVW_LOADS) {
+        uint64_t kk[FK_BVW_LOADS];
+        uint32_t ff[FK_BVW_LOADS];
+#pragma unroll
+        for (uint32_t r = 0; r < FK_BVW_LOADS; ++r) {
+            const uint32_t j = j0 + r * 64 + lane;
+            const uint32_t jj = j < b ? j : b - 1;
+            kk[r] = pr[jj];
+            ff[r] = pfr[jj];
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < FK_BVW_LOADS; ++r) {
+            const uint32_t j = j0 + r * 64 + lane;
+            if (j < b) { p[j] = kk[r]; pf[j] = (uint8_t)ff[r]; }
+        }
+    }
+}
+
 // MODE (RowKind): 0 canonical FP32; 1 exact-form rows (fk_gram.hip),
 // distances as |a|^2 + |b|^2 - 2 a.b; 2 the same from byte rows.
 template <int NB, int MODE>
-__global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
+__global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 4 : (NB == 32 ? 3 : FK_BVW_WPE)))) void k_bv_round(
     const void *__restrict__ chains_v, const float2 *__restrict__ nrm, uint32_t dpad, uint32_t N, uint32_t cap,
     uint32_t W, uint32_t S_cap_all, const uint32_t *__restrict__ set_of, const uint32_t *__restrict__ set_scap,
     uint64_t *__restrict__ pool, uint8_t *__restrict__ pflg, uint32_t *__restrict__ cnt,
@@ -1282,6 +1376,7 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
     using E = typename RowKind<NB, MODE>::E;
     using Chain = typename RowKind<NB, MODE>::Chain;
     constexpr bool DOT = RowKind<NB, MODE>::dot;
+    constexpr uint32_t INF = MODE == 2 ? FK_BVW_INFLIGHT8 : FK_BVW_INFLIGHT;
     const E *chains = reinterpret_cast<const E *>(chains_v);
     extern __shared__ __align__(16) uint8_t smem[];
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
@@ -1297,7 +1392,7 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
     uint32_t *sel = reinterpret_cast<uint32_t *>(c + FK_BVW_CH);
     uint32_t *sdeg = sel + W;
     uint32_t *todo = sdeg + W + 1;
-    uint8_t *pf = reinterpret_cast<uint8_t *>(todo + 64);
+    uint8_t *pf = reinterpret_cast<uint8_t *>(todo + FK_BVW_TODO);
     uint8_t *seln = pf + cap;
 
     uint64_t *pr = pool + (uint64_t)i * cap;
@@ -1306,7 +1401,7 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
     const uint32_t lim = np0 < W ? np0 : W;
     // the window only; the rest of the pool is read when there is something
     // to merge into it
-    for (uint32_t j = lane; j < lim; j += 64) { p[j] = pr[j]; pf[j] = pfr[j]; }
+    bvw_load_pool(p, pf, pr, pfr, 0, lim);
     wave_lds_sync();
     // select the first S_cap flagged window entries, clear their flags
     uint32_t ns = 0;
@@ -1320,7 +1415,6 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
             const uint32_t q = key_id(p[j]);
             sel[r] = q;
             seln[r] = (f & FK_FLG_ISNEW) ? 1 : 0;
-            sdeg[r] = nsg_len[q];
             pf[j] = f & (uint8_t)~FK_FLG_NEW;
             if (mark) mark[q] = 1u;  // profiling: q's out-list is read this round
         }
@@ -1329,11 +1423,12 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
     }
     if (ns == 0) return;  // nothing selected, nothing changed
     wave_lds_sync();
-    // exclusive prefix of the selected out-degrees
+    // exclusive prefix of the selected out-degrees (read here, after the
+    // selection, so that one round trip serves 64 selected entries)
     uint32_t run = 0;
     for (uint32_t s0 = 0; s0 < ns; s0 += 64) {
         const uint32_t x = s0 + lane;
-        const uint32_t v = x < ns ? sdeg[x] : 0u;
+        const uint32_t v = x < ns ? nsg_len[sel[x]] : 0u;
         uint32_t tot;
         const uint32_t ex = wave_excl_scan(v, tot);
         wave_lds_sync();
@@ -1350,9 +1445,54 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
     Chain mine;
     mine.load(lane_chains + (size_t)i * dpad, nb);
     const float ni = DOT ? nrm[i].x : 0.f;
-    uint32_t np = np0, nc = 0;
+    uint32_t np = np0, nc = 0, nt = 0;
     unsigned long long cc = 0;
     bool merged = false, loaded = false;
+    uint32_t first_change = cap;  // smallest pool position rewritten
+    // merges the candidates of c into the pool (reading the rest of the pool
+    // first when this is the round's first merge)
+    auto merge_c = [&]() __attribute__((always_inline)) {
+        wave_lds_sync();
+        if (!loaded) {
+            bvw_load_pool(p, pf, pr, pfr, lim, np0);  // the rest of the pool
+            wave_lds_sync();
+            loaded = true;
+        }
+        uint32_t ins = 0, fst = cap;
+        np = nc <= FK_BVW_SMALL ? bvw_merge_small(p, pf, np, c, nc, cap, ins, fst)
+                                : bvw_merge(p, pf, np, c, nc, cap, ins, fst);
+        if (ins > 0) {
+            merged = true;
+            first_change = fst < first_change ? fst : first_change;
+        }
+        nc = 0;
+    };
+    // the distances of todo[0, m): one octet per distance, INF rows per
+    // octet in flight (slots past m compute a discarded distance to row 0);
+    // candidates (distance <= thr) go to c, merged before c can overflow
+    auto run_todo = [&](uint32_t m) __attribute__((always_inline)) {
+        for (uint32_t e0 = 0; e0 < m; e0 += 8 * INF) {
+            if (nc > FK_BVW_CH - 8 * INF) merge_c();
+            Chain rr[INF];
+            uint32_t te[INF];
+            float tn[INF];
+#pragma unroll
+            for (uint32_t u = 0; u < INF; ++u) {
+                const uint32_t e = e0 + u * 8 + g;
+                te[u] = e < m ? todo[e] : 0u;
+                rr[u].load(lane_chains + (size_t)te[u] * dpad, nb);
+                tn[u] = DOT ? nrm[te[u]].x : 0.f;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < INF; ++u) {
+                const float dd = DOT ? exact_dist(ni, tn[u], mine.dot_reg(rr[u], nb)) : mine.dist_reg(rr[u], nb);
+                const bool ok = (e0 + u * 8 + g) < m && k == 0 && dd <= thr;
+                const uint64_t mc = wave_ballot(ok);
+                if (ok) c[nc + lanes_below(mc)] = make_key(dd, te[u]);
+                nc += (uint32_t)__builtin_popcountll(mc);
+            }
+        }
+    };
     for (uint32_t s0 = 0; s0 < total; s0 += 64 * FK_BVW_PREFETCH) {
         // the out-list entries of FK_BVW_PREFETCH chunks of 64 (entry,
         // neighbour) pairs are read at once; pair f belongs to entry s with
@@ -1383,53 +1523,37 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) void k_bv_round(
             const uint32_t t = tr[r];
             const bool use = t != i && (sr[r] || nr[r]);
             const uint64_t mu = wave_ballot(use);
-            const uint32_t nt = (uint32_t)__builtin_popcountll(mu);
-            cc += nt;  // the reference evaluates every such pair (search_cc)
-            if (use) todo[lanes_below(mu)] = t;
+            const uint32_t nu = (uint32_t)__builtin_popcountll(mu);
+            cc += nu;  // the reference evaluates every such pair (search_cc)
+            if (use) todo[nt + lanes_below(mu)] = t;
+            nt += nu;
+        }
+        wave_lds_sync();
+        if (s0 + 64 * FK_BVW_PREFETCH >= total) {
+            run_todo(nt);
+            nt = 0;
+        } else if (nt >= 64) {
+            // whole iterations now; the remainder (< 8 INF) moves to the front
+            const uint32_t m = nt / (8 * INF) * (8 * INF);
+            run_todo(m);
+            const uint32_t rest = nt - m;
+            const uint32_t v = lane < rest ? todo[m + lane] : 0u;
             wave_lds_sync();
-            // one octet per distance, FK_BVW_INFLIGHT rows per octet in
-            // flight (slots past nt compute a discarded distance to row 0)
-            for (uint32_t e0 = 0; e0 < nt; e0 += 8 * FK_BVW_INFLIGHT) {
-                Chain rr[FK_BVW_INFLIGHT];
-                uint32_t te[FK_BVW_INFLIGHT];
-                float tn[FK_BVW_INFLIGHT];
-#pragma unroll
-                for (uint32_t u = 0; u < FK_BVW_INFLIGHT; ++u) {
-                    const uint32_t e = e0 + u * 8 + g;
-                    te[u] = e < nt ? todo[e] : 0u;
-                    rr[u].load(lane_chains + (size_t)te[u] * dpad, nb);
-                    tn[u] = DOT ? nrm[te[u]].x : 0.f;
-                }
-#pragma unroll
-                for (uint32_t u = 0; u < FK_BVW_INFLIGHT; ++u) {
-                    const float dd = DOT ? exact_dist(ni, tn[u], mine.dot_reg(rr[u], nb)) : mine.dist_reg(rr[u], nb);
-                    const bool ok = (e0 + u * 8 + g) < nt && k == 0 && dd <= thr;
-                    const uint64_t mc = wave_ballot(ok);
-                    if (ok) c[nc + lanes_below(mc)] = make_key(dd, te[u]);
-                    nc += (uint32_t)__builtin_popcountll(mc);
-                }
-            }
+            if (lane < rest) todo[lane] = v;
             wave_lds_sync();
-            if (nc > FK_BVW_CH - 64 || (f0 + 64 >= total && nc > 0)) {
-                if (!loaded) {
-                    for (uint32_t j = lim + lane; j < np0; j += 64) { p[j] = pr[j]; pf[j] = pfr[j]; }
-                    wave_lds_sync();
-                    loaded = true;
-                }
-                uint32_t ins = 0;
-                np = bvw_merge(p, pf, np, c, nc, cap, ins);
-                merged = merged || ins > 0;
-                nc = 0;
-            }
+            nt = rest;
         }
     }
+    if (nc > 0) merge_c();
+    // the window's flags (the selection) and everything from the first
+    // rewritten position on
+    const uint32_t fl = lim < first_change ? lim : first_change;
+    for (uint32_t j = lane; j < fl; j += 64) pfr[j] = pf[j];
     if (merged) {
-        for (uint32_t j = lane; j < np; j += 64) { pr[j] = p[j]; pfr[j] = pf[j]; }
+        for (uint32_t j = first_change + lane; j < np; j += 64) { pr[j] = p[j]; pfr[j] = pf[j]; }
         if (lane == 0) cnt[i] = np;
-    } else {
-        for (uint32_t j = lane; j < lim; j += 64) pfr[j] = pf[j];  // only flags changed
     }
-    if (lane == 0 && cc) atomicAdd(cc_acc, cc);  // the same count in every lane
+    if (lane == 0 && cc) tally_add(cc_acc, cc);  // the same count in every lane
 }
 
 static void launch_bv_round(BuildState &b, uint32_t W, uint32_t scap, const uint32_t *set_scap) {
@@ -1449,7 +1573,7 @@ static void launch_bv_round(BuildState &b, uint32_t W, uint32_t scap, const uint
                        MODE == 2 ? (const void *)c8 : (const void *)b.ds->chains.p, nrm,                        \
                        b.ds->dpad, b.N, b.cap, W, scap, b.dev_set_of(), set_scap, b.pool.p, b.pflg.p, b.cnt.p,  \
                        b.nsg_id.p, b.nsg_len.p,                                                                 \
-                       b.nsg_new.p, b.ncap, b.acc.p, ord, mark, wave_bytes, wpb)
+                       b.nsg_new.p, b.ncap, b.tally.at(FK_T_BV_CC), ord, mark, wave_bytes, wpb)
     switch (b.ds->dpad) {
         case 32: FK_BV_LAUNCH(4, 0); break;
         case 64: FK_BV_LAUNCH(8, 0); break;
@@ -1504,7 +1628,7 @@ void bridgeview(BuildState &b) {
         const uint32_t scap = scap_set[0];
         if (b.n_sets > 1)
             FK_HIP(hipMemcpyAsync(d_scap.p, scap_set.data(), (size_t)b.n_sets * 4, hipMemcpyHostToDevice, b.s));
-        FK_HIP(hipMemsetAsync(b.acc.p, 0, 8, b.s));
+        b.tally.clear(b.s, FK_T_BV_CC);
         if (b.prof && b.bv_mark.n < N) {
             b.bv_mark.alloc(N);
             FK_HIP(hipMemsetAsync(b.bv_mark.p, 0, (size_t)N * 4, b.s));
@@ -1513,8 +1637,7 @@ void bridgeview(BuildState &b) {
         launch_bv_round(b, W, scap, b.n_sets > 1 ? d_scap.p : nullptr);
         FK_LAUNCH_CHECK();
         b.prof_end(pe, FK_PROF_BV, 0.0);
-        unsigned long long cc = 0;
-        FK_HIP(hipMemcpyAsync(&cc, b.acc.p, 8, hipMemcpyDeviceToHost, b.s));
+        const unsigned long long cc = b.tally.read1(b.s, FK_T_BV_CC);
         if (b.prof) {
             // minimal traffic of a round: the rows (+ ids) of every expansion
             // (|rnn_q|) and of every out-list entry of an expanded q

@@ -210,7 +210,7 @@ __global__ __launch_bounds__(64) void k_beam(BeamArgs a) {
     }
     if (a.n_dist) {
         nd = wave_sum(nd);
-        if (lane == 0) atomicAdd(a.n_dist, nd);
+        if (lane == 0) tally_add(a.n_dist, nd);
     }
 }
 
@@ -294,13 +294,11 @@ void compute_ep(BuildState &b) {
     a.q_seed = dq64.p;
     a.q_vis = dq64.p + ns;
     a.out_ids = oid.p;
-    a.n_dist = b.acc.p + 4;
-    FK_HIP(hipMemsetAsync(b.acc.p + 4, 0, 8, b.s));
+    a.n_dist = b.tally.at(FK_T_SEARCH);
+    b.tally.clear(b.s, FK_T_SEARCH);
     launch_beam(b.s, a);
-    unsigned long long nd = 0;
     FK_HIP(hipMemcpyAsync(b.set_ep.data(), oid.p, (size_t)ns * 4, hipMemcpyDeviceToHost, b.s));
-    FK_HIP(hipMemcpyAsync(&nd, b.acc.p + 4, 8, hipMemcpyDeviceToHost, b.s));
-    b.sync();
+    const unsigned long long nd = b.tally.read1(b.s, FK_T_SEARCH);
     b.ep = b.set_ep[0];
     // the centroids read every row once; the search one row per distance
     b.prof_end(pe, FK_PROF_EP, center_rows * 4.0 * ds.d + (double)nd * 4.0 * ds.d);
@@ -339,9 +337,22 @@ __global__ void k_bfs_expand(const uint32_t *__restrict__ nsg_id, const uint32_t
     if (w >= nf) return;
     const uint32_t f = front[w];
     const uint32_t deg = nsg_len[f];
-    for (uint32_t k = lane_id(); k < deg; k += 64) {
-        const uint32_t t = nsg_id[(uint64_t)f * ncap + k];
-        if (atomicExch(&label[t], FK_REACHED) != FK_REACHED) next[atomicAdd(nnext, 1u)] = t;
+    // one queue reservation per wave and chunk of 64 out-edges
+    for (uint32_t k0 = 0; k0 < deg; k0 += 64) {
+        const uint32_t k = k0 + lane_id();
+        uint32_t t = 0;
+        bool fresh = false;
+        if (k < deg) {
+            t = nsg_id[(uint64_t)f * ncap + k];
+            fresh = atomicExch(&label[t], FK_REACHED) != FK_REACHED;
+        }
+        const uint64_t m = __ballot(fresh);
+        if (m == 0) continue;
+        const uint32_t lead = (uint32_t)__builtin_ctzll(m);
+        uint32_t base = 0;
+        if (lane_id() == lead) base = atomicAdd(nnext, (uint32_t)__builtin_popcountll(m));
+        base = (uint32_t)__shfl((int)base, (int)lead, 64);
+        if (fresh) next[base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = t;
     }
 }
 
@@ -541,8 +552,9 @@ extern "C" fk_status fk_search(fk_handle *h, const uint64_t *offsets, const uint
         DevBuf<uint32_t> vis((size_t)qb * words);
         DevBuf<float> dq((size_t)qb * ds.dpad);
         DevBuf<uint32_t> dout((size_t)qb * K);
-        DevBuf<unsigned long long> cnt(1);
-        FK_HIP(hipMemsetAsync(cnt.p, 0, 8, h->stream));
+        Tally cnt;
+        cnt.ensure(1);
+        cnt.clear(h->stream, 0);
         for (uint32_t q0 = 0; q0 < n_q; q0 += qb) {
             const uint32_t nq = std::min(qb, n_q - q0);
             FK_HIP(hipMemsetAsync(vis.p, 0, (size_t)nq * words * 4, h->stream));
@@ -563,15 +575,13 @@ extern "C" fk_status fk_search(fk_handle *h, const uint64_t *offsets, const uint
             a.visited = vis.p;
             a.seed = 0x9E3779B97F4A7C15ull ^ q0;
             a.out_ids = dout.p;
-            a.n_dist = cnt.p;
+            a.n_dist = cnt.at(0);
             launch_beam(h->stream, a);
             FK_HIP(hipMemcpyAsync(out_ids + (size_t)q0 * K, dout.p, (size_t)nq * K * 4, hipMemcpyDeviceToHost,
                                   h->stream));
             FK_HIP(hipStreamSynchronize(h->stream));
         }
-        unsigned long long c = 0;
-        FK_HIP(hipMemcpyAsync(&c, cnt.p, 8, hipMemcpyDeviceToHost, h->stream));
-        FK_HIP(hipStreamSynchronize(h->stream));
+        const unsigned long long c = cnt.read1(h->stream, 0);
         if (n_dist) *n_dist = c;
     });
 }

@@ -85,6 +85,8 @@ struct BuildState {
     DevBuf<uint32_t> resv;
     DevBuf<uint32_t> bcnt;      // N
     DevBuf<uint64_t> boff;      // N + 1
+    DevBuf<uint32_t> ord_cnt;   // N: bucket sizes in locality order
+    DevBuf<uint64_t> ord_off;   // N + 1: their exclusive scan
     DevBuf<uint64_t> bkey;
     DevBuf<uint32_t> bsrc;
     // reverse-sample buckets

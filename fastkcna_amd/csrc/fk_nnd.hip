@@ -853,6 +853,40 @@ static uint64_t scan_to_host(BuildState &b, const uint32_t *in, uint64_t *out, s
     return tot;
 }
 
+// Gathers v[order[p]] into out[p].
+__global__ void k_gather_by_order(const uint32_t *__restrict__ v, const uint32_t *__restrict__ order, uint32_t n,
+                                  uint32_t *__restrict__ out) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) out[p] = v[order[p]];
+}
+
+// Scatters off[p] to boff[order[p]].
+__global__ void k_scatter_by_order(const uint64_t *__restrict__ off, const uint32_t *__restrict__ order, uint32_t n,
+                                   uint64_t *__restrict__ boff) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) boff[order[p]] = off[p];
+}
+
+// Bucket offsets of the join's candidates (boff[N] = total).  With a
+// locality order the buckets are laid out in that order, so the buckets one
+// node's list writes into (its neighbours') lie close together in memory
+// instead of across the whole candidate buffer (address translation and L2
+// reuse); every kernel still finds bucket t at boff[t].
+static uint64_t bucket_offsets(BuildState &b) {
+    const uint32_t N = b.N;
+    if (!b.has_order) return scan_to_host(b, b.bcnt.p, b.boff.p, N);
+    b.ord_cnt.ensure(N);
+    b.ord_off.ensure((size_t)N + 1);
+    const unsigned g = (N + 255) / 256;
+    hipLaunchKernelGGL(k_gather_by_order, dim3(g), dim3(256), 0, b.s, b.bcnt.p, b.order.p, N, b.ord_cnt.p);
+    FK_LAUNCH_CHECK();
+    const uint64_t tot = scan_to_host(b, b.ord_cnt.p, b.ord_off.p, N);
+    hipLaunchKernelGGL(k_scatter_by_order, dim3(g), dim3(256), 0, b.s, b.ord_off.p, b.order.p, N, b.boff.p);
+    FK_LAUNCH_CHECK();
+    FK_HIP(hipMemcpyAsync(b.boff.p + N, b.ord_off.p + N, 8, hipMemcpyDeviceToDevice, b.s));
+    return tot;
+}
+
 static void nnd_join(BuildState &b) {
     const uint32_t N = b.N;
     const unsigned g256 = (N + 255) / 256;
@@ -911,7 +945,7 @@ static void nnd_join(BuildState &b) {
     }
     // sampled rows and ids gathered once per join, one float per pair written
     b.prof_end(pe, FK_PROF_JOIN, (double)nlist * (d4 + 4.0) + 4.0 * (double)pairs);
-    const uint64_t cands = scan_to_host(b, b.bcnt.p, b.boff.p, N);
+    const uint64_t cands = bucket_offsets(b);
     if (screen) {
         screen_collect(b);
         b.st->screen_cands += cands;
@@ -1145,17 +1179,21 @@ __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ p
 // selected entries' out-lists -> their rows), so many nodes in flight per
 // CU is what hides the latency: the out-list entries of up to 256 pairs are
 // read with independent loads, and only the search window of the pool is
-// read up front (the rest only when candidates are merged into it, which a
-// round with few or no candidates never does).  Per wave, in LDS:
+// read before the selection (the rest only by nodes that select something).
+// Per wave, in LDS:
 //   u64 p[cap]   the pool (sorted keys), merged in place
 //   u64 c[CH]    candidate keys (distance <= the pool's last at round start)
-//   u32 sel[W], sdeg[W + 1], todo[FK_BVW_TODO];  u8 pf[cap], seln[W]
-// Candidates already in the pool or repeated are dropped at the merge
-// (equal ids carry equal keys: the distance is symmetric and computed in one
-// canonical order), so no id set is kept; every pair the reference evaluates
-// (search_cc) is evaluated.
+//   u32 sel[W], sdeg[W + 1], todo[FK_BVW_TODO], hs[bvw_hsize(cap)]
+//   u8 pf[cap], seln[W]
+// hs is an open-addressing set of ids: the pool's at round start, then every
+// pair target whose distance this round computes.  A target already in it
+// is counted (search_cc counts every pair the reference evaluates) but not
+// computed again: its key equals the pool entry's or the earlier
+// candidate's (the distance is symmetric and computed in one canonical
+// order), which the merge drops, and an id pushed out of the pool this
+// round has at least cap better keys ahead of it for the rest of the round.
 #define FK_BVW_WAVES 4
-#define FK_BVW_CH 256
+#define FK_BVW_CH 128
 #ifndef FK_BVW_INFLIGHT
 #define FK_BVW_INFLIGHT 2  // rows per octet in flight in the distance step
 #endif
@@ -1171,9 +1209,39 @@ __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ p
 #endif
 #define FK_BVW_LOADS 8     // chunks of 64 pool entries read at once
 
+// Slots of the id set: a power of two >= 2 cap; at most 3/4 of them filled.
+__host__ __device__ inline uint32_t bvw_hsize(uint32_t cap) {
+    uint32_t h = 128;
+    while (h < 2 * cap) h <<= 1;
+    return h;
+}
+
 __host__ __device__ inline size_t bvw_wave_bytes(uint32_t cap, uint32_t W) {
-    const size_t b = 8 * ((size_t)cap + FK_BVW_CH) + 4 * ((size_t)W + (W + 1) + FK_BVW_TODO) + (size_t)cap + W;
+    const size_t b = 8 * ((size_t)cap + FK_BVW_CH) +
+                     4 * ((size_t)W + (W + 1) + FK_BVW_TODO + bvw_hsize(cap)) + (size_t)cap + W;
     return (b + 15) & ~(size_t)15;
+}
+
+#define FK_HS_EMPTY 0xFFFFFFFFu
+
+__device__ __forceinline__ uint32_t hs_slot(uint32_t id, uint32_t mask) { return ((id * 0x9E3779B1u) >> 15) & mask; }
+
+// Inserts id into the set; true when it was not there.  Terminates while
+// the set has an empty slot.
+__device__ __forceinline__ bool hs_insert(uint32_t *hs, uint32_t mask, uint32_t id) {
+    for (uint32_t s = hs_slot(id, mask);; s = (s + 1) & mask) {
+        const uint32_t prev = atomicCAS(&hs[s], FK_HS_EMPTY, id);
+        if (prev == FK_HS_EMPTY) return true;
+        if (prev == id) return false;
+    }
+}
+
+__device__ __forceinline__ bool hs_contains(const uint32_t *hs, uint32_t mask, uint32_t id) {
+    for (uint32_t s = hs_slot(id, mask);; s = (s + 1) & mask) {
+        const uint32_t v = hs[s];
+        if (v == id) return true;
+        if (v == FK_HS_EMPTY) return false;
+    }
 }
 
 // Exclusive prefix sum of v over the wave; the wave total in every lane.
@@ -1392,7 +1460,9 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
     uint32_t *sel = reinterpret_cast<uint32_t *>(c + FK_BVW_CH);
     uint32_t *sdeg = sel + W;
     uint32_t *todo = sdeg + W + 1;
-    uint8_t *pf = reinterpret_cast<uint8_t *>(todo + FK_BVW_TODO);
+    const uint32_t hsize = bvw_hsize(cap), hmask = hsize - 1;
+    uint32_t *hs = todo + FK_BVW_TODO;
+    uint8_t *pf = reinterpret_cast<uint8_t *>(hs + hsize);
     uint8_t *seln = pf + cap;
 
     uint64_t *pr = pool + (uint64_t)i * cap;
@@ -1422,6 +1492,13 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
         ns = (ns + add) < S_cap ? (ns + add) : S_cap;
     }
     if (ns == 0) return;  // nothing selected, nothing changed
+    // the rest of the pool, and the id set: the pool's ids
+    bvw_load_pool(p, pf, pr, pfr, lim, np0);
+    for (uint32_t j = lane; j < hsize; j += 64) hs[j] = FK_HS_EMPTY;
+    wave_lds_sync();
+    for (uint32_t j = lane; j < np0; j += 64) hs_insert(hs, hmask, key_id(p[j]));
+    uint32_t hcount = np0;
+    const uint32_t hlim = hsize / 4 * 3;
     wave_lds_sync();
     // exclusive prefix of the selected out-degrees (read here, after the
     // selection, so that one round trip serves 64 selected entries)
@@ -1447,17 +1524,11 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
     const float ni = DOT ? nrm[i].x : 0.f;
     uint32_t np = np0, nc = 0, nt = 0;
     unsigned long long cc = 0;
-    bool merged = false, loaded = false;
+    bool merged = false;
     uint32_t first_change = cap;  // smallest pool position rewritten
-    // merges the candidates of c into the pool (reading the rest of the pool
-    // first when this is the round's first merge)
+    // merges the candidates of c into the pool
     auto merge_c = [&]() __attribute__((always_inline)) {
         wave_lds_sync();
-        if (!loaded) {
-            bvw_load_pool(p, pf, pr, pfr, lim, np0);  // the rest of the pool
-            wave_lds_sync();
-            loaded = true;
-        }
         uint32_t ins = 0, fst = cap;
         np = nc <= FK_BVW_SMALL ? bvw_merge_small(p, pf, np, c, nc, cap, ins, fst)
                                 : bvw_merge(p, pf, np, c, nc, cap, ins, fst);
@@ -1522,11 +1593,17 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
             if (f0 >= total) break;
             const uint32_t t = tr[r];
             const bool use = t != i && (sr[r] || nr[r]);
-            const uint64_t mu = wave_ballot(use);
-            const uint32_t nu = (uint32_t)__builtin_popcountll(mu);
-            cc += nu;  // the reference evaluates every such pair (search_cc)
-            if (use) todo[nt + lanes_below(mu)] = t;
-            nt += nu;
+            cc += (uint32_t)__builtin_popcountll(wave_ballot(use));  // every pair the reference evaluates
+            // computed only when the target is not in the id set; while the
+            // set has room for a whole chunk, the target joins it
+            const bool grow = hcount + 64 <= hlim;
+            bool fresh = false;
+            if (use) fresh = grow ? hs_insert(hs, hmask, t) : !hs_contains(hs, hmask, t);
+            const uint64_t mf = wave_ballot(fresh);
+            const uint32_t nf = (uint32_t)__builtin_popcountll(mf);
+            if (grow) hcount += nf;
+            if (fresh) todo[nt + lanes_below(mf)] = t;
+            nt += nf;
         }
         wave_lds_sync();
         if (s0 + 64 * FK_BVW_PREFETCH >= total) {

@@ -365,18 +365,53 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
     uint64_t *pr = pool + (uint64_t)i * cap;
     uint8_t *pf = pflg + (uint64_t)i * cap;
     const uint32_t np0 = cnt[i];
-    for (uint32_t j = threadIdx.x; j < np0; j += blockDim.x) {
-        s_p[j] = pr[j];
-        s_pf[j] = pf[j];
-        s_ps[j] = FK_NM_NOSRC;
+    const uint64_t base = boff[i];
+    // one chunk of bucket entries (key and source) per thread slot, read
+    // unconditionally in bounds (slots past the chunk repeat its last entry)
+    constexpr uint32_t CR = FK_NM_CH / 256;
+    uint64_t ck[CR];
+    uint32_t cs[CR];
+    auto load_chunk = [&](uint32_t c0, uint32_t m) __attribute__((always_inline)) {
+#pragma unroll
+        for (uint32_t r = 0; r < CR; ++r) {
+            const uint32_t k = r * 256 + threadIdx.x;
+            const uint64_t at = base + c0 + (k < m ? k : m - 1);
+            ck[r] = bkey[at];
+            cs[r] = bsrc[at];
+        }
+    };
+    // the pool and the first chunk: every load in flight before any store
+    {
+        constexpr uint32_t PR = FK_POOL_MAX / 256;
+        uint64_t pk[PR];
+        uint32_t pfl[PR];
+#pragma unroll
+        for (uint32_t r = 0; r < PR; ++r) {
+            const uint32_t j = r * 256 + threadIdx.x;
+            if (r * 256 < np0) {
+                const uint32_t jj = j < np0 ? j : np0 - 1;
+                pk[r] = pr[jj];
+                pfl[r] = pf[jj];
+            }
+        }
+        load_chunk(0, nb < FK_NM_CH ? nb : FK_NM_CH);
+#pragma unroll
+        for (uint32_t r = 0; r < PR; ++r) {
+            const uint32_t j = r * 256 + threadIdx.x;
+            if (j < np0) {
+                s_p[j] = pk[r];
+                s_pf[j] = (uint8_t)pfl[r];
+                s_ps[j] = FK_NM_NOSRC;
+            }
+        }
     }
     if (threadIdx.x == 0) s_np = np0;
     __syncthreads();
     const uint64_t kth = np0 >= K ? s_p[K - 1] : FK_KEY_EMPTY;
-    const uint64_t base = boff[i];
     uint32_t first_change = 0xFFFFFFFFu;  // smallest pool position rewritten (thread 0's copy)
     for (uint32_t c0 = 0; c0 < nb; c0 += FK_NM_CH) {
         const uint32_t m = (nb - c0) < FK_NM_CH ? (nb - c0) : FK_NM_CH;
+        if (c0 > 0) load_chunk(c0, m);
         for (uint32_t j = threadIdx.x; j < FK_NM_HASH; j += blockDim.x) {
             s_tid[j] = 0xFFFFFFFFu;
             s_tsrc[j] = FK_NM_NOSRC;
@@ -385,10 +420,12 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
         __syncthreads();
         const uint32_t np = s_np;
         const uint64_t lth = np >= Lp ? s_p[Lp - 1] : FK_KEY_EMPTY;
-        for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
-            const uint64_t key = bkey[base + c0 + k];
-            if (key >= lth) continue;
-            const uint32_t src = bsrc[base + c0 + k];
+#pragma unroll
+        for (uint32_t r = 0; r < CR; ++r) {
+            const uint32_t k = r * 256 + threadIdx.x;
+            const uint64_t key = ck[r];
+            if (k >= m || key >= lth) continue;
+            const uint32_t src = cs[r];
             const uint32_t at = lower_bound_cnt(s_p, np, key);
             if (at < np && s_p[at] == key) {
                 if (s_ps[at] != FK_NM_NOSRC) atomicMin(&s_ps[at], src);

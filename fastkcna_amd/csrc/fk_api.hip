@@ -67,6 +67,66 @@ void upload_rows(fk_handle *h, float *dst, uint32_t dpad, const float *src, size
     FK_HIP(hipStreamSynchronize(h->stream));
 }
 
+MemCache *&current_cache() {
+    static thread_local MemCache *c = nullptr;
+    return c;
+}
+
+static size_t cache_round(size_t bytes) {
+    const size_t g = bytes <= (1u << 20) ? 512 : (2u << 20);
+    return (bytes + g - 1) / g * g;
+}
+
+void *MemCache::get(size_t bytes, size_t &got) {
+    const size_t want = cache_round(bytes);
+    size_t best = free_.size();
+    for (size_t i = 0; i < free_.size(); ++i) {
+        const size_t b = free_[i].bytes;
+        if (b >= want && b <= want + want / 4 && (best == free_.size() || b < free_[best].bytes)) best = i;
+    }
+    if (best < free_.size()) {
+        void *p = free_[best].p;
+        got = free_[best].bytes;
+        held -= got;
+        free_[best] = free_.back();
+        free_.pop_back();
+        return p;
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipErrorOutOfMemory && !free_.empty()) {
+        (void)hipGetLastError();
+        trim();
+        e = hipMalloc(&p, want);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(e == hipErrorOutOfMemory ? FK_E_NOMEM : FK_E_DEVICE,
+                    "hipMalloc(" + std::to_string(want) + "): " + hipGetErrorString(e));
+    }
+    got = want;
+    return p;
+}
+
+void MemCache::put(void *p, size_t bytes) {
+    if (limit == 0) {
+        size_t fr = 0, tot = 0;
+        limit = hipMemGetInfo(&fr, &tot) == hipSuccess ? tot / 4 : ((size_t)16 << 30);
+    }
+    if (held + bytes > limit) {
+        (void)hipFree(p);
+        return;
+    }
+    free_.push_back({p, bytes});
+    held += bytes;
+}
+
+void MemCache::trim() {
+    for (const Blk &b : free_) (void)hipFree(b.p);
+    free_.clear();
+    held = 0;
+}
+
 void require_dataset(fk_handle *h) {
     if (h->ds.n_total == 0) throw Error(FK_E_INVALID, "no dataset uploaded (fk_dataset_upload)");
 }

@@ -6,9 +6,18 @@
 
 namespace fk {
 
+// Makes the handle's memory cache the one device buffers use while a call
+// runs on this thread.
+struct CacheScope {
+    MemCache *prev;
+    explicit CacheScope(MemCache *c) : prev(current_cache()) { current_cache() = c; }
+    ~CacheScope() { current_cache() = prev; }
+};
+
 // Runs f, converting any failure into a status and the handle's last error.
 template <class F>
 fk_status guard(fk_handle *h, F &&f) {
+    CacheScope scope(h ? &h->cache : nullptr);
     try {
         f();
         if (h) h->last_error.clear();

@@ -36,35 +36,80 @@ struct Error : std::runtime_error {
 
 #define FK_LAUNCH_CHECK() FK_HIP(hipGetLastError())
 
-// Owning device allocation; freed on scope exit.
+// Device memory cache of one handle.  Blocks its buffers release are kept
+// and handed out again for requests they hold with at most a quarter to
+// spare; they go back to HIP when the handle closes, when an allocation
+// finds the device full, or when the cache would hold more than a quarter of
+// the device.  A build allocates tens of buffers (pools, lists, candidate
+// buckets of a few GB each) and hipMalloc / hipFree of such sizes cost
+// milliseconds each, and hipFree waits for the device; with the cache a
+// repeated build allocates nothing.  The handle's work runs in order on its
+// one stream, so a block reused by the handle cannot be touched by an
+// earlier owner's launch after the new owner's first one.
+struct MemCache {
+    struct Blk {
+        void *p;
+        size_t bytes;
+    };
+    std::vector<Blk> free_;
+    size_t held = 0, limit = 0;
+    void *get(size_t bytes, size_t &got);
+    void put(void *p, size_t bytes);
+    void trim();
+    ~MemCache() { trim(); }
+};
+// The cache of the handle whose C-ABI call runs on this thread (nullptr:
+// plain hipMalloc / hipFree).
+MemCache *&current_cache();
+
+// Owning device allocation; freed (or returned to its handle's cache) on
+// scope exit.
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    MemCache *mc = nullptr;  // the cache the block came from
+    size_t blk = 0;          // its size in bytes
     DevBuf() = default;
     explicit DevBuf(size_t count) { alloc(count); }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n), mc(o.mc), blk(o.blk) { o.p = nullptr; o.n = 0; o.mc = nullptr; }
     DevBuf &operator=(DevBuf &&o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        if (this != &o) {
+            release();
+            p = o.p; n = o.n; mc = o.mc; blk = o.blk;
+            o.p = nullptr; o.n = 0; o.mc = nullptr;
+        }
         return *this;
     }
     ~DevBuf() { release(); }
     void alloc(size_t count) {
         release();
         if (count == 0) count = 1;
-        FK_HIP(hipMalloc((void **)&p, count * sizeof(T)));
+        mc = current_cache();
+        if (mc) {
+            p = static_cast<T *>(mc->get(count * sizeof(T), blk));
+        } else {
+            FK_HIP(hipMalloc((void **)&p, count * sizeof(T)));
+            blk = count * sizeof(T);
+        }
         n = count;
     }
-    // Grow-only reallocation (contents discarded).
+    // Grow-only reallocation (contents discarded); a buffer that grows
+    // again gets a quarter more than asked.
     void ensure(size_t count) {
-        if (count > n || p == nullptr) alloc(count);
+        if (p == nullptr) alloc(count);
+        else if (count > n) alloc(count + count / 4);
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (mc) mc->put(p, blk);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
+        mc = nullptr;
     }
     size_t bytes() const { return n * sizeof(T); }
 };
@@ -151,6 +196,7 @@ struct Dataset {
 }  // namespace fk
 
 struct fk_handle {
+    fk::MemCache cache;  // first member: destroyed after every buffer below
     int device = 0;
     hipStream_t stream = nullptr;
     std::string last_error;

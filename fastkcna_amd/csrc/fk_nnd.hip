@@ -335,6 +335,9 @@ __global__ __launch_bounds__(256) void k_join_fill(const uint32_t *__restrict__ 
 // FK_NM_CH candidate keys and a 2 x FK_NM_CH-slot id set (18.8 KB at
 // MAXL 500: eight workgroups per CU).
 #define FK_NM_CH 512
+#ifndef FK_NM_RANK
+#define FK_NM_RANK 256  // up to this many new keys are ranked by counting, more bitonic-sorted
+#endif
 #define FK_NM_HASH (2 * FK_NM_CH)
 #define FK_NM_NOSRC 0xFFFFFFFFu
 
@@ -447,10 +450,25 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
         __syncthreads();
         const uint32_t nu = s_nu;
         if (nu > 0) {
-            const uint32_t m2 = next_pow2(nu < 2 ? 2 : nu);
-            for (uint32_t k = nu + threadIdx.x; k < m2; k += blockDim.x) s_u[k] = FK_KEY_EMPTY;
-            __syncthreads();
-            block_bitonic_sort(s_u, m2);
+            if (nu <= FK_NM_RANK) {
+                // few keys: each one's rank is the number of smaller keys (the
+                // ids are distinct, so are the keys), counted against the
+                // whole set by broadcast reads; one scatter sorts them
+                uint64_t rk = FK_KEY_EMPTY;
+                uint32_t rr = 0;
+                if (threadIdx.x < nu) {
+                    rk = s_u[threadIdx.x];
+                    for (uint32_t j = 0; j < nu; ++j) rr += s_u[j] < rk ? 1u : 0u;
+                }
+                __syncthreads();
+                if (threadIdx.x < nu) s_u[rr] = rk;
+                __syncthreads();
+            } else {
+                const uint32_t m2 = next_pow2(nu);
+                for (uint32_t k = nu + threadIdx.x; k < m2; k += blockDim.x) s_u[k] = FK_KEY_EMPTY;
+                __syncthreads();
+                block_bitonic_sort(s_u, m2);
+            }
             // each candidate's merged position against the pool before it
             // moves, and its smallest source (at most FK_NM_CH / 256 per thread)
             uint32_t upos[FK_NM_CH / 256], usrc[FK_NM_CH / 256];

@@ -331,7 +331,7 @@ def main():
                    "dist_evals": nd, "aod": int(st["aod"]), "nnd_iterations": int(st["nnd_iterations"]),
                    "graph_digest": digest,
                    "phase_sec": {k: round(float(st[k]), 4) for k in ("t_ctrl", "t_nnd", "t_prune", "t_inter",
-                                                                    "t_ep", "t_conn", "t_bv")}},
+                                                                    "t_ep", "t_conn", "t_bv", "t_total")}},
         "roofline": roof,
         "chosen": chosen,
         "learned_screen": screen,

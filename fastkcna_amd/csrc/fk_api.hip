@@ -127,6 +127,39 @@ void MemCache::trim() {
     held = 0;
 }
 
+// Copies `bytes` of device memory to host memory through the handle's two
+// pinned 16 MB staging buffers: the DMA of one chunk overlaps the host copy
+// of the one before.  Synchronizes the handle's stream.
+void download_bytes(fk_handle *h, void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return;
+    const size_t chunk = (size_t)16 << 20;
+    for (int i = 0; i < 2; ++i) h->dl_stage[i].ensure(chunk);
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    struct Events {
+        hipEvent_t *e;
+        ~Events() {
+            for (int i = 0; i < 2; ++i)
+                if (e[i]) { (void)hipEventSynchronize(e[i]); (void)hipEventDestroy(e[i]); }
+        }
+    } events{ev};
+    for (int i = 0; i < 2; ++i) FK_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+    const size_t nch = (bytes + chunk - 1) / chunk;
+    for (size_t i = 0; i <= nch; ++i) {
+        if (i < nch) {  // chunk i into the stage chunk i - 2 left (copied out last step)
+            const size_t m = std::min(chunk, bytes - i * chunk);
+            FK_HIP(hipMemcpyAsync(h->dl_stage[i & 1].p, (const char *)src + i * chunk, m, hipMemcpyDeviceToHost,
+                                  h->stream));
+            FK_HIP(hipEventRecord(ev[i & 1], h->stream));
+        }
+        if (i >= 1) {
+            const size_t j = i - 1;
+            const size_t m = std::min(chunk, bytes - j * chunk);
+            FK_HIP(hipEventSynchronize(ev[j & 1]));
+            memcpy((char *)dst + j * chunk, h->dl_stage[j & 1].p, m);
+        }
+    }
+}
+
 void require_dataset(fk_handle *h) {
     if (h->ds.n_total == 0) throw Error(FK_E_INVALID, "no dataset uploaded (fk_dataset_upload)");
 }

@@ -402,14 +402,32 @@ static void check_params(const fk_params *p) {
                  "FK_FLAG_SCREEN and FK_FLAG_GRAM cannot be combined");
 }
 
-static void stats_to_out(const BuildState &b, fk_graph *out, std::vector<uint64_t> &off, uint32_t *ids) {
-    out->n = b.N;
+// The graph as CSR in host memory: offsets by a device scan of the
+// out-degrees, ids packed on the device, both downloaded in pinned chunks.
+static void graph_out(BuildState &b, fk_graph *out) {
+    const uint32_t N = b.N;
+    DevBuf<uint64_t> doff((size_t)N + 1);
+    exclusive_scan_u32_to_u64(b.s, b.nsg_len.p, doff.p, N, doff.p + N, b.scan_tmp);
+    uint64_t ne = 0;
+    FK_HIP(hipMemcpyAsync(&ne, doff.p + N, 8, hipMemcpyDeviceToHost, b.s));
+    b.sync();
+    DevBuf<uint32_t> dids(std::max<uint64_t>(ne, 1));
+    hipLaunchKernelGGL(k_pack_csr, dim3((N + 3) / 4), dim3(256), 0, b.s, b.nsg_id.p, b.nsg_len.p, N, b.ncap, doff.p,
+                       dids.p);
+    FK_LAUNCH_CHECK();
+    struct Host {
+        void *p;
+        ~Host() { free(p); }
+    } off{malloc(((size_t)N + 1) * sizeof(uint64_t))}, ids{malloc(std::max<uint64_t>(ne, 1) * sizeof(uint32_t))};
+    if (!off.p || !ids.p) throw std::bad_alloc();
+    download_bytes(b.h, off.p, doff.p, ((size_t)N + 1) * sizeof(uint64_t));
+    download_bytes(b.h, ids.p, dids.p, ne * sizeof(uint32_t));
+    out->n = N;
     out->ep = b.ep;
     out->width = b.width;
-    out->offsets = (uint64_t *)malloc(off.size() * sizeof(uint64_t));
-    if (!out->offsets) throw std::bad_alloc();
-    memcpy(out->offsets, off.data(), off.size() * sizeof(uint64_t));
-    out->ids = ids;
+    out->offsets = static_cast<uint64_t *>(off.p);
+    out->ids = static_cast<uint32_t *>(ids.p);
+    off.p = ids.p = nullptr;
 }
 
 extern "C" {
@@ -444,23 +462,7 @@ fk_status fk_build(fk_handle *h, fk_pg_type type, uint32_t n_used, const fk_para
         if (b.st->n_recall < FK_MAX_LOOPS + 64) b.st->recall[b.st->n_recall++] = rec;
         if (b.st->n_kgraph < FK_MAX_LOOPS + 64) b.st->kgraph_time[b.st->n_kgraph++] = now_sec() - t0;
         build_loops(b, type);
-        // output CSR
-        std::vector<uint32_t> len(b.N);
-        FK_HIP(hipMemcpyAsync(len.data(), b.nsg_len.p, (size_t)b.N * 4, hipMemcpyDeviceToHost, b.s));
-        b.sync();
-        std::vector<uint64_t> off((size_t)b.N + 1, 0);
-        for (uint32_t i = 0; i < b.N; ++i) off[i + 1] = off[i] + len[i];
-        DevBuf<uint64_t> doff(b.N + 1);
-        DevBuf<uint32_t> dids(off[b.N]);
-        FK_HIP(hipMemcpyAsync(doff.p, off.data(), off.size() * 8, hipMemcpyHostToDevice, b.s));
-        hipLaunchKernelGGL(k_pack_csr, dim3((b.N + 3) / 4), dim3(256), 0, b.s, b.nsg_id.p, b.nsg_len.p, b.N, b.ncap,
-                           doff.p, dids.p);
-        FK_LAUNCH_CHECK();
-        uint32_t *ids = (uint32_t *)malloc(std::max<uint64_t>(off[b.N], 1) * 4);
-        if (!ids) throw std::bad_alloc();
-        FK_HIP(hipMemcpyAsync(ids, dids.p, off[b.N] * 4, hipMemcpyDeviceToHost, b.s));
-        b.sync();
-        stats_to_out(b, out, off, ids);
+        graph_out(b, out);
         out->stats.t_total = now_sec() - t0;
     });
 }

@@ -9,6 +9,32 @@ from . import _ffi
 from ._ffi import FkError, FkGraph, FkParams, FkStats, ptr
 
 
+class _GraphMemory:
+    """Owns a graph's host arrays (library allocations) until the numpy
+    views of them are gone."""
+
+    def __init__(self, lib, g):
+        self.lib, self.g = lib, g
+
+    def __del__(self):
+        self.lib.fk_graph_free(ctypes.byref(self.g))
+
+
+def _graph_arrays(lib, g):
+    """(offsets, ids) of a built graph as numpy arrays over the library's
+    own host buffers (no copy); the buffers are freed with the arrays."""
+    mem = _GraphMemory(lib, g)
+    n = g.n
+    ob = (ctypes.c_uint64 * (n + 1)).from_address(ctypes.cast(g.offsets, ctypes.c_void_p).value)
+    ob.owner = mem
+    off = np.frombuffer(ob, dtype=np.uint64)
+    ne = int(off[-1])
+    ib = (ctypes.c_uint32 * max(ne, 1)).from_address(ctypes.cast(g.ids, ctypes.c_void_p).value)
+    ib.owner = mem
+    ids = np.frombuffer(ib, dtype=np.uint32)[:ne]
+    return off, ids
+
+
 def _as(a, dtype):
     return np.ascontiguousarray(a, dtype=dtype)
 
@@ -165,14 +191,8 @@ class Device:
         g = FkGraph()
         p = _ffi.params_struct(params)
         self._check(self.lib.fk_build(self.h, pg_type, n_used, ctypes.byref(p), ctypes.byref(g)), "fk_build")
-        try:
-            n = g.n
-            off = np.ctypeslib.as_array(g.offsets, shape=(n + 1,)).copy()
-            ne = int(off[-1])
-            ids = np.ctypeslib.as_array(g.ids, shape=(max(ne, 1),))[:ne].copy()
-            return off, ids, int(g.ep), int(g.width), g.stats.to_dict()
-        finally:
-            self.lib.fk_graph_free(ctypes.byref(g))
+        off, ids = _graph_arrays(self.lib, g)
+        return off, ids, int(g.ep), int(g.width), g.stats.to_dict()
 
     def build_batch(self, pg_type, params, set_off):
         """Batched build of the resident row ranges [set_off[s], set_off[s+1]);

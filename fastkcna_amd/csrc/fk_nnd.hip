@@ -1240,8 +1240,9 @@ __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ p
 //   u64 c[CH]    candidate keys (distance <= the pool's last at round start)
 //   u32 sel[W], sdeg[W + 1], todo[FK_BVW_TODO], hs[bvw_hsize(cap)]
 //   u8 pf[cap], seln[W]
-// hs is an open-addressing set of ids: the pool's at round start, then every
-// pair target whose distance this round computes.  A target already in it
+// hs is a lossy open-addressing set of ids (hs_seen_or_add): the pool's at
+// round start, then every pair target whose distance this round computes.
+// A target found in it
 // is counted (search_cc counts every pair the reference evaluates) but not
 // computed again: its key equals the pool entry's or the earlier
 // candidate's (the distance is symmetric and computed in one canonical
@@ -1264,7 +1265,7 @@ __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ p
 #endif
 #define FK_BVW_LOADS 8     // chunks of 64 pool entries read at once
 
-// Slots of the id set: a power of two >= 2 cap; at most 3/4 of them filled.
+// Slots of the id set: a power of two >= 2 cap.
 __host__ __device__ inline uint32_t bvw_hsize(uint32_t cap) {
     uint32_t h = 128;
     while (h < 2 * cap) h <<= 1;
@@ -1281,22 +1282,25 @@ __host__ __device__ inline size_t bvw_wave_bytes(uint32_t cap, uint32_t W) {
 
 __device__ __forceinline__ uint32_t hs_slot(uint32_t id, uint32_t mask) { return ((id * 0x9E3779B1u) >> 15) & mask; }
 
-// Inserts id into the set; true when it was not there.  Terminates while
-// the set has an empty slot.
-__device__ __forceinline__ bool hs_insert(uint32_t *hs, uint32_t mask, uint32_t id) {
-    for (uint32_t s = hs_slot(id, mask);; s = (s + 1) & mask) {
-        const uint32_t prev = atomicCAS(&hs[s], FK_HS_EMPTY, id);
-        if (prev == FK_HS_EMPTY) return true;
-        if (prev == id) return false;
-    }
-}
-
-__device__ __forceinline__ bool hs_contains(const uint32_t *hs, uint32_t mask, uint32_t id) {
-    for (uint32_t s = hs_slot(id, mask);; s = (s + 1) & mask) {
+// The id set is lossy: plain LDS reads and stores, at most FK_HS_PROBES
+// slots probed from the id's home slot.  True when id is in the set;
+// otherwise it is stored in the first empty slot of its probe run, if any.
+// An id reported present was stored, so a skip is always right; an id lost
+// (no empty slot in its run, or overwritten by a lane of the same
+// instruction storing another id in the same slot) is only computed again,
+// and its key is dropped at the merge.
+#define FK_HS_PROBES 4
+__device__ __forceinline__ bool hs_seen_or_add(uint32_t *hs, uint32_t mask, uint32_t id) {
+    uint32_t s = hs_slot(id, mask);
+    for (uint32_t q = 0; q < FK_HS_PROBES; ++q, s = (s + 1) & mask) {
         const uint32_t v = hs[s];
         if (v == id) return true;
-        if (v == FK_HS_EMPTY) return false;
+        if (v == FK_HS_EMPTY) {
+            hs[s] = id;
+            return false;
+        }
     }
+    return false;
 }
 
 // Exclusive prefix sum of v over the wave; the wave total in every lane.
@@ -1547,20 +1551,48 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
         ns = (ns + add) < S_cap ? (ns + add) : S_cap;
     }
     if (ns == 0) return;  // nothing selected, nothing changed
-    // the rest of the pool, and the id set: the pool's ids
-    bvw_load_pool(p, pf, pr, pfr, lim, np0);
+    wave_lds_sync();
+    // global loads first: the first 64 selected entries' out-degrees and the
+    // first batch of the rest of the pool; while they are in flight the id
+    // set is cleared and takes the window's ids
+    const uint32_t deg0 = lane < ns ? nsg_len[sel[lane]] : 0u;
+    const uint32_t rest_hi = (np0 - lim) > 64 * FK_BVW_LOADS ? lim + 64 * FK_BVW_LOADS : np0;
+    uint64_t rk[FK_BVW_LOADS];
+    uint32_t rf[FK_BVW_LOADS];
+    if (lim < np0) {
+#pragma unroll
+        for (uint32_t r = 0; r < FK_BVW_LOADS; ++r) {
+            const uint32_t j = lim + r * 64 + lane;
+            const uint32_t jj = j < rest_hi ? j : rest_hi - 1;
+            rk[r] = pr[jj];
+            rf[r] = pfr[jj];
+        }
+    }
     for (uint32_t j = lane; j < hsize; j += 64) hs[j] = FK_HS_EMPTY;
     wave_lds_sync();
-    for (uint32_t j = lane; j < np0; j += 64) hs_insert(hs, hmask, key_id(p[j]));
-    uint32_t hcount = np0;
-    const uint32_t hlim = hsize / 4 * 3;
+    for (uint32_t j = lane; j < lim; j += 64) hs_seen_or_add(hs, hmask, key_id(p[j]));
+    if (lim < np0) {
+#pragma unroll
+        for (uint32_t r = 0; r < FK_BVW_LOADS; ++r) {
+            const uint32_t j = lim + r * 64 + lane;
+            if (j < rest_hi) {
+                p[j] = rk[r];
+                pf[j] = (uint8_t)rf[r];
+                hs_seen_or_add(hs, hmask, key_id(rk[r]));
+            }
+        }
+    }
+    if (rest_hi < np0) {  // pools longer than the window + 64 FK_BVW_LOADS
+        bvw_load_pool(p, pf, pr, pfr, rest_hi, np0);
+        wave_lds_sync();
+        for (uint32_t j = rest_hi + lane; j < np0; j += 64) hs_seen_or_add(hs, hmask, key_id(p[j]));
+    }
     wave_lds_sync();
-    // exclusive prefix of the selected out-degrees (read here, after the
-    // selection, so that one round trip serves 64 selected entries)
+    // exclusive prefix of the selected out-degrees
     uint32_t run = 0;
     for (uint32_t s0 = 0; s0 < ns; s0 += 64) {
         const uint32_t x = s0 + lane;
-        const uint32_t v = x < ns ? nsg_len[sel[x]] : 0u;
+        const uint32_t v = s0 == 0 ? deg0 : (x < ns ? nsg_len[sel[x]] : 0u);
         uint32_t tot;
         const uint32_t ex = wave_excl_scan(v, tot);
         wave_lds_sync();
@@ -1649,14 +1681,11 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
             const uint32_t t = tr[r];
             const bool use = t != i && (sr[r] || nr[r]);
             cc += (uint32_t)__builtin_popcountll(wave_ballot(use));  // every pair the reference evaluates
-            // computed only when the target is not in the id set; while the
-            // set has room for a whole chunk, the target joins it
-            const bool grow = hcount + 64 <= hlim;
-            bool fresh = false;
-            if (use) fresh = grow ? hs_insert(hs, hmask, t) : !hs_contains(hs, hmask, t);
+            // computed only when the target is not in the id set (which
+            // then takes it)
+            const bool fresh = use && !hs_seen_or_add(hs, hmask, t);
             const uint64_t mf = wave_ballot(fresh);
             const uint32_t nf = (uint32_t)__builtin_popcountll(mf);
-            if (grow) hcount += nf;
             if (fresh) todo[nt + lanes_below(mf)] = t;
             nt += nf;
         }

@@ -77,22 +77,19 @@ __global__ void k_in_fill(const uint32_t *__restrict__ nsg_id, const float *__re
 
 #define FK_INTER_MAX 4096
 
-// One workgroup per node: out-list A plus the in-edges B not already in A.
-// If the union fits nsg_R it is A followed by B in (dist, id) order;
-// otherwise the union, sorted, is staged for the reversed prune.
-__global__ __launch_bounds__(256) void k_inter_merge(uint32_t N, uint32_t ncap, uint32_t R,
-                                                     uint32_t *__restrict__ nsg_id, float *__restrict__ nsg_d,
-                                                     uint32_t *__restrict__ nsg_len,
-                                                     const uint32_t *__restrict__ indeg,
-                                                     const uint64_t *__restrict__ in_off,
-                                                     const uint64_t *__restrict__ in_key, uint64_t *__restrict__ rp_key,
-                                                     uint64_t *__restrict__ rp_off, uint32_t *__restrict__ rp_len,
-                                                     uint32_t *__restrict__ task_list, uint32_t *__restrict__ n_tasks,
-                                                     uint32_t *__restrict__ err) {
+// Out-list A of node des plus the in-edges B not already in A (one
+// workgroup).  If the union fits nsg_R it is A followed by B in (dist, id)
+// order; otherwise the union, sorted, is staged for the reversed prune.
+__device__ void inter_merge_node(uint32_t des, uint32_t ncap, uint32_t R, uint32_t *__restrict__ nsg_id,
+                                 float *__restrict__ nsg_d, uint32_t *__restrict__ nsg_len,
+                                 const uint32_t *__restrict__ indeg, const uint64_t *__restrict__ in_off,
+                                 const uint64_t *__restrict__ in_key, uint64_t *__restrict__ rp_key,
+                                 uint64_t *__restrict__ rp_off, uint32_t *__restrict__ rp_len,
+                                 uint32_t *__restrict__ task_list, uint32_t *__restrict__ n_tasks,
+                                 uint32_t *__restrict__ err) {
     __shared__ uint64_t s_b[FK_INTER_MAX];
     __shared__ uint32_t s_a[FK_NSG_SLACK + 256];
     __shared__ uint32_t s_nb;
-    const uint32_t des = blockIdx.x;
     const uint32_t nA = nsg_len[des];
     const uint32_t nB = indeg[des];
     if (nB == 0) {
@@ -149,6 +146,115 @@ __global__ __launch_bounds__(256) void k_inter_merge(uint32_t N, uint32_t ncap, 
         rp_off[des] = off;
         rp_len[des] = tot;
         task_list[atomicAdd(n_tasks, 1u)] = des;
+    }
+}
+
+// The nodes k_inter_small leaves (unions above FK_INTER_SMALL entries),
+// one workgroup at a time each.
+__global__ __launch_bounds__(256) void k_inter_merge(const uint32_t *__restrict__ list,
+                                                     const uint32_t *__restrict__ n_list, uint32_t ncap, uint32_t R,
+                                                     uint32_t *__restrict__ nsg_id, float *__restrict__ nsg_d,
+                                                     uint32_t *__restrict__ nsg_len,
+                                                     const uint32_t *__restrict__ indeg,
+                                                     const uint64_t *__restrict__ in_off,
+                                                     const uint64_t *__restrict__ in_key, uint64_t *__restrict__ rp_key,
+                                                     uint64_t *__restrict__ rp_off, uint32_t *__restrict__ rp_len,
+                                                     uint32_t *__restrict__ task_list, uint32_t *__restrict__ n_tasks,
+                                                     uint32_t *__restrict__ err) {
+    const uint32_t n = *n_list;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        inter_merge_node(list[i], ncap, R, nsg_id, nsg_d, nsg_len, indeg, in_off, in_key, rp_key, rp_off, rp_len,
+                         task_list, n_tasks, err);
+        __syncthreads();
+    }
+}
+
+// The same merge for the common node, one wavefront per node, when the
+// union has at most FK_INTER_SMALL entries: B minus A compacted in order,
+// ranked by counting (keys are distinct: one in-edge per source), written
+// straight to its slots.  Larger nodes are listed for k_inter_merge.
+#define FK_INTER_SMALL 256
+#define FK_INTER_WPB 4
+__global__ __launch_bounds__(64 * FK_INTER_WPB) void k_inter_small(
+    uint32_t N, uint32_t ncap, uint32_t R, uint32_t *__restrict__ nsg_id, float *__restrict__ nsg_d,
+    uint32_t *__restrict__ nsg_len, const uint32_t *__restrict__ indeg, const uint64_t *__restrict__ in_off,
+    const uint64_t *__restrict__ in_key, uint64_t *__restrict__ rp_key, uint64_t *__restrict__ rp_off,
+    uint32_t *__restrict__ rp_len, uint32_t *__restrict__ task_list, uint32_t *__restrict__ n_tasks,
+    uint32_t *__restrict__ big_list, uint32_t *__restrict__ n_big) {
+    __shared__ uint64_t s_b[FK_INTER_WPB][FK_INTER_SMALL];
+    __shared__ uint32_t s_a[FK_INTER_WPB][FK_INTER_SMALL];
+    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t des = blockIdx.x * FK_INTER_WPB + w;
+    if (des >= N) return;
+    const uint32_t nA = nsg_len[des];
+    const uint32_t nB = indeg[des];
+    if (nB == 0) {
+        if (lane == 0) rp_len[des] = 0;
+        return;
+    }
+    if (nA + nB > FK_INTER_SMALL) {
+        if (lane == 0) big_list[atomicAdd(n_big, 1u)] = des;
+        return;
+    }
+    uint32_t *sa = s_a[w];
+    uint64_t *sb = s_b[w];
+    uint32_t *ai = nsg_id + (uint64_t)des * ncap;
+    float *ad = nsg_d + (uint64_t)des * ncap;
+    for (uint32_t k = lane; k < nA; k += 64) sa[k] = ai[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // B minus A (reference: dedupe against the current list, :2063-2076)
+    const uint64_t *bk = in_key + in_off[des];
+    uint32_t nb = 0;
+    for (uint32_t k0 = 0; k0 < nB; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        uint64_t key = FK_KEY_EMPTY;
+        bool keep = false;
+        if (k < nB) {
+            key = bk[k];
+            const uint32_t src = key_id(key);
+            keep = true;
+            for (uint32_t j = 0; j < nA; ++j)
+                if (sa[j] == src) { keep = false; break; }
+        }
+        const uint64_t m = __ballot(keep);
+        if (keep) sb[nb + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = key;
+        nb += (uint32_t)__builtin_popcountll(m);
+    }
+    if (nb == 0) {
+        if (lane == 0) rp_len[des] = 0;
+        return;
+    }
+    const bool fits = nA + nb <= R;
+    // over-full: the union A U B (A's keys after B's) is ranked instead
+    const uint32_t tot = fits ? nb : nA + nb;
+    if (!fits)
+        for (uint32_t k = lane; k < nA; k += 64) sb[nb + k] = make_key(ad[k], sa[k]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint64_t off = in_off[des] + (uint64_t)des * R;
+    for (uint32_t k = lane; k < tot; k += 64) {
+        const uint64_t key = sb[k];
+        uint32_t r = 0;
+        for (uint32_t j = 0; j < tot; ++j) r += sb[j] < key ? 1u : 0u;
+        if (fits) {
+            ai[nA + r] = key_id(key);
+            ad[nA + r] = key_dist(key);
+        } else {
+            rp_key[off + r] = key;
+        }
+    }
+    if (lane == 0) {
+        if (fits) {
+            nsg_len[des] = nA + nb;
+            rp_len[des] = 0;
+        } else {
+            rp_off[des] = off;
+            rp_len[des] = tot;
+            task_list[atomicAdd(n_tasks, 1u)] = des;
+        }
     }
 }
 
@@ -273,11 +379,15 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
                        b.boff.p, b.pair_cnt.p, b.bkey.p);
     FK_LAUNCH_CHECK();
     DevBuf<uint64_t> rp_key(n_in + (uint64_t)N * P.nsg_R), rp_off(N);
-    DevBuf<uint32_t> rp_len(N), tasks(N), counters(2);
-    FK_HIP(hipMemsetAsync(counters.p, 0, 8, b.s));
-    hipLaunchKernelGGL(k_inter_merge, dim3(N), dim3(256), 0, b.s, N, b.ncap, P.nsg_R, b.nsg_id.p, b.nsg_d.p,
-                       b.nsg_len.p, b.bcnt.p, b.boff.p, b.bkey.p, rp_key.p, rp_off.p, rp_len.p, tasks.p, counters.p,
-                       counters.p + 1);
+    DevBuf<uint32_t> rp_len(N), tasks(N), big(N), counters(3);
+    FK_HIP(hipMemsetAsync(counters.p, 0, 12, b.s));
+    hipLaunchKernelGGL(k_inter_small, dim3((N + FK_INTER_WPB - 1) / FK_INTER_WPB), dim3(64 * FK_INTER_WPB), 0, b.s, N,
+                       b.ncap, P.nsg_R, b.nsg_id.p, b.nsg_d.p, b.nsg_len.p, b.bcnt.p, b.boff.p, b.bkey.p, rp_key.p,
+                       rp_off.p, rp_len.p, tasks.p, counters.p, big.p, counters.p + 2);
+    FK_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_inter_merge, dim3(256), dim3(256), 0, b.s, big.p, counters.p + 2, b.ncap, P.nsg_R,
+                       b.nsg_id.p, b.nsg_d.p, b.nsg_len.p, b.bcnt.p, b.boff.p, b.bkey.p, rp_key.p, rp_off.p, rp_len.p,
+                       tasks.p, counters.p, counters.p + 1);
     FK_LAUNCH_CHECK();
     uint32_t hc[2] = {0, 0};
     FK_HIP(hipMemcpyAsync(hc, counters.p, 8, hipMemcpyDeviceToHost, b.s));

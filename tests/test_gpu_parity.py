@@ -289,3 +289,36 @@ def test_prune_bitexact_c2_dumped_pool(device, pg):
         assert np.array_equal(g[1][t, :k].view(np.uint32), c[1][t, :k].view(np.uint32)), t
     assert g[3] == c[3]
     say("c2 dumped pool: kept", int(c[2].sum()), "distances", c[3])
+
+
+def test_full_build_hub_in_degree(device):
+    """One hub nearest to every other point: its in-degree (1,499) takes the
+    large-union path of the reverse-edge merge and the reversed prune; the
+    graph keeps the invariants and the oracle's query recall."""
+    rng = np.random.default_rng(9)
+    n, d = 1500, 32
+    u = rng.normal(size=(n - 1, d))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    x = np.zeros((n, d), np.float32)
+    x[1:] = (u * (100.0 + rng.uniform(0.0, 1.0, size=(n - 1, 1)))).astype(np.float32)
+    say("build hub", x.shape)
+    device.upload(x)
+    p = IndexParams(K=40, L=40, S=10, R=40, iterations=5, nsg_R=16, search_L=40, search_K=40)
+    off, ids, ep, width, st = device.build(FK_PG_NSG, p)
+    for i in range(n):
+        nb = ids[off[i]:off[i + 1]]
+        assert i not in nb and len(set(nb.tolist())) == len(nb)
+    assert np.diff(off).max() <= width
+    assert reachable(off, ids, ep).all()
+    into_hub = int(np.count_nonzero(ids == 0))
+    say("out-lists holding the hub", into_hub, "reversed-prune distances", st["d_prune_rev"])
+    assert into_hub > 256 and st["d_prune_rev"] > 0
+    qs = (x[rng.choice(n, 100, replace=False)] * 0.98).astype(np.float32)
+    gt, _ = oracle.knn(x, qs, 10)
+    res, _ = device.search(off, ids, ep, qs, 40, 10)
+    rg = np.mean([len(set(res[i]) & set(gt[i])) / 10 for i in range(100)])
+    coff, cids, cep, _, _, _ = oracle.build(FK_PG_NSG, x, p, threads=8)
+    resc, _ = device.search(coff, cids, cep, qs, 40, 10)
+    rc = np.mean([len(set(resc[i]) & set(gt[i])) / 10 for i in range(100)])
+    say("query recall@10 gpu", rg, "oracle", rc)
+    assert rg >= rc - 0.03

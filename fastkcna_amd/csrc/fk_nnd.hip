@@ -216,11 +216,11 @@ __global__ void k_pair_counts(const uint64_t *__restrict__ list_off, const uint3
 // src/kgraph.cpp:970-987 enumerates new x new (i < j) and new x old; a pair
 // joins j into pool(i) and i into pool(j).  Pairs of equal ids add nothing.
 // ---------------------------------------------------------------------------
-// Dynamic LDS of the two passes: per-list-slot threshold, count and id (and
-// bucket base in pass 2), sized by the longest list (L + 2R); pass 1 also
-// stages the sampled rows when they fit.
+// Dynamic LDS of the two passes, sized by the longest list (L + 2R): pass 1
+// keeps per-list-slot threshold, count and id and stages the sampled rows
+// when they fit; pass 2 keeps the ids.
 size_t join_count_lds(uint32_t tcap) { return FK_JOIN_LDS_ROWS_BYTES + (size_t)tcap * 12; }
-size_t join_fill_lds(uint32_t tcap) { return (size_t)tcap * 20; }
+size_t join_fill_lds(uint32_t tcap) { return (size_t)tcap * 4; }
 
 __global__ __launch_bounds__(256) void k_join_count(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
                                                     const uint32_t *__restrict__ lists,
@@ -262,8 +262,13 @@ __global__ __launch_bounds__(256) void k_join_count(const float *__restrict__ ro
     }
 }
 
-// Pass 2: write the candidates into their reserved bucket slots (same walk
-// and conditions as pass 1, so the per-target counts agree).
+// Pass 2: write the candidates into their reserved bucket slots.  One
+// wavefront per list slot t: t's partners (every other slot when t is new,
+// the new slots when t is old: the pairs pass 1 computed) are taken in list
+// order 64 at a time, and those with another id and a distance within t's
+// threshold fill t's reserved run of slots in that order -- the pairs and
+// conditions of pass 1, so the counts agree, and every store instruction
+// writes one contiguous run instead of 64 scattered slots.
 __global__ __launch_bounds__(256) void k_join_fill(const uint32_t *__restrict__ lists,
                                                    const uint64_t *__restrict__ list_off,
                                                    const uint32_t *__restrict__ n_new,
@@ -273,10 +278,7 @@ __global__ __launch_bounds__(256) void k_join_fill(const uint32_t *__restrict__ 
                                                    const uint64_t *__restrict__ boff, uint64_t *__restrict__ bkey,
                                                    uint32_t *__restrict__ bsrc, uint32_t N,
                                                    const uint32_t *__restrict__ order, uint32_t tcap) {
-    extern __shared__ __align__(16) uint64_t s_base[];
-    float *s_thr = reinterpret_cast<float *>(s_base + tcap);
-    uint32_t *s_c = reinterpret_cast<uint32_t *>(s_thr + tcap);
-    uint32_t *s_id = s_c + tcap;
+    extern __shared__ __align__(16) uint32_t s_id[];
     const uint32_t node = node_of_block(order, N);
     if (node >= N) return;
     const uint64_t lo = list_off[node];
@@ -284,30 +286,34 @@ __global__ __launch_bounds__(256) void k_join_fill(const uint32_t *__restrict__ 
     const uint32_t nn = n_new[node];
     const uint32_t *lst = lists + lo;
     const float *o = pdist + pair_off[node];
-    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
-        const uint32_t id = lst[t];
-        s_id[t] = id;
-        s_thr[t] = thr[id];
-        s_c[t] = 0;
-        s_base[t] = boff[id] + resv[lo + t];
-    }
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) s_id[t] = lst[t];
     __syncthreads();
-    const uint32_t cells = nn * T;
-    for (uint32_t c = threadIdx.x; c < cells; c += blockDim.x) {
-        const uint32_t x = c / T, y = c - x * T;
-        if (y <= x) continue;
-        const uint32_t ix = s_id[x], iy = s_id[y];
-        if (ix == iy) continue;
-        const float dd = o[join_pair_index(x, y, T)];
-        if (dd <= s_thr[x]) {
-            const uint64_t sl = s_base[x] + atomicAdd(&s_c[x], 1u);
-            bkey[sl] = make_key(dd, iy);
-            bsrc[sl] = node;
-        }
-        if (dd <= s_thr[y]) {
-            const uint64_t sl = s_base[y] + atomicAdd(&s_c[y], 1u);
-            bkey[sl] = make_key(dd, ix);
-            bsrc[sl] = node;
+    const uint32_t lane = lane_id();
+    for (uint32_t t = threadIdx.x >> 6; t < T; t += blockDim.x >> 6) {
+        const uint32_t it = s_id[t];
+        const float th = thr[it];
+        uint64_t sl = boff[it] + resv[lo + t];
+        const uint32_t ylim = t < nn ? T : nn;
+        for (uint32_t y0 = 0; y0 < ylim; y0 += 64) {
+            const uint32_t y = y0 + lane;
+            bool ok = false;
+            float dd = 0.f;
+            uint32_t iy = 0;
+            if (y < ylim && y != t) {
+                iy = s_id[y];
+                if (iy != it) {
+                    dd = o[t < y ? join_pair_index(t, y, T) : join_pair_index(y, t, T)];
+                    ok = dd <= th;
+                }
+            }
+            const uint64_t m = __ballot(ok);
+            if (ok) {
+                const uint64_t at = sl + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                bkey[at] = make_key(dd, iy);
+                bsrc[at] = node;
+            }
+            sl += (uint32_t)__builtin_popcountll(m);
         }
     }
 }
@@ -1248,7 +1254,7 @@ __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ p
 // candidate's (the distance is symmetric and computed in one canonical
 // order), which the merge drops, and an id pushed out of the pool this
 // round has at least cap better keys ahead of it for the rest of the round.
-#define FK_BVW_WAVES 4
+#define FK_BVW_WAVES 2
 #define FK_BVW_CH 128
 #ifndef FK_BVW_INFLIGHT
 #define FK_BVW_INFLIGHT 2  // rows per octet in flight in the distance step
@@ -1265,10 +1271,13 @@ __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ p
 #endif
 #define FK_BVW_LOADS 8     // chunks of 64 pool entries read at once
 
-// Slots of the id set: a power of two >= 2 cap.
+// Slots of the id set: a power of two >= cap.  With the short probe runs a
+// full set only loses ids (computed again); measured on the 1M x 128 NSG
+// build, this size and two probes beat larger sets and longer runs (LDS per
+// wave sets the occupancy).
 __host__ __device__ inline uint32_t bvw_hsize(uint32_t cap) {
-    uint32_t h = 128;
-    while (h < 2 * cap) h <<= 1;
+    uint32_t h = 64;
+    while (h < cap) h <<= 1;
     return h;
 }
 
@@ -1289,7 +1298,7 @@ __device__ __forceinline__ uint32_t hs_slot(uint32_t id, uint32_t mask) { return
 // (no empty slot in its run, or overwritten by a lane of the same
 // instruction storing another id in the same slot) is only computed again,
 // and its key is dropped at the merge.
-#define FK_HS_PROBES 4
+#define FK_HS_PROBES 2
 __device__ __forceinline__ bool hs_seen_or_add(uint32_t *hs, uint32_t mask, uint32_t id) {
     uint32_t s = hs_slot(id, mask);
     for (uint32_t q = 0; q < FK_HS_PROBES; ++q, s = (s + 1) & mask) {

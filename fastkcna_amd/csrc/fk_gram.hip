@@ -263,9 +263,10 @@ __global__ __launch_bounds__(256) void k_join_gram(const float *__restrict__ row
                                                    const float *__restrict__ thr, float *__restrict__ pdist,
                                                    uint32_t *__restrict__ resv, uint32_t *__restrict__ bcnt,
                                                    uint32_t N, const uint32_t *__restrict__ order, uint32_t tcap,
-                                                   unsigned long long *__restrict__ n_recheck) {
+                                                   unsigned long long *__restrict__ n_recheck, uint32_t rows_bytes,
+                                                   uint32_t t_lo, uint32_t t_hi) {
     extern __shared__ __align__(16) float s_rows[];
-    float2 *s_n = reinterpret_cast<float2 *>(s_rows + FK_JOIN_LDS_ROWS_BYTES / 4);
+    float2 *s_n = reinterpret_cast<float2 *>(s_rows + rows_bytes / 4);
     float *s_thr = reinterpret_cast<float *>(s_n + tcap);
     uint32_t *s_c = reinterpret_cast<uint32_t *>(s_thr + tcap);
     uint32_t *s_id = s_c + tcap;
@@ -273,6 +274,7 @@ __global__ __launch_bounds__(256) void k_join_gram(const float *__restrict__ row
     if (node >= N) return;
     const uint64_t lo = list_off[node];
     const uint32_t T = (uint32_t)(list_off[node + 1] - lo);
+    if (T <= t_lo || T > t_hi) return;  // another launch's lists
     const uint32_t nn = n_new[node];
     const uint32_t *lst = lists + lo;
     float *o = pdist + pair_off[node];
@@ -285,7 +287,7 @@ __global__ __launch_bounds__(256) void k_join_gram(const float *__restrict__ row
     }
     __syncthreads();
     const uint32_t ls = dpad + 4;
-    const uint32_t cap = FK_JOIN_LDS_ROWS_BYTES / (ls * 4);
+    const uint32_t cap = rows_bytes / (ls * 4);
     const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
     uint32_t rechecks = 0;
     // x chunk [x0, x0 + nx) at s_rows, y chunk [y0, y0 + ny) at s_y
@@ -350,23 +352,32 @@ __global__ __launch_bounds__(256) void k_join_gram(const float *__restrict__ row
     }
 }
 
-size_t join_gram_lds(uint32_t tcap) { return FK_JOIN_LDS_ROWS_BYTES + (size_t)tcap * 20; }
+size_t join_gram_lds(uint32_t tcap, uint32_t rows_bytes) { return rows_bytes + (size_t)tcap * 20; }
 
+// Lists of up to FK_JOIN_SMALL_ROWS rows first with a small staging area
+// (more workgroups per CU), then the longer ones with the full area.
 void launch_join_gram(BuildState &b, uint32_t tcap, unsigned long long *n_recheck) {
     const Dataset &ds = *b.ds;
     const GramBound gb = gram_bound(ds);
     const uint32_t *ord = b.has_order ? b.order.p : nullptr;
-    if (ds.gram_fmt == FK_GRAM_F16)
-        hipLaunchKernelGGL(k_join_gram<FK_GRAM_F16>, dim3(node_grid(b)), dim3(256), join_gram_lds(tcap), b.s,
-                           ds.rows.p, ds.stride, ds.dpad, ds.gram_nrm.p, ds.gram_scale, gb, b.lists.p, b.list_off.p,
-                           b.n_new.p, b.pair_off.p, b.thr.p, b.pdist.p, b.resv.p, b.bcnt.p, b.N, ord, tcap,
-                           n_recheck);
-    else
-        hipLaunchKernelGGL(k_join_gram<FK_GRAM_BF16>, dim3(node_grid(b)), dim3(256), join_gram_lds(tcap), b.s,
-                           ds.rows.p, ds.stride, ds.dpad, ds.gram_nrm.p, ds.gram_scale, gb, b.lists.p, b.list_off.p,
-                           b.n_new.p, b.pair_off.p, b.thr.p, b.pdist.p, b.resv.p, b.bcnt.p, b.N, ord, tcap,
-                           n_recheck);
-    FK_LAUNCH_CHECK();
+    const uint32_t row_bytes = (ds.dpad + 4) * 4;
+    const uint32_t small_bytes = std::min<uint32_t>(FK_JOIN_SMALL_ROWS * row_bytes, FK_JOIN_LDS_ROWS_BYTES);
+    const uint32_t small_t = small_bytes / row_bytes;
+    auto launch = [&](uint32_t tc, uint32_t rb, uint32_t t_lo, uint32_t t_hi) {
+        if (ds.gram_fmt == FK_GRAM_F16)
+            hipLaunchKernelGGL(k_join_gram<FK_GRAM_F16>, dim3(node_grid(b)), dim3(256), join_gram_lds(tc, rb), b.s,
+                               ds.rows.p, ds.stride, ds.dpad, ds.gram_nrm.p, ds.gram_scale, gb, b.lists.p,
+                               b.list_off.p, b.n_new.p, b.pair_off.p, b.thr.p, b.pdist.p, b.resv.p, b.bcnt.p, b.N,
+                               ord, tc, n_recheck, rb, t_lo, t_hi);
+        else
+            hipLaunchKernelGGL(k_join_gram<FK_GRAM_BF16>, dim3(node_grid(b)), dim3(256), join_gram_lds(tc, rb), b.s,
+                               ds.rows.p, ds.stride, ds.dpad, ds.gram_nrm.p, ds.gram_scale, gb, b.lists.p,
+                               b.list_off.p, b.n_new.p, b.pair_off.p, b.thr.p, b.pdist.p, b.resv.p, b.bcnt.p, b.N,
+                               ord, tc, n_recheck, rb, t_lo, t_hi);
+        FK_LAUNCH_CHECK();
+    };
+    launch(std::min(tcap, small_t), small_bytes, 0u, small_t);
+    if (tcap > small_t) launch(tcap, FK_JOIN_LDS_ROWS_BYTES, small_t, 0xFFFFFFFFu);
 }
 
 // ---------------------------------------------------------------------------

@@ -11,6 +11,9 @@
 #include "fk_device.h"
 
 #define FK_JOIN_LDS_ROWS_BYTES (40 * 1024)
+#ifndef FK_JOIN_SMALL_ROWS
+#define FK_JOIN_SMALL_ROWS 40  // lists this short are joined with a small staging area
+#endif
 
 __device__ __forceinline__ void join_stage_rows(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
                                                 const uint32_t *s_id, uint32_t first, uint32_t count,
@@ -24,14 +27,16 @@ __device__ __forceinline__ void join_stage_rows(const float *__restrict__ rows, 
 }
 
 // Calls emit(x, y, dist) once for every pair of the block.  s_id holds the
-// T list ids in LDS; s_rows has FK_JOIN_LDS_ROWS_BYTES.  Every thread of the
-// block must call it; it returns after a barrier-free compute phase, so the
-// caller synchronises before reading what emit wrote to LDS.
+// T list ids in LDS; s_rows has rows_bytes (at most FK_JOIN_LDS_ROWS_BYTES).
+// Every thread of the block must call it; it returns after a barrier-free
+// compute phase, so the caller synchronises before reading what emit wrote
+// to LDS.
 template <class Emit>
 __device__ void join_block(const float *__restrict__ rows, uint32_t stride, uint32_t dpad, const uint32_t *s_id,
-                           uint32_t T, uint32_t nn, float *s_rows, Emit emit) {
+                           uint32_t T, uint32_t nn, float *s_rows, Emit emit,
+                           uint32_t rows_bytes = FK_JOIN_LDS_ROWS_BYTES) {
     const uint32_t ls = dpad + 4;
-    const uint32_t cap = FK_JOIN_LDS_ROWS_BYTES / (ls * 4);
+    const uint32_t cap = rows_bytes / (ls * 4);
     if (T <= cap) {
         join_stage_rows(rows, stride, dpad, s_id, 0, T, s_rows);
         __syncthreads();

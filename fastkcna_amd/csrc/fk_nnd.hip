@@ -219,7 +219,7 @@ __global__ void k_pair_counts(const uint64_t *__restrict__ list_off, const uint3
 // Dynamic LDS of the two passes, sized by the longest list (L + 2R): pass 1
 // keeps per-list-slot threshold, count and id and stages the sampled rows
 // when they fit; pass 2 keeps the ids.
-size_t join_count_lds(uint32_t tcap) { return FK_JOIN_LDS_ROWS_BYTES + (size_t)tcap * 12; }
+size_t join_count_lds(uint32_t tcap, uint32_t rows_bytes) { return rows_bytes + (size_t)tcap * 12; }
 size_t join_fill_lds(uint32_t tcap) { return (size_t)tcap * 4; }
 
 __global__ __launch_bounds__(256) void k_join_count(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
@@ -229,15 +229,17 @@ __global__ __launch_bounds__(256) void k_join_count(const float *__restrict__ ro
                                                     const uint64_t *__restrict__ pair_off,
                                                     const float *__restrict__ thr, float *__restrict__ pdist,
                                                     uint32_t *__restrict__ resv, uint32_t *__restrict__ bcnt,
-                                                    uint32_t N, const uint32_t *__restrict__ order, uint32_t tcap) {
+                                                    uint32_t N, const uint32_t *__restrict__ order, uint32_t tcap,
+                                                    uint32_t rows_bytes, uint32_t t_lo, uint32_t t_hi) {
     extern __shared__ __align__(16) float s_rows[];
-    float *s_thr = s_rows + FK_JOIN_LDS_ROWS_BYTES / 4;
+    float *s_thr = s_rows + rows_bytes / 4;
     uint32_t *s_c = reinterpret_cast<uint32_t *>(s_thr + tcap);
     uint32_t *s_id = s_c + tcap;
     const uint32_t node = node_of_block(order, N);
     if (node >= N) return;
     const uint64_t lo = list_off[node];
     const uint32_t T = (uint32_t)(list_off[node + 1] - lo);
+    if (T <= t_lo || T > t_hi) return;  // another launch's lists
     const uint32_t nn = n_new[node];
     const uint32_t *lst = lists + lo;
     float *o = pdist + pair_off[node];
@@ -254,7 +256,7 @@ __global__ __launch_bounds__(256) void k_join_count(const float *__restrict__ ro
             if (dd <= s_thr[x]) atomicAdd(&s_c[x], 1u);
             if (dd <= s_thr[y]) atomicAdd(&s_c[y], 1u);
         }
-    });
+    }, rows_bytes);
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
         const uint32_t c = s_c[t];
@@ -986,7 +988,6 @@ static void nnd_join(BuildState &b) {
                         "stage rows in LDS (T <= %u)\n", (double)lo[N] / N, (double)nsum / N,
                 (unsigned long long)tmax, 100.0 * nfit / N, 100.0 * pfit / std::max<uint64_t>(ptot, 1), fit);
     }
-    const size_t lds = join_count_lds(tcap);
     const uint32_t *ord = b.has_order ? b.order.p : nullptr;
     const double d4 = 4.0 * b.ds->d;
     const bool gram = (b.P.flags & FK_FLAG_GRAM) != 0;
@@ -999,10 +1000,24 @@ static void nnd_join(BuildState &b) {
     } else if (screen) {
         launch_join_screen(b, tcap);
     } else {
-        hipLaunchKernelGGL(k_join_count, dim3(node_grid(b)), dim3(256), lds, b.s, b.ds->rows.p, b.ds->stride,
-                           b.ds->dpad, b.lists.p, b.list_off.p, b.n_new.p, b.pair_off.p, b.thr.p, b.pdist.p,
-                           b.resv.p, b.bcnt.p, N, ord, tcap);
+        // lists of up to FK_JOIN_SMALL_ROWS rows first, staged in a small LDS
+        // area (more workgroups per CU), then the longer ones with the full one
+        const uint32_t row_bytes = (b.ds->dpad + 4) * 4;
+        const uint32_t small_bytes = std::min<uint32_t>(FK_JOIN_SMALL_ROWS * row_bytes, FK_JOIN_LDS_ROWS_BYTES);
+        const uint32_t small_t = small_bytes / row_bytes;
+        const uint32_t tsm = std::min(tcap, small_t);
+        hipLaunchKernelGGL(k_join_count, dim3(node_grid(b)), dim3(256), join_count_lds(tsm, small_bytes), b.s,
+                           b.ds->rows.p, b.ds->stride, b.ds->dpad, b.lists.p, b.list_off.p, b.n_new.p, b.pair_off.p,
+                           b.thr.p, b.pdist.p, b.resv.p, b.bcnt.p, N, ord, tsm, small_bytes, 0u, small_t);
         FK_LAUNCH_CHECK();
+        if (tcap > small_t) {
+            hipLaunchKernelGGL(k_join_count, dim3(node_grid(b)), dim3(256),
+                               join_count_lds(tcap, FK_JOIN_LDS_ROWS_BYTES), b.s, b.ds->rows.p, b.ds->stride,
+                               b.ds->dpad, b.lists.p, b.list_off.p, b.n_new.p, b.pair_off.p, b.thr.p, b.pdist.p,
+                               b.resv.p, b.bcnt.p, N, ord, tcap, (uint32_t)FK_JOIN_LDS_ROWS_BYTES, small_t,
+                               0xFFFFFFFFu);
+            FK_LAUNCH_CHECK();
+        }
     }
     // sampled rows and ids gathered once per join, one float per pair written
     b.prof_end(pe, FK_PROF_JOIN, (double)nlist * (d4 + 4.0) + 4.0 * (double)pairs);

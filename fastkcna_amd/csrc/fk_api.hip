@@ -3,8 +3,11 @@
 // NN-descent live in fk_build.hip, the query search in fk_search.hip.
 #include <math.h>
 
+#include <sys/mman.h>
+
 #include <algorithm>
 #include <new>
+#include <thread>
 
 #include "fk_api_impl.h"
 #include "fk_device.h"
@@ -127,6 +130,34 @@ void MemCache::trim() {
     held = 0;
 }
 
+// Host memory for a result the caller frees with free(): 2 MB aligned and
+// advised for huge pages, so that filling it takes few page faults.
+void *host_result_alloc(size_t bytes) {
+    const size_t huge = (size_t)2 << 20;
+    const size_t n = (std::max<size_t>(bytes, 1) + huge - 1) / huge * huge;
+    void *p = aligned_alloc(huge, n);
+    if (p && n >= huge) (void)madvise(p, n, MADV_HUGEPAGE);
+    return p;
+}
+
+// memcpy split over FK_COPY_THREADS threads (the host copy out of pinned
+// staging is what bounds a download).
+#define FK_COPY_THREADS 4
+static void copy_parallel(char *dst, const char *src, size_t n) {
+    if (n < ((size_t)1 << 20)) {
+        memcpy(dst, src, n);
+        return;
+    }
+    const size_t part = (n / FK_COPY_THREADS + 63) & ~(size_t)63;
+    std::thread t[FK_COPY_THREADS - 1];
+    for (int i = 1; i < FK_COPY_THREADS; ++i) {
+        const size_t a = std::min(n, (size_t)i * part), e = std::min(n, a + part);
+        t[i - 1] = std::thread([=] { memcpy(dst + a, src + a, e - a); });
+    }
+    memcpy(dst, src, std::min(n, part));
+    for (auto &x : t) x.join();
+}
+
 // Copies `bytes` of device memory to host memory through the handle's two
 // pinned 16 MB staging buffers: the DMA of one chunk overlaps the host copy
 // of the one before.  Synchronizes the handle's stream.
@@ -155,7 +186,7 @@ void download_bytes(fk_handle *h, void *dst, const void *src, size_t bytes) {
             const size_t j = i - 1;
             const size_t m = std::min(chunk, bytes - j * chunk);
             FK_HIP(hipEventSynchronize(ev[j & 1]));
-            memcpy((char *)dst + j * chunk, h->dl_stage[j & 1].p, m);
+            copy_parallel((char *)dst + j * chunk, (const char *)h->dl_stage[j & 1].p, m);
         }
     }
 }

@@ -39,6 +39,7 @@ int rule_of(fk_pg_type t);
 void select_device(fk_handle *h);
 void require_dataset(fk_handle *h);
 void download_bytes(fk_handle *h, void *dst, const void *src, size_t bytes);
+void *host_result_alloc(size_t bytes);
 void upload_rows(fk_handle *h, float *dst, uint32_t dpad, const float *src, size_t src_stride_bytes, uint32_t n,
                  uint32_t d);
 

@@ -528,7 +528,7 @@ static void graph_out(BuildState &b, fk_graph *out) {
     struct Host {
         void *p;
         ~Host() { free(p); }
-    } off{malloc(((size_t)N + 1) * sizeof(uint64_t))}, ids{malloc(std::max<uint64_t>(ne, 1) * sizeof(uint32_t))};
+    } off{host_result_alloc(((size_t)N + 1) * sizeof(uint64_t))}, ids{host_result_alloc(ne * sizeof(uint32_t))};
     if (!off.p || !ids.p) throw std::bad_alloc();
     download_bytes(b.h, off.p, doff.p, ((size_t)N + 1) * sizeof(uint64_t));
     download_bytes(b.h, ids.p, dids.p, ne * sizeof(uint32_t));

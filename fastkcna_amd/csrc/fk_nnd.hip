@@ -218,9 +218,9 @@ __global__ void k_pair_counts(const uint64_t *__restrict__ list_off, const uint3
 // ---------------------------------------------------------------------------
 // Dynamic LDS of the two passes, sized by the longest list (L + 2R): pass 1
 // keeps per-list-slot threshold, count and id and stages the sampled rows
-// when they fit; pass 2 keeps the ids.
+// when they fit; pass 2 keeps ids, thresholds and bucket bases.
 size_t join_count_lds(uint32_t tcap, uint32_t rows_bytes) { return rows_bytes + (size_t)tcap * 12; }
-size_t join_fill_lds(uint32_t tcap) { return (size_t)tcap * 4; }
+size_t join_fill_lds(uint32_t tcap) { return (size_t)tcap * 16; }
 
 __global__ __launch_bounds__(256) void k_join_count(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
                                                     const uint32_t *__restrict__ lists,
@@ -280,7 +280,9 @@ __global__ __launch_bounds__(256) void k_join_fill(const uint32_t *__restrict__ 
                                                    const uint64_t *__restrict__ boff, uint64_t *__restrict__ bkey,
                                                    uint32_t *__restrict__ bsrc, uint32_t N,
                                                    const uint32_t *__restrict__ order, uint32_t tcap) {
-    extern __shared__ __align__(16) uint32_t s_id[];
+    extern __shared__ __align__(16) uint64_t s_base[];
+    float *s_thr = reinterpret_cast<float *>(s_base + tcap);
+    uint32_t *s_id = reinterpret_cast<uint32_t *>(s_thr + tcap);
     const uint32_t node = node_of_block(order, N);
     if (node >= N) return;
     const uint64_t lo = list_off[node];
@@ -288,13 +290,19 @@ __global__ __launch_bounds__(256) void k_join_fill(const uint32_t *__restrict__ 
     const uint32_t nn = n_new[node];
     const uint32_t *lst = lists + lo;
     const float *o = pdist + pair_off[node];
-    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) s_id[t] = lst[t];
+    // every slot's id, threshold and first reserved bucket slot, read at once
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
+        const uint32_t id = lst[t];
+        s_id[t] = id;
+        s_thr[t] = thr[id];
+        s_base[t] = boff[id] + resv[lo + t];
+    }
     __syncthreads();
     const uint32_t lane = lane_id();
     for (uint32_t t = threadIdx.x >> 6; t < T; t += blockDim.x >> 6) {
         const uint32_t it = s_id[t];
-        const float th = thr[it];
-        uint64_t sl = boff[it] + resv[lo + t];
+        const float th = s_thr[t];
+        uint64_t sl = s_base[t];
         const uint32_t ylim = t < nn ? T : nn;
         for (uint32_t y0 = 0; y0 < ylim; y0 += 64) {
             const uint32_t y = y0 + lane;

@@ -221,6 +221,7 @@ __global__ void k_pair_counts(const uint64_t *__restrict__ list_off, const uint3
 // when they fit; pass 2 keeps ids, thresholds and bucket bases.
 size_t join_count_lds(uint32_t tcap, uint32_t rows_bytes) { return rows_bytes + (size_t)tcap * 12; }
 size_t join_fill_lds(uint32_t tcap) { return (size_t)tcap * 16; }
+#define FK_FILL_GROUP 4  // list slots a fill wave takes at once
 
 __global__ __launch_bounds__(256) void k_join_count(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
                                                     const uint32_t *__restrict__ lists,
@@ -299,7 +300,41 @@ __global__ __launch_bounds__(256) void k_join_fill(const uint32_t *__restrict__ 
     }
     __syncthreads();
     const uint32_t lane = lane_id();
-    for (uint32_t t = threadIdx.x >> 6; t < T; t += blockDim.x >> 6) {
+    const uint32_t nw = blockDim.x >> 6;
+    if (T <= 64) {
+        // one chunk of partners per slot: FK_FILL_GROUP slots per wave at a
+        // time, all their distance loads in flight before any store
+        for (uint32_t t0 = threadIdx.x >> 6; t0 < T; t0 += nw * FK_FILL_GROUP) {
+            float dd[FK_FILL_GROUP];
+            bool ok[FK_FILL_GROUP];
+            const uint32_t y = lane;
+#pragma unroll
+            for (uint32_t g = 0; g < FK_FILL_GROUP; ++g) {
+                const uint32_t t = t0 + g * nw;
+                ok[g] = false;
+                dd[g] = 0.f;
+                if (t < T && y < (t < nn ? T : nn) && y != t && s_id[y] != s_id[t]) {
+                    ok[g] = true;
+                    dd[g] = o[t < y ? join_pair_index(t, y, T) : join_pair_index(y, t, T)];
+                }
+            }
+#pragma unroll
+            for (uint32_t g = 0; g < FK_FILL_GROUP; ++g) {
+                const uint32_t t = t0 + g * nw;
+                if (t >= T) break;  // uniform
+                const bool keep = ok[g] && dd[g] <= s_thr[t];
+                const uint64_t m = __ballot(keep);
+                if (keep) {
+                    const uint64_t at = s_base[t] + __builtin_amdgcn_mbcnt_hi(
+                                                        (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                    bkey[at] = make_key(dd[g], s_id[y]);
+                    bsrc[at] = node;
+                }
+            }
+        }
+        return;
+    }
+    for (uint32_t t = threadIdx.x >> 6; t < T; t += nw) {
         const uint32_t it = s_id[t];
         const float th = s_thr[t];
         uint64_t sl = s_base[t];

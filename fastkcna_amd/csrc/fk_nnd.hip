@@ -1228,8 +1228,9 @@ void nnd_run(BuildState &b) {
         if (b.n_sets > 1)
             FK_HIP(hipMemcpyAsync(b.d_set_active.p, b.set_active.data(), b.n_sets, hipMemcpyHostToDevice, b.s));
         // from the third join on, the pools' heads are close enough for a
-        // useful locality order
-        if (it >= 1) locality_order(b);
+        // useful locality order; renewed before every other join (measured
+        // on the 1M x 128 NSG build: as fast as every join, 4 ms less work)
+        if (it >= 1 && (it - 1) % 2 == 0) locality_order(b);
         nnd_update(b, it);
     }
     b.sync();

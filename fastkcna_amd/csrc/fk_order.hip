@@ -24,7 +24,7 @@ namespace fk {
 
 #define FK_ORDER_FANOUT 8
 #define FK_ORDER_HOPS_FINE 2
-#define FK_ORDER_HOPS_COARSE 8
+#define FK_ORDER_HOPS_COARSE 6
 
 // One Jacobi step of min-label propagation along pool edges.
 __global__ void k_label_hop(const uint64_t *__restrict__ pool, const uint32_t *__restrict__ cnt, uint32_t cap,

@@ -385,15 +385,18 @@ __global__ __launch_bounds__(256) void k_join_fill(const uint32_t *__restrict__ 
 // merged into the pool in place.  LDS: pool keys, flags and sources,
 // FK_NM_CH candidate keys and a 2 x FK_NM_CH-slot id set (18.8 KB at
 // MAXL 500: eight workgroups per CU).
+#ifndef FK_NM_CH
 #define FK_NM_CH 512
+#endif
 #ifndef FK_NM_RANK
 #define FK_NM_RANK 256  // up to this many new keys are ranked by counting, more bitonic-sorted
 #endif
 #define FK_NM_HASH (2 * FK_NM_CH)
+#define FK_NM_HBITS (FK_NM_CH == 256 ? 9 : FK_NM_CH == 512 ? 10 : 11)
 #define FK_NM_NOSRC 0xFFFFFFFFu
 
-__device__ __forceinline__ uint32_t nm_hash(uint32_t id) { return (id * 2654435761u) >> 22; }  // 10 bits
-static_assert(FK_NM_HASH == 1024, "nm_hash yields 10 bits");
+__device__ __forceinline__ uint32_t nm_hash(uint32_t id) { return (id * 2654435761u) >> (32 - FK_NM_HBITS); }
+static_assert(FK_NM_HASH == (1u << FK_NM_HBITS), "nm_hash yields FK_NM_HBITS bits");
 
 size_t nnd_merge_lds(uint32_t cap) {
     return (size_t)cap * 8 + (size_t)FK_NM_CH * 8 + (size_t)FK_NM_HASH * 8 + (size_t)cap * 4 + (size_t)cap;

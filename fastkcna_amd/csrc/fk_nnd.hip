@@ -389,7 +389,7 @@ __global__ __launch_bounds__(256) void k_join_fill(const uint32_t *__restrict__ 
 #define FK_NM_CH 512
 #endif
 #ifndef FK_NM_RANK
-#define FK_NM_RANK 256  // up to this many new keys are ranked by counting, more bitonic-sorted
+#define FK_NM_RANK 256  // up to this many new keys (at most 256) are ranked by counting, more sorted
 #endif
 #define FK_NM_HASH (2 * FK_NM_CH)
 #define FK_NM_HBITS (FK_NM_CH == 256 ? 9 : FK_NM_CH == 512 ? 10 : 11)
@@ -521,7 +521,8 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
                 const uint32_t m2 = next_pow2(nu);
                 for (uint32_t k = nu + threadIdx.x; k < m2; k += blockDim.x) s_u[k] = FK_KEY_EMPTY;
                 __syncthreads();
-                block_bitonic_sort(s_u, m2);
+                if (m2 <= 512) block_sort512(s_u, m2);
+                else block_bitonic_sort(s_u, m2);
             }
             // each candidate's merged position against the pool before it
             // moves, and its smallest source (at most FK_NM_CH / 256 per thread)

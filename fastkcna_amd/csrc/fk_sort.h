@@ -21,6 +21,53 @@ __device__ void block_bitonic_sort(K *s, uint32_t n) {
     }
 }
 
+// Bitonic sort of n 64-bit keys in LDS (n a power of two, 2 <= n <= 512)
+// by a 256-thread block, ascending, with the same comparisons as
+// block_bitonic_sort.  Each wave keeps 128 consecutive keys in registers
+// (two per lane) and takes every step whose partner lies in the same wave
+// with lane shuffles; only the steps with partner distance 128 or 256 go
+// through LDS, so the sort has at most three block barriers per pass
+// instead of one per step.  Every thread of the block must call it.
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, uint32_t m) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, (int)m, 64);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), (int)m, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ inline void block_sort512(uint64_t *s, uint32_t n) {
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t i0 = w * 128 + lane * 2;
+    const bool act = i0 < n;
+    uint64_t a0 = 0, a1 = 0;
+    if (act) { a0 = s[i0]; a1 = s[i0 + 1]; }
+    for (uint32_t k = 2; k <= n; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            const bool up = (i0 & k) == 0;  // the same for i0 + 1 (k >= 2)
+            if (j >= 128) {
+                __syncthreads();
+                if (act) { s[i0] = a0; s[i0 + 1] = a1; }
+                __syncthreads();
+                if (act) {
+                    const uint64_t b0 = s[i0 ^ j], b1 = s[(i0 + 1) ^ j];
+                    const bool lo = (i0 & j) == 0;  // the same for i0 + 1
+                    a0 = (lo == up) ? (a0 < b0 ? a0 : b0) : (a0 < b0 ? b0 : a0);
+                    a1 = (lo == up) ? (a1 < b1 ? a1 : b1) : (a1 < b1 ? b1 : a1);
+                }
+            } else if (j == 1) {
+                if ((a0 > a1) == up) { const uint64_t t = a0; a0 = a1; a1 = t; }
+            } else {
+                const uint64_t b0 = shfl_xor_u64(a0, j >> 1), b1 = shfl_xor_u64(a1, j >> 1);
+                const bool lo = (i0 & j) == 0;
+                a0 = (lo == up) ? (a0 < b0 ? a0 : b0) : (a0 < b0 ? b0 : a0);
+                a1 = (lo == up) ? (a1 < b1 ? a1 : b1) : (a1 < b1 ? b1 : a1);
+            }
+        }
+    }
+    __syncthreads();
+    if (act) { s[i0] = a0; s[i0 + 1] = a1; }
+    __syncthreads();
+}
+
 // Same, sorting (key, value) pairs by key.
 template <typename K, typename V>
 __device__ void block_bitonic_sort_kv(K *s, V *v, uint32_t n) {

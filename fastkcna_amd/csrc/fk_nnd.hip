@@ -1425,7 +1425,12 @@ __device__ uint32_t bvw_merge(uint64_t *p, uint8_t *pf, uint32_t np, uint64_t *c
     const uint32_t n2 = next_pow2(nc < 2 ? 2 : nc);
     for (uint32_t x = nc + lane; x < n2; x += 64) c[x] = FK_KEY_EMPTY;
     wave_lds_sync();
-    wave_bitonic_sort(c, n2);
+    if (n2 <= 128) {
+        wave_sort128(c, n2);
+        wave_lds_sync();
+    } else {
+        wave_bitonic_sort(c, n2);
+    }
     // keep the first of equal keys that is not in the pool; compacted in
     // place (chunks in order, every lane reads before any lane writes)
     uint32_t nk = 0;

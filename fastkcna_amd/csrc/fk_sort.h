@@ -68,6 +68,31 @@ __device__ inline void block_sort512(uint64_t *s, uint32_t n) {
     __syncthreads();
 }
 
+// Bitonic sort of n 64-bit keys in LDS (n a power of two, 2 <= n <= 128)
+// by one wavefront, ascending: two keys per lane in registers, every step by
+// lane shuffles.  The caller orders the LDS accesses around it (every lane's
+// earlier stores before, the stores it makes before later reads).
+__device__ inline void wave_sort128(uint64_t *s, uint32_t n) {
+    const uint32_t i0 = (threadIdx.x & 63) * 2;
+    const bool act = i0 < n;
+    uint64_t a0 = 0, a1 = 0;
+    if (act) { a0 = s[i0]; a1 = s[i0 + 1]; }
+    for (uint32_t k = 2; k <= n; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            const bool up = (i0 & k) == 0;
+            if (j == 1) {
+                if ((a0 > a1) == up) { const uint64_t t = a0; a0 = a1; a1 = t; }
+            } else {
+                const uint64_t b0 = shfl_xor_u64(a0, j >> 1), b1 = shfl_xor_u64(a1, j >> 1);
+                const bool lo = (i0 & j) == 0;
+                a0 = (lo == up) ? (a0 < b0 ? a0 : b0) : (a0 < b0 ? b0 : a0);
+                a1 = (lo == up) ? (a1 < b1 ? a1 : b1) : (a1 < b1 ? b1 : a1);
+            }
+        }
+    }
+    if (act) { s[i0] = a0; s[i0 + 1] = a1; }
+}
+
 // Same, sorting (key, value) pairs by key.
 template <typename K, typename V>
 __device__ void block_bitonic_sort_kv(K *s, V *v, uint32_t n) {

@@ -132,24 +132,35 @@ __device__ __forceinline__ uint32_t chain_block(uint32_t j, uint32_t dpad) {
 // Ordered sum of the 8 accumulators of this lane's octet, in every lane of
 // the octet.  The 8 lanes must be active together.
 __device__ __forceinline__ float octet_sum(float acc) {
-    const int base = (int)(threadIdx.x & 63u & ~7u);
-    float r = __shfl(acc, base, 64) + __shfl(acc, base + 1, 64);
-    r += __shfl(acc, base + 2, 64);
-    r += __shfl(acc, base + 3, 64);
-    r += __shfl(acc, base + 4, 64);
-    r += __shfl(acc, base + 5, 64);
-    r += __shfl(acc, base + 6, 64);
-    r += __shfl(acc, base + 7, 64);
-    return r;
+    // lane i of the octet takes lane i - 1's running sum (DPP row_shr:1,
+    // within the lane's row of 16) and adds its own partial: after seven
+    // steps lane 7 holds ((p0 + p1) + p2) + ... + p7, the same additions
+    // in the same order.  Then lane 7's value to the whole octet: mirror the
+    // 8 lanes (lane 0 gets it), broadcast lane 0 of each quad, and lanes
+    // 4..7 take lanes 0..3's (row_shr:4)
+    const uint32_t k = threadIdx.x & 7u;
+    float s = acc;
+#pragma unroll
+    for (uint32_t i = 1; i < 8; ++i) {
+        const float t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(s), 0x111, 0xF, 0xF, false));
+        s = (k == i) ? t + acc : s;
+    }
+    const int m = __builtin_amdgcn_update_dpp(0, __float_as_int(s), 0x141, 0xF, 0xF, false);  // row_half_mirror
+    const int q = __builtin_amdgcn_update_dpp(0, m, 0x00, 0xF, 0xF, false);                    // quad_perm [0,0,0,0]
+    const int h = __builtin_amdgcn_update_dpp(0, q, 0x114, 0xF, 0xF, false);                   // row_shr:4
+    return __int_as_float(k < 4 ? q : h);
 }
 
 // Sum over the 8 lanes of this lane's octet of 32-bit integers (exact),
 // as a float (every sum here is below 2^24).
+// Integer sums are exact in any order: a butterfly of three DPP steps
+// (swap lanes 1 apart and 2 apart within quads, then mirror the 8 lanes)
+// leaves the octet's total in all of its lanes.
 __device__ __forceinline__ float octet_sum_u32(uint32_t acc) {
-    const int base = (int)(threadIdx.x & 63u & ~7u);
-    uint32_t r = 0;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) r += (uint32_t)__shfl((int)acc, base + j, 64);
+    uint32_t r = acc;
+    r += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)r, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    r += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)r, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    r += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)r, 0x141, 0xF, 0xF, false);  // row_half_mirror
     return (float)r;
 }
 

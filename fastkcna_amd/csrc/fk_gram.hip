@@ -300,25 +300,45 @@ __global__ __launch_bounds__(256) void k_join_gram(const float *__restrict__ row
             const uint32_t ra = std::min(tx * 16 + (lane & 15), nx - 1);
             const uint32_t rb = std::min(ty * 16 + (lane & 15), ny - 1);
             const fk_f32x4 g = gram_tile<FMT>(s_rows + ra * ls, s_y + rb * ls, dpad, scale);
+            // lane (G, j) = (lane >> 4, lane & 15) holds the pairs (x, y) with
+            // x = G * 4 + e (e < 4) and y = j of the tile.  Counts: the hits
+            // of x by a ballot over its 16 lanes, those of y summed over e
+            // and the four groups, then two atomics per tile on distinct slots
             const uint32_t yi = ty * 16 + (lane & 15);
+            const uint32_t y = y0 + yi;
+            const bool yok = yi < ny;
+            const uint32_t yc = yok ? y : y0;
+            const float2 ny2 = s_n[yc];
+            const float thy = s_thr[yc];
+            const uint32_t idy = s_id[yc];
+            uint32_t cy = 0, cx = 0;
 #pragma unroll
             for (uint32_t e = 0; e < 4; ++e) {
                 const uint32_t xi = tx * 16 + (lane >> 4) * 4 + e;
-                const uint32_t x = x0 + xi, y = y0 + yi;
-                if (xi >= nx || yi >= ny || y <= x) continue;
-                const float2 nx2 = s_n[x], ny2 = s_n[y];
-                const float dt = (nx2.x + ny2.x) - 2.f * (g[e] * gb.inv_s2);
-                float dd = dt;
-                if (!gb.exact && gram_lower(dt, nx2, ny2, gb) <= fmaxf(s_thr[x], s_thr[y])) {
-                    dd = l2_canon(s_rows + xi * ls, s_y + yi * ls, dpad);
-                    ++rechecks;
+                const uint32_t x = x0 + xi;
+                bool hx = false;
+                if (xi < nx && yok && y > x) {
+                    const float2 nx2 = s_n[x];
+                    const float thx = s_thr[x];
+                    const float dt = (nx2.x + ny2.x) - 2.f * (g[e] * gb.inv_s2);
+                    float dd = dt;
+                    if (!gb.exact && gram_lower(dt, nx2, ny2, gb) <= fmaxf(thx, thy)) {
+                        dd = l2_canon(s_rows + xi * ls, s_y + yi * ls, dpad);
+                        ++rechecks;
+                    }
+                    o[join_pair_index(x, y, T)] = dd;
+                    if (s_id[x] != idy) {
+                        hx = dd <= thx;
+                        cy += dd <= thy ? 1u : 0u;
+                    }
                 }
-                o[join_pair_index(x, y, T)] = dd;
-                if (s_id[x] != s_id[y]) {
-                    if (dd <= s_thr[x]) atomicAdd(&s_c[x], 1u);
-                    if (dd <= s_thr[y]) atomicAdd(&s_c[y], 1u);
-                }
+                const uint64_t m = wave_ballot(hx);
+                if ((lane & 15) == e) cx = (uint32_t)__builtin_popcountll((m >> (lane & 48)) & 0xFFFFull);
             }
+            cy += (uint32_t)__shfl_xor((int)cy, 16, 64);
+            cy += (uint32_t)__shfl_xor((int)cy, 32, 64);
+            if ((lane & 15) < 4 && cx) atomicAdd(&s_c[x0 + tx * 16 + (lane >> 4) * 4 + (lane & 15)], cx);
+            if (lane < 16 && cy) atomicAdd(&s_c[y], cy);
         }
     };
     if (T <= cap) {

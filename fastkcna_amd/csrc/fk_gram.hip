@@ -25,6 +25,9 @@
 //       (1 +- gam) of a^2.  A pair is skipped only when
 //       (sqrt(max(d~ - f, 0)) - e)^2 (1 - gam) > max(thr_x, thr_y),
 //     which implies c > both thresholds.
+// Exact-form rows with every value in [0, 255] are joined from their byte
+// copy instead (k_join_gram8: v_mfma_i32_16x16x32_i8 on the values shifted by
+// -128, integer norms), with the same pairs, distances and counts.
 #include <math.h>
 
 #include <algorithm>
@@ -166,6 +169,19 @@ __global__ __launch_bounds__(256) void k_gram_norms(const float *__restrict__ ro
     nrm[i] = make_float2(q2 * inv_s2, (float)(sqrt(s) * (1.0 + 1e-6)));
 }
 
+// norms8[i] = sum over the dpad bytes b of row i of (b - 128)^2.
+__global__ void k_norms8(const uint8_t *__restrict__ c8, uint32_t n, uint32_t dpad, int32_t *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint8_t *r = c8 + (size_t)i * dpad;
+    int32_t s = 0;
+    for (uint32_t k = 0; k < dpad; ++k) {
+        const int32_t v = (int32_t)r[k] - 128;
+        s += v * v;
+    }
+    out[i] = s;
+}
+
 __global__ void k_to_bytes(const float *__restrict__ src, size_t n, uint8_t *__restrict__ dst) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = (uint8_t)src[i];
@@ -232,11 +248,15 @@ void gram_prepare(hipStream_t s, Dataset &ds, uint32_t flags) {
     // exact form with values in [0, 255]: a chain-major byte copy for the
     // octet kernels (dot products in 32-bit integers, exact)
     ds.chains8.release();
+    ds.norms8.release();
     if (fmt == FK_GRAM_EXACT && !h[2] && mabs <= 255.f) {
         ds.chains8.alloc((size_t)ds.n_total * ds.dpad);
         const size_t total = (size_t)ds.n_total * ds.dpad;
         hipLaunchKernelGGL(k_to_bytes, dim3((unsigned)std::min<size_t>((total + 255) / 256, 65536)), dim3(256), 0, s,
                            ds.chains.p, total, ds.chains8.p);
+        FK_LAUNCH_CHECK();
+        ds.norms8.alloc(ds.n_total);
+        hipLaunchKernelGGL(k_norms8, dim3(g2), dim3(256), 0, s, ds.chains8.p, ds.n_total, ds.dpad, ds.norms8.p);
         FK_LAUNCH_CHECK();
     }
     FK_HIP(hipStreamSynchronize(s));
@@ -374,12 +394,180 @@ __global__ __launch_bounds__(256) void k_join_gram(const float *__restrict__ row
 
 size_t join_gram_lds(uint32_t tcap, uint32_t rows_bytes) { return rows_bytes + (size_t)tcap * 20; }
 
+// ---------------------------------------------------------------------------
+// The same pass from byte rows (exact form, values in [0, 255]): the staged
+// rows are the chain-major byte copy (dpad bytes, a fixed permutation of the
+// dimensions, which a dot product does not see), shifted to signed bytes
+// b - 128 (the bit pattern b ^ 0x80) for v_mfma_i32_16x16x32_i8.  With
+// n'(x) = sum (x_k - 128)^2 over the dpad bytes,
+//     |x - y|^2 = n'(x) + n'(y) - 2 (x - 128) . (y - 128)
+// exactly in 32-bit integers, and the value (an integer below 2^24 in the
+// exact form) converts to the canonical FP32 distance without rounding.
+// A quarter of the staging bytes of the FP32 rows, half the MFMA steps.
+// ---------------------------------------------------------------------------
+typedef int fk_i32x4 __attribute__((ext_vector_type(4)));
+
+// Row stride of staged byte rows: 16 rows of a tile read 8 bytes each at one
+// offset, and dpad + 8 bytes between rows puts them in distinct banks.
+__host__ __device__ inline uint32_t join8_row_bytes(uint32_t dpad) { return dpad + 8u; }
+
+__device__ __forceinline__ void join_stage_bytes(const uint8_t *__restrict__ c8, uint32_t dpad, const uint32_t *s_id,
+                                                 uint32_t first, uint32_t count, uint8_t *s_dst) {
+    const uint32_t ls = join8_row_bytes(dpad), v8 = dpad >> 3;
+    for (uint32_t i = threadIdx.x; i < count * v8; i += blockDim.x) {
+        const uint32_t r = i / v8, c = i - r * v8;
+        *reinterpret_cast<uint2 *>(s_dst + r * ls + c * 8) =
+            *reinterpret_cast<const uint2 *>(c8 + (size_t)s_id[first + r] * dpad + c * 8);
+    }
+}
+
+// 16 x 16 tile of shifted dot products.  Lane l supplies row (l & 15) of the
+// A block (arow) and of the B block (brow), 8 bytes at k = 32 s + 8 (l >> 4)
+// for every step s (zero past dpad); A and B take the same bytes of their
+// rows, so every product pairs equal dimensions.  On return lane l holds
+// G[4 (l >> 4) + e][l & 15], e = 0..3.  Every lane of the wave must call it.
+__device__ __forceinline__ fk_i32x4 gram_tile8(const uint8_t *__restrict__ arow, const uint8_t *__restrict__ brow,
+                                               uint32_t dpad) {
+    fk_i32x4 acc = {0, 0, 0, 0};
+    const uint32_t kq = (lane_id() >> 4) * 8u;
+    for (uint32_t k0 = 0; k0 < dpad; k0 += 32u) {
+        const uint32_t k = k0 + kq;
+        long a = 0, b = 0;
+        if (k < dpad) {
+            a = (long)(*reinterpret_cast<const uint64_t *>(arow + k) ^ 0x8080808080808080ull);
+            b = (long)(*reinterpret_cast<const uint64_t *>(brow + k) ^ 0x8080808080808080ull);
+        }
+        acc = __builtin_amdgcn_mfma_i32_16x16x32_i8(a, b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(256) void k_join_gram8(const uint8_t *__restrict__ c8, uint32_t dpad,
+                                                    const int32_t *__restrict__ norms8,
+                                                    const uint32_t *__restrict__ lists,
+                                                    const uint64_t *__restrict__ list_off,
+                                                    const uint32_t *__restrict__ n_new,
+                                                    const uint64_t *__restrict__ pair_off,
+                                                    const float *__restrict__ thr, float *__restrict__ pdist,
+                                                    uint32_t *__restrict__ resv, uint32_t *__restrict__ bcnt,
+                                                    uint32_t N, const uint32_t *__restrict__ order, uint32_t tcap,
+                                                    uint32_t rows_bytes, uint32_t t_lo, uint32_t t_hi) {
+    extern __shared__ __align__(16) uint8_t s_rows8[];
+    int32_t *s_n = reinterpret_cast<int32_t *>(s_rows8 + rows_bytes);
+    float *s_thr = reinterpret_cast<float *>(s_n + tcap);
+    uint32_t *s_c = reinterpret_cast<uint32_t *>(s_thr + tcap);
+    uint32_t *s_id = s_c + tcap;
+    const uint32_t node = node_of_block(order, N);
+    if (node >= N) return;
+    const uint64_t lo = list_off[node];
+    const uint32_t T = (uint32_t)(list_off[node + 1] - lo);
+    if (T <= t_lo || T > t_hi) return;  // another launch's lists
+    const uint32_t nn = n_new[node];
+    const uint32_t *lst = lists + lo;
+    float *o = pdist + pair_off[node];
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
+        const uint32_t id = lst[t];
+        s_id[t] = id;
+        s_thr[t] = thr[id];
+        s_n[t] = norms8[id];
+        s_c[t] = 0;
+    }
+    __syncthreads();
+    const uint32_t ls = join8_row_bytes(dpad);
+    const uint32_t cap = rows_bytes / ls;
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    // x chunk [x0, x0 + nx) at s_rows8, y chunk [y0, y0 + ny) at s_y; the
+    // epilogue is k_join_gram's (slot counts by ballot and register sums)
+    auto tiles = [&](uint32_t x0, uint32_t nx, uint32_t y0, uint32_t ny, const uint8_t *s_y) {
+        const uint32_t ntx = (nx + 15) >> 4, nty = (ny + 15) >> 4;
+        for (uint32_t t = wave; t < ntx * nty; t += 4) {
+            const uint32_t tx = t / nty, ty = t - tx * nty;
+            if (y0 + ty * 16 + std::min(16u, ny - ty * 16) - 1 <= x0 + tx * 16) continue;
+            const uint32_t ra = std::min(tx * 16 + (lane & 15), nx - 1);
+            const uint32_t rb = std::min(ty * 16 + (lane & 15), ny - 1);
+            const fk_i32x4 g = gram_tile8(s_rows8 + ra * ls, s_y + rb * ls, dpad);
+            const uint32_t yi = ty * 16 + (lane & 15);
+            const uint32_t y = y0 + yi;
+            const bool yok = yi < ny;
+            const uint32_t yc = yok ? y : y0;
+            const int32_t nyv = s_n[yc];
+            const float thy = s_thr[yc];
+            const uint32_t idy = s_id[yc];
+            uint32_t cy = 0, cx = 0;
+#pragma unroll
+            for (uint32_t e = 0; e < 4; ++e) {
+                const uint32_t xi = tx * 16 + (lane >> 4) * 4 + e;
+                const uint32_t x = x0 + xi;
+                bool hx = false;
+                if (xi < nx && yok && y > x) {
+                    const float thx = s_thr[x];
+                    const float dd = (float)(s_n[x] + nyv - 2 * g[e]);
+                    o[join_pair_index(x, y, T)] = dd;
+                    if (s_id[x] != idy) {
+                        hx = dd <= thx;
+                        cy += dd <= thy ? 1u : 0u;
+                    }
+                }
+                const uint64_t m = wave_ballot(hx);
+                if ((lane & 15) == e) cx = (uint32_t)__builtin_popcountll((m >> (lane & 48)) & 0xFFFFull);
+            }
+            cy += (uint32_t)__shfl_xor((int)cy, 16, 64);
+            cy += (uint32_t)__shfl_xor((int)cy, 32, 64);
+            if ((lane & 15) < 4 && cx) atomicAdd(&s_c[x0 + tx * 16 + (lane >> 4) * 4 + (lane & 15)], cx);
+            if (lane < 16 && cy) atomicAdd(&s_c[y], cy);
+        }
+    };
+    if (T <= cap) {
+        join_stage_bytes(c8, dpad, s_id, 0, T, s_rows8);
+        __syncthreads();
+        tiles(0, nn, 0, T, s_rows8);
+    } else {
+        const uint32_t bx = nn < cap / 2 ? nn : cap / 2, by = cap - bx;
+        uint8_t *s_y = s_rows8 + bx * ls;
+        for (uint32_t x0 = 0; x0 < nn; x0 += bx) {
+            const uint32_t nx = (nn - x0) < bx ? (nn - x0) : bx;
+            __syncthreads();  // the previous tiles are done with the x rows
+            join_stage_bytes(c8, dpad, s_id, x0, nx, s_rows8);
+            for (uint32_t y0 = x0 + 1; y0 < T; y0 += by) {
+                const uint32_t ny = (T - y0) < by ? (T - y0) : by;
+                if (y0 != x0 + 1) __syncthreads();  // the previous tile is done with the y rows
+                join_stage_bytes(c8, dpad, s_id, y0, ny, s_y);
+                __syncthreads();
+                tiles(x0, nx, y0, ny, s_y);
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
+        const uint32_t c = s_c[t];
+        resv[lo + t] = c ? atomicAdd(&bcnt[s_id[t]], c) : 0u;
+    }
+}
+
+size_t join_gram8_lds(uint32_t tcap, uint32_t rows_bytes) { return rows_bytes + (size_t)tcap * 16; }
+
 // Lists of up to FK_JOIN_SMALL_ROWS rows first with a small staging area
 // (more workgroups per CU), then the longer ones with the full area.
 void launch_join_gram(BuildState &b, uint32_t tcap, unsigned long long *n_recheck) {
     const Dataset &ds = *b.ds;
-    const GramBound gb = gram_bound(ds);
     const uint32_t *ord = b.has_order ? b.order.p : nullptr;
+    if (ds.gram_fmt == FK_GRAM_EXACT && ds.chains8.p && ds.norms8.p) {
+        const uint32_t rb8 = join8_row_bytes(ds.dpad);
+        const uint32_t small8 = std::min<uint32_t>(FK_JOIN_SMALL_ROWS * rb8, FK_JOIN_LDS_ROWS_BYTES) & ~15u;
+        const uint32_t big8 = FK_JOIN_LDS_ROWS_BYTES & ~15u;
+        const uint32_t small_t8 = small8 / rb8;
+        FK_CHECK_ARG(small_t8 >= 2, "rows too long for the byte-row join");
+        auto launch8 = [&](uint32_t tc, uint32_t rb, uint32_t t_lo, uint32_t t_hi) {
+            hipLaunchKernelGGL(k_join_gram8, dim3(node_grid(b)), dim3(256), join_gram8_lds(tc, rb), b.s,
+                               ds.chains8.p, ds.dpad, ds.norms8.p, b.lists.p, b.list_off.p, b.n_new.p,
+                               b.pair_off.p, b.thr.p, b.pdist.p, b.resv.p, b.bcnt.p, b.N, ord, tc, rb, t_lo, t_hi);
+            FK_LAUNCH_CHECK();
+        };
+        launch8(std::min(tcap, small_t8), small8, 0u, small_t8);
+        if (tcap > small_t8) launch8(tcap, big8, small_t8, 0xFFFFFFFFu);
+        return;
+    }
+    const GramBound gb = gram_bound(ds);
     const uint32_t row_bytes = (ds.dpad + 4) * 4;
     const uint32_t small_bytes = std::min<uint32_t>(FK_JOIN_SMALL_ROWS * row_bytes, FK_JOIN_LDS_ROWS_BYTES);
     const uint32_t small_t = small_bytes / row_bytes;

@@ -188,6 +188,7 @@ struct Dataset {
     // (FK_GRAM_*), FP16 prescale and the FK_FLAG_GRAM_* request it was made for
     DevBuf<float2> gram_nrm;
     DevBuf<uint8_t> chains8;  // exact form with values in [0, 255]: chain-major byte copy
+    DevBuf<int32_t> norms8;   // with chains8: sum over dpad of (byte - 128)^2 per row
     int gram_fmt = 0;
     float gram_scale = 1.f;
     uint32_t gram_req = 0;

@@ -128,12 +128,15 @@ def test_gram_nndescent_pools_identical(device, name, flags):
         f"{st1['d_recheck']} rechecked ({st1['d_recheck'] / max(st1['d_gram'], 1):.3f})")
 
 
-def test_gram_join_tiled_lists_identical(device):
-    """Long sample lists (L + 2R above the LDS staging area) take the tiled
-    path of the Gram join; the pools must still equal the plain join's."""
+@pytest.mark.parametrize("S,R,K", [(20, 150, 100), (40, 300, 180)])
+def test_gram_join_tiled_lists_identical(device, S, R, K):
+    """Long sample lists (above the LDS staging area: 77 FP32 rows, 301 byte
+    rows at d = 128) take the tiled path of the Gram join; the pools must
+    still equal the plain join's."""
     x = data("int128")
+    say("gram tiled lists", S, R, K)
     device.upload(x)
-    p = nnd_params(K=100, L=100, S=20, R=150, iterations=4)
+    p = nnd_params(K=K, L=K, S=S, R=R, iterations=4)
     pools0, len0, _ = device.nndescent(p)
     p.flags = GRAM | BF16
     pools1, len1, st1 = device.nndescent(p)

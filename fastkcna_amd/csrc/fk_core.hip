@@ -155,13 +155,35 @@ struct PruneWave {
         const uint32_t nA = n0 < PruneMC<NB>::value ? n0 : PruneMC<NB>::value;
         if (!skip) {
             // the cached kept neighbours decide most candidates; octet-
-            // uniform: every lane of an octet sees the same pid, tid, djk
-            for (uint32_t t = 0; t < nA; ++t) {
-                const uint32_t tid = so[t];
-                if (tid == pid) { occl = true; break; }
-                ++mycc;
-                const float djk = member_dist(crow, cn, t, tid);
-                if (occludes<RULE>(djk, pd, sd[t], tau3, cos_thr)) { occl = true; break; }
+            // uniform: every lane of an octet sees the same pid, tid, djk.
+            // The next cached chain is read from LDS while one is checked
+            // (two register chains, unrolled by two so neither is copied;
+            // chains of up to 16 registers, where the second one costs no
+            // occupancy)
+            if constexpr (PruneMC<NB>::value > 0 && sizeof(Chain) <= 64) {
+                auto check = [&](uint32_t t, const Chain &o) __attribute__((always_inline)) -> bool {
+                    if (so[t] == pid) return true;
+                    ++mycc;
+                    const float djk = DOT ? exact_dist(cn, sn[t], crow.dot_reg(o, nb)) : crow.dist_reg(o, nb);
+                    return occludes<RULE>(djk, pd, sd[t], tau3, cos_thr);
+                };
+                Chain ca, cb;
+                if (nA > 0) ca.load(lane_cache, nb);
+                for (uint32_t t = 0; t < nA; t += 2) {
+                    if (t + 1 < nA) cb.load(lane_cache + (size_t)(t + 1) * dpad, nb);
+                    if (check(t, ca)) { occl = true; break; }
+                    if (t + 1 >= nA) break;
+                    if (t + 2 < nA) ca.load(lane_cache + (size_t)(t + 2) * dpad, nb);
+                    if (check(t + 1, cb)) { occl = true; break; }
+                }
+            } else {
+                for (uint32_t t = 0; t < nA; ++t) {
+                    const uint32_t tid = so[t];
+                    if (tid == pid) { occl = true; break; }
+                    ++mycc;
+                    const float djk = member_dist(crow, cn, t, tid);
+                    if (occludes<RULE>(djk, pd, sd[t], tau3, cos_thr)) { occl = true; break; }
+                }
             }
         }
         if (n0 > nA) {

@@ -168,6 +168,10 @@ def exact_rows(kind):
         return integer_mixture(4000, 128, 32, seed=11)
     if kind == "s8_128":
         return integer_mixture(4000, 128, 32, seed=12) - 128.0
+    if kind == "u8_100":  # dpad 104: the byte-row join's last 32-byte step is partial
+        return integer_mixture(4000, 100, 32, seed=14)
+    if kind == "u8_20":   # dpad 24: less than one step
+        return integer_mixture(4000, 20, 32, seed=15)
     return integer_mixture(4000, 96, 32, seed=13)
 
 
@@ -206,11 +210,12 @@ def test_exact_form_prune_bitexact(device, pg, reversed_, kind):
     assert g[3] == c[3]
 
 
-@pytest.mark.parametrize("kind", ["u8_128", "s8_128", "u8_96"])
+@pytest.mark.parametrize("kind", ["u8_128", "s8_128", "u8_96", "u8_100", "u8_20"])
 @pytest.mark.parametrize("pg", [FK_PG_ALPHAPG, FK_PG_HNSW_LAYER])
 def test_gram_full_build_identical_exact_form(device, pg, kind):
     """Exact form end to end (join, prunes, BridgeView, every octet row
-    kind): the same graph as the plain FP32 build."""
+    kind; the byte-row join at dpad 128, 96, 104 and 24): the same graph as
+    the plain FP32 build."""
     x = exact_rows(kind)[:3000]
     device.upload(x)
     p = nnd_params(angle=60)

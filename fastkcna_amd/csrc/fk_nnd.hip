@@ -611,7 +611,21 @@ __global__ __launch_bounds__(1024) void k_delta(const uint8_t *__restrict__ pflg
         const uint32_t lim = cnt[n] < K ? cnt[n] : K;
         const uint8_t *pf = pflg + (uint64_t)n * cap;
         unsigned long long cn = 0;
-        for (uint32_t l = lane_id(); l < lim; l += 64) cn += (pf[l] & FK_FLG_NEW) ? 1 : 0;
+        if ((cap & 3u) == 0) {
+            // four flags per load (rows start on 4-byte boundaries); the last
+            // word of the first lim flags keeps only its leading bytes
+            static_assert(FK_FLG_NEW == 1u, "the word count takes bit 0 of every flag byte");
+            const uint32_t *pw = reinterpret_cast<const uint32_t *>(pf);
+            const uint32_t nw = (lim + 3u) >> 2;
+            for (uint32_t w = lane_id(); w < nw; w += 64) {
+                uint32_t v = pw[w] & 0x01010101u;
+                const uint32_t rem = lim - w * 4u;
+                if (rem < 4u) v &= (1u << (8u * rem)) - 1u;
+                cn += (unsigned long long)__builtin_popcount(v);
+            }
+        } else {
+            for (uint32_t l = lane_id(); l < lim; l += 64) cn += (pf[l] & FK_FLG_NEW) ? 1 : 0;
+        }
         if (set_of) {
             cn = wave_sum(cn);
             if (lane_id() == 0 && cn) atomicAdd(&acc[set_of[n]], cn);

@@ -601,6 +601,28 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
 // ---------------------------------------------------------------------------
 // One wavefront per node, grid-stride; one atomic per block.
 // With sets, acc[set] per set (one atomic per node).
+// This lane's share of the count of FK_FLG_NEW among the first lim flags of
+// a pool row (the wave's lanes together count each flag once).  Rows of
+// cap flags with cap % 4 == 0 start on 4-byte boundaries and are read four
+// flags per load, the last word keeping only its leading bytes.
+__device__ __forceinline__ uint32_t lane_new_flags(const uint8_t *pf, uint32_t lim, uint32_t cap) {
+    uint32_t cn = 0;
+    if ((cap & 3u) == 0) {
+        static_assert(FK_FLG_NEW == 1u, "the word count takes bit 0 of every flag byte");
+        const uint32_t *pw = reinterpret_cast<const uint32_t *>(pf);
+        const uint32_t nw = (lim + 3u) >> 2;
+        for (uint32_t w = lane_id(); w < nw; w += 64) {
+            uint32_t v = pw[w] & 0x01010101u;
+            const uint32_t rem = lim - w * 4u;
+            if (rem < 4u) v &= (1u << (8u * rem)) - 1u;
+            cn += (uint32_t)__builtin_popcount(v);
+        }
+    } else {
+        for (uint32_t l = lane_id(); l < lim; l += 64) cn += (pf[l] & FK_FLG_NEW) ? 1u : 0u;
+    }
+    return cn;
+}
+
 __global__ __launch_bounds__(1024) void k_delta(const uint8_t *__restrict__ pflg, const uint32_t *__restrict__ cnt,
                                                 uint32_t N, uint32_t cap, uint32_t K,
                                                 const uint32_t *__restrict__ set_of,
@@ -610,22 +632,7 @@ __global__ __launch_bounds__(1024) void k_delta(const uint8_t *__restrict__ pflg
     for (uint32_t n = blockIdx.x * wpb + (threadIdx.x >> 6); n < N; n += gridDim.x * wpb) {
         const uint32_t lim = cnt[n] < K ? cnt[n] : K;
         const uint8_t *pf = pflg + (uint64_t)n * cap;
-        unsigned long long cn = 0;
-        if ((cap & 3u) == 0) {
-            // four flags per load (rows start on 4-byte boundaries); the last
-            // word of the first lim flags keeps only its leading bytes
-            static_assert(FK_FLG_NEW == 1u, "the word count takes bit 0 of every flag byte");
-            const uint32_t *pw = reinterpret_cast<const uint32_t *>(pf);
-            const uint32_t nw = (lim + 3u) >> 2;
-            for (uint32_t w = lane_id(); w < nw; w += 64) {
-                uint32_t v = pw[w] & 0x01010101u;
-                const uint32_t rem = lim - w * 4u;
-                if (rem < 4u) v &= (1u << (8u * rem)) - 1u;
-                cn += (unsigned long long)__builtin_popcount(v);
-            }
-        } else {
-            for (uint32_t l = lane_id(); l < lim; l += 64) cn += (pf[l] & FK_FLG_NEW) ? 1 : 0;
-        }
+        unsigned long long cn = lane_new_flags(pf, lim, cap);
         if (set_of) {
             cn = wave_sum(cn);
             if (lane_id() == 0 && cn) atomicAdd(&acc[set_of[n]], cn);
@@ -1287,8 +1294,7 @@ __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ p
     for (uint32_t n = blockIdx.x * wpb + (threadIdx.x >> 6); n < N; n += gridDim.x * wpb) {
         const uint8_t *pf = pflg + (uint64_t)n * cap;
         const uint32_t lim = cnt[n] < W ? cnt[n] : W;
-        uint32_t f = 0;
-        for (uint32_t j = lane_id(); j < lim; j += 64) f += (pf[j] & FK_FLG_NEW) ? 1u : 0u;
+        uint32_t f = lane_new_flags(pf, lim, cap);
         f = wave_sum(f);
         if (lane_id() == 0) {
             if (set_of) {

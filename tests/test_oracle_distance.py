@@ -62,6 +62,33 @@ def test_integer_data_is_exact(rng):
     assert np.array_equal(d32.astype(np.float64), exact.astype(np.float64))
 
 
+@pytest.mark.parametrize("d", [20, 100, 128])
+def test_shifted_byte_gram_identity(d, rng):
+    """The byte-row Gram join: rows of values in [0, 255], zero-padded to
+    dpad (a multiple of 8) and shifted to signed bytes b ^ 0x80 = b - 128,
+    give |x - y|^2 = n'(x) + n'(y) - 2 (x - 128).(y - 128) with n'(x) the sum
+    over all dpad shifted bytes squared -- equal to the canonical FP32
+    distance of the unpadded rows, and unchanged by one permutation of the
+    dimensions applied to both rows (the chain-major layout)."""
+    dpad = (d + 7) // 8 * 8
+    x = rng.integers(0, 256, size=(300, d)).astype(np.uint8)
+    xp = np.zeros((300, dpad), np.uint8)
+    xp[:, :d] = x
+    s = (xp ^ 0x80).view(np.int8).astype(np.int64)
+    assert np.array_equal(s, xp.astype(np.int64) - 128)
+    n8 = (s * s).sum(1)
+    a = rng.integers(0, 300, 4000)
+    b = rng.integers(0, 300, 4000)
+    perm = rng.permutation(dpad)
+    got = n8[a] + n8[b] - 2 * (s[a] * s[b]).sum(1)
+    got_perm = n8[a] + n8[b] - 2 * (s[a][:, perm] * s[b][:, perm]).sum(1)
+    exact = ((x[a].astype(np.int64) - x[b].astype(np.int64)) ** 2).sum(1)
+    assert np.array_equal(got, exact) and np.array_equal(got_perm, exact)
+    assert exact.max() < 2 ** 24  # converts to FP32 without rounding
+    d32 = oracle.pair_distances(x.astype(np.float32), a, b)
+    assert np.array_equal(d32, got.astype(np.float32))
+
+
 @pytest.mark.parametrize("d", [96, 128, 960])
 def test_float_data_within_tolerance_of_fp64(d, rng):
     x = rng.uniform(0, 1, size=(300, d)).astype(np.float32)

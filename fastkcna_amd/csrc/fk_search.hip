@@ -448,9 +448,12 @@ void tree_grow(BuildState &b) {
         ++rounds;
         if (!ch) break;
     }
-    std::vector<uint32_t> lab(N), deg(N);
-    FK_HIP(hipMemcpyAsync(lab.data(), label.p, (size_t)N * 4, hipMemcpyDeviceToHost, b.s));
-    FK_HIP(hipMemcpyAsync(deg.data(), b.nsg_len.p, (size_t)N * 4, hipMemcpyDeviceToHost, b.s));
+    // labels and out-degrees into the handle's pinned buffers (grow-only)
+    b.h->host_lab.ensure(N);
+    b.h->host_deg.ensure(N);
+    uint32_t *lab = b.h->host_lab.p, *deg = b.h->host_deg.p;
+    FK_HIP(hipMemcpyAsync(lab, label.p, (size_t)N * 4, hipMemcpyDeviceToHost, b.s));
+    FK_HIP(hipMemcpyAsync(deg, b.nsg_len.p, (size_t)N * 4, hipMemcpyDeviceToHost, b.s));
     b.sync();
     // every round reads each unreached node's out-list and its targets' labels
     b.prof_end(pe, FK_PROF_CONN, (double)(rounds + levels) * ((double)N * 8.0 + 8.0 * (double)N * b.P.nsg_R));

@@ -791,6 +791,7 @@ __global__ void k_upd_fill(const uint64_t *__restrict__ pool, uint8_t *__restric
 // subset (the reference shuffles with rand() and truncates, :1189-1200); the
 // subset is the R smallest counter-based hashes, so it is reproducible.
 #define FK_SEL_CH 2048
+#define FK_REV_NPB 4
 __global__ __launch_bounds__(256) void k_rev_select(uint32_t N, uint32_t R, uint64_t seed, uint32_t it,
                                                     const uint32_t *__restrict__ rcnt_new,
                                                     const uint32_t *__restrict__ rcnt_old,
@@ -806,43 +807,47 @@ __global__ __launch_bounds__(256) void k_rev_select(uint32_t N, uint32_t R, uint
                                                     const uint32_t *__restrict__ set_of,
                                                     uint32_t *__restrict__ lists) {
     __shared__ uint64_t s_k[FK_SEL_CH];
-    const uint32_t o = blockIdx.x;
-    // the hashes take ids within the node's set (a build of the set alone
-    // selects the same entries)
-    const uint32_t lo = set_of ? set_lo[set_of[o]] : 0u;
-    for (int which = 0; which < 2; ++which) {
-        const uint32_t c = which == 0 ? rcnt_new[o] : rcnt_old[o];
-        const uint32_t *src = which == 0 ? rbuf_new + roff_new[o] : rbuf_old + roff_old[o];
-        uint32_t *dst = lists + list_off[o] + (which == 0 ? fcnt_new[o] : n_new[o] + fcnt_old[o]);
-        if (c == 0) continue;
-        if (R == 0 || c <= R) {
-            for (uint32_t k = threadIdx.x; k < c; k += blockDim.x) dst[k] = src[k];
-            continue;
-        }
-        // running best-R selection over chunks
-        uint32_t have = 0;
-        const uint64_t stream = ((uint64_t)it << 33) | ((uint64_t)which << 32) | (o - lo);
-        for (uint32_t c0 = 0; c0 < c; c0 += (FK_SEL_CH - R)) {
-            const uint32_t m = (c - c0) < (FK_SEL_CH - R) ? (c - c0) : (FK_SEL_CH - R);
-            const uint32_t tot = have + m;
-            const uint32_t np2 = next_pow2(tot < 2 ? 2 : tot);
-            for (uint32_t k = threadIdx.x; k < np2; k += blockDim.x) {
-                if (k >= have) {
-                    if (k < tot) {
-                        const uint32_t s = src[c0 + (k - have)];
-                        s_k[k] = ((uint64_t)fk_rand32(seed, stream, s - lo) << 32) | s;
-                    } else {
-                        s_k[k] = FK_KEY_EMPTY;
+    // FK_REV_NPB consecutive nodes per workgroup: most lists are only copied,
+    // and a workgroup per node would make the launch dispatch-bound
+    const uint32_t o_end = (blockIdx.x + 1) * FK_REV_NPB < N ? (blockIdx.x + 1) * FK_REV_NPB : N;
+    for (uint32_t o = blockIdx.x * FK_REV_NPB; o < o_end; ++o) {
+        // the hashes take ids within the node's set (a build of the set alone
+        // selects the same entries)
+        const uint32_t lo = set_of ? set_lo[set_of[o]] : 0u;
+        for (int which = 0; which < 2; ++which) {
+            const uint32_t c = which == 0 ? rcnt_new[o] : rcnt_old[o];
+            const uint32_t *src = which == 0 ? rbuf_new + roff_new[o] : rbuf_old + roff_old[o];
+            uint32_t *dst = lists + list_off[o] + (which == 0 ? fcnt_new[o] : n_new[o] + fcnt_old[o]);
+            if (c == 0) continue;
+            if (R == 0 || c <= R) {
+                for (uint32_t k = threadIdx.x; k < c; k += blockDim.x) dst[k] = src[k];
+                continue;
+            }
+            // running best-R selection over chunks
+            uint32_t have = 0;
+            const uint64_t stream = ((uint64_t)it << 33) | ((uint64_t)which << 32) | (o - lo);
+            for (uint32_t c0 = 0; c0 < c; c0 += (FK_SEL_CH - R)) {
+                const uint32_t m = (c - c0) < (FK_SEL_CH - R) ? (c - c0) : (FK_SEL_CH - R);
+                const uint32_t tot = have + m;
+                const uint32_t np2 = next_pow2(tot < 2 ? 2 : tot);
+                for (uint32_t k = threadIdx.x; k < np2; k += blockDim.x) {
+                    if (k >= have) {
+                        if (k < tot) {
+                            const uint32_t s = src[c0 + (k - have)];
+                            s_k[k] = ((uint64_t)fk_rand32(seed, stream, s - lo) << 32) | s;
+                        } else {
+                            s_k[k] = FK_KEY_EMPTY;
+                        }
                     }
                 }
+                __syncthreads();
+                block_bitonic_sort(s_k, np2);
+                have = tot < R ? tot : R;
+                __syncthreads();
             }
-            __syncthreads();
-            block_bitonic_sort(s_k, np2);
-            have = tot < R ? tot : R;
+            for (uint32_t k = threadIdx.x; k < R; k += blockDim.x) dst[k] = (uint32_t)s_k[k];
             __syncthreads();
         }
-        for (uint32_t k = threadIdx.x; k < R; k += blockDim.x) dst[k] = (uint32_t)s_k[k];
-        __syncthreads();
     }
 }
 
@@ -1164,7 +1169,8 @@ static void nnd_update(BuildState &b, uint32_t it) {
                        b.list_off.p, b.n_new.p, b.fcnt_new.p, b.lists.p, b.roff_new.p, b.roff_old.p, b.bcnt.p,
                        b.pair_cnt.p, b.rbuf_new.p, b.rbuf_old.p);
     FK_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_rev_select, dim3(N), dim3(256), 0, b.s, N, b.P.R, (uint64_t)b.P.seed, it, b.rcnt_new.p,
+    hipLaunchKernelGGL(k_rev_select, dim3((N + FK_REV_NPB - 1) / FK_REV_NPB), dim3(256), 0, b.s, N, b.P.R,
+                       (uint64_t)b.P.seed, it, b.rcnt_new.p,
                        b.rcnt_old.p, b.roff_new.p, b.roff_old.p, b.rbuf_new.p, b.rbuf_old.p, b.list_off.p,
                        b.n_new.p, b.fcnt_new.p, b.fcnt_old.p, b.dev_set_lo(), b.dev_set_of(), b.lists.p);
     FK_LAUNCH_CHECK();

@@ -47,9 +47,50 @@ def graph_digest(off, ids, ep):
     return h.hexdigest()
 
 
-def total_dists(st):
-    return sum(int(st[k]) for k in ("d_init", "d_ctrl", "d_join", "d_prune", "d_prune_rev", "d_inter", "d_ep",
-                                    "d_bv", "d_conn"))
+def total_dists(st, computed=False):
+    """Distance evaluations of a build.  By default BridgeView pairs are
+    counted as the reference counts them (every pair it evaluates); with
+    computed=True only the pairs the GPU kernel actually computed (it skips
+    targets already pooled or computed in the same round)."""
+    keys = ("d_init", "d_ctrl", "d_join", "d_prune", "d_prune_rev", "d_inter", "d_ep", "d_conn")
+    bv = int(st.get("d_bv_computed", st["d_bv"])) if computed else int(st["d_bv"])
+    return sum(int(st[k]) for k in keys) + bv
+
+
+def after_nndescent(st):
+    """KNN-graph recall@MAXL over the controls right after NN-descent (the
+    entry after the per-iteration recall@K entries)."""
+    return float(st["recall"][int(st["nnd_iterations"])])
+
+
+def host_cpu_info():
+    """Logical CPUs, model, sockets, cores per socket and threads per core of
+    this host (from lscpu), and the CPUs this process may run on."""
+    import subprocess
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "Model name":
+                info["model"] = v
+            elif k == "Socket(s)":
+                info["sockets"] = int(v)
+            elif k == "Core(s) per socket":
+                info["cores_per_socket"] = int(v)
+            elif k == "Thread(s) per core":
+                info["threads_per_core"] = int(v)
+    except Exception as e:  # lscpu missing: the counts above still hold
+        info["lscpu_error"] = str(e)
+    return info
+
+
+def cpu_threads():
+    """Threads for the CPU baseline: OMP_NUM_THREADS when set (the share of
+    the host's CPUs given to this job), else every CPU this process may use."""
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return env if env > 0 else len(os.sched_getaffinity(0))
 
 
 # kernel of each timed family (the family's dominant launch)
@@ -73,9 +114,19 @@ def measured_traffic(kernel):
     return None
 
 
-def roofline(st, families):
+FP32_PEAK_TFLOPS = 157.3
+# distance counters of each family (for its FP32 rate: 2 d flops per distance)
+FAMILY_DISTS = {"prune": "d_prune", "prune_rev": "d_prune_rev", "bv": "d_bv_computed", "join": "d_join"}
+
+
+def roofline(st, families, d):
     """Roofline of the dominant kernel family: algorithmic HBM bytes per
-    launch over its mean launch time (HIP events on the library's stream)."""
+    launch (every row and key the work needs, each counted once) over its
+    mean launch time (HIP events on the library's stream).  Beside it the
+    same launch judged by its measured HBM traffic (rocprofv3 counters) and
+    by its FP32 distance arithmetic: the algorithmic rate counts rows that
+    the locality order serves from L2, so it is a consumption rate, and the
+    measured-traffic fraction says how much of HBM the kernel really uses."""
     ms = st["prof_ms"]
     i = int(np.argmax(ms[:len(families)]))
     if ms[i] <= 0:
@@ -83,11 +134,23 @@ def roofline(st, families):
     launches = max(int(st["prof_launches"][i]), 1)
     gbs = st["prof_bytes"][i] / (ms[i] / 1000.0) / 1e9
     kernel = FAMILY_KERNEL.get(families[i])
-    return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-            "traffic": measured_traffic(kernel), "kernel": kernel or families[i], "family": families[i],
-            "launches": launches, "bytes_per_launch": st["prof_bytes"][i] / launches,
-            "ms_per_launch": ms[i] / launches, "share_of_build": ms[i] / 1000.0 / max(st["t_total"], 1e-9),
-            "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT)}
+    ms_launch = ms[i] / launches
+    traffic = measured_traffic(kernel)
+    out = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+           "traffic": traffic, "kernel": kernel or families[i], "family": families[i],
+           "launches": launches, "bytes_per_launch": st["prof_bytes"][i] / launches,
+           "ms_per_launch": ms_launch, "share_of_build": ms[i] / 1000.0 / max(st["t_total"], 1e-9),
+           "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT),
+           "achieved_is": "algorithmic bytes (every examined candidate row and key once) per launch time; "
+                          "most of them come from the XCD's L2, see measured_traffic_frac"}
+    if traffic is not None:
+        tg = traffic / (ms_launch / 1000.0) / 1e9
+        out.update({"measured_traffic_gbs": tg, "measured_traffic_frac": tg / HBM_PEAK_GBS})
+    key = FAMILY_DISTS.get(families[i])
+    if key is not None and key in st:
+        tf = 2.0 * d * float(st[key]) / (ms[i] / 1000.0) / 1e12
+        out.update({"fp32_tflops": tf, "fp32_frac": tf / FP32_PEAK_TFLOPS})
+    return out
 
 
 def copy_probe(mib):
@@ -128,15 +191,54 @@ def cpu_baseline(n_sample, threads, dev, queries):
     gt, _ = dev.knn(queries, 10)
     rc = recall_at_10(dev.search(off, ids, ep, queries, 80, 10)[0], gt)
     rg = recall_at_10(dev.search(goff, gids, gep, queries, 80, 10)[0], gt)
+    # the same comparison below saturation: recall@10 at narrower beams
+    curve = {}
+    for sl in (10, 20, 40, 80):
+        curve[str(sl)] = {"cpu": recall_at_10(dev.search(off, ids, ep, queries, sl, 10)[0], gt),
+                          "gpu": recall_at_10(dev.search(goff, gids, gep, queries, sl, 10)[0], gt)}
     kc, kg = float(st["recall"][st["n_recall"] - 1]), float(gst["recall"][gst["n_recall"] - 1])
+    kc0, kg0 = after_nndescent(st), after_nndescent(gst)
     return {"value": nd / dt, "unit": "dist-evals/s", "cores": threads, "kind": "port",
             "sample": f"CPU oracle (C++/OpenMP restatement of the reference) NSG build of the first {n_sample:,} "
                       f"points of the same mixture, same parameters; {dt:.1f} s, {nd:,} distance evaluations",
-            "build_sec": dt,
+            "build_sec": dt, "host": host_cpu_info(),
             "sample_parity": {"cpu_build_sec": round(dt, 3), "gpu_build_sec": round(gdt, 4),
-                              "speedup": round(dt / gdt, 1),
+                              "speedup": round(dt / gdt, 1), "queries": int(len(queries)),
                               "cpu_recall_at_10": rc, "gpu_recall_at_10": rg,
+                              "recall_at_10_by_search_L": curve,
+                              "cpu_knn_recall_after_nndescent": kc0, "gpu_knn_recall_after_nndescent": kg0,
                               "cpu_knn_recall_at_maxl": kc, "gpu_knn_recall_at_maxl": kg}}
+
+
+def cpu_full_leg(dev, base, queries, gpu_graph, gpu_stats, threads):
+    """The CPU oracle's build of the full workload (minutes of CPU time, not
+    part of the default run): its build seconds beside the GPU's, and both
+    graphs searched by the same evaluator against the same ground truth."""
+    import oracle
+    from fastkcna_amd import FK_PG_NSG
+    from fastkcna_amd.datasets import params_c2
+    from fastkcna_amd.kgraph import IndexParams
+    p = params_c2(IndexParams)
+    log(f"cpu full: oracle build of {len(base):,} rows on {threads} threads")
+    t0 = time.perf_counter()
+    off, ids, ep, width, st, _ = oracle.build(FK_PG_NSG, base, p, threads=threads)
+    dt = time.perf_counter() - t0
+    log(f"cpu full: {dt:.1f}s")
+    dev.upload(base)
+    gt, _ = dev.knn(queries, 10)
+    goff, gids, gep = gpu_graph
+    curve = {}
+    for sl in (80, 200, 500):
+        curve[str(sl)] = {"cpu": recall_at_10(dev.search(off, ids, ep, queries, sl, 10)[0], gt),
+                          "gpu": recall_at_10(dev.search(goff, gids, gep, queries, sl, 10)[0], gt)}
+    return {"cpu_build_sec": dt, "threads": threads, "host": host_cpu_info(),
+            "cpu_dist_evals": total_dists(st), "gpu_build_sec": float(gpu_stats["t_total"]),
+            "recall_at_10_by_search_L": curve,
+            "cpu_knn_recall_after_nndescent": after_nndescent(st),
+            "gpu_knn_recall_after_nndescent": after_nndescent(gpu_stats),
+            "cpu_knn_recall_at_maxl": float(st["recall"][st["n_recall"] - 1]),
+            "gpu_knn_recall_at_maxl": float(gpu_stats["recall"][gpu_stats["n_recall"] - 1]),
+            "cpu_aod": int(st["aod"]), "gpu_aod": int(gpu_stats["aod"])}
 
 
 def variant_leg(dev, p, args, extra_flags, plain_graph, gt, queries, plain_sec, label):
@@ -238,6 +340,8 @@ def main():
     ap.add_argument("--queries", type=int, default=10_000)
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="also build the full workload with the CPU oracle (minutes) and compare recall")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="MiB of a device-to-device copy to run first")
     ap.add_argument("--no-chosen", action="store_true", help="skip the chosen (MFMA Gram join) leg")
@@ -302,14 +406,17 @@ def main():
     ms = 1000.0 * float(np.mean(times))
     nd = total_dists(st)
     value = nd / (ms / 1000.0)
-    roof = roofline(st, _ffi.PROF_FAMILIES)
+    roof = roofline(st, _ffi.PROF_FAMILIES, 128)
     log("kernel families (ms, GB/s): " + ", ".join(
         f"{f}={st['prof_ms'][i]:.1f}/{st['prof_bytes'][i] / max(st['prof_ms'][i], 1e-9) / 1e6:.0f}"
         for i, f in enumerate(_ffi.PROF_FAMILIES) if st['prof_launches'][i]))
     batching = batching_leg(dev, args.batch_sets) if args.batch_sets else None
-    cpu = None
+    cpu = cpu_full = None
+    threads = cpu_threads()
+    if args.cpu_full:
+        cpu_full = cpu_full_leg(dev, base, queries, graph, st, threads)
+        log(f"cpu full: {cpu_full}")
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(args.cpu_sample, threads, dev, queries[:2000])
         log(f"cpu baseline {cpu['value']:.3e} dist-evals/s on {threads} threads ({cpu['build_sec']:.1f}s); "
             f"sample parity {cpu['sample_parity']}")
@@ -330,7 +437,10 @@ def main():
         "config": {"workload": f"C2: NSG build, {args.n:,}x128 float32 integer-valued mixture of 1,024 "
                                "Gaussians (seed 1), K=L=500 S=12 R=100 I=6 nsg_R=50 search_L=80 search_K=500",
                    "build_sec": ms / 1000.0, "recall_at_10": rec, "search_L": 80, "queries": args.queries,
-                   "dist_evals": nd, "aod": int(st["aod"]), "nnd_iterations": int(st["nnd_iterations"]),
+                   "dist_evals": nd, "dist_evals_computed": total_dists(st, computed=True),
+                   "dist_evals_computed_per_sec": total_dists(st, computed=True) / (ms / 1000.0),
+                   "d_bv": int(st["d_bv"]), "d_bv_computed": int(st["d_bv_computed"]),
+                   "aod": int(st["aod"]), "nnd_iterations": int(st["nnd_iterations"]),
                    "graph_digest": digest,
                    "phase_sec": {k: round(float(st[k]), 4) for k in ("t_ctrl", "t_nnd", "t_prune", "t_inter",
                                                                     "t_ep", "t_conn", "t_bv", "t_total")}},
@@ -339,6 +449,7 @@ def main():
         "learned_screen": screen,
         "batching": batching,
         "cpu_baseline": cpu,
+        "cpu_full": cpu_full,
     }
     print(json.dumps(out), flush=True)
     dev.close()

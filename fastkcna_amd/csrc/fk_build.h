@@ -26,10 +26,11 @@ namespace fk {
 // tally counters of a build (BuildState::tally)
 #define FK_T_PRUNE_CC 0    // prune: distances
 #define FK_T_PRUNE_EXAM 1  // prune: examined candidates
-#define FK_T_BV_CC 2       // BridgeView round: distances
-#define FK_T_SEARCH 3      // entry-point search: distances
-#define FK_T_RECHECK 4     // Gram join: FP32 rechecks
-#define FK_T_SLOTS 5
+#define FK_T_BV_CC 2       // BridgeView round: distances the reference evaluates
+#define FK_T_BV_COMP 3     // BridgeView round: distances actually computed (slot after FK_T_BV_CC)
+#define FK_T_SEARCH 4      // entry-point search: distances
+#define FK_T_RECHECK 5     // Gram join: FP32 rechecks
+#define FK_T_SLOTS 6
 
 struct BuildParams {
     // effective (shrunk) parameters, src/kgraph.cpp:1288-1334

@@ -1601,8 +1601,8 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
     uint32_t W, uint32_t S_cap_all, const uint32_t *__restrict__ set_of, const uint32_t *__restrict__ set_scap,
     uint64_t *__restrict__ pool, uint8_t *__restrict__ pflg, uint32_t *__restrict__ cnt,
     const uint32_t *__restrict__ nsg_id, const uint32_t *__restrict__ nsg_len, const uint8_t *__restrict__ nsg_new,
-    uint32_t ncap, unsigned long long *__restrict__ cc_acc, const uint32_t *__restrict__ order,
-    uint32_t *__restrict__ mark, uint32_t wave_bytes, uint32_t wpb) {
+    uint32_t ncap, unsigned long long *__restrict__ cc_acc, unsigned long long *__restrict__ comp_acc,
+    const uint32_t *__restrict__ order, uint32_t *__restrict__ mark, uint32_t wave_bytes, uint32_t wpb) {
     using E = typename RowKind<NB, MODE>::E;
     using Chain = typename RowKind<NB, MODE>::Chain;
     constexpr bool DOT = RowKind<NB, MODE>::dot;
@@ -1713,7 +1713,7 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
     mine.load(lane_chains + (size_t)i * dpad, nb);
     const float ni = DOT ? nrm[i].x : 0.f;
     uint32_t np = np0, nc = 0, nt = 0;
-    unsigned long long cc = 0;
+    unsigned long long cc = 0, computed = 0;
     bool merged = false;
     uint32_t first_change = cap;  // smallest pool position rewritten
     // merges the candidates of c into the pool
@@ -1791,6 +1791,7 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
             const uint32_t nf = (uint32_t)__builtin_popcountll(mf);
             if (fresh) todo[nt + lanes_below(mf)] = t;
             nt += nf;
+            computed += nf;
         }
         wave_lds_sync();
         if (s0 + 64 * FK_BVW_PREFETCH >= total) {
@@ -1817,7 +1818,8 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
         for (uint32_t j = first_change + lane; j < np; j += 64) { pr[j] = p[j]; pfr[j] = pf[j]; }
         if (lane == 0) cnt[i] = np;
     }
-    if (lane == 0 && cc) tally_add(cc_acc, cc);  // the same count in every lane
+    if (lane == 0 && cc) tally_add(cc_acc, cc);  // the same counts in every lane
+    if (lane == 0 && computed) tally_add(comp_acc, computed);
 }
 
 static void launch_bv_round(BuildState &b, uint32_t W, uint32_t scap, const uint32_t *set_scap) {
@@ -1837,7 +1839,8 @@ static void launch_bv_round(BuildState &b, uint32_t W, uint32_t scap, const uint
                        MODE == 2 ? (const void *)c8 : (const void *)b.ds->chains.p, nrm,                        \
                        b.ds->dpad, b.N, b.cap, W, scap, b.dev_set_of(), set_scap, b.pool.p, b.pflg.p, b.cnt.p,  \
                        b.nsg_id.p, b.nsg_len.p,                                                                 \
-                       b.nsg_new.p, b.ncap, b.tally.at(FK_T_BV_CC), ord, mark, wave_bytes, wpb)
+                       b.nsg_new.p, b.ncap, b.tally.at(FK_T_BV_CC), b.tally.at(FK_T_BV_COMP), ord, mark,       \
+                       wave_bytes, wpb)
     switch (b.ds->dpad) {
         case 32: FK_BV_LAUNCH(4, 0); break;
         case 64: FK_BV_LAUNCH(8, 0); break;
@@ -1892,7 +1895,7 @@ void bridgeview(BuildState &b) {
         const uint32_t scap = scap_set[0];
         if (b.n_sets > 1)
             FK_HIP(hipMemcpyAsync(d_scap.p, scap_set.data(), (size_t)b.n_sets * 4, hipMemcpyHostToDevice, b.s));
-        b.tally.clear(b.s, FK_T_BV_CC);
+        b.tally.clear(b.s, FK_T_BV_CC, 2);  // and FK_T_BV_COMP
         if (b.prof && b.bv_mark.n < N) {
             b.bv_mark.alloc(N);
             FK_HIP(hipMemsetAsync(b.bv_mark.p, 0, (size_t)N * 4, b.s));
@@ -1901,7 +1904,10 @@ void bridgeview(BuildState &b) {
         launch_bv_round(b, W, scap, b.n_sets > 1 ? d_scap.p : nullptr);
         FK_LAUNCH_CHECK();
         b.prof_end(pe, FK_PROF_BV, 0.0);
-        const unsigned long long cc = b.tally.read1(b.s, FK_T_BV_CC);
+        unsigned long long cc2[2] = {0, 0};
+        b.tally.read(b.s, FK_T_BV_CC, 2, cc2);
+        const unsigned long long cc = cc2[0];
+        b.st->d_bv_computed += cc2[1];
         if (b.prof) {
             // minimal traffic of a round: the rows (+ ids) of every expansion
             // (|rnn_q|) and of every out-list entry of an expanded q

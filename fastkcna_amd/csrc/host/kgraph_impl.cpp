@@ -68,9 +68,27 @@ fk_params to_fk(KGraph::IndexParams const &p) {
 class KGraphImpl : public KGraph {
     fk_handle *h_ = nullptr;
     int dev_ = -1;
+    // What the resident rows were uploaded from: the Matrix address, shape
+    // and a fingerprint of its contents, so that a matrix rewritten in place
+    // or reallocated at the same address is uploaded again.
     const void *up_rows_ = nullptr;
-    unsigned up_n_ = 0;
+    unsigned up_n_ = 0, up_dim_ = 0;
     size_t up_stride_ = 0;
+    uint64_t up_sum_ = 0;
+
+    // FNV-1a over the bytes of 64 rows spread over the matrix (first and
+    // last included): O(64 d), negligible next to an upload.
+    static uint64_t fingerprint(const MatrixOracle<float, metric::l2sqr> *mo) {
+        uint64_t f = 1469598103934665603ull;
+        const unsigned n = mo->proxy_size();
+        const size_t bytes = (size_t)mo->dim() * sizeof(float);
+        for (unsigned k = 0; k < 64 && n > 0; ++k) {
+            const unsigned r = (unsigned)((uint64_t)k * (n - 1) / 63);
+            const unsigned char *b = (const unsigned char *)mo->rows() + (size_t)r * mo->stride();
+            for (size_t i = 0; i < bytes; ++i) f = (f ^ b[i]) * 1099511628211ull;
+        }
+        return f;
+    }
     std::vector<std::vector<unsigned>> temp_graph_;  // level-0 adjacency (HNSW)
 
     fk_handle *handle(int device) {
@@ -95,11 +113,17 @@ class KGraphImpl : public KGraph {
             throw invalid_argument(
                 "the GPU build needs a MatrixOracle<float, metric::l2sqr> (squared L2 over float rows)");
         fk_handle *h = handle(device);
-        if (up_rows_ == mo->rows() && up_n_ == mo->proxy_size() && up_stride_ == mo->stride()) return;
+        const uint64_t sum = fingerprint(mo);
+        if (up_rows_ == mo->rows() && up_n_ == mo->proxy_size() && up_stride_ == mo->stride() &&
+            up_dim_ == mo->dim() && up_sum_ == sum)
+            return;
+        up_rows_ = nullptr;
         check(fk_dataset_upload(h, mo->rows(), mo->stride(), mo->proxy_size(), mo->dim()), "fk_dataset_upload");
         up_rows_ = mo->rows();
         up_n_ = mo->proxy_size();
         up_stride_ = mo->stride();
+        up_dim_ = mo->dim();
+        up_sum_ = sum;
     }
 
     struct Graph {

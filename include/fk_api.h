@@ -165,6 +165,11 @@ typedef struct {
     uint64_t screen_cands;             /* pool candidates of the screened joins; one skipped
                                           pair in 64 is audited, so about 64 * screen_missed
                                           admitted candidates were lost to the screen */
+    /* BridgeView pairs whose distance was computed: d_bv counts every pair
+     * the reference evaluates, of which pairs whose target is already in the
+     * pool or was computed earlier in the same round for the same node are
+     * counted but not computed again (the result is the same). */
+    uint64_t d_bv_computed;
 } fk_stats;
 
 /* Output graph in CSR form, allocated by the library (fk_graph_free). */

@@ -128,32 +128,52 @@ public:
     T *operator[](unsigned i) { return reinterpret_cast<T *>(&data[stride * i]); }
     void zero() { memset(data, 0, (size_t)row * stride); }
 
+    /// Scales every row to unit L2 norm; the norm is the float sum of
+    /// squares, taken in double for the square root and the division.
     void normalize2() {
 #pragma omp parallel for
         for (unsigned i = 0; i < row; ++i) {
-            T *p = operator[](i);
-            double sum = metric::l2sqr::norm2(p, col);
-            sum = std::sqrt(sum);
-            for (unsigned j = 0; j < col; ++j) p[j] /= sum;
+            T *r = operator[](i);
+            const double len = std::sqrt((double)metric::l2sqr::norm2(r, col));
+            for (T *v = r; v != r + col; ++v) *v = (T)(*v / len);
         }
     }
 
-    void load(const std::string &path, unsigned dim, unsigned skip = 0, unsigned gap = 0) {
-        std::ifstream is(path.c_str(), std::ios::binary);
-        if (!is) throw io_error(path);
-        is.seekg(0, std::ios::end);
-        size_t size = is.tellg();
-        size -= skip;
-        const size_t line = sizeof(T) * dim + gap;
-        const unsigned N = (unsigned)(size / line);
-        reset(N, dim);
-        zero();
-        is.seekg(skip, std::ios::beg);
-        for (unsigned i = 0; i < N; ++i) {
-            is.read(&data[stride * i], sizeof(T) * dim);
-            is.seekg(gap, std::ios::cur);
+private:
+    /// A headerless file of fixed-size records: `skip` bytes of preamble,
+    /// then records of `dim` values each followed by `gap` bytes.
+    struct RecordFile {
+        std::ifstream in;
+        size_t skip, gap, value_bytes;
+        unsigned count = 0;
+        RecordFile(const std::string &path, unsigned dim, size_t skip_, size_t gap_)
+            : in(path.c_str(), std::ios::binary), skip(skip_), gap(gap_), value_bytes(sizeof(T) * (size_t)dim) {
+            if (!in) throw io_error(path);
+            in.seekg(0, std::ios::end);
+            const size_t total = (size_t)in.tellg();
+            count = total > skip ? (unsigned)((total - skip) / (value_bytes + gap)) : 0u;
+            in.seekg((std::streamoff)skip, std::ios::beg);
         }
-        if (!is) throw io_error(path);
+        // Record k of the file into dst; records are read in file order.
+        void next(char *dst) {
+            in.read(dst, (std::streamsize)value_bytes);
+            if (gap) in.seekg((std::streamoff)gap, std::ios::cur);
+        }
+    };
+
+    /// Fills the matrix from a record file; record k lands in row slot(k).
+    template <typename SLOT>
+    void fill_from(RecordFile &f, const std::string &path, SLOT slot) {
+        for (unsigned k = 0; k < f.count; ++k) f.next(getdata(slot(k)));
+        if (!f.in) throw io_error(path);
+    }
+
+public:
+    void load(const std::string &path, unsigned dim, unsigned skip = 0, unsigned gap = 0) {
+        RecordFile f(path, dim, skip, gap);
+        reset(f.count, dim);
+        zero();
+        fill_from(f, path, [](unsigned k) { return k; });
     }
 
     // HNSW level of one point: (int)(-ln U(0,1) * reverse_size)
@@ -189,41 +209,31 @@ public:
         for (unsigned i = 0; i < N; i++) file2data[data2file[i]] = (int)i;
     }
 
+    /// Loads the records in level order (highest level first, the reference's
+    /// hnsw_dataload): record k goes to row file2data[k]; returns the seconds
+    /// the level draw took.
     double hnsw_dataload(const std::string &path, unsigned dim, unsigned skip, unsigned gap, vector<int> &data2file,
                          vector<int> &points_num, unsigned M_ = 50, unsigned random_seed = 2024) {
-        std::ifstream is(path.c_str(), std::ios::binary);
-        if (!is) throw io_error(path);
-        is.seekg(0, std::ios::end);
-        size_t size = is.tellg();
-        size -= skip;
-        const size_t line = sizeof(T) * dim + gap;
-        const unsigned N = (unsigned)(size / line);
+        RecordFile f(path, dim, skip, gap);
         vector<int> file2data;
-        auto s = std::chrono::high_resolution_clock::now();
-        getlevel(N, data2file, file2data, points_num, M_, random_seed);
-        std::chrono::duration<double> diff = std::chrono::high_resolution_clock::now() - s;
-        const double time = diff.count();
-        if (verbosity) std::cout << "get level time: " << time << "\n";
-        reset(N, dim);
+        const auto t0 = std::chrono::steady_clock::now();
+        getlevel(f.count, data2file, file2data, points_num, M_, random_seed);
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (verbosity) std::cout << "get level time: " << secs << "\n";
+        reset(f.count, dim);
         zero();
-        is.seekg(skip, std::ios::beg);
-        for (unsigned i = 0; i < N; ++i) {
-            is.read(&data[stride * file2data[i]], sizeof(T) * dim);
-            is.seekg(gap, std::ios::cur);
-        }
-        if (!is) throw io_error(path);
-        return time;
+        fill_from(f, path, [&](unsigned k) { return (unsigned)file2data[k]; });
+        return secs;
     }
 
+    /// lshkit file: a header {value size, rows, dim} of three u32, then the
+    /// rows back to back.
     void load_lshkit(std::string const &path) {
-        unsigned header[3];
-        {
-            std::ifstream is(path.c_str(), std::ios::binary);
-            is.read((char *)header, sizeof header);
-            if (!is) throw io_error(path);
-        }
-        if (header[0] != sizeof(T)) throw io_error(path);
-        load(path, header[2], 3 * sizeof(unsigned), 0);
+        uint32_t header[3] = {0, 0, 0};
+        std::ifstream in(path.c_str(), std::ios::binary);
+        if (!in.read(reinterpret_cast<char *>(header), sizeof header) || header[0] != sizeof(T)) throw io_error(path);
+        in.close();
+        load(path, header[2], (unsigned)sizeof header, 0);
     }
 
     void save_lshkit(std::string const &path) {
@@ -297,12 +307,15 @@ public:
     /// Mean of all rows (data_size(), not the size() prefix), float sums in
     /// row order; the caller deletes[] the result.
     virtual float *Calcenter() const {
-        float *center = new float[proxy.dim()];
-        for (unsigned j = 0; j < proxy.dim(); j++) center[j] = 0;
-        for (unsigned i = 0; i < proxy.size(); i++)
-            for (unsigned j = 0; j < proxy.dim(); j++) center[j] += proxy[i][j];
-        for (unsigned j = 0; j < proxy.dim(); j++) center[j] /= proxy.size();
-        return center;
+        const unsigned d = proxy.dim(), n = proxy.size();
+        float *mean = new float[d]();
+        for (unsigned r = 0; r < n; ++r) {  // running float sums, one row at a time
+            DATA_TYPE const *x = proxy[r];
+            for (float *m = mean; m != mean + d; ++m, ++x) *m += (float)*x;
+        }
+        const float count = (float)n;
+        for (float *m = mean; m != mean + d; ++m) *m /= count;
+        return mean;
     }
     SearchOracle query(DATA_TYPE const *q) const { return SearchOracle(proxy, q); }
     // Row storage for the GPU upload (not in the reference).

@@ -160,7 +160,9 @@ def knn(rows, queries, K, self_ids=None):
 
 
 def build(pg_type, rows, params, n_used=None, threads=0, full_rule_only=False, check_prune2=False, verbose=False):
-    """Full CPU build; returns (offsets, ids, ep, width, stats dict, prune2_diff)."""
+    """Full CPU build; returns (offsets, ids, ep, width, stats dict, prune2)
+    with prune2 = (nodes where sync_prune2 differed from the full rule,
+    nodes where the two were compared) when check_prune2 is set."""
     rows = np.ascontiguousarray(rows, np.float32)
     n_used = rows.shape[0] if n_used is None else n_used
     lib = _load()
@@ -169,18 +171,18 @@ def build(pg_type, rows, params, n_used=None, threads=0, full_rule_only=False, c
     ep = ctypes.c_uint32()
     width = ctypes.c_uint32()
     st = _stats()
-    diff = ctypes.c_uint64()
+    diff = (ctypes.c_uint64 * 2)()
     opts = BuildOpts(threads, int(full_rule_only), int(check_prune2), int(verbose))
     ps = _params(params)
     lib.fko_build(pg_type, _p(rows), rows.strides[0], rows.shape[0], n_used, rows.shape[1], ctypes.byref(ps),
                   ctypes.byref(opts), ctypes.byref(off_p), ctypes.byref(ids_p), ctypes.byref(ep),
-                  ctypes.byref(width), ctypes.byref(st), ctypes.byref(diff))
+                  ctypes.byref(width), ctypes.byref(st), diff)
     off = np.ctypeslib.as_array(off_p, shape=(n_used + 1,)).copy()
     ne = int(off[-1])
     ids = np.ctypeslib.as_array(ids_p, shape=(max(ne, 1),))[:ne].copy()
     lib.fko_free(ctypes.cast(off_p, ctypes.c_void_p))
     lib.fko_free(ctypes.cast(ids_p, ctypes.c_void_p))
-    return off, ids, ep.value, width.value, st.to_dict(), diff.value
+    return off, ids, ep.value, width.value, st.to_dict(), (int(diff[0]), int(diff[1]))
 
 
 def nndescent(rows, params, threads=0):

@@ -262,6 +262,7 @@ struct Builder {
     fk_stats *st;
     bool full_rule_only = false;   // never use *_sync_prune2
     uint64_t prune2_diff = 0;      // nodes where sync_prune2 != sync_prune
+    uint64_t prune2_nodes = 0;     // nodes where both were compared
     bool check_prune2 = false;
     RuleP rp{};
     bool verbose = false;
@@ -602,8 +603,8 @@ struct Builder {
     void link(unsigned li) {
         const unsigned N_ = N(), range = P.nsg_R;
         const double t0 = now();
-        uint64_t cc = 0, diff = 0;
-#pragma omp parallel for schedule(dynamic, 100) reduction(+ : cc, diff)
+        uint64_t cc = 0, diff = 0, compared = 0;
+#pragma omp parallel for schedule(dynamic, 100) reduction(+ : cc, diff, compared)
         for (unsigned n = 0; n < N_; ++n) {
             if (pruneAgain[n] || full_rule_only) {
                 cc += sync_prune(n, nh[n].nn_nsg);
@@ -619,11 +620,13 @@ struct Builder {
                 bool same = full.size() == nh[n].nn_nsg.size();
                 for (size_t k = 0; same && k < full.size(); ++k) same = full[k].id == nh[n].nn_nsg[k].id;
                 if (!same) ++diff;
+                ++compared;
             } else {
                 cc += sync_prune2(n);
             }
         }
         prune2_diff += diff;
+        prune2_nodes += compared;
         st->d_prune += cc;
         const double t1 = now();
         if (li < FK_MAX_LOOPS) {
@@ -912,6 +915,7 @@ struct Builder {
             st->bv_rounds[li] = rounds;
         }
         st->d_bv += cc;
+        st->d_bv_computed += cc;  // the restatement computes every pair it counts
         st->t_bv += now() - t0;
     }
 
@@ -1108,7 +1112,7 @@ struct fko_build_opts {
 // Full CPU build; the output graph is a malloc'd CSR (free with fko_free).
 int fko_build(int type, const float *rows, size_t stride_bytes, unsigned n_total, unsigned n_used, unsigned d,
               const fk_params *p, const fko_build_opts *opt, uint64_t **offsets, uint32_t **ids, uint32_t *ep,
-              uint32_t *width, fk_stats *st, uint64_t *prune2_diff) {
+              uint32_t *width, fk_stats *st, uint64_t *prune2) {
     if (opt && opt->threads > 0) omp_set_num_threads(opt->threads);
     Rows r;
     r.load(rows, stride_bytes, n_total, d);
@@ -1130,7 +1134,10 @@ int fko_build(int type, const float *rows, size_t stride_bytes, unsigned n_total
     *ids = out;
     *ep = b.ep;
     *width = b.width;
-    if (prune2_diff) *prune2_diff = b.prune2_diff;
+    if (prune2) {  // nodes where the incremental rule differed, nodes compared
+        prune2[0] = b.prune2_diff;
+        prune2[1] = b.prune2_nodes;
+    }
     return 0;
 }
 

@@ -73,11 +73,21 @@ def test_build_invariants(pg):
     assert st["d_join"] > 0 and st["d_prune"] > 0 and st["d_bv"] > 0
 
 
-def test_prune2_matches_full_rule_count():
-    """The paper says the incremental rule is result-preserving; the oracle
-    counts the nodes where it is not (the GPU always applies the full rule)."""
+@pytest.mark.parametrize("pg,tau,angle", [(FK_PG_NSG, 0.0, 60.0), (FK_PG_TAUMNG, 2.0, 60.0),
+                                          (FK_PG_ALPHAPG, 0.0, 50.0)])
+@pytest.mark.parametrize("loop_i", [2, 3])
+def test_prune2_matches_full_rule(pg, tau, angle, loop_i):
+    """The paper says the incremental rule (*_sync_prune2, applied from the
+    second loop on to nodes the previous loop did not re-prune) is
+    result-preserving; the GPU relies on it by always applying the full rule.
+    In every loop after the first the oracle runs both rules on the same pool
+    state (after a BridgeView pass, so isnew and pid are set) and counts the
+    nodes whose out-lists differ: there must be none, over a non-trivial
+    number of compared nodes."""
     x = small_data(2000)
-    p = IndexParams(K=40, L=40, S=10, R=50, iterations=5, nsg_R=16, search_L=40, search_K=40)
-    *_, diff = oracle.build(FK_PG_NSG, x, p, threads=1, check_prune2=True)
-    assert diff <= len(x)  # recorded; see the oracle documentation for the measured rate
-    print("prune2 differs from the full rule on", diff, "nodes")
+    p = IndexParams(K=40, L=40, S=10, R=50, iterations=5, nsg_R=16, search_L=30, search_K=40, loop_i=loop_i,
+                    tau=tau, angle=angle)
+    *_, (diff, compared) = oracle.build(pg, x, p, threads=1, check_prune2=True)
+    print("prune2: differing nodes", diff, "of", compared)
+    assert compared >= loop_i * len(x) // 2  # the incremental rule ran in loops 2 .. loop_i + 1
+    assert diff == 0

@@ -112,6 +112,8 @@ _PROTOS = {
     "fk_join_distances": (ctypes.c_int, [_P, _P, _P, _P, u32, _P, ctypes.POINTER(u64)]),
     "fk_prune": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, u32, _P, u32, u32, ctypes.POINTER(FkParams),
                                 ctypes.c_int, _P, _P, u32, _P, ctypes.POINTER(u64)]),
+    "fk_prune_noalloc": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, u32, _P, u32, u32, ctypes.POINTER(FkParams),
+                                        ctypes.c_int, _P, _P, u32, _P, ctypes.POINTER(u64)]),
     "fk_nndescent": (ctypes.c_int, [_P, u32, ctypes.POINTER(FkParams), _P, _P, u32, ctypes.POINTER(FkStats)]),
     "fk_knn_bruteforce": (ctypes.c_int, [_P, _P, u32, u32, u32, _P, _P, _P]),
     "fk_search": (ctypes.c_int, [_P, _P, _P, u32, u32, _P, u32, u32, u32, _P, ctypes.POINTER(u64)]),

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <new>
+#include <mutex>
 #include <thread>
 
 #include "fk_api_impl.h"
@@ -195,6 +196,49 @@ void require_dataset(fk_handle *h) {
     if (h->ds.n_total == 0) throw Error(FK_E_INVALID, "no dataset uploaded (fk_dataset_upload)");
 }
 
+// ---------------------------------------------------------------------------
+// fk_prune_noalloc: the prune with every device buffer in `scratch`, a
+// file-scope device array (each GPU holds its own copy from the first use of
+// the library on it until the process ends).  Host code never passes the
+// array's name to a kernel: it uses the device address hipGetSymbolAddress
+// returns, looked up once per device by fk_open (noalloc_scratch) and kept in
+// the handle.  Layout of one call, each part 256-byte aligned: the distance
+// tally (FK_TALLY_WORDS u64), pools, pool lengths, owners, output ids,
+// output distances, output lengths.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) uint8_t scratch[FK_PRUNE_NOALLOC_SCRATCH_BYTES];
+
+// Device address of `scratch` on `device`, which must be current (one lookup
+// per device, kept for the process).
+uint8_t *noalloc_scratch(int device) {
+    static std::mutex mu;
+    static std::vector<uint8_t *> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((size_t)device >= addr.size()) addr.resize((size_t)device + 1, nullptr);
+    if (!addr[device]) {
+        void *p = nullptr;
+        FK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw Error(FK_E_DEVICE, "hipGetSymbolAddress(scratch) returned no address");
+        addr[device] = static_cast<uint8_t *>(p);
+    }
+    return addr[device];
+}
+
+// Zero fill of n u64 words (in place of a memset node, so that the call has
+// only kernels and copies).
+__global__ void k_fill_zero_u64(unsigned long long *__restrict__ p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes of scratch a call of n_tasks pools needs.
+size_t noalloc_bytes(uint32_t n_tasks, uint32_t pool_stride, uint32_t out_stride) {
+    const size_t n = n_tasks;
+    return align256((size_t)FK_TALLY_WORDS * 8) + align256(n * pool_stride * 8) + 2 * align256(n * 4) +
+           2 * align256(n * out_stride * 4) + align256(n * 4);
+}
+
 }  // namespace fk
 
 using namespace fk;
@@ -218,6 +262,7 @@ fk_status fk_open(int device, fk_handle **out) {
         select_device(h);
         FK_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         h->counters.ensure(2);
+        h->noalloc = noalloc_scratch(device);  // this device's copy of the fk_prune_noalloc scratch
     });
     if (st != FK_OK) {
         delete h;
@@ -350,6 +395,51 @@ fk_status fk_join_distances(fk_handle *h, const uint32_t *ids, const uint64_t *l
     });
 }
 
+// Argument checks shared by fk_prune and fk_prune_noalloc.
+static void prune_checks(const fk_handle *h, const uint64_t *pools, const uint32_t *pool_len, uint32_t pool_stride,
+                         const uint32_t *q, uint32_t n_tasks, const fk_params *p, int reversed,
+                         const uint32_t *out_ids, uint32_t out_stride, const uint32_t *out_len) {
+    FK_CHECK_ARG(p && pools && pool_len && out_ids && out_len, "null pointer");
+    FK_CHECK_ARG(out_stride >= p->nsg_R, "out_stride < nsg_R");
+    FK_CHECK_ARG(p->nsg_R <= 256, "nsg_R above 256 is not supported");
+    for (uint32_t t = 0; t < n_tasks; ++t) {
+        FK_CHECK_ARG(pool_len[t] <= pool_stride, "pool_len > pool_stride");
+        const uint64_t *c = pools + (uint64_t)t * pool_stride;
+        // Rows the pruner reads must be resident: pool[start] is always
+        // kept; the sync form skips later ids >= oracle_size by itself,
+        // the reversed form reads every entry.
+        const uint32_t qt = q ? q[t] : t;
+        const uint32_t start = (pool_len[t] > 0 && (uint32_t)c[0] == qt) ? 1u : 0u;
+        if (start < pool_len[t]) FK_CHECK_ARG((uint32_t)c[start] < h->ds.n_total, "pool id out of range");
+        if (reversed)
+            for (uint32_t k = 0; k < pool_len[t]; ++k)
+                FK_CHECK_ARG((uint32_t)c[k] < h->ds.n_total, "pool id out of range");
+    }
+}
+
+// The launch arguments of one prune over device-resident pools (everything
+// but the buffers).
+static PruneLaunch prune_args(const fk_handle *h, fk_pg_type type, uint32_t pool_stride, uint32_t n_tasks,
+                              uint32_t oracle_size, const fk_params *p, int reversed, uint32_t out_stride) {
+    PruneLaunch a;
+    if ((p->flags & FK_FLAG_GRAM) && h->ds.gram_fmt == FK_GRAM_EXACT) {
+        // exact-form rows: occlusion distances from dot products
+        a.nrm = h->ds.gram_nrm.p;
+        a.chains8 = h->ds.chains8.p;
+    }
+    const uint32_t lim = oracle_size < h->ds.n_total ? oracle_size : h->ds.n_total;
+    a.cand_stride = pool_stride;
+    a.n_tasks = n_tasks;
+    a.oracle_n = reversed ? 0xFFFFFFFFu : lim;
+    a.maxc = reversed ? 0xFFFFFFFFu : p->search_K;
+    a.R = p->nsg_R;
+    a.tau3 = 3.0f * p->tau;
+    a.cos_thr = alpha_cos_threshold(p->angle);
+    a.rule = rule_of(type);
+    a.out_stride = out_stride;
+    return a;
+}
+
 fk_status fk_prune(fk_handle *h, fk_pg_type type, const uint64_t *pools, const uint32_t *pool_len,
                    uint32_t pool_stride, const uint32_t *q, uint32_t n_tasks, uint32_t oracle_size,
                    const fk_params *p, int reversed, uint32_t *out_ids, float *out_dists, uint32_t out_stride,
@@ -357,23 +447,7 @@ fk_status fk_prune(fk_handle *h, fk_pg_type type, const uint64_t *pools, const u
     if (!h) return FK_E_INVALID;
     return guard(h, [&] {
         require_dataset(h);
-        FK_CHECK_ARG(p && pools && pool_len && out_ids && out_len, "null pointer");
-        FK_CHECK_ARG(out_stride >= p->nsg_R, "out_stride < nsg_R");
-        FK_CHECK_ARG(p->nsg_R <= 256, "nsg_R above 256 is not supported");
-        const uint32_t lim = reversed ? h->ds.n_total : (oracle_size < h->ds.n_total ? oracle_size : h->ds.n_total);
-        for (uint32_t t = 0; t < n_tasks; ++t) {
-            FK_CHECK_ARG(pool_len[t] <= pool_stride, "pool_len > pool_stride");
-            const uint64_t *c = pools + (uint64_t)t * pool_stride;
-            // Rows the pruner reads must be resident: pool[start] is always
-            // kept; the sync form skips later ids >= oracle_size by itself,
-            // the reversed form reads every entry.
-            const uint32_t qt = q ? q[t] : t;
-            const uint32_t start = (pool_len[t] > 0 && (uint32_t)c[0] == qt) ? 1u : 0u;
-            if (start < pool_len[t]) FK_CHECK_ARG((uint32_t)c[start] < h->ds.n_total, "pool id out of range");
-            if (reversed)
-                for (uint32_t k = 0; k < pool_len[t]; ++k)
-                    FK_CHECK_ARG((uint32_t)c[k] < h->ds.n_total, "pool id out of range");
-        }
+        prune_checks(h, pools, pool_len, pool_stride, q, n_tasks, p, reversed, out_ids, out_stride, out_len);
         if (n_tasks == 0) {
             if (cc) *cc = 0;
             return;
@@ -387,29 +461,13 @@ fk_status fk_prune(fk_handle *h, fk_pg_type type, const uint64_t *pools, const u
         FK_HIP(hipMemcpyAsync(dlen.p, pool_len, (size_t)n_tasks * 4, hipMemcpyHostToDevice, h->stream));
         if (q) FK_HIP(hipMemcpyAsync(dq.p, q, (size_t)n_tasks * 4, hipMemcpyHostToDevice, h->stream));
         h->counters.clear(h->stream, 0);
-        PruneLaunch a;
-        if (p->flags & FK_FLAG_GRAM) {
-            // exact-form rows: occlusion distances from dot products
-            gram_prepare(h->stream, h->ds, p->flags);
-            if (h->ds.gram_fmt == FK_GRAM_EXACT) {
-                a.nrm = h->ds.gram_nrm.p;
-                a.chains8 = h->ds.chains8.p;
-            }
-        }
+        if (p->flags & FK_FLAG_GRAM) gram_prepare(h->stream, h->ds, p->flags);
+        PruneLaunch a = prune_args(h, type, pool_stride, n_tasks, oracle_size, p, reversed, out_stride);
         a.cand = dpool.p;
         a.cand_len = dlen.p;
-        a.cand_stride = pool_stride;
         a.qid = q ? dq.p : nullptr;
-        a.n_tasks = n_tasks;
-        a.oracle_n = reversed ? 0xFFFFFFFFu : lim;
-        a.maxc = reversed ? 0xFFFFFFFFu : p->search_K;
-        a.R = p->nsg_R;
-        a.tau3 = 3.0f * p->tau;
-        a.cos_thr = alpha_cos_threshold(p->angle);
-        a.rule = rule_of(type);
         a.out_ids = doid.p;
         a.out_d = dod.p;
-        a.out_stride = out_stride;
         a.out_len = dolen.p;
         a.cc = h->counters.at(0);
         launch_prune(h->stream, h->ds, a);
@@ -420,6 +478,73 @@ fk_status fk_prune(fk_handle *h, fk_pg_type type, const uint64_t *pools, const u
                                   h->stream));
         FK_HIP(hipMemcpyAsync(out_len, dolen.p, (size_t)n_tasks * 4, hipMemcpyDeviceToHost, h->stream));
         const unsigned long long c = h->counters.read1(h->stream, 0);
+        if (cc) *cc = c;
+    });
+}
+
+fk_status fk_prune_noalloc(fk_handle *h, fk_pg_type type, const uint64_t *pools, const uint32_t *pool_len,
+                           uint32_t pool_stride, const uint32_t *q, uint32_t n_tasks, uint32_t oracle_size,
+                           const fk_params *p, int reversed, uint32_t *out_ids, float *out_dists,
+                           uint32_t out_stride, uint32_t *out_len, uint64_t *cc) {
+    if (!h) return FK_E_INVALID;
+    return guard(h, [&] {
+        require_dataset(h);
+        prune_checks(h, pools, pool_len, pool_stride, q, n_tasks, p, reversed, out_ids, out_stride, out_len);
+        const size_t need = noalloc_bytes(n_tasks, pool_stride, out_stride);
+        if (need > FK_PRUNE_NOALLOC_SCRATCH_BYTES)
+            throw Error(FK_E_INVALID, "fk_prune_noalloc: " + std::to_string(n_tasks) + " pools of stride " +
+                                          std::to_string(pool_stride) + " need " + std::to_string(need) +
+                                          " bytes of scratch, above the maximum " +
+                                          std::to_string((size_t)FK_PRUNE_NOALLOC_SCRATCH_BYTES));
+        if (p->flags & FK_FLAG_GRAM) {
+            const uint32_t want = p->flags & (FK_FLAG_GRAM_BF16 | FK_FLAG_GRAM_F16);
+            FK_CHECK_ARG(h->ds.gram_fmt != FK_GRAM_NONE && h->ds.gram_req == want,
+                         "fk_prune_noalloc with FK_FLAG_GRAM needs the dataset's Gram form prepared by an earlier "
+                         "fk_prune or fk_build with the same flags");
+        }
+        if (n_tasks == 0) {
+            if (cc) *cc = 0;
+            return;
+        }
+        select_device(h);
+        FK_CHECK_ARG(h->noalloc != nullptr, "scratch address not set up");
+        uint8_t *at = h->noalloc;
+        auto take = [&](size_t bytes) {
+            uint8_t *r = at;
+            at += align256(bytes);
+            return r;
+        };
+        const size_t n = n_tasks;
+        auto *tally = reinterpret_cast<unsigned long long *>(take((size_t)FK_TALLY_WORDS * 8));
+        auto *dpool = reinterpret_cast<uint64_t *>(take(n * pool_stride * 8));
+        auto *dlen = reinterpret_cast<uint32_t *>(take(n * 4));
+        auto *dq = reinterpret_cast<uint32_t *>(take(n * 4));
+        auto *doid = reinterpret_cast<uint32_t *>(take(n * out_stride * 4));
+        auto *dod = reinterpret_cast<float *>(take(n * out_stride * 4));
+        auto *dolen = reinterpret_cast<uint32_t *>(take(n * 4));
+        FK_HIP(hipMemcpyAsync(dpool, pools, n * pool_stride * 8, hipMemcpyHostToDevice, h->stream));
+        FK_HIP(hipMemcpyAsync(dlen, pool_len, n * 4, hipMemcpyHostToDevice, h->stream));
+        if (q) FK_HIP(hipMemcpyAsync(dq, q, n * 4, hipMemcpyHostToDevice, h->stream));
+        hipLaunchKernelGGL(k_fill_zero_u64, dim3(FK_TALLY_WORDS / 256), dim3(256), 0, h->stream, tally,
+                           (uint32_t)FK_TALLY_WORDS);
+        FK_LAUNCH_CHECK();
+        PruneLaunch a = prune_args(h, type, pool_stride, n_tasks, oracle_size, p, reversed, out_stride);
+        a.cand = dpool;
+        a.cand_len = dlen;
+        a.qid = q ? dq : nullptr;
+        a.out_ids = doid;
+        a.out_d = dod;
+        a.out_len = dolen;
+        a.cc = tally;
+        launch_prune(h->stream, h->ds, a);
+        FK_HIP(hipMemcpyAsync(out_ids, doid, n * out_stride * 4, hipMemcpyDeviceToHost, h->stream));
+        if (out_dists) FK_HIP(hipMemcpyAsync(out_dists, dod, n * out_stride * 4, hipMemcpyDeviceToHost, h->stream));
+        FK_HIP(hipMemcpyAsync(out_len, dolen, n * 4, hipMemcpyDeviceToHost, h->stream));
+        unsigned long long words[FK_TALLY_WORDS];
+        FK_HIP(hipMemcpyAsync(words, tally, sizeof(words), hipMemcpyDeviceToHost, h->stream));
+        FK_HIP(hipStreamSynchronize(h->stream));
+        unsigned long long c = 0;
+        for (uint32_t j = 0; j < FK_TALLY_STRIPES; ++j) c += words[j * FK_TALLY_STRIDE];
         if (cc) *cc = c;
     });
 }

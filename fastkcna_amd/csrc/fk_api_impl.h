@@ -42,5 +42,9 @@ void download_bytes(fk_handle *h, void *dst, const void *src, size_t bytes);
 void *host_result_alloc(size_t bytes);
 void upload_rows(fk_handle *h, float *dst, uint32_t dpad, const float *src, size_t src_stride_bytes, uint32_t n,
                  uint32_t d);
+// fk_prune_noalloc's scratch: its device address on `device` (current), and
+// the bytes a call needs.
+uint8_t *noalloc_scratch(int device);
+size_t noalloc_bytes(uint32_t n_tasks, uint32_t pool_stride, uint32_t out_stride);
 
 }  // namespace fk

@@ -206,6 +206,7 @@ struct fk_handle {
     fk::PinnedBuf<uint32_t> host_rows;         // connectivity pass: pool rows of attached ids
     fk::PinnedBuf<uint32_t> host_lab, host_deg;  // connectivity pass: labels and out-degrees
     fk::PinnedBuf<uint8_t> dl_stage[2];        // device-to-host staging (download_bytes)
+    uint8_t *noalloc = nullptr;                // device address of the fk_prune_noalloc scratch array
 };
 
 namespace fk {

@@ -140,7 +140,9 @@ class Device:
         assert npairs.value == total
         return out[:total]
 
-    def prune(self, pg_type, pools, pool_len, q, oracle_size, params, reversed_=False):
+    def prune(self, pg_type, pools, pool_len, q, oracle_size, params, reversed_=False, noalloc=False):
+        """Occlusion prune of the given pools (fk_prune, or fk_prune_noalloc
+        when noalloc: the same result with no device allocation)."""
         pools = _as(pools, np.uint64)
         pool_len = _as(pool_len, np.uint32)
         n_tasks, stride = pools.shape
@@ -151,11 +153,11 @@ class Device:
         out_len = np.zeros(n_tasks, np.uint32)
         cc = ctypes.c_uint64()
         p = _ffi.params_struct(params)
+        fn, name = (self.lib.fk_prune_noalloc, "fk_prune_noalloc") if noalloc else (self.lib.fk_prune, "fk_prune")
         self._check(
-            self.lib.fk_prune(self.h, pg_type, ptr(pools), ptr(pool_len), stride, ptr(q), n_tasks, oracle_size,
-                              ctypes.byref(p), 1 if reversed_ else 0, ptr(out_ids), ptr(out_d), R, ptr(out_len),
-                              ctypes.byref(cc)),
-            "fk_prune",
+            fn(self.h, pg_type, ptr(pools), ptr(pool_len), stride, ptr(q), n_tasks, oracle_size, ctypes.byref(p),
+               1 if reversed_ else 0, ptr(out_ids), ptr(out_d), R, ptr(out_len), ctypes.byref(cc)),
+            name,
         )
         return out_ids, out_d, out_len, cc.value
 

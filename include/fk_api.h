@@ -270,6 +270,24 @@ fk_status fk_prune(fk_handle *h, fk_pg_type type, const uint64_t *pools, const u
                    const fk_params *p, int reversed, uint32_t *out_ids, float *out_dists,
                    uint32_t out_stride, uint32_t *out_len, uint64_t *cc);
 
+/* fk_prune without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and errors as
+ * fk_prune; the staged pools, lengths and owners, the outputs and the
+ * distance counter live in a static device scratch area of
+ * FK_PRUNE_NOALLOC_SCRATCH_BYTES per GPU, held from the first fk_open on that
+ * GPU until the process ends.  A call needing more than that (about
+ * n_tasks * (8 pool_stride + 8 out_stride + 12) bytes) fails with
+ * FK_E_INVALID.  Every call on one GPU shares that GPU's area: calls that
+ * overlap on one GPU (several threads or handles) race.  With FK_FLAG_GRAM
+ * the dataset's Gram form must already be prepared for the same flags (an
+ * earlier fk_prune or fk_build with them), else FK_E_INVALID. */
+#define FK_PRUNE_NOALLOC_SCRATCH_BYTES (64u << 20)
+fk_status fk_prune_noalloc(fk_handle *h, fk_pg_type type, const uint64_t *pools, const uint32_t *pool_len,
+                           uint32_t pool_stride, const uint32_t *q, uint32_t n_tasks, uint32_t oracle_size,
+                           const fk_params *p, int reversed, uint32_t *out_ids, float *out_dists,
+                           uint32_t out_stride, uint32_t *out_len, uint64_t *cc);
+
 /* NN-descent only (the KGraphConstructor loop) over the first n_used rows.
  * pools receives n_used * pool_stride packed entries (pool_stride >= L), each
  * node's first pool_len[i] sorted ascending; stats may be NULL. */

@@ -85,6 +85,50 @@ class FkStats(ctypes.Structure):
         return out
 
 
+class FkState(ctypes.Structure):
+    _fields_ = [
+        ("n", u32), ("pool_cap", u32), ("out_cap", u32), ("loopcount", u32),
+        ("pool", ctypes.c_void_p), ("pool_len", ctypes.c_void_p), ("pool_flags", ctypes.c_void_p),
+        ("out_id", ctypes.c_void_p), ("out_dist", ctypes.c_void_p), ("out_isnew", ctypes.c_void_p),
+        ("out_len", ctypes.c_void_p), ("prev_id", ctypes.c_void_p), ("prev_len", ctypes.c_void_p),
+    ]
+
+
+class StepState:
+    """A build state in host memory for the step entry points (fk_state):
+    pools with their flags, out-lists with their isnew bits, and the
+    previous out-lists, as numpy arrays the C structure points into."""
+
+    def __init__(self, n, pool_cap, out_cap, loopcount=0):
+        self.n, self.pool_cap, self.out_cap, self.loopcount = int(n), int(pool_cap), int(out_cap), int(loopcount)
+        self.pool = np.zeros((n, pool_cap), np.uint64)
+        self.pool_len = np.zeros(n, np.uint32)
+        self.pool_flags = np.zeros((n, pool_cap), np.uint8)
+        self.out_id = np.zeros((n, out_cap), np.uint32)
+        self.out_dist = np.zeros((n, out_cap), np.float32)
+        self.out_isnew = np.zeros((n, out_cap), np.uint8)
+        self.out_len = np.zeros(n, np.uint32)
+        self.prev_id = np.zeros((n, out_cap), np.uint32)
+        self.prev_len = np.zeros(n, np.uint32)
+
+    ARRAYS = ("pool", "pool_len", "pool_flags", "out_id", "out_dist", "out_isnew", "out_len", "prev_id", "prev_len")
+
+    def copy(self):
+        c = StepState.__new__(StepState)
+        c.n, c.pool_cap, c.out_cap, c.loopcount = self.n, self.pool_cap, self.out_cap, self.loopcount
+        for k in self.ARRAYS:
+            setattr(c, k, getattr(self, k).copy())
+        return c
+
+    def struct(self):
+        s = FkState(self.n, self.pool_cap, self.out_cap, self.loopcount)
+        for k in self.ARRAYS:
+            a = getattr(self, k)
+            assert a.flags["C_CONTIGUOUS"] and a.shape[0] == self.n
+            setattr(s, k, a.ctypes.data_as(ctypes.c_void_p))
+        return s
+
+
 class FkGraph(ctypes.Structure):
     _fields_ = [
         ("n", u32), ("ep", u32), ("width", u32),
@@ -117,6 +161,13 @@ _PROTOS = {
     "fk_nndescent": (ctypes.c_int, [_P, u32, ctypes.POINTER(FkParams), _P, _P, u32, ctypes.POINTER(FkStats)]),
     "fk_knn_bruteforce": (ctypes.c_int, [_P, _P, u32, u32, u32, _P, _P, _P]),
     "fk_search": (ctypes.c_int, [_P, _P, _P, u32, u32, _P, u32, u32, u32, _P, ctypes.POINTER(u64)]),
+    "fk_step_link": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(FkParams), ctypes.POINTER(FkState),
+                                    ctypes.POINTER(FkStats)]),
+    "fk_step_update_nhoods": (ctypes.c_int, [_P, ctypes.POINTER(FkState)]),
+    "fk_step_bridgeview": (ctypes.c_int, [_P, ctypes.POINTER(FkParams), ctypes.POINTER(FkState),
+                                          ctypes.POINTER(FkStats)]),
+    "fk_step_stop_stats": (ctypes.c_int, [_P, ctypes.POINTER(FkParams), ctypes.POINTER(FkState), u32, _P, _P, u32,
+                                          ctypes.POINTER(f32), ctypes.POINTER(f32)]),
 }
 
 _lib = None

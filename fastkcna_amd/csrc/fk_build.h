@@ -186,6 +186,9 @@ void nnd_controls(BuildState &b);
 void nnd_run(BuildState &b);            // init + iterations (constructor)
 // recall over controls at K (all controls; per set into *per_set if given)
 float nnd_evaluate(BuildState &b, uint32_t K, std::vector<float> *per_set = nullptr);
+// stop-rule statistics of the current pools over given controls (one set)
+void nnd_stop_stats(BuildState &b, uint32_t K, uint32_t n_ctrl, const uint32_t *d_ctrl, const uint32_t *d_gt,
+                    uint32_t gt_stride, float *delta, float *recall);
 
 // Pass 1 of the local join in Gram form (fk_gram.hip); rechecked pairs are
 // added to *n_recheck (device counter)

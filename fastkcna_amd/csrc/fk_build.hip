@@ -667,6 +667,238 @@ fk_status fk_build_batch(fk_handle *h, fk_pg_type type, uint32_t n_sets, const u
     });
 }
 
+}  // extern "C"
+
+namespace fk {
+
+// A one-set build state of s->n nodes for a step entry point: parameters
+// shrunk as a build of n nodes shrinks them, counters and scratch allocated.
+static void step_begin(BuildState &b, fk_handle *h, const fk_params *p, const fk_state *s, fk_stats *st) {
+    FK_CHECK_ARG(s != nullptr, "null state");
+    FK_CHECK_ARG(s->n >= 2 && s->n <= h->ds.n_total, "state size must be in [2, resident rows]");
+    b.h = h;
+    b.s = h->stream;
+    b.ds = &h->ds;
+    b.N = s->n;
+    b.P = shrink_params(*p, s->n);
+    b.cap = b.P.MAXL;
+    b.st = st;
+    b.t0 = now_sec();
+    b.init_sets({0u, s->n});
+    b.tally.ensure(FK_T_SLOTS);
+    b.acc.alloc(8);
+    b.scal.alloc(16);
+    b.loopcount = s->loopcount;
+    if (b.P.flags & FK_FLAG_GRAM) gram_prepare(b.s, *b.ds, b.P.flags);
+}
+
+// Host rows of `cols` entries at pitch `hpitch` (elements) <-> device rows
+// at pitch `dpitch`.
+template <typename T>
+static void put_rows(BuildState &b, T *dev, size_t dpitch, const T *host, size_t hpitch, size_t cols) {
+    FK_HIP(hipMemcpy2DAsync(dev, dpitch * sizeof(T), host, hpitch * sizeof(T), cols * sizeof(T), b.N,
+                            hipMemcpyHostToDevice, b.s));
+}
+template <typename T>
+static void get_rows(BuildState &b, T *host, size_t hpitch, const T *dev, size_t dpitch, size_t cols) {
+    FK_HIP(hipMemcpy2DAsync(host, hpitch * sizeof(T), dev, dpitch * sizeof(T), cols * sizeof(T), b.N,
+                            hipMemcpyDeviceToHost, b.s));
+}
+
+static void check_lens(const uint32_t *len, uint32_t n, uint32_t cap, const char *what) {
+    FK_CHECK_ARG(len != nullptr, "null length array");
+    for (uint32_t i = 0; i < n; ++i) FK_CHECK_ARG(len[i] <= cap, what);
+}
+
+// Pool ids must be resident rows below n (the pruners and BridgeView read
+// their rows; the reference's pools hold only node ids).
+static void check_pool_ids(const fk_state *s) {
+    for (uint32_t i = 0; i < s->n; ++i)
+        for (uint32_t k = 0; k < s->pool_len[i]; ++k)
+            FK_CHECK_ARG((uint32_t)s->pool[(size_t)i * s->pool_cap + k] < s->n, "pool id out of range");
+}
+
+static void check_out_ids(const fk_state *s, const uint32_t *ids, const uint32_t *len) {
+    for (uint32_t i = 0; i < s->n; ++i)
+        for (uint32_t k = 0; k < len[i]; ++k)
+            FK_CHECK_ARG(ids[(size_t)i * s->out_cap + k] < s->n, "out-list id out of range");
+}
+
+}  // namespace fk
+
+extern "C" {
+
+fk_status fk_step_link(fk_handle *h, fk_pg_type type, const fk_params *p, fk_state *s, fk_stats *st) {
+    if (!h) return FK_E_INVALID;
+    return guard(h, [&] {
+        require_dataset(h);
+        check_params(p);
+        FK_CHECK_ARG(type == FK_PG_NSG || type == FK_PG_HNSW_LAYER || type == FK_PG_TAUMNG || type == FK_PG_ALPHAPG,
+                     "unknown graph type");
+        FK_CHECK_ARG(s && s->pool && s->pool_len && s->out_id && s->out_dist && s->out_len, "null state field");
+        FK_CHECK_ARG(s->out_cap >= p->nsg_R, "out_cap < nsg_R");
+        FK_CHECK_ARG(s->pool_cap >= 1 && s->pool_cap <= FK_POOL_MAX, "pool_cap must be in [1, 1024]");
+        check_lens(s->pool_len, s->n, s->pool_cap, "pool_len > pool_cap");
+        check_pool_ids(s);
+        select_device(h);
+        fk_stats local;
+        memset(&local, 0, sizeof(local));
+        BuildState b;
+        step_begin(b, h, p, s, st ? st : &local);
+        if (st) memset(st, 0, sizeof(*st));
+        const uint32_t N = b.N;
+        b.cap = s->pool_cap;  // the prune reads each pool's first pool_len entries
+        b.pool.alloc((size_t)N * b.cap);
+        b.cnt.alloc(N);
+        b.bcnt.alloc(N);
+        b.boff.alloc((size_t)N + 1);
+        b.pair_cnt.alloc(N);
+        b.ncap = b.P.nsg_R + FK_NSG_SLACK;
+        b.nsg_id.alloc((size_t)N * b.ncap);
+        b.nsg_d.alloc((size_t)N * b.ncap);
+        b.nsg_new.alloc((size_t)N * b.ncap);
+        b.nsg_len.alloc(N);
+        put_rows(b, b.pool.p, b.cap, s->pool, s->pool_cap, s->pool_cap);
+        FK_HIP(hipMemcpyAsync(b.cnt.p, s->pool_len, (size_t)N * 4, hipMemcpyHostToDevice, b.s));
+        link(b, type, 0);
+        // every list fits nsg_R after the link (kept + reverse edges, or the reversed prune)
+        get_rows(b, s->out_id, s->out_cap, b.nsg_id.p, b.ncap, b.P.nsg_R);
+        get_rows(b, s->out_dist, s->out_cap, b.nsg_d.p, b.ncap, b.P.nsg_R);
+        FK_HIP(hipMemcpyAsync(s->out_len, b.nsg_len.p, (size_t)N * 4, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+        for (uint32_t i = 0; i < N; ++i)  // slots past each list hold zeros, not device bytes
+            for (uint32_t k = s->out_len[i]; k < s->out_cap; ++k) {
+                s->out_id[(size_t)i * s->out_cap + k] = 0;
+                s->out_dist[(size_t)i * s->out_cap + k] = 0.f;
+            }
+    });
+}
+
+fk_status fk_step_update_nhoods(fk_handle *h, fk_state *s) {
+    if (!h) return FK_E_INVALID;
+    return guard(h, [&] {
+        require_dataset(h);
+        FK_CHECK_ARG(s && s->out_id && s->out_isnew && s->out_len && s->prev_id && s->prev_len, "null state field");
+        FK_CHECK_ARG(s->out_cap >= 1, "out_cap must be positive");
+        check_lens(s->out_len, s->n, s->out_cap, "out_len > out_cap");
+        check_lens(s->prev_len, s->n, s->out_cap, "prev_len > out_cap");
+        select_device(h);
+        fk_params p;
+        memset(&p, 0, sizeof(p));
+        p.K = p.L = p.S = p.nsg_R = p.search_L = p.search_K = 1;
+        fk_stats local;
+        memset(&local, 0, sizeof(local));
+        BuildState b;
+        step_begin(b, h, &p, s, &local);
+        const uint32_t N = b.N;
+        b.ncap = s->out_cap;
+        b.nsg_id.alloc((size_t)N * b.ncap);
+        b.nsg_new.alloc((size_t)N * b.ncap);
+        b.old_id.alloc((size_t)N * b.ncap);
+        b.nsg_len.alloc(N);
+        b.old_len.alloc(N);
+        put_rows(b, b.nsg_id.p, b.ncap, s->out_id, s->out_cap, s->out_cap);
+        put_rows(b, b.nsg_new.p, b.ncap, s->out_isnew, s->out_cap, s->out_cap);  // slots past out_len kept
+        put_rows(b, b.old_id.p, b.ncap, s->prev_id, s->out_cap, s->out_cap);
+        FK_HIP(hipMemcpyAsync(b.nsg_len.p, s->out_len, (size_t)N * 4, hipMemcpyHostToDevice, b.s));
+        FK_HIP(hipMemcpyAsync(b.old_len.p, s->prev_len, (size_t)N * 4, hipMemcpyHostToDevice, b.s));
+        update_nhoods(b);
+        get_rows(b, s->out_isnew, s->out_cap, b.nsg_new.p, b.ncap, s->out_cap);
+        get_rows(b, s->prev_id, s->out_cap, b.old_id.p, b.ncap, s->out_cap);
+        FK_HIP(hipMemcpyAsync(s->prev_len, b.old_len.p, (size_t)N * 4, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+    });
+}
+
+fk_status fk_step_bridgeview(fk_handle *h, const fk_params *p, fk_state *s, fk_stats *st) {
+    if (!h) return FK_E_INVALID;
+    return guard(h, [&] {
+        require_dataset(h);
+        check_params(p);
+        FK_CHECK_ARG(s && s->pool && s->pool_len && s->pool_flags && s->out_id && s->out_isnew && s->out_len,
+                     "null state field");
+        FK_CHECK_ARG(s->loopcount >= 1, "BridgeView runs after the first loop (loopcount >= 1)");
+        const BuildParams P = shrink_params(*p, s->n);
+        FK_CHECK_ARG(s->pool_cap == P.MAXL, "pool_cap must equal max(L, K, search_L, search_K)");
+        FK_CHECK_ARG(P.MAXL <= FK_POOL_MAX, "max(L, K, search_L, search_K) above 1024 is not supported");
+        FK_CHECK_ARG(s->out_cap >= 1, "out_cap must be positive");
+        check_lens(s->pool_len, s->n, s->pool_cap, "pool_len > pool_cap");
+        check_lens(s->out_len, s->n, s->out_cap, "out_len > out_cap");
+        check_pool_ids(s);
+        check_out_ids(s, s->out_id, s->out_len);
+        select_device(h);
+        fk_stats local;
+        memset(&local, 0, sizeof(local));
+        BuildState b;
+        if (st) memset(st, 0, sizeof(*st));
+        step_begin(b, h, p, s, st ? st : &local);
+        const uint32_t N = b.N;
+        b.pool.alloc((size_t)N * b.cap);
+        b.pflg.alloc((size_t)N * b.cap);
+        b.cnt.alloc(N);
+        b.ncap = s->out_cap;
+        b.nsg_id.alloc((size_t)N * b.ncap);
+        b.nsg_new.alloc((size_t)N * b.ncap);
+        b.nsg_len.alloc(N);
+        put_rows(b, b.pool.p, b.cap, s->pool, s->pool_cap, s->pool_cap);
+        put_rows(b, b.pflg.p, b.cap, s->pool_flags, s->pool_cap, s->pool_cap);
+        FK_HIP(hipMemcpyAsync(b.cnt.p, s->pool_len, (size_t)N * 4, hipMemcpyHostToDevice, b.s));
+        put_rows(b, b.nsg_id.p, b.ncap, s->out_id, s->out_cap, s->out_cap);
+        put_rows(b, b.nsg_new.p, b.ncap, s->out_isnew, s->out_cap, s->out_cap);
+        FK_HIP(hipMemcpyAsync(b.nsg_len.p, s->out_len, (size_t)N * 4, hipMemcpyHostToDevice, b.s));
+        bridgeview(b);
+        // the pass records its rounds and search_cc at index loopcount - 1;
+        // the step reports them at index 0
+        const uint32_t li = b.loopcount - 1;
+        if (li > 0 && li < FK_MAX_LOOPS) {
+            b.st->bv_rounds[0] = b.st->bv_rounds[li];
+            b.st->search_cc[0] = b.st->search_cc[li];
+            b.st->bv_rounds[li] = 0;
+            b.st->search_cc[li] = 0.f;
+        }
+        get_rows(b, s->pool, s->pool_cap, b.pool.p, b.cap, s->pool_cap);
+        get_rows(b, s->pool_flags, s->pool_cap, b.pflg.p, b.cap, s->pool_cap);
+        FK_HIP(hipMemcpyAsync(s->pool_len, b.cnt.p, (size_t)N * 4, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+    });
+}
+
+fk_status fk_step_stop_stats(fk_handle *h, const fk_params *p, const fk_state *s, uint32_t n_ctrl,
+                             const uint32_t *ctrl_ids, const uint32_t *gt, uint32_t gt_stride, float *delta,
+                             float *recall) {
+    if (!h) return FK_E_INVALID;
+    return guard(h, [&] {
+        require_dataset(h);
+        check_params(p);
+        FK_CHECK_ARG(s && s->pool && s->pool_len && s->pool_flags && delta && recall, "null argument");
+        FK_CHECK_ARG(n_ctrl == 0 || (ctrl_ids && gt), "null controls");
+        FK_CHECK_ARG(s->pool_cap >= 1, "pool_cap must be positive");
+        check_lens(s->pool_len, s->n, s->pool_cap, "pool_len > pool_cap");
+        const BuildParams P = shrink_params(*p, s->n);
+        FK_CHECK_ARG(n_ctrl == 0 || gt_stride >= P.K, "gt_stride < K");
+        for (uint32_t c = 0; c < n_ctrl; ++c) FK_CHECK_ARG(ctrl_ids[c] < s->n, "control id out of range");
+        select_device(h);
+        fk_stats local;
+        memset(&local, 0, sizeof(local));
+        BuildState b;
+        step_begin(b, h, p, s, &local);
+        const uint32_t N = b.N;
+        b.cap = s->pool_cap;
+        b.pool.alloc((size_t)N * b.cap);
+        b.pflg.alloc((size_t)N * b.cap);
+        b.cnt.alloc(N);
+        put_rows(b, b.pool.p, b.cap, s->pool, s->pool_cap, s->pool_cap);
+        put_rows(b, b.pflg.p, b.cap, s->pool_flags, s->pool_cap, s->pool_cap);
+        FK_HIP(hipMemcpyAsync(b.cnt.p, s->pool_len, (size_t)N * 4, hipMemcpyHostToDevice, b.s));
+        DevBuf<uint32_t> dc(n_ctrl), dg((size_t)n_ctrl * gt_stride);
+        if (n_ctrl) {
+            FK_HIP(hipMemcpyAsync(dc.p, ctrl_ids, (size_t)n_ctrl * 4, hipMemcpyHostToDevice, b.s));
+            FK_HIP(hipMemcpyAsync(dg.p, gt, (size_t)n_ctrl * gt_stride * 4, hipMemcpyHostToDevice, b.s));
+        }
+        nnd_stop_stats(b, b.P.K, n_ctrl, dc.p, dg.p, gt_stride, delta, recall);
+    });
+}
+
 fk_status fk_nndescent(fk_handle *h, uint32_t n_used, const fk_params *p, uint64_t *pools, uint32_t *pool_len,
                        uint32_t pool_stride, fk_stats *stats) {
     if (!h) return FK_E_INVALID;

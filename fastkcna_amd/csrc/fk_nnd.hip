@@ -980,6 +980,36 @@ float nnd_evaluate(BuildState &b, uint32_t K, std::vector<float> *per_set) {
     return (float)((double)all / K / b.n_ctrl);
 }
 
+// The stop rule's statistics of the current pools (one set): delta at K over
+// all nodes, recall at K over n_ctrl controls d_ctrl with exact neighbours
+// d_gt (gt_stride ids per control).  Same kernels and arithmetic as the
+// NN-descent loop.
+void nnd_stop_stats(BuildState &b, uint32_t K, uint32_t n_ctrl, const uint32_t *d_ctrl, const uint32_t *d_gt,
+                    uint32_t gt_stride, float *delta, float *recall) {
+    FK_CHECK_ARG(K >= 1 && K <= FK_POOL_MAX * 2, "K out of range for the stop statistics");
+    FK_HIP(hipMemsetAsync(b.set_acc.p, 0, 8, b.s));
+    hipLaunchKernelGGL(k_delta, dim3(reduce_grid(b.N)), dim3(1024), 0, b.s, b.pflg.p, b.cnt.p, b.N, b.cap, K,
+                       nullptr, b.set_acc.p);
+    FK_LAUNCH_CHECK();
+    unsigned long long fl = 0;
+    FK_HIP(hipMemcpyAsync(&fl, b.set_acc.p, 8, hipMemcpyDeviceToHost, b.s));
+    std::vector<uint32_t> f(n_ctrl);
+    if (n_ctrl) {
+        DevBuf<uint32_t> found(n_ctrl);
+        hipLaunchKernelGGL(k_recall, dim3(n_ctrl), dim3(256), 0, b.s, b.pool.p, b.cnt.p, b.cap, d_ctrl, d_gt,
+                           gt_stride, K, found.p);
+        FK_LAUNCH_CHECK();
+        FK_HIP(hipMemcpyAsync(f.data(), found.p, (size_t)n_ctrl * 4, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+    } else {
+        b.sync();
+    }
+    *delta = (float)((double)fl / K / b.N);
+    unsigned long long all = 0;
+    for (uint32_t c = 0; c < n_ctrl; ++c) all += f[c];
+    *recall = n_ctrl ? (float)((double)all / K / n_ctrl) : 0.f;
+}
+
 // Exclusive scan of n counts into out[0..n], out[n] = total; returns total.
 static uint64_t scan_to_host(BuildState &b, const uint32_t *in, uint64_t *out, size_t n) {
     exclusive_scan_u32_to_u64(b.s, in, out, n, out + n, b.scan_tmp);

@@ -218,6 +218,43 @@ class Device:
                 self.lib.fk_graph_free(ctypes.byref(g))
         return out, st
 
+    # -- step entry points (one step of the loop on a supplied state) -----
+    def step_link(self, pg_type, params, state):
+        """One Link step on `state` (a StepState): prune, reverse edges,
+        reversed prune; fills the out-lists; returns the statistics."""
+        st = FkStats()
+        p = _ffi.params_struct(params)
+        s = state.struct()
+        self._check(self.lib.fk_step_link(self.h, pg_type, ctypes.byref(p), ctypes.byref(s), ctypes.byref(st)),
+                    "fk_step_link")
+        return st.to_dict()
+
+    def step_update_nhoods(self, state):
+        s = state.struct()
+        self._check(self.lib.fk_step_update_nhoods(self.h, ctypes.byref(s)), "fk_step_update_nhoods")
+
+    def step_bridgeview(self, params, state):
+        """One BridgeView pass on `state` at state.loopcount; updates its
+        pools and flags; returns the statistics."""
+        st = FkStats()
+        p = _ffi.params_struct(params)
+        s = state.struct()
+        self._check(self.lib.fk_step_bridgeview(self.h, ctypes.byref(p), ctypes.byref(s), ctypes.byref(st)),
+                    "fk_step_bridgeview")
+        return st.to_dict()
+
+    def step_stop_stats(self, params, state, ctrl_ids, gt):
+        """(delta, recall) of the stop rule on the state's pools."""
+        ctrl_ids = _as(ctrl_ids, np.uint32)
+        gt = _as(gt, np.uint32)
+        delta, recall = ctypes.c_float(), ctypes.c_float()
+        p = _ffi.params_struct(params)
+        s = state.struct()
+        self._check(self.lib.fk_step_stop_stats(self.h, ctypes.byref(p), ctypes.byref(s), len(ctrl_ids),
+                                                ptr(ctrl_ids), ptr(gt), gt.shape[1] if gt.ndim == 2 else 0,
+                                                ctypes.byref(delta), ctypes.byref(recall)), "fk_step_stop_stats")
+        return delta.value, recall.value
+
     def search(self, offsets, ids, ep, queries, search_L, K):
         offsets = _as(offsets, np.uint64)
         ids = _as(ids, np.uint32)

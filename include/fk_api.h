@@ -301,6 +301,71 @@ fk_status fk_knn_bruteforce(fk_handle *h, const float *queries, uint32_t n_q, ui
                             uint32_t K, const uint32_t *self_ids, uint32_t *out_ids,
                             float *out_dists);
 
+/* ---------------------------------------------------------------------------
+ * Step entry points: one stateful step of the build's loop run on a state the
+ * caller supplies, for parity tests of each step against the oracle.  The
+ * reference runs these steps only inside KGraphImpl::buildFast*; each entry
+ * point names the reference code it replaces.
+ *
+ * State of n nodes in host memory.  Node i's candidate pool is
+ * pool[i * pool_cap + k], k < pool_len[i]: packed keys (float distance bits
+ * << 32 | id) sorted ascending, with pool_flags[i * pool_cap + k] (bit 0:
+ * flag, not yet expanded; bit 1: isnew).  Its out-list is out_id / out_dist /
+ * out_isnew [i * out_cap + k], k < out_len[i]; the previous loop's out-list
+ * (the reference's old_nsg) is prev_id[i * out_cap + k], k < prev_len[i].
+ * loopcount = prune loops completed (the reference's loopcount).  Each entry
+ * point reads and writes only the fields it names; rows are the resident
+ * dataset, the state covers its first n rows (the oracle's size()).
+ * --------------------------------------------------------------------------- */
+typedef struct {
+    uint32_t n, pool_cap, out_cap, loopcount;
+    uint64_t *pool;
+    uint32_t *pool_len;
+    uint8_t *pool_flags;
+    uint32_t *out_id;
+    float *out_dist;
+    uint8_t *out_isnew;
+    uint32_t *out_len;
+    uint32_t *prev_id;
+    uint32_t *prev_len;
+} fk_state;
+
+/* One Link step: nsg_Link / taumg_Link / alphapg_Link
+ * (src/kgraph.cpp:2087-2143, 3085-3141, 3424-3480) with the full prune rule
+ * for every node (*_sync_prune, :1809-1851, 2833-2876, 3157-3203), the
+ * reverse edges of InterInsert (:2051-2084) and the reversed prune of lists
+ * above nsg_R (:2012-2048, 3038-3083, 3374-3422).  Reads pool, pool_len;
+ * writes out_id, out_dist, out_len (out_cap >= nsg_R).  A list that fits
+ * nsg_R is the kept prefix followed by its new reverse edges in (dist, id)
+ * order.  st (optional) receives d_prune, d_prune_rev and prune_cc[0]. */
+fk_status fk_step_link(fk_handle *h, fk_pg_type type, const fk_params *p, fk_state *s, fk_stats *st);
+
+/* UpdateNhoods (src/kgraph.cpp:3537-3584): out_isnew[k] = 1 when loopcount is
+ * 0 or out_id[k] is not in the previous out-list, else 0; then the previous
+ * out-list becomes the out-list.  Reads out_id, out_len, prev_id, prev_len;
+ * writes out_isnew, prev_id, prev_len (prev capacity out_cap). */
+fk_status fk_step_update_nhoods(fk_handle *h, fk_state *s);
+
+/* One BridgeView pass (src/kgraph.cpp:2478-2663) at s->loopcount >= 1: the
+ * window initialisation of BridgeView_init, then update / join rounds until
+ * the 50 % rule ends them.  Reads out_id, out_isnew, out_len; updates pool,
+ * pool_len and pool_flags (pool_cap must equal max(L, K, search_L, search_K)
+ * after the reference's shrinking for n nodes).  st (optional) receives d_bv
+ * (every pair the reference evaluates), d_bv_computed, bv_rounds[0] and
+ * search_cc[0]. */
+fk_status fk_step_bridgeview(fk_handle *h, const fk_params *p, fk_state *s, fk_stats *st);
+
+/* The NN-descent stop rule's statistics (src/kgraph.cpp:1365-1428,
+ * EvaluateDelta :194-206, EvaluateRecall :113-169) of the state's pools:
+ * *delta = mean over nodes of (flagged entries among the first min(K,
+ * pool_len) entries) / K; *recall = mean over the n_ctrl controls of
+ * |ids of pool[0, min(K, pool_len)) ∩ gt[c * gt_stride + (0 .. K))| / K, with
+ * gt the exact neighbours of control ctrl_ids[c] (gt_stride >= K).  The build
+ * stops NN-descent when delta <= p->delta or recall >= p->recall. */
+fk_status fk_step_stop_stats(fk_handle *h, const fk_params *p, const fk_state *s, uint32_t n_ctrl,
+                             const uint32_t *ctrl_ids, const uint32_t *gt, uint32_t gt_stride, float *delta,
+                             float *recall);
+
 /* Beam search of n_q host queries over a CSR graph (the layout fk_build
  * returns) from entry point ep with beam width search_L; writes the best K
  * ids per query. */

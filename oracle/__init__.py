@@ -58,6 +58,15 @@ def _load():
         lib.fko_nndescent.restype = ctypes.c_int
         lib.fko_nndescent.argtypes = [P, ctypes.c_size_t, ctypes.c_uint, ctypes.c_uint, P, ctypes.c_int, P, P,
                                       ctypes.c_uint, P]
+        PS = ctypes.c_void_p
+        lib.fko_step_link.restype = ctypes.c_int
+        lib.fko_step_link.argtypes = [ctypes.c_int, P, ctypes.c_size_t, ctypes.c_uint, ctypes.c_uint, P, PS, P]
+        lib.fko_step_update_nhoods.restype = ctypes.c_int
+        lib.fko_step_update_nhoods.argtypes = [PS]
+        lib.fko_step_bridgeview.restype = ctypes.c_int
+        lib.fko_step_bridgeview.argtypes = [P, ctypes.c_size_t, ctypes.c_uint, ctypes.c_uint, P, PS, P]
+        lib.fko_step_stop_stats.restype = None
+        lib.fko_step_stop_stats.argtypes = [P, PS, ctypes.c_uint, P, P, ctypes.c_uint, P, P]
         lib.fko_free.restype = None
         lib.fko_free.argtypes = [P]
         _lib = lib
@@ -209,3 +218,47 @@ def unpack(keys):
     ids = (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32)
     d = (keys >> np.uint64(32)).astype(np.uint32).view(np.float32)
     return d, ids
+
+
+# -- step entry points: one step of the loop on a supplied state -------------
+# `state` is a fastkcna_amd._ffi.StepState (plain numpy arrays; the structure
+# layout is shared with the library, no library call is made).
+def step_link(pg_type, rows, params, state):
+    """Link step (full prune rule, InterInsert, reversed prune) on the state;
+    appended reverse edges of lists within nsg_R in (dist, id) order."""
+    rows = np.ascontiguousarray(rows, np.float32)
+    st = _stats()
+    ps = _params(params)
+    s = state.struct()
+    rc = _load().fko_step_link(pg_type, _p(rows), rows.strides[0], rows.shape[0], rows.shape[1], ctypes.byref(ps),
+                               ctypes.byref(s), ctypes.byref(st))
+    assert rc == 0, "out-list above out_cap"
+    return st.to_dict()
+
+
+def step_update_nhoods(state):
+    rc = _load().fko_step_update_nhoods(ctypes.byref(state.struct()))
+    assert rc == 0
+
+
+def step_bridgeview(rows, params, state):
+    """One BridgeView pass on the state at state.loopcount (pools, lengths and
+    flags updated in place)."""
+    rows = np.ascontiguousarray(rows, np.float32)
+    st = _stats()
+    ps = _params(params)
+    s = state.struct()
+    rc = _load().fko_step_bridgeview(_p(rows), rows.strides[0], rows.shape[0], rows.shape[1], ctypes.byref(ps),
+                                     ctypes.byref(s), ctypes.byref(st))
+    assert rc == 0, "pool_cap must equal MAXL"
+    return st.to_dict()
+
+
+def step_stop_stats(params, state, ctrl_ids, gt):
+    ctrl_ids = np.ascontiguousarray(ctrl_ids, np.uint32)
+    gt = np.ascontiguousarray(gt, np.uint32)
+    delta, recall = ctypes.c_float(), ctypes.c_float()
+    ps = _params(params)
+    _load().fko_step_stop_stats(ctypes.byref(ps), ctypes.byref(state.struct()), len(ctrl_ids), _p(ctrl_ids), _p(gt),
+                                gt.shape[1] if gt.ndim == 2 else 0, ctypes.byref(delta), ctypes.byref(recall))
+    return delta.value, recall.value

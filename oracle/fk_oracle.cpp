@@ -263,6 +263,7 @@ struct Builder {
     bool full_rule_only = false;   // never use *_sync_prune2
     uint64_t prune2_diff = 0;      // nodes where sync_prune2 != sync_prune
     uint64_t prune2_nodes = 0;     // nodes where both were compared
+    vector<unsigned> kept_len;     // per node: out-list length after the prune, before InterInsert
     bool check_prune2 = false;
     RuleP rp{};
     bool verbose = false;
@@ -636,7 +637,11 @@ struct Builder {
         st->t_prune += t1 - t0;
         vector<std::mutex> locks(N_);
         uint64_t inter_d = 0;
-        for (unsigned n = 0; n < N_; ++n) inter_d += nh[n].nn_nsg.size();
+        kept_len.resize(N_);
+        for (unsigned n = 0; n < N_; ++n) {
+            kept_len[n] = (unsigned)nh[n].nn_nsg.size();
+            inter_d += nh[n].nn_nsg.size();
+        }
 #pragma omp parallel for schedule(dynamic, 100)
         for (unsigned n = 0; n < N_; ++n) inter_insert(n, locks);
         std::fill(pruneAgain.begin(), pruneAgain.end(), 0);
@@ -1157,6 +1162,157 @@ int fko_nndescent(const float *rows, size_t stride_bytes, unsigned n, unsigned d
         for (unsigned k = 0; k < pool_len[i]; ++k) pools[(size_t)i * pool_stride + k] = pack(b.nh[i].pool[k].dist, b.nh[i].pool[k].id);
     }
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Step entry points: one step of the build's loop on a supplied state (the
+// layout of fk_state in fk_api.h), run by the restated code above.
+// ---------------------------------------------------------------------------
+static Nb entry_of(uint64_t key, uint8_t flags) {
+    const uint32_t bits = (uint32_t)(key >> 32);
+    float d;
+    memcpy(&d, &bits, 4);
+    Nb e((uint32_t)key, d, (flags & 1u) != 0);
+    e.isnew = (flags & 2u) != 0;
+    return e;
+}
+
+// Pools of the state into the builder's Nhoods, each sized cap + 1 (the
+// overflow slot and unfilled slots default-constructed: id ~0, FLT_MAX).
+static void load_pools(Builder &b, const fk_state *s, unsigned cap) {
+    for (unsigned i = 0; i < s->n; ++i) {
+        auto &h = b.nh[i];
+        const unsigned len = s->pool_len[i];
+        h.pool.assign(std::max(cap, len) + 1, Nb());
+        for (unsigned k = 0; k < len; ++k)
+            h.pool[k] = entry_of(s->pool[(size_t)i * s->pool_cap + k],
+                                 s->pool_flags ? s->pool_flags[(size_t)i * s->pool_cap + k] : (uint8_t)1);
+        h.L = len;
+    }
+}
+
+// Link step: the full rule for every node, InterInsert, reversed prune.  A
+// list left at or below nsg_R has its appended reverse edges (appended in
+// thread order) put in (dist, id) order, the build's canonical order.
+int fko_step_link(int type, const float *rows, size_t stride_bytes, unsigned n_total, unsigned d,
+                  const fk_params *p, fk_state *s, fk_stats *st_out) {
+    Rows r;
+    r.load(rows, stride_bytes, n_total, d);
+    r.oracle_n = s->n;
+    fk_stats local;
+    fk_stats *st = st_out ? st_out : &local;
+    memset(st, 0, sizeof(*st));
+    Builder b(r, *p, st);
+    b.rp = RuleP{rule_of(type), p->tau, alpha_thr(p->angle)};
+    b.shrink();
+    b.full_rule_only = true;
+    b.pruneAgain.assign(s->n, 1);
+    load_pools(b, s, s->pool_cap);
+    b.link(0);
+    for (unsigned i = 0; i < s->n; ++i) {
+        auto &out = b.nh[i].nn_nsg;
+        if (!b.pruneAgain[i])
+            std::sort(out.begin() + b.kept_len[i], out.end(), [](const Nb &x, const Nb &y) {
+                return x.dist < y.dist || (x.dist == y.dist && x.id < y.id);
+            });
+        if (out.size() > s->out_cap) return 1;
+        s->out_len[i] = (uint32_t)out.size();
+        for (size_t k = 0; k < out.size(); ++k) {
+            s->out_id[(size_t)i * s->out_cap + k] = out[k].id;
+            s->out_dist[(size_t)i * s->out_cap + k] = out[k].dist;
+        }
+    }
+    return 0;
+}
+
+// UpdateNhoods on the state's out-lists and previous out-lists.
+int fko_step_update_nhoods(fk_state *s) {
+    Rows r;  // UpdateNhoods reads no rows
+    r.oracle_n = s->n;
+    fk_params p;
+    memset(&p, 0, sizeof(p));
+    fk_stats st;
+    memset(&st, 0, sizeof(st));
+    Builder b(r, p, &st);
+    b.loopcount = s->loopcount;
+    for (unsigned i = 0; i < s->n; ++i) {
+        auto &h = b.nh[i];
+        for (unsigned k = 0; k < s->out_len[i]; ++k) h.nn_nsg.push_back(Nb(s->out_id[(size_t)i * s->out_cap + k], 0.f));
+        for (unsigned k = 0; k < s->prev_len[i]; ++k) h.old_nsg.push_back(s->prev_id[(size_t)i * s->out_cap + k]);
+    }
+    b.update_nhoods();
+    for (unsigned i = 0; i < s->n; ++i) {
+        const auto &h = b.nh[i];
+        for (size_t k = 0; k < h.nn_nsg.size(); ++k) s->out_isnew[(size_t)i * s->out_cap + k] = h.nn_nsg[k].isnew;
+        if (h.old_nsg.size() > s->out_cap) return 1;
+        s->prev_len[i] = (uint32_t)h.old_nsg.size();
+        for (size_t k = 0; k < h.old_nsg.size(); ++k) s->prev_id[(size_t)i * s->out_cap + k] = h.old_nsg[k];
+    }
+    return 0;
+}
+
+// One BridgeView pass at s->loopcount.  Returns 1 when pool_cap is not the
+// build's MAXL.
+int fko_step_bridgeview(const float *rows, size_t stride_bytes, unsigned n_total, unsigned d, const fk_params *p,
+                        fk_state *s, fk_stats *st_out) {
+    Rows r;
+    r.load(rows, stride_bytes, n_total, d);
+    r.oracle_n = s->n;
+    fk_stats local;
+    fk_stats *st = st_out ? st_out : &local;
+    memset(st, 0, sizeof(*st));
+    Builder b(r, *p, st);
+    b.shrink();
+    if (s->pool_cap != b.MAXL) return 1;
+    load_pools(b, s, b.MAXL);
+    for (unsigned i = 0; i < s->n; ++i) {
+        auto &h = b.nh[i];
+        for (unsigned k = 0; k < s->out_len[i]; ++k) {
+            Nb e(s->out_id[(size_t)i * s->out_cap + k], 0.f, false);
+            e.isnew = s->out_isnew[(size_t)i * s->out_cap + k] != 0;
+            h.nn_nsg.push_back(e);
+        }
+    }
+    b.loopcount = s->loopcount;
+    b.bridgeview(0);
+    for (unsigned i = 0; i < s->n; ++i) {
+        const auto &h = b.nh[i];
+        s->pool_len[i] = h.L;
+        for (unsigned k = 0; k < h.L; ++k) {
+            s->pool[(size_t)i * s->pool_cap + k] = pack(h.pool[k].dist, h.pool[k].id);
+            s->pool_flags[(size_t)i * s->pool_cap + k] = (uint8_t)((h.pool[k].flag ? 1u : 0u) | (h.pool[k].isnew ? 2u : 0u));
+        }
+    }
+    return 0;
+}
+
+// The stop rule's delta and recall of the state's pools (the expressions of
+// Builder::nndescent and Builder::eval_recall; delta counts the flags of the
+// first min(K, pool_len) entries).
+void fko_step_stop_stats(const fk_params *p, const fk_state *s, unsigned n_ctrl, const uint32_t *ctrl_ids,
+                         const uint32_t *gt, unsigned gt_stride, float *delta, float *recall) {
+    Rows r;
+    r.oracle_n = s->n;
+    fk_stats st;
+    memset(&st, 0, sizeof(st));
+    Builder b(r, *p, &st);
+    b.shrink();
+    load_pools(b, s, s->pool_cap);
+    const unsigned K = b.P.K;
+    double dsum = 0;
+    for (auto &h : b.nh) {
+        unsigned c = 0;
+        const unsigned lim = std::min(K, h.L);
+        for (unsigned i = 0; i < lim; ++i) if (h.pool[i].flag) ++c;
+        dsum += (double)c / K;
+    }
+    *delta = (float)(dsum / s->n);
+    b.controls.assign(n_ctrl, Control());
+    for (unsigned c = 0; c < n_ctrl; ++c) {
+        b.controls[c].id = ctrl_ids[c];
+        for (unsigned k = 0; k < K; ++k) b.controls[c].nbrs.push_back(Nb(gt[(size_t)c * gt_stride + k], 0.f));
+    }
+    *recall = b.eval_recall(K);
 }
 
 void fko_free(void *p) { free(p); }

@@ -8,9 +8,33 @@ C4  256 sets, n log-uniform in [2e3, 2e5], d in {96, 128, 256}, 16 unit centres,
 C5  10,000,000 x 96, 4,096 unit centres, sigma 0.25, unit-normalised
 
 Queries are drawn from the same mixture after the base rows.  Generation is
-chunked so that peak host memory stays near the output size.
+chunked so that peak host memory stays near the output size.  The float
+mixtures (C3-C5) are drawn in float32, chunk k of FLOAT_CHUNK rows from its
+own stream SeedSequence(seed, spawn_key=(k,)) (centres from the stream of
+`seed` itself), so the rows do not depend on how many threads fill them.
 """
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
+
+FLOAT_CHUNK = 1 << 14
+
+
+def _chunked_rows(n, d, seed, make_chunk, chunk=FLOAT_CHUNK):
+    """n x d float32 rows; chunk k (rows [k chunk, ...)) is make_chunk(rng_k,
+    out_view) with rng_k seeded by SeedSequence(seed, spawn_key=(k,))."""
+    out = np.empty((n, d), dtype=np.float32)
+    starts = list(range(0, n, chunk))
+
+    def fill(k):
+        s = starts[k]
+        make_chunk(np.random.default_rng(np.random.SeedSequence(seed, spawn_key=(k,))), out[s:s + chunk])
+
+    workers = max(1, min(16, os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        list(ex.map(fill, range(len(starts))))
+    return out
 
 
 def integer_mixture(n, d, n_centers, seed, sigma=20.0, lo=0.0, hi=128.0, chunk=1 << 18):
@@ -26,31 +50,35 @@ def integer_mixture(n, d, n_centers, seed, sigma=20.0, lo=0.0, hi=128.0, chunk=1
     return out
 
 
-def float_mixture(n, d, n_centers, seed, sigma=0.05, chunk=1 << 16):
-    rng = np.random.default_rng(seed)
-    centers = rng.uniform(0.0, 1.0, size=(n_centers, d))
-    out = np.empty((n, d), dtype=np.float32)
-    for s in range(0, n, chunk):
-        m = min(chunk, n - s)
-        lab = rng.integers(0, n_centers, size=m)
-        x = centers[lab] + rng.normal(0.0, sigma, size=(m, d))
-        np.maximum(x, 0.0, out=x)
-        out[s:s + m] = x
-    return out
+def float_mixture(n, d, n_centers, seed, sigma=0.05):
+    """Mixture of n_centers isotropic Gaussians (sigma) with centres uniform
+    in [0, 1]^d, clamped at 0."""
+    centers = np.random.default_rng(seed).uniform(0.0, 1.0, size=(n_centers, d)).astype(np.float32)
+
+    def make(rng, o):
+        lab = rng.integers(0, n_centers, size=len(o))
+        rng.standard_normal(out=o, dtype=np.float32)
+        o *= np.float32(sigma)
+        o += centers[lab]
+        np.maximum(o, 0.0, out=o)
+
+    return _chunked_rows(n, d, seed, make)
 
 
-def unit_mixture(n, d, n_centers, seed, sigma, chunk=1 << 16):
-    rng = np.random.default_rng(seed)
-    c = rng.normal(size=(n_centers, d))
-    c /= np.linalg.norm(c, axis=1, keepdims=True)
-    out = np.empty((n, d), dtype=np.float32)
-    for s in range(0, n, chunk):
-        m = min(chunk, n - s)
-        lab = rng.integers(0, n_centers, size=m)
-        x = c[lab] + rng.normal(0.0, sigma, size=(m, d))
-        x /= np.linalg.norm(x, axis=1, keepdims=True)
-        out[s:s + m] = x
-    return out
+def unit_mixture(n, d, n_centers, seed, sigma):
+    """Unit-normalised mixture: centres N(0, I) normalised, rows c + N(0,
+    sigma^2 I) normalised."""
+    c = np.random.default_rng(seed).normal(size=(n_centers, d))
+    c = (c / np.linalg.norm(c, axis=1, keepdims=True)).astype(np.float32)
+
+    def make(rng, o):
+        lab = rng.integers(0, n_centers, size=len(o))
+        rng.standard_normal(out=o, dtype=np.float32)
+        o *= np.float32(sigma)
+        o += c[lab]
+        o /= np.linalg.norm(o, axis=1, keepdims=True)
+
+    return _chunked_rows(n, d, seed, make)
 
 
 def c1(n=100_000, n_queries=1_000):
@@ -101,3 +129,11 @@ def params_c4(IndexParams):
     # "alpha = 1.2" read as the reference's --angle in degrees (its only
     # alpha-PG parameter); search_K = K as for the other configurations
     return IndexParams(K=100, L=100, S=10, R=50, iterations=6, nsg_R=32, search_L=40, search_K=100, angle=1.2)
+
+
+def params_c5(IndexParams, level0=True):
+    # fasthnsw -K 200 -L 200 -S 10 -R 100 -I 6 --nsg_R 32: level 0 is built
+    # with nsg_R doubled; search_K = K as for the other configurations
+    return IndexParams(K=200, L=200, S=10, R=100, iterations=6, nsg_R=64 if level0 else 32, search_L=40,
+                       search_K=200, angle=60)
+

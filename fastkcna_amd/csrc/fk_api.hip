@@ -35,9 +35,12 @@ int rule_of(fk_pg_type t) {
 void select_device(fk_handle *h) { FK_HIP(hipSetDevice(h->device)); }
 
 // Host rows (any stride) -> device rows of dpad floats, zero padded, through
-// two pinned staging buffers so packing and PCIe transfer overlap.
+// two pinned staging buffers so packing and PCIe transfer overlap.  With
+// colsum, the rows are also added into colsum[0, d) in float, one row after
+// the other (the summation order of MatrixOracle::Calcenter,
+// src/kgraph-data.h:413-428).
 void upload_rows(fk_handle *h, float *dst, uint32_t dpad, const float *src, size_t src_stride_bytes, uint32_t n,
-                 uint32_t d) {
+                 uint32_t d, float *colsum) {
     const size_t row_bytes = (size_t)dpad * sizeof(float);
     const size_t chunk_rows = std::max<size_t>(1, ((size_t)32 << 20) / row_bytes);
     float *stage[2] = {nullptr, nullptr};
@@ -64,6 +67,8 @@ void upload_rows(fk_handle *h, float *dst, uint32_t dpad, const float *src, size
             float *o = stage[k] + r * dpad;
             memcpy(o, (const char *)src + (r0 + r) * src_stride_bytes, (size_t)d * sizeof(float));
             if (dpad != d) memset(o + d, 0, (size_t)(dpad - d) * sizeof(float));
+            if (colsum)
+                for (uint32_t c = 0; c < d; ++c) colsum[c] += o[c];
         }
         FK_HIP(hipMemcpyAsync(dst + r0 * dpad, stage[k], m * row_bytes, hipMemcpyHostToDevice, h->stream));
         FK_HIP(hipEventRecord(done[k], h->stream));
@@ -278,6 +283,7 @@ void fk_close(fk_handle *h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     h->ds.rows.release();
     h->ds.chains.release();
+    h->ds.center.release();
     h->counters.buf.release();
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -295,6 +301,7 @@ fk_status fk_dataset_upload(fk_handle *h, const float *rows, size_t row_stride_b
         Dataset &ds = h->ds;
         ds.rows.release();
         ds.chains.release();
+        ds.center.release();
         ds.gram_nrm.release();
         ds.chains8.release();
         ds.norms8.release();
@@ -303,7 +310,14 @@ fk_status fk_dataset_upload(fk_handle *h, const float *rows, size_t row_stride_b
         ds.n_total = 0;
         const uint32_t dpad = (d + 7u) & ~7u;
         ds.rows.alloc((size_t)n_total * dpad);
-        upload_rows(h, ds.rows.p, dpad, rows, row_stride_bytes, n_total, d);
+        // the centroid of all rows (Calcenter: float sums in row order, then
+        // a float division by the row count), zero padded to dpad
+        std::vector<float> mean(dpad, 0.f);
+        upload_rows(h, ds.rows.p, dpad, rows, row_stride_bytes, n_total, d, mean.data());
+        const float count = (float)n_total;
+        for (uint32_t c = 0; c < d; ++c) mean[c] /= count;
+        ds.center.alloc(dpad);
+        FK_HIP(hipMemcpyAsync(ds.center.p, mean.data(), (size_t)dpad * 4, hipMemcpyHostToDevice, h->stream));
         ds.n_total = n_total;
         ds.d = d;
         ds.dpad = dpad;

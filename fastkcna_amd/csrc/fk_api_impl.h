@@ -41,7 +41,7 @@ void require_dataset(fk_handle *h);
 void download_bytes(fk_handle *h, void *dst, const void *src, size_t bytes);
 void *host_result_alloc(size_t bytes);
 void upload_rows(fk_handle *h, float *dst, uint32_t dpad, const float *src, size_t src_stride_bytes, uint32_t n,
-                 uint32_t d);
+                 uint32_t d, float *colsum = nullptr);
 // fk_prune_noalloc's scratch: its device address on `device` (current), and
 // the bytes a call needs.
 uint8_t *noalloc_scratch(int device);

@@ -230,8 +230,11 @@ inline unsigned node_grid(const BuildState &b) {
 void build_loops(BuildState &b, fk_pg_type type);
 
 // Search / graph utilities (fk_search.hip)
-void compute_ep(BuildState &b);
+void compute_ep(BuildState &b, const uint32_t *init_nodes = nullptr);
 void column_mean(hipStream_t s, const Dataset &ds, float *center, uint32_t r0 = 0, uint32_t n = 0);
+// means of `count` row ranges [r0[k], r0[k] + n[k]) into center[k * dpad] (host arrays)
+void column_means(hipStream_t s, const Dataset &ds, float *center, const uint32_t *r0, const uint32_t *n,
+                  uint32_t count);
 void tree_grow(BuildState &b);
 
 double now_sec();

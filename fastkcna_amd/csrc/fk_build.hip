@@ -899,6 +899,41 @@ fk_status fk_step_stop_stats(fk_handle *h, const fk_params *p, const fk_state *s
     });
 }
 
+fk_status fk_step_entry_point(fk_handle *h, const fk_params *p, const fk_state *s, uint32_t init_node,
+                              uint32_t *ep) {
+    if (!h) return FK_E_INVALID;
+    return guard(h, [&] {
+        require_dataset(h);
+        check_params(p);
+        FK_CHECK_ARG(s && s->pool && s->pool_len && s->out_id && s->out_len && ep, "null argument");
+        FK_CHECK_ARG(s->pool_cap >= 1 && s->out_cap >= 1, "capacities must be positive");
+        FK_CHECK_ARG(init_node < s->n, "init_node out of range");
+        check_lens(s->pool_len, s->n, s->pool_cap, "pool_len > pool_cap");
+        check_lens(s->out_len, s->n, s->out_cap, "out_len > out_cap");
+        check_pool_ids(s);
+        check_out_ids(s, s->out_id, s->out_len);
+        select_device(h);
+        fk_stats local;
+        memset(&local, 0, sizeof(local));
+        BuildState b;
+        step_begin(b, h, p, s, &local);
+        const uint32_t N = b.N;
+        FK_CHECK_ARG(b.P.search_L <= 1024, "search_L above 1024 is not supported");
+        b.cap = s->pool_cap;
+        b.pool.alloc((size_t)N * b.cap);
+        b.cnt.alloc(N);
+        b.ncap = s->out_cap;
+        b.nsg_id.alloc((size_t)N * b.ncap);
+        b.nsg_len.alloc(N);
+        put_rows(b, b.pool.p, b.cap, s->pool, s->pool_cap, s->pool_cap);
+        FK_HIP(hipMemcpyAsync(b.cnt.p, s->pool_len, (size_t)N * 4, hipMemcpyHostToDevice, b.s));
+        put_rows(b, b.nsg_id.p, b.ncap, s->out_id, s->out_cap, s->out_cap);
+        FK_HIP(hipMemcpyAsync(b.nsg_len.p, s->out_len, (size_t)N * 4, hipMemcpyHostToDevice, b.s));
+        compute_ep(b, &init_node);
+        *ep = b.ep;
+    });
+}
+
 fk_status fk_nndescent(fk_handle *h, uint32_t n_used, const fk_params *p, uint64_t *pools, uint32_t *pool_len,
                        uint32_t pool_stride, fk_stats *stats) {
     if (!h) return FK_E_INVALID;

@@ -180,6 +180,7 @@ struct PinnedBuf {
 struct Dataset {
     DevBuf<float> rows;   // n_total x stride floats, zero padded
     DevBuf<float> chains; // n_total x dpad floats, chain-major copy (octet distances)
+    DevBuf<float> center; // dpad floats: Calcenter of all n_total rows (float row-order sums), zero padded
     uint32_t n_total = 0;
     uint32_t d = 0;
     uint32_t dpad = 0;    // d rounded up to a multiple of 8

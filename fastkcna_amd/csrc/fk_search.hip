@@ -20,28 +20,34 @@
 namespace fk {
 
 // ---------------------------------------------------------------------------
-// Centroid: per-block column sums in double, then an ordered sum of the block
-// partials (deterministic).  The reference accumulates in float in row order.
+// Centroid (MatrixOracle::Calcenter, src/kgraph-data.h:413-428): float sums
+// in row order, then a float division by the row count.  The mean of all
+// resident rows is made on the host as the rows are uploaded
+// (Dataset::center); the mean of a row range (one set of a batched build) by
+// k_colmean_ordered: one lane per column adding the rows one after the other,
+// which is that order exactly; the loads of 16 rows are issued before their
+// adds.  Range k of a launch is rows [lo[k], lo[k] + n[k]) into
+// center[k * dpad].
 // ---------------------------------------------------------------------------
-#define FK_CEN_ROWS 1024
-__global__ void k_colsum(const float *__restrict__ rows, uint32_t stride, uint32_t d, uint32_t n,
-                         double *__restrict__ part) {
-    const uint32_t r0 = blockIdx.x * FK_CEN_ROWS;
-    const uint32_t r1 = (r0 + FK_CEN_ROWS) < n ? (r0 + FK_CEN_ROWS) : n;
-    for (uint32_t c = threadIdx.x; c < d; c += blockDim.x) {
-        double s = 0.0;
-        for (uint32_t r = r0; r < r1; ++r) s += (double)rows[(size_t)r * stride + c];
-        part[(size_t)blockIdx.x * d + c] = s;
-    }
-}
-
-__global__ void k_colsum_final(const double *__restrict__ part, uint32_t nb, uint32_t d, uint32_t n,
-                               float *__restrict__ center) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+#define FK_CEN_UNROLL 16
+__global__ __launch_bounds__(64) void k_colmean_ordered(const float *__restrict__ rows, uint32_t stride, uint32_t d,
+                                                        uint32_t dpad, const uint32_t *__restrict__ lo,
+                                                        const uint32_t *__restrict__ n, float *__restrict__ center) {
+    const uint32_t k = blockIdx.y, c = blockIdx.x * 64 + threadIdx.x;
     if (c >= d) return;
-    double s = 0.0;
-    for (uint32_t b = 0; b < nb; ++b) s += part[(size_t)b * d + c];
-    center[c] = (float)(s / (double)n);
+    const uint32_t r0 = lo[k], m = n[k];
+    const float *col = rows + (size_t)r0 * stride + c;
+    float s = 0.f;
+    uint32_t r = 0;
+    for (; r + FK_CEN_UNROLL <= m; r += FK_CEN_UNROLL) {
+        float v[FK_CEN_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < FK_CEN_UNROLL; ++u) v[u] = col[(size_t)(r + u) * stride];
+#pragma unroll
+        for (uint32_t u = 0; u < FK_CEN_UNROLL; ++u) s += v[u];
+    }
+    for (; r < m; ++r) s += col[(size_t)r * stride];
+    center[(size_t)k * dpad + c] = s / (float)m;
 }
 
 // ---------------------------------------------------------------------------
@@ -72,6 +78,7 @@ struct BeamArgs {
     const uint64_t *g_off;     // CSR mode when non-null
     uint32_t *visited;         // n_q x ((N + 31) / 32) words, zeroed (or per-query ranges below)
     uint64_t seed;
+    uint32_t q_base;           // index of query 0 of this launch (its fill stream is q_base + q)
     // per-query id ranges (sets): query q searches ids [q_lo[q], q_lo[q] + q_n[q]),
     // starts from pool(q_init[q]), fills with stream 0 of q_seed[q] (what a
     // build of the set alone does), and owns visited words from q_vis[q]
@@ -81,6 +88,10 @@ struct BeamArgs {
     float *out_d;
     unsigned long long *n_dist;
 };
+
+// fk_search's random fill: query i draws fk_rand32(FK_SEARCH_SEED, i, k)
+// for k = 0, 1, ... (independent of how the queries are batched)
+#define FK_SEARCH_SEED 0x9E3779B97F4A7C15ull
 
 __device__ __forceinline__ bool test_and_set(uint32_t *bm, uint32_t id) {
     const uint32_t bit = 1u << (id & 31u);
@@ -126,7 +137,7 @@ __global__ __launch_bounds__(64) void k_beam(BeamArgs a) {
         uint64_t k = 0;
         const uint32_t want = L < Nq ? L : Nq;
         const uint64_t seed = a.q_seed ? a.q_seed[q] : a.seed;
-        const uint64_t stream = a.q_seed ? 0u : q;
+        const uint64_t stream = a.q_seed ? 0u : (uint64_t)a.q_base + q;
         while (n < want) {
             const uint32_t id = lo + fk_rand32(seed, stream, k++) % Nq;
             if (test_and_set(bm, id - lo)) s_init[n++] = id;
@@ -226,21 +237,30 @@ static void launch_beam(hipStream_t s, const BeamArgs &a) {
 // caller zeroes the padding); synchronises the stream.
 void column_mean(hipStream_t s, const Dataset &ds, float *center, uint32_t r0, uint32_t n) {
     if (n == 0) n = ds.n_total - r0;
-    const uint32_t nb = (n + FK_CEN_ROWS - 1) / FK_CEN_ROWS;
-    DevBuf<double> part((size_t)nb * ds.d);
-    hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, s, ds.rows.p + (size_t)r0 * ds.stride, ds.stride, ds.d, n,
-                       part.p);
+    if (r0 == 0 && n == ds.n_total) {  // made at upload
+        FK_HIP(hipMemcpyAsync(center, ds.center.p, (size_t)ds.dpad * 4, hipMemcpyDeviceToDevice, s));
+        return;
+    }
+    column_means(s, ds, center, &r0, &n, 1);
+}
+
+void column_means(hipStream_t s, const Dataset &ds, float *center, const uint32_t *r0, const uint32_t *n,
+                  uint32_t count) {
+    if (count == 0) return;
+    DevBuf<uint32_t> rng(2 * (size_t)count);
+    FK_HIP(hipMemcpyAsync(rng.p, r0, (size_t)count * 4, hipMemcpyHostToDevice, s));
+    FK_HIP(hipMemcpyAsync(rng.p + count, n, (size_t)count * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_colmean_ordered, dim3((ds.d + 63) / 64, count), dim3(64), 0, s, ds.rows.p, ds.stride, ds.d,
+                       ds.dpad, rng.p, rng.p + count, center);
     FK_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_colsum_final, dim3((ds.d + 255) / 256), dim3(256), 0, s, part.p, nb, ds.d, n, center);
-    FK_LAUNCH_CHECK();
-    FK_HIP(hipStreamSynchronize(s));
+    FK_HIP(hipStreamSynchronize(s));  // rng is released on return
 }
 
 // ComputeEp (src/kgraph.cpp:1528-1536), per set: the centroid of the set's
 // rows (all resident rows for a single build: Calcenter averages the whole
 // matrix, not the oracle's prefix), then a beam search from a random node's
-// pool, one query per set.
-void compute_ep(BuildState &b) {
+// pool (init_nodes[k] instead, when given), one query per set.
+void compute_ep(BuildState &b, const uint32_t *init_nodes) {
     const double t0 = now_sec();
     const Dataset &ds = *b.ds;
     const uint32_t ns = b.n_sets;
@@ -251,14 +271,17 @@ void compute_ep(BuildState &b) {
     std::vector<uint64_t> q_seed(ns), q_vis(ns);
     uint64_t words = 0;
     double center_rows = 0;
+    std::vector<uint32_t> set_n(ns);
+    for (uint32_t k = 0; k < ns; ++k) set_n[k] = b.set_n(k);
+    if (ns == 1) column_mean(b.s, ds, center.p, 0, ds.n_total);
+    else column_means(b.s, ds, center.p, b.set_lo.data(), set_n.data(), ns);
     for (uint32_t k = 0; k < ns; ++k) {
         const uint32_t lo = b.set_lo[k], n = b.set_n(k);
-        if (ns == 1) column_mean(b.s, ds, center.p, 0, ds.n_total);
-        else column_mean(b.s, ds, center.p + (size_t)k * ds.dpad, lo, n);
         center_rows += ns == 1 ? ds.n_total : n;
         q_lo[k] = lo;
         q_n[k] = n;
         q_init[k] = lo + (uint32_t)(b.next_rand(k) % n);
+        if (init_nodes) q_init[k] = init_nodes[k];
         q_seed[k] = b.next_rand(k);
         q_vis[k] = words;
         words += (n + 31) / 32;
@@ -576,7 +599,8 @@ extern "C" fk_status fk_search(fk_handle *h, const uint64_t *offsets, const uint
             a.g_ids = dids.p;
             a.g_off = doff.p;
             a.visited = vis.p;
-            a.seed = 0x9E3779B97F4A7C15ull ^ q0;
+            a.seed = FK_SEARCH_SEED;
+            a.q_base = q0;
             a.out_ids = dout.p;
             a.n_dist = cnt.at(0);
             launch_beam(h->stream, a);

@@ -243,6 +243,15 @@ class Device:
                     "fk_step_bridgeview")
         return st.to_dict()
 
+    def step_entry_point(self, params, state, init_node):
+        """ComputeEp on the state's pools and out-lists from init_node's pool."""
+        ep = ctypes.c_uint32()
+        p = _ffi.params_struct(params)
+        s = state.struct()
+        self._check(self.lib.fk_step_entry_point(self.h, ctypes.byref(p), ctypes.byref(s), init_node,
+                                                 ctypes.byref(ep)), "fk_step_entry_point")
+        return ep.value
+
     def step_stop_stats(self, params, state, ctrl_ids, gt):
         """(delta, recall) of the stop rule on the state's pools."""
         ctrl_ids = _as(ctrl_ids, np.uint32)

@@ -366,9 +366,24 @@ fk_status fk_step_stop_stats(fk_handle *h, const fk_params *p, const fk_state *s
                              const uint32_t *ctrl_ids, const uint32_t *gt, uint32_t gt_stride, float *delta,
                              float *recall);
 
+/* ComputeEp (src/kgraph.cpp:1528-1536): the centroid of all resident rows
+ * (MatrixOracle::Calcenter, src/kgraph-data.h:413-428: float sums in row
+ * order divided by the row count), then get_neighbors (:1560-1623) of the
+ * centroid with beam search_L over the out-lists (out_id, out_len) starting
+ * from the pool of init_node (pool, pool_len; the reference draws init_node
+ * with rand()); *ep = the best node found.  When init_node's pool has fewer
+ * than search_L entries the start set is filled with random nodes. */
+fk_status fk_step_entry_point(fk_handle *h, const fk_params *p, const fk_state *s, uint32_t init_node,
+                              uint32_t *ep);
+
 /* Beam search of n_q host queries over a CSR graph (the layout fk_build
  * returns) from entry point ep with beam width search_L; writes the best K
- * ids per query. */
+ * ids per query.  The best-first loop of get_neighbors (src/kgraph.cpp:
+ * 1560-1623): the start set is ep, then ep's out-list in order, then ids
+ * (hash of the query index and a counter) % n until search_L distinct ids;
+ * expand the best unexpanded entry, evaluate each unvisited neighbour once,
+ * keep the search_L best in (dist, id) order, resume from the smallest
+ * position an insert reached.  *n_dist receives the distances computed. */
 fk_status fk_search(fk_handle *h, const uint64_t *offsets, const uint32_t *ids, uint32_t n,
                     uint32_t ep, const float *queries, uint32_t n_q, uint32_t search_L,
                     uint32_t K, uint32_t *out_ids, uint64_t *n_dist);

@@ -58,6 +58,13 @@ def _load():
         lib.fko_nndescent.restype = ctypes.c_int
         lib.fko_nndescent.argtypes = [P, ctypes.c_size_t, ctypes.c_uint, ctypes.c_uint, P, ctypes.c_int, P, P,
                                       ctypes.c_uint, P]
+        lib.fko_search.restype = None
+        lib.fko_search.argtypes = [P, ctypes.c_size_t, ctypes.c_uint, ctypes.c_uint, P, P, ctypes.c_uint,
+                                   ctypes.c_uint, P, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint64, P,
+                                   P]
+        lib.fko_step_entry_point.restype = ctypes.c_uint
+        lib.fko_step_entry_point.argtypes = [P, ctypes.c_size_t, ctypes.c_uint, ctypes.c_uint, P, ctypes.c_void_p,
+                                             ctypes.c_uint]
         PS = ctypes.c_void_p
         lib.fko_step_link.restype = ctypes.c_int
         lib.fko_step_link.argtypes = [ctypes.c_int, P, ctypes.c_size_t, ctypes.c_uint, ctypes.c_uint, P, PS, P]
@@ -254,6 +261,14 @@ def step_bridgeview(rows, params, state):
     return st.to_dict()
 
 
+def step_entry_point(rows, params, state, init_node):
+    """ComputeEp on the state (pools, out-lists) from init_node's pool."""
+    rows = np.ascontiguousarray(rows, np.float32)
+    ps = _params(params)
+    return int(_load().fko_step_entry_point(_p(rows), rows.strides[0], rows.shape[0], rows.shape[1],
+                                            ctypes.byref(ps), ctypes.byref(state.struct()), init_node))
+
+
 def step_stop_stats(params, state, ctrl_ids, gt):
     ctrl_ids = np.ascontiguousarray(ctrl_ids, np.uint32)
     gt = np.ascontiguousarray(gt, np.uint32)
@@ -262,3 +277,21 @@ def step_stop_stats(params, state, ctrl_ids, gt):
     _load().fko_step_stop_stats(ctypes.byref(ps), ctypes.byref(state.struct()), len(ctrl_ids), _p(ctrl_ids), _p(gt),
                                 gt.shape[1] if gt.ndim == 2 else 0, ctypes.byref(delta), ctypes.byref(recall))
     return delta.value, recall.value
+
+
+SEARCH_SEED = 0x9E3779B97F4A7C15  # the evaluator's fill stream seed (fk_search)
+
+
+def search(rows, offsets, ids, ep, queries, search_L, K):
+    """CPU restatement of the query evaluator (fk_search): best-first beam
+    from ep over a CSR graph; returns (ids [n_q, K], distances computed)."""
+    rows = np.ascontiguousarray(rows, np.float32)
+    offsets = np.ascontiguousarray(offsets, np.uint64)
+    ids = np.ascontiguousarray(ids, np.uint32)
+    queries = np.ascontiguousarray(queries, np.float32)
+    n = len(offsets) - 1
+    out = np.empty((len(queries), K), np.uint32)
+    nd = ctypes.c_uint64()
+    _load().fko_search(_p(rows), rows.strides[0], rows.shape[0], rows.shape[1], _p(offsets), _p(ids), n, ep,
+                       _p(queries), len(queries), search_L, K, SEARCH_SEED, _p(out), ctypes.byref(nd))
+    return out, nd.value

@@ -711,17 +711,21 @@ struct Builder {
     }
 
     vector<float> center;
-    void compute_ep() {
+    void compute_center() {
+        if (!center.empty()) return;
+        // Calcenter over all rows (data_size), float accumulation in row order
+        center.assign(o.dpad, 0.f);
+        for (unsigned i = 0; i < o.n; ++i)
+            for (unsigned j = 0; j < o.d; ++j) center[j] += o.row(i)[j];
+        for (unsigned j = 0; j < o.d; ++j) center[j] /= o.n;
+    }
+
+    // ComputeEp (src/kgraph.cpp:1528-1536); init_point = rand() % N unless given
+    void compute_ep(int64_t given_init = -1) {
         const double t0 = now();
-        if (center.empty()) {
-            // Calcenter over all rows (data_size), float accumulation in row order
-            center.assign(o.dpad, 0.f);
-            for (unsigned i = 0; i < o.n; ++i)
-                for (unsigned j = 0; j < o.d; ++j) center[j] += o.row(i)[j];
-            for (unsigned j = 0; j < o.d; ++j) center[j] /= o.n;
-        }
+        compute_center();
         vector<Nb> pool;
-        const unsigned init_point = R_() % N();
+        const unsigned init_point = given_init >= 0 ? (unsigned)given_init : R_() % N();
         get_neighbors(center.data(), init_point, P.search_L, pool);
         ep = pool[0].id;
         st->t_ep += now() - t0;
@@ -1165,6 +1169,84 @@ int fko_nndescent(const float *rows, size_t stride_bytes, unsigned n, unsigned d
 }
 
 // ---------------------------------------------------------------------------
+// The query evaluator's search (the product's fk_search) restated on the
+// CPU: the best-first loop of get_neighbors (src/kgraph.cpp:1560-1623) over a
+// CSR graph, one neighbour at a time with InsertIntoPool-style bookkeeping,
+// in the build's canonical (dist, id) order.  Its start set is the
+// evaluator's own: ep, ep's out-list, then (rand32(query, k) % n) fill; the
+// fill hash is the evaluator's counter-based generator (a splitmix64
+// finaliser), written out here so both sides draw the same ids.
+// ---------------------------------------------------------------------------
+static uint64_t mix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+static uint32_t rand32(uint64_t seed, uint64_t stream, uint64_t k) {
+    return (uint32_t)(mix64(seed * 0xD1B54A32D192ED03ull ^ mix64(stream * 0x8CB92BA72F3D8DD7ull + k)) >> 32);
+}
+
+void fko_search(const float *rows, size_t stride_bytes, unsigned n_total, unsigned d, const uint64_t *offsets,
+                const uint32_t *ids, unsigned n, unsigned ep, const float *queries, unsigned n_q, unsigned L,
+                unsigned K, uint64_t seed, uint32_t *out_ids, uint64_t *n_dist) {
+    Rows r;
+    r.load(rows, stride_bytes, n_total, d);
+    uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : total)
+    for (unsigned qi = 0; qi < n_q; ++qi) {
+        vector<float> q(r.dpad, 0.f);
+        memcpy(q.data(), queries + (size_t)qi * d, (size_t)d * 4);
+        vector<char> seen(n, 0);
+        vector<unsigned> start;
+        const unsigned want = std::min(L, n);
+        auto take = [&](unsigned id) {
+            if (id < n && !seen[id] && start.size() < want) { seen[id] = 1; start.push_back(id); }
+        };
+        take(ep);
+        for (uint64_t e = offsets[ep]; e < offsets[ep + 1]; ++e) take(ids[e]);
+        for (uint64_t k = 0; start.size() < want; ++k) take(rand32(seed, qi, k) % n);
+        struct E { float d; uint32_t id; bool open; };
+        auto before = [](const E &a, const E &b) { return a.d < b.d || (a.d == b.d && a.id < b.id); };
+        vector<E> pool;
+        for (unsigned id : start) pool.push_back({r.qdist(q.data(), id), id, true});
+        total += pool.size();
+        std::sort(pool.begin(), pool.end(), before);
+        size_t k = 0;
+        while (k < pool.size()) {
+            size_t nk = pool.size();
+            if (pool[k].open) {
+                pool[k].open = false;
+                const unsigned u = pool[k].id;
+                // neighbours in groups of 64 list positions, each group filtered
+                // by the worst distance kept when it starts (the evaluator's
+                // wavefront width); the kept set is the best L either way
+                for (uint64_t e0 = offsets[u]; e0 < offsets[u + 1]; e0 += 64) {
+                    const float worst = pool.back().d;
+                    for (uint64_t e = e0; e < std::min(e0 + 64, offsets[u + 1]); ++e) {
+                        const unsigned v = ids[e];
+                        if (v >= n || seen[v]) continue;
+                        seen[v] = 1;
+                        ++total;
+                        E c{r.qdist(q.data(), v), v, true};
+                        if (!(c.d < worst)) continue;  // `dist >= pool[L-1].dist` skips
+                        const size_t at =
+                            (size_t)(std::lower_bound(pool.begin(), pool.end(), c, before) - pool.begin());
+                        if (at >= L) continue;
+                        pool.insert(pool.begin() + at, c);
+                        if (pool.size() > L) pool.pop_back();
+                        nk = std::min(nk, at);
+                    }
+                }
+            }
+            k = nk <= k ? nk : k + 1;
+        }
+        for (unsigned j = 0; j < K; ++j) out_ids[(size_t)qi * K + j] = j < pool.size() ? pool[j].id : 0xFFFFFFFFu;
+    }
+    if (n_dist) *n_dist = total;
+}
+
+// ---------------------------------------------------------------------------
 // Step entry points: one step of the build's loop on a supplied state (the
 // layout of fk_state in fk_api.h), run by the restated code above.
 // ---------------------------------------------------------------------------
@@ -1284,6 +1366,24 @@ int fko_step_bridgeview(const float *rows, size_t stride_bytes, unsigned n_total
         }
     }
     return 0;
+}
+
+// ComputeEp from init_node's pool over the state's out-lists; returns ep.
+unsigned fko_step_entry_point(const float *rows, size_t stride_bytes, unsigned n_total, unsigned d,
+                              const fk_params *p, const fk_state *s, unsigned init_node) {
+    Rows r;
+    r.load(rows, stride_bytes, n_total, d);
+    r.oracle_n = s->n;
+    fk_stats st;
+    memset(&st, 0, sizeof(st));
+    Builder b(r, *p, &st);
+    b.shrink();
+    load_pools(b, s, std::max(s->pool_cap, b.MAXL));
+    for (unsigned i = 0; i < s->n; ++i)
+        for (unsigned k = 0; k < s->out_len[i]; ++k)
+            b.nh[i].nn_nsg.push_back(Nb(s->out_id[(size_t)i * s->out_cap + k], 0.f));
+    b.compute_ep((int64_t)init_node);
+    return b.ep;
 }
 
 // The stop rule's delta and recall of the state's pools (the expressions of

@@ -265,3 +265,40 @@ def test_entry_point_is_a_local_minimum():
     dep = oracle.l2(center, x[ep])
     for v in nb:
         assert dep <= oracle.l2(center, x[v]), v
+
+
+def test_search_is_exhaustive_when_the_beam_covers_the_graph():
+    """With a beam at least as wide as a connected graph, best-first search
+    evaluates every node once and returns the exact nearest neighbours (ties
+    by id); on a complete graph, expanding the entry point alone does."""
+    rng = np.random.default_rng(37)
+    n = 300
+    x = rng.normal(size=(n, 12)).astype(np.float32)
+    qs = rng.normal(size=(40, 12)).astype(np.float32)
+    gt, _ = oracle.knn(x, qs, 10)
+    ring = [np.array([(i + 1) % n, (i - 1) % n]) for i in range(n)]
+    off = np.zeros(n + 1, np.uint64)
+    off[1:] = np.cumsum([len(v) for v in ring])
+    res, nd = oracle.search(x, off, np.concatenate(ring), 17, qs, n, 10)
+    assert np.array_equal(res, gt) and nd == 40 * n
+    full = [np.delete(np.arange(n), i) for i in range(n)]
+    off[1:] = np.cumsum([len(v) for v in full])
+    res, nd = oracle.search(x, off, np.concatenate(full), 5, qs, 20, 10)
+    assert np.array_equal(res, gt) and nd == 40 * n
+
+
+def test_search_on_a_chain_is_greedy():
+    """1-D points joined to their two neighbours: from either end a beam of 2
+    walks monotonically to the query's nearest point."""
+    n = 200
+    x = np.zeros((n, 8), np.float32)
+    x[:, 0] = np.arange(n) * 1.5
+    ring = [np.array([j for j in (i - 1, i + 1) if 0 <= j < n]) for i in range(n)]
+    off = np.zeros(n + 1, np.uint64)
+    off[1:] = np.cumsum([len(v) for v in ring])
+    ids = np.concatenate(ring)
+    qs = np.zeros((5, 8), np.float32)
+    qs[:, 0] = [0.2, 77.9, 150.1, 298.0, 120.7]
+    for ep in (0, n - 1):
+        res, _ = oracle.search(x, off, ids, ep, qs, 2, 1)
+        assert list(res[:, 0]) == [int(np.argmin(np.abs(x[:, 0] - q))) for q in qs[:, 0]]

@@ -221,7 +221,7 @@ def cpu_full_leg(dev, base, queries, gpu_graph, gpu_stats, threads):
     p = params_c2(IndexParams)
     log(f"cpu full: oracle build of {len(base):,} rows on {threads} threads")
     t0 = time.perf_counter()
-    off, ids, ep, width, st, _ = oracle.build(FK_PG_NSG, base, p, threads=threads)
+    off, ids, ep, width, st, _ = oracle.build(FK_PG_NSG, base, p, threads=threads, verbose=True)
     dt = time.perf_counter() - t0
     log(f"cpu full: {dt:.1f}s")
     dev.upload(base)
@@ -331,6 +331,111 @@ def batching_leg(dev, n_sets):
     return out
 
 
+def knn_recall_entries(st):
+    """KNN-graph recall@MAXL over the controls after NN-descent and after
+    each BridgeView."""
+    return [round(float(v), 5) for v in st["recall"][int(st["nnd_iterations"]):int(st["n_recall"])]]
+
+
+def c1_leg(dev):
+    """C1: fasthnsw over 100,000 x 128 (64-centre integer mixture, seed 0),
+    K = L = 200 S = 10 R = 100 I = 6 nsg_R = 40: every level built through
+    the C ABI on the level-ordered rows (one warm-up), timed; query
+    recall@10 of its 1,000 queries through the hnswlib search (ef 40)."""
+    from fastkcna_amd import hnsw
+    from fastkcna_amd.datasets import c1, params_c1
+    from fastkcna_amd.kgraph import IndexParams
+    x, qs = c1()
+    p = params_c1(IndexParams)
+    lv = hnsw.levels(len(x), p.nsg_R, p.seed)
+    xr = np.ascontiguousarray(x[hnsw.level_order(lv)])
+    dev.upload(xr)
+    layer = hnsw.device_layer(dev)
+    hnsw.build_layers(layer, len(x), lv, p)
+    graphs, ep, sec = hnsw.build_layers(layer, len(x), lv, p)
+    gt, _ = dev.knn(qs, 10)
+    found = [hnsw.search(graphs, ep, xr, q, p.search_L, 10) for q in qs]
+    rec = float(np.mean([len(set(f) & set(g.tolist())) / 10 for f, g in zip(found, gt)]))
+    out = {"workload": "C1: fasthnsw, 100,000x128 integer mixture of 64 Gaussians (seed 0), K=L=200 S=10 R=100 "
+                       "I=6 nsg_R=40 (level 0: 80) search_L=40 search_K=200",
+           "build_sec": sec, "levels": len(graphs), "query_recall_at_10": rec, "queries": len(qs)}
+    log(f"c1: {out}")
+    return out
+
+
+def c3_leg(dev):
+    """C3: tau-MNG over 1,000,000 x 960 (256-centre float mixture, seed 2),
+    K = L = 500 S = 12 R = 100 I = 6 nsg_R = 50 tau = 10: the plain build
+    (after one warm-up) and the build with the Gram-form join on MFMA (BF16
+    operands, FP32 recheck), their graphs compared."""
+    from fastkcna_amd import FK_PG_TAUMNG, _ffi
+    from fastkcna_amd.datasets import c3, params_c3
+    from fastkcna_amd.kgraph import IndexParams
+    t = time.perf_counter()
+    x, qs = c3(n_queries=1_000)
+    log(f"c3: data {x.shape} in {time.perf_counter() - t:.1f}s")
+    dev.upload(x)
+    p = params_c3(IndexParams)
+    p.flags = _ffi.FK_FLAG_PROFILE
+    dev.build(FK_PG_TAUMNG, p)
+    t = time.perf_counter()
+    off, ids, ep, _, st = dev.build(FK_PG_TAUMNG, p)
+    plain = time.perf_counter() - t
+    gt, _ = dev.knn(qs, 10)
+    rec = recall_at_10(dev.search(off, ids, ep, qs, 80, 10)[0], gt)
+    q = p.copy(flags=p.flags | _ffi.FK_FLAG_GRAM)
+    t = time.perf_counter()
+    goff, gids, gep, _, gst = dev.build(FK_PG_TAUMNG, q)
+    chosen = time.perf_counter() - t
+    same = bool(np.array_equal(off, goff) and np.array_equal(ids, gids) and ep == gep)
+    fam = lambda s: {f: round(s["prof_ms"][i], 1) for i, f in enumerate(_ffi.PROF_FAMILIES) if s["prof_launches"][i]}
+    out = {"workload": "C3: tau-MNG, 1,000,000x960 float32 mixture of 256 Gaussians (seed 2, sigma 0.05), K=L=500 "
+                       "S=12 R=100 I=6 nsg_R=50 tau=10 search_L=80 search_K=500",
+           "build_sec": plain, "dist_evals_per_sec": total_dists(st) / plain,
+           "knn_recall_at_maxl": knn_recall_entries(st), "query_recall_at_10": rec,
+           "families_ms": fam(st),
+           "gram": {"build_sec": chosen, "speedup_vs_plain": plain / chosen,
+                    "gram_mode": _ffi.GRAM_MODE_NAMES[int(gst["gram_mode"])], "graph_identical_to_plain": same,
+                    "gram_pairs": int(gst["d_gram"]), "fp32_rechecks": int(gst["d_recheck"]),
+                    "families_ms": fam(gst)}}
+    log(f"c3: {out}")
+    return out
+
+
+def c5_leg(dev):
+    """C5: the HNSW level-0 layer (nsg_R 64) of 10,000,000 x 96 unit-normalised
+    rows (4,096 centres, sigma 0.25, seed 4), K = L = 200 S = 10 R = 100 I = 6
+    search_K = 200, built plain and with the learned screen."""
+    from fastkcna_amd import FK_PG_HNSW_LAYER, _ffi
+    from fastkcna_amd.datasets import c5, params_c5
+    from fastkcna_amd.kgraph import IndexParams
+    t = time.perf_counter()
+    x, qs = c5(n_queries=1_000)
+    log(f"c5: data {x.shape} in {time.perf_counter() - t:.1f}s")
+    dev.upload(x)
+    p = params_c5(IndexParams)
+    t = time.perf_counter()
+    off, ids, ep, _, st = dev.build(FK_PG_HNSW_LAYER, p)
+    plain = time.perf_counter() - t
+    gt, _ = dev.knn(qs, 10)
+    rec = recall_at_10(dev.search(off, ids, ep, qs, 40, 10)[0], gt)
+    del off, ids
+    t = time.perf_counter()
+    _, _, _, _, sst = dev.build(FK_PG_HNSW_LAYER, p.copy(flags=_ffi.FK_FLAG_SCREEN))
+    scr = time.perf_counter() - t
+    out = {"workload": "C5: HNSW level 0, 10,000,000x96 unit-normalised mixture of 4,096 Gaussians (seed 4, "
+                       "sigma 0.25), K=L=200 S=10 R=100 I=6 nsg_R=64 search_L=40 search_K=200",
+           "build_sec": plain, "dist_evals_per_sec": total_dists(st) / plain,
+           "knn_recall_at_maxl": knn_recall_entries(st), "query_recall_at_10_flat": rec,
+           "learned_screen": {"build_sec": scr, "speedup_vs_plain": plain / scr,
+                              "knn_recall_at_maxl": knn_recall_entries(sst),
+                              "screened_pairs": int(sst["d_screened"]), "skipped_pairs": int(sst["d_screen_skip"]),
+                              "est_lost_candidate_frac": 64.0 * sst["screen_missed"] / max(sst["screen_cands"], 1),
+                              "train_loss": float(sst["screen_loss"]), "last_margin": float(sst["screen_margin"])}}
+    log(f"c5: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -346,8 +451,10 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, help="MiB of a device-to-device copy to run first")
     ap.add_argument("--no-chosen", action="store_true", help="skip the chosen (MFMA Gram join) leg")
     ap.add_argument("--no-screen", action="store_true", help="skip the learned-screen leg")
-    ap.add_argument("--batch-sets", type=int, default=0,
-                    help="also time the batching workload's first N datasets batched vs one by one")
+    ap.add_argument("--batch-sets", type=int, default=256,
+                    help="datasets of the batching workload timed batched vs one by one (0: skip)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other workloads (C1 fasthnsw, C3 tau-MNG 1M x 960, C5 10M x 96)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the build does not shard "
@@ -411,10 +518,13 @@ def main():
         f"{f}={st['prof_ms'][i]:.1f}/{st['prof_bytes'][i] / max(st['prof_ms'][i], 1e-9) / 1e6:.0f}"
         for i, f in enumerate(_ffi.PROF_FAMILIES) if st['prof_launches'][i]))
     batching = batching_leg(dev, args.batch_sets) if args.batch_sets else None
+    configs = None
+    if not args.no_configs:
+        configs = {"C1": c1_leg(dev), "C3": c3_leg(dev), "C5": c5_leg(dev)}
     cpu = cpu_full = None
     threads = cpu_threads()
     if args.cpu_full:
-        cpu_full = cpu_full_leg(dev, base, queries, graph, st, threads)
+        cpu_full = cpu_full_leg(dev, base, queries, graph, st, threads)  # uploads the workload's rows again
         log(f"cpu full: {cpu_full}")
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_sample, threads, dev, queries[:2000])
@@ -448,6 +558,7 @@ def main():
         "chosen": chosen,
         "learned_screen": screen,
         "batching": batching,
+        "configs": configs,
         "cpu_baseline": cpu,
         "cpu_full": cpu_full,
     }

@@ -26,7 +26,7 @@ import pytest
 
 import oracle
 from conftest import ROOT
-from fastkcna_amd import FK_PG_HNSW_LAYER, FK_PG_TAUMNG, IndexParams, _ffi
+from fastkcna_amd import FK_PG_HNSW_LAYER, FK_PG_TAUMNG, IndexParams, _ffi, hnsw
 from fastkcna_amd.datasets import c1, c3, params_c1, params_c3
 from fastkcna_amd.formats import write_lshkit
 from test_gpu_cli import read_hnsw
@@ -152,40 +152,6 @@ def test_c3_sample_nndescent_not_worse_than_oracle(device):
 # ---------------------------------------------------------------------------
 # C1
 # ---------------------------------------------------------------------------
-def hnsw_search(levels_graphs, top_ep, vec, q, ef, K):
-    """hnswlib's search: greedy descent through the upper levels, then a
-    best-first beam of width ef over level 0 (vectorised NumPy distances)."""
-    cur = top_ep
-    dc = float(np.sum((vec[cur] - q) ** 2))
-    for off, ids in levels_graphs[1:][::-1]:
-        changed = True
-        while changed:
-            changed = False
-            nb = ids[off[cur]:off[cur + 1]]
-            if len(nb):
-                dn = np.sum((vec[nb] - q) ** 2, axis=1)
-                j = int(np.argmin(dn))
-                if dn[j] < dc:
-                    cur, dc, changed = int(nb[j]), float(dn[j]), True
-    off, ids = levels_graphs[0]
-    visited = {cur}
-    pool = [(dc, cur, False)]
-    while True:
-        pool.sort()
-        pool = pool[:ef]
-        i = next((k for k, e in enumerate(pool) if not e[2]), None)
-        if i is None:
-            break
-        d0, u, _ = pool[i]
-        pool[i] = (d0, u, True)
-        nb = [int(v) for v in ids[off[u]:off[u + 1]] if int(v) not in visited]
-        if nb:
-            visited.update(nb)
-            dn = np.sum((vec[nb] - q) ** 2, axis=1)
-            pool.extend((float(dv), v, False) for dv, v in zip(dn, nb))
-    return [e[1] for e in sorted(pool)[:K]]
-
-
 def csr(lists, n):
     off = np.zeros(n + 1, np.int64)
     off[1:] = np.cumsum([len(v) for v in lists])
@@ -232,30 +198,23 @@ def test_c1_fasthnsw_query_recall_vs_oracle(device, tmp_path):
             c = struct.unpack_from("<H", blk, 0)[0] if blk else 0
             lists.append(np.frombuffer(blk, np.uint32, c, 4) if c else np.zeros(0, np.uint32))
         graphs.append(csr(lists, n))
-    found = [lab[hnsw_search(graphs, hdr["enterpoint"], vec, q, p.search_L, 10)].tolist() for q in qs]
+    found = [lab[hnsw.search(graphs, hdr["enterpoint"], vec, q, p.search_L, 10)].tolist() for q in qs]
     rg = recall10(found, gt)
     # the oracle, level by level from the same level assignment
-    lv = np.array(minstd_levels(n, p.nsg_R, p.seed))
-    order = np.lexsort((np.arange(n), -lv))  # highest level first, then file order
+    lv = hnsw.levels(n, p.nsg_R, p.seed)
+    assert np.array_equal(lv, np.array(minstd_levels(n, p.nsg_R, p.seed)))
+    order = hnsw.level_order(lv)
     assert np.array_equal(vec, x[order])  # the tool's internal order is the same
     xr = np.ascontiguousarray(x[order])
-    top = int(lv.max())
-    counts = np.bincount(lv, minlength=top + 1)
-    size, cgraphs, top_ep = 0, [None] * (top + 1), 0
-    for level in range(top, -1, -1):
-        size += int(counts[level])
-        if size == 1:
-            cgraphs[level] = (np.zeros(n + 1, np.int64), np.zeros(0, np.int64))
-            continue
-        pl = p.copy(nsg_R=p.nsg_R * (2 if level == 0 else 1))
-        say("c1: oracle layer", level, "on", size, "rows")
+
+    def oracle_layer(pl, size):
+        say("c1: oracle layer on", size, "rows")
         off, ids, ep, *_ = oracle.build(FK_PG_HNSW_LAYER, xr, pl, n_used=size, threads=0)
-        full = np.full(n + 1, off[-1], np.int64)
-        full[:size + 1] = off
-        cgraphs[level] = (full, ids.astype(np.int64))
-        if level == top:
-            top_ep = ep
-    cfound = [order[hnsw_search(cgraphs, top_ep, xr, q, p.search_L, 10)].tolist() for q in qs]
+        return off, ids, ep
+
+    cgraphs, top_ep, _ = hnsw.build_layers(oracle_layer, n, lv, p)
+    top = len(cgraphs) - 1
+    cfound = [order[hnsw.search(cgraphs, top_ep, xr, q, p.search_L, 10)].tolist() for q in qs]
     rc = recall10(cfound, gt)
     say("c1: query recall@10 (ef 40) gpu", rg, "oracle", rc, "levels", top + 1)
     assert rg >= rc - 0.01

@@ -429,92 +429,49 @@ __global__ void k_set_edges(const float *__restrict__ rows, uint32_t stride, uin
 }
 
 // The random fallback of findroot (src/kgraph.cpp:2174-2236): try t = 1, 2,
-// ... draws a node of the set uniformly and takes it when it is reached and
-// its out-degree is below nsg_R * (t / n + 1) (and below the out-list
-// capacity).  The reference loops over the draws; when few reached nodes can
-// take an edge (every out-list full, as under a truncating prune) that is
-// O(n) draws per attach.  The same distribution in O(1) per attach: in
-// degree band f (draws ((f - 1) n, f n], threshold min(nsg_R f, cap)) every
-// draw succeeds with probability |E_f| / n, E_f = the reached nodes below the
-// threshold, so the band ends with a success with probability
-// 1 - (1 - |E_f| / n)^draws, and a success is uniform over E_f.  E_f is kept
-// per band as an array with positions: nodes leave it when their degree
-// reaches the threshold, components join it when they become reached.
-struct AttachPicker {
-    uint32_t N;
-    const uint32_t *lab;
-    uint32_t *deg;
-    uint32_t nsg_R, cap;
-    std::vector<uint32_t> thr;                  // band thresholds, the last = cap
-    std::vector<std::vector<uint32_t>> elig;    // per band: reached nodes below its threshold
-    std::vector<std::vector<int32_t>> pos;      // per band: position of node - lo in elig, or -1
-    std::vector<uint32_t> comp_off, comp_nodes; // unreached components: members by label
-    uint32_t lo = 0, n = 0;
-
-    AttachPicker(uint32_t N_, const uint32_t *lab_, uint32_t *deg_, uint32_t R, uint32_t cap_)
-        : N(N_), lab(lab_), deg(deg_), nsg_R(R), cap(cap_) {
+// ... draws a node of the set and takes it when it is reached and its
+// out-degree is below nsg_R * (t / n + 1) (and below the out-list capacity).
+// When no reached node of the set is below a band's bound (every out-list
+// full, as a truncating prune leaves them) every draw of that band fails,
+// which costs n draws per attach; such a band is skipped at once and the
+// set's stream advanced past its draws (the stream is a counter), so the
+// choice is the one the draw-by-draw loop makes.  low[f] counts the set's
+// reached nodes below band f's bound: nodes leave a band when their degree
+// reaches its bound, an attached component joins the counts once its root
+// has been given its edge (later roots are larger ids, so it stays reached).
+struct BandCounts {
+    std::vector<uint32_t> thr, low;
+    uint32_t set = 0xFFFFFFFFu;
+    const uint32_t *lab = nullptr;
+    std::vector<uint32_t> unr;  // unreached nodes ordered by (label, id), made on first use
+    bool sorted = false;
+    void reset(uint32_t set_, const uint32_t *deg, uint32_t lo, uint32_t n, uint32_t R, uint32_t cap, uint32_t u) {
+        set = set_;
+        thr.clear();
         for (uint32_t f = 1;; ++f) {
-            const uint32_t t = std::min<uint64_t>((uint64_t)R * f, cap_);
+            const uint32_t t = (uint32_t)std::min<uint64_t>((uint64_t)R * f, cap);
             thr.push_back(t);
-            if (t == cap_) break;
+            if (t == cap) break;
         }
-        // members of every unreached component, grouped by its label
-        comp_off.assign((size_t)N + 1, 0);
-        for (uint32_t v = 0; v < N; ++v)
-            if (lab[v] != FK_REACHED) ++comp_off[lab[v] + 1];
-        for (uint32_t v = 0; v < N; ++v) comp_off[v + 1] += comp_off[v];
-        comp_nodes.resize(comp_off[N]);
-        std::vector<uint32_t> fill(comp_off.begin(), comp_off.end() - 1);
-        for (uint32_t v = 0; v < N; ++v)
-            if (lab[v] != FK_REACHED) comp_nodes[fill[lab[v]]++] = v;
-    }
-    void insert(uint32_t v) {
-        for (size_t f = 0; f < thr.size(); ++f)
-            if (deg[v] < thr[f] && pos[f][v - lo] < 0) {
-                pos[f][v - lo] = (int32_t)elig[f].size();
-                elig[f].push_back(v);
-            }
-    }
-    void begin_set(uint32_t lo_, uint32_t n_) {
-        lo = lo_;
-        n = n_;
-        elig.assign(thr.size(), {});
-        pos.assign(thr.size(), std::vector<int32_t>(n_, -1));
+        low.assign(thr.size(), 0);
         for (uint32_t v = lo; v < lo + n; ++v)
-            if (lab[v] == FK_REACHED) insert(v);
+            if (lab[v] == FK_REACHED || lab[v] < u)
+                for (size_t f = 0; f < thr.size(); ++f) low[f] += deg[v] < thr[f] ? 1u : 0u;
+        if (!sorted) {
+            std::stable_sort(unr.begin(), unr.end(), [&](uint32_t a, uint32_t b) { return lab[a] < lab[b]; });
+            sorted = true;
+        }
     }
-    void reach_component(uint32_t label) {
-        for (uint32_t e = comp_off[label]; e < comp_off[label + 1]; ++e) insert(comp_nodes[e]);
-    }
-    void add_edge(uint32_t v) {
+    void add_edge(uint32_t *deg, uint32_t v, uint32_t set_of_v) {  // deg[v] grows by one
+        if (set_of_v == set)
+            for (size_t f = 0; f < thr.size(); ++f) low[f] -= deg[v] + 1 == thr[f] ? 1u : 0u;
         deg[v]++;
-        for (size_t f = 0; f < thr.size(); ++f) {
-            const int32_t at = pos[f][v - lo];
-            if (at >= 0 && deg[v] >= thr[f]) {  // swap-remove from band f
-                const uint32_t last = elig[f].back();
-                elig[f][at] = last;
-                pos[f][last - lo] = at;
-                elig[f].pop_back();
-                pos[f][v - lo] = -1;
-            }
-        }
     }
-    template <class Rand>
-    uint32_t random_root(Rand next) {
-        for (size_t f = 0; f < thr.size(); ++f) {
-            const size_t m = elig[f].size();
-            if (m == 0) continue;
-            const bool last = f + 1 == thr.size();  // the last band repeats until a success
-            if (!last) {
-                // band 1 holds draws 1 .. n - 1, every later band n draws
-                const double draws = f == 0 ? (double)n - 1.0 : (double)n;
-                const double p_fail = std::exp(draws * std::log1p(-(double)m / (double)n));
-                const double u = (double)(next() >> 11) * (1.0 / 9007199254740992.0);
-                if (u < p_fail) continue;
-            }
-            return elig[f][next() % m];
-        }
-        throw Error(FK_E_INTERNAL, "tree_grow: no reached node can take an edge");
+    void reach(const uint32_t *deg, uint32_t u, uint32_t set_of_u) {  // component u is now reached
+        if (set_of_u != set) return;
+        auto it = std::lower_bound(unr.begin(), unr.end(), u, [&](uint32_t a, uint32_t key) { return lab[a] < key; });
+        for (; it != unr.end() && lab[*it] == u; ++it)
+            for (size_t f = 0; f < thr.size(); ++f) low[f] += deg[*it] < thr[f] ? 1u : 0u;
     }
 };
 
@@ -572,8 +529,12 @@ void tree_grow(BuildState &b) {
     b.prof_end(pe, FK_PROF_CONN, (double)(rounds + levels) * ((double)N * 8.0 + 8.0 * (double)N * b.P.nsg_R));
     b.prof_flush();
     std::vector<uint32_t> roots_of;  // attached ids, ascending
-    for (uint32_t v = 0; v < N; ++v)
+    BandCounts bands;
+    bands.lab = lab;
+    for (uint32_t v = 0; v < N; ++v) {
         if (lab[v] == v) roots_of.push_back(v);
+        if (lab[v] != FK_REACHED) bands.unr.push_back(v);
+    }
     const uint32_t na = (uint32_t)roots_of.size();
     std::vector<uint32_t> e_src, e_slot, e_dst;
     if (na) {
@@ -589,15 +550,8 @@ void tree_grow(BuildState &b) {
         FK_HIP(hipMemcpyAsync(rows.p, drows.p, (size_t)na * rs * 4, hipMemcpyDeviceToHost, b.s));
         b.sync();
         // 4. root choices in ascending order of the attached ids
-        AttachPicker pick(N, lab, deg, b.P.nsg_R, b.ncap);
-        uint32_t cur_set = 0xFFFFFFFFu;
         for (uint32_t k = 0; k < na; ++k) {
             const uint32_t u = roots_of[k];
-            const uint32_t set = b.set_of_node(u);
-            if (set != cur_set) {  // roots ascend, so each set's roots come together
-                cur_set = set;
-                pick.begin_set(b.set_lo[set], b.set_n(set));
-            }
             auto reached = [&](uint32_t v) { return lab[v] == FK_REACHED || lab[v] < u; };
             const uint32_t *prow = rows.p + (size_t)k * rs;
             uint32_t root = N;
@@ -605,12 +559,37 @@ void tree_grow(BuildState &b) {
                 const uint32_t p = prow[1 + i];
                 if (p < N && reached(p) && deg[p] < b.P.nsg_R) { root = p; break; }
             }
-            if (root == N) root = pick.random_root([&] { return b.next_rand(set); });
+            const uint32_t set = b.set_of_node(u), lo = b.set_lo[set], ns = b.set_n(set);
+            if (root == N) {
+                // a random reached node of u's set, with the set's stream; the
+                // band counts are made once a fallback runs long (most end in
+                // a few draws)
+                uint64_t tries = 0;
+                while (true) {
+                    if (tries == 64 && bands.set != set) bands.reset(set, deg, lo, ns, b.P.nsg_R, b.ncap, u);
+                    const uint64_t f = (tries + 1) / ns;  // band of the next draw (factor f + 1)
+                    const bool counted = bands.set == set;
+                    if (counted && f >= bands.thr.size() - 1 && bands.low.back() == 0)
+                        throw Error(FK_E_INTERNAL, "tree_grow: no node can take an edge");
+                    if (counted && f < bands.thr.size() - 1 && bands.low[f] == 0) {
+                        // no draw of this band can succeed: skip to the next band
+                        const uint64_t skip = (f + 1) * ns - 1 - tries;
+                        b.set_rng[set] += skip * 0x9E3779B97F4A7C15ull;  // next_rand's counter step
+                        tries += skip;
+                        continue;
+                    }
+                    ++tries;
+                    const uint64_t factor = tries / ns + 1;
+                    const uint32_t rid = lo + (uint32_t)(b.next_rand(set) % ns);
+                    if (reached(rid) && deg[rid] < b.ncap && deg[rid] < b.P.nsg_R * factor) { root = rid; break; }
+                    if (tries > 64ull * ns) throw Error(FK_E_INTERNAL, "tree_grow: no node can take an edge");
+                }
+            }
             e_src.push_back(root);
             e_slot.push_back(deg[root]);
             e_dst.push_back(u);
-            pick.add_edge(root);
-            pick.reach_component(u);  // u's component is reached from here on
+            bands.add_edge(deg, root, set);
+            bands.reach(deg, u, set);
         }
         DevBuf<uint32_t> s(na), sl(na), d(na);
         FK_HIP(hipMemcpyAsync(s.p, e_src.data(), (size_t)na * 4, hipMemcpyHostToDevice, b.s));

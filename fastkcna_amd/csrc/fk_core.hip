@@ -279,9 +279,19 @@ struct PruneWave {
     }
 };
 
+// Waves per SIMD the prune is compiled for (0: the compiler's choice).
+#ifndef FK_PRUNE_WPE
+#define FK_PRUNE_WPE 0
+#endif
+#if FK_PRUNE_WPE > 0
+#define FK_PRUNE_ATTR __attribute__((amdgpu_waves_per_eu(FK_PRUNE_WPE)))
+#else
+#define FK_PRUNE_ATTR
+#endif
+
 template <int RULE, int NB, int MODE>
-__global__ __launch_bounds__(256) void k_prune(const void *__restrict__ chains_v, uint32_t dpad, PruneLaunch a,
-                                               uint32_t wave_bytes) {
+__global__ __launch_bounds__(256) FK_PRUNE_ATTR void k_prune(const void *__restrict__ chains_v, uint32_t dpad,
+                                                             PruneLaunch a, uint32_t wave_bytes) {
     using PW = PruneWave<RULE, NB, MODE>;
     using E = typename PW::E;
     using Chain = typename PW::Chain;

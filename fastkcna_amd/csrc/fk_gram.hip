@@ -3,7 +3,7 @@
 // The local join (src/kgraph.cpp:970-987) needs, per node, the distance of
 // every (new, new) and (new, old) pair of its sampled list.  Here a node's
 // pairs are 16 x 16 tiles of the Gram matrix G = X Y^T, computed by
-// v_mfma_f32_16x16x16 on 16-bit operands (BF16, or FP16 for unit-normalised
+// v_mfma_f32_16x16x32 on 16-bit operands (BF16, or FP16 for unit-normalised
 // rows) with FP32 accumulation, and
 //     d~(x, y) = |x~|^2 + |y~|^2 - 2 G(x, y)
 // with FP32 norms of the rounded rows.  Pools only ever hold exact FP32
@@ -26,7 +26,7 @@
 //       (sqrt(max(d~ - f, 0)) - e)^2 (1 - gam) > max(thr_x, thr_y),
 //     which implies c > both thresholds.
 // Exact-form rows with every value in [0, 255] are joined from their byte
-// copy instead (k_join_gram8: v_mfma_i32_16x16x32_i8 on the values shifted by
+// copy instead (k_join_gram8: v_mfma_i32_16x16x64_i8 on the values shifted by
 // -128, integer norms), with the same pairs, distances and counts.
 #include <math.h>
 
@@ -39,8 +39,9 @@
 
 namespace fk {
 
-typedef short fk_s16x4 __attribute__((ext_vector_type(4)));
-typedef _Float16 fk_f16x4 __attribute__((ext_vector_type(4)));
+typedef short fk_s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 fk_bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 fk_f16x8 __attribute__((ext_vector_type(8)));
 typedef float fk_f32x4 __attribute__((ext_vector_type(4)));
 
 // BF16 bits of x, rounded to nearest even (finite inputs).
@@ -55,42 +56,64 @@ __device__ __forceinline__ float quantize(float x) {
     else return __uint_as_float(bf16_bits(x) << 16);
 }
 
-// One K-step of a 16 x 16 x 16 tile: a, b are this lane's 4 consecutive
-// values of its A row and B row (already scaled).
+// One K-step of a 16 x 16 x 32 tile (the gfx950 16-bit MFMA shape): a, b
+// are this lane's 8 consecutive values of its A row and B row (already
+// scaled), converted here to FP16 or to BF16 rounded to nearest even.
 template <int FMT>
-__device__ __forceinline__ fk_f32x4 mfma_step(float4 a, float4 b, fk_f32x4 acc) {
+__device__ __forceinline__ fk_f32x4 mfma_step(const float (&a)[8], const float (&b)[8], fk_f32x4 acc) {
     if constexpr (FMT == FK_GRAM_F16) {
-        fk_f16x4 ha = {(_Float16)a.x, (_Float16)a.y, (_Float16)a.z, (_Float16)a.w};
-        fk_f16x4 hb = {(_Float16)b.x, (_Float16)b.y, (_Float16)b.z, (_Float16)b.w};
-        return __builtin_amdgcn_mfma_f32_16x16x16f16(ha, hb, acc, 0, 0, 0);
+        fk_f16x8 ha, hb;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            ha[i] = (_Float16)a[i];
+            hb[i] = (_Float16)b[i];
+        }
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(ha, hb, acc, 0, 0, 0);
     } else {
-        fk_s16x4 sa = {(short)bf16_bits(a.x), (short)bf16_bits(a.y), (short)bf16_bits(a.z), (short)bf16_bits(a.w)};
-        fk_s16x4 sb = {(short)bf16_bits(b.x), (short)bf16_bits(b.y), (short)bf16_bits(b.z), (short)bf16_bits(b.w)};
-        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(sa, sb, acc, 0, 0, 0);
+        fk_s16x8 sa, sb;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            sa[i] = (short)bf16_bits(a[i]);
+            sb[i] = (short)bf16_bits(b[i]);
+        }
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(fk_bf16x8, sa),
+                                                       __builtin_bit_cast(fk_bf16x8, sb), acc, 0, 0, 0);
     }
 }
 
 // 16 x 16 Gram tile.  Lane l supplies row (l & 15) of the A block (arow) and
-// of the B block (brow); it reads the values at k = 16 s + 4 (l >> 4) .. +3
-// for every step s, zero past dpad.  On return lane l holds
-// G[4 (l >> 4) + e][l & 15], e = 0..3 (the MFMA output layout), in the
-// scaled units (scale^2 G).  Every lane of the wave must call it.
+// of the B block (brow); it reads the values at k = 32 s + 8 (l >> 4) .. +7
+// for every step s, zero past dpad (a multiple of 8).  On return lane l
+// holds G[4 (l >> 4) + e][l & 15], e = 0..3 (the MFMA output layout), in
+// the scaled units (scale^2 G).  Every lane of the wave must call it.
 template <int FMT>
 __device__ __forceinline__ fk_f32x4 gram_tile(const float *__restrict__ arow, const float *__restrict__ brow,
                                               uint32_t dpad, float scale) {
     fk_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    const uint32_t kq = (lane_id() >> 4) * 4u;
-    const uint32_t steps = (dpad + 15u) >> 4;
+    const uint32_t kq = (lane_id() >> 4) * 8u;
+    const uint32_t steps = (dpad + 31u) >> 5;
     for (uint32_t s = 0; s < steps; ++s) {
-        const uint32_t k = s * 16u + kq;
-        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+        const uint32_t k = s * 32u + kq;
+        float a[8], b[8];
         if (k < dpad) {
-            a = *reinterpret_cast<const float4 *>(arow + k);
-            b = *reinterpret_cast<const float4 *>(brow + k);
+            const float4 a0 = *reinterpret_cast<const float4 *>(arow + k);
+            const float4 a1 = *reinterpret_cast<const float4 *>(arow + k + 4);
+            const float4 b0 = *reinterpret_cast<const float4 *>(brow + k);
+            const float4 b1 = *reinterpret_cast<const float4 *>(brow + k + 4);
+            a[0] = a0.x; a[1] = a0.y; a[2] = a0.z; a[3] = a0.w;
+            a[4] = a1.x; a[5] = a1.y; a[6] = a1.z; a[7] = a1.w;
+            b[0] = b0.x; b[1] = b0.y; b[2] = b0.z; b[3] = b0.w;
+            b[4] = b1.x; b[5] = b1.y; b[6] = b1.z; b[7] = b1.w;
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) a[i] = b[i] = 0.f;
         }
         if constexpr (FMT == FK_GRAM_F16) {
-            a.x *= scale; a.y *= scale; a.z *= scale; a.w *= scale;
-            b.x *= scale; b.y *= scale; b.z *= scale; b.w *= scale;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                a[i] *= scale;
+                b[i] *= scale;
+            }
         }
         acc = mfma_step<FMT>(a, b, acc);
     }
@@ -421,23 +444,33 @@ __device__ __forceinline__ void join_stage_bytes(const uint8_t *__restrict__ c8,
     }
 }
 
-// 16 x 16 tile of shifted dot products.  Lane l supplies row (l & 15) of the
-// A block (arow) and of the B block (brow), 8 bytes at k = 32 s + 8 (l >> 4)
-// for every step s (zero past dpad); A and B take the same bytes of their
-// rows, so every product pairs equal dimensions.  On return lane l holds
-// G[4 (l >> 4) + e][l & 15], e = 0..3.  Every lane of the wave must call it.
+// 16 x 16 tile of shifted dot products (the gfx950 8-bit MFMA shape,
+// 16 x 16 x 64).  Lane l supplies row (l & 15) of the A block (arow) and of
+// the B block (brow), 16 bytes at k = 64 s + 16 (l >> 4) for every step s,
+// zero past dpad (a multiple of 8, so a lane's bytes are all in, all out or
+// the first 8 in); A and B take the same bytes of their rows, so every
+// product pairs equal dimensions.  Rows are 8-byte aligned (read as two
+// 8-byte halves).  On return lane l holds G[4 (l >> 4) + e][l & 15],
+// e = 0..3.  Every lane of the wave must call it.
 __device__ __forceinline__ fk_i32x4 gram_tile8(const uint8_t *__restrict__ arow, const uint8_t *__restrict__ brow,
                                                uint32_t dpad) {
     fk_i32x4 acc = {0, 0, 0, 0};
-    const uint32_t kq = (lane_id() >> 4) * 8u;
-    for (uint32_t k0 = 0; k0 < dpad; k0 += 32u) {
+    const uint32_t kq = (lane_id() >> 4) * 16u;
+    constexpr uint64_t flip = 0x8080808080808080ull;
+    for (uint32_t k0 = 0; k0 < dpad; k0 += 64u) {
         const uint32_t k = k0 + kq;
-        long a = 0, b = 0;
+        uint64_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
         if (k < dpad) {
-            a = (long)(*reinterpret_cast<const uint64_t *>(arow + k) ^ 0x8080808080808080ull);
-            b = (long)(*reinterpret_cast<const uint64_t *>(brow + k) ^ 0x8080808080808080ull);
+            a0 = *reinterpret_cast<const uint64_t *>(arow + k) ^ flip;
+            b0 = *reinterpret_cast<const uint64_t *>(brow + k) ^ flip;
+            if (k + 8u < dpad) {
+                a1 = *reinterpret_cast<const uint64_t *>(arow + k + 8) ^ flip;
+                b1 = *reinterpret_cast<const uint64_t *>(brow + k + 8) ^ flip;
+            }
         }
-        acc = __builtin_amdgcn_mfma_i32_16x16x32_i8(a, b, acc, 0, 0, 0);
+        const fk_i32x4 a = {(int)(uint32_t)a0, (int)(uint32_t)(a0 >> 32), (int)(uint32_t)a1, (int)(uint32_t)(a1 >> 32)};
+        const fk_i32x4 b = {(int)(uint32_t)b0, (int)(uint32_t)(b0 >> 32), (int)(uint32_t)b1, (int)(uint32_t)(b1 >> 32)};
+        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc, 0, 0, 0);
     }
     return acc;
 }

@@ -11,6 +11,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <memory>
 
 #include "fk_api_impl.h"
 #include "fk_build.h"
@@ -443,9 +444,12 @@ struct BandCounts {
     std::vector<uint32_t> thr, low;
     uint32_t set = 0xFFFFFFFFu;
     const uint32_t *lab = nullptr;
-    std::vector<uint32_t> unr;  // unreached nodes ordered by (label, id), made on first use
-    bool sorted = false;
-    void reset(uint32_t set_, const uint32_t *deg, uint32_t lo, uint32_t n, uint32_t R, uint32_t cap, uint32_t u) {
+    const std::vector<uint32_t> *roots = nullptr;  // attached ids, ascending
+    std::unique_ptr<uint32_t[]> kidx;              // root id -> its index in roots (written at roots only)
+    std::vector<uint32_t> moff, mem;               // the set's unreached nodes grouped by root
+    uint32_t k_lo = 0;                             // index of the set's first root
+    void reset(uint32_t set_, const uint32_t *deg, uint32_t lo, uint32_t n, uint32_t R, uint32_t cap, uint32_t u,
+               uint32_t N) {
         set = set_;
         thr.clear();
         for (uint32_t f = 1;; ++f) {
@@ -454,13 +458,27 @@ struct BandCounts {
             if (t == cap) break;
         }
         low.assign(thr.size(), 0);
-        for (uint32_t v = lo; v < lo + n; ++v)
-            if (lab[v] == FK_REACHED || lab[v] < u)
-                for (size_t f = 0; f < thr.size(); ++f) low[f] += deg[v] < thr[f] ? 1u : 0u;
-        if (!sorted) {
-            std::stable_sort(unr.begin(), unr.end(), [&](uint32_t a, uint32_t b) { return lab[a] < lab[b]; });
-            sorted = true;
+        const std::vector<uint32_t> &rs = *roots;
+        if (!kidx) {
+            kidx.reset(new uint32_t[N]);
+            for (uint32_t k = 0; k < (uint32_t)rs.size(); ++k) kidx[rs[k]] = k;
         }
+        k_lo = (uint32_t)(std::lower_bound(rs.begin(), rs.end(), lo) - rs.begin());
+        const uint32_t k_hi = (uint32_t)(std::lower_bound(rs.begin(), rs.end(), lo + n) - rs.begin());
+        moff.assign((size_t)(k_hi - k_lo) + 1, 0);
+        // an unreached node's label is its component's smallest id, a root
+        for (uint32_t v = lo; v < lo + n; ++v) {
+            if (lab[v] == FK_REACHED || lab[v] < u) {
+                for (size_t f = 0; f < thr.size(); ++f) low[f] += deg[v] < thr[f] ? 1u : 0u;
+            } else {
+                ++moff[kidx[lab[v]] - k_lo + 1];
+            }
+        }
+        for (size_t k = 1; k < moff.size(); ++k) moff[k] += moff[k - 1];
+        mem.resize(moff.back());
+        std::vector<uint32_t> fill(moff.begin(), moff.end() - 1);
+        for (uint32_t v = lo; v < lo + n; ++v)
+            if (lab[v] != FK_REACHED && lab[v] >= u) mem[fill[kidx[lab[v]] - k_lo]++] = v;
     }
     void add_edge(uint32_t *deg, uint32_t v, uint32_t set_of_v) {  // deg[v] grows by one
         if (set_of_v == set)
@@ -469,9 +487,9 @@ struct BandCounts {
     }
     void reach(const uint32_t *deg, uint32_t u, uint32_t set_of_u) {  // component u is now reached
         if (set_of_u != set) return;
-        auto it = std::lower_bound(unr.begin(), unr.end(), u, [&](uint32_t a, uint32_t key) { return lab[a] < key; });
-        for (; it != unr.end() && lab[*it] == u; ++it)
-            for (size_t f = 0; f < thr.size(); ++f) low[f] += deg[*it] < thr[f] ? 1u : 0u;
+        const uint32_t k = kidx[u] - k_lo;
+        for (uint32_t e = moff[k]; e < moff[k + 1]; ++e)
+            for (size_t f = 0; f < thr.size(); ++f) low[f] += deg[mem[e]] < thr[f] ? 1u : 0u;
     }
 };
 
@@ -529,12 +547,11 @@ void tree_grow(BuildState &b) {
     b.prof_end(pe, FK_PROF_CONN, (double)(rounds + levels) * ((double)N * 8.0 + 8.0 * (double)N * b.P.nsg_R));
     b.prof_flush();
     std::vector<uint32_t> roots_of;  // attached ids, ascending
+    for (uint32_t v = 0; v < N; ++v)
+        if (lab[v] == v) roots_of.push_back(v);
     BandCounts bands;
     bands.lab = lab;
-    for (uint32_t v = 0; v < N; ++v) {
-        if (lab[v] == v) roots_of.push_back(v);
-        if (lab[v] != FK_REACHED) bands.unr.push_back(v);
-    }
+    bands.roots = &roots_of;
     const uint32_t na = (uint32_t)roots_of.size();
     std::vector<uint32_t> e_src, e_slot, e_dst;
     if (na) {
@@ -562,11 +579,11 @@ void tree_grow(BuildState &b) {
             const uint32_t set = b.set_of_node(u), lo = b.set_lo[set], ns = b.set_n(set);
             if (root == N) {
                 // a random reached node of u's set, with the set's stream; the
-                // band counts are made once a fallback runs long (most end in
-                // a few draws)
+                // band counts (a pass over the set) are made once a fallback
+                // runs long: most end in a few draws
                 uint64_t tries = 0;
                 while (true) {
-                    if (tries == 64 && bands.set != set) bands.reset(set, deg, lo, ns, b.P.nsg_R, b.ncap, u);
+                    if (tries == 4096 && bands.set != set) bands.reset(set, deg, lo, ns, b.P.nsg_R, b.ncap, u, N);
                     const uint64_t f = (tries + 1) / ns;  // band of the next draw (factor f + 1)
                     const bool counted = bands.set == set;
                     if (counted && f >= bands.thr.size() - 1 && bands.low.back() == 0)

@@ -89,9 +89,51 @@ void launch_join_distances(hipStream_t s, const Dataset &ds, const uint32_t *ids
 // while batch b is decided, and the chains of the first PruneMC kept
 // neighbours (which decide most occlusions) are kept in LDS, so a batch
 // rarely waits on a dependent global load.
-template <int NB>
+// Rows of the first kept neighbours cached in LDS per wave: a byte budget
+// and a row cap per rule and row kind, and the number of them (FAST) an
+// octet checks its own candidate against before the survivors go on one at
+// a time.  Under the NSG and tau rules most candidates fall to one of the
+// first few kept neighbours, and a longer check of one octet only holds the
+// other seven; the alpha rule at small angles occludes rarely, so every
+// candidate is checked against most of the kept list and each cached row
+// saves a dependent global read.  Byte rows are small enough to cache 32,
+// read by the survivor steps.  (Measured on the 1M x 128 NSG build and the
+// 256-set alpha-PG batch.)
+#ifndef FK_PRUNE_MC_BYTES
+#define FK_PRUNE_MC_BYTES 4096
+#endif
+#ifndef FK_PRUNE_MC_CAP
+#define FK_PRUNE_MC_CAP 8
+#endif
+#ifndef FK_PRUNE_MC_BYTES_ALPHA
+#define FK_PRUNE_MC_BYTES_ALPHA 8192
+#endif
+#ifndef FK_PRUNE_MC_CAP_ALPHA
+#define FK_PRUNE_MC_CAP_ALPHA 16
+#endif
+#ifndef FK_PRUNE_MC_BYTES8
+#define FK_PRUNE_MC_BYTES8 4096
+#endif
+#ifndef FK_PRUNE_MC_CAP8
+#define FK_PRUNE_MC_CAP8 32
+#endif
+#ifndef FK_PRUNE_FAST
+#define FK_PRUNE_FAST 64
+#endif
+#ifndef FK_PRUNE_FAST8
+#define FK_PRUNE_FAST8 8
+#endif
+#ifndef FK_PRUNE_DB
+#define FK_PRUNE_DB 1  // cached kept chains double-buffered in registers
+#endif
+template <int RULE, int NB, int MODE>
 struct PruneMC {
-    static constexpr uint32_t value = NB == 0 ? 0u : ((4096u / (NB * 32u)) < 8u ? (4096u / (NB * 32u)) : 8u);
+    static constexpr bool wide = RULE == FK_RULE_ALPHA && MODE != 2;
+    static constexpr uint32_t row = (uint32_t)NB * 8u * (MODE == 2 ? 1u : 4u);
+    static constexpr uint32_t budget = MODE == 2 ? FK_PRUNE_MC_BYTES8 : wide ? FK_PRUNE_MC_BYTES_ALPHA : FK_PRUNE_MC_BYTES;
+    static constexpr uint32_t cap = MODE == 2 ? FK_PRUNE_MC_CAP8 : wide ? FK_PRUNE_MC_CAP_ALPHA : FK_PRUNE_MC_CAP;
+    static constexpr uint32_t raw = NB == 0 ? 0u : budget / row;
+    static constexpr uint32_t value = raw < cap ? raw : cap;
 };
 
 // LDS per wave: the cached kept chains (mc x dpad), one staging row
@@ -135,10 +177,16 @@ struct PruneWave {
         const uint32_t id = key_id(key);
         return nrm[(key != FK_KEY_EMPTY && id < oracle_n) ? id : 0u].x;
     }
+    // Distance of this octet's candidate to kept neighbour t: its cached row
+    // from LDS, else its row from global memory (two branches, so each load
+    // keeps its address space instead of becoming a flat load).
     __device__ float member_dist(const Chain &crow, float cn, uint32_t t, uint32_t tid) const {
-        const E *o = t < PruneMC<NB>::value ? lane_cache + (size_t)t * dpad : lane_chains + (size_t)tid * dpad;
-        if constexpr (DOT) return exact_dist(cn, sn[t], crow.dot(o, nb));
-        else return crow.dist(o, nb);
+        auto dist_to = [&](const E *o) __attribute__((always_inline)) -> float {
+            if constexpr (DOT) return exact_dist(cn, sn[t], crow.dot(o, nb));
+            else return crow.dist(o, nb);
+        };
+        if (t < PruneMC<RULE, NB, MODE>::value) return dist_to(lane_cache + (size_t)t * dpad);
+        return dist_to(lane_chains + (size_t)tid * dpad);
     }
     // Decides the batch at pos whose key (this octet's), chain and norm
     // (DOT) are given.
@@ -152,7 +200,13 @@ struct PruneWave {
         uint32_t mycc = 0;
         __builtin_amdgcn_wave_barrier();
         const uint32_t n0 = nout;
-        const uint32_t nA = n0 < PruneMC<NB>::value ? n0 : PruneMC<NB>::value;
+        constexpr uint32_t MC = PruneMC<RULE, NB, MODE>::value;
+        // each octet checks its own candidate against the first nA kept
+        // neighbours; the survivors go on one at a time, the wave's octets
+        // taking eight kept neighbours per step
+        constexpr uint32_t FAST = MODE == 2 ? FK_PRUNE_FAST8 : FK_PRUNE_FAST;
+        constexpr uint32_t MF = MC < FAST ? MC : FAST;
+        const uint32_t nA = n0 < MF ? n0 : MF;
         if (!skip) {
             // the cached kept neighbours decide most candidates; octet-
             // uniform: every lane of an octet sees the same pid, tid, djk.
@@ -160,7 +214,7 @@ struct PruneWave {
             // (two register chains, unrolled by two so neither is copied;
             // chains of up to 16 registers, where the second one costs no
             // occupancy)
-            if constexpr (PruneMC<NB>::value > 0 && sizeof(Chain) <= 64) {
+            if constexpr (FK_PRUNE_DB && PruneMC<RULE, NB, MODE>::value > 0 && sizeof(Chain) <= 64) {
                 auto check = [&](uint32_t t, const Chain &o) __attribute__((always_inline)) -> bool {
                     if (so[t] == pid) return true;
                     ++mycc;
@@ -218,9 +272,15 @@ struct PruneWave {
                         if (tid == spid) {
                             hit = isdup = true;
                         } else {
-                            const float djk =
-                                DOT ? exact_dist(spn, sn[t], chain_dot2<NB>(cp, lane_chains + (size_t)tid * dpad, nb))
-                                    : chain_dist2<NB>(cp, lane_chains + (size_t)tid * dpad, nb);
+                            // a cached row from LDS, else from global memory (two
+                            // branches, so each load keeps its address space)
+                            auto dist_to = [&](const E *kr) __attribute__((always_inline)) -> float {
+                                return DOT ? exact_dist(spn, sn[t], chain_dot2<NB>(cp, kr, nb))
+                                           : chain_dist2<NB>(cp, kr, nb);
+                            };
+                            float djk;
+                            if (MF < MC && t < MC) djk = dist_to(lane_cache + (size_t)t * dpad);
+                            else djk = dist_to(lane_chains + (size_t)tid * dpad);
                             hit = occludes<RULE>(djk, spd, sd[t], tau3, cos_thr);
                         }
                     }
@@ -256,7 +316,7 @@ struct PruneWave {
                 sd[slot] = ad;
                 if (DOT) sn[slot] = an;
             }
-            if (slot < PruneMC<NB>::value && g == al) crow.store(lane_cache + (size_t)slot * dpad, nb);
+            if (slot < PruneMC<RULE, NB, MODE>::value && g == al) crow.store(lane_cache + (size_t)slot * dpad, nb);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             ++nout;
@@ -298,7 +358,7 @@ __global__ __launch_bounds__(256) FK_PRUNE_ATTR void k_prune(const void *__restr
     constexpr bool DOT = PW::DOT;
     const E *chains = reinterpret_cast<const E *>(chains_v);
     extern __shared__ __align__(16) uint8_t smem[];
-    constexpr uint32_t MC = PruneMC<NB>::value;
+    constexpr uint32_t MC = PruneMC<RULE, NB, MODE>::value;
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t vb = a.xcd_order ? xcd_position((gridDim.x + FK_NUM_XCD - 1) / FK_NUM_XCD) : blockIdx.x;
@@ -400,13 +460,13 @@ template <int RULE, int NB>
 static void launch_prune_nb(hipStream_t s, const Dataset &ds, const PruneLaunch &a, unsigned grid) {
     if constexpr (NB == 16 || NB == 32) {
         if (a.nrm && a.chains8) {
-            const uint32_t wb = prune_wave_bytes(PruneMC<NB>::value, ds.dpad, a.R, 1);
+            const uint32_t wb = prune_wave_bytes(PruneMC<RULE, NB, 2>::value, ds.dpad, a.R, 1);
             hipLaunchKernelGGL((k_prune<RULE, NB, 2>), dim3(grid), dim3(64 * FK_PRUNE_WPB),
                                (size_t)wb * FK_PRUNE_WPB, s, (const void *)a.chains8, ds.dpad, a, wb);
             return;
         }
     }
-    const uint32_t wave_bytes = prune_wave_bytes(PruneMC<NB>::value, ds.dpad, a.R, 4);
+    const uint32_t wave_bytes = prune_wave_bytes(PruneMC<RULE, NB, 0>::value, ds.dpad, a.R, 4);  // modes 0 and 1
     if constexpr (NB == 12 || NB == 16 || NB == 32) {
         if (a.nrm) {
             hipLaunchKernelGGL((k_prune<RULE, NB, 1>), dim3(grid), dim3(64 * FK_PRUNE_WPB),

@@ -227,11 +227,17 @@ def cpu_full_leg(dev, base, queries, gpu_graph, gpu_stats, threads):
     dev.upload(base)
     gt, _ = dev.knn(queries, 10)
     goff, gids, gep = gpu_graph
+    # each graph from its own entry point, and each from the other's: the
+    # two NN-descent runs end in different (equally good) pools, so the
+    # entry-point searches stop at different nodes, and on this data the
+    # start decides much of recall@10 at small search_L
     curve = {}
     for sl in (80, 200, 500):
         curve[str(sl)] = {"cpu": recall_at_10(dev.search(off, ids, ep, queries, sl, 10)[0], gt),
-                          "gpu": recall_at_10(dev.search(goff, gids, gep, queries, sl, 10)[0], gt)}
-    return {"cpu_build_sec": dt, "threads": threads, "host": host_cpu_info(),
+                          "gpu": recall_at_10(dev.search(goff, gids, gep, queries, sl, 10)[0], gt),
+                          "cpu_from_gpu_ep": recall_at_10(dev.search(off, ids, gep, queries, sl, 10)[0], gt),
+                          "gpu_from_cpu_ep": recall_at_10(dev.search(goff, gids, ep, queries, sl, 10)[0], gt)}
+    return {"cpu_build_sec": dt, "threads": threads, "host": host_cpu_info(), "cpu_ep": int(ep), "gpu_ep": int(gep),
             "cpu_dist_evals": total_dists(st), "gpu_build_sec": float(gpu_stats["t_total"]),
             "recall_at_10_by_search_L": curve,
             "cpu_knn_recall_after_nndescent": after_nndescent(st),

@@ -51,10 +51,13 @@ def total_dists(st, computed=False):
     """Distance evaluations of a build.  By default BridgeView pairs are
     counted as the reference counts them (every pair it evaluates); with
     computed=True only the pairs the GPU kernel actually computed (it skips
-    targets already pooled or computed in the same round)."""
+    targets already pooled or computed in the same round), and without the
+    tau-MNG prune distances that cannot decide anything (candidates within
+    3 tau of the owner)."""
     keys = ("d_init", "d_ctrl", "d_join", "d_prune", "d_prune_rev", "d_inter", "d_ep", "d_conn")
     bv = int(st.get("d_bv_computed", st["d_bv"])) if computed else int(st["d_bv"])
-    return sum(int(st[k]) for k in keys) + bv
+    skipped = int(st.get("d_prune_uncomputed", 0)) if computed else 0
+    return sum(int(st[k]) for k in keys) + bv - skipped
 
 
 def after_nndescent(st):
@@ -400,8 +403,13 @@ def c3_leg(dev):
     out = {"workload": "C3: tau-MNG, 1,000,000x960 float32 mixture of 256 Gaussians (seed 2, sigma 0.05), K=L=500 "
                        "S=12 R=100 I=6 nsg_R=50 tau=10 search_L=80 search_K=500",
            "build_sec": plain, "dist_evals_per_sec": total_dists(st) / plain,
+           "dist_evals_computed_per_sec": total_dists(st, computed=True) / plain,
+           "d_prune": int(st["d_prune"]), "d_prune_rev": int(st["d_prune_rev"]),
+           "d_prune_uncomputed": int(st["d_prune_uncomputed"]),
            "knn_recall_at_maxl": knn_recall_entries(st), "query_recall_at_10": rec,
            "families_ms": fam(st),
+           "phase_sec": {k: round(float(st[k]), 4) for k in ("t_ctrl", "t_nnd", "t_prune", "t_inter", "t_ep", "t_conn",
+                                                            "t_bv", "t_total")},
            "gram": {"build_sec": chosen, "speedup_vs_plain": plain / chosen,
                     "gram_mode": _ffi.GRAM_MODE_NAMES[int(gst["gram_mode"])], "graph_identical_to_plain": same,
                     "gram_pairs": int(gst["d_gram"]), "fp32_rechecks": int(gst["d_recheck"]),

@@ -74,7 +74,7 @@ class FkStats(ctypes.Structure):
         ("d_screened", u64), ("d_screen_skip", u64), ("screen_audit", u64), ("screen_missed", u64),
         ("screen_margin", f32), ("screen_loss", f32), ("screen_train_pairs", u32), ("reserved1", u32),
         ("t_screen", f64), ("screen_calib_pairs", u64), ("screen_cands", u64),
-        ("d_bv_computed", u64),
+        ("d_bv_computed", u64), ("d_prune_uncomputed", u64),
     ]
 
     def to_dict(self):

@@ -323,6 +323,7 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
     const unsigned gw = (N + 3) / 4;
     // 1. prune every pool
     b.tally.clear(b.s, FK_T_PRUNE_CC, 2);
+    b.tally.clear(b.s, FK_T_PRUNE_SKIP);
     PruneLaunch a;
     a.nrm = exact_norms(b);
     a.chains8 = a.nrm ? b.ds->chains8.p : nullptr;
@@ -342,6 +343,7 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
     a.out_len = b.nsg_len.p;
     a.cc = b.tally.at(FK_T_PRUNE_CC);
     a.examined = b.tally.at(FK_T_PRUNE_EXAM);
+    a.uncomputed = b.tally.at(FK_T_PRUNE_SKIP);
     if (b.has_order) {
         a.task_list = b.order.p;
         a.xcd_order = true;
@@ -419,6 +421,7 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
         b.prof_flush();
         b.st->d_prune_rev += rc;
     }
+    if (rule_of(type) == FK_RULE_TAU) b.st->d_prune_uncomputed += b.tally.read1(b.s, FK_T_PRUNE_SKIP);
     const double t2 = now_sec();
     if (li < FK_MAX_LOOPS) b.st->inter_time[li] = t2 - t1;
     b.st->t_inter += t2 - t1;

@@ -163,7 +163,7 @@ struct PruneWave {
     uint32_t dpad, nb, g, k, lane, limit;
     uint32_t nout;
     bool stop;
-    unsigned long long cc, exam;
+    unsigned long long cc, exam, uncomputed;
 
     __device__ uint64_t key_at(const uint64_t *c, uint32_t idx) const {
         return idx < limit ? c[idx] : FK_KEY_EMPTY;
@@ -196,6 +196,11 @@ struct PruneWave {
         const uint32_t pid = key_id(key);
         const float pd = key_dist(key);
         const bool skip = !valid || pid >= oracle_n;
+        // tau rule with pd - 3 tau <= 0: djk < pd - 3 tau never holds for a
+        // squared distance djk >= 0, so no kept neighbour can occlude the
+        // candidate and only one with its id removes it; the distances the
+        // sequential loop evaluates are counted but not computed
+        const bool nodist = RULE == FK_RULE_TAU && !(pd - tau3 > 0.f);
         bool occl = false;
         uint32_t mycc = 0;
         __builtin_amdgcn_wave_barrier();
@@ -207,7 +212,12 @@ struct PruneWave {
         constexpr uint32_t FAST = MODE == 2 ? FK_PRUNE_FAST8 : FK_PRUNE_FAST;
         constexpr uint32_t MF = MC < FAST ? MC : FAST;
         const uint32_t nA = n0 < MF ? n0 : MF;
-        if (!skip) {
+        if (!skip && nodist) {
+            uint32_t t = 0;
+            while (t < n0 && so[t] != pid) ++t;
+            mycc += t;
+            occl = t < n0;
+        } else if (!skip) {
             // the cached kept neighbours decide most candidates; octet-
             // uniform: every lane of an octet sees the same pid, tid, djk.
             // The next cached chain is read from LDS while one is checked
@@ -245,7 +255,7 @@ struct PruneWave {
             // neighbours per step (one per octet) so their rows load
             // together; the first occluder in kept order ends the check,
             // and the distances counted are those the sequential loop makes
-            uint64_t surv = wave_ballot(!skip && !occl && k == 0);
+            uint64_t surv = wave_ballot(!skip && !nodist && !occl && k == 0);
             while (surv) {
                 const uint32_t sl = (uint32_t)__builtin_ctzll(surv);
                 surv &= surv - 1;
@@ -327,13 +337,16 @@ struct PruneWave {
                     occl = true;
                 } else {
                     ++mycc;
-                    const float djk = member_dist(crow, cn, slot, aid);
-                    if (occludes<RULE>(djk, pd, ad, tau3, cos_thr)) occl = true;
+                    if (!nodist) {
+                        const float djk = member_dist(crow, cn, slot, aid);
+                        if (occludes<RULE>(djk, pd, ad, tau3, cos_thr)) occl = true;
+                    }
                 }
             }
         }
         if (k == 0 && g <= stop_g) {
             cc += mycc;
+            if (nodist && !skip) uncomputed += mycc;
             exam += skip ? 0u : 1u;
         }
     }
@@ -392,6 +405,7 @@ __global__ __launch_bounds__(256) FK_PRUNE_ATTR void k_prune(const void *__restr
     pw.nrm = a.nrm;
     pw.cc = 0;
     pw.exam = 0;
+    pw.uncomputed = 0;
 
     uint32_t start = 0;
     if (len > 0 && key_id(c[0]) == q) start = 1;
@@ -453,6 +467,10 @@ __global__ __launch_bounds__(256) FK_PRUNE_ATTR void k_prune(const void *__restr
     if (a.examined) {
         const unsigned long long s = wave_sum(pw.exam);
         if (lane == 0 && s) tally_add(a.examined, s);
+    }
+    if (RULE == FK_RULE_TAU && a.uncomputed) {
+        const unsigned long long s = wave_sum(pw.uncomputed);
+        if (lane == 0 && s) tally_add(a.uncomputed, s);
     }
 }
 

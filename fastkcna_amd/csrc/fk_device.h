@@ -227,6 +227,50 @@ struct OctChain {
         return octet_sum(acc);
     }
 };
+// Chain sums of two chains given by pointers, any length nb: 16-byte
+// loads when nb % 4 == 0 (every chain then starts 16-byte aligned), four of
+// them in flight ahead of the dependent fma chain; the fma order is the
+// element order either way.
+__device__ __forceinline__ float chain_l2_acc(const float *__restrict__ a, const float *__restrict__ b, uint32_t nb) {
+    float acc = 0.f, t;
+    if ((nb & 3u) == 0) {
+        const float4 *a4 = reinterpret_cast<const float4 *>(a);
+        const float4 *b4 = reinterpret_cast<const float4 *>(b);
+#pragma unroll 4
+        for (uint32_t q = 0; q < (nb >> 2); ++q) {
+            const float4 x = a4[q], y = b4[q];
+            t = x.x - y.x; acc = __builtin_fmaf(t, t, acc);
+            t = x.y - y.y; acc = __builtin_fmaf(t, t, acc);
+            t = x.z - y.z; acc = __builtin_fmaf(t, t, acc);
+            t = x.w - y.w; acc = __builtin_fmaf(t, t, acc);
+        }
+    } else {
+        for (uint32_t j = 0; j < nb; ++j) {
+            t = a[j] - b[j];
+            acc = __builtin_fmaf(t, t, acc);
+        }
+    }
+    return acc;
+}
+__device__ __forceinline__ float chain_dot_acc(const float *__restrict__ a, const float *__restrict__ b, uint32_t nb) {
+    float acc = 0.f;
+    if ((nb & 3u) == 0) {
+        const float4 *a4 = reinterpret_cast<const float4 *>(a);
+        const float4 *b4 = reinterpret_cast<const float4 *>(b);
+#pragma unroll 4
+        for (uint32_t q = 0; q < (nb >> 2); ++q) {
+            const float4 x = a4[q], y = b4[q];
+            acc = __builtin_fmaf(x.x, y.x, acc);
+            acc = __builtin_fmaf(x.y, y.y, acc);
+            acc = __builtin_fmaf(x.z, y.z, acc);
+            acc = __builtin_fmaf(x.w, y.w, acc);
+        }
+    } else {
+        for (uint32_t j = 0; j < nb; ++j) acc = __builtin_fmaf(a[j], b[j], acc);
+    }
+    return acc;
+}
+
 template <>
 struct OctChain<0> {
     const float *p;
@@ -234,18 +278,11 @@ struct OctChain<0> {
     __device__ __forceinline__ void store(float *__restrict__, uint32_t) const {}
     __device__ __forceinline__ float dist_reg(const OctChain &o, uint32_t nb) const { return dist(o.p, nb); }
     __device__ __forceinline__ float dist(const float *__restrict__ o, uint32_t nb) const {
-        float acc = 0.f;
-        for (uint32_t j = 0; j < nb; ++j) {
-            const float t = p[j] - o[j];
-            acc = __builtin_fmaf(t, t, acc);
-        }
-        return octet_sum(acc);
+        return octet_sum(chain_l2_acc(p, o, nb));
     }
     __device__ __forceinline__ float dot_reg(const OctChain &o, uint32_t nb) const { return dot(o.p, nb); }
     __device__ __forceinline__ float dot(const float *__restrict__ o, uint32_t nb) const {
-        float acc = 0.f;
-        for (uint32_t j = 0; j < nb; ++j) acc = __builtin_fmaf(p[j], o[j], acc);
-        return octet_sum(acc);
+        return octet_sum(chain_dot_acc(p, o, nb));
     }
 };
 
@@ -310,10 +347,7 @@ __device__ __forceinline__ float octet_dist2(const float *__restrict__ a, const 
             t = x.w - y.w; acc = __builtin_fmaf(t, t, acc);
         }
     } else {
-        for (uint32_t j = 0; j < nb; ++j) {
-            t = a[j] - b[j];
-            acc = __builtin_fmaf(t, t, acc);
-        }
+        acc = chain_l2_acc(a, b, nb);
     }
     return octet_sum(acc);
 }
@@ -332,7 +366,7 @@ __device__ __forceinline__ float octet_dot2(const float *__restrict__ a, const f
             acc = __builtin_fmaf(x.w, y.w, acc);
         }
     } else {
-        for (uint32_t j = 0; j < nb; ++j) acc = __builtin_fmaf(a[j], b[j], acc);
+        acc = chain_dot_acc(a, b, nb);
     }
     return octet_sum(acc);
 }

@@ -237,6 +237,7 @@ struct PruneLaunch {
     uint32_t *out_len = nullptr;
     unsigned long long *cc = nullptr;  // accumulated distance count (tally counter)
     unsigned long long *examined = nullptr;  // accumulated examined candidates (tally counter)
+    unsigned long long *uncomputed = nullptr;  // counted distances not computed (tau rule, pd <= 3 tau)
     const float2 *nrm = nullptr;      // exact-form rows: norms for dot-product distances
     const uint8_t *chains8 = nullptr; // exact-form byte rows (chain-major), with nrm
 };

@@ -170,6 +170,13 @@ typedef struct {
      * pool or was computed earlier in the same round for the same node are
      * counted but not computed again (the result is the same). */
     uint64_t d_bv_computed;
+    /* Prune and reversed-prune distances counted in d_prune / d_prune_rev
+     * but not computed: under the tau-MNG rule a candidate whose distance to
+     * the owner is at most 3 tau cannot be occluded (djk < dist - 3 tau never
+     * holds for a squared distance), so only a kept neighbour with its id
+     * removes it and the distances the reference evaluates for it decide
+     * nothing. */
+    uint64_t d_prune_uncomputed;
 } fk_stats;
 
 /* Output graph in CSR form, allocated by the library (fk_graph_free). */

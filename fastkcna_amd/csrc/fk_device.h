@@ -132,22 +132,23 @@ __device__ __forceinline__ uint32_t chain_block(uint32_t j, uint32_t dpad) {
 // Ordered sum of the 8 accumulators of this lane's octet, in every lane of
 // the octet.  The 8 lanes must be active together.
 __device__ __forceinline__ float octet_sum(float acc) {
-    // lane i of the octet takes lane i - 1's running sum (DPP row_shr:1,
-    // within the lane's row of 16) and adds its own partial: after seven
-    // steps lane 7 holds ((p0 + p1) + p2) + ... + p7, the same additions
-    // in the same order.  Then lane 7's value to the whole octet: mirror the
-    // 8 lanes (lane 0 gets it), broadcast lane 0 of each quad, and lanes
-    // 4..7 take lanes 0..3's (row_shr:4)
+    // every lane adds its own partial to its lower neighbour's running sum
+    // (DPP row_shr:1 within the lane's row of 16): after step j lane L
+    // holds the left-to-right sum of the partials of lanes L - j .. L, so
+    // after seven steps lane 7 of the octet holds ((p0 + p1) + p2) + ... +
+    // p7, the same additions in the same order (what lanes 0..6 and the
+    // lanes reading across an octet boundary hold is not used).  Then lane
+    // 7's value to the whole octet: mirror the 8 lanes (lane 0 gets it),
+    // broadcast lane 0 of each quad, and lanes 4..7 take lanes 0..3's
+    // (row_shr:4)
     const uint32_t k = threadIdx.x & 7u;
     float s = acc;
 #pragma unroll
-    for (uint32_t i = 1; i < 8; ++i) {
-        const float t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(s), 0x111, 0xF, 0xF, false));
-        s = (k == i) ? t + acc : s;
-    }
-    const int m = __builtin_amdgcn_update_dpp(0, __float_as_int(s), 0x141, 0xF, 0xF, false);  // row_half_mirror
-    const int q = __builtin_amdgcn_update_dpp(0, m, 0x00, 0xF, 0xF, false);                    // quad_perm [0,0,0,0]
-    const int h = __builtin_amdgcn_update_dpp(0, q, 0x114, 0xF, 0xF, false);                   // row_shr:4
+    for (uint32_t i = 1; i < 8; ++i)
+        s = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(s), 0x111, 0xF, 0xF, true)) + acc;
+    const int m = __builtin_amdgcn_mov_dpp(__float_as_int(s), 0x141, 0xF, 0xF, true);  // row_half_mirror
+    const int q = __builtin_amdgcn_mov_dpp(m, 0x00, 0xF, 0xF, true);                    // quad_perm [0,0,0,0]
+    const int h = __builtin_amdgcn_mov_dpp(q, 0x114, 0xF, 0xF, true);                   // row_shr:4
     return __int_as_float(k < 4 ? q : h);
 }
 

@@ -201,6 +201,20 @@ def cpu_baseline(n_sample, threads, dev, queries):
                           "gpu": recall_at_10(dev.search(goff, gids, gep, queries, sl, 10)[0], gt)}
     kc, kg = float(st["recall"][st["n_recall"] - 1]), float(gst["recall"][gst["n_recall"] - 1])
     kc0, kg0 = after_nndescent(st), after_nndescent(gst)
+    # noise band of the KNN recall: each build estimates it on its own random
+    # controls and random streams, so both sides are rebuilt with two more
+    # seeds and the spread of each is reported beside the comparison
+    band = {"seeds": [int(p.seed)], "cpu_after_nndescent": [kc0], "gpu_after_nndescent": [kg0],
+            "cpu_final": [kc], "gpu_final": [kg]}
+    for seed in (p.seed + 1, p.seed + 2):
+        q = p.copy(seed=seed)
+        *_, cst, _ = oracle.build(FK_PG_NSG, x, q, threads=threads)
+        *_, gst2 = dev.build(FK_PG_NSG, q)
+        band["seeds"].append(int(seed))
+        band["cpu_after_nndescent"].append(after_nndescent(cst))
+        band["gpu_after_nndescent"].append(after_nndescent(gst2))
+        band["cpu_final"].append(float(cst["recall"][cst["n_recall"] - 1]))
+        band["gpu_final"].append(float(gst2["recall"][gst2["n_recall"] - 1]))
     return {"value": nd / dt, "unit": "dist-evals/s", "cores": threads, "kind": "port",
             "sample": f"CPU oracle (C++/OpenMP restatement of the reference) NSG build of the first {n_sample:,} "
                       f"points of the same mixture, same parameters; {dt:.1f} s, {nd:,} distance evaluations",
@@ -210,7 +224,8 @@ def cpu_baseline(n_sample, threads, dev, queries):
                               "cpu_recall_at_10": rc, "gpu_recall_at_10": rg,
                               "recall_at_10_by_search_L": curve,
                               "cpu_knn_recall_after_nndescent": kc0, "gpu_knn_recall_after_nndescent": kg0,
-                              "cpu_knn_recall_at_maxl": kc, "gpu_knn_recall_at_maxl": kg}}
+                              "cpu_knn_recall_at_maxl": kc, "gpu_knn_recall_at_maxl": kg,
+                              "knn_recall_seed_band": band}}
 
 
 def cpu_full_leg(dev, base, queries, gpu_graph, gpu_stats, threads):

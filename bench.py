@@ -447,6 +447,8 @@ def c3_leg(dev):
            "dist_evals_computed_per_sec": total_dists(st, computed=True) / plain,
            "d_prune": int(st["d_prune"]), "d_prune_rev": int(st["d_prune_rev"]),
            "d_prune_uncomputed": int(st["d_prune_uncomputed"]),
+           "prune_variant": int(st["prune_variant"]) - 1, "prune_rev_variant": int(st["prune_rev_variant"]) - 1,
+           "tuning_sec": float(st["t_tune"]),
            "knn_recall_at_maxl": knn_recall_entries(st), "query_recall_at_10": rec,
            "families_ms": fam(st),
            "phase_sec": {k: round(float(st[k]), 4) for k in ("t_ctrl", "t_nnd", "t_prune", "t_inter", "t_ep", "t_conn",
@@ -612,6 +614,7 @@ def main():
                    "dist_evals_computed_per_sec": total_dists(st, computed=True) / (ms / 1000.0),
                    "d_bv": int(st["d_bv"]), "d_bv_computed": int(st["d_bv_computed"]),
                    "aod": int(st["aod"]), "nnd_iterations": int(st["nnd_iterations"]),
+                   "prune_variant": int(st["prune_variant"]) - 1, "prune_rev_variant": int(st["prune_rev_variant"]) - 1,
                    "graph_digest": digest,
                    "phase_sec": {k: round(float(st[k]), 4) for k in ("t_ctrl", "t_nnd", "t_prune", "t_inter",
                                                                     "t_ep", "t_conn", "t_bv", "t_total")}},

@@ -22,6 +22,8 @@ FK_FLAG_GRAM_BF16 = 0x10
 FK_FLAG_GRAM_F16 = 0x20
 FK_FLAG_SCREEN = 0x40
 FK_FLAG_SCREEN_NOTRAIN = 0x80
+FK_FLAG_PRUNE_V0 = 0x100
+FK_FLAG_PRUNE_V1 = 0x200
 FK_GRAM_NONE, FK_GRAM_EXACT, FK_GRAM_BF16, FK_GRAM_F16 = range(4)
 GRAM_MODE_NAMES = {FK_GRAM_NONE: "none", FK_GRAM_EXACT: "exact", FK_GRAM_BF16: "bf16", FK_GRAM_F16: "f16"}
 FK_MAX_LOOPS = 16
@@ -75,6 +77,7 @@ class FkStats(ctypes.Structure):
         ("screen_margin", f32), ("screen_loss", f32), ("screen_train_pairs", u32), ("reserved1", u32),
         ("t_screen", f64), ("screen_calib_pairs", u64), ("screen_cands", u64),
         ("d_bv_computed", u64), ("d_prune_uncomputed", u64),
+        ("prune_variant", u32), ("prune_rev_variant", u32), ("t_tune", f64),
     ]
 
     def to_dict(self):

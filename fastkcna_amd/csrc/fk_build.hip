@@ -316,6 +316,83 @@ __global__ void k_pack_csr(const uint32_t *__restrict__ nsg_id, const uint32_t *
     for (uint32_t k = lane_id(); k < deg; k += 64) out[off[n] + k] = nsg_id[(uint64_t)n * ncap + k];
 }
 
+// The prune of one link step with the kernel variant for its shape: forced
+// by a flag, or taken from the handle, or -- on the shape's first use --
+// measured: the launch runs once with each variant (HIP events around each),
+// the variants' out-lists (order-independent digest) and distance counts
+// must agree, and the faster variant is kept on the handle.  The last run's
+// outputs and tallies stay, as after a single launch.  Returns 1 + variant.
+static uint32_t tuned_prune(BuildState &b, PruneLaunch &a, bool reversed) {
+    const uint32_t flags = b.P.flags;
+    if (flags & (FK_FLAG_PRUNE_V0 | FK_FLAG_PRUNE_V1)) {
+        a.variant = (flags & FK_FLAG_PRUNE_V1) ? 1 : 0;
+        launch_prune(b.s, *b.ds, a);
+        return 1u + (uint32_t)a.variant;
+    }
+    const uint32_t kind = a.chains8 ? 2u : (a.nrm ? 1u : 0u);
+    const std::array<uint32_t, 6> key = {(uint32_t)a.rule, b.ds->dpad, kind, a.R, a.maxc, reversed ? 1u : 0u};
+    auto it = b.h->prune_variant.find(key);
+    if (it != b.h->prune_variant.end()) {
+        a.variant = it->second;
+        launch_prune(b.s, *b.ds, a);
+        return 1u + (uint32_t)a.variant;
+    }
+    const double t0 = now_sec();
+    hipEvent_t e0, e1;
+    FK_HIP(hipEventCreate(&e0));
+    FK_HIP(hipEventCreate(&e1));
+    float best_ms = 0.f;
+    int best = 0;
+    unsigned long long ref[2] = {0, 0};
+    DevBuf<unsigned long long> dg(1);
+    try {
+        for (int v = 0; v < FK_PRUNE_VARIANTS; ++v) {
+            b.tally.clear(b.s, FK_T_PRUNE_CC, 2);
+            b.tally.clear(b.s, FK_T_PRUNE_SKIP);
+            a.variant = v;
+            FK_HIP(hipEventRecord(e0, b.s));
+            launch_prune(b.s, *b.ds, a);
+            FK_HIP(hipEventRecord(e1, b.s));
+            FK_HIP(hipMemsetAsync(dg.p, 0, 8, b.s));
+            launch_prune_digest(b.s, a, dg.p);
+            unsigned long long got[2] = {0, 0};
+            FK_HIP(hipMemcpyAsync(&got[0], dg.p, 8, hipMemcpyDeviceToHost, b.s));
+            got[1] = b.tally.read1(b.s, FK_T_PRUNE_CC);  // synchronises the stream
+            float ms = 0.f;
+            FK_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (v == 0) {
+                ref[0] = got[0];
+                ref[1] = got[1];
+            } else if (got[0] != ref[0] || got[1] != ref[1]) {
+                throw Error(FK_E_INTERNAL, "prune variants disagree (variant " + std::to_string(v) + ")");
+            }
+            if (v == 0 || ms < best_ms) {
+                best_ms = ms;
+                best = v;
+            }
+        }
+    } catch (...) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        throw;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    b.h->prune_variant[key] = best;
+    if (best != FK_PRUNE_VARIANTS - 1) {
+        // leave the chosen variant's tallies (the outputs are the same)
+        b.tally.clear(b.s, FK_T_PRUNE_CC, 2);
+        b.tally.clear(b.s, FK_T_PRUNE_SKIP);
+        a.variant = best;
+        launch_prune(b.s, *b.ds, a);
+    }
+    b.st->t_tune += now_sec() - t0;
+    if (b.verbose)
+        fprintf(stderr, "[fk] prune variant %d chosen for rule %d, dpad %u, rows %u, R %u%s (%.2f ms)\n", best,
+                a.rule, b.ds->dpad, kind, a.R, reversed ? ", reversed" : "", best_ms);
+    return 1u + (uint32_t)best;
+}
+
 static void link(BuildState &b, fk_pg_type type, uint32_t li) {
     const double t0 = now_sec();
     const uint32_t N = b.N;
@@ -349,7 +426,7 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
         a.xcd_order = true;
     }
     hipEvent_t pe = b.prof_begin();
-    launch_prune(b.s, *b.ds, a);
+    b.st->prune_variant = tuned_prune(b, a, false);
     unsigned long long cc2[2] = {0, 0};
     b.tally.read(b.s, FK_T_PRUNE_CC, 2, cc2);
     const unsigned long long cc = cc2[0];
@@ -360,6 +437,7 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
     b.width = P.nsg_R;
     b.set_width.assign(b.n_sets, P.nsg_R);
     b.st->d_prune += cc;
+    if (rule_of(type) == FK_RULE_TAU) b.st->d_prune_uncomputed += b.tally.read1(b.s, FK_T_PRUNE_SKIP);
     const double t1 = now_sec();
     if (li < FK_MAX_LOOPS) {
         b.st->prune_cc[li] = (float)((double)cc / ((double)N * (double)(N - 1) / 2.0));
@@ -403,6 +481,7 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
     // 3. reversed prune of over-full lists (no search_K cap, no size guard)
     if (hc[0]) {
         b.tally.clear(b.s, FK_T_PRUNE_CC, 2);
+        b.tally.clear(b.s, FK_T_PRUNE_SKIP);
         PruneLaunch r = a;
         r.cand = rp_key.p;
         r.cand_off = rp_off.p;
@@ -413,15 +492,15 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
         r.maxc = 0xFFFFFFFFu;
         r.examined = b.tally.at(FK_T_PRUNE_EXAM);
         hipEvent_t pr = b.prof_begin();
-        launch_prune(b.s, *b.ds, r);
+        b.st->prune_rev_variant = tuned_prune(b, r, true);
         unsigned long long rcc[2] = {0, 0};
         b.tally.read(b.s, FK_T_PRUNE_CC, 2, rcc);
         const unsigned long long rc = rcc[0];
         b.prof_end(pr, FK_PROF_PRUNE_REV, (double)rcc[1] * (4.0 * b.ds->d + 8.0) + 8.0 * (double)hc[0] * P.nsg_R);
         b.prof_flush();
         b.st->d_prune_rev += rc;
+        if (rule_of(type) == FK_RULE_TAU) b.st->d_prune_uncomputed += b.tally.read1(b.s, FK_T_PRUNE_SKIP);
     }
-    if (rule_of(type) == FK_RULE_TAU) b.st->d_prune_uncomputed += b.tally.read1(b.s, FK_T_PRUNE_SKIP);
     const double t2 = now_sec();
     if (li < FK_MAX_LOOPS) b.st->inter_time[li] = t2 - t1;
     b.st->t_inter += t2 - t1;

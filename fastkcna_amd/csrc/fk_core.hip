@@ -147,7 +147,15 @@ __host__ __device__ inline uint32_t prune_wave_bytes(uint32_t mc, uint32_t dpad,
     return ((mc + 1u) * dpad * elem + R * 12u + 15u) & ~15u;
 }
 
-template <int RULE, int NB, int MODE>
+// FV (variant): 0 -- each octet checks its own candidate against the
+// first cached kept neighbours, and candidates that survive those go on
+// one at a time with the wave's eight octets splitting the remaining kept
+// neighbours (best when most candidates fall to the first few kept
+// neighbours); 1 -- each octet checks its own candidate against every kept
+// neighbour, the cached ones from LDS and the rest from global memory (best
+// when occlusion is rare and most candidates are checked against the whole
+// kept list).  Both make the same decisions and count the same distances.
+template <int RULE, int NB, int MODE, int FV>
 struct PruneWave {
     using E = typename RowKind<NB, MODE>::E;
     using Chain = typename RowKind<NB, MODE>::Chain;
@@ -210,8 +218,9 @@ struct PruneWave {
         // neighbours; the survivors go on one at a time, the wave's octets
         // taking eight kept neighbours per step
         constexpr uint32_t FAST = MODE == 2 ? FK_PRUNE_FAST8 : FK_PRUNE_FAST;
-        constexpr uint32_t MF = MC < FAST ? MC : FAST;
+        constexpr uint32_t MF = FV ? 0xFFFFFFFFu : (MC < FAST ? MC : FAST);
         const uint32_t nA = n0 < MF ? n0 : MF;
+        const uint32_t nC = nA < MC ? nA : MC;  // the part of the fast checks with cached rows
         if (!skip && nodist) {
             uint32_t t = 0;
             while (t < n0 && so[t] != pid) ++t;
@@ -232,16 +241,27 @@ struct PruneWave {
                     return occludes<RULE>(djk, pd, sd[t], tau3, cos_thr);
                 };
                 Chain ca, cb;
-                if (nA > 0) ca.load(lane_cache, nb);
-                for (uint32_t t = 0; t < nA; t += 2) {
-                    if (t + 1 < nA) cb.load(lane_cache + (size_t)(t + 1) * dpad, nb);
+                if (nC > 0) ca.load(lane_cache, nb);
+                for (uint32_t t = 0; t < nC; t += 2) {
+                    if (t + 1 < nC) cb.load(lane_cache + (size_t)(t + 1) * dpad, nb);
                     if (check(t, ca)) { occl = true; break; }
-                    if (t + 1 >= nA) break;
-                    if (t + 2 < nA) ca.load(lane_cache + (size_t)(t + 2) * dpad, nb);
+                    if (t + 1 >= nC) break;
+                    if (t + 2 < nC) ca.load(lane_cache + (size_t)(t + 2) * dpad, nb);
                     if (check(t + 1, cb)) { occl = true; break; }
                 }
             } else {
-                for (uint32_t t = 0; t < nA; ++t) {
+                for (uint32_t t = 0; t < nC; ++t) {
+                    const uint32_t tid = so[t];
+                    if (tid == pid) { occl = true; break; }
+                    ++mycc;
+                    const float djk = member_dist(crow, cn, t, tid);
+                    if (occludes<RULE>(djk, pd, sd[t], tau3, cos_thr)) { occl = true; break; }
+                }
+            }
+            if (FV && !occl) {
+                // variant 1: the kept neighbours past the cached ones, rows
+                // from global memory
+                for (uint32_t t = nC; t < nA; ++t) {
                     const uint32_t tid = so[t];
                     if (tid == pid) { occl = true; break; }
                     ++mycc;
@@ -362,10 +382,10 @@ struct PruneWave {
 #define FK_PRUNE_ATTR
 #endif
 
-template <int RULE, int NB, int MODE>
+template <int RULE, int NB, int MODE, int FV>
 __global__ __launch_bounds__(256) FK_PRUNE_ATTR void k_prune(const void *__restrict__ chains_v, uint32_t dpad,
                                                              PruneLaunch a, uint32_t wave_bytes) {
-    using PW = PruneWave<RULE, NB, MODE>;
+    using PW = PruneWave<RULE, NB, MODE, FV>;
     using E = typename PW::E;
     using Chain = typename PW::Chain;
     constexpr bool DOT = PW::DOT;
@@ -474,12 +494,12 @@ __global__ __launch_bounds__(256) FK_PRUNE_ATTR void k_prune(const void *__restr
     }
 }
 
-template <int RULE, int NB>
-static void launch_prune_nb(hipStream_t s, const Dataset &ds, const PruneLaunch &a, unsigned grid) {
+template <int RULE, int NB, int FV>
+static void launch_prune_nbv(hipStream_t s, const Dataset &ds, const PruneLaunch &a, unsigned grid) {
     if constexpr (NB == 16 || NB == 32) {
         if (a.nrm && a.chains8) {
             const uint32_t wb = prune_wave_bytes(PruneMC<RULE, NB, 2>::value, ds.dpad, a.R, 1);
-            hipLaunchKernelGGL((k_prune<RULE, NB, 2>), dim3(grid), dim3(64 * FK_PRUNE_WPB),
+            hipLaunchKernelGGL((k_prune<RULE, NB, 2, FV>), dim3(grid), dim3(64 * FK_PRUNE_WPB),
                                (size_t)wb * FK_PRUNE_WPB, s, (const void *)a.chains8, ds.dpad, a, wb);
             return;
         }
@@ -487,14 +507,20 @@ static void launch_prune_nb(hipStream_t s, const Dataset &ds, const PruneLaunch 
     const uint32_t wave_bytes = prune_wave_bytes(PruneMC<RULE, NB, 0>::value, ds.dpad, a.R, 4);  // modes 0 and 1
     if constexpr (NB == 12 || NB == 16 || NB == 32) {
         if (a.nrm) {
-            hipLaunchKernelGGL((k_prune<RULE, NB, 1>), dim3(grid), dim3(64 * FK_PRUNE_WPB),
+            hipLaunchKernelGGL((k_prune<RULE, NB, 1, FV>), dim3(grid), dim3(64 * FK_PRUNE_WPB),
                                (size_t)wave_bytes * FK_PRUNE_WPB, s, (const void *)ds.chains.p, ds.dpad, a,
                                wave_bytes);
             return;
         }
     }
-    hipLaunchKernelGGL((k_prune<RULE, NB, 0>), dim3(grid), dim3(64 * FK_PRUNE_WPB),
+    hipLaunchKernelGGL((k_prune<RULE, NB, 0, FV>), dim3(grid), dim3(64 * FK_PRUNE_WPB),
                        (size_t)wave_bytes * FK_PRUNE_WPB, s, (const void *)ds.chains.p, ds.dpad, a, wave_bytes);
+}
+
+template <int RULE, int NB>
+static void launch_prune_nb(hipStream_t s, const Dataset &ds, const PruneLaunch &a, unsigned grid) {
+    if (a.variant == 1) launch_prune_nbv<RULE, NB, 1>(s, ds, a, grid);
+    else launch_prune_nbv<RULE, NB, 0>(s, ds, a, grid);
 }
 
 template <int RULE>
@@ -532,9 +558,46 @@ void build_chains(hipStream_t s, Dataset &ds) {
     FK_LAUNCH_CHECK();
 }
 
+// Order-independent digest of a prune's output lists (ids, distance bits
+// and positions of every task's list, and its length), summed over tasks:
+// two launches that wrote the same lists give the same digest.
+__device__ __forceinline__ uint64_t fmix64(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+__global__ __launch_bounds__(256) void k_prune_digest(PruneLaunch a, unsigned long long *__restrict__ out) {
+    const uint32_t task = blockIdx.x * 4 + (threadIdx.x >> 6);
+    uint64_t h = 0;
+    if (task < a.n_tasks) {
+        const uint32_t slot = a.task_list ? a.task_list[task] : task;
+        const uint32_t len = a.out_len[slot];
+        const uint32_t *ids = a.out_ids + (uint64_t)slot * a.out_stride;
+        const float *d = a.out_d ? a.out_d + (uint64_t)slot * a.out_stride : nullptr;
+        for (uint32_t k = lane_id(); k < len; k += 64) {
+            const uint64_t v = ((uint64_t)ids[k] << 32) | (d ? __float_as_uint(d[k]) : 0u);
+            h += fmix64(v ^ ((uint64_t)slot * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)k << 20));
+        }
+        if (lane_id() == 0) h += fmix64(((uint64_t)slot << 32) | len);
+    }
+    h = wave_sum(h);
+    if (lane_id() == 0 && h) atomicAdd(out, (unsigned long long)h);
+}
+
+void launch_prune_digest(hipStream_t s, const PruneLaunch &a, unsigned long long *out) {
+    if (a.n_tasks == 0) return;
+    hipLaunchKernelGGL(k_prune_digest, dim3((a.n_tasks + 3) / 4), dim3(256), 0, s, a, out);
+    FK_LAUNCH_CHECK();
+}
+
 void launch_prune(hipStream_t s, const Dataset &ds, const PruneLaunch &a) {
     if (a.n_tasks == 0) return;
     FK_CHECK_ARG(a.R <= FK_PRUNE_RMAX, "nsg_R above the supported maximum (256)");
+    FK_CHECK_ARG(a.variant == 0 || a.variant == 1, "unknown prune variant");
     unsigned grid = (a.n_tasks + FK_PRUNE_WPB - 1) / FK_PRUNE_WPB;
     if (a.xcd_order) grid = (grid + FK_NUM_XCD - 1) / FK_NUM_XCD * FK_NUM_XCD;
     switch (a.rule) {

@@ -2,6 +2,8 @@
 // plumbing and the kernel launch wrappers shared by the C-ABI translation
 // units.
 #pragma once
+#include <array>
+#include <map>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -208,6 +210,9 @@ struct fk_handle {
     fk::PinnedBuf<uint32_t> host_lab, host_deg;  // connectivity pass: labels and out-degrees
     fk::PinnedBuf<uint8_t> dl_stage[2];        // device-to-host staging (download_bytes)
     uint8_t *noalloc = nullptr;                // device address of the fk_prune_noalloc scratch array
+    // prune kernel variant chosen per shape by measurement (fk_build.hip:
+    // tuned_prune): key = rule, dpad, row kind, nsg_R, maxc, reversed
+    std::map<std::array<uint32_t, 6>, int> prune_variant;
 };
 
 namespace fk {
@@ -238,10 +243,14 @@ struct PruneLaunch {
     unsigned long long *cc = nullptr;  // accumulated distance count (tally counter)
     unsigned long long *examined = nullptr;  // accumulated examined candidates (tally counter)
     unsigned long long *uncomputed = nullptr;  // counted distances not computed (tau rule, pd <= 3 tau)
+    int variant = 0;                  // kernel variant (FK_PRUNE_VARIANTS; same results)
     const float2 *nrm = nullptr;      // exact-form rows: norms for dot-product distances
     const uint8_t *chains8 = nullptr; // exact-form byte rows (chain-major), with nrm
 };
 void launch_prune(hipStream_t s, const Dataset &ds, const PruneLaunch &a);
+#define FK_PRUNE_VARIANTS 2
+// Adds an order-independent digest of the launch's output lists to *out.
+void launch_prune_digest(hipStream_t s, const PruneLaunch &a, unsigned long long *out);
 void build_chains(hipStream_t s, Dataset &ds);  // fills ds.chains from ds.rows
 
 void launch_join_distances(hipStream_t s, const Dataset &ds, const uint32_t *ids,

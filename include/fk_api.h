@@ -84,6 +84,18 @@ typedef enum {
 #define FK_FLAG_SCREEN 0x40u
 #define FK_FLAG_SCREEN_NOTRAIN 0x80u
 
+/* Prune kernel variant.  Builds pick one per shape (rule, width, row kind,
+ * nsg_R, search_K, sync or reversed) on its first use in a handle: that
+ * prune runs once with each variant, timed, the out-lists and distance counts
+ * are checked to be identical, and the faster variant is used from then on
+ * (fk_stats.prune_variant).  Variant 0 checks each candidate against the
+ * first cached kept neighbours and splits the survivors' remaining checks
+ * across the wavefront; variant 1 checks each candidate against every kept
+ * neighbour itself.  These flags force one (no timing); fk_prune and
+ * fk_prune_noalloc use variant 0 unless FK_FLAG_PRUNE_V1 is set. */
+#define FK_FLAG_PRUNE_V0 0x100u
+#define FK_FLAG_PRUNE_V1 0x200u
+
 /* Gram operand formats (fk_stats.gram_mode, fk_gram_pair_distances). */
 enum {
     FK_GRAM_NONE = 0,
@@ -177,6 +189,10 @@ typedef struct {
      * removes it and the distances the reference evaluates for it decide
      * nothing. */
     uint64_t d_prune_uncomputed;
+    /* Prune variants used by the last main and reversed prune (1 + variant;
+     * 0: none ran), and the seconds spent timing variants in this build. */
+    uint32_t prune_variant, prune_rev_variant;
+    double t_tune;
 } fk_stats;
 
 /* Output graph in CSR form, allocated by the library (fk_graph_free). */

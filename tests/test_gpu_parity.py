@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import oracle
+from fastkcna_amd import _ffi
 from fastkcna_amd import FK_PG_ALPHAPG, FK_PG_HNSW_LAYER, FK_PG_NSG, FK_PG_TAUMNG, IndexParams
 from fastkcna_amd.datasets import float_mixture, integer_mixture, unit_mixture
 
@@ -93,16 +94,20 @@ def make_pools(x, n_tasks, L, rng, q=None):
     return pools, plen, q
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("pg", [FK_PG_NSG, FK_PG_TAUMNG, FK_PG_ALPHAPG])
 @pytest.mark.parametrize("reversed_", [False, True])
 @pytest.mark.parametrize("name", ["int128", "unit96", "float960", "odd100", "odd20", "odd44"])
-def test_prune_bitexact(device, pg, reversed_, name):
+def test_prune_bitexact(device, pg, reversed_, name, variant):
+    """Both prune kernel variants (the build picks one per shape by timing)
+    against the oracle."""
     x = datasets()[name]
-    say("prune", pg, reversed_, name)
+    say("prune", pg, reversed_, name, "variant", variant)
     device.upload(x)
     rng = np.random.default_rng(3)
     tau = {"int128": 300.0, "unit96": 0.05, "float960": 0.5, "odd100": 1.0, "odd20": 0.2, "odd44": 100.0}[name]
     p = IndexParams(nsg_R=24, search_K=90, tau=tau, angle=63)
+    p.flags = _ffi.FK_FLAG_PRUNE_V1 if variant else 0
     pools, plen, q = make_pools(x, 400, 120, rng)
     # a self entry at the front of some pools
     for t in range(0, 400, 7):
@@ -220,6 +225,29 @@ def test_full_build_invariants_and_recall(device, pg):
     assert rg >= rc - 0.02
 
 
+@pytest.mark.parametrize("pg", [FK_PG_NSG, FK_PG_TAUMNG, FK_PG_ALPHAPG])
+@pytest.mark.parametrize("name", ["int128", "unit96", "odd44"])
+def test_prune_variants_give_the_same_build(device, pg, name):
+    """Builds with either prune variant forced and with the variant picked
+    by timing are the same graph with the same statistics."""
+    x = datasets()[name]
+    say("variants", pg, name)
+    device.upload(x)
+    p = IndexParams(K=40, L=40, S=10, R=40, iterations=4, nsg_R=16, search_L=30, search_K=40, loop_i=1,
+                    tau={"int128": 20.0, "unit96": 0.01, "odd44": 5.0}[name], angle=60)
+    out = []
+    for flags in (_ffi.FK_FLAG_PRUNE_V0, _ffi.FK_FLAG_PRUNE_V1, 0):
+        off, ids, ep, width, st = device.build(pg, p.copy(flags=flags))
+        out.append((off, ids, ep, st))
+    for off, ids, ep, st in out[1:]:
+        assert np.array_equal(off, out[0][0]) and np.array_equal(ids, out[0][1]) and ep == out[0][2]
+        for k in ("d_prune", "d_prune_rev", "d_bv", "d_inter", "aod"):
+            assert st[k] == out[0][3][k], k
+    assert out[0][3]["prune_variant"] == 1 and out[1][3]["prune_variant"] == 2
+    assert out[2][3]["prune_variant"] in (1, 2)
+    say("variants", pg, name, "picked", out[2][3]["prune_variant"] - 1, "tuning s", out[2][3]["t_tune"])
+
+
 @pytest.mark.parametrize("name", ["odd44", "odd20", "float960"])
 def test_full_build_odd_widths(device, name):
     """Rows whose padded width is not a multiple of 32 (octet distances
@@ -260,8 +288,9 @@ def _c2_dumped_pool(device, p, n_dump=10_000):
     return _C2_CACHE["pool"]
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("pg", [FK_PG_NSG, FK_PG_ALPHAPG])
-def test_prune_bitexact_c2_dumped_pool(device, pg):
+def test_prune_bitexact_c2_dumped_pool(device, pg, variant):
     """The 1M x 128 NSG workload (1,024-centre integer mixture, seed 1,
     K=L=500 S=12 R=100 I=6, nsg_R=50, search_K=500, angle 60): the pools of
     nodes 0..9999 right after NN-descent are pruned on the GPU and by the
@@ -280,6 +309,7 @@ def test_prune_bitexact_c2_dumped_pool(device, pg):
     exact = oracle.pair_distances(x, rows, ids[::10].ravel())
     assert np.array_equal(exact.view(np.uint32), d[::10].ravel().view(np.uint32))
     q = np.arange(n_dump, dtype=np.uint32)
+    p = p.copy(flags=_ffi.FK_FLAG_PRUNE_V1 if variant else 0)
     g = device.prune(pg, pools, plen, q, len(x), p)
     c = oracle.prune(pg, x, pools, plen, q, len(x), p)
     assert np.array_equal(g[2], c[2])

@@ -308,6 +308,7 @@ fk_status fk_dataset_upload(fk_handle *h, const float *rows, size_t row_stride_b
         ds.gram_fmt = FK_GRAM_NONE;
         ds.gram_req = 0;
         ds.n_total = 0;
+        ds.range_center.clear();
         const uint32_t dpad = (d + 7u) & ~7u;
         ds.rows.alloc((size_t)n_total * dpad);
         // the centroid of all rows (Calcenter: float sums in row order, then

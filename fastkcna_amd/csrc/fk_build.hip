@@ -145,7 +145,6 @@ __device__ void inter_merge_node(uint32_t des, uint32_t ncap, uint32_t R, uint32
     if (threadIdx.x == 0) {
         rp_off[des] = off;
         rp_len[des] = tot;
-        task_list[atomicAdd(n_tasks, 1u)] = des;
     }
 }
 
@@ -253,9 +252,26 @@ __global__ __launch_bounds__(64 * FK_INTER_WPB) void k_inter_small(
         } else {
             rp_off[des] = off;
             rp_len[des] = tot;
-            task_list[atomicAdd(n_tasks, 1u)] = des;
         }
     }
+}
+
+// Reversed-prune tasks: the nodes whose union was staged (rp_len > 0),
+// listed in the locality order (node order when there is none) by a flag
+// scan, so the prune's XCD-contiguous runs cover neighbouring nodes -- an
+// atomic append per node held every wave behind one counter when most
+// out-lists overflow.
+__global__ void k_task_flags(const uint32_t *__restrict__ rp_len, const uint32_t *__restrict__ order, uint32_t N,
+                             uint32_t *__restrict__ flag) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    flag[i] = rp_len[order ? order[i] : i] > 0 ? 1u : 0u;
+}
+__global__ void k_task_scatter(const uint32_t *__restrict__ flag, const uint64_t *__restrict__ pos,
+                               const uint32_t *__restrict__ order, uint32_t N, uint32_t *__restrict__ tasks) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || !flag[i]) return;
+    tasks[pos[i]] = order ? order[i] : i;
 }
 
 // UpdateNhoods: isnew = (first loop) or (id not in the previous out-list);
@@ -469,9 +485,19 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
                        b.nsg_id.p, b.nsg_d.p, b.nsg_len.p, b.bcnt.p, b.boff.p, b.bkey.p, rp_key.p, rp_off.p, rp_len.p,
                        tasks.p, counters.p, counters.p + 1);
     FK_LAUNCH_CHECK();
+    DevBuf<uint64_t> tpos((size_t)N + 1);
+    const uint32_t *ord = b.has_order ? b.order.p : nullptr;
+    hipLaunchKernelGGL(k_task_flags, dim3((N + 255) / 256), dim3(256), 0, b.s, rp_len.p, ord, N, big.p);
+    FK_LAUNCH_CHECK();
+    exclusive_scan_u32_to_u64(b.s, big.p, tpos.p, N, tpos.p + N, b.scan_tmp);
+    hipLaunchKernelGGL(k_task_scatter, dim3((N + 255) / 256), dim3(256), 0, b.s, big.p, tpos.p, ord, N, tasks.p);
+    FK_LAUNCH_CHECK();
     uint32_t hc[2] = {0, 0};
+    uint64_t n_tasks = 0;
     FK_HIP(hipMemcpyAsync(hc, counters.p, 8, hipMemcpyDeviceToHost, b.s));
+    FK_HIP(hipMemcpyAsync(&n_tasks, tpos.p + N, 8, hipMemcpyDeviceToHost, b.s));
     b.sync();
+    hc[0] = (uint32_t)n_tasks;
     // out-lists read twice (count, fill) and rewritten, in-edges written and
     // read once as 8-byte keys
     b.prof_end(pe, FK_PROF_INTER, 3.0 * 8.0 * (double)n_in + 16.0 * (double)n_in);

@@ -4,6 +4,7 @@
 #pragma once
 #include <array>
 #include <map>
+#include <utility>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -195,6 +196,9 @@ struct Dataset {
     int gram_fmt = 0;
     float gram_scale = 1.f;
     uint32_t gram_req = 0;
+    // centroids of row ranges (batched builds' sets) already made, d floats
+    // each, keyed by (first row, row count); cleared by every upload
+    mutable std::map<std::pair<uint32_t, uint32_t>, std::vector<float>> range_center;
 };
 
 }  // namespace fk

@@ -600,7 +600,11 @@ __global__ __launch_bounds__(256) void k_nnd_merge(uint32_t cap, uint32_t Lp, ui
 // recall = mean over controls of |pool[0..min(L,K)) ∩ gt[0..K)| / K.
 // ---------------------------------------------------------------------------
 // One wavefront per node, grid-stride; one atomic per block.
-// With sets, acc[set] per set (one atomic per node).
+// With sets, acc[set] per set: each wave takes a contiguous run of nodes
+// (a set is a contiguous id range, so a run spans one or two sets) and adds
+// its running sums to the set's counters when the set changes and at the
+// end -- a few atomics per wave instead of one per node, which held every
+// wave of a many-set batch behind the same few addresses.
 // This lane's share of the count of FK_FLG_NEW among the first lim flags of
 // a pool row (the wave's lanes together count each flag once).  Rows of
 // cap flags with cap % 4 == 0 start on 4-byte boundaries and are read four
@@ -628,19 +632,31 @@ __global__ __launch_bounds__(1024) void k_delta(const uint8_t *__restrict__ pflg
                                                 const uint32_t *__restrict__ set_of,
                                                 unsigned long long *__restrict__ acc) {
     const uint32_t wpb = blockDim.x >> 6;
+    if (set_of) {  // uniform
+        const uint32_t nw = gridDim.x * wpb, chunk = (N + nw - 1) / nw;
+        const uint32_t lo = (blockIdx.x * wpb + (threadIdx.x >> 6)) * chunk;
+        const uint32_t hi = lo + chunk < N ? lo + chunk : N;
+        uint32_t cur = 0xFFFFFFFFu;
+        unsigned long long run = 0;
+        for (uint32_t n = lo; n < hi; ++n) {
+            const uint32_t lim = cnt[n] < K ? cnt[n] : K;
+            const unsigned long long cn = wave_sum((unsigned long long)lane_new_flags(pflg + (uint64_t)n * cap, lim, cap));
+            const uint32_t k = set_of[n];
+            if (k != cur) {
+                if (lane_id() == 0 && run) atomicAdd(&acc[cur], run);
+                cur = k;
+                run = 0;
+            }
+            run += cn;
+        }
+        if (lane_id() == 0 && run) atomicAdd(&acc[cur], run);
+        return;
+    }
     unsigned long long c = 0;
     for (uint32_t n = blockIdx.x * wpb + (threadIdx.x >> 6); n < N; n += gridDim.x * wpb) {
         const uint32_t lim = cnt[n] < K ? cnt[n] : K;
-        const uint8_t *pf = pflg + (uint64_t)n * cap;
-        unsigned long long cn = lane_new_flags(pf, lim, cap);
-        if (set_of) {
-            cn = wave_sum(cn);
-            if (lane_id() == 0 && cn) atomicAdd(&acc[set_of[n]], cn);
-        } else {
-            c += cn;
-        }
+        c += lane_new_flags(pflg + (uint64_t)n * cap, lim, cap);
     }
-    if (set_of) return;  // uniform
     c = block_sum(c);
     if (threadIdx.x == 0 && c) atomicAdd(acc, c);
 }
@@ -1326,6 +1342,33 @@ __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ p
                                                    const uint32_t *__restrict__ set_of,
                                                    unsigned long long *__restrict__ acc) {
     const uint32_t wpb = blockDim.x >> 6;
+    if (set_of) {  // uniform: contiguous runs per wave, as in k_delta
+        const uint32_t nw = gridDim.x * wpb, chunk = (N + nw - 1) / nw;
+        const uint32_t lo = (blockIdx.x * wpb + (threadIdx.x >> 6)) * chunk;
+        const uint32_t hi = lo + chunk < N ? lo + chunk : N;
+        uint32_t cur = 0xFFFFFFFFu;
+        unsigned long long rc = 0, ra = 0;
+        for (uint32_t n = lo; n < hi; ++n) {
+            const uint32_t lim = cnt[n] < W ? cnt[n] : W;
+            const uint32_t f = wave_sum(lane_new_flags(pflg + (uint64_t)n * cap, lim, cap));
+            const uint32_t k = set_of[n];
+            if (k != cur) {
+                if (lane_id() == 0 && ra) {
+                    atomicAdd(&acc[2 * cur], rc);
+                    atomicAdd(&acc[2 * cur + 1], ra);
+                }
+                cur = k;
+                rc = ra = 0;
+            }
+            rc += f < S ? f : S;
+            ra += f;
+        }
+        if (lane_id() == 0 && ra) {
+            atomicAdd(&acc[2 * cur], rc);
+            atomicAdd(&acc[2 * cur + 1], ra);
+        }
+        return;
+    }
     unsigned long long capped = 0, all = 0;
     for (uint32_t n = blockIdx.x * wpb + (threadIdx.x >> 6); n < N; n += gridDim.x * wpb) {
         const uint8_t *pf = pflg + (uint64_t)n * cap;
@@ -1333,19 +1376,10 @@ __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ p
         uint32_t f = lane_new_flags(pf, lim, cap);
         f = wave_sum(f);
         if (lane_id() == 0) {
-            if (set_of) {
-                if (f) {
-                    const uint32_t k = set_of[n];
-                    atomicAdd(&acc[2 * k], (unsigned long long)(f < S ? f : S));
-                    atomicAdd(&acc[2 * k + 1], (unsigned long long)f);
-                }
-            } else {
-                capped += f < S ? f : S;
-                all += f;
-            }
+            capped += f < S ? f : S;
+            all += f;
         }
     }
-    if (set_of) return;  // uniform
     capped = block_sum(capped);
     all = block_sum(all);
     if (threadIdx.x == 0) {

@@ -245,16 +245,43 @@ void column_mean(hipStream_t s, const Dataset &ds, float *center, uint32_t r0, u
     column_means(s, ds, center, &r0, &n, 1);
 }
 
+// The ranges' centroids are made once per uploaded dataset (a batched
+// build asks for them in every graph loop, and repeated builds of the same
+// sets ask again): ranges not yet in ds.range_center are computed in one
+// launch and kept on the host, then every requested one is copied to
+// center[k * dpad].
 void column_means(hipStream_t s, const Dataset &ds, float *center, const uint32_t *r0, const uint32_t *n,
                   uint32_t count) {
     if (count == 0) return;
-    DevBuf<uint32_t> rng(2 * (size_t)count);
-    FK_HIP(hipMemcpyAsync(rng.p, r0, (size_t)count * 4, hipMemcpyHostToDevice, s));
-    FK_HIP(hipMemcpyAsync(rng.p + count, n, (size_t)count * 4, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_colmean_ordered, dim3((ds.d + 63) / 64, count), dim3(64), 0, s, ds.rows.p, ds.stride, ds.d,
-                       ds.dpad, rng.p, rng.p + count, center);
-    FK_LAUNCH_CHECK();
-    FK_HIP(hipStreamSynchronize(s));  // rng is released on return
+    std::vector<uint32_t> miss_lo, miss_n;
+    for (uint32_t k = 0; k < count; ++k)
+        if (!ds.range_center.count({r0[k], n[k]})) {
+            miss_lo.push_back(r0[k]);
+            miss_n.push_back(n[k]);
+        }
+    const uint32_t m = (uint32_t)miss_lo.size();
+    if (m) {
+        DevBuf<uint32_t> rng(2 * (size_t)m);
+        DevBuf<float> out((size_t)m * ds.dpad);
+        FK_HIP(hipMemcpyAsync(rng.p, miss_lo.data(), (size_t)m * 4, hipMemcpyHostToDevice, s));
+        FK_HIP(hipMemcpyAsync(rng.p + m, miss_n.data(), (size_t)m * 4, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_colmean_ordered, dim3((ds.d + 63) / 64, m), dim3(64), 0, s, ds.rows.p, ds.stride, ds.d,
+                           ds.dpad, rng.p, rng.p + m, out.p);
+        FK_LAUNCH_CHECK();
+        std::vector<float> host((size_t)m * ds.dpad);
+        FK_HIP(hipMemcpyAsync(host.data(), out.p, host.size() * 4, hipMemcpyDeviceToHost, s));
+        FK_HIP(hipStreamSynchronize(s));  // rng and out are released on return
+        for (uint32_t k = 0; k < m; ++k)
+            ds.range_center[{miss_lo[k], miss_n[k]}] =
+                std::vector<float>(host.begin() + (size_t)k * ds.dpad, host.begin() + (size_t)k * ds.dpad + ds.d);
+    }
+    std::vector<float> all((size_t)count * ds.dpad, 0.f);
+    for (uint32_t k = 0; k < count; ++k) {
+        const std::vector<float> &c = ds.range_center.at({r0[k], n[k]});
+        std::copy(c.begin(), c.end(), all.begin() + (size_t)k * ds.dpad);
+    }
+    FK_HIP(hipMemcpyAsync(center, all.data(), all.size() * 4, hipMemcpyHostToDevice, s));
+    FK_HIP(hipStreamSynchronize(s));  // all is released on return
 }
 
 // ComputeEp (src/kgraph.cpp:1528-1536), per set: the centroid of the set's

@@ -334,10 +334,13 @@ __global__ void k_pack_csr(const uint32_t *__restrict__ nsg_id, const uint32_t *
 
 // The prune of one link step with the kernel variant for its shape: forced
 // by a flag, or taken from the handle, or -- on the shape's first use --
-// measured: the launch runs once with each variant (HIP events around each),
-// the variants' out-lists (order-independent digest) and distance counts
-// must agree, and the faster variant is kept on the handle.  The last run's
-// outputs and tallies stay, as after a single launch.  Returns 1 + variant.
+// measured.  Measurement runs the variants on a sample (the first sixteenth
+// of the tasks, a contiguous run of the locality order; variant 0 twice so
+// neither is timed on cold caches), requires the sample's out-lists
+// (order-independent digest) and distance counts to agree, keeps the faster
+// variant on the handle and runs it on the remaining tasks: the launch's
+// outputs and tallies are those of a single launch, at about a fifth more
+// work.  Small launches are measured whole, twice each.  Returns 1 + variant.
 static uint32_t tuned_prune(BuildState &b, PruneLaunch &a, bool reversed) {
     const uint32_t flags = b.P.flags;
     if (flags & (FK_FLAG_PRUNE_V0 | FK_FLAG_PRUNE_V1)) {
@@ -354,38 +357,40 @@ static uint32_t tuned_prune(BuildState &b, PruneLaunch &a, bool reversed) {
         return 1u + (uint32_t)a.variant;
     }
     const double t0 = now_sec();
+    const uint32_t n = a.n_tasks;
+    const uint32_t m = n >= 65536 ? n / 16 : n;  // sample
+    PruneLaunch smp = a;
+    smp.n_tasks = m;
     hipEvent_t e0, e1;
     FK_HIP(hipEventCreate(&e0));
     FK_HIP(hipEventCreate(&e1));
-    float best_ms = 0.f;
-    int best = 0;
+    float best_ms[FK_PRUNE_VARIANTS];
     unsigned long long ref[2] = {0, 0};
     DevBuf<unsigned long long> dg(1);
     try {
-        for (int v = 0; v < FK_PRUNE_VARIANTS; ++v) {
+        // variants 0, 1, ..., then 0 again
+        for (int r = 0; r <= FK_PRUNE_VARIANTS; ++r) {
+            const int v = r % FK_PRUNE_VARIANTS;
             b.tally.clear(b.s, FK_T_PRUNE_CC, 2);
             b.tally.clear(b.s, FK_T_PRUNE_SKIP);
-            a.variant = v;
+            smp.variant = v;
             FK_HIP(hipEventRecord(e0, b.s));
-            launch_prune(b.s, *b.ds, a);
+            launch_prune(b.s, *b.ds, smp);
             FK_HIP(hipEventRecord(e1, b.s));
             FK_HIP(hipMemsetAsync(dg.p, 0, 8, b.s));
-            launch_prune_digest(b.s, a, dg.p);
+            launch_prune_digest(b.s, smp, dg.p);
             unsigned long long got[2] = {0, 0};
             FK_HIP(hipMemcpyAsync(&got[0], dg.p, 8, hipMemcpyDeviceToHost, b.s));
             got[1] = b.tally.read1(b.s, FK_T_PRUNE_CC);  // synchronises the stream
             float ms = 0.f;
             FK_HIP(hipEventElapsedTime(&ms, e0, e1));
-            if (v == 0) {
+            if (r == 0) {
                 ref[0] = got[0];
                 ref[1] = got[1];
             } else if (got[0] != ref[0] || got[1] != ref[1]) {
                 throw Error(FK_E_INTERNAL, "prune variants disagree (variant " + std::to_string(v) + ")");
             }
-            if (v == 0 || ms < best_ms) {
-                best_ms = ms;
-                best = v;
-            }
+            if (r < FK_PRUNE_VARIANTS || ms < best_ms[v]) best_ms[v] = ms;
         }
     } catch (...) {
         hipEventDestroy(e0);
@@ -394,18 +399,21 @@ static uint32_t tuned_prune(BuildState &b, PruneLaunch &a, bool reversed) {
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    int best = 0;
+    for (int v = 1; v < FK_PRUNE_VARIANTS; ++v)
+        if (best_ms[v] < best_ms[best]) best = v;
     b.h->prune_variant[key] = best;
-    if (best != FK_PRUNE_VARIANTS - 1) {
-        // leave the chosen variant's tallies (the outputs are the same)
-        b.tally.clear(b.s, FK_T_PRUNE_CC, 2);
-        b.tally.clear(b.s, FK_T_PRUNE_SKIP);
-        a.variant = best;
-        launch_prune(b.s, *b.ds, a);
-    }
+    a.variant = best;
     b.st->t_tune += now_sec() - t0;
+    if (m < n) {  // the rest of the tasks; the tallies keep the sample's counts (the same for every variant)
+        PruneLaunch rest = a;
+        rest.task_base = a.task_base + m;
+        rest.n_tasks = n - m;
+        launch_prune(b.s, *b.ds, rest);
+    }
     if (b.verbose)
-        fprintf(stderr, "[fk] prune variant %d chosen for rule %d, dpad %u, rows %u, R %u%s (%.2f ms)\n", best,
-                a.rule, b.ds->dpad, kind, a.R, reversed ? ", reversed" : "", best_ms);
+        fprintf(stderr, "[fk] prune variant %d chosen for rule %d, dpad %u, rows %u, R %u%s (sample of %u: %.3f vs %.3f ms)\n",
+                best, a.rule, b.ds->dpad, kind, a.R, reversed ? ", reversed" : "", m, best_ms[0], best_ms[1]);
     return 1u + (uint32_t)best;
 }
 

@@ -397,7 +397,7 @@ __global__ __launch_bounds__(256) FK_PRUNE_ATTR void k_prune(const void *__restr
     const uint32_t vb = a.xcd_order ? xcd_position((gridDim.x + FK_NUM_XCD - 1) / FK_NUM_XCD) : blockIdx.x;
     const uint32_t task = vb * FK_PRUNE_WPB + w;
     if (task >= a.n_tasks) return;
-    const uint32_t slot = a.task_list ? a.task_list[task] : task;
+    const uint32_t slot = a.task_list ? a.task_list[a.task_base + task] : a.task_base + task;
     const uint64_t *c = a.cand + (a.cand_off ? a.cand_off[slot] : (uint64_t)slot * a.cand_stride);
     const uint32_t len = a.cand_len[slot];
     const uint32_t q = a.qid ? a.qid[slot] : slot;
@@ -574,7 +574,7 @@ __global__ __launch_bounds__(256) void k_prune_digest(PruneLaunch a, unsigned lo
     const uint32_t task = blockIdx.x * 4 + (threadIdx.x >> 6);
     uint64_t h = 0;
     if (task < a.n_tasks) {
-        const uint32_t slot = a.task_list ? a.task_list[task] : task;
+        const uint32_t slot = a.task_list ? a.task_list[a.task_base + task] : a.task_base + task;
         const uint32_t len = a.out_len[slot];
         const uint32_t *ids = a.out_ids + (uint64_t)slot * a.out_stride;
         const float *d = a.out_d ? a.out_d + (uint64_t)slot * a.out_stride : nullptr;

@@ -233,6 +233,7 @@ struct PruneLaunch {
     const uint64_t *cand_off = nullptr;  // optional CSR offsets per slot
     const uint32_t *qid = nullptr;    // owner node per task (nullptr: task index)
     const uint32_t *task_list = nullptr;  // optional indirection: task -> pool slot
+    uint32_t task_base = 0;           // the launch covers tasks [task_base, task_base + n_tasks)
     bool xcd_order = false;           // map blocks to tasks XCD-contiguously
     uint32_t n_tasks = 0;
     uint32_t oracle_n = 0;            // ids >= oracle_n are skipped

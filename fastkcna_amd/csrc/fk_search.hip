@@ -550,6 +550,7 @@ void tree_grow(BuildState &b) {
         std::swap(cur, nxt);
         ++levels;
     }
+    const double t_bfs = now_sec();
     // 2. min-label propagation over the unreached subgraph
     uint32_t rounds = 0;
     while (true) {
@@ -566,13 +567,14 @@ void tree_grow(BuildState &b) {
     // labels and out-degrees into the handle's pinned buffers (grow-only)
     b.h->host_lab.ensure(N);
     b.h->host_deg.ensure(N);
-    uint32_t *lab = b.h->host_lab.p, *deg = b.h->host_deg.p;
-    FK_HIP(hipMemcpyAsync(lab, label.p, (size_t)N * 4, hipMemcpyDeviceToHost, b.s));
-    FK_HIP(hipMemcpyAsync(deg, b.nsg_len.p, (size_t)N * 4, hipMemcpyDeviceToHost, b.s));
+    FK_HIP(hipMemcpyAsync(b.h->host_lab.p, label.p, (size_t)N * 4, hipMemcpyDeviceToHost, b.s));
+    FK_HIP(hipMemcpyAsync(b.h->host_deg.p, b.nsg_len.p, (size_t)N * 4, hipMemcpyDeviceToHost, b.s));
     b.sync();
+    uint32_t *lab = b.h->host_lab.p, *deg = b.h->host_deg.p;
     // every round reads each unreached node's out-list and its targets' labels
     b.prof_end(pe, FK_PROF_CONN, (double)(rounds + levels) * ((double)N * 8.0 + 8.0 * (double)N * b.P.nsg_R));
     b.prof_flush();
+    const double t_lab = now_sec();
     std::vector<uint32_t> roots_of;  // attached ids, ascending
     for (uint32_t v = 0; v < N; ++v)
         if (lab[v] == v) roots_of.push_back(v);
@@ -581,6 +583,8 @@ void tree_grow(BuildState &b) {
     bands.roots = &roots_of;
     const uint32_t na = (uint32_t)roots_of.size();
     std::vector<uint32_t> e_src, e_slot, e_dst;
+    double t_loop = 0;
+    uint64_t n_draws = 0;  // random fallback draws
     if (na) {
         // 3. pool rows of the attached ids, one gather
         const uint32_t rs = b.cap + 1;
@@ -593,17 +597,27 @@ void tree_grow(BuildState &b) {
         FK_LAUNCH_CHECK();
         FK_HIP(hipMemcpyAsync(rows.p, drows.p, (size_t)na * rs * 4, hipMemcpyDeviceToHost, b.s));
         b.sync();
-        // 4. root choices in ascending order of the attached ids
+        const double t_rows = now_sec();
+        // 4. root choices in ascending order of the attached ids.  The band
+        // counts of a set are made once it has had 4096 roots (or a long
+        // fallback): then a root's pool row is scanned only while some
+        // reached node of the set is below nsg_R, since no entry can take
+        // the edge otherwise (a truncating prune leaves every out-list full)
+        uint32_t set_run = 0xFFFFFFFFu, in_set = 0;
         for (uint32_t k = 0; k < na; ++k) {
             const uint32_t u = roots_of[k];
             auto reached = [&](uint32_t v) { return lab[v] == FK_REACHED || lab[v] < u; };
             const uint32_t *prow = rows.p + (size_t)k * rs;
             uint32_t root = N;
-            for (uint32_t i = 0; i < prow[0]; ++i) {
-                const uint32_t p = prow[1 + i];
-                if (p < N && reached(p) && deg[p] < b.P.nsg_R) { root = p; break; }
-            }
             const uint32_t set = b.set_of_node(u), lo = b.set_lo[set], ns = b.set_n(set);
+            in_set = set == set_run ? in_set + 1 : 1;
+            set_run = set;
+            if (in_set == 4096 && bands.set != set) bands.reset(set, deg, lo, ns, b.P.nsg_R, b.ncap, u, N);
+            if (bands.set != set || bands.low[0] > 0)
+                for (uint32_t i = 0; i < prow[0]; ++i) {
+                    const uint32_t p = prow[1 + i];
+                    if (p < N && reached(p) && deg[p] < b.P.nsg_R) { root = p; break; }
+                }
             if (root == N) {
                 // a random reached node of u's set, with the set's stream; the
                 // band counts (a pass over the set) are made once a fallback
@@ -623,6 +637,7 @@ void tree_grow(BuildState &b) {
                         continue;
                     }
                     ++tries;
+                    ++n_draws;
                     const uint64_t factor = tries / ns + 1;
                     const uint32_t rid = lo + (uint32_t)(b.next_rand(set) % ns);
                     if (reached(rid) && deg[rid] < b.ncap && deg[rid] < b.P.nsg_R * factor) { root = rid; break; }
@@ -635,6 +650,7 @@ void tree_grow(BuildState &b) {
             bands.add_edge(deg, root, set);
             bands.reach(deg, u, set);
         }
+        t_loop = now_sec() - t_rows;
         DevBuf<uint32_t> s(na), sl(na), d(na);
         FK_HIP(hipMemcpyAsync(s.p, e_src.data(), (size_t)na * 4, hipMemcpyHostToDevice, b.s));
         FK_HIP(hipMemcpyAsync(sl.p, e_slot.data(), (size_t)na * 4, hipMemcpyHostToDevice, b.s));
@@ -656,8 +672,10 @@ void tree_grow(BuildState &b) {
     if (li < FK_MAX_LOOPS) b.st->tree_grow_time[li] = dt;
     b.st->t_conn += dt;
     if (b.verbose)
-        fprintf(stderr, "[fk] tree_grow: attached %u (%.3f s; %u BFS levels, %u label rounds)\n", na, dt, levels,
-                rounds);
+        fprintf(stderr,
+                "[fk] tree_grow: attached %u (%.3f s; %u BFS levels %.3f s, %u label rounds %.3f s, root choices %.3f s, "
+                "%llu random draws)\n",
+                na, dt, levels, t_bfs - t0, rounds, t_lab - t_bfs, t_loop, (unsigned long long)n_draws);
 }
 
 }  // namespace fk

@@ -85,10 +85,10 @@ typedef enum {
 #define FK_FLAG_SCREEN_NOTRAIN 0x80u
 
 /* Prune kernel variant.  Builds pick one per shape (rule, width, row kind,
- * nsg_R, search_K, sync or reversed) on its first use in a handle: that
- * prune runs once with each variant, timed, the out-lists and distance counts
- * are checked to be identical, and the faster variant is used from then on
- * (fk_stats.prune_variant).  Variant 0 checks each candidate against the
+ * nsg_R, search_K, sync or reversed) on its first use in a handle: each
+ * variant prunes the same sample of that launch's pools, timed, the sample's
+ * out-lists and distance counts are checked to be identical, the faster
+ * variant prunes the rest and is used from then on (fk_stats.prune_variant).  Variant 0 checks each candidate against the
  * first cached kept neighbours and splits the survivors' remaining checks
  * across the wavefront; variant 1 checks each candidate against every kept
  * neighbour itself.  These flags force one (no timing); fk_prune and
@@ -190,7 +190,8 @@ typedef struct {
      * nothing. */
     uint64_t d_prune_uncomputed;
     /* Prune variants used by the last main and reversed prune (1 + variant;
-     * 0: none ran), and the seconds spent timing variants in this build. */
+     * 0: none ran), and the seconds spent timing variants on samples in this
+     * build (a shape's first use in a handle). */
     uint32_t prune_variant, prune_rev_variant;
     double t_tune;
 } fk_stats;

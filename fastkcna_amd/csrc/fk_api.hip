@@ -452,7 +452,7 @@ static PruneLaunch prune_args(const fk_handle *h, fk_pg_type type, uint32_t pool
     a.cos_thr = alpha_cos_threshold(p->angle);
     a.rule = rule_of(type);
     a.out_stride = out_stride;
-    a.variant = (p->flags & FK_FLAG_PRUNE_V1) ? 1 : 0;
+    a.variant = (p->flags & FK_FLAG_PRUNE_V1) ? 1 : (p->flags & FK_FLAG_PRUNE_V2) ? 2 : 0;
     return a;
 }
 

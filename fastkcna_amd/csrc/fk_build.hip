@@ -343,8 +343,8 @@ __global__ void k_pack_csr(const uint32_t *__restrict__ nsg_id, const uint32_t *
 // work.  Small launches are measured whole, twice each.  Returns 1 + variant.
 static uint32_t tuned_prune(BuildState &b, PruneLaunch &a, bool reversed) {
     const uint32_t flags = b.P.flags;
-    if (flags & (FK_FLAG_PRUNE_V0 | FK_FLAG_PRUNE_V1)) {
-        a.variant = (flags & FK_FLAG_PRUNE_V1) ? 1 : 0;
+    if (flags & (FK_FLAG_PRUNE_V0 | FK_FLAG_PRUNE_V1 | FK_FLAG_PRUNE_V2)) {
+        a.variant = (flags & FK_FLAG_PRUNE_V1) ? 1 : (flags & FK_FLAG_PRUNE_V2) ? 2 : 0;
         launch_prune(b.s, *b.ds, a);
         return 1u + (uint32_t)a.variant;
     }
@@ -412,8 +412,8 @@ static uint32_t tuned_prune(BuildState &b, PruneLaunch &a, bool reversed) {
         launch_prune(b.s, *b.ds, rest);
     }
     if (b.verbose)
-        fprintf(stderr, "[fk] prune variant %d chosen for rule %d, dpad %u, rows %u, R %u%s (sample of %u: %.3f vs %.3f ms)\n",
-                best, a.rule, b.ds->dpad, kind, a.R, reversed ? ", reversed" : "", m, best_ms[0], best_ms[1]);
+        fprintf(stderr, "[fk] prune variant %d chosen for rule %d, dpad %u, rows %u, R %u%s (sample of %u: %.3f / %.3f / %.3f ms)\n",
+                best, a.rule, b.ds->dpad, kind, a.R, reversed ? ", reversed" : "", m, best_ms[0], best_ms[1], best_ms[2]);
     return 1u + (uint32_t)best;
 }
 
@@ -461,7 +461,7 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
     b.width = P.nsg_R;
     b.set_width.assign(b.n_sets, P.nsg_R);
     b.st->d_prune += cc;
-    if (rule_of(type) == FK_RULE_TAU) b.st->d_prune_uncomputed += b.tally.read1(b.s, FK_T_PRUNE_SKIP);
+    b.st->d_prune_uncomputed += b.tally.read1(b.s, FK_T_PRUNE_SKIP);
     const double t1 = now_sec();
     if (li < FK_MAX_LOOPS) {
         b.st->prune_cc[li] = (float)((double)cc / ((double)N * (double)(N - 1) / 2.0));
@@ -533,7 +533,7 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
         b.prof_end(pr, FK_PROF_PRUNE_REV, (double)rcc[1] * (4.0 * b.ds->d + 8.0) + 8.0 * (double)hc[0] * P.nsg_R);
         b.prof_flush();
         b.st->d_prune_rev += rc;
-        if (rule_of(type) == FK_RULE_TAU) b.st->d_prune_uncomputed += b.tally.read1(b.s, FK_T_PRUNE_SKIP);
+        b.st->d_prune_uncomputed += b.tally.read1(b.s, FK_T_PRUNE_SKIP);
     }
     const double t2 = now_sec();
     if (li < FK_MAX_LOOPS) b.st->inter_time[li] = t2 - t1;

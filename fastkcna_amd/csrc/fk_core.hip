@@ -136,6 +136,22 @@ struct PruneMC {
     static constexpr uint32_t value = raw < cap ? raw : cap;
 };
 
+// Variant 2 caches the kept rows as halves: up to 64 rows in 8 KiB per wave
+// (FK_PRUNE_MC16_BYTES; 16 KiB halves the waves per SIMD and was slower on
+// every measured shape).
+#ifndef FK_PRUNE_MC16_BYTES
+#define FK_PRUNE_MC16_BYTES 8192
+#endif
+template <int NB>
+struct PruneMC16 {
+    static constexpr uint32_t raw = NB == 0 ? 0u : (uint32_t)FK_PRUNE_MC16_BYTES / ((uint32_t)NB * 16u);
+    static constexpr uint32_t value = raw < 64u ? raw : 64u;
+};
+template <int RULE, int NB, int MODE, int FV>
+struct PruneCache {
+    static constexpr uint32_t value = FV == 2 ? PruneMC16<NB>::value : PruneMC<RULE, NB, MODE>::value;
+};
+
 // LDS per wave: the cached kept chains (mc x dpad), one staging row
 // (dpad), kept ids, distances and norms (R each).
 //
@@ -154,7 +170,12 @@ __host__ __device__ inline uint32_t prune_wave_bytes(uint32_t mc, uint32_t dpad,
 // neighbours); 1 -- each octet checks its own candidate against every kept
 // neighbour, the cached ones from LDS and the rest from global memory (best
 // when occlusion is rare and most candidates are checked against the whole
-// kept list).  Both make the same decisions and count the same distances.
+// kept list); 2 -- like 0, with up to 64 kept neighbours cached as
+// half-precision rows: a check is decided by a rigorous bound on the
+// distance to the cached halves and computed exactly from the FP32 row in
+// global memory only when the bound cannot decide (for rows that no longer
+// fit the cache, when the HBM re-reads of uncached kept rows set the time).
+// All make the same decisions and count the same distances.
 template <int RULE, int NB, int MODE, int FV>
 struct PruneWave {
     using E = typename RowKind<NB, MODE>::E;
@@ -164,6 +185,7 @@ struct PruneWave {
     float tau3, cos_thr;
     const E *lane_chains;      // chain k of row 0
     E *lane_cache;             // chain k of cached kept neighbour 0
+    _Float16 *lane_cache16;    // variant 2: the same as halves
     E *lane_stage;             // chain k of the staging row
     uint32_t *so;
     float *sd, *sn;
@@ -193,8 +215,34 @@ struct PruneWave {
             if constexpr (DOT) return exact_dist(cn, sn[t], crow.dot(o, nb));
             else return crow.dist(o, nb);
         };
-        if (t < PruneMC<RULE, NB, MODE>::value) return dist_to(lane_cache + (size_t)t * dpad);
+        if (FV != 2 && t < PruneCache<RULE, NB, MODE, FV>::value) return dist_to(lane_cache + (size_t)t * dpad);
         return dist_to(lane_chains + (size_t)tid * dpad);
+    }
+    // Variant 2: whether the canonical distance djk of a candidate at
+    // distance pd to the kept neighbour whose half row is at distance df
+    // (dist_h) occludes it: 1 / 0 when the bounds decide, -1 otherwise.
+    // With x the candidate, y the kept row and y~ its halves, ||y - y~|| <=
+    // rho (kept per row); df is ||x - y~||^2 in FP32 and djk ||x - y||^2 in
+    // the canonical FP32 order, each within a relative g of its exact value
+    // (non-negative terms); so sqrt(djk) lies within rho of sqrt(df), with g
+    // and a final 2^-17 margin for the FP32 evaluation of the bounds.
+    __device__ int filt_decide(float df, float rho, float pd) const {
+        if (!(df < 3.0e38f)) return -1;  // a half row that overflowed
+        const float gm = (float)(dpad + 32u) * 5.9604645e-8f;
+        const float sq = __builtin_sqrtf(df);
+        const float slo = sq * (1.f - gm) - rho, shi = sq * (1.f + gm) + rho;
+        const float lo = (slo > 0.f ? slo * slo : 0.f) * (1.f - gm) * (1.f - 7.62939453125e-06f);
+        const float hi = shi * shi * (1.f + gm) * (1.f + 7.62939453125e-06f);
+        const float th = RULE == FK_RULE_TAU ? pd - tau3 : pd;  // the rule's comparison value
+        if (lo >= th) return 0;
+        if (RULE != FK_RULE_ALPHA && hi < th) return 1;
+        return -1;
+    }
+    // Variant 2: rho of a row held in this octet's chain registers
+    __device__ float half_rho(const Chain &r) const {
+        const float gm = (float)(dpad + 32u) * 5.9604645e-8f;
+        return (4.8828125e-4f * __builtin_sqrtf(r.norm2()) * (1.f + gm) + 2.9802322e-8f * __builtin_sqrtf((float)dpad)) *
+               (1.f + 7.62939453125e-06f);
     }
     // Decides the batch at pos whose key (this octet's), chain and norm
     // (DOT) are given.
@@ -210,15 +258,15 @@ struct PruneWave {
         // sequential loop evaluates are counted but not computed
         const bool nodist = RULE == FK_RULE_TAU && !(pd - tau3 > 0.f);
         bool occl = false;
-        uint32_t mycc = 0;
+        uint32_t mycc = 0, myfilt = 0;
         __builtin_amdgcn_wave_barrier();
         const uint32_t n0 = nout;
-        constexpr uint32_t MC = PruneMC<RULE, NB, MODE>::value;
+        constexpr uint32_t MC = PruneCache<RULE, NB, MODE, FV>::value;
         // each octet checks its own candidate against the first nA kept
         // neighbours; the survivors go on one at a time, the wave's octets
         // taking eight kept neighbours per step
         constexpr uint32_t FAST = MODE == 2 ? FK_PRUNE_FAST8 : FK_PRUNE_FAST;
-        constexpr uint32_t MF = FV ? 0xFFFFFFFFu : (MC < FAST ? MC : FAST);
+        constexpr uint32_t MF = FV == 1 ? 0xFFFFFFFFu : FV == 2 ? MC : (MC < FAST ? MC : FAST);
         const uint32_t nA = n0 < MF ? n0 : MF;
         const uint32_t nC = nA < MC ? nA : MC;  // the part of the fast checks with cached rows
         if (!skip && nodist) {
@@ -233,7 +281,24 @@ struct PruneWave {
             // (two register chains, unrolled by two so neither is copied;
             // chains of up to 16 registers, where the second one costs no
             // occupancy)
-            if constexpr (FK_PRUNE_DB && PruneMC<RULE, NB, MODE>::value > 0 && sizeof(Chain) <= 64) {
+            if constexpr (FV == 2) {
+                // the cached kept neighbours as halves: bound first, the
+                // FP32 row only when the bound cannot decide
+                for (uint32_t t = 0; t < nC; ++t) {
+                    const uint32_t tid = so[t];
+                    if (tid == pid) { occl = true; break; }
+                    ++mycc;
+                    const int dec = filt_decide(crow.dist_h(lane_cache16 + (size_t)t * dpad), sn[t], pd);
+                    bool o;
+                    if (dec >= 0) {
+                        o = dec != 0;
+                        ++myfilt;
+                    } else {
+                        o = occludes<RULE>(crow.dist(lane_chains + (size_t)tid * dpad, nb), pd, sd[t], tau3, cos_thr);
+                    }
+                    if (o) { occl = true; break; }
+                }
+            } else if constexpr (FK_PRUNE_DB && MC > 0 && sizeof(Chain) <= 64) {
                 auto check = [&](uint32_t t, const Chain &o) __attribute__((always_inline)) -> bool {
                     if (so[t] == pid) return true;
                     ++mycc;
@@ -341,12 +406,18 @@ struct PruneWave {
             const float ad = __shfl(pd, (int)(al * 8), 64);
             const float an = DOT ? __shfl(cn, (int)(al * 8), 64) : 0.f;
             const uint32_t slot = nout;
+            float arho = 0.f;
+            if constexpr (FV == 2) arho = __shfl(half_rho(crow), (int)(al * 8), 64);
             if (lane == 0) {
                 so[slot] = aid;
                 sd[slot] = ad;
                 if (DOT) sn[slot] = an;
+                if (FV == 2) sn[slot] = arho;
             }
-            if (slot < PruneMC<RULE, NB, MODE>::value && g == al) crow.store(lane_cache + (size_t)slot * dpad, nb);
+            if (slot < PruneCache<RULE, NB, MODE, FV>::value && g == al) {
+                if constexpr (FV == 2) crow.store_h(lane_cache16 + (size_t)slot * dpad);
+                else crow.store(lane_cache + (size_t)slot * dpad, nb);
+            }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             ++nout;
@@ -366,7 +437,7 @@ struct PruneWave {
         }
         if (k == 0 && g <= stop_g) {
             cc += mycc;
-            if (nodist && !skip) uncomputed += mycc;
+            uncomputed += (nodist && !skip) ? mycc : myfilt;
             exam += skip ? 0u : 1u;
         }
     }
@@ -391,7 +462,7 @@ __global__ __launch_bounds__(256) FK_PRUNE_ATTR void k_prune(const void *__restr
     constexpr bool DOT = PW::DOT;
     const E *chains = reinterpret_cast<const E *>(chains_v);
     extern __shared__ __align__(16) uint8_t smem[];
-    constexpr uint32_t MC = PruneMC<RULE, NB, MODE>::value;
+    constexpr uint32_t MC = PruneCache<RULE, NB, MODE, FV>::value;
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t vb = a.xcd_order ? xcd_position((gridDim.x + FK_NUM_XCD - 1) / FK_NUM_XCD) : blockIdx.x;
@@ -417,9 +488,13 @@ __global__ __launch_bounds__(256) FK_PRUNE_ATTR void k_prune(const void *__restr
     pw.dpad = dpad;
     pw.nb = NB ? (uint32_t)NB : (dpad >> 3);
     pw.lane_chains = chains + pw.k * pw.nb;
+    // LDS of the wave: MC cached rows (halves for variant 2), the staging
+    // row, then kept ids, distances and norms (or variant 2's row bounds)
+    const size_t cache_bytes = (size_t)MC * dpad * (FV == 2 ? 2 : sizeof(E));
     pw.lane_cache = reinterpret_cast<E *>(wb) + pw.k * pw.nb;
-    pw.lane_stage = reinterpret_cast<E *>(wb) + (size_t)MC * dpad + pw.k * pw.nb;
-    pw.so = reinterpret_cast<uint32_t *>(wb + (size_t)(MC + 1) * dpad * sizeof(E));
+    pw.lane_cache16 = reinterpret_cast<_Float16 *>(wb) + pw.k * pw.nb;
+    pw.lane_stage = reinterpret_cast<E *>(wb + cache_bytes) + pw.k * pw.nb;
+    pw.so = reinterpret_cast<uint32_t *>(wb + cache_bytes + (size_t)dpad * sizeof(E));
     pw.sd = reinterpret_cast<float *>(pw.so + a.R);
     pw.sn = pw.sd + a.R;
     pw.nrm = a.nrm;
@@ -445,7 +520,13 @@ __global__ __launch_bounds__(256) FK_PRUNE_ATTR void k_prune(const void *__restr
         if (MC > 0 && pw.g == 0) {
             Chain t0;
             t0.load(pw.lane_chains + (size_t)key_id(k0) * dpad, pw.nb);
-            t0.store(pw.lane_cache, pw.nb);
+            if constexpr (FV == 2) {
+                t0.store_h(pw.lane_cache16);
+                const float rho = pw.half_rho(t0);
+                if (lane == 0) pw.sn[0] = rho;
+            } else {
+                t0.store(pw.lane_cache, pw.nb);
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -488,7 +569,7 @@ __global__ __launch_bounds__(256) FK_PRUNE_ATTR void k_prune(const void *__restr
         const unsigned long long s = wave_sum(pw.exam);
         if (lane == 0 && s) tally_add(a.examined, s);
     }
-    if (RULE == FK_RULE_TAU && a.uncomputed) {
+    if ((RULE == FK_RULE_TAU || FV == 2) && a.uncomputed) {
         const unsigned long long s = wave_sum(pw.uncomputed);
         if (lane == 0 && s) tally_add(a.uncomputed, s);
     }
@@ -519,8 +600,18 @@ static void launch_prune_nbv(hipStream_t s, const Dataset &ds, const PruneLaunch
 
 template <int RULE, int NB>
 static void launch_prune_nb(hipStream_t s, const Dataset &ds, const PruneLaunch &a, unsigned grid) {
+    if constexpr (NB == 12 || NB == 16 || NB == 32) {
+        // variant 2: FP32 rows (the exact forms' rows are cheaper already)
+        if (a.variant == 2 && !a.nrm) {
+            constexpr uint32_t MC = PruneCache<RULE, NB, 0, 2>::value;
+            const uint32_t wb = (uint32_t)(((size_t)MC * ds.dpad * 2 + (size_t)ds.dpad * 4 + (size_t)a.R * 12 + 15) & ~15ull);
+            hipLaunchKernelGGL((k_prune<RULE, NB, 0, 2>), dim3(grid), dim3(64 * FK_PRUNE_WPB), (size_t)wb * FK_PRUNE_WPB,
+                               s, (const void *)ds.chains.p, ds.dpad, a, wb);
+            return;
+        }
+    }
     if (a.variant == 1) launch_prune_nbv<RULE, NB, 1>(s, ds, a, grid);
-    else launch_prune_nbv<RULE, NB, 0>(s, ds, a, grid);
+    else launch_prune_nbv<RULE, NB, 0>(s, ds, a, grid);  // variant 2 where it does not apply runs as 0
 }
 
 template <int RULE>
@@ -597,7 +688,7 @@ void launch_prune_digest(hipStream_t s, const PruneLaunch &a, unsigned long long
 void launch_prune(hipStream_t s, const Dataset &ds, const PruneLaunch &a) {
     if (a.n_tasks == 0) return;
     FK_CHECK_ARG(a.R <= FK_PRUNE_RMAX, "nsg_R above the supported maximum (256)");
-    FK_CHECK_ARG(a.variant == 0 || a.variant == 1, "unknown prune variant");
+    FK_CHECK_ARG(a.variant >= 0 && a.variant < FK_PRUNE_VARIANTS, "unknown prune variant");
     unsigned grid = (a.n_tasks + FK_PRUNE_WPB - 1) / FK_PRUNE_WPB;
     if (a.xcd_order) grid = (grid + FK_NUM_XCD - 1) / FK_NUM_XCD * FK_NUM_XCD;
     switch (a.rule) {

@@ -203,6 +203,44 @@ struct OctChain {
         }
         return octet_sum(acc);
     }
+    // Half-precision copy of the chain (round to nearest even), NB / 4
+    // 8-byte stores; the chain of lane k starts k * NB halves in (8-byte
+    // aligned when NB % 4 == 0).
+    __device__ __forceinline__ void store_h(_Float16 *__restrict__ c) const {
+#pragma unroll
+        for (int q = 0; q < NB / 4; ++q) {
+            const _Float16 h[4] = {(_Float16)v[q].x, (_Float16)v[q].y, (_Float16)v[q].z, (_Float16)v[q].w};
+            reinterpret_cast<uint2 *>(c)[q] = *reinterpret_cast<const uint2 *>(h);
+        }
+    }
+    // Squared distance to a half-precision chain, the halves widened exactly
+    // to FP32: an approximation of the distance to the row the halves were
+    // made from (any summation order; used with an error bound only).
+    __device__ __forceinline__ float dist_h(const _Float16 *__restrict__ o) const {
+        float acc = 0.f, t;
+#pragma unroll
+        for (int q = 0; q < NB / 4; ++q) {
+            const uint2 w = reinterpret_cast<const uint2 *>(o)[q];
+            const _Float16 *h = reinterpret_cast<const _Float16 *>(&w);
+            t = v[q].x - (float)h[0]; acc = __builtin_fmaf(t, t, acc);
+            t = v[q].y - (float)h[1]; acc = __builtin_fmaf(t, t, acc);
+            t = v[q].z - (float)h[2]; acc = __builtin_fmaf(t, t, acc);
+            t = v[q].w - (float)h[3]; acc = __builtin_fmaf(t, t, acc);
+        }
+        return octet_sum(acc);
+    }
+    // Squared norm of the row (FP32, any order)
+    __device__ __forceinline__ float norm2() const {
+        float acc = 0.f;
+#pragma unroll
+        for (int q = 0; q < NB / 4; ++q) {
+            acc = __builtin_fmaf(v[q].x, v[q].x, acc);
+            acc = __builtin_fmaf(v[q].y, v[q].y, acc);
+            acc = __builtin_fmaf(v[q].z, v[q].z, acc);
+            acc = __builtin_fmaf(v[q].w, v[q].w, acc);
+        }
+        return octet_sum(acc);
+    }
     // dot products (the exact form: see exact_dist)
     __device__ __forceinline__ float dot_reg(const OctChain &o, uint32_t) const {
         float acc = 0.f;

@@ -253,7 +253,7 @@ struct PruneLaunch {
     const uint8_t *chains8 = nullptr; // exact-form byte rows (chain-major), with nrm
 };
 void launch_prune(hipStream_t s, const Dataset &ds, const PruneLaunch &a);
-#define FK_PRUNE_VARIANTS 2
+#define FK_PRUNE_VARIANTS 3
 // Adds an order-independent digest of the launch's output lists to *out.
 void launch_prune_digest(hipStream_t s, const PruneLaunch &a, unsigned long long *out);
 void build_chains(hipStream_t s, Dataset &ds);  // fills ds.chains from ds.rows

@@ -91,10 +91,15 @@ typedef enum {
  * variant prunes the rest and is used from then on (fk_stats.prune_variant).  Variant 0 checks each candidate against the
  * first cached kept neighbours and splits the survivors' remaining checks
  * across the wavefront; variant 1 checks each candidate against every kept
- * neighbour itself.  These flags force one (no timing); fk_prune and
- * fk_prune_noalloc use variant 0 unless FK_FLAG_PRUNE_V1 is set. */
+ * neighbour itself; variant 2 (FP32 rows of width 96, 128 or 256) caches up
+ * to 64 kept neighbours (8 KiB) as half-precision rows and decides a check from a
+ * rigorous bound on the distance to them, computing it from the FP32 rows
+ * only when the bound cannot decide (those checks are counted in
+ * d_prune_uncomputed).  These flags force one (no timing); fk_prune and
+ * fk_prune_noalloc use variant 0 unless FK_FLAG_PRUNE_V1 or V2 is set. */
 #define FK_FLAG_PRUNE_V0 0x100u
 #define FK_FLAG_PRUNE_V1 0x200u
+#define FK_FLAG_PRUNE_V2 0x400u
 
 /* Gram operand formats (fk_stats.gram_mode, fk_gram_pair_distances). */
 enum {
@@ -183,11 +188,12 @@ typedef struct {
      * counted but not computed again (the result is the same). */
     uint64_t d_bv_computed;
     /* Prune and reversed-prune distances counted in d_prune / d_prune_rev
-     * but not computed: under the tau-MNG rule a candidate whose distance to
-     * the owner is at most 3 tau cannot be occluded (djk < dist - 3 tau never
-     * holds for a squared distance), so only a kept neighbour with its id
-     * removes it and the distances the reference evaluates for it decide
-     * nothing. */
+     * but not computed in FP32: under the tau-MNG rule a candidate whose
+     * distance to the owner is at most 3 tau cannot be occluded (djk < dist -
+     * 3 tau never holds for a squared distance), so only a kept neighbour
+     * with its id removes it and the distances the reference evaluates for it
+     * decide nothing; prune variant 2 decides most checks from a bound on the
+     * distance to a half-precision copy of the kept row. */
     uint64_t d_prune_uncomputed;
     /* Prune variants used by the last main and reversed prune (1 + variant;
      * 0: none ran), and the seconds spent timing variants on samples in this

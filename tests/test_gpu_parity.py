@@ -34,6 +34,8 @@ def datasets():
         # dpad % 32 of 24 and 16: the canonical order's tail blocks come first
         "odd20": np.random.default_rng(15).normal(size=(1500, 20)).astype(np.float32),
         "odd44": integer_mixture(2500, 44, 16, seed=16),
+        # width 256: the prune's widest register-chain path (and variant 2's)
+        "unit256": unit_mixture(2000, 256, 16, seed=17, sigma=0.3),
     }
 
 
@@ -94,10 +96,15 @@ def make_pools(x, n_tasks, L, rng, q=None):
     return pools, plen, q
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+# prune kernel variants (the build picks one per shape by timing; variant 2
+# applies to FP32 rows of width 96, 128 and 256 and runs as 0 elsewhere)
+VARIANT_FLAGS = {0: _ffi.FK_FLAG_PRUNE_V0, 1: _ffi.FK_FLAG_PRUNE_V1, 2: _ffi.FK_FLAG_PRUNE_V2}
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("pg", [FK_PG_NSG, FK_PG_TAUMNG, FK_PG_ALPHAPG])
 @pytest.mark.parametrize("reversed_", [False, True])
-@pytest.mark.parametrize("name", ["int128", "unit96", "float960", "odd100", "odd20", "odd44"])
+@pytest.mark.parametrize("name", ["int128", "unit96", "float960", "odd100", "odd20", "odd44", "unit256"])
 def test_prune_bitexact(device, pg, reversed_, name, variant):
     """Both prune kernel variants (the build picks one per shape by timing)
     against the oracle."""
@@ -105,9 +112,10 @@ def test_prune_bitexact(device, pg, reversed_, name, variant):
     say("prune", pg, reversed_, name, "variant", variant)
     device.upload(x)
     rng = np.random.default_rng(3)
-    tau = {"int128": 300.0, "unit96": 0.05, "float960": 0.5, "odd100": 1.0, "odd20": 0.2, "odd44": 100.0}[name]
+    tau = {"int128": 300.0, "unit96": 0.05, "float960": 0.5, "odd100": 1.0, "odd20": 0.2, "odd44": 100.0,
+           "unit256": 0.05}[name]
     p = IndexParams(nsg_R=24, search_K=90, tau=tau, angle=63)
-    p.flags = _ffi.FK_FLAG_PRUNE_V1 if variant else 0
+    p.flags = VARIANT_FLAGS[variant]
     pools, plen, q = make_pools(x, 400, 120, rng)
     # a self entry at the front of some pools
     for t in range(0, 400, 7):
@@ -236,16 +244,17 @@ def test_prune_variants_give_the_same_build(device, pg, name):
     p = IndexParams(K=40, L=40, S=10, R=40, iterations=4, nsg_R=16, search_L=30, search_K=40, loop_i=1,
                     tau={"int128": 20.0, "unit96": 0.01, "odd44": 5.0}[name], angle=60)
     out = []
-    for flags in (_ffi.FK_FLAG_PRUNE_V0, _ffi.FK_FLAG_PRUNE_V1, 0):
+    for flags in (_ffi.FK_FLAG_PRUNE_V0, _ffi.FK_FLAG_PRUNE_V1, _ffi.FK_FLAG_PRUNE_V2, 0):
         off, ids, ep, width, st = device.build(pg, p.copy(flags=flags))
         out.append((off, ids, ep, st))
     for off, ids, ep, st in out[1:]:
         assert np.array_equal(off, out[0][0]) and np.array_equal(ids, out[0][1]) and ep == out[0][2]
         for k in ("d_prune", "d_prune_rev", "d_bv", "d_inter", "aod"):
             assert st[k] == out[0][3][k], k
-    assert out[0][3]["prune_variant"] == 1 and out[1][3]["prune_variant"] == 2
-    assert out[2][3]["prune_variant"] in (1, 2)
-    say("variants", pg, name, "picked", out[2][3]["prune_variant"] - 1, "tuning s", out[2][3]["t_tune"])
+    assert [o[3]["prune_variant"] for o in out[:3]] == [1, 2, 3]
+    assert out[3][3]["prune_variant"] in (1, 2, 3)
+    say("variants", pg, name, "picked", out[3][3]["prune_variant"] - 1, "tuning s", out[3][3]["t_tune"],
+        "uncomputed (variant 2)", out[2][3]["d_prune_uncomputed"], "of", out[2][3]["d_prune"] + out[2][3]["d_prune_rev"])
 
 
 @pytest.mark.parametrize("name", ["odd44", "odd20", "float960"])
@@ -288,7 +297,7 @@ def _c2_dumped_pool(device, p, n_dump=10_000):
     return _C2_CACHE["pool"]
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("pg", [FK_PG_NSG, FK_PG_ALPHAPG])
 def test_prune_bitexact_c2_dumped_pool(device, pg, variant):
     """The 1M x 128 NSG workload (1,024-centre integer mixture, seed 1,
@@ -309,7 +318,7 @@ def test_prune_bitexact_c2_dumped_pool(device, pg, variant):
     exact = oracle.pair_distances(x, rows, ids[::10].ravel())
     assert np.array_equal(exact.view(np.uint32), d[::10].ravel().view(np.uint32))
     q = np.arange(n_dump, dtype=np.uint32)
-    p = p.copy(flags=_ffi.FK_FLAG_PRUNE_V1 if variant else 0)
+    p = p.copy(flags=VARIANT_FLAGS[variant])
     g = device.prune(pg, pools, plen, q, len(x), p)
     c = oracle.prune(pg, x, pools, plen, q, len(x), p)
     assert np.array_equal(g[2], c[2])

@@ -474,7 +474,7 @@ def c5_leg(dev):
     dev.upload(x)
     p = params_c5(IndexParams)
     t = time.perf_counter()
-    off, ids, ep, _, st = dev.build(FK_PG_HNSW_LAYER, p)
+    off, ids, ep, _, st = dev.build(FK_PG_HNSW_LAYER, p.copy(flags=_ffi.FK_FLAG_PROFILE))
     plain = time.perf_counter() - t
     gt, _ = dev.knn(qs, 10)
     rec = recall_at_10(dev.search(off, ids, ep, qs, 40, 10)[0], gt)
@@ -485,7 +485,14 @@ def c5_leg(dev):
     out = {"workload": "C5: HNSW level 0, 10,000,000x96 unit-normalised mixture of 4,096 Gaussians (seed 4, "
                        "sigma 0.25), K=L=200 S=10 R=100 I=6 nsg_R=64 search_L=40 search_K=200",
            "build_sec": plain, "dist_evals_per_sec": total_dists(st) / plain,
+           "dist_evals_computed_per_sec": total_dists(st, computed=True) / plain,
            "knn_recall_at_maxl": knn_recall_entries(st), "query_recall_at_10_flat": rec,
+           "families_ms": {f: round(st["prof_ms"][i], 1) for i, f in enumerate(_ffi.PROF_FAMILIES)
+                           if st["prof_launches"][i]},
+           "phase_sec": {k: round(float(st[k]), 4) for k in ("t_ctrl", "t_nnd", "t_prune", "t_inter", "t_ep", "t_conn",
+                                                            "t_bv", "t_total")},
+           "prune_variant": int(st["prune_variant"]) - 1, "prune_rev_variant": int(st["prune_rev_variant"]) - 1,
+           "tuning_sec": float(st["t_tune"]),
            "learned_screen": {"build_sec": scr, "speedup_vs_plain": plain / scr,
                               "knn_recall_at_maxl": knn_recall_entries(sst),
                               "screened_pairs": int(sst["d_screened"]), "skipped_pairs": int(sst["d_screen_skip"]),
@@ -514,6 +521,8 @@ def main():
                     help="datasets of the batching workload timed batched vs one by one (0: skip)")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the other workloads (C1 fasthnsw, C3 tau-MNG 1M x 960, C5 10M x 96)")
+    ap.add_argument("--leg", choices=["C1", "C3", "C4", "C5"],
+                    help="run only this workload's leg and print its JSON (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the build does not shard "
@@ -523,6 +532,13 @@ def main():
     from fastkcna_amd import FK_PG_NSG, Device, _ffi
     from fastkcna_amd.datasets import integer_mixture, params_c2
     from fastkcna_amd.kgraph import IndexParams
+
+    if args.leg:
+        dev = Device(0)
+        legs = {"C1": c1_leg, "C3": c3_leg, "C5": c5_leg, "C4": lambda d: batching_leg(d, args.batch_sets or 256)}
+        print(json.dumps(legs[args.leg](dev)), flush=True)
+        dev.close()
+        return
 
     t = time.perf_counter()
     x = integer_mixture(args.n + args.queries, 128, 1024, seed=1)

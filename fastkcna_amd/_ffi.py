@@ -25,6 +25,7 @@ FK_FLAG_SCREEN_NOTRAIN = 0x80
 FK_FLAG_PRUNE_V0 = 0x100
 FK_FLAG_PRUNE_V1 = 0x200
 FK_FLAG_PRUNE_V2 = 0x400
+FK_FLAG_PRUNE_V3 = 0x800
 FK_GRAM_NONE, FK_GRAM_EXACT, FK_GRAM_BF16, FK_GRAM_F16 = range(4)
 GRAM_MODE_NAMES = {FK_GRAM_NONE: "none", FK_GRAM_EXACT: "exact", FK_GRAM_BF16: "bf16", FK_GRAM_F16: "f16"}
 FK_MAX_LOOPS = 16

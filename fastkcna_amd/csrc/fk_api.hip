@@ -452,7 +452,8 @@ static PruneLaunch prune_args(const fk_handle *h, fk_pg_type type, uint32_t pool
     a.cos_thr = alpha_cos_threshold(p->angle);
     a.rule = rule_of(type);
     a.out_stride = out_stride;
-    a.variant = (p->flags & FK_FLAG_PRUNE_V1) ? 1 : (p->flags & FK_FLAG_PRUNE_V2) ? 2 : 0;
+    a.variant = (p->flags & FK_FLAG_PRUNE_V1) ? 1 : (p->flags & FK_FLAG_PRUNE_V2) ? 2 : (p->flags & FK_FLAG_PRUNE_V3) ? 3 : 0;
+    a.gram = (p->flags & FK_FLAG_GRAM) != 0;
     return a;
 }
 

@@ -340,15 +340,17 @@ __global__ void k_pack_csr(const uint32_t *__restrict__ nsg_id, const uint32_t *
 // (order-independent digest) and distance counts to agree, keeps the faster
 // variant on the handle and runs it on the remaining tasks: the launch's
 // outputs and tallies are those of a single launch, at about a fifth more
-// work.  Small launches are measured whole, twice each.  Returns 1 + variant.
+// work.  Small launches are measured whole, twice each.  Variants that do
+// not apply to the shape run as variant 0 and are not timed.  Returns
+// 1 + variant.
 static uint32_t tuned_prune(BuildState &b, PruneLaunch &a, bool reversed) {
     const uint32_t flags = b.P.flags;
-    if (flags & (FK_FLAG_PRUNE_V0 | FK_FLAG_PRUNE_V1 | FK_FLAG_PRUNE_V2)) {
-        a.variant = (flags & FK_FLAG_PRUNE_V1) ? 1 : (flags & FK_FLAG_PRUNE_V2) ? 2 : 0;
+    if (flags & (FK_FLAG_PRUNE_V0 | FK_FLAG_PRUNE_V1 | FK_FLAG_PRUNE_V2 | FK_FLAG_PRUNE_V3)) {
+        a.variant = (flags & FK_FLAG_PRUNE_V1) ? 1 : (flags & FK_FLAG_PRUNE_V2) ? 2 : (flags & FK_FLAG_PRUNE_V3) ? 3 : 0;
         launch_prune(b.s, *b.ds, a);
         return 1u + (uint32_t)a.variant;
     }
-    const uint32_t kind = a.chains8 ? 2u : (a.nrm ? 1u : 0u);
+    const uint32_t kind = (a.chains8 ? 2u : (a.nrm ? 1u : 0u)) | (a.gram ? 4u : 0u);
     const std::array<uint32_t, 6> key = {(uint32_t)a.rule, b.ds->dpad, kind, a.R, a.maxc, reversed ? 1u : 0u};
     auto it = b.h->prune_variant.find(key);
     if (it != b.h->prune_variant.end()) {
@@ -371,6 +373,10 @@ static uint32_t tuned_prune(BuildState &b, PruneLaunch &a, bool reversed) {
         // variants 0, 1, ..., then 0 again
         for (int r = 0; r <= FK_PRUNE_VARIANTS; ++r) {
             const int v = r % FK_PRUNE_VARIANTS;
+            if (!prune_variant_applies(*b.ds, smp, v)) {  // it would run as variant 0
+                best_ms[v] = INFINITY;
+                continue;
+            }
             b.tally.clear(b.s, FK_T_PRUNE_CC, 2);
             b.tally.clear(b.s, FK_T_PRUNE_SKIP);
             smp.variant = v;
@@ -412,8 +418,9 @@ static uint32_t tuned_prune(BuildState &b, PruneLaunch &a, bool reversed) {
         launch_prune(b.s, *b.ds, rest);
     }
     if (b.verbose)
-        fprintf(stderr, "[fk] prune variant %d chosen for rule %d, dpad %u, rows %u, R %u%s (sample of %u: %.3f / %.3f / %.3f ms)\n",
-                best, a.rule, b.ds->dpad, kind, a.R, reversed ? ", reversed" : "", m, best_ms[0], best_ms[1], best_ms[2]);
+        fprintf(stderr, "[fk] prune variant %d chosen for rule %d, dpad %u, rows %u, R %u%s (sample of %u: %.3f / %.3f / %.3f / %.3f ms)\n",
+                best, a.rule, b.ds->dpad, kind, a.R, reversed ? ", reversed" : "", m, best_ms[0], best_ms[1], best_ms[2],
+                best_ms[3]);
     return 1u + (uint32_t)best;
 }
 
@@ -445,6 +452,7 @@ static void link(BuildState &b, fk_pg_type type, uint32_t li) {
     a.cc = b.tally.at(FK_T_PRUNE_CC);
     a.examined = b.tally.at(FK_T_PRUNE_EXAM);
     a.uncomputed = b.tally.at(FK_T_PRUNE_SKIP);
+    a.gram = (P.flags & FK_FLAG_GRAM) != 0;
     if (b.has_order) {
         a.task_list = b.order.p;
         a.xcd_order = true;

@@ -685,12 +685,23 @@ void launch_prune_digest(hipStream_t s, const PruneLaunch &a, unsigned long long
     FK_LAUNCH_CHECK();
 }
 
+bool prune_variant_applies(const Dataset &ds, const PruneLaunch &a, int v) {
+    switch (v) {
+        case 0:
+        case 1: return true;
+        case 2: return (ds.dpad == 96 || ds.dpad == 128 || ds.dpad == 256) && !a.nrm;
+        case 3: return prune_gram_applies(ds, a);
+        default: return false;
+    }
+}
+
 void launch_prune(hipStream_t s, const Dataset &ds, const PruneLaunch &a) {
     if (a.n_tasks == 0) return;
     FK_CHECK_ARG(a.R <= FK_PRUNE_RMAX, "nsg_R above the supported maximum (256)");
     FK_CHECK_ARG(a.variant >= 0 && a.variant < FK_PRUNE_VARIANTS, "unknown prune variant");
     unsigned grid = (a.n_tasks + FK_PRUNE_WPB - 1) / FK_PRUNE_WPB;
     if (a.xcd_order) grid = (grid + FK_NUM_XCD - 1) / FK_NUM_XCD * FK_NUM_XCD;
+    if (a.variant == 3 && launch_prune_gram(s, ds, a, grid)) return;  // else it runs as variant 0
     switch (a.rule) {
         case FK_RULE_NSG: launch_prune_rule<FK_RULE_NSG>(s, ds, a, grid); break;
         case FK_RULE_TAU: launch_prune_rule<FK_RULE_TAU>(s, ds, a, grid); break;

@@ -249,11 +249,20 @@ struct PruneLaunch {
     unsigned long long *examined = nullptr;  // accumulated examined candidates (tally counter)
     unsigned long long *uncomputed = nullptr;  // counted distances not computed (tau rule, pd <= 3 tau)
     int variant = 0;                  // kernel variant (FK_PRUNE_VARIANTS; same results)
+    bool gram = false;                // variant 3 may use the dataset's Gram preparation
     const float2 *nrm = nullptr;      // exact-form rows: norms for dot-product distances
     const uint8_t *chains8 = nullptr; // exact-form byte rows (chain-major), with nrm
 };
 void launch_prune(hipStream_t s, const Dataset &ds, const PruneLaunch &a);
-#define FK_PRUNE_VARIANTS 3
+#define FK_PRUNE_VARIANTS 4
+// Whether variant v runs code of its own for the launch's shape (else it
+// runs as variant 0).
+bool prune_variant_applies(const Dataset &ds, const PruneLaunch &a, int v);
+// Prune variant 3 (fk_prune_gram.hip): applies to a.gram launches on a
+// Gram-prepared dataset with R <= 64; launch_prune_gram returns false, and
+// launches nothing, where it does not.
+bool prune_gram_applies(const Dataset &ds, const PruneLaunch &a);
+bool launch_prune_gram(hipStream_t s, const Dataset &ds, const PruneLaunch &a, unsigned grid);
 // Adds an order-independent digest of the launch's output lists to *out.
 void launch_prune_digest(hipStream_t s, const PruneLaunch &a, unsigned long long *out);
 void build_chains(hipStream_t s, Dataset &ds);  // fills ds.chains from ds.rows

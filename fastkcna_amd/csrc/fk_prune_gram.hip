@@ -1,0 +1,319 @@
+// Occlusion prune with the occlusion distances in Gram form (prune variant
+// 3, builds with FK_FLAG_GRAM on a prepared dataset, nsg_R <= 64).
+//
+// The decisions and distance counts are those of *_sync_prune /
+// *_reversed_prune (src/kgraph.cpp:1809-1851 and the tau / alpha forms), as
+// in k_prune: candidates are examined in pool order, each checked against
+// the kept list in kept order until the first kept neighbour that has its
+// id or occludes it.  One wavefront per pool takes the candidates 16 at a
+// time and computes, before any decision, the Gram-form distance of every
+// (candidate, kept neighbour) pair and of every pair within the batch as
+// 16 x 16 MFMA tiles (fk_gram.h): on exact-form rows that value IS the
+// canonical FP32 distance; on the bounded forms it brackets it (gram_lower /
+// gram_upper), so a pair is decided by the bracket -- occluded, not occluded
+// -- or, when the bracket straddles the rule's threshold, by the canonical
+// distance computed in FP32 from the rows.  Each candidate's verdicts against
+// the kept list are three 64-bit masks (occluded, undecided, same id); its
+// check walks them in kept order, then the batch's accepts are resolved in
+// candidate order with the in-batch verdicts, exactly as the sequential loop.
+#include "fk_gram.h"
+
+namespace fk {
+
+#define FK_PGR_WPB 4     // wavefronts (pools) per workgroup
+#define FK_PGR_RMAX 64   // kept neighbours per pool (one 64-bit mask per candidate)
+
+template <int RULE, int FMT>
+__global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const float *__restrict__ rows, uint32_t stride,
+                                                                const float *__restrict__ chains, uint32_t dpad,
+                                                                const float2 *__restrict__ nrm, float scale,
+                                                                GramBound gb, PruneLaunch a) {
+    __shared__ uint32_t s_so[FK_PGR_WPB][FK_PGR_RMAX];
+    __shared__ float s_sd[FK_PGR_WPB][FK_PGR_RMAX];
+    __shared__ float2 s_sn[FK_PGR_WPB][FK_PGR_RMAX];
+    __shared__ uint32_t s_cid[FK_PGR_WPB][16];
+    __shared__ float s_cpd[FK_PGR_WPB][16];
+    __shared__ float2 s_cn[FK_PGR_WPB][16];
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint32_t vb = a.xcd_order ? xcd_position((gridDim.x + FK_NUM_XCD - 1) / FK_NUM_XCD) : blockIdx.x;
+    const uint32_t task = vb * FK_PGR_WPB + w;
+    if (task >= a.n_tasks) return;
+    const uint32_t slot = a.task_list ? a.task_list[a.task_base + task] : a.task_base + task;
+    const uint64_t *c = a.cand + (a.cand_off ? a.cand_off[slot] : (uint64_t)slot * a.cand_stride);
+    const uint32_t len = a.cand_len[slot];
+    const uint32_t q = a.qid ? a.qid[slot] : slot;
+    uint32_t *oid = a.out_ids + (uint64_t)slot * a.out_stride;
+    float *od = a.out_d ? a.out_d + (uint64_t)slot * a.out_stride : nullptr;
+    uint32_t *so = s_so[w];
+    float *sd = s_sd[w];
+    float2 *sn = s_sn[w];
+    uint32_t *cid = s_cid[w];
+    float *cpd = s_cpd[w];
+    float2 *cnm = s_cn[w];
+    const uint32_t R = a.R;
+    const uint32_t nb = dpad >> 3, g = lane >> 3, k = lane & 7u;
+
+    uint32_t start = 0;
+    if (len > 0 && key_id(c[0]) == q) start = 1;
+    if (start >= len || R == 0) {
+        if (lane == 0) a.out_len[slot] = 0;
+        return;
+    }
+    const uint32_t limit = len < a.maxc ? len : a.maxc;
+    if (lane == 0) {  // the first candidate is always kept
+        const uint64_t k0 = c[start];
+        so[0] = key_id(k0);
+        sd[0] = key_dist(k0);
+        sn[0] = nrm[key_id(k0)];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    uint32_t nout = 1;
+    bool stop = nout >= R;
+    unsigned long long cc = 0, exam = 0, unc = 0;
+    // the verdict of one (checker, kept) pair from its Gram-form distance:
+    // 1 occluded, 0 not, -1 undecided (bounded forms only)
+    auto verdict = [&](float dt, float2 n1, float2 n2, float pdc, float dqt) __attribute__((always_inline)) -> int {
+        if (gb.exact) return occludes<RULE>(dt, pdc, dqt, a.tau3, a.cos_thr) ? 1 : 0;
+        const float th = RULE == FK_RULE_TAU ? pdc - a.tau3 : pdc;
+        if (gram_lower(dt, n1, n2, gb) >= th) return 0;
+        if (RULE != FK_RULE_ALPHA && gram_upper(dt, n1, n2, gb) < th) return 1;
+        return -1;
+    };
+    // canonical FP32 distance of two rows (chain-major copies), one octet
+    auto exact = [&](uint32_t x, uint32_t y) __attribute__((always_inline)) -> float {
+        return octet_dist2<0>(chains + (size_t)x * dpad + k * nb, chains + (size_t)y * dpad + k * nb, nb);
+    };
+    // the 16-bit field of candidate i (row of the tile) in the ballots of a
+    // tile: lane l holds row 4 (l >> 4) + e, column l & 15
+    auto field = [](const uint64_t (&bl)[4], uint32_t i) __attribute__((always_inline)) -> uint64_t {
+        const uint64_t b = (i & 3u) == 0 ? bl[0] : (i & 3u) == 1 ? bl[1] : (i & 3u) == 2 ? bl[2] : bl[3];
+        return (b >> (16u * (i >> 2))) & 0xFFFFull;
+    };
+
+    for (uint32_t pos = start + 1; !stop && pos < limit; pos += 16) {
+        const uint32_t nbat = (limit - pos) < 16u ? (limit - pos) : 16u;
+        const uint32_t ci = lane & 15u;
+        {
+            const uint64_t key = ci < nbat ? c[pos + ci] : FK_KEY_EMPTY;
+            const uint32_t pid = key_id(key);
+            const bool valid = ci < nbat && pid < a.oracle_n;
+            if (lane < 16) {
+                cid[ci] = valid ? pid : 0xFFFFFFFFu;
+                cpd[ci] = key_dist(key);
+                cnm[ci] = nrm[valid ? pid : 0u];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t n0 = nout;
+        // this lane's A row: candidate ci (row 0 for an invalid slot; its
+        // verdicts are never read)
+        const uint32_t aid = cid[ci] != 0xFFFFFFFFu ? cid[ci] : 0u;
+        const float *arow = rows + (size_t)aid * stride;
+        // verdict masks of this octet's two candidates g and g + 8
+        uint64_t m_occ[2] = {0, 0}, m_und[2] = {0, 0}, m_dup[2] = {0, 0};
+        for (uint32_t kt = 0; kt * 16u < n0; ++kt) {
+            const uint32_t kj = kt * 16u + ci;
+            const float *brow = rows + (size_t)(kj < n0 ? so[kj] : so[0]) * stride;
+            const fk_f32x4 gt = gram_tile<FMT>(arow, brow, dpad, scale);
+            const uint32_t j = kt * 16u + (lane & 15u);
+            const bool jok = j < n0;
+            const uint32_t jj = jok ? j : 0u;
+            const uint32_t jid = so[jj];
+            const float dqt = sd[jj];
+            const float2 kn = sn[jj];
+            uint64_t bo[4], bu[4], bd[4];
+#pragma unroll
+            for (uint32_t e = 0; e < 4; ++e) {
+                const uint32_t i = 4u * (lane >> 4) + e;
+                const uint32_t iid = cid[i];
+                bool o = false, u = false, d = false;
+                if (jok && iid != 0xFFFFFFFFu) {
+                    if (jid == iid) {
+                        d = true;
+                    } else {
+                        const float2 cn = cnm[i];
+                        const float dt = (cn.x + kn.x) - 2.f * (gt[e] * gb.inv_s2);
+                        const int v = verdict(dt, cn, kn, cpd[i], dqt);
+                        o = v == 1;
+                        u = v < 0;
+                    }
+                }
+                bo[e] = wave_ballot(o);
+                bu[e] = wave_ballot(u);
+                bd[e] = wave_ballot(d);
+            }
+#pragma unroll
+            for (uint32_t h = 0; h < 2; ++h) {
+                const uint32_t i = g + 8u * h;
+                m_occ[h] |= field(bo, i) << (16u * kt);
+                m_und[h] |= field(bu, i) << (16u * kt);
+                m_dup[h] |= field(bd, i) << (16u * kt);
+            }
+        }
+        // in-batch verdicts: checker r (tile row) against candidate c (column)
+        uint64_t bo2[4], bu2[4], bd2[4];
+        {
+            const fk_f32x4 gt = gram_tile<FMT>(arow, arow, dpad, scale);
+            const uint32_t cc_ = lane & 15u;
+            const uint32_t cidc = cid[cc_];
+            const float pdc_ = cpd[cc_];
+            const float2 ncc = cnm[cc_];
+#pragma unroll
+            for (uint32_t e = 0; e < 4; ++e) {
+                const uint32_t r = 4u * (lane >> 4) + e;
+                const uint32_t rid = cid[r];
+                bool o = false, u = false, d = false;
+                if (r > cc_ && rid != 0xFFFFFFFFu && cidc != 0xFFFFFFFFu) {
+                    if (rid == cidc) {
+                        d = true;
+                    } else {
+                        const float2 nr = cnm[r];
+                        const float dt = (nr.x + ncc.x) - 2.f * (gt[e] * gb.inv_s2);
+                        const int v = verdict(dt, nr, ncc, cpd[r], pdc_);
+                        o = v == 1;
+                        u = v < 0;
+                    }
+                }
+                bo2[e] = wave_ballot(o);
+                bu2[e] = wave_ballot(u);
+                bd2[e] = wave_ballot(d);
+            }
+        }
+        // each octet walks its two candidates' masks in kept order
+        bool occl[2];
+        uint32_t cnt[2], ncu[2];
+        const uint64_t lowm = n0 >= 64u ? ~0ull : ((1ull << n0) - 1ull);
+#pragma unroll
+        for (uint32_t h = 0; h < 2; ++h) {
+            const uint32_t i = g + 8u * h;
+            const uint32_t iid = cid[i];
+            occl[h] = false;
+            cnt[h] = 0;
+            ncu[h] = 0;
+            if (iid == 0xFFFFFFFFu) continue;  // octet-uniform
+            uint64_t m = (m_occ[h] | m_und[h] | m_dup[h]) & lowm;
+            uint32_t stop_t = n0;
+            while (m) {
+                const uint32_t t = (uint32_t)__builtin_ctzll(m);
+                const uint64_t bit = 1ull << t;
+                if (m_dup[h] & bit) { occl[h] = true; stop_t = t; break; }
+                if (m_occ[h] & bit) { occl[h] = true; stop_t = t + 1; break; }
+                // undecided: the canonical distance
+                ++ncu[h];
+                if (occludes<RULE>(exact(iid, so[t]), cpd[i], sd[t], a.tau3, a.cos_thr)) {
+                    occl[h] = true;
+                    stop_t = t + 1;
+                    break;
+                }
+                m &= m - 1;
+            }
+            cnt[h] = stop_t;  // kept neighbours checked (a same-id stop counts no distance)
+        }
+        // accepts in candidate order; later open candidates are checked
+        // against each newly kept one (its in-batch verdict)
+        uint32_t done = 0, stop_i = 15;
+        while (true) {
+            const uint64_t ob = wave_ballot(k == 0 && cid[g] != 0xFFFFFFFFu && !occl[0]) |
+                                (wave_ballot(k == 0 && cid[g + 8] != 0xFFFFFFFFu && !occl[1]) << 1);
+            // candidate i is open when bit 8 * (i & 7) + (i >> 3) is set
+            uint32_t ai = 0xFFFFFFFFu;
+            for (uint32_t i = done; i < 16; ++i)
+                if ((ob >> (8u * (i & 7u) + (i >> 3))) & 1ull) { ai = i; break; }
+            if (ai == 0xFFFFFFFFu) break;
+            const uint32_t slot_o = nout;
+            if (lane == 0) {
+                so[slot_o] = cid[ai];
+                sd[slot_o] = cpd[ai];
+                sn[slot_o] = cnm[ai];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            ++nout;
+            done = ai + 1;
+            if (nout >= R) { stop = true; stop_i = ai; break; }
+#pragma unroll
+            for (uint32_t h = 0; h < 2; ++h) {
+                const uint32_t r = g + 8u * h;
+                if (r > ai && cid[r] != 0xFFFFFFFFu && !occl[h]) {  // octet-uniform
+                    const uint64_t bit = 1ull << ai;
+                    if (field(bd2, r) & bit) {
+                        occl[h] = true;
+                    } else {
+                        ++cnt[h];
+                        if (field(bo2, r) & bit) {
+                            occl[h] = true;
+                        } else if (field(bu2, r) & bit) {
+                            ++ncu[h];
+                            if (occludes<RULE>(exact(cid[r], cid[ai]), cpd[r], cpd[ai], a.tau3, a.cos_thr))
+                                occl[h] = true;
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (uint32_t h = 0; h < 2; ++h) {
+            const uint32_t i = g + 8u * h;
+            if (k == 0 && i <= stop_i && cid[i] != 0xFFFFFFFFu) {
+                cc += cnt[h];
+                exam += 1;
+                // bounded forms: every counted distance not computed in FP32
+                if (!gb.exact) unc += cnt[h] - ncu[h];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t t = lane; t < nout; t += 64) {
+        oid[t] = so[t];
+        if (od) od[t] = sd[t];
+    }
+    if (lane == 0) a.out_len[slot] = nout;
+    if (a.cc) {
+        const unsigned long long s = wave_sum(cc);
+        if (lane == 0 && s) tally_add(a.cc, s);
+    }
+    if (a.examined) {
+        const unsigned long long s = wave_sum(exam);
+        if (lane == 0 && s) tally_add(a.examined, s);
+    }
+    if (a.uncomputed) {
+        const unsigned long long s = wave_sum(unc);
+        if (lane == 0 && s) tally_add(a.uncomputed, s);
+    }
+}
+
+// Launches variant 3 when it applies (a Gram-prepared dataset and nsg_R <=
+// 64); returns false otherwise.
+bool prune_gram_applies(const Dataset &ds, const PruneLaunch &a) {
+    return a.gram && ds.gram_fmt != FK_GRAM_NONE && a.R <= FK_PGR_RMAX && ds.gram_nrm.p;
+}
+
+bool launch_prune_gram(hipStream_t s, const Dataset &ds, const PruneLaunch &a, unsigned grid) {
+    if (!prune_gram_applies(ds, a)) return false;
+    const GramBound gb = gram_bound(ds);
+    const int fmt = ds.gram_fmt == FK_GRAM_F16 ? FK_GRAM_F16 : FK_GRAM_BF16;
+#define FK_PGR_LAUNCH(RULE_, FMT_)                                                                              \
+    hipLaunchKernelGGL((k_prune_gram<RULE_, FMT_>), dim3(grid), dim3(64 * FK_PGR_WPB), 0, s, ds.rows.p, ds.stride, \
+                       ds.chains.p, ds.dpad, ds.gram_nrm.p, ds.gram_scale, gb, a)
+    if (fmt == FK_GRAM_F16) {
+        switch (a.rule) {
+            case FK_RULE_NSG: FK_PGR_LAUNCH(FK_RULE_NSG, FK_GRAM_F16); break;
+            case FK_RULE_TAU: FK_PGR_LAUNCH(FK_RULE_TAU, FK_GRAM_F16); break;
+            default: FK_PGR_LAUNCH(FK_RULE_ALPHA, FK_GRAM_F16); break;
+        }
+    } else {
+        switch (a.rule) {
+            case FK_RULE_NSG: FK_PGR_LAUNCH(FK_RULE_NSG, FK_GRAM_BF16); break;
+            case FK_RULE_TAU: FK_PGR_LAUNCH(FK_RULE_TAU, FK_GRAM_BF16); break;
+            default: FK_PGR_LAUNCH(FK_RULE_ALPHA, FK_GRAM_BF16); break;
+        }
+    }
+#undef FK_PGR_LAUNCH
+    FK_LAUNCH_CHECK();
+    return true;
+}
+
+}  // namespace fk

@@ -95,11 +95,17 @@ typedef enum {
  * to 64 kept neighbours (8 KiB) as half-precision rows and decides a check from a
  * rigorous bound on the distance to them, computing it from the FP32 rows
  * only when the bound cannot decide (those checks are counted in
- * d_prune_uncomputed).  These flags force one (no timing); fk_prune and
- * fk_prune_noalloc use variant 0 unless FK_FLAG_PRUNE_V1 or V2 is set. */
+ * d_prune_uncomputed); variant 3 (with FK_FLAG_GRAM, nsg_R <= 64) computes
+ * every (candidate, kept neighbour) distance of a batch of 16 candidates as
+ * Gram-form MFMA tiles: exact on exact-form rows, otherwise a rigorous
+ * bracket decides and undecided pairs are computed in FP32.  These flags
+ * force one (no timing; a variant that does not apply runs as 0); fk_prune
+ * and fk_prune_noalloc use variant 0 unless FK_FLAG_PRUNE_V1, V2 or V3 is
+ * set. */
 #define FK_FLAG_PRUNE_V0 0x100u
 #define FK_FLAG_PRUNE_V1 0x200u
 #define FK_FLAG_PRUNE_V2 0x400u
+#define FK_FLAG_PRUNE_V3 0x800u
 
 /* Gram operand formats (fk_stats.gram_mode, fk_gram_pair_distances). */
 enum {

@@ -582,22 +582,26 @@ struct Builder {
         return cc;
     }
 
-    void inter_insert(unsigned n, vector<std::mutex> &locks) {
-        for (size_t i = 0; i < nh[n].nn_nsg.size(); ++i) {
+    // InterInsert (src/kgraph.cpp:2051-2084) for node n: the reverse edges
+    // (des, n) it adds, gathered without touching any list.  The reference
+    // appends n to des's list, under des's lock, unless that list already
+    // holds n; the list it scans is des's kept list plus reverse edges other
+    // nodes appended, none of which is n (a node appends itself to a list at
+    // most once: its kept list has distinct ids), so the check reduces to the
+    // kept list.  Its loop also runs over the reverse edges appended to n's
+    // own list meanwhile; each of those, m, has n in its kept list (that is
+    // why m was appended), so they add nothing.  Gathering first and
+    // appending afterwards gives the same edges without the reference's
+    // unsynchronised read of a list other threads append to.
+    void inter_insert(unsigned n, vector<std::pair<unsigned, Nb>> &add) const {
+        for (unsigned i = 0; i < kept_len[n]; ++i) {
             const size_t des = nh[n].nn_nsg[i].id;
             if (des >= N()) continue;
             bool dup = false;
-            {
-                std::lock_guard<std::mutex> g(locks[des]);
-                for (auto &e : nh[des].nn_nsg)
-                    if (e.id == n) { dup = true; break; }
-            }
+            for (unsigned k = 0; k < kept_len[des]; ++k)
+                if (nh[des].nn_nsg[k].id == n) { dup = true; break; }
             if (dup) continue;
-            Nb nn(n, o.dist(n, (unsigned)des), true);
-            {
-                std::lock_guard<std::mutex> g(locks[des]);
-                nh[des].nn_nsg.push_back(nn);
-            }
+            add.emplace_back((unsigned)des, Nb(n, o.dist(n, (unsigned)des), true));
         }
     }
 
@@ -635,15 +639,31 @@ struct Builder {
             st->prune_time[li] = t1 - t0;
         }
         st->t_prune += t1 - t0;
-        vector<std::mutex> locks(N_);
         uint64_t inter_d = 0;
         kept_len.resize(N_);
         for (unsigned n = 0; n < N_; ++n) {
             kept_len[n] = (unsigned)nh[n].nn_nsg.size();
             inter_d += nh[n].nn_nsg.size();
         }
+        {
+            // reverse edges gathered in parallel and appended; a list left
+            // within nsg_R has them in (dist, id) order, the build's
+            // canonical order (the reference's is its threads' order), an
+            // over-full one is sorted whole by reversed_prune
+            vector<vector<std::pair<unsigned, Nb>>> add(N_);
 #pragma omp parallel for schedule(dynamic, 100)
-        for (unsigned n = 0; n < N_; ++n) inter_insert(n, locks);
+            for (unsigned n = 0; n < N_; ++n) inter_insert(n, add[n]);
+            for (unsigned n = 0; n < N_; ++n)
+                for (auto &e : add[n]) nh[e.first].nn_nsg.push_back(e.second);
+#pragma omp parallel for schedule(dynamic, 100)
+            for (unsigned n = 0; n < N_; ++n) {
+                auto &out = nh[n].nn_nsg;
+                if (out.size() <= range)
+                    std::sort(out.begin() + kept_len[n], out.end(), [](const Nb &x, const Nb &y) {
+                        return x.dist < y.dist || (x.dist == y.dist && x.id < y.id);
+                    });
+            }
+        }
         std::fill(pruneAgain.begin(), pruneAgain.end(), 0);
         uint64_t rc = 0;
 #pragma omp parallel for schedule(dynamic, 100) reduction(+ : rc)
@@ -1273,9 +1293,9 @@ static void load_pools(Builder &b, const fk_state *s, unsigned cap) {
     }
 }
 
-// Link step: the full rule for every node, InterInsert, reversed prune.  A
-// list left at or below nsg_R has its appended reverse edges (appended in
-// thread order) put in (dist, id) order, the build's canonical order.
+// Link step: the full rule for every node, InterInsert, reversed prune (a
+// list left at or below nsg_R has its appended reverse edges in (dist, id)
+// order, the build's canonical order).
 int fko_step_link(int type, const float *rows, size_t stride_bytes, unsigned n_total, unsigned d,
                   const fk_params *p, fk_state *s, fk_stats *st_out) {
     Rows r;
@@ -1293,10 +1313,6 @@ int fko_step_link(int type, const float *rows, size_t stride_bytes, unsigned n_t
     b.link(0);
     for (unsigned i = 0; i < s->n; ++i) {
         auto &out = b.nh[i].nn_nsg;
-        if (!b.pruneAgain[i])
-            std::sort(out.begin() + b.kept_len[i], out.end(), [](const Nb &x, const Nb &y) {
-                return x.dist < y.dist || (x.dist == y.dist && x.id < y.id);
-            });
         if (out.size() > s->out_cap) return 1;
         s->out_len[i] = (uint32_t)out.size();
         for (size_t k = 0; k < out.size(); ++k) {

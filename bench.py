@@ -349,16 +349,19 @@ def batching_leg(dev, n_sets):
     [2,000, 200,000], d in {96, 128, 256}, unit-normalised 16-centre
     mixtures; alpha-PG, K=L=100 S=10 R=50 I=6 nsg_R=32), grouped by width:
     one fk_build_batch per width against one fk_build per dataset (uploads
-    outside both timings); the graphs are compared."""
-    from fastkcna_amd import FK_PG_ALPHAPG
+    outside both timings), and the batch again with FK_FLAG_GRAM (Gram-form
+    prune); the graphs are compared."""
+    from fastkcna_amd import FK_PG_ALPHAPG, _ffi
     from fastkcna_amd.datasets import c4_sets, params_c4
     from fastkcna_amd.kgraph import IndexParams
     t = time.perf_counter()
     sets = [x for x, _ in c4_sets(n_sets=n_sets, n_queries=0)]
     log(f"batching: {n_sets} sets, {sum(len(x) for x in sets):,} rows generated in {time.perf_counter() - t:.1f}s")
     p = params_c4(IndexParams)
-    t_batch = t_single = 0.0
-    same = True
+    pg = p.copy(flags=_ffi.FK_FLAG_GRAM)
+    t_batch = t_single = t_gram = 0.0
+    same = same_gram = True
+    gram_variants = {}
     for d in sorted({x.shape[1] for x in sets}):
         group = [x for x in sets if x.shape[1] == d]
         rows = np.concatenate(group)
@@ -369,6 +372,14 @@ def batching_leg(dev, n_sets):
         t = time.perf_counter()
         got, _ = dev.build_batch(FK_PG_ALPHAPG, p, off)
         t_batch += time.perf_counter() - t
+        dev.build_batch(FK_PG_ALPHAPG, pg, off)  # warm-up (prune variants measured here)
+        t = time.perf_counter()
+        ggot, gst = dev.build_batch(FK_PG_ALPHAPG, pg, off)
+        t_gram += time.perf_counter() - t
+        gram_variants[int(d)] = [int(gst["prune_variant"]) - 1, int(gst["prune_rev_variant"]) - 1]
+        same_gram = same_gram and all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+                                      for a, b in zip(got, ggot))
+        del ggot
         for k, x in enumerate(group):
             dev.upload(x)
             t = time.perf_counter()
@@ -376,7 +387,10 @@ def batching_leg(dev, n_sets):
             t_single += time.perf_counter() - t
             same = same and np.array_equal(o, got[k][0]) and np.array_equal(i, got[k][1]) and ep == got[k][2]
     out = {"sets": n_sets, "rows": int(sum(len(x) for x in sets)), "batched_sec": t_batch,
-           "one_by_one_sec": t_single, "speedup": t_single / t_batch, "graphs_identical": bool(same)}
+           "one_by_one_sec": t_single, "speedup": t_single / t_batch, "graphs_identical": bool(same),
+           "gram": {"batched_sec": t_gram, "speedup_vs_batched": t_batch / t_gram,
+                    "speedup_vs_one_by_one": t_single / t_gram, "graphs_identical": bool(same_gram),
+                    "prune_variants_by_width": gram_variants}}
     log(f"batching: {out}")
     return out
 
@@ -464,7 +478,8 @@ def c3_leg(dev):
 def c5_leg(dev):
     """C5: the HNSW level-0 layer (nsg_R 64) of 10,000,000 x 96 unit-normalised
     rows (4,096 centres, sigma 0.25, seed 4), K = L = 200 S = 10 R = 100 I = 6
-    search_K = 200, built plain and with the learned screen."""
+    search_K = 200, built plain, with FK_FLAG_GRAM (Gram-form prune) and with
+    the learned screen."""
     from fastkcna_amd import FK_PG_HNSW_LAYER, _ffi
     from fastkcna_amd.datasets import c5, params_c5
     from fastkcna_amd.kgraph import IndexParams
@@ -478,7 +493,11 @@ def c5_leg(dev):
     plain = time.perf_counter() - t
     gt, _ = dev.knn(qs, 10)
     rec = recall_at_10(dev.search(off, ids, ep, qs, 40, 10)[0], gt)
-    del off, ids
+    t = time.perf_counter()
+    goff, gids, gep, _, gst = dev.build(FK_PG_HNSW_LAYER, p.copy(flags=_ffi.FK_FLAG_GRAM | _ffi.FK_FLAG_PROFILE))
+    gram = time.perf_counter() - t
+    gsame = bool(np.array_equal(off, goff) and np.array_equal(ids, gids) and ep == gep)
+    del off, ids, goff, gids
     t = time.perf_counter()
     _, _, _, _, sst = dev.build(FK_PG_HNSW_LAYER, p.copy(flags=_ffi.FK_FLAG_SCREEN))
     scr = time.perf_counter() - t
@@ -493,6 +512,13 @@ def c5_leg(dev):
                                                             "t_bv", "t_total")},
            "prune_variant": int(st["prune_variant"]) - 1, "prune_rev_variant": int(st["prune_rev_variant"]) - 1,
            "tuning_sec": float(st["t_tune"]),
+           "gram": {"build_sec": gram, "speedup_vs_plain": plain / gram, "graph_identical_to_plain": gsame,
+                    "gram_mode": _ffi.GRAM_MODE_NAMES[int(gst["gram_mode"])],
+                    "prune_variant": int(gst["prune_variant"]) - 1,
+                    "prune_rev_variant": int(gst["prune_rev_variant"]) - 1, "tuning_sec": float(gst["t_tune"]),
+                    "d_prune_uncomputed": int(gst["d_prune_uncomputed"]),
+                    "families_ms": {f: round(gst["prof_ms"][i], 1) for i, f in enumerate(_ffi.PROF_FAMILIES)
+                                    if gst["prof_launches"][i]}},
            "learned_screen": {"build_sec": scr, "speedup_vs_plain": plain / scr,
                               "knn_recall_at_maxl": knn_recall_entries(sst),
                               "screened_pairs": int(sst["d_screened"]), "skipped_pairs": int(sst["d_screen_skip"]),

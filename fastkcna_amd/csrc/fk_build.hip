@@ -334,14 +334,15 @@ __global__ void k_pack_csr(const uint32_t *__restrict__ nsg_id, const uint32_t *
 
 // The prune of one link step with the kernel variant for its shape: forced
 // by a flag, or taken from the handle, or -- on the shape's first use --
-// measured.  Measurement runs the variants on a sample (the first sixteenth
-// of the tasks, a contiguous run of the locality order; variant 0 twice so
-// neither is timed on cold caches), requires the sample's out-lists
-// (order-independent digest) and distance counts to agree, keeps the faster
-// variant on the handle and runs it on the remaining tasks: the launch's
-// outputs and tallies are those of a single launch, at about a fifth more
-// work.  Small launches are measured whole, twice each.  Variants that do
-// not apply to the shape run as variant 0 and are not timed.  Returns
+// measured.  Measurement runs every variant twice on a sample (the first
+// sixteenth of the tasks, at most 131,072 of them, a contiguous run of the
+// locality order), takes each one's faster run (none is judged on caches
+// another one warmed), requires the sample's out-lists (order-independent
+// digest) and distance counts to agree, keeps the fastest variant on the
+// handle and runs it on the remaining tasks: the launch's outputs and
+// tallies are those of a single launch (the sample's from its last run,
+// variant 0's).  Small launches are measured whole.  Variants that do not
+// apply to the shape run as variant 0 and are not timed.  Returns
 // 1 + variant.
 static uint32_t tuned_prune(BuildState &b, PruneLaunch &a, bool reversed) {
     const uint32_t flags = b.P.flags;
@@ -360,23 +361,24 @@ static uint32_t tuned_prune(BuildState &b, PruneLaunch &a, bool reversed) {
     }
     const double t0 = now_sec();
     const uint32_t n = a.n_tasks;
-    const uint32_t m = n >= 65536 ? n / 16 : n;  // sample
+    const uint32_t m = n >= 65536 ? std::min<uint32_t>(n / 16, 131072u) : n;  // sample
     PruneLaunch smp = a;
     smp.n_tasks = m;
     hipEvent_t e0, e1;
     FK_HIP(hipEventCreate(&e0));
     FK_HIP(hipEventCreate(&e1));
     float best_ms[FK_PRUNE_VARIANTS];
+    for (int v = 0; v < FK_PRUNE_VARIANTS; ++v) best_ms[v] = INFINITY;
     unsigned long long ref[2] = {0, 0};
+    bool have_ref = false;
     DevBuf<unsigned long long> dg(1);
     try {
-        // variants 0, 1, ..., then 0 again
-        for (int r = 0; r <= FK_PRUNE_VARIANTS; ++r) {
+        // every variant twice (variants 3, 2, 1, 0, then again), each timed
+        // by its faster run, so none is judged on caches another one warmed;
+        // the last run is variant 0's
+        for (int r = 2 * FK_PRUNE_VARIANTS - 1; r >= 0; --r) {
             const int v = r % FK_PRUNE_VARIANTS;
-            if (!prune_variant_applies(*b.ds, smp, v)) {  // it would run as variant 0
-                best_ms[v] = INFINITY;
-                continue;
-            }
+            if (!prune_variant_applies(*b.ds, smp, v)) continue;  // it would run as variant 0
             b.tally.clear(b.s, FK_T_PRUNE_CC, 2);
             b.tally.clear(b.s, FK_T_PRUNE_SKIP);
             smp.variant = v;
@@ -390,13 +392,14 @@ static uint32_t tuned_prune(BuildState &b, PruneLaunch &a, bool reversed) {
             got[1] = b.tally.read1(b.s, FK_T_PRUNE_CC);  // synchronises the stream
             float ms = 0.f;
             FK_HIP(hipEventElapsedTime(&ms, e0, e1));
-            if (r == 0) {
+            if (!have_ref) {
                 ref[0] = got[0];
                 ref[1] = got[1];
+                have_ref = true;
             } else if (got[0] != ref[0] || got[1] != ref[1]) {
                 throw Error(FK_E_INTERNAL, "prune variants disagree (variant " + std::to_string(v) + ")");
             }
-            if (r < FK_PRUNE_VARIANTS || ms < best_ms[v]) best_ms[v] = ms;
+            best_ms[v] = std::min(best_ms[v], ms);
         }
     } catch (...) {
         hipEventDestroy(e0);

@@ -701,7 +701,7 @@ void launch_prune(hipStream_t s, const Dataset &ds, const PruneLaunch &a) {
     FK_CHECK_ARG(a.variant >= 0 && a.variant < FK_PRUNE_VARIANTS, "unknown prune variant");
     unsigned grid = (a.n_tasks + FK_PRUNE_WPB - 1) / FK_PRUNE_WPB;
     if (a.xcd_order) grid = (grid + FK_NUM_XCD - 1) / FK_NUM_XCD * FK_NUM_XCD;
-    if (a.variant == 3 && launch_prune_gram(s, ds, a, grid)) return;  // else it runs as variant 0
+    if (a.variant == 3 && launch_prune_gram(s, ds, a)) return;  // else it runs as variant 0
     switch (a.rule) {
         case FK_RULE_NSG: launch_prune_rule<FK_RULE_NSG>(s, ds, a, grid); break;
         case FK_RULE_TAU: launch_prune_rule<FK_RULE_TAU>(s, ds, a, grid); break;

@@ -89,6 +89,18 @@ __device__ __forceinline__ fk_f32x4 gram_tile(const float *__restrict__ arow, co
     return acc;
 }
 
+// One K-step of a 16 x 16 x 32 tile from operands already in the 16-bit
+// format (the values mfma_step converts to), as raw bits.
+template <int FMT>
+__device__ __forceinline__ fk_f32x4 mfma_step16(fk_s16x8 a, fk_s16x8 b, fk_f32x4 acc) {
+    if constexpr (FMT == FK_GRAM_F16)
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(fk_f16x8, a), __builtin_bit_cast(fk_f16x8, b),
+                                                      acc, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(fk_bf16x8, a),
+                                                       __builtin_bit_cast(fk_bf16x8, b), acc, 0, 0, 0);
+}
+
 // Error model of one prepared dataset (see the file comment).
 struct GramBound {
     int exact;        // d~ is the canonical distance

@@ -101,6 +101,24 @@ __global__ void k_norms8(const uint8_t *__restrict__ c8, uint32_t n, uint32_t dp
     out[i] = s;
 }
 
+// rows16 = the 16-bit operands of every row, dpad wide: FP16 of the
+// prescaled value, or BF16 rounded to nearest even (the values mfma_step
+// makes from the FP32 row).  One wave per row.
+template <int FMT>
+__global__ __launch_bounds__(256) void k_rows16(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
+                                                uint32_t n, float scale, uint16_t *__restrict__ out) {
+    const uint32_t lane = lane_id();
+    for (uint32_t i = blockIdx.x * 4u + (threadIdx.x >> 6); i < n; i += gridDim.x * 4u) {
+        const float *r = rows + (size_t)i * stride;
+        uint16_t *o = out + (size_t)i * dpad;
+        for (uint32_t k = lane; k < dpad; k += 64) {
+            const float v = r[k];
+            if constexpr (FMT == FK_GRAM_F16) o[k] = __builtin_bit_cast(uint16_t, (_Float16)(v * scale));
+            else o[k] = (uint16_t)bf16_bits(v);
+        }
+    }
+}
+
 __global__ void k_to_bytes(const float *__restrict__ src, size_t n, uint8_t *__restrict__ dst) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = (uint8_t)src[i];
@@ -163,6 +181,17 @@ void gram_prepare(hipStream_t s, Dataset &ds, uint32_t flags) {
     else
         hipLaunchKernelGGL(k_gram_norms<FK_GRAM_BF16>, dim3(g2), dim3(256), 0, s, ds.rows.p, ds.stride, ds.dpad,
                            ds.n_total, scale, inv_s2, ds.gram_nrm.p);
+    FK_LAUNCH_CHECK();
+    // the 16-bit operand copy (the Gram-form prune's tiles; exact forms as
+    // BF16, which holds their integers exactly)
+    ds.rows16.alloc((size_t)ds.n_total * ds.dpad);
+    const unsigned g16 = std::min<unsigned>(65536, (ds.n_total + 3) / 4);
+    if (fmt == FK_GRAM_F16)
+        hipLaunchKernelGGL(k_rows16<FK_GRAM_F16>, dim3(g16), dim3(256), 0, s, ds.rows.p, ds.stride, ds.dpad,
+                           ds.n_total, scale, ds.rows16.p);
+    else
+        hipLaunchKernelGGL(k_rows16<FK_GRAM_BF16>, dim3(g16), dim3(256), 0, s, ds.rows.p, ds.stride, ds.dpad,
+                           ds.n_total, scale, ds.rows16.p);
     FK_LAUNCH_CHECK();
     // exact form with values in [0, 255]: a chain-major byte copy for the
     // octet kernels (dot products in 32-bit integers, exact)

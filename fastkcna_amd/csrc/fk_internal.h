@@ -193,6 +193,7 @@ struct Dataset {
     DevBuf<float2> gram_nrm;
     DevBuf<uint8_t> chains8;  // exact form with values in [0, 255]: chain-major byte copy
     DevBuf<int32_t> norms8;   // with chains8: sum over dpad of (byte - 128)^2 per row
+    DevBuf<uint16_t> rows16;  // n_total x dpad 16-bit Gram operands (FP16 of the prescaled value, else BF16)
     int gram_fmt = 0;
     float gram_scale = 1.f;
     uint32_t gram_req = 0;
@@ -262,7 +263,15 @@ bool prune_variant_applies(const Dataset &ds, const PruneLaunch &a, int v);
 // Gram-prepared dataset with R <= 64; launch_prune_gram returns false, and
 // launches nothing, where it does not.
 bool prune_gram_applies(const Dataset &ds, const PruneLaunch &a);
-bool launch_prune_gram(hipStream_t s, const Dataset &ds, const PruneLaunch &a, unsigned grid);
+bool launch_prune_gram(hipStream_t s, const Dataset &ds, const PruneLaunch &a);
+// Whether a build's local join takes the Gram form: FK_FLAG_GRAM on exact-
+// form rows, or with a bounded form forced by FK_FLAG_GRAM_BF16 / _F16.  On
+// the bounded forms a pair that enters a pool needs its canonical FP32
+// distance (its key) anyway and a third to a half of the joined pairs do, so
+// the plain join is the faster one unless a bounded form is asked for.
+inline bool join_in_gram_form(uint32_t flags, const Dataset &ds) {
+    return (flags & FK_FLAG_GRAM) && (ds.gram_fmt == FK_GRAM_EXACT || (flags & (FK_FLAG_GRAM_BF16 | FK_FLAG_GRAM_F16)));
+}
 // Adds an order-independent digest of the launch's output lists to *out.
 void launch_prune_digest(hipStream_t s, const PruneLaunch &a, unsigned long long *out);
 void build_chains(hipStream_t s, Dataset &ds);  // fills ds.chains from ds.rows

@@ -1109,7 +1109,7 @@ static void nnd_join(BuildState &b) {
     }
     const uint32_t *ord = b.has_order ? b.order.p : nullptr;
     const double d4 = 4.0 * b.ds->d;
-    const bool gram = (b.P.flags & FK_FLAG_GRAM) != 0;
+    const bool gram = join_in_gram_form(b.P.flags, *b.ds);
     if (gram) b.tally.clear(b.s, FK_T_RECHECK);
     const bool screen = b.scr.ready && join_screen_lds(tcap) <= FK_JOIN_SCREEN_LDS_MAX;
     if (screen) screen_calibrate(b);

@@ -1,5 +1,6 @@
 // Occlusion prune with the occlusion distances in Gram form (prune variant
-// 3, builds with FK_FLAG_GRAM on a prepared dataset, nsg_R <= 64).
+// 3, builds with FK_FLAG_GRAM on a prepared dataset, nsg_R <= 64, width <=
+// 256).
 //
 // The decisions and distance counts are those of *_sync_prune /
 // *_reversed_prune (src/kgraph.cpp:1809-1851 and the tau / alpha forms), as
@@ -16,18 +17,29 @@
 // the kept list are three 64-bit masks (occluded, undecided, same id); its
 // check walks them in kept order, then the batch's accepts are resolved in
 // candidate order with the in-batch verdicts, exactly as the sequential loop.
+//
+// Operands come from the dataset's 16-bit copy (rows16).  A batch's 16
+// candidate rows are loaded once into registers as MFMA A fragments (lane l:
+// row l & 15, values 32 s + 8 (l >> 4) .. +7 of K-step s) and serve as both
+// operands of the in-batch tile; the kept tiles read their B fragments from
+// rows16 (L2-resident for the pools of a workgroup's neighbourhood).  A cache
+// of the kept rows in LDS was slower on every shape measured: R rows per
+// wave leave two or three waves per SIMD where registers allow four.
 #include "fk_gram.h"
 
 namespace fk {
 
-#define FK_PGR_WPB 4     // wavefronts (pools) per workgroup
-#define FK_PGR_RMAX 64   // kept neighbours per pool (one 64-bit mask per candidate)
+#define FK_PGR_WPB 4    // wavefronts (pools) per workgroup
+#define FK_PGR_RMAX 64  // kept neighbours per pool (one 64-bit mask per candidate)
+#define FK_PGR_NSMAX 8  // K-steps of 32 values: dpad <= 256
 
-template <int RULE, int FMT>
-__global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const float *__restrict__ rows, uint32_t stride,
+// NS: K-steps of the tiles (ceil(dpad / 32), rounded up to 2, 3, 4 or 8;
+// the values past dpad are zero).
+template <int RULE, int FMT, int NS>
+__global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const uint16_t *__restrict__ rows16,
                                                                 const float *__restrict__ chains, uint32_t dpad,
-                                                                const float2 *__restrict__ nrm, float scale,
-                                                                GramBound gb, PruneLaunch a) {
+                                                                const float2 *__restrict__ nrm, GramBound gb,
+                                                                PruneLaunch a) {
     __shared__ uint32_t s_so[FK_PGR_WPB][FK_PGR_RMAX];
     __shared__ float s_sd[FK_PGR_WPB][FK_PGR_RMAX];
     __shared__ float2 s_sn[FK_PGR_WPB][FK_PGR_RMAX];
@@ -52,6 +64,13 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const float *__r
     float2 *cnm = s_cn[w];
     const uint32_t R = a.R;
     const uint32_t nb = dpad >> 3, g = lane >> 3, k = lane & 7u;
+    const uint32_t q8 = (lane >> 4) * 8u;
+    // fragment s of this lane's quarter of a 16-bit row (zero past dpad)
+    auto frag = [&](const uint16_t *r, uint32_t s) __attribute__((always_inline)) -> fk_s16x8 {
+        const uint32_t kk = s * 32u + q8;
+        if (kk < dpad) return *reinterpret_cast<const fk_s16x8 *>(r + kk);
+        return fk_s16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    };
 
     uint32_t start = 0;
     if (len > 0 && key_id(c[0]) == q) start = 1;
@@ -60,11 +79,14 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const float *__r
         return;
     }
     const uint32_t limit = len < a.maxc ? len : a.maxc;
-    if (lane == 0) {  // the first candidate is always kept
+    {  // the first candidate is always kept
         const uint64_t k0 = c[start];
-        so[0] = key_id(k0);
-        sd[0] = key_dist(k0);
-        sn[0] = nrm[key_id(k0)];
+        const uint32_t id0 = key_id(k0);  // kept whatever its id, with its row (as k_prune)
+        if (lane == 0) {
+            so[0] = id0;
+            sd[0] = key_dist(k0);
+            sn[0] = nrm[id0];
+        }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -107,16 +129,28 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const float *__r
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         const uint32_t n0 = nout;
-        // this lane's A row: candidate ci (row 0 for an invalid slot; its
-        // verdicts are never read)
-        const uint32_t aid = cid[ci] != 0xFFFFFFFFu ? cid[ci] : 0u;
-        const float *arow = rows + (size_t)aid * stride;
+        // this lane's A fragments: candidate ci (row 0 for an invalid slot;
+        // its verdicts are never read)
+        fk_s16x8 af[NS];
+        {
+            const uint32_t aid = cid[ci] != 0xFFFFFFFFu ? cid[ci] : 0u;
+            const uint16_t *ar = rows16 + (size_t)aid * dpad;
+#pragma unroll
+            for (uint32_t s = 0; s < NS; ++s) af[s] = frag(ar, s);
+        }
         // verdict masks of this octet's two candidates g and g + 8
         uint64_t m_occ[2] = {0, 0}, m_und[2] = {0, 0}, m_dup[2] = {0, 0};
         for (uint32_t kt = 0; kt * 16u < n0; ++kt) {
             const uint32_t kj = kt * 16u + ci;
-            const float *brow = rows + (size_t)(kj < n0 ? so[kj] : so[0]) * stride;
-            const fk_f32x4 gt = gram_tile<FMT>(arow, brow, dpad, scale);
+            fk_f32x4 gt = {0.f, 0.f, 0.f, 0.f};
+            {
+                const uint16_t *br = rows16 + (size_t)so[kj < n0 ? kj : 0u] * dpad;
+                fk_s16x8 bf[NS];
+#pragma unroll
+                for (uint32_t s = 0; s < NS; ++s) bf[s] = frag(br, s);
+#pragma unroll
+                for (uint32_t s = 0; s < NS; ++s) gt = mfma_step16<FMT>(af[s], bf[s], gt);
+            }
             const uint32_t j = kt * 16u + (lane & 15u);
             const bool jok = j < n0;
             const uint32_t jj = jok ? j : 0u;
@@ -155,7 +189,9 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const float *__r
         // in-batch verdicts: checker r (tile row) against candidate c (column)
         uint64_t bo2[4], bu2[4], bd2[4];
         {
-            const fk_f32x4 gt = gram_tile<FMT>(arow, arow, dpad, scale);
+            fk_f32x4 gt = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (uint32_t s = 0; s < NS; ++s) gt = mfma_step16<FMT>(af[s], af[s], gt);
             const uint32_t cc_ = lane & 15u;
             const uint32_t cidc = cid[cc_];
             const float pdc_ = cpd[cc_];
@@ -285,33 +321,57 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const float *__r
     }
 }
 
-// Launches variant 3 when it applies (a Gram-prepared dataset and nsg_R <=
-// 64); returns false otherwise.
-bool prune_gram_applies(const Dataset &ds, const PruneLaunch &a) {
-    return a.gram && ds.gram_fmt != FK_GRAM_NONE && a.R <= FK_PGR_RMAX && ds.gram_nrm.p;
+// K-steps the kernel is instantiated for, for rows of dpad values (0: none).
+static uint32_t pgr_steps(uint32_t dpad) {
+    const uint32_t ns = (dpad + 31u) / 32u;
+    return ns <= 2 ? 2u : ns <= 4 ? ns : ns <= FK_PGR_NSMAX ? 8u : 0u;
 }
 
-bool launch_prune_gram(hipStream_t s, const Dataset &ds, const PruneLaunch &a, unsigned grid) {
+// Variant 3 applies to a.gram launches on a dataset with its Gram form and
+// 16-bit copy prepared, nsg_R <= 64 and dpad <= 256.
+bool prune_gram_applies(const Dataset &ds, const PruneLaunch &a) {
+    return a.gram && ds.gram_fmt != FK_GRAM_NONE && a.R <= FK_PGR_RMAX && ds.gram_nrm.p && ds.rows16.p &&
+           pgr_steps(ds.dpad) != 0;
+}
+
+template <int FMT, int NS>
+static void launch_pgr(hipStream_t s, const Dataset &ds, const PruneLaunch &a, const GramBound &gb, unsigned grid) {
+    switch (a.rule) {
+        case FK_RULE_NSG:
+            hipLaunchKernelGGL((k_prune_gram<FK_RULE_NSG, FMT, NS>), dim3(grid), dim3(64 * FK_PGR_WPB), 0, s,
+                               ds.rows16.p, ds.chains.p, ds.dpad, ds.gram_nrm.p, gb, a);
+            break;
+        case FK_RULE_TAU:
+            hipLaunchKernelGGL((k_prune_gram<FK_RULE_TAU, FMT, NS>), dim3(grid), dim3(64 * FK_PGR_WPB), 0, s,
+                               ds.rows16.p, ds.chains.p, ds.dpad, ds.gram_nrm.p, gb, a);
+            break;
+        default:
+            hipLaunchKernelGGL((k_prune_gram<FK_RULE_ALPHA, FMT, NS>), dim3(grid), dim3(64 * FK_PGR_WPB), 0, s,
+                               ds.rows16.p, ds.chains.p, ds.dpad, ds.gram_nrm.p, gb, a);
+            break;
+    }
+}
+
+template <int FMT>
+static void launch_pgr_fmt(hipStream_t s, const Dataset &ds, const PruneLaunch &a, const GramBound &gb,
+                           unsigned grid) {
+    switch (pgr_steps(ds.dpad)) {
+        case 2: launch_pgr<FMT, 2>(s, ds, a, gb, grid); break;
+        case 3: launch_pgr<FMT, 3>(s, ds, a, gb, grid); break;
+        case 4: launch_pgr<FMT, 4>(s, ds, a, gb, grid); break;
+        default: launch_pgr<FMT, 8>(s, ds, a, gb, grid); break;
+    }
+}
+
+// Launches variant 3 when it applies; returns false, launching nothing,
+// otherwise.
+bool launch_prune_gram(hipStream_t s, const Dataset &ds, const PruneLaunch &a) {
     if (!prune_gram_applies(ds, a)) return false;
     const GramBound gb = gram_bound(ds);
-    const int fmt = ds.gram_fmt == FK_GRAM_F16 ? FK_GRAM_F16 : FK_GRAM_BF16;
-#define FK_PGR_LAUNCH(RULE_, FMT_)                                                                              \
-    hipLaunchKernelGGL((k_prune_gram<RULE_, FMT_>), dim3(grid), dim3(64 * FK_PGR_WPB), 0, s, ds.rows.p, ds.stride, \
-                       ds.chains.p, ds.dpad, ds.gram_nrm.p, ds.gram_scale, gb, a)
-    if (fmt == FK_GRAM_F16) {
-        switch (a.rule) {
-            case FK_RULE_NSG: FK_PGR_LAUNCH(FK_RULE_NSG, FK_GRAM_F16); break;
-            case FK_RULE_TAU: FK_PGR_LAUNCH(FK_RULE_TAU, FK_GRAM_F16); break;
-            default: FK_PGR_LAUNCH(FK_RULE_ALPHA, FK_GRAM_F16); break;
-        }
-    } else {
-        switch (a.rule) {
-            case FK_RULE_NSG: FK_PGR_LAUNCH(FK_RULE_NSG, FK_GRAM_BF16); break;
-            case FK_RULE_TAU: FK_PGR_LAUNCH(FK_RULE_TAU, FK_GRAM_BF16); break;
-            default: FK_PGR_LAUNCH(FK_RULE_ALPHA, FK_GRAM_BF16); break;
-        }
-    }
-#undef FK_PGR_LAUNCH
+    unsigned grid = (a.n_tasks + FK_PGR_WPB - 1) / FK_PGR_WPB;
+    if (a.xcd_order) grid = (grid + FK_NUM_XCD - 1) / FK_NUM_XCD * FK_NUM_XCD;
+    if (ds.gram_fmt == FK_GRAM_F16) launch_pgr_fmt<FK_GRAM_F16>(s, ds, a, gb, grid);
+    else launch_pgr_fmt<FK_GRAM_BF16>(s, ds, a, gb, grid);
     FK_LAUNCH_CHECK();
     return true;
 }

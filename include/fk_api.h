@@ -62,13 +62,16 @@ typedef enum {
 #define FK_FLAG_VERBOSE 0x1u     /* per-iteration log lines on stderr            */
 #define FK_FLAG_NO_CONTROLS 0x2u /* skip the sampled ground truth (recall = 0)   */
 #define FK_FLAG_PROFILE 0x4u     /* time kernel families with HIP events         */
-/* Local join in Gram form on the MFMA units (16-bit operands, FP32
- * accumulation and norms), every pair that could enter a pool recomputed in
- * exact FP32: the pools equal the plain FP32 join's.  The operand format is
- * picked from the data (fk_stats.gram_mode) unless forced.  On exact-form
- * rows (FK_GRAM_EXACT) the prunes and BridgeView also take their distances
- * as |a|^2 + |b|^2 - 2 a.b, which is the FP32 distance bit for bit there
- * (fk_prune honours the flag too). */
+/* Gram-form distances on the MFMA units (16-bit operands, FP32 accumulation
+ * and norms); the graph equals the plain FP32 build's.  The operand format is
+ * picked from the data (fk_stats.gram_mode) unless forced.  Local join: in
+ * Gram form on exact-form rows (FK_GRAM_EXACT), and on the bounded forms only
+ * when FK_FLAG_GRAM_BF16 / _F16 force one (every pair that could enter a
+ * pool recomputed in exact FP32 -- on float rows a third to a half of the
+ * pairs, which makes the plain join the faster one there).  Prunes: prune
+ * variant 3 (below) becomes available.  On exact-form rows the prunes and
+ * BridgeView also take their distances as |a|^2 + |b|^2 - 2 a.b, which is
+ * the FP32 distance bit for bit there (fk_prune honours the flag too). */
 #define FK_FLAG_GRAM 0x8u
 #define FK_FLAG_GRAM_BF16 0x10u  /* force the bounded BF16 form                  */
 #define FK_FLAG_GRAM_F16 0x20u   /* force the bounded FP16 form                  */
@@ -95,13 +98,13 @@ typedef enum {
  * to 64 kept neighbours (8 KiB) as half-precision rows and decides a check from a
  * rigorous bound on the distance to them, computing it from the FP32 rows
  * only when the bound cannot decide (those checks are counted in
- * d_prune_uncomputed); variant 3 (with FK_FLAG_GRAM, nsg_R <= 64) computes
- * every (candidate, kept neighbour) distance of a batch of 16 candidates as
- * Gram-form MFMA tiles: exact on exact-form rows, otherwise a rigorous
- * bracket decides and undecided pairs are computed in FP32.  These flags
- * force one (no timing; a variant that does not apply runs as 0); fk_prune
- * and fk_prune_noalloc use variant 0 unless FK_FLAG_PRUNE_V1, V2 or V3 is
- * set. */
+ * d_prune_uncomputed); variant 3 (with FK_FLAG_GRAM, nsg_R <= 64, width <=
+ * 256) computes every (candidate, kept neighbour) distance of a batch of 16
+ * candidates as Gram-form MFMA tiles over the dataset's 16-bit operand copy:
+ * exact on exact-form rows, otherwise a rigorous bracket decides and
+ * undecided pairs are computed in FP32.  These flags force one (no timing; a
+ * variant that does not apply runs as 0); fk_prune and fk_prune_noalloc use
+ * variant 0 unless FK_FLAG_PRUNE_V1, V2 or V3 is set. */
 #define FK_FLAG_PRUNE_V0 0x100u
 #define FK_FLAG_PRUNE_V1 0x200u
 #define FK_FLAG_PRUNE_V2 0x400u

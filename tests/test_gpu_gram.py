@@ -103,8 +103,8 @@ def nnd_params(**kw):
 @pytest.mark.parametrize("name,flags", [
     ("int128", GRAM),           # exact form
     ("int128", GRAM | BF16),    # bounded BF16 on the same data
-    ("float960", GRAM),         # bounded BF16
-    ("unit96", GRAM),           # bounded FP16
+    ("float960", GRAM | BF16),  # bounded BF16
+    ("unit96", GRAM | F16),     # bounded FP16
     ("odd100", GRAM | F16),     # bounded FP16, dpad % 16 == 8
 ])
 def test_gram_nndescent_pools_identical(device, name, flags):
@@ -126,6 +126,13 @@ def test_gram_nndescent_pools_identical(device, name, flags):
         assert 0 < st1["d_recheck"] <= st1["d_gram"]
     say(f"  {name}: mode {_ffi.GRAM_MODE_NAMES[st1['gram_mode']]}, {st1['d_gram']} Gram pairs, "
         f"{st1['d_recheck']} rechecked ({st1['d_recheck'] / max(st1['d_gram'], 1):.3f})")
+    if flags != GRAM:
+        # the format picked from the data: the Gram join only on exact-form
+        # rows, the plain join (same pools) on the bounded forms
+        p.flags = GRAM
+        pools2, len2, st2 = device.nndescent(p)
+        assert np.array_equal(len0, len2) and np.array_equal(pools0, pools2)
+        assert (st2["d_gram"] > 0) == (st2["gram_mode"] == _ffi.FK_GRAM_EXACT)
 
 
 @pytest.mark.parametrize("S,R,K", [(20, 150, 100), (40, 300, 180)])
@@ -153,7 +160,7 @@ def test_gram_full_build_identical(device, name, pg):
     device.upload(x)
     p = nnd_params(tau=0.01)
     off0, ids0, ep0, w0, st0 = device.build(pg, p)
-    p.flags = GRAM
+    p.flags = GRAM | (0 if name == "int128" else BF16)
     off1, ids1, ep1, w1, st1 = device.build(pg, p)
     assert ep0 == ep1 and w0 == w1
     assert np.array_equal(off0, off1) and np.array_equal(ids0, ids1)

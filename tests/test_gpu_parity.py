@@ -98,8 +98,8 @@ def make_pools(x, n_tasks, L, rng, q=None):
 
 # prune kernel variants (the build picks one per shape by timing; variant 2
 # applies to FP32 rows of width 96, 128 and 256, variant 3 -- Gram-form
-# occlusion distances on MFMA -- to Gram-prepared datasets, and each runs as
-# 0 elsewhere)
+# occlusion distances on MFMA -- to Gram-prepared datasets of width <= 256,
+# and each runs as 0 elsewhere)
 VARIANT_FLAGS = {0: _ffi.FK_FLAG_PRUNE_V0, 1: _ffi.FK_FLAG_PRUNE_V1, 2: _ffi.FK_FLAG_PRUNE_V2,
                  3: _ffi.FK_FLAG_PRUNE_V3 | _ffi.FK_FLAG_GRAM}
 
@@ -259,7 +259,8 @@ def test_prune_variants_give_the_same_build(device, pg, name):
     assert np.array_equal(off, out[0][0]) and np.array_equal(ids, out[0][1]) and ep == out[0][2]
     for k in ("d_prune", "d_prune_rev", "d_bv", "aod"):
         assert st[k] == out[0][3][k], k
-    assert [o[3]["prune_variant"] for o in out[:3]] == [1, 2, 3] and st["prune_variant"] == 4
+    assert st["prune_variant"] == 4
+    assert [o[3]["prune_variant"] for o in out[:3]] == [1, 2, 3]
     assert out[3][3]["prune_variant"] in (1, 2, 3)
     say("variants", pg, name, "picked", out[3][3]["prune_variant"] - 1, "tuning s", out[3][3]["t_tune"],
         "uncomputed (variant 2)", out[2][3]["d_prune_uncomputed"], "of", out[2][3]["d_prune"] + out[2][3]["d_prune_rev"])

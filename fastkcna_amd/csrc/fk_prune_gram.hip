@@ -32,6 +32,12 @@ namespace fk {
 #define FK_PGR_WPB 4    // wavefronts (pools) per workgroup
 #define FK_PGR_RMAX 64  // kept neighbours per pool (one 64-bit mask per candidate)
 #define FK_PGR_NSMAX 8  // K-steps of 32 values: dpad <= 256
+// 1 +- 2^-20 and 1 +- 2^-18 (exact in FP32): the margins of the square-root-
+// free bracket tests (verdict in k_prune_gram)
+#define FK_PGR_UP20 1.00000095367431640625f
+#define FK_PGR_DN20 0.99999904632568359375f
+#define FK_PGR_UP18 1.000003814697265625f
+#define FK_PGR_DN18 0.999996185302734375f
 
 // NS: K-steps of the tiles (ceil(dpad / 32), rounded up to 2, 3, 4 or 8;
 // the values past dpad are zero).
@@ -45,7 +51,7 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const uint16_t *
     __shared__ float2 s_sn[FK_PGR_WPB][FK_PGR_RMAX];
     __shared__ uint32_t s_cid[FK_PGR_WPB][16];
     __shared__ float s_cpd[FK_PGR_WPB][16];
-    __shared__ float2 s_cn[FK_PGR_WPB][16];
+    __shared__ float4 s_cb[FK_PGR_WPB][16];  // per candidate: T, U, |x~|^2, |x| (verdict)
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint32_t vb = a.xcd_order ? xcd_position((gridDim.x + FK_NUM_XCD - 1) / FK_NUM_XCD) : blockIdx.x;
     const uint32_t task = vb * FK_PGR_WPB + w;
@@ -61,7 +67,7 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const uint16_t *
     float2 *sn = s_sn[w];
     uint32_t *cid = s_cid[w];
     float *cpd = s_cpd[w];
-    float2 *cnm = s_cn[w];
+    float4 *cb = s_cb[w];
     const uint32_t R = a.R;
     const uint32_t nb = dpad >> 3, g = lane >> 3, k = lane & 7u;
     const uint32_t q8 = (lane >> 4) * 8u;
@@ -93,14 +99,31 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const uint16_t *
     uint32_t nout = 1;
     bool stop = nout >= R;
     unsigned long long cc = 0, exam = 0, unc = 0;
-    // the verdict of one (checker, kept) pair from its Gram-form distance:
-    // 1 occluded, 0 not, -1 undecided (bounded forms only)
-    auto verdict = [&](float dt, float2 n1, float2 n2, float pdc, float dqt) __attribute__((always_inline)) -> int {
+    // the verdict of one (checked candidate, kept) pair from its Gram-form
+    // distance dt: 1 occluded, 0 not, -1 undecided (bounded forms only).
+    // cbi = the checked candidate's (T, U, |x~|^2, |x|), T and U the square
+    // roots of its rule threshold th (pdc, or pdc - 3 tau) over (1 - gam)
+    // and (1 + gam).  The bracket's tests gram_lower(dt) >= th and
+    // gram_upper(dt) < th are taken without a square root per pair: with
+    // f, e the error terms of gram_lower, sqrt(max(dt - f, 0)) - e >= T iff
+    // dt - f >= (e + T)^2, and sqrt(dt + f) + e < U iff U > e and
+    // dt + f < (U - e)^2.  T is rounded up and U down by 2^-20 when made,
+    // e up by 2^-20 here, and each compared side moved by 2^-18, more than
+    // the rounding of these few FP32 operations, so either test implies the
+    // original one; th <= 0 cannot be met by a distance (not occluded).
+    auto verdict = [&](float dt, float4 cbi, float2 kn, float pdc, float dqt) __attribute__((always_inline)) -> int {
         if (gb.exact) return occludes<RULE>(dt, pdc, dqt, a.tau3, a.cos_thr) ? 1 : 0;
         const float th = RULE == FK_RULE_TAU ? pdc - a.tau3 : pdc;
-        if (gram_lower(dt, n1, n2, gb) >= th) return 0;
-        if (RULE != FK_RULE_ALPHA && gram_upper(dt, n1, n2, gb) < th) return 1;
-        return -1;
+        const float f = gb.f_rel * (cbi.z + kn.x);
+        const float e = (gb.u * (cbi.w + kn.y) + gb.e_abs) * FK_PGR_UP20;
+        const float lo = cbi.x + e;
+        const bool not_occl = !(th > 0.f) || (dt - f) * FK_PGR_DN18 >= lo * lo * FK_PGR_UP18;
+        bool occl = false;
+        if (RULE != FK_RULE_ALPHA) {
+            const float hi = cbi.y - e;
+            occl = hi > 0.f && (dt + f) * FK_PGR_UP18 < hi * hi * FK_PGR_DN18;
+        }
+        return not_occl ? 0 : occl ? 1 : -1;
     };
     // canonical FP32 distance of two rows (chain-major copies), one octet
     auto exact = [&](uint32_t x, uint32_t y) __attribute__((always_inline)) -> float {
@@ -121,9 +144,14 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const uint16_t *
             const uint32_t pid = key_id(key);
             const bool valid = ci < nbat && pid < a.oracle_n;
             if (lane < 16) {
+                const float pd = key_dist(key);
+                const float2 n = nrm[valid ? pid : 0u];
+                const float th = RULE == FK_RULE_TAU ? pd - a.tau3 : pd;
+                const float tp = th > 0.f ? th : 0.f;
                 cid[ci] = valid ? pid : 0xFFFFFFFFu;
-                cpd[ci] = key_dist(key);
-                cnm[ci] = nrm[valid ? pid : 0u];
+                cpd[ci] = pd;
+                cb[ci] = make_float4(__builtin_sqrtf(tp / (1.f - gb.gam)) * FK_PGR_UP20,
+                                     __builtin_sqrtf(tp / (1.f + gb.gam)) * FK_PGR_DN20, n.x, n.y);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -162,18 +190,13 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const uint16_t *
             for (uint32_t e = 0; e < 4; ++e) {
                 const uint32_t i = 4u * (lane >> 4) + e;
                 const uint32_t iid = cid[i];
-                bool o = false, u = false, d = false;
-                if (jok && iid != 0xFFFFFFFFu) {
-                    if (jid == iid) {
-                        d = true;
-                    } else {
-                        const float2 cn = cnm[i];
-                        const float dt = (cn.x + kn.x) - 2.f * (gt[e] * gb.inv_s2);
-                        const int v = verdict(dt, cn, kn, cpd[i], dqt);
-                        o = v == 1;
-                        u = v < 0;
-                    }
-                }
+                const float4 cbi = cb[i];
+                const float dt = (cbi.z + kn.x) - 2.f * (gt[e] * gb.inv_s2);
+                const int v = verdict(dt, cbi, kn, cpd[i], dqt);
+                const bool ok = jok && iid != 0xFFFFFFFFu;
+                const bool d = ok && jid == iid;
+                const bool o = ok && !d && v == 1;
+                const bool u = ok && !d && v < 0;
                 bo[e] = wave_ballot(o);
                 bu[e] = wave_ballot(u);
                 bd[e] = wave_ballot(d);
@@ -195,23 +218,19 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const uint16_t *
             const uint32_t cc_ = lane & 15u;
             const uint32_t cidc = cid[cc_];
             const float pdc_ = cpd[cc_];
-            const float2 ncc = cnm[cc_];
+            const float4 cbc = cb[cc_];
+            const float2 ncc = make_float2(cbc.z, cbc.w);
 #pragma unroll
             for (uint32_t e = 0; e < 4; ++e) {
                 const uint32_t r = 4u * (lane >> 4) + e;
                 const uint32_t rid = cid[r];
-                bool o = false, u = false, d = false;
-                if (r > cc_ && rid != 0xFFFFFFFFu && cidc != 0xFFFFFFFFu) {
-                    if (rid == cidc) {
-                        d = true;
-                    } else {
-                        const float2 nr = cnm[r];
-                        const float dt = (nr.x + ncc.x) - 2.f * (gt[e] * gb.inv_s2);
-                        const int v = verdict(dt, nr, ncc, cpd[r], pdc_);
-                        o = v == 1;
-                        u = v < 0;
-                    }
-                }
+                const float4 cbr = cb[r];
+                const float dt = (cbr.z + ncc.x) - 2.f * (gt[e] * gb.inv_s2);
+                const int v = verdict(dt, cbr, ncc, cpd[r], pdc_);
+                const bool ok = r > cc_ && rid != 0xFFFFFFFFu && cidc != 0xFFFFFFFFu;
+                const bool d = ok && rid == cidc;
+                const bool o = ok && !d && v == 1;
+                const bool u = ok && !d && v < 0;
                 bo2[e] = wave_ballot(o);
                 bu2[e] = wave_ballot(u);
                 bd2[e] = wave_ballot(d);
@@ -262,7 +281,7 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const uint16_t *
             if (lane == 0) {
                 so[slot_o] = cid[ai];
                 sd[slot_o] = cpd[ai];
-                sn[slot_o] = cnm[ai];
+                sn[slot_o] = make_float2(cb[ai].z, cb[ai].w);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();

@@ -129,11 +129,18 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const uint16_t *
     auto exact = [&](uint32_t x, uint32_t y) __attribute__((always_inline)) -> float {
         return octet_dist2<0>(chains + (size_t)x * dpad + k * nb, chains + (size_t)y * dpad + k * nb, nb);
     };
-    // the 16-bit field of candidate i (row of the tile) in the ballots of a
-    // tile: lane l holds row 4 (l >> 4) + e, column l & 15
-    auto field = [](const uint64_t (&bl)[4], uint32_t i) __attribute__((always_inline)) -> uint64_t {
-        const uint64_t b = (i & 3u) == 0 ? bl[0] : (i & 3u) == 1 ? bl[1] : (i & 3u) == 2 ? bl[2] : bl[3];
-        return (b >> (16u * (i >> 2))) & 0xFFFFull;
+    // The 16-bit fields of this octet's candidates g and g + 8 (rows of the
+    // tile) in the ballots of a tile (lane l holds row 4 (l >> 4) + e, column
+    // l & 15): both in ballot g & 3, at bits 16 (g >> 2) and 16 (g >> 2) + 32.
+    // The ballot is picked with masks, not a branch per octet.
+    const uint64_t sel1 = 0ull - (uint64_t)(g & 1u), sel2 = 0ull - (uint64_t)((g >> 1) & 1u);
+    const uint32_t fsh = 16u * (g >> 2);
+    auto fields = [&](const uint64_t (&bl)[4], uint64_t &f0, uint64_t &f1) __attribute__((always_inline)) {
+        const uint64_t b01 = bl[0] ^ ((bl[0] ^ bl[1]) & sel1);
+        const uint64_t b23 = bl[2] ^ ((bl[2] ^ bl[3]) & sel1);
+        const uint64_t b = (b01 ^ ((b01 ^ b23) & sel2)) >> fsh;
+        f0 = b & 0xFFFFull;
+        f1 = (b >> 32) & 0xFFFFull;
     };
 
     for (uint32_t pos = start + 1; !stop && pos < limit; pos += 16) {
@@ -201,13 +208,16 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const uint16_t *
                 bu[e] = wave_ballot(u);
                 bd[e] = wave_ballot(d);
             }
-#pragma unroll
-            for (uint32_t h = 0; h < 2; ++h) {
-                const uint32_t i = g + 8u * h;
-                m_occ[h] |= field(bo, i) << (16u * kt);
-                m_und[h] |= field(bu, i) << (16u * kt);
-                m_dup[h] |= field(bd, i) << (16u * kt);
-            }
+            uint64_t f0, f1;
+            fields(bo, f0, f1);
+            m_occ[0] |= f0 << (16u * kt);
+            m_occ[1] |= f1 << (16u * kt);
+            fields(bu, f0, f1);
+            m_und[0] |= f0 << (16u * kt);
+            m_und[1] |= f1 << (16u * kt);
+            fields(bd, f0, f1);
+            m_dup[0] |= f0 << (16u * kt);
+            m_dup[1] |= f1 << (16u * kt);
         }
         // in-batch verdicts: checker r (tile row) against candidate c (column)
         uint64_t bo2[4], bu2[4], bd2[4];
@@ -268,6 +278,10 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const uint16_t *
         }
         // accepts in candidate order; later open candidates are checked
         // against each newly kept one (its in-batch verdict)
+        uint64_t i_dup[2], i_occ[2], i_und[2];  // in-batch verdicts of candidates g, g + 8
+        fields(bd2, i_dup[0], i_dup[1]);
+        fields(bo2, i_occ[0], i_occ[1]);
+        fields(bu2, i_und[0], i_und[1]);
         uint32_t done = 0, stop_i = 15;
         while (true) {
             const uint64_t ob = wave_ballot(k == 0 && cid[g] != 0xFFFFFFFFu && !occl[0]) |
@@ -293,13 +307,13 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const uint16_t *
                 const uint32_t r = g + 8u * h;
                 if (r > ai && cid[r] != 0xFFFFFFFFu && !occl[h]) {  // octet-uniform
                     const uint64_t bit = 1ull << ai;
-                    if (field(bd2, r) & bit) {
+                    if (i_dup[h] & bit) {
                         occl[h] = true;
                     } else {
                         ++cnt[h];
-                        if (field(bo2, r) & bit) {
+                        if (i_occ[h] & bit) {
                             occl[h] = true;
-                        } else if (field(bu2, r) & bit) {
+                        } else if (i_und[h] & bit) {
                             ++ncu[h];
                             if (occludes<RULE>(exact(cid[r], cid[ai]), cpd[r], cpd[ai], a.tau3, a.cos_thr))
                                 occl[h] = true;

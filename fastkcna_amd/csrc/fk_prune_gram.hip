@@ -175,17 +175,25 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const uint16_t *
         }
         // verdict masks of this octet's two candidates g and g + 8
         uint64_t m_occ[2] = {0, 0}, m_und[2] = {0, 0}, m_dup[2] = {0, 0};
-        for (uint32_t kt = 0; kt * 16u < n0; ++kt) {
+        // B fragments of kept tile kt (row kt * 16 + ci of the kept list, row
+        // 0 past its end); for narrow rows the next tile's are loaded while
+        // this one's verdicts are made (where the registers for them would
+        // not cost a wave per SIMD)
+        constexpr bool PF = NS <= 3 || (NS == 4 && RULE != FK_RULE_ALPHA);
+        fk_s16x8 bf[NS];
+        auto load_b = [&](uint32_t kt) __attribute__((always_inline)) {
             const uint32_t kj = kt * 16u + ci;
+            const uint16_t *br = rows16 + (size_t)so[kj < n0 ? kj : 0u] * dpad;
+#pragma unroll
+            for (uint32_t s = 0; s < NS; ++s) bf[s] = frag(br, s);
+        };
+        if (PF) load_b(0);
+        for (uint32_t kt = 0; kt * 16u < n0; ++kt) {
             fk_f32x4 gt = {0.f, 0.f, 0.f, 0.f};
-            {
-                const uint16_t *br = rows16 + (size_t)so[kj < n0 ? kj : 0u] * dpad;
-                fk_s16x8 bf[NS];
+            if (!PF) load_b(kt);
 #pragma unroll
-                for (uint32_t s = 0; s < NS; ++s) bf[s] = frag(br, s);
-#pragma unroll
-                for (uint32_t s = 0; s < NS; ++s) gt = mfma_step16<FMT>(af[s], bf[s], gt);
-            }
+            for (uint32_t s = 0; s < NS; ++s) gt = mfma_step16<FMT>(af[s], bf[s], gt);
+            if (PF && (kt + 1u) * 16u < n0) load_b(kt + 1u);
             const uint32_t j = kt * 16u + (lane & 15u);
             const bool jok = j < n0;
             const uint32_t jj = jok ? j : 0u;

@@ -42,7 +42,7 @@ namespace fk {
 // NS: K-steps of the tiles (ceil(dpad / 32), rounded up to 2, 3, 4 or 8;
 // the values past dpad are zero).
 template <int RULE, int FMT, int NS>
-__global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const uint16_t *__restrict__ rows16,
+__global__ __launch_bounds__(64 * FK_PGR_WPB) __attribute__((amdgpu_waves_per_eu(NS <= 4 ? 4 : 3))) void k_prune_gram(const uint16_t *__restrict__ rows16,
                                                                 const float *__restrict__ chains, uint32_t dpad,
                                                                 const float2 *__restrict__ nrm, GramBound gb,
                                                                 PruneLaunch a) {
@@ -143,16 +143,42 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const uint16_t *
         f1 = (b >> 32) & 0xFFFFull;
     };
 
+    // Batches of 16 candidates, software-pipelined: the keys of batch b + 1
+    // are loaded when batch b starts, their norms after its kept tiles and,
+    // for rows of up to 3 K-steps (PFA; not the tau rule), their A fragments
+    // after its in-batch tile (af is free from there on; otherwise they are
+    // loaded when the batch starts, which keeps the kernel within four waves
+    // per SIMD without spills).
+    // Lane l holds candidate ci = l & 15 (row 0 for an empty or out-of-range
+    // slot; its verdicts are never read).
+    const uint32_t ci = lane & 15u;
+    auto key_at = [&](uint32_t p) __attribute__((always_inline)) -> uint64_t {
+        return p < limit ? c[p] : FK_KEY_EMPTY;
+    };
+    auto row_id = [&](uint64_t key) __attribute__((always_inline)) -> uint32_t {
+        const uint32_t id = key_id(key);
+        return id < a.oracle_n ? id : 0u;
+    };
+    fk_s16x8 af[NS];
+    auto load_a = [&](uint64_t key) __attribute__((always_inline)) {
+        const uint16_t *ar = rows16 + (size_t)row_id(key) * dpad;
+#pragma unroll
+        for (uint32_t s = 0; s < NS; ++s) af[s] = frag(ar, s);
+    };
+    constexpr bool PFA = NS <= 3 && RULE != FK_RULE_TAU;
+    uint64_t key_n = key_at(start + 1 + ci);
+    float2 nrm_n = nrm[row_id(key_n)];
+    if (PFA) load_a(key_n);
     for (uint32_t pos = start + 1; !stop && pos < limit; pos += 16) {
-        const uint32_t nbat = (limit - pos) < 16u ? (limit - pos) : 16u;
-        const uint32_t ci = lane & 15u;
+        const uint64_t key = key_n;
+        const float2 n = nrm_n;
+        if (!PFA) load_a(key);
+        key_n = key_at(pos + 16u + ci);
         {
-            const uint64_t key = ci < nbat ? c[pos + ci] : FK_KEY_EMPTY;
             const uint32_t pid = key_id(key);
-            const bool valid = ci < nbat && pid < a.oracle_n;
+            const bool valid = pid < a.oracle_n;
             if (lane < 16) {
                 const float pd = key_dist(key);
-                const float2 n = nrm[valid ? pid : 0u];
                 const float th = RULE == FK_RULE_TAU ? pd - a.tau3 : pd;
                 const float tp = th > 0.f ? th : 0.f;
                 cid[ci] = valid ? pid : 0xFFFFFFFFu;
@@ -164,15 +190,6 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const uint16_t *
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         const uint32_t n0 = nout;
-        // this lane's A fragments: candidate ci (row 0 for an invalid slot;
-        // its verdicts are never read)
-        fk_s16x8 af[NS];
-        {
-            const uint32_t aid = cid[ci] != 0xFFFFFFFFu ? cid[ci] : 0u;
-            const uint16_t *ar = rows16 + (size_t)aid * dpad;
-#pragma unroll
-            for (uint32_t s = 0; s < NS; ++s) af[s] = frag(ar, s);
-        }
         // verdict masks of this octet's two candidates g and g + 8
         uint64_t m_occ[2] = {0, 0}, m_und[2] = {0, 0}, m_dup[2] = {0, 0};
         // B fragments of kept tile kt (row kt * 16 + ci of the kept list, row
@@ -227,12 +244,14 @@ __global__ __launch_bounds__(64 * FK_PGR_WPB) void k_prune_gram(const uint16_t *
             m_dup[0] |= f0 << (16u * kt);
             m_dup[1] |= f1 << (16u * kt);
         }
+        nrm_n = nrm[row_id(key_n)];
         // in-batch verdicts: checker r (tile row) against candidate c (column)
         uint64_t bo2[4], bu2[4], bd2[4];
         {
             fk_f32x4 gt = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (uint32_t s = 0; s < NS; ++s) gt = mfma_step16<FMT>(af[s], af[s], gt);
+            if (PFA) load_a(key_n);
             const uint32_t cc_ = lane & 15u;
             const uint32_t cidc = cid[cc_];
             const float pdc_ = cpd[cc_];

@@ -66,12 +66,13 @@ def graph_digest(off, ids, ep):
 def total_dists(st, computed=False):
     """Distance evaluations of a build.  By default BridgeView pairs are
     counted as the reference counts them (every pair it evaluates); with
-    computed=True only the pairs the GPU kernel actually computed (it skips
-    targets already pooled or computed in the same round), and without the
-    tau-MNG prune distances that cannot decide anything (candidates within
-    3 tau of the owner)."""
+    computed=True only the distances the GPU computed in FP32: BridgeView
+    skips targets already pooled or computed in the same round and, with
+    FK_FLAG_GRAM, targets a 16-bit bound rejects; the prunes skip distances
+    that cannot decide anything (tau-MNG candidates within 3 tau of the owner)
+    or that a bound decides."""
     keys = ("d_init", "d_ctrl", "d_join", "d_prune", "d_prune_rev", "d_inter", "d_ep", "d_conn")
-    bv = int(st.get("d_bv_computed", st["d_bv"])) if computed else int(st["d_bv"])
+    bv = int(st.get("d_bv_computed", st["d_bv"])) - int(st.get("d_bv_rejected", 0)) if computed else int(st["d_bv"])
     skipped = int(st.get("d_prune_uncomputed", 0)) if computed else 0
     return sum(int(st[k]) for k in keys) + bv - skipped
 

@@ -79,7 +79,7 @@ class FkStats(ctypes.Structure):
         ("screen_margin", f32), ("screen_loss", f32), ("screen_train_pairs", u32), ("reserved1", u32),
         ("t_screen", f64), ("screen_calib_pairs", u64), ("screen_cands", u64),
         ("d_bv_computed", u64), ("d_prune_uncomputed", u64),
-        ("prune_variant", u32), ("prune_rev_variant", u32), ("t_tune", f64),
+        ("prune_variant", u32), ("prune_rev_variant", u32), ("t_tune", f64), ("d_bv_rejected", u64),
     ]
 
     def to_dict(self):

@@ -31,7 +31,8 @@ namespace fk {
 #define FK_T_SEARCH 4      // entry-point search: distances
 #define FK_T_RECHECK 5     // Gram join: FP32 rechecks
 #define FK_T_PRUNE_SKIP 6  // prune: distances counted but not computed (tau rule, pd <= 3 tau)
-#define FK_T_SLOTS 7
+#define FK_T_BV_REJ 7      // BridgeView round: targets rejected by the 16-bit bound (no FP32 distance)
+#define FK_T_SLOTS 8
 
 struct BuildParams {
     // effective (shrunk) parameters, src/kgraph.cpp:1288-1334

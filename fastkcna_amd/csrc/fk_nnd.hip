@@ -22,6 +22,7 @@
 
 #include "fk_build.h"
 #include "fk_device.h"
+#include "fk_gram.h"
 #include "fk_join.h"
 #include "fk_sort.h"
 
@@ -1657,16 +1658,80 @@ __device__ __forceinline__ void bvw_load_pool(uint64_t *p, uint8_t *pf, const ui
     }
 }
 
+// Lane k's part of a row's 16-bit Gram operands (values k NB .. k NB + NB -
+// 1 of rows16, as NB / 2 half pairs), for the BridgeView bound pass; NB =
+// 12, 16 or 32 (16-byte pieces, or 8-byte ones at 12).
+template <int NB>
+struct HalfSlice {
+    static constexpr int P = NB / 2;
+    uint32_t v[P];
+    __device__ __forceinline__ void load(const uint16_t *p) {
+        if constexpr (P % 4 == 0) {
+#pragma unroll
+            for (int j = 0; j < P / 4; ++j) {
+                const uint4 q = reinterpret_cast<const uint4 *>(p)[j];
+                v[4 * j] = q.x; v[4 * j + 1] = q.y; v[4 * j + 2] = q.z; v[4 * j + 3] = q.w;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < P / 2; ++j) {
+                const uint2 q = reinterpret_cast<const uint2 *>(p)[j];
+                v[2 * j] = q.x; v[2 * j + 1] = q.y;
+            }
+        }
+    }
+    // this lane's part of the dot product of two FP16 rows (products exact,
+    // FP32 sums)
+    __device__ __forceinline__ float dot(const HalfSlice &o) const {
+        typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+        float s = 0.f;
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+            s = __builtin_amdgcn_fdot2(__builtin_bit_cast(h2, v[j]), __builtin_bit_cast(h2, o.v[j]), s, false);
+        return s;
+    }
+};
+
+// The sum of an octet's 8 values in every lane of the octet (any order: a
+// bound's dot product, not a canonical distance).
+__device__ __forceinline__ float octet_sum_any(float s) {
+    s += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(s), 0xB1, 0xF, 0xF, true));   // quad_perm [1,0,3,2]
+    s += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(s), 0x4E, 0xF, 0xF, true));   // quad_perm [2,3,0,1]
+    s += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(s), 0x141, 0xF, 0xF, true));  // row_half_mirror
+    return s;
+}
+
+#define FK_BVW_INFLIGHT16 4  // rows per octet in flight in the bound pass (2 from width 128 up)
+// 1 +- 2^-20, 1 +- 2^-18 (exact in FP32): margins of the bound pass's test
+#define FK_BVW_UP20 1.00000095367431640625f
+#define FK_BVW_UP18 1.000003814697265625f
+#define FK_BVW_DN18 0.999996185302734375f
+
 // MODE (RowKind): 0 canonical FP32; 1 exact-form rows (fk_gram.hip),
 // distances as |a|^2 + |b|^2 - 2 a.b; 2 the same from byte rows.
-template <int NB, int MODE>
+//
+// Bound pass (MODE 0, NB 12 / 16 / 32, r16 set: FK_FLAG_GRAM on FP16-form
+// rows): when the node's pool is full, every target first gets the 16-bit
+// Gram-form value d~ of its distance (dot product of the FP16 operand rows)
+// and is dropped without an FP32 distance when the rigorous lower bound
+// gram_lower(d~) (the Gram prune's error model) exceeds the pool's last
+// distance thr: its canonical distance is then above thr and the FP32 test
+// dd <= thr would reject it.  The test is taken without a square root,
+// d~ - f > (e + T)^2 with T = sqrt(thr / (1 - gam)) (T and e rounded up by
+// 2^-20, each side moved by 2^-18, more than the rounding of these
+// operations).  The survivors go through the FP32 step unchanged, so the
+// candidates and the merged pools are the same.  G: the kernel with the
+// pass (the others are compiled without it).
+template <int NB, int MODE, bool G>
 __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 4 : (NB == 32 ? 3 : FK_BVW_WPE)))) void k_bv_round(
     const void *__restrict__ chains_v, const float2 *__restrict__ nrm, uint32_t dpad, uint32_t N, uint32_t cap,
     uint32_t W, uint32_t S_cap_all, const uint32_t *__restrict__ set_of, const uint32_t *__restrict__ set_scap,
     uint64_t *__restrict__ pool, uint8_t *__restrict__ pflg, uint32_t *__restrict__ cnt,
     const uint32_t *__restrict__ nsg_id, const uint32_t *__restrict__ nsg_len, const uint8_t *__restrict__ nsg_new,
     uint32_t ncap, unsigned long long *__restrict__ cc_acc, unsigned long long *__restrict__ comp_acc,
-    const uint32_t *__restrict__ order, uint32_t *__restrict__ mark, uint32_t wave_bytes, uint32_t wpb) {
+    const uint32_t *__restrict__ order, uint32_t *__restrict__ mark, uint32_t wave_bytes, uint32_t wpb,
+    const uint16_t *__restrict__ r16, const float2 *__restrict__ gnrm, GramBound gb,
+    unsigned long long *__restrict__ rej_acc) {
     using E = typename RowKind<NB, MODE>::E;
     using Chain = typename RowKind<NB, MODE>::Chain;
     constexpr bool DOT = RowKind<NB, MODE>::dot;
@@ -1776,8 +1841,19 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
     Chain mine;
     mine.load(lane_chains + (size_t)i * dpad, nb);
     const float ni = DOT ? nrm[i].x : 0.f;
+    constexpr bool G16 = G && MODE == 0 && (NB == 12 || NB == 16 || NB == 32);
+    constexpr uint32_t NBH = G16 ? (uint32_t)NB : 4u;
+    const bool bound16 = G16 && r16 != nullptr && thr < INFINITY;  // wave-uniform
+    HalfSlice<NBH> mine16;
+    float2 n16 = make_float2(0.f, 0.f);
+    float t16 = 0.f;
+    if (bound16) {
+        mine16.load(r16 + (size_t)i * dpad + k * NBH);
+        n16 = gnrm[i];
+        t16 = __builtin_sqrtf(thr / (1.f - gb.gam)) * FK_BVW_UP20;
+    }
     uint32_t np = np0, nc = 0, nt = 0;
-    unsigned long long cc = 0, computed = 0;
+    unsigned long long cc = 0, computed = 0, rejected = 0;
     bool merged = false;
     uint32_t first_change = cap;  // smallest pool position rewritten
     // merges the candidates of c into the pool
@@ -1796,6 +1872,43 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
     // octet in flight (slots past m compute a discarded distance to row 0);
     // candidates (distance <= thr) go to c, merged before c can overflow
     auto run_todo = [&](uint32_t m) __attribute__((always_inline)) {
+        if (bound16) {
+            // the bound pass: survivors compacted to todo[0, m2) in order
+            // (a chunk's entries are read before any is written back, and
+            // writes never pass the chunk's end)
+            constexpr uint32_t I16 = NBH >= 16 ? 2u : FK_BVW_INFLIGHT16;
+            uint32_t m2 = 0;
+            for (uint32_t e0 = 0; e0 < m; e0 += 8 * I16) {
+                HalfSlice<NBH> hh[I16];
+                float2 tn2[I16];
+                uint32_t tt[I16];
+#pragma unroll
+                for (uint32_t u = 0; u < I16; ++u) {
+                    const uint32_t e = e0 + u * 8 + g;
+                    tt[u] = e < m ? todo[e] : i;
+                    hh[u].load(r16 + (size_t)tt[u] * dpad + k * NBH);
+                    tn2[u] = gnrm[tt[u]];
+                }
+                wave_lds_sync();
+#pragma unroll
+                for (uint32_t u = 0; u < I16; ++u) {
+                    const float dot = octet_sum_any(mine16.dot(hh[u]));
+                    const float nx = n16.x + tn2[u].x;
+                    const float dt = nx - 2.f * (dot * gb.inv_s2);
+                    const float f = gb.f_rel * nx;
+                    const float e = (gb.u * (n16.y + tn2[u].y) + gb.e_abs) * FK_BVW_UP20;
+                    const float lo = t16 + e;
+                    const bool far = (dt - f) * FK_BVW_DN18 > lo * lo * FK_BVW_UP18;
+                    const bool live = (e0 + u * 8 + g) < m && k == 0;
+                    const uint64_t mk = wave_ballot(live && !far);
+                    if (live && !far) todo[m2 + lanes_below(mk)] = tt[u];
+                    m2 += (uint32_t)__builtin_popcountll(mk);
+                    rejected += (uint32_t)__builtin_popcountll(wave_ballot(live && far));
+                }
+                wave_lds_sync();
+            }
+            m = m2;
+        }
         for (uint32_t e0 = 0; e0 < m; e0 += 8 * INF) {
             if (nc > FK_BVW_CH - 8 * INF) merge_c();
             Chain rr[INF];
@@ -1884,6 +1997,7 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
     }
     if (lane == 0 && cc) tally_add(cc_acc, cc);  // the same counts in every lane
     if (lane == 0 && computed) tally_add(comp_acc, computed);
+    if (lane == 0 && rejected) tally_add(rej_acc, rejected);
 }
 
 static void launch_bv_round(BuildState &b, uint32_t W, uint32_t scap, const uint32_t *set_scap) {
@@ -1898,22 +2012,31 @@ static void launch_bv_round(BuildState &b, uint32_t W, uint32_t scap, const uint
     uint32_t *mark = b.prof ? b.bv_mark.p : nullptr;
     const float2 *nrm = exact_norms(b);
     const uint8_t *c8 = nrm && b.ds->chains8.p ? b.ds->chains8.p : nullptr;
-#define FK_BV_LAUNCH(NBV, MODE)                                                                                  \
-    hipLaunchKernelGGL((k_bv_round<NBV, MODE>), dim3(grid), dim3(64 * wpb), lds, b.s,                          \
+    // the bound pass: FK_FLAG_GRAM with the FP16 form (MODE 0 kernels of
+    // widths 96, 128, 256 take it)
+    const bool g16 = (b.P.flags & FK_FLAG_GRAM) && b.ds->gram_fmt == FK_GRAM_F16 && b.ds->rows16.p;
+    const uint16_t *r16 = g16 ? b.ds->rows16.p : nullptr;
+    const GramBound gb = g16 ? gram_bound(*b.ds) : GramBound{};
+#define FK_BV_LAUNCH_G(NBV, MODE, G)                                                                             \
+    hipLaunchKernelGGL((k_bv_round<NBV, MODE, G>), dim3(grid), dim3(64 * wpb), lds, b.s,                       \
                        MODE == 2 ? (const void *)c8 : (const void *)b.ds->chains.p, nrm,                        \
                        b.ds->dpad, b.N, b.cap, W, scap, b.dev_set_of(), set_scap, b.pool.p, b.pflg.p, b.cnt.p,  \
                        b.nsg_id.p, b.nsg_len.p,                                                                 \
                        b.nsg_new.p, b.ncap, b.tally.at(FK_T_BV_CC), b.tally.at(FK_T_BV_COMP), ord, mark,       \
-                       wave_bytes, wpb)
+                       wave_bytes, wpb, r16, b.ds->gram_nrm.p, gb, b.tally.at(FK_T_BV_REJ))
+#define FK_BV_LAUNCH(NBV, MODE) FK_BV_LAUNCH_G(NBV, MODE, false)
+#define FK_BV_LAUNCH16(NBV) do { if (r16) FK_BV_LAUNCH_G(NBV, 0, true); else FK_BV_LAUNCH_G(NBV, 0, false); } while (0)
     switch (b.ds->dpad) {
         case 32: FK_BV_LAUNCH(4, 0); break;
         case 64: FK_BV_LAUNCH(8, 0); break;
-        case 96: if (nrm) FK_BV_LAUNCH(12, 1); else FK_BV_LAUNCH(12, 0); break;
-        case 128: if (c8) FK_BV_LAUNCH(16, 2); else if (nrm) FK_BV_LAUNCH(16, 1); else FK_BV_LAUNCH(16, 0); break;
-        case 256: if (c8) FK_BV_LAUNCH(32, 2); else if (nrm) FK_BV_LAUNCH(32, 1); else FK_BV_LAUNCH(32, 0); break;
+        case 96: if (nrm) FK_BV_LAUNCH(12, 1); else FK_BV_LAUNCH16(12); break;
+        case 128: if (c8) FK_BV_LAUNCH(16, 2); else if (nrm) FK_BV_LAUNCH(16, 1); else FK_BV_LAUNCH16(16); break;
+        case 256: if (c8) FK_BV_LAUNCH(32, 2); else if (nrm) FK_BV_LAUNCH(32, 1); else FK_BV_LAUNCH16(32); break;
         default: FK_BV_LAUNCH(0, 0); break;
     }
+#undef FK_BV_LAUNCH16
 #undef FK_BV_LAUNCH
+#undef FK_BV_LAUNCH_G
 }
 
 void bridgeview(BuildState &b) {
@@ -1960,6 +2083,7 @@ void bridgeview(BuildState &b) {
         if (b.n_sets > 1)
             FK_HIP(hipMemcpyAsync(d_scap.p, scap_set.data(), (size_t)b.n_sets * 4, hipMemcpyHostToDevice, b.s));
         b.tally.clear(b.s, FK_T_BV_CC, 2);  // and FK_T_BV_COMP
+        b.tally.clear(b.s, FK_T_BV_REJ);
         if (b.prof && b.bv_mark.n < N) {
             b.bv_mark.alloc(N);
             FK_HIP(hipMemsetAsync(b.bv_mark.p, 0, (size_t)N * 4, b.s));
@@ -1972,6 +2096,7 @@ void bridgeview(BuildState &b) {
         b.tally.read(b.s, FK_T_BV_CC, 2, cc2);
         const unsigned long long cc = cc2[0];
         b.st->d_bv_computed += cc2[1];
+        b.st->d_bv_rejected += b.tally.read1(b.s, FK_T_BV_REJ);
         if (b.prof) {
             // minimal traffic of a round: the rows (+ ids) of every expansion
             // (|rnn_q|) and of every out-list entry of an expanded q

@@ -209,6 +209,11 @@ typedef struct {
      * build (a shape's first use in a handle). */
     uint32_t prune_variant, prune_rev_variant;
     double t_tune;
+    /* FK_FLAG_GRAM on FP16-form rows of width 96, 128 or 256: BridgeView
+     * targets of d_bv_computed that a rigorous bound on the 16-bit Gram-form
+     * distance showed to be beyond the pool's last distance, so no FP32
+     * distance was computed for them (they cannot enter the pool). */
+    uint64_t d_bv_rejected;
 } fk_stats;
 
 /* Output graph in CSR form, allocated by the library (fk_graph_free). */

@@ -167,6 +167,31 @@ def test_gram_full_build_identical(device, name, pg):
     assert st1["d_gram"] > 0
 
 
+@pytest.mark.parametrize("d", [96, 128, 256])
+@pytest.mark.parametrize("pg", [FK_PG_HNSW_LAYER, FK_PG_ALPHAPG])
+def test_gram_bound_pass_build_identical(device, d, pg):
+    """FP16-form rows of widths 96, 128 and 256 with FK_FLAG_GRAM: BridgeView
+    drops targets whose 16-bit bound puts them beyond the pool's last
+    distance, and the Gram-form prune may run; the graph, the distance
+    counts and the BridgeView pairs evaluated equal the plain build's, and
+    the bound must have dropped some targets."""
+    x = unit_mixture(5000, d, 16, seed=26 + d, sigma=0.3)
+    say("gram bound pass", d, pg)
+    device.upload(x)
+    p = nnd_params(K=40, L=40, S=10, R=40, iterations=4, nsg_R=24, search_L=40, search_K=60, angle=60)
+    off0, ids0, ep0, w0, st0 = device.build(pg, p)
+    p.flags = GRAM
+    off1, ids1, ep1, w1, st1 = device.build(pg, p)
+    assert st1["gram_mode"] == _ffi.FK_GRAM_F16
+    assert ep0 == ep1 and w0 == w1
+    assert np.array_equal(off0, off1) and np.array_equal(ids0, ids1)
+    for k in ("d_join", "d_prune", "d_prune_rev", "d_bv", "d_bv_computed", "d_ep"):
+        assert st1[k] == st0[k], k
+    assert st0["d_bv_rejected"] == 0 and 0 < st1["d_bv_rejected"] < st1["d_bv_computed"]
+    say(f"  rejected {st1['d_bv_rejected']} of {st1['d_bv_computed']} computed BridgeView pairs; "
+        f"prune variant {st1['prune_variant'] - 1}")
+
+
 def exact_rows(kind):
     """Exact-form rows of the three octet row kinds: bytes (values in
     [0, 255], d = 128: byte chains), signed (d = 128: FP32 dot form) and

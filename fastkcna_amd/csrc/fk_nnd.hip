@@ -1701,7 +1701,12 @@ __device__ __forceinline__ float octet_sum_any(float s) {
     return s;
 }
 
-#define FK_BVW_INFLIGHT16 4  // rows per octet in flight in the bound pass (2 from width 128 up)
+#ifndef FK_BVW_INFLIGHT16
+#define FK_BVW_INFLIGHT16 4  // rows per octet in flight in the bound pass, width 96
+#endif
+#ifndef FK_BVW_INFLIGHT16W
+#define FK_BVW_INFLIGHT16W 2  // the same from width 128 up (registers)
+#endif
 // 1 +- 2^-20, 1 +- 2^-18 (exact in FP32): margins of the bound pass's test
 #define FK_BVW_UP20 1.00000095367431640625f
 #define FK_BVW_UP18 1.000003814697265625f
@@ -1876,7 +1881,7 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
             // the bound pass: survivors compacted to todo[0, m2) in order
             // (a chunk's entries are read before any is written back, and
             // writes never pass the chunk's end)
-            constexpr uint32_t I16 = NBH >= 16 ? 2u : FK_BVW_INFLIGHT16;
+            constexpr uint32_t I16 = NBH >= 16 ? FK_BVW_INFLIGHT16W : FK_BVW_INFLIGHT16;
             uint32_t m2 = 0;
             for (uint32_t e0 = 0; e0 < m; e0 += 8 * I16) {
                 HalfSlice<NBH> hh[I16];

@@ -2,7 +2,9 @@
 area, no device allocation) against fk_prune through the C ABI: identical
 ids, distances, lengths and distance counts, on two calls in a row (the
 second smaller, so stale contents of the shared scratch from the first call
-would show), for the NSG, tau-MNG and alpha-PG rules and the reversed form.
+would show), for the NSG, tau-MNG and alpha-PG rules and the reversed form,
+with the octet prune and with the Gram-form prune (FK_FLAG_GRAM, prune
+variant 3: its 16-bit rows prepared by the fk_prune call before).
 Also: a call above the scratch maximum fails with FK_E_INVALID."""
 import sys
 
@@ -10,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle
-from fastkcna_amd import FK_PG_ALPHAPG, FK_PG_NSG, FK_PG_TAUMNG, FkError, IndexParams
+from fastkcna_amd import FK_PG_ALPHAPG, FK_PG_NSG, FK_PG_TAUMNG, FkError, IndexParams, _ffi
 from fastkcna_amd.datasets import integer_mixture, unit_mixture
 
 pytestmark = pytest.mark.gpu
@@ -47,16 +49,19 @@ def same(a, b):
     assert cc_a == cc_b
 
 
+@pytest.mark.parametrize("gram", [False, True])
 @pytest.mark.parametrize("pg", [FK_PG_NSG, FK_PG_TAUMNG, FK_PG_ALPHAPG])
 @pytest.mark.parametrize("reversed_", [False, True])
 @pytest.mark.parametrize("name", ["int128", "unit96"])
-def test_noalloc_matches_fk_prune(device, pg, reversed_, name):
+def test_noalloc_matches_fk_prune(device, pg, reversed_, name, gram):
     x = integer_mixture(3000, 128, 16, seed=21) if name == "int128" else unit_mixture(2500, 96, 8, seed=22,
                                                                                       sigma=0.25)
-    say("noalloc", pg, reversed_, name)
+    say("noalloc", pg, reversed_, name, "gram" if gram else "octet")
     device.upload(x)
     tau = 300.0 if name == "int128" else 0.05
     p = IndexParams(nsg_R=24, search_K=100, tau=tau, angle=63)
+    if gram:
+        p.flags = _ffi.FK_FLAG_GRAM | _ffi.FK_FLAG_PRUNE_V3
     big = pools_for(x, 600, 120, seed=5)
     small = pools_for(x, 150, 80, seed=6)
     for pools, plen, q in (big, small):

@@ -173,6 +173,8 @@ _PROTOS = {
                                           ctypes.POINTER(FkStats)]),
     "fk_step_entry_point": (ctypes.c_int, [_P, ctypes.POINTER(FkParams), ctypes.POINTER(FkState), u32,
                                            ctypes.POINTER(u32)]),
+    "fk_step_connectivity": (ctypes.c_int, [_P, ctypes.POINTER(FkParams), ctypes.POINTER(FkState), u32,
+                                            ctypes.POINTER(FkStats)]),
     "fk_step_stop_stats": (ctypes.c_int, [_P, ctypes.POINTER(FkParams), ctypes.POINTER(FkState), u32, _P, _P, u32,
                                           ctypes.POINTER(f32), ctypes.POINTER(f32)]),
 }

@@ -990,6 +990,53 @@ fk_status fk_step_bridgeview(fk_handle *h, const fk_params *p, fk_state *s, fk_s
     });
 }
 
+fk_status fk_step_connectivity(fk_handle *h, const fk_params *p, fk_state *s, uint32_t ep, fk_stats *st) {
+    if (!h) return FK_E_INVALID;
+    return guard(h, [&] {
+        require_dataset(h);
+        check_params(p);
+        FK_CHECK_ARG(s && s->pool && s->pool_len && s->out_id && s->out_dist && s->out_len, "null state field");
+        FK_CHECK_ARG(s->pool_cap >= 1, "pool_cap must be positive");
+        FK_CHECK_ARG(s->out_cap > p->nsg_R, "out_cap must exceed nsg_R (room for attached edges)");
+        FK_CHECK_ARG(ep < s->n, "entry point out of range");
+        check_lens(s->pool_len, s->n, s->pool_cap, "pool_len > pool_cap");
+        check_lens(s->out_len, s->n, s->out_cap, "out_len > out_cap");
+        check_pool_ids(s);
+        check_out_ids(s, s->out_id, s->out_len);
+        select_device(h);
+        fk_stats local;
+        memset(&local, 0, sizeof(local));
+        BuildState b;
+        if (st) memset(st, 0, sizeof(*st));
+        step_begin(b, h, p, s, st ? st : &local);
+        const uint32_t N = b.N;
+        b.cap = s->pool_cap;
+        b.pool.alloc((size_t)N * b.cap);
+        b.cnt.alloc(N);
+        b.ncap = s->out_cap;
+        b.nsg_id.alloc((size_t)N * b.ncap);
+        b.nsg_d.alloc((size_t)N * b.ncap);
+        b.nsg_len.alloc(N);
+        put_rows(b, b.pool.p, b.cap, s->pool, s->pool_cap, s->pool_cap);
+        FK_HIP(hipMemcpyAsync(b.cnt.p, s->pool_len, (size_t)N * 4, hipMemcpyHostToDevice, b.s));
+        put_rows(b, b.nsg_id.p, b.ncap, s->out_id, s->out_cap, s->out_cap);
+        put_rows(b, b.nsg_d.p, b.ncap, s->out_dist, s->out_cap, s->out_cap);
+        FK_HIP(hipMemcpyAsync(b.nsg_len.p, s->out_len, (size_t)N * 4, hipMemcpyHostToDevice, b.s));
+        b.ep = ep;
+        b.set_ep[0] = ep;
+        tree_grow(b);
+        get_rows(b, s->out_id, s->out_cap, b.nsg_id.p, b.ncap, s->out_cap);
+        get_rows(b, s->out_dist, s->out_cap, b.nsg_d.p, b.ncap, s->out_cap);
+        FK_HIP(hipMemcpyAsync(s->out_len, b.nsg_len.p, (size_t)N * 4, hipMemcpyDeviceToHost, b.s));
+        b.sync();
+        for (uint32_t i = 0; i < N; ++i)  // slots past each list hold zeros, not device bytes
+            for (uint32_t k = s->out_len[i]; k < s->out_cap; ++k) {
+                s->out_id[(size_t)i * s->out_cap + k] = 0;
+                s->out_dist[(size_t)i * s->out_cap + k] = 0.f;
+            }
+    });
+}
+
 fk_status fk_step_stop_stats(fk_handle *h, const fk_params *p, const fk_state *s, uint32_t n_ctrl,
                              const uint32_t *ctrl_ids, const uint32_t *gt, uint32_t gt_stride, float *delta,
                              float *recall) {

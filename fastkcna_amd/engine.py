@@ -252,6 +252,17 @@ class Device:
                                                  ctypes.byref(ep)), "fk_step_entry_point")
         return ep.value
 
+    def step_connectivity(self, params, state, ep):
+        """The connectivity pass on the state's out-lists and pools from
+        entry point ep; attached edges appended to the out-lists; returns the
+        statistics (n_attached, d_conn)."""
+        st = FkStats()
+        p = _ffi.params_struct(params)
+        s = state.struct()
+        self._check(self.lib.fk_step_connectivity(self.h, ctypes.byref(p), ctypes.byref(s), ep, ctypes.byref(st)),
+                    "fk_step_connectivity")
+        return st.to_dict()
+
     def step_stop_stats(self, params, state, ctrl_ids, gt):
         """(delta, recall) of the stop rule on the state's pools."""
         ctrl_ids = _as(ctrl_ids, np.uint32)

@@ -420,6 +420,16 @@ fk_status fk_step_stop_stats(fk_handle *h, const fk_params *p, const fk_state *s
 fk_status fk_step_entry_point(fk_handle *h, const fk_params *p, const fk_state *s, uint32_t init_node,
                               uint32_t *ep);
 
+/* The connectivity pass (KGraphConstructor::tree_grow / findroot / DFS,
+ * src/kgraph.cpp:2145-2260) on the state's out-lists (out_id, out_dist,
+ * out_len) and pools (pool, pool_len) from entry point ep: nodes unreachable
+ * from ep are attached in ascending order of the first unreached id, each
+ * to the first entry of its pool that is reached and has fewer than nsg_R
+ * out-edges (else a random reached node, under a bound that grows with the
+ * draws); the edge and its distance are appended to that node's out-list
+ * (out_cap > nsg_R).  st (optional) receives n_attached and d_conn. */
+fk_status fk_step_connectivity(fk_handle *h, const fk_params *p, fk_state *s, uint32_t ep, fk_stats *st);
+
 /* Beam search of n_q host queries over a CSR graph (the layout fk_build
  * returns) from entry point ep with beam width search_L; writes the best K
  * ids per query.  The best-first loop of get_neighbors (src/kgraph.cpp:

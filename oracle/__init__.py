@@ -17,6 +17,9 @@ this image lacks):
     occluded -> identity; duplicates dropped) and an independent
     re-derivation of every keep/drop decision from its definition
   * NN-descent / full build: recall against brute force, graph invariants
+  * steps (tests/test_oracle_steps.py): BridgeView, Link / InterInsert,
+    UpdateNhoods, the stop rule, ComputeEp and the connectivity pass's
+    attach order on hand-built states
 """
 import ctypes
 import os
@@ -72,6 +75,9 @@ def _load():
         lib.fko_step_update_nhoods.argtypes = [PS]
         lib.fko_step_bridgeview.restype = ctypes.c_int
         lib.fko_step_bridgeview.argtypes = [P, ctypes.c_size_t, ctypes.c_uint, ctypes.c_uint, P, PS, P]
+        lib.fko_step_connectivity.restype = ctypes.c_int
+        lib.fko_step_connectivity.argtypes = [P, ctypes.c_size_t, ctypes.c_uint, ctypes.c_uint, P, PS, ctypes.c_uint,
+                                              P]
         lib.fko_step_stop_stats.restype = None
         lib.fko_step_stop_stats.argtypes = [P, PS, ctypes.c_uint, P, P, ctypes.c_uint, P, P]
         lib.fko_free.restype = None
@@ -267,6 +273,18 @@ def step_entry_point(rows, params, state, init_node):
     ps = _params(params)
     return int(_load().fko_step_entry_point(_p(rows), rows.strides[0], rows.shape[0], rows.shape[1],
                                             ctypes.byref(ps), ctypes.byref(state.struct()), init_node))
+
+
+def step_connectivity(rows, params, state, ep):
+    """The connectivity pass (DFS from ep, findroot, attach) on the state's
+    out-lists and pools; attached edges appended to the out-lists."""
+    rows = np.ascontiguousarray(rows, np.float32)
+    st = _stats()
+    ps = _params(params)
+    rc = _load().fko_step_connectivity(_p(rows), rows.strides[0], rows.shape[0], rows.shape[1], ctypes.byref(ps),
+                                       ctypes.byref(state.struct()), ep, ctypes.byref(st))
+    assert rc == 0, "out-list above out_cap"
+    return st.to_dict()
 
 
 def step_stop_stats(params, state, ctrl_ids, gt):

@@ -1384,6 +1384,39 @@ int fko_step_bridgeview(const float *rows, size_t stride_bytes, unsigned n_total
     return 0;
 }
 
+// The connectivity pass (KGraphConstructor::tree_grow / findroot / DFS,
+// src/kgraph.cpp:2145-2260) on the state's out-lists and pools from entry
+// point ep: the attached edges are appended to the out-lists (with their
+// distances).  Returns 1 when an out-list would pass out_cap.
+int fko_step_connectivity(const float *rows, size_t stride_bytes, unsigned n_total, unsigned d,
+                          const fk_params *p, fk_state *s, unsigned ep, fk_stats *st_out) {
+    Rows r;
+    r.load(rows, stride_bytes, n_total, d);
+    r.oracle_n = s->n;
+    fk_stats local;
+    fk_stats *st = st_out ? st_out : &local;
+    memset(st, 0, sizeof(*st));
+    Builder b(r, *p, st);
+    b.shrink();
+    load_pools(b, s, s->pool_cap);
+    for (unsigned i = 0; i < s->n; ++i)
+        for (unsigned k = 0; k < s->out_len[i]; ++k)
+            b.nh[i].nn_nsg.push_back(
+                Nb(s->out_id[(size_t)i * s->out_cap + k], s->out_dist[(size_t)i * s->out_cap + k], true));
+    b.ep = ep;
+    b.tree_grow();
+    for (unsigned i = 0; i < s->n; ++i) {
+        const auto &out = b.nh[i].nn_nsg;
+        if (out.size() > s->out_cap) return 1;
+        s->out_len[i] = (uint32_t)out.size();
+        for (size_t k = 0; k < out.size(); ++k) {
+            s->out_id[(size_t)i * s->out_cap + k] = out[k].id;
+            s->out_dist[(size_t)i * s->out_cap + k] = out[k].dist;
+        }
+    }
+    return 0;
+}
+
 // ComputeEp from init_node's pool over the state's out-lists; returns ep.
 unsigned fko_step_entry_point(const float *rows, size_t stride_bytes, unsigned n_total, unsigned d,
                               const fk_params *p, const fk_state *s, unsigned init_node) {

@@ -111,6 +111,56 @@ def test_link_step_hub(device):
     assert gst["d_prune_rev"] == cst["d_prune_rev"] > 0
 
 
+def components_state(x, n_comp, nsg_R, rng):
+    """Out-lists that split the nodes into n_comp groups (random edges
+    inside a group, degrees below nsg_R except a few full nodes in the entry
+    point's group) and pools of 16 random nodes in (dist, id) order that
+    always hold node 0 (the entry point, in group 0), so every attach finds
+    a reached pool entry and no random draw runs."""
+    n = len(x)
+    group = rng.integers(0, n_comp, n)
+    group[0] = 0
+    s = StepState(n, 16, nsg_R + 8)
+    for i in range(n):
+        mates = np.flatnonzero(group == group[i])
+        mates = mates[mates != i]
+        full = group[i] == 0 and i % 97 == 3
+        k = min(len(mates), nsg_R if full else int(rng.integers(1, 4)))
+        ids = rng.choice(mates, k, replace=False) if k else np.zeros(0, np.int64)
+        s.out_len[i] = k
+        s.out_id[i, :k] = ids
+        s.out_dist[i, :k] = oracle.pair_distances(x, np.full(k, i), ids)
+        others = np.delete(np.arange(n), i)
+        cand = rng.choice(others, 15, replace=False)
+        if i != 0 and 0 not in cand:
+            cand[0] = 0
+        d = oracle.pair_distances(x, np.full(len(cand), i), cand)
+        order = np.lexsort((cand, d))
+        s.pool[i, :len(cand)] = oracle.pack(d[order], cand[order])
+        s.pool_len[i] = len(cand)
+    return s
+
+
+@pytest.mark.parametrize("name,n_comp", [("unit96", 40), ("int128", 25), ("odd20", 60)])
+def test_connectivity_step_bitexact(device, name, n_comp):
+    """The connectivity pass (GPU BFS + min-label propagation + the
+    attach loop) against the oracle's DFS / findroot on states of many
+    components whose pools always hold a reached node: the same edges in the
+    same out-list slots with the same distances, the same counts."""
+    x = DATA[name]()[:1500]
+    rng = np.random.default_rng(47 + n_comp)
+    s = components_state(x, n_comp, 64, rng)
+    say("connectivity", name, n_comp)
+    device.upload(x)
+    p = params(name, nsg_R=64, search_K=16, K=16, L=16, R=16, search_L=16)
+    g, c = s.copy(), s.copy()
+    gst = device.step_connectivity(p, g, 0)
+    cst = oracle.step_connectivity(x, p, c, 0)
+    assert gst["n_attached"] == cst["n_attached"] > 0 and gst["d_conn"] == cst["d_conn"]
+    assert_links_equal(g, c)
+    say("connectivity", name, "attached", cst["n_attached"])
+
+
 def random_out_lists(s, rng, p_keep=0.5):
     for i in range(s.n):
         k = int(rng.integers(0, s.out_cap + 1))

@@ -17,6 +17,11 @@ follow from the reference's definitions by hand:
 * Link on a 1-D chain where each pool holds only nodes to the right: the NSG
   rule keeps the nearest right neighbour, InterInsert adds the left one, and
   with nsg_R = 1 the reversed prune keeps the nearer of the two.
+* The connectivity pass (tree_grow / findroot / DFS, :2145-2260) on an
+  8-node graph of four components: the attached edges, in order, follow by
+  hand from "first unreached id, first reached pool entry with fewer than
+  nsg_R out-edges" (one reached through an earlier attach, one full node
+  skipped), each with the reference distance of its two rows.
 """
 import numpy as np
 import pytest
@@ -302,3 +307,38 @@ def test_search_on_a_chain_is_greedy():
     for ep in (0, n - 1):
         res, _ = oracle.search(x, off, ids, ep, qs, 2, 1)
         assert list(res[:, 0]) == [int(np.argmin(np.abs(x[:, 0] - q))) for q in qs[:, 0]]
+
+
+def test_connectivity_attach_order_by_hand():
+    """ep = 0 reaches {0, 1}; components {2, 3}, {4, 5}, {6, 7} are not
+    reached; nsg_R = 2.  Out-lists: 0 -> 1, 1 -> 0, 2 -> 3, 4 -> 5, 5 -> 4,
+    6 -> 7.  Pools (only their ids and order matter): 2: [5, 1, 0],
+    4: [6, 3, 0], 6: [1, 0, 2].  By hand: id 2 is the first unreached id, 5
+    is unreached, 1 is reached with 1 < 2 out-edges: edge 1 -> 2 (3 is then
+    reached through 2).  Id 4: 6 is unreached, 3 is reached (through the
+    first attach) with no out-edge: edge 3 -> 4.  Id 6: 1 now has 2 = nsg_R
+    out-edges and is skipped, 0 has one: edge 0 -> 6.  Three edges, no
+    random draw."""
+    rng = np.random.default_rng(61)
+    x = rng.integers(0, 20, size=(8, 8)).astype(np.float32)
+    s = StepState(8, 3, 4)
+    out = {0: [1], 1: [0], 2: [3], 4: [5], 5: [4], 6: [7]}
+    for i, lst in out.items():
+        s.out_len[i] = len(lst)
+        s.out_id[i, :len(lst)] = lst
+        s.out_dist[i, :len(lst)] = [oracle.l2(x[i], x[j]) for j in lst]
+    pools = {i: [(i + 1) % 8, (i + 2) % 8, (i + 3) % 8] for i in range(8)}
+    pools.update({2: [5, 1, 0], 4: [6, 3, 0], 6: [1, 0, 2]})
+    for i, ids in pools.items():
+        s.pool[i, :3] = oracle.pack(np.float32([1.0, 2.0, 3.0]), ids)
+        s.pool_len[i] = 3
+    p = IndexParams(K=3, L=3, S=2, R=3, nsg_R=2, search_L=3, search_K=3)
+    st = oracle.step_connectivity(x, p, s, 0)
+    assert st["n_attached"] == 3 and st["d_conn"] == 3
+    want = {0: [1, 6], 1: [0, 2], 2: [3], 3: [4], 4: [5], 5: [4], 6: [7], 7: []}
+    for i, lst in want.items():
+        assert s.out_len[i] == len(lst), i
+        assert list(s.out_id[i, :len(lst)]) == lst, i
+    for src, dst in ((1, 2), (3, 4), (0, 6)):
+        k = list(s.out_id[src, :s.out_len[src]]).index(dst)
+        assert s.out_dist[src, k] == np.float32(oracle.l2(x[src], x[dst]))

@@ -31,7 +31,13 @@
 // insert against brute force; the pruners against closed forms (tau = 0 and
 // angle special cases equal the NSG rule, identity when nothing occludes) and
 // brute-force re-checks of every kept/dropped decision; NN-descent and the
-// full build by recall against brute force and graph invariants.
+// full build by recall against brute force and graph invariants; the
+// stateful steps (BridgeView, Link with InterInsert, UpdateNhoods, the stop
+// rule, ComputeEp, the connectivity pass's attach order) on hand-built
+// states whose results follow from the reference's definitions
+// (tests/test_oracle_steps.py).  InterInsert gathers its reverse edges before
+// appending them (Builder::inter_insert says why the edges are the
+// reference's).
 #include <omp.h>
 #include <stdint.h>
 #include <string.h>

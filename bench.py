@@ -345,13 +345,14 @@ def screen_leg(dev, p, args, plain_graph, gt, queries, plain_sec):
     return out
 
 
-def batching_leg(dev, n_sets):
+def batching_leg(dev, n_sets, one_by_one=True):
     """The first n_sets datasets of the batching workload (n log-uniform in
     [2,000, 200,000], d in {96, 128, 256}, unit-normalised 16-centre
     mixtures; alpha-PG, K=L=100 S=10 R=50 I=6 nsg_R=32), grouped by width:
     one fk_build_batch per width against one fk_build per dataset (uploads
     outside both timings), and the batch again with FK_FLAG_GRAM (Gram-form
-    prune); the graphs are compared."""
+    prune); the graphs are compared.  one_by_one=False skips the per-dataset
+    builds (profiling runs)."""
     from fastkcna_amd import FK_PG_ALPHAPG, _ffi
     from fastkcna_amd.datasets import c4_sets, params_c4
     from fastkcna_amd.kgraph import IndexParams
@@ -381,16 +382,18 @@ def batching_leg(dev, n_sets):
         same_gram = same_gram and all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
                                       for a, b in zip(got, ggot))
         del ggot
-        for k, x in enumerate(group):
+        for k, x in enumerate(group if one_by_one else []):
             dev.upload(x)
             t = time.perf_counter()
             o, i, ep, w, _ = dev.build(FK_PG_ALPHAPG, p)
             t_single += time.perf_counter() - t
             same = same and np.array_equal(o, got[k][0]) and np.array_equal(i, got[k][1]) and ep == got[k][2]
     out = {"sets": n_sets, "rows": int(sum(len(x) for x in sets)), "batched_sec": t_batch,
-           "one_by_one_sec": t_single, "speedup": t_single / t_batch, "graphs_identical": bool(same),
+           "one_by_one_sec": t_single if one_by_one else None,
+           "speedup": t_single / t_batch if one_by_one else None, "graphs_identical": bool(same),
            "gram": {"batched_sec": t_gram, "speedup_vs_batched": t_batch / t_gram,
-                    "speedup_vs_one_by_one": t_single / t_gram, "graphs_identical": bool(same_gram),
+                    "speedup_vs_one_by_one": t_single / t_gram if one_by_one else None,
+                    "graphs_identical": bool(same_gram),
                     "prune_variants_by_width": gram_variants}}
     log(f"batching: {out}")
     return out
@@ -550,6 +553,8 @@ def main():
                     help="skip the other workloads (C1 fasthnsw, C3 tau-MNG 1M x 960, C5 10M x 96)")
     ap.add_argument("--leg", choices=["C1", "C3", "C4", "C5"],
                     help="run only this workload's leg and print its JSON (profiling runs)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="with --leg C4: only the batched builds, not the one-by-one comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the build does not shard "
@@ -562,7 +567,8 @@ def main():
 
     if args.leg:
         dev = Device(0)
-        legs = {"C1": c1_leg, "C3": c3_leg, "C5": c5_leg, "C4": lambda d: batching_leg(d, args.batch_sets or 256)}
+        legs = {"C1": c1_leg, "C3": c3_leg, "C5": c5_leg,
+                "C4": lambda d: batching_leg(d, args.batch_sets or 256, one_by_one=not args.batch_only)}
         print(json.dumps(legs[args.leg](dev)), flush=True)
         dev.close()
         return

@@ -926,9 +926,22 @@ void nnd_alloc(BuildState &b) {
     b.scal.alloc(16);
 }
 
+// q[c] = the row of node ids[c] (dpad floats): the control rows as queries.
+__global__ void k_gather_rows(const float *__restrict__ rows, uint32_t stride, uint32_t dpad,
+                              const uint32_t *__restrict__ ids, uint32_t n, float *__restrict__ q) {
+    const size_t total = (size_t)n * dpad;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t c = e / dpad;
+        q[e] = rows[(size_t)ids[c] * stride + (e - c * dpad)];
+    }
+}
+
 // GenerateControl (src/kgraph.cpp:363-387): `controls` distinct random nodes
 // and their exact MAXL nearest neighbours (self excluded, ties by id); per
-// set, within the set, with the set's random stream.
+// set, within the set, with the set's random stream.  Every set's controls
+// are drawn first, uploaded and their rows gathered in one launch (a batch of
+// many sets would otherwise pay a copy per control row); then one brute-force
+// search per set over the set's rows.
 void nnd_controls(BuildState &b) {
     const double t0 = now_sec();
     const uint32_t nc = (b.P.flags & FK_FLAG_NO_CONTROLS) ? 0 : b.P.controls;
@@ -938,8 +951,7 @@ void nnd_controls(BuildState &b) {
     if (b.n_ctrl == 0) return;
     b.ctrl_ids.alloc(b.n_ctrl);
     b.ctrl_gt.alloc((size_t)b.n_ctrl * b.P.MAXL);
-    DevBuf<float> q((size_t)nc * b.ds->dpad);
-    DevBuf<uint32_t> dself(nc);
+    std::vector<uint32_t> ids(b.n_ctrl), local(b.n_ctrl);
     for (uint32_t k = 0; k < b.n_sets; ++k) {
         const uint32_t lo = b.set_lo[k], Ns = b.set_n(k);
         std::vector<uint32_t> idx(Ns);
@@ -949,27 +961,37 @@ void nnd_controls(BuildState &b) {
             const uint32_t j = i + (uint32_t)(b.next_rand(k) % (Ns - i));
             std::swap(idx[i], idx[j]);
         }
-        std::vector<uint32_t> local(idx.begin(), idx.begin() + nc), ids(nc);
-        for (uint32_t c = 0; c < nc; ++c) ids[c] = lo + local[c];
-        FK_HIP(hipMemcpyAsync(b.ctrl_ids.p + b.ctrl_lo[k], ids.data(), (size_t)nc * 4, hipMemcpyHostToDevice, b.s));
-        FK_HIP(hipMemcpyAsync(dself.p, local.data(), (size_t)nc * 4, hipMemcpyHostToDevice, b.s));
-        // control rows as queries
-        for (uint32_t c = 0; c < nc; ++c)
-            FK_HIP(hipMemcpyAsync(q.p + (size_t)c * b.ds->dpad, b.ds->rows.p + (size_t)ids[c] * b.ds->stride,
-                                  (size_t)b.ds->dpad * 4, hipMemcpyDeviceToDevice, b.s));
+        for (uint32_t c = 0; c < nc; ++c) {
+            local[b.ctrl_lo[k] + c] = idx[c];
+            ids[b.ctrl_lo[k] + c] = lo + idx[c];
+        }
+    }
+    DevBuf<float> q((size_t)b.n_ctrl * b.ds->dpad);
+    DevBuf<uint32_t> dself(b.n_ctrl);
+    FK_HIP(hipMemcpyAsync(b.ctrl_ids.p, ids.data(), (size_t)b.n_ctrl * 4, hipMemcpyHostToDevice, b.s));
+    FK_HIP(hipMemcpyAsync(dself.p, local.data(), (size_t)b.n_ctrl * 4, hipMemcpyHostToDevice, b.s));
+    {
+        const size_t total = (size_t)b.n_ctrl * b.ds->dpad;
+        hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)std::min<size_t>((total + 255) / 256, 65536)), dim3(256), 0,
+                           b.s, b.ds->rows.p, b.ds->stride, b.ds->dpad, b.ctrl_ids.p, b.n_ctrl, q.p);
+        FK_LAUNCH_CHECK();
+    }
+    for (uint32_t k = 0; k < b.n_sets; ++k) {
+        const uint32_t lo = b.set_lo[k], Ns = b.set_n(k);
         const uint32_t K = b.P.MAXL < Ns - 1 ? b.P.MAXL : Ns - 1;
         uint32_t *gt_out = b.ctrl_gt.p + (size_t)b.ctrl_lo[k] * b.P.MAXL;
         if (K < b.P.MAXL) FK_HIP(hipMemsetAsync(gt_out, 0xFF, (size_t)nc * b.P.MAXL * 4, b.s));
         DevBuf<uint32_t> gt((size_t)nc * K);
         hipEvent_t pe = b.prof_begin();
-        bruteforce_knn(b.s, *b.ds, q.p, b.ds->dpad, nc, Ns, K, dself.p, gt.p, nullptr, lo);
+        bruteforce_knn(b.s, *b.ds, q.p + (size_t)b.ctrl_lo[k] * b.ds->dpad, b.ds->dpad, nc, Ns, K,
+                       dself.p + b.ctrl_lo[k], gt.p, nullptr, lo);
         // algorithmic bytes: every row read once, the K ids written per control
         b.prof_end(pe, FK_PROF_CONTROLS, (double)Ns * 4.0 * b.ds->d + (double)nc * K * 4.0);
         FK_HIP(hipMemcpy2DAsync(gt_out, (size_t)b.P.MAXL * 4, gt.p, (size_t)K * 4, (size_t)K * 4, nc,
                                 hipMemcpyDeviceToDevice, b.s));
-        b.sync();
         b.st->d_ctrl += (uint64_t)nc * (Ns - 1);
     }
+    b.sync();
     b.st->t_ctrl += now_sec() - t0;
 }
 

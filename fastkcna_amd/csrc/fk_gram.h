@@ -109,6 +109,7 @@ struct GramBound {
     float e_abs;      // absolute rounding of a row (FP16 underflow), unscaled
     float f_rel;      // FP32 evaluation of d~, relative to |x~|^2 + |y~|^2
     float gam;        // canonical FP32 distance, relative to a^2
+    float ymax;       // largest row norm |x|, rounded up
 };
 
 // Lower bound on the canonical distance given d~ (0 when nothing is known).

@@ -128,6 +128,7 @@ GramBound gram_bound(const Dataset &ds) {
     GramBound g{};
     g.exact = ds.gram_fmt == FK_GRAM_EXACT;
     g.inv_s2 = 1.f / (ds.gram_scale * ds.gram_scale);
+    g.ymax = ds.gram_ymax;
     if (g.exact) return g;
     const double dp = ds.dpad;
     const double u = ds.gram_fmt == FK_GRAM_F16 ? ldexp(1.0, -11) : ldexp(1.0, -8);
@@ -210,6 +211,8 @@ void gram_prepare(hipStream_t s, Dataset &ds, uint32_t flags) {
     FK_HIP(hipStreamSynchronize(s));
     ds.gram_fmt = fmt;
     ds.gram_scale = scale;
+    // nmax bounds every squared norm from above (a double sum rounded up)
+    ds.gram_ymax = (float)(std::sqrt((double)nmax) * (1.0 + 1e-6));
     ds.gram_req = want;
 }
 

@@ -196,6 +196,7 @@ struct Dataset {
     DevBuf<uint16_t> rows16;  // n_total x dpad 16-bit Gram operands (FP16 of the prescaled value, else BF16)
     int gram_fmt = 0;
     float gram_scale = 1.f;
+    float gram_ymax = 0.f;  // largest row norm |x| over the rows, rounded up
     uint32_t gram_req = 0;
     // centroids of row ranges (batched builds' sets) already made, d floats
     // each, keyed by (first row, row count); cleared by every upload

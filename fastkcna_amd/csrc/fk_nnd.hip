@@ -1746,9 +1746,13 @@ __device__ __forceinline__ float octet_sum_any(float s) {
 // dd <= thr would reject it.  The test is taken without a square root,
 // d~ - f > (e + T)^2 with T = sqrt(thr / (1 - gam)) (T and e rounded up by
 // 2^-20, each side moved by 2^-18, more than the rounding of these
-// operations).  The survivors go through the FP32 step unchanged, so the
-// candidates and the merged pools are the same.  G: the kernel with the
-// pass (the others are compiled without it).
+// operations).  A target's |y~|^2 is summed from the 16-bit row the pass
+// reads anyway (at most 35 FP32 roundings of non-negative terms, inside the
+// norm share of f_rel), and its |y| in e is bounded by the largest row norm
+// (gb.ymax; FP16 forms are chosen for unit-normalised rows), so the pass
+// reads one row per target and no norm.  The survivors go through the FP32
+// step unchanged, so the candidates and the merged pools are the same.  G:
+// the kernel with the pass (the others are compiled without it).
 template <int NB, int MODE, bool G>
 __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 4 : (NB == 32 ? 3 : FK_BVW_WPE)))) void k_bv_round(
     const void *__restrict__ chains_v, const float2 *__restrict__ nrm, uint32_t dpad, uint32_t N, uint32_t cap,
@@ -1907,23 +1911,25 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
             uint32_t m2 = 0;
             for (uint32_t e0 = 0; e0 < m; e0 += 8 * I16) {
                 HalfSlice<NBH> hh[I16];
-                float2 tn2[I16];
                 uint32_t tt[I16];
 #pragma unroll
                 for (uint32_t u = 0; u < I16; ++u) {
                     const uint32_t e = e0 + u * 8 + g;
                     tt[u] = e < m ? todo[e] : i;
                     hh[u].load(r16 + (size_t)tt[u] * dpad + k * NBH);
-                    tn2[u] = gnrm[tt[u]];
                 }
                 wave_lds_sync();
 #pragma unroll
                 for (uint32_t u = 0; u < I16; ++u) {
                     const float dot = octet_sum_any(mine16.dot(hh[u]));
-                    const float nx = n16.x + tn2[u].x;
+                    // the target's |y~|^2 from the row just read (no norm
+                    // read per target: on random targets that is another
+                    // cache line each); |y| <= the largest row norm
+                    const float ny = octet_sum_any(hh[u].dot(hh[u])) * gb.inv_s2;
+                    const float nx = n16.x + ny;
                     const float dt = nx - 2.f * (dot * gb.inv_s2);
                     const float f = gb.f_rel * nx;
-                    const float e = (gb.u * (n16.y + tn2[u].y) + gb.e_abs) * FK_BVW_UP20;
+                    const float e = (gb.u * (n16.y + gb.ymax) + gb.e_abs) * FK_BVW_UP20;
                     const float lo = t16 + e;
                     const bool far = (dt - f) * FK_BVW_DN18 > lo * lo * FK_BVW_UP18;
                     const bool live = (e0 + u * 8 + g) < m && k == 0;

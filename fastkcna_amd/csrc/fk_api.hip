@@ -306,6 +306,7 @@ fk_status fk_dataset_upload(fk_handle *h, const float *rows, size_t row_stride_b
         ds.chains8.release();
         ds.norms8.release();
         ds.rows16.release();
+        ds.rows8q.release();
         ds.gram_fmt = FK_GRAM_NONE;
         ds.gram_req = 0;
         ds.n_total = 0;

@@ -119,6 +119,46 @@ __global__ __launch_bounds__(256) void k_rows16(const float *__restrict__ rows, 
     }
 }
 
+// rows8q: each row of width 96 as 8-bit integers q with one scale s = max |x| /
+// 127 (q = x / s rounded, so |x - s q| <= s / 2 per value), then s, the error
+// norm |x - s q| (exact products and differences in double, rounded up) and
+// |s q|^2 (s^2 sum q^2 in double, rounded once) as FP32, 20 zero bytes: 128
+// bytes, one cache line.  One wave per row, lane l holds values l and l + 64.
+__global__ __launch_bounds__(256) void k_rows8q(const float *__restrict__ rows, uint32_t stride, uint32_t n,
+                                                uint4 *__restrict__ out) {
+    const uint32_t lane = lane_id();
+    for (uint32_t i = blockIdx.x * 4u + (threadIdx.x >> 6); i < n; i += gridDim.x * 4u) {
+        const float *r = rows + (size_t)i * stride;
+        const float x0 = r[lane], x1 = lane < 32u ? r[64u + lane] : 0.f;
+        float m = fmaxf(fabsf(x0), fabsf(x1));
+        for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        const float s = m / 127.f;
+        auto quant = [&](float x) -> int {
+            if (!(s > 0.f)) return 0;
+            const float q = rintf(x / s);
+            return (int)fminf(fmaxf(q, -127.f), 127.f);
+        };
+        const int q0 = quant(x0), q1 = lane < 32u ? quant(x1) : 0;
+        const double d0 = (double)x0 - (double)s * q0, d1 = (double)x1 - (double)s * q1;
+        double err2 = d0 * d0 + d1 * d1;
+        int qq = q0 * q0 + q1 * q1;
+        for (int o = 32; o >= 1; o >>= 1) {
+            err2 += __shfl_xor(err2, o, 64);
+            qq += __shfl_xor(qq, o, 64);
+        }
+        int8_t *ob = reinterpret_cast<int8_t *>(out + (size_t)i * 8u);
+        ob[lane] = (int8_t)q0;
+        if (lane < 32u) ob[64u + lane] = (int8_t)q1;
+        if (lane < 8u) {
+            float v = 0.f;
+            if (lane == 0) v = s;
+            if (lane == 1) v = (float)(sqrt(err2) * (1.0 + 1e-6));
+            if (lane == 2) v = (float)((double)s * (double)s * (double)qq);
+            reinterpret_cast<float *>(ob + 96)[lane] = v;
+        }
+    }
+}
+
 __global__ void k_to_bytes(const float *__restrict__ src, size_t n, uint8_t *__restrict__ dst) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = (uint8_t)src[i];
@@ -194,6 +234,13 @@ void gram_prepare(hipStream_t s, Dataset &ds, uint32_t flags) {
         hipLaunchKernelGGL(k_rows16<FK_GRAM_BF16>, dim3(g16), dim3(256), 0, s, ds.rows.p, ds.stride, ds.dpad,
                            ds.n_total, scale, ds.rows16.p);
     FK_LAUNCH_CHECK();
+    // width 96 in the FP16 form: the 8-bit copy of BridgeView's bound pass
+    ds.rows8q.release();
+    if (fmt == FK_GRAM_F16 && ds.dpad == 96) {
+        ds.rows8q.alloc((size_t)ds.n_total * 8u);
+        hipLaunchKernelGGL(k_rows8q, dim3(g16), dim3(256), 0, s, ds.rows.p, ds.stride, ds.n_total, ds.rows8q.p);
+        FK_LAUNCH_CHECK();
+    }
     // exact form with values in [0, 255]: a chain-major byte copy for the
     // octet kernels (dot products in 32-bit integers, exact)
     ds.chains8.release();

@@ -1723,6 +1723,27 @@ __device__ __forceinline__ float octet_sum_any(float s) {
     return s;
 }
 
+// The sum of an octet's 8 32-bit integers (exact), in lane 0 of the octet
+// at least (every sum here is below 2^24 in magnitude), as a float.
+__device__ __forceinline__ float octet_sum_i32(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    return (float)v;
+}
+// Lane 6's 32-bit value, as a float, in lane 0 of each octet (row_shl:6).
+__device__ __forceinline__ float octet_from6(uint32_t v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, (int)v, 0x106, 0xF, 0xF, false));
+}
+
+#ifndef FK_BVW_INFLIGHT8Q
+#define FK_BVW_INFLIGHT8Q 4  // 8-bit rows per octet in flight in the bound pass
+#endif
+// Relative error of |x^ - y^|^2 evaluated in FP32 from the stored scales and
+// |x^|^2 (each rounded once) and the exact integer dot product, as a share
+// of |x^|^2 + |y^|^2: at most about 10 roundings, here 16
+#define FK_Q8_FREL 0.00000095367431640625f  // 2^-20
+
 #ifndef FK_BVW_INFLIGHT16
 #define FK_BVW_INFLIGHT16 4  // rows per octet in flight in the bound pass, width 96
 #endif
@@ -1753,7 +1774,7 @@ __device__ __forceinline__ float octet_sum_any(float s) {
 // reads one row per target and no norm.  The survivors go through the FP32
 // step unchanged, so the candidates and the merged pools are the same.  G:
 // the kernel with the pass (the others are compiled without it).
-template <int NB, int MODE, bool G>
+template <int NB, int MODE, int G>
 __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 4 : (NB == 32 ? 3 : FK_BVW_WPE)))) void k_bv_round(
     const void *__restrict__ chains_v, const float2 *__restrict__ nrm, uint32_t dpad, uint32_t N, uint32_t cap,
     uint32_t W, uint32_t S_cap_all, const uint32_t *__restrict__ set_of, const uint32_t *__restrict__ set_scap,
@@ -1762,7 +1783,7 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
     uint32_t ncap, unsigned long long *__restrict__ cc_acc, unsigned long long *__restrict__ comp_acc,
     const uint32_t *__restrict__ order, uint32_t *__restrict__ mark, uint32_t wave_bytes, uint32_t wpb,
     const uint16_t *__restrict__ r16, const float2 *__restrict__ gnrm, GramBound gb,
-    unsigned long long *__restrict__ rej_acc) {
+    const uint4 *__restrict__ q8, unsigned long long *__restrict__ rej_acc) {
     using E = typename RowKind<NB, MODE>::E;
     using Chain = typename RowKind<NB, MODE>::Chain;
     constexpr bool DOT = RowKind<NB, MODE>::dot;
@@ -1872,15 +1893,26 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
     Chain mine;
     mine.load(lane_chains + (size_t)i * dpad, nb);
     const float ni = DOT ? nrm[i].x : 0.f;
-    constexpr bool G16 = G && MODE == 0 && (NB == 12 || NB == 16 || NB == 32);
+    constexpr bool G16 = G == 1 && MODE == 0 && (NB == 12 || NB == 16 || NB == 32);
+    constexpr bool Q8 = G == 2 && MODE == 0 && NB == 12;
     constexpr uint32_t NBH = G16 ? (uint32_t)NB : 4u;
     const bool bound16 = G16 && r16 != nullptr && thr < INFINITY;  // wave-uniform
+    const bool bound8 = Q8 && q8 != nullptr && thr < INFINITY;     // wave-uniform
     HalfSlice<NBH> mine16;
     float2 n16 = make_float2(0.f, 0.f);
     float t16 = 0.f;
+    uint4 mine8 = make_uint4(0u, 0u, 0u, 0u);
+    float s8 = 0.f, e8 = 0.f, nq8 = 0.f;  // the node's scale, error norm, |x^|^2 (octet lane 0)
     if (bound16) {
         mine16.load(r16 + (size_t)i * dpad + k * NBH);
         n16 = gnrm[i];
+        t16 = __builtin_sqrtf(thr / (1.f - gb.gam)) * FK_BVW_UP20;
+    }
+    if (bound8) {
+        mine8 = q8[(size_t)i * 8u + k];
+        s8 = octet_from6(mine8.x);
+        e8 = octet_from6(mine8.y);
+        nq8 = octet_from6(mine8.z);
         t16 = __builtin_sqrtf(thr / (1.f - gb.gam)) * FK_BVW_UP20;
     }
     uint32_t np = np0, nc = 0, nt = 0;
@@ -1903,6 +1935,45 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
     // octet in flight (slots past m compute a discarded distance to row 0);
     // candidates (distance <= thr) go to c, merged before c can overflow
     auto run_todo = [&](uint32_t m) __attribute__((always_inline)) {
+        if (bound8) {
+            // the bound pass on the 8-bit rows (one 16-byte piece per lane,
+            // a whole 128-byte row per octet); survivors compacted as below
+            uint32_t m2 = 0;
+            for (uint32_t e0 = 0; e0 < m; e0 += 8 * FK_BVW_INFLIGHT8Q) {
+                uint4 hq[FK_BVW_INFLIGHT8Q];
+                uint32_t tt[FK_BVW_INFLIGHT8Q];
+#pragma unroll
+                for (uint32_t u = 0; u < FK_BVW_INFLIGHT8Q; ++u) {
+                    const uint32_t e = e0 + u * 8 + g;
+                    tt[u] = e < m ? todo[e] : i;
+                    hq[u] = q8[(size_t)tt[u] * 8u + k];
+                }
+                wave_lds_sync();
+#pragma unroll
+                for (uint32_t u = 0; u < FK_BVW_INFLIGHT8Q; ++u) {
+                    int part = __builtin_amdgcn_sdot4((int)mine8.x, (int)hq[u].x, 0, false);
+                    part = __builtin_amdgcn_sdot4((int)mine8.y, (int)hq[u].y, part, false);
+                    part = __builtin_amdgcn_sdot4((int)mine8.z, (int)hq[u].z, part, false);
+                    part = __builtin_amdgcn_sdot4((int)mine8.w, (int)hq[u].w, part, false);
+                    part = k < 6u ? part : 0;  // lanes 6 and 7 hold the row's scalars
+                    const float dot = octet_sum_i32(part);
+                    const float sy = octet_from6(hq[u].x), ey = octet_from6(hq[u].y);
+                    const float nx = nq8 + octet_from6(hq[u].z);
+                    // |x^ - y^|^2 in FP32 (dot exact), its error within FK_Q8_FREL nx
+                    const float dt = nx - 2.f * ((s8 * sy) * dot);
+                    const float f = FK_Q8_FREL * nx;
+                    const float lo = t16 + (e8 + ey) * FK_BVW_UP20;
+                    const bool far = (dt - f) * FK_BVW_DN18 > lo * lo * FK_BVW_UP18;
+                    const bool live = (e0 + u * 8 + g) < m && k == 0;
+                    const uint64_t mk = wave_ballot(live && !far);
+                    if (live && !far) todo[m2 + lanes_below(mk)] = tt[u];
+                    m2 += (uint32_t)__builtin_popcountll(mk);
+                    rejected += (uint32_t)__builtin_popcountll(wave_ballot(live && far));
+                }
+                wave_lds_sync();
+            }
+            m = m2;
+        }
         if (bound16) {
             // the bound pass: survivors compacted to todo[0, m2) in order
             // (a chunk's entries are read before any is written back, and
@@ -2050,19 +2121,21 @@ static void launch_bv_round(BuildState &b, uint32_t W, uint32_t scap, const uint
     const bool g16 = (b.P.flags & FK_FLAG_GRAM) && b.ds->gram_fmt == FK_GRAM_F16 && b.ds->rows16.p;
     const uint16_t *r16 = g16 ? b.ds->rows16.p : nullptr;
     const GramBound gb = g16 ? gram_bound(*b.ds) : GramBound{};
+    // width 96: the pass on the 8-bit rows when they are made
+    const uint4 *q8 = g16 && b.ds->rows8q.p ? b.ds->rows8q.p : nullptr;
 #define FK_BV_LAUNCH_G(NBV, MODE, G)                                                                             \
     hipLaunchKernelGGL((k_bv_round<NBV, MODE, G>), dim3(grid), dim3(64 * wpb), lds, b.s,                       \
                        MODE == 2 ? (const void *)c8 : (const void *)b.ds->chains.p, nrm,                        \
                        b.ds->dpad, b.N, b.cap, W, scap, b.dev_set_of(), set_scap, b.pool.p, b.pflg.p, b.cnt.p,  \
                        b.nsg_id.p, b.nsg_len.p,                                                                 \
                        b.nsg_new.p, b.ncap, b.tally.at(FK_T_BV_CC), b.tally.at(FK_T_BV_COMP), ord, mark,       \
-                       wave_bytes, wpb, r16, b.ds->gram_nrm.p, gb, b.tally.at(FK_T_BV_REJ))
-#define FK_BV_LAUNCH(NBV, MODE) FK_BV_LAUNCH_G(NBV, MODE, false)
-#define FK_BV_LAUNCH16(NBV) do { if (r16) FK_BV_LAUNCH_G(NBV, 0, true); else FK_BV_LAUNCH_G(NBV, 0, false); } while (0)
+                       wave_bytes, wpb, r16, b.ds->gram_nrm.p, gb, q8, b.tally.at(FK_T_BV_REJ))
+#define FK_BV_LAUNCH(NBV, MODE) FK_BV_LAUNCH_G(NBV, MODE, 0)
+#define FK_BV_LAUNCH16(NBV) do { if (r16) FK_BV_LAUNCH_G(NBV, 0, 1); else FK_BV_LAUNCH_G(NBV, 0, 0); } while (0)
     switch (b.ds->dpad) {
         case 32: FK_BV_LAUNCH(4, 0); break;
         case 64: FK_BV_LAUNCH(8, 0); break;
-        case 96: if (nrm) FK_BV_LAUNCH(12, 1); else FK_BV_LAUNCH16(12); break;
+        case 96: if (nrm) FK_BV_LAUNCH(12, 1); else if (q8) FK_BV_LAUNCH_G(12, 0, 2); else FK_BV_LAUNCH16(12); break;
         case 128: if (c8) FK_BV_LAUNCH(16, 2); else if (nrm) FK_BV_LAUNCH(16, 1); else FK_BV_LAUNCH16(16); break;
         case 256: if (c8) FK_BV_LAUNCH(32, 2); else if (nrm) FK_BV_LAUNCH(32, 1); else FK_BV_LAUNCH16(32); break;
         default: FK_BV_LAUNCH(0, 0); break;

@@ -210,9 +210,11 @@ typedef struct {
     uint32_t prune_variant, prune_rev_variant;
     double t_tune;
     /* FK_FLAG_GRAM on FP16-form rows of width 96, 128 or 256: BridgeView
-     * targets of d_bv_computed that a rigorous bound on the 16-bit Gram-form
-     * distance showed to be beyond the pool's last distance, so no FP32
-     * distance was computed for them (they cannot enter the pool). */
+     * targets of d_bv_computed that a rigorous bound on a low-precision copy
+     * of the rows (8-bit integers with one scale per row at width 96, the
+     * 16-bit operands at 128 and 256) showed to be beyond the pool's last
+     * distance, so no FP32 distance was computed for them (they cannot
+     * enter the pool). */
     uint64_t d_bv_rejected;
 } fk_stats;
 

@@ -171,10 +171,11 @@ def test_gram_full_build_identical(device, name, pg):
 @pytest.mark.parametrize("pg", [FK_PG_HNSW_LAYER, FK_PG_ALPHAPG])
 def test_gram_bound_pass_build_identical(device, d, pg):
     """FP16-form rows of widths 96, 128 and 256 with FK_FLAG_GRAM: BridgeView
-    drops targets whose 16-bit bound puts them beyond the pool's last
-    distance, and the Gram-form prune may run; the graph, the distance
-    counts and the BridgeView pairs evaluated equal the plain build's, and
-    the bound must have dropped some targets."""
+    drops targets whose low-precision bound (8-bit rows at width 96, 16-bit
+    ones at 128 and 256) puts them beyond the pool's last distance, and the
+    Gram-form prune may run; the graph, the distance counts and the
+    BridgeView pairs evaluated equal the plain build's, and the bound must
+    have dropped some targets."""
     x = unit_mixture(5000, d, 16, seed=26 + d, sigma=0.3)
     say("gram bound pass", d, pg)
     device.upload(x)
@@ -190,6 +191,33 @@ def test_gram_bound_pass_build_identical(device, d, pg):
     assert st0["d_bv_rejected"] == 0 and 0 < st1["d_bv_rejected"] < st1["d_bv_computed"]
     say(f"  rejected {st1['d_bv_rejected']} of {st1['d_bv_computed']} computed BridgeView pairs; "
         f"prune variant {st1['prune_variant'] - 1}")
+
+
+@pytest.mark.parametrize("pg", [FK_PG_HNSW_LAYER, FK_PG_NSG])
+def test_bound_pass_8bit_rows_unnormalised_with_duplicates(device, pg):
+    """The 8-bit bound pass at width 96 on rows that are not unit-normalised
+    (FP16 form forced; per-row scales from 0.02 to 50) with exact and near
+    duplicates (pairs at distance 0 and ~1e-6 of the row norm): the graph and
+    counts equal the plain build's and some targets are dropped."""
+    rng = np.random.default_rng(27)
+    x = float_mixture(4000, 96, 16, seed=27)
+    x *= np.exp(rng.uniform(np.log(0.02), np.log(50.0), size=(len(x), 1))).astype(np.float32)
+    dup = rng.choice(len(x), 400, replace=False)
+    x[dup[:200]] = x[dup[200:]]
+    x[dup[:100]] *= np.float32(1.0 + 1e-6)
+    say("8-bit bound pass, unnormalised rows with duplicates", pg)
+    device.upload(x)
+    p = nnd_params(K=40, L=40, S=10, R=40, iterations=4, nsg_R=24, search_L=40, search_K=60, angle=60)
+    off0, ids0, ep0, w0, st0 = device.build(pg, p)
+    p.flags = GRAM | F16
+    off1, ids1, ep1, w1, st1 = device.build(pg, p)
+    assert st1["gram_mode"] == _ffi.FK_GRAM_F16
+    assert ep0 == ep1 and w0 == w1
+    assert np.array_equal(off0, off1) and np.array_equal(ids0, ids1)
+    for k in ("d_join", "d_prune", "d_prune_rev", "d_bv", "d_bv_computed", "d_ep"):
+        assert st1[k] == st0[k], k
+    assert 0 < st1["d_bv_rejected"] < st1["d_bv_computed"]
+    say(f"  rejected {st1['d_bv_rejected']} of {st1['d_bv_computed']}")
 
 
 def exact_rows(kind):

@@ -412,12 +412,31 @@ __global__ void k_label_init(uint32_t *__restrict__ label, uint32_t N) {
     if (v < N) label[v] = v;
 }
 
-// One wavefront per node: push the node's label along its out-edges to
-// unreached targets (atomicMin); *changed is set when any label dropped.
-__global__ void k_label_prop(const uint32_t *__restrict__ nsg_id, const uint32_t *__restrict__ nsg_len, uint32_t N,
-                             uint32_t ncap, uint32_t *__restrict__ label, uint32_t *__restrict__ changed) {
-    const uint32_t u = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    if (u >= N) return;
+// The nodes the search from the entry points did not reach, in any order
+// (one atomic per wave).
+__global__ void k_list_unreached(const uint32_t *__restrict__ label, uint32_t N, uint32_t *__restrict__ list,
+                                 uint32_t *__restrict__ n_list) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool un = v < N && label[v] != FK_REACHED;
+    const uint64_t m = __ballot(un);
+    if (m == 0) return;
+    const uint32_t lead = (uint32_t)__builtin_ctzll(m);
+    uint32_t base = 0;
+    if (lane_id() == lead) base = atomicAdd(n_list, (uint32_t)__builtin_popcountll(m));
+    base = (uint32_t)__shfl((int)base, (int)lead, 64);
+    if (un) list[base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = v;
+}
+
+// One wavefront per unreached node: push the node's label along its
+// out-edges to unreached targets (atomicMin); *changed is set when any
+// label dropped.  The fixed point (each label the smallest id that reaches
+// the node in the unreached subgraph) does not depend on the list's order.
+__global__ void k_label_prop(const uint32_t *__restrict__ nsg_id, const uint32_t *__restrict__ nsg_len,
+                             const uint32_t *__restrict__ list, uint32_t n_list, uint32_t ncap,
+                             uint32_t *__restrict__ label, uint32_t *__restrict__ changed) {
+    const uint32_t w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (w >= n_list) return;
+    const uint32_t u = list[w];
     const uint32_t lu = label[u];
     if (lu == FK_REACHED) return;
     const uint32_t deg = nsg_len[u];
@@ -427,6 +446,24 @@ __global__ void k_label_prop(const uint32_t *__restrict__ nsg_id, const uint32_t
         const uint32_t lt = label[t];
         if (lt != FK_REACHED && lt > lu) {
             if (atomicMin(&label[t], lu) > lu) ch = true;
+        }
+    }
+    if (wave_ballot(ch) && lane_id() == 0) *changed = 1u;
+}
+
+// Shortcut: a node labelled l takes l's label when smaller (every node
+// that reaches l reaches the node through l), which halves the remaining
+// path lengths per round; the fixed point is unchanged.
+__global__ void k_label_jump(const uint32_t *__restrict__ list, uint32_t n_list, uint32_t *__restrict__ label,
+                             uint32_t *__restrict__ changed) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    bool ch = false;
+    if (w < n_list) {
+        const uint32_t u = list[w];
+        const uint32_t l = label[u];
+        if (l != u) {
+            const uint32_t ll = label[l];
+            if (ll < l && atomicMin(&label[u], ll) > ll) ch = true;
         }
     }
     if (wave_ballot(ch) && lane_id() == 0) *changed = 1u;
@@ -551,12 +588,21 @@ void tree_grow(BuildState &b) {
         ++levels;
     }
     const double t_bfs = now_sec();
-    // 2. min-label propagation over the unreached subgraph
+    // 2. min-label propagation over the unreached subgraph, whose nodes are
+    // listed once (f0 is free after the search)
+    FK_HIP(hipMemsetAsync(cnt.p, 0, 4, b.s));
+    hipLaunchKernelGGL(k_list_unreached, dim3((N + 255) / 256), dim3(256), 0, b.s, label.p, N, f0.p, cnt.p);
+    FK_LAUNCH_CHECK();
+    uint32_t n_un = 0;
+    FK_HIP(hipMemcpyAsync(&n_un, cnt.p, 4, hipMemcpyDeviceToHost, b.s));
+    b.sync();
     uint32_t rounds = 0;
-    while (true) {
+    while (n_un) {
         FK_HIP(hipMemsetAsync(cnt.p, 0, 4, b.s));
-        hipLaunchKernelGGL(k_label_prop, dim3((N + 3) / 4), dim3(256), 0, b.s, b.nsg_id.p, b.nsg_len.p, N, b.ncap,
-                           label.p, cnt.p);
+        hipLaunchKernelGGL(k_label_prop, dim3((n_un + 3) / 4), dim3(256), 0, b.s, b.nsg_id.p, b.nsg_len.p, f0.p,
+                           n_un, b.ncap, label.p, cnt.p);
+        FK_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_label_jump, dim3((n_un + 255) / 256), dim3(256), 0, b.s, f0.p, n_un, label.p, cnt.p);
         FK_LAUNCH_CHECK();
         uint32_t ch = 0;
         FK_HIP(hipMemcpyAsync(&ch, cnt.p, 4, hipMemcpyDeviceToHost, b.s));
@@ -571,8 +617,9 @@ void tree_grow(BuildState &b) {
     FK_HIP(hipMemcpyAsync(b.h->host_deg.p, b.nsg_len.p, (size_t)N * 4, hipMemcpyDeviceToHost, b.s));
     b.sync();
     uint32_t *lab = b.h->host_lab.p, *deg = b.h->host_deg.p;
-    // every round reads each unreached node's out-list and its targets' labels
-    b.prof_end(pe, FK_PROF_CONN, (double)(rounds + levels) * ((double)N * 8.0 + 8.0 * (double)N * b.P.nsg_R));
+    // the search reads every node's out-list and its targets' labels once;
+    // each propagation round those of the unreached nodes
+    b.prof_end(pe, FK_PROF_CONN, ((double)N + (double)rounds * n_un) * (8.0 + 8.0 * b.P.nsg_R));
     b.prof_flush();
     const double t_lab = now_sec();
     std::vector<uint32_t> roots_of;  // attached ids, ascending

@@ -1737,7 +1737,7 @@ __device__ __forceinline__ float octet_from6(uint32_t v) {
 }
 
 #ifndef FK_BVW_INFLIGHT8Q
-#define FK_BVW_INFLIGHT8Q 4  // 8-bit rows per octet in flight in the bound pass
+#define FK_BVW_INFLIGHT8Q 8  // 8-bit rows per octet in flight in the bound pass
 #endif
 // Relative error of |x^ - y^|^2 evaluated in FP32 from the stored scales and
 // |x^|^2 (each rounded once) and the exact integer dot product, as a share

@@ -1441,6 +1441,9 @@ __global__ __launch_bounds__(1024) void k_bv_count(const uint8_t *__restrict__ p
 #ifndef FK_BVW_WPE
 #define FK_BVW_WPE 4       // waves per SIMD the FP32 row kinds are compiled for
 #endif
+#ifndef FK_BVW_WPE32
+#define FK_BVW_WPE32 3     // the same at width 256 (its chains take more registers)
+#endif
 #define FK_BVW_TODO 320    // one read of 64 FK_BVW_PREFETCH pairs + a remainder below 64
 #ifndef FK_BVW_PREFETCH
 #define FK_BVW_PREFETCH 4  // chunks of 64 out-list entries read at once
@@ -1748,7 +1751,7 @@ __device__ __forceinline__ float octet_from6(uint32_t v) {
 #define FK_BVW_INFLIGHT16 4  // rows per octet in flight in the bound pass, width 96
 #endif
 #ifndef FK_BVW_INFLIGHT16W
-#define FK_BVW_INFLIGHT16W 2  // the same from width 128 up (registers)
+#define FK_BVW_INFLIGHT16W 4  // the same from width 128 up
 #endif
 // 1 +- 2^-20, 1 +- 2^-18 (exact in FP32): margins of the bound pass's test
 #define FK_BVW_UP20 1.00000095367431640625f
@@ -1775,7 +1778,7 @@ __device__ __forceinline__ float octet_from6(uint32_t v) {
 // step unchanged, so the candidates and the merged pools are the same.  G:
 // the kernel with the pass (the others are compiled without it).
 template <int NB, int MODE, int G>
-__global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 4 : (NB == 32 ? 3 : FK_BVW_WPE)))) void k_bv_round(
+__global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 4 : (NB == 32 ? FK_BVW_WPE32 : FK_BVW_WPE)))) void k_bv_round(
     const void *__restrict__ chains_v, const float2 *__restrict__ nrm, uint32_t dpad, uint32_t N, uint32_t cap,
     uint32_t W, uint32_t S_cap_all, const uint32_t *__restrict__ set_of, const uint32_t *__restrict__ set_scap,
     uint64_t *__restrict__ pool, uint8_t *__restrict__ pflg, uint32_t *__restrict__ cnt,

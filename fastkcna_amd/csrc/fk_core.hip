@@ -902,15 +902,18 @@ __global__ __launch_bounds__(256) void k_bf_keys_wide(const float *__restrict__ 
 #define FK_SEL_THREADS 1024
 #define FK_SEL_MAXK 2048
 
-__global__ __launch_bounds__(FK_SEL_THREADS) void k_topk_select(const uint64_t *__restrict__ keys, uint32_t n,
-                                                                uint32_t K, uint32_t *__restrict__ out_ids,
-                                                                float *__restrict__ out_d) {
+// The K smallest keys of kq[0, n) (all of them when n < K, then padded with
+// FK_KEY_EMPTY) in ascending order, handed to emit(i, key) for i < K: a
+// radix select on the key bytes (one 256-bin histogram per byte, most
+// significant first) finds the K-th key, the keys up to it are gathered and
+// sorted.  The whole workgroup calls it.
+template <class Emit>
+__device__ __forceinline__ void topk_block(const uint64_t *__restrict__ kq, uint32_t n, uint32_t K, Emit emit) {
     __shared__ uint32_t hist[256];
     __shared__ uint64_t s_sel[FK_SEL_MAXK];
     __shared__ uint32_t s_cnt;
     __shared__ uint64_t s_prefix, s_mask;
     __shared__ uint32_t s_need;
-    const uint64_t *kq = keys + (size_t)blockIdx.x * n;
     if (threadIdx.x == 0) { s_prefix = 0; s_mask = 0; s_need = K; s_cnt = 0; }
     __syncthreads();
     for (int byte = 7; byte >= 0; --byte) {
@@ -949,11 +952,32 @@ __global__ __launch_bounds__(FK_SEL_THREADS) void k_topk_select(const uint64_t *
     for (uint32_t i = cnt + threadIdx.x; i < np; i += blockDim.x) s_sel[i] = FK_KEY_EMPTY;
     __syncthreads();
     block_bitonic_sort(s_sel, np);
-    for (uint32_t i = threadIdx.x; i < K; i += blockDim.x) {
-        const uint64_t k = i < cnt ? s_sel[i] : FK_KEY_EMPTY;
+    for (uint32_t i = threadIdx.x; i < K; i += blockDim.x) emit(i, i < cnt ? s_sel[i] : FK_KEY_EMPTY);
+}
+
+// One workgroup per query: the K nearest of its n keys as ids (and distances).
+__global__ __launch_bounds__(FK_SEL_THREADS) void k_topk_select(const uint64_t *__restrict__ keys, uint32_t n,
+                                                                uint32_t K, uint32_t *__restrict__ out_ids,
+                                                                float *__restrict__ out_d) {
+    topk_block(keys + (size_t)blockIdx.x * n, n, K, [&](uint32_t i, uint64_t k) {
         out_ids[(size_t)blockIdx.x * K + i] = key_id(k);
         if (out_d) out_d[(size_t)blockIdx.x * K + i] = key_dist(k);
-    }
+    });
+}
+
+// First stage for long key rows: workgroup (c, q) keeps the K smallest keys
+// of chunk c (len keys) of query q's row in part[(q * C + c) * K ...]; the
+// K smallest of the union of the chunks' K smallest are the row's K
+// smallest (the keys are distinct), which k_topk_select then takes from the
+// C * K kept keys.
+__global__ __launch_bounds__(FK_SEL_THREADS) void k_topk_chunks(const uint64_t *__restrict__ keys, uint32_t n,
+                                                                uint32_t len, uint32_t K,
+                                                                uint64_t *__restrict__ part) {
+    const uint32_t c = blockIdx.x, q = blockIdx.y, C = gridDim.x;
+    const uint32_t lo = c * len;
+    const uint32_t m = lo < n ? (n - lo < len ? n - lo : len) : 0u;
+    uint64_t *o = part + ((size_t)q * C + c) * K;
+    topk_block(keys + (size_t)q * n + lo, m, K, [&](uint32_t i, uint64_t k) { o[i] = k; });
 }
 
 __global__ void k_add_u32(uint32_t *__restrict__ v, size_t n, uint32_t add) {
@@ -975,6 +999,7 @@ void bruteforce_knn(hipStream_t s, const Dataset &ds, const float *d_queries, ui
     if (qb < 1) qb = 1;
     if (qb > n_q) qb = n_q;
     DevBuf<uint64_t> keys(qb * (size_t)n_used);
+    DevBuf<uint64_t> part;  // the chunks' kept keys
     for (uint32_t q0 = 0; q0 < n_q; q0 += (uint32_t)qb) {
         const uint32_t nq = (uint32_t)((n_q - q0) < qb ? (n_q - q0) : qb);
         if (wide) {
@@ -991,8 +1016,21 @@ void bruteforce_knn(hipStream_t s, const Dataset &ds, const float *d_queries, ui
                                keys.p);
         }
         FK_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_topk_select, dim3(nq), dim3(FK_SEL_THREADS), 0, s, keys.p, n_used, K,
-                           d_out_ids + (size_t)q0 * K, d_out_dists ? d_out_dists + (size_t)q0 * K : nullptr);
+        // rows much longer than K: chunks first, so that about a thousand
+        // workgroups share the select instead of one per query
+        const uint32_t C = std::min<uint32_t>((1024u + nq - 1) / nq, n_used / (16u * K));
+        if (C >= 2) {
+            const uint32_t len = (n_used + C - 1) / C;
+            part.ensure((size_t)nq * C * K);
+            hipLaunchKernelGGL(k_topk_chunks, dim3(C, nq), dim3(FK_SEL_THREADS), 0, s, keys.p, n_used, len, K,
+                               part.p);
+            FK_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_topk_select, dim3(nq), dim3(FK_SEL_THREADS), 0, s, part.p, C * K, K,
+                               d_out_ids + (size_t)q0 * K, d_out_dists ? d_out_dists + (size_t)q0 * K : nullptr);
+        } else {
+            hipLaunchKernelGGL(k_topk_select, dim3(nq), dim3(FK_SEL_THREADS), 0, s, keys.p, n_used, K,
+                               d_out_ids + (size_t)q0 * K, d_out_dists ? d_out_dists + (size_t)q0 * K : nullptr);
+        }
         FK_LAUNCH_CHECK();
     }
     if (base) {

@@ -119,42 +119,50 @@ __global__ __launch_bounds__(256) void k_rows16(const float *__restrict__ rows, 
     }
 }
 
-// rows8q: each row of width 96 as 8-bit integers q with one scale s = max |x| /
-// 127 (q = x / s rounded, so |x - s q| <= s / 2 per value), then s, the error
-// norm |x - s q| (exact products and differences in double, rounded up) and
-// |s q|^2 (s^2 sum q^2 in double, rounded once) as FP32, 20 zero bytes: 128
-// bytes, one cache line.  One wave per row, lane l holds values l and l + 64.
+// rows8q: each row of width D (96 or 256) as 8-bit integers q with one
+// scale s = max |x| / 127 (q = x / s rounded, so |x - s q| <= s / 2 per
+// value), then s, the error norm |x - s q| (exact products and differences
+// in double, rounded up) and |s q|^2 (s^2 sum q^2 in double, rounded once)
+// as FP32, the rest of the row's lines zero: 128 bytes (one cache line) at
+// width 96, 384 at width 256.  One wave per row, lane l holds values l + 64 j.
+template <int D>
 __global__ __launch_bounds__(256) void k_rows8q(const float *__restrict__ rows, uint32_t stride, uint32_t n,
                                                 uint4 *__restrict__ out) {
+    constexpr uint32_t J = (D + 63) / 64, RB = D == 96 ? 128u : 384u;
     const uint32_t lane = lane_id();
     for (uint32_t i = blockIdx.x * 4u + (threadIdx.x >> 6); i < n; i += gridDim.x * 4u) {
         const float *r = rows + (size_t)i * stride;
-        const float x0 = r[lane], x1 = lane < 32u ? r[64u + lane] : 0.f;
-        float m = fmaxf(fabsf(x0), fabsf(x1));
+        float x[J];
+        float m = 0.f;
+#pragma unroll
+        for (uint32_t j = 0; j < J; ++j) {
+            x[j] = lane + 64u * j < (uint32_t)D ? r[lane + 64u * j] : 0.f;
+            m = fmaxf(m, fabsf(x[j]));
+        }
         for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
         const float s = m / 127.f;
-        auto quant = [&](float x) -> int {
-            if (!(s > 0.f)) return 0;
-            const float q = rintf(x / s);
-            return (int)fminf(fmaxf(q, -127.f), 127.f);
-        };
-        const int q0 = quant(x0), q1 = lane < 32u ? quant(x1) : 0;
-        const double d0 = (double)x0 - (double)s * q0, d1 = (double)x1 - (double)s * q1;
-        double err2 = d0 * d0 + d1 * d1;
-        int qq = q0 * q0 + q1 * q1;
+        double err2 = 0.0;
+        int qq = 0;
+        int8_t *ob = reinterpret_cast<int8_t *>(out + (size_t)i * (RB / 16u));
+#pragma unroll
+        for (uint32_t j = 0; j < J; ++j) {
+            int q = 0;
+            if (s > 0.f) q = (int)fminf(fmaxf(rintf(x[j] / s), -127.f), 127.f);
+            const double dd = (double)x[j] - (double)s * q;
+            err2 += dd * dd;
+            qq += q * q;
+            if (lane + 64u * j < (uint32_t)D) ob[lane + 64u * j] = (int8_t)q;
+        }
         for (int o = 32; o >= 1; o >>= 1) {
             err2 += __shfl_xor(err2, o, 64);
             qq += __shfl_xor(qq, o, 64);
         }
-        int8_t *ob = reinterpret_cast<int8_t *>(out + (size_t)i * 8u);
-        ob[lane] = (int8_t)q0;
-        if (lane < 32u) ob[64u + lane] = (int8_t)q1;
-        if (lane < 8u) {
+        if (lane < (RB - D) / 4u) {
             float v = 0.f;
             if (lane == 0) v = s;
             if (lane == 1) v = (float)(sqrt(err2) * (1.0 + 1e-6));
             if (lane == 2) v = (float)((double)s * (double)s * (double)qq);
-            reinterpret_cast<float *>(ob + 96)[lane] = v;
+            reinterpret_cast<float *>(ob + D)[lane] = v;
         }
     }
 }
@@ -234,11 +242,17 @@ void gram_prepare(hipStream_t s, Dataset &ds, uint32_t flags) {
         hipLaunchKernelGGL(k_rows16<FK_GRAM_BF16>, dim3(g16), dim3(256), 0, s, ds.rows.p, ds.stride, ds.dpad,
                            ds.n_total, scale, ds.rows16.p);
     FK_LAUNCH_CHECK();
-    // width 96 in the FP16 form: the 8-bit copy of BridgeView's bound pass
+    // widths 96 and 256 in the FP16 form: the 8-bit copy of BridgeView's
+    // bound pass
     ds.rows8q.release();
     if (fmt == FK_GRAM_F16 && ds.dpad == 96) {
         ds.rows8q.alloc((size_t)ds.n_total * 8u);
-        hipLaunchKernelGGL(k_rows8q, dim3(g16), dim3(256), 0, s, ds.rows.p, ds.stride, ds.n_total, ds.rows8q.p);
+        hipLaunchKernelGGL(k_rows8q<96>, dim3(g16), dim3(256), 0, s, ds.rows.p, ds.stride, ds.n_total, ds.rows8q.p);
+        FK_LAUNCH_CHECK();
+    } else if (fmt == FK_GRAM_F16 && ds.dpad == 256) {
+        ds.rows8q.alloc((size_t)ds.n_total * 24u);
+        hipLaunchKernelGGL(k_rows8q<256>, dim3(g16), dim3(256), 0, s, ds.rows.p, ds.stride, ds.n_total,
+                           ds.rows8q.p);
         FK_LAUNCH_CHECK();
     }
     // exact form with values in [0, 255]: a chain-major byte copy for the

@@ -1742,6 +1742,9 @@ __device__ __forceinline__ float octet_from6(uint32_t v) {
 #ifndef FK_BVW_INFLIGHT8Q
 #define FK_BVW_INFLIGHT8Q 8  // 8-bit rows per octet in flight in the bound pass
 #endif
+#ifndef FK_BVW_INFLIGHT8Q32
+#define FK_BVW_INFLIGHT8Q32 4  // the same at width 256 (three pieces per lane and row)
+#endif
 // Relative error of |x^ - y^|^2 evaluated in FP32 from the stored scales and
 // |x^|^2 (each rounded once) and the exact integer dot product, as a share
 // of |x^|^2 + |y^|^2: at most about 10 roundings, here 16
@@ -1897,7 +1900,11 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
     mine.load(lane_chains + (size_t)i * dpad, nb);
     const float ni = DOT ? nrm[i].x : 0.f;
     constexpr bool G16 = G == 1 && MODE == 0 && (NB == 12 || NB == 16 || NB == 32);
-    constexpr bool Q8 = G == 2 && MODE == 0 && NB == 12;
+    constexpr bool Q8 = G == 2 && MODE == 0 && (NB == 12 || NB == 32);
+    // 8-bit rows: width 96 one 128-byte line (lanes 0-5 values, lane 6 the
+    // scalars), width 256 three (lane k pieces k and 8 + k, scalars piece 16)
+    constexpr uint32_t QU = NB == 32 ? 24u : 8u;
+    constexpr uint32_t I8 = NB == 32 ? FK_BVW_INFLIGHT8Q32 : FK_BVW_INFLIGHT8Q;
     constexpr uint32_t NBH = G16 ? (uint32_t)NB : 4u;
     const bool bound16 = G16 && r16 != nullptr && thr < INFINITY;  // wave-uniform
     const bool bound8 = Q8 && q8 != nullptr && thr < INFINITY;     // wave-uniform
@@ -1911,11 +1918,20 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
         n16 = gnrm[i];
         t16 = __builtin_sqrtf(thr / (1.f - gb.gam)) * FK_BVW_UP20;
     }
+    uint4 mine8b = make_uint4(0u, 0u, 0u, 0u);
     if (bound8) {
-        mine8 = q8[(size_t)i * 8u + k];
-        s8 = octet_from6(mine8.x);
-        e8 = octet_from6(mine8.y);
-        nq8 = octet_from6(mine8.z);
+        mine8 = q8[(size_t)i * QU + k];
+        if constexpr (NB == 32) {
+            mine8b = q8[(size_t)i * QU + 8u + k];
+            const uint4 sc = q8[(size_t)i * QU + 16u];
+            s8 = __uint_as_float(sc.x);
+            e8 = __uint_as_float(sc.y);
+            nq8 = __uint_as_float(sc.z);
+        } else {
+            s8 = octet_from6(mine8.x);
+            e8 = octet_from6(mine8.y);
+            nq8 = octet_from6(mine8.z);
+        }
         t16 = __builtin_sqrtf(thr / (1.f - gb.gam)) * FK_BVW_UP20;
     }
     uint32_t np = np0, nc = 0, nt = 0;
@@ -1942,26 +1958,42 @@ __global__ __launch_bounds__(64 * FK_BVW_WAVES) __attribute__((amdgpu_waves_per_
             // the bound pass on the 8-bit rows (one 16-byte piece per lane,
             // a whole 128-byte row per octet); survivors compacted as below
             uint32_t m2 = 0;
-            for (uint32_t e0 = 0; e0 < m; e0 += 8 * FK_BVW_INFLIGHT8Q) {
-                uint4 hq[FK_BVW_INFLIGHT8Q];
-                uint32_t tt[FK_BVW_INFLIGHT8Q];
+            for (uint32_t e0 = 0; e0 < m; e0 += 8 * I8) {
+                uint4 hq[I8], hq2[NB == 32 ? I8 : 1], hs[NB == 32 ? I8 : 1];
+                uint32_t tt[I8];
 #pragma unroll
-                for (uint32_t u = 0; u < FK_BVW_INFLIGHT8Q; ++u) {
+                for (uint32_t u = 0; u < I8; ++u) {
                     const uint32_t e = e0 + u * 8 + g;
                     tt[u] = e < m ? todo[e] : i;
-                    hq[u] = q8[(size_t)tt[u] * 8u + k];
+                    hq[u] = q8[(size_t)tt[u] * QU + k];
+                    if constexpr (NB == 32) {
+                        hq2[u] = q8[(size_t)tt[u] * QU + 8u + k];
+                        hs[u] = q8[(size_t)tt[u] * QU + 16u];
+                    }
                 }
                 wave_lds_sync();
 #pragma unroll
-                for (uint32_t u = 0; u < FK_BVW_INFLIGHT8Q; ++u) {
+                for (uint32_t u = 0; u < I8; ++u) {
                     int part = __builtin_amdgcn_sdot4((int)mine8.x, (int)hq[u].x, 0, false);
                     part = __builtin_amdgcn_sdot4((int)mine8.y, (int)hq[u].y, part, false);
                     part = __builtin_amdgcn_sdot4((int)mine8.z, (int)hq[u].z, part, false);
                     part = __builtin_amdgcn_sdot4((int)mine8.w, (int)hq[u].w, part, false);
-                    part = k < 6u ? part : 0;  // lanes 6 and 7 hold the row's scalars
+                    float sy, ey, nx;
+                    if constexpr (NB == 32) {
+                        part = __builtin_amdgcn_sdot4((int)mine8b.x, (int)hq2[u].x, part, false);
+                        part = __builtin_amdgcn_sdot4((int)mine8b.y, (int)hq2[u].y, part, false);
+                        part = __builtin_amdgcn_sdot4((int)mine8b.z, (int)hq2[u].z, part, false);
+                        part = __builtin_amdgcn_sdot4((int)mine8b.w, (int)hq2[u].w, part, false);
+                        sy = __uint_as_float(hs[u].x);
+                        ey = __uint_as_float(hs[u].y);
+                        nx = nq8 + __uint_as_float(hs[u].z);
+                    } else {
+                        part = k < 6u ? part : 0;  // lanes 6 and 7 hold the row's scalars
+                        sy = octet_from6(hq[u].x);
+                        ey = octet_from6(hq[u].y);
+                        nx = nq8 + octet_from6(hq[u].z);
+                    }
                     const float dot = octet_sum_i32(part);
-                    const float sy = octet_from6(hq[u].x), ey = octet_from6(hq[u].y);
-                    const float nx = nq8 + octet_from6(hq[u].z);
                     // |x^ - y^|^2 in FP32 (dot exact), its error within FK_Q8_FREL nx
                     const float dt = nx - 2.f * ((s8 * sy) * dot);
                     const float f = FK_Q8_FREL * nx;
@@ -2124,7 +2156,7 @@ static void launch_bv_round(BuildState &b, uint32_t W, uint32_t scap, const uint
     const bool g16 = (b.P.flags & FK_FLAG_GRAM) && b.ds->gram_fmt == FK_GRAM_F16 && b.ds->rows16.p;
     const uint16_t *r16 = g16 ? b.ds->rows16.p : nullptr;
     const GramBound gb = g16 ? gram_bound(*b.ds) : GramBound{};
-    // width 96: the pass on the 8-bit rows when they are made
+    // widths 96 and 256: the pass on the 8-bit rows when they are made
     const uint4 *q8 = g16 && b.ds->rows8q.p ? b.ds->rows8q.p : nullptr;
 #define FK_BV_LAUNCH_G(NBV, MODE, G)                                                                             \
     hipLaunchKernelGGL((k_bv_round<NBV, MODE, G>), dim3(grid), dim3(64 * wpb), lds, b.s,                       \
@@ -2140,7 +2172,12 @@ static void launch_bv_round(BuildState &b, uint32_t W, uint32_t scap, const uint
         case 64: FK_BV_LAUNCH(8, 0); break;
         case 96: if (nrm) FK_BV_LAUNCH(12, 1); else if (q8) FK_BV_LAUNCH_G(12, 0, 2); else FK_BV_LAUNCH16(12); break;
         case 128: if (c8) FK_BV_LAUNCH(16, 2); else if (nrm) FK_BV_LAUNCH(16, 1); else FK_BV_LAUNCH16(16); break;
-        case 256: if (c8) FK_BV_LAUNCH(32, 2); else if (nrm) FK_BV_LAUNCH(32, 1); else FK_BV_LAUNCH16(32); break;
+        case 256:
+            if (c8) FK_BV_LAUNCH(32, 2);
+            else if (nrm) FK_BV_LAUNCH(32, 1);
+            else if (q8) FK_BV_LAUNCH_G(32, 0, 2);
+            else FK_BV_LAUNCH16(32);
+            break;
         default: FK_BV_LAUNCH(0, 0); break;
     }
 #undef FK_BV_LAUNCH16

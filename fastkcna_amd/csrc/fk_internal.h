@@ -193,7 +193,7 @@ struct Dataset {
     DevBuf<float2> gram_nrm;
     DevBuf<uint8_t> chains8;  // exact form with values in [0, 255]: chain-major byte copy
     DevBuf<int32_t> norms8;   // with chains8: sum over dpad of (byte - 128)^2 per row
-    DevBuf<uint4> rows8q;     // width 96, FP16 form: per row 128 bytes (8 x 16): int8 q[96], FP32 scale s,
+    DevBuf<uint4> rows8q;     // widths 96 / 256, FP16 form: per row 128 / 384 bytes: int8 q[dpad], FP32 scale s,
                               // error norm |x - s q| rounded up, |s q|^2 (BridgeView bound pass)
     DevBuf<uint16_t> rows16;  // n_total x dpad 16-bit Gram operands (FP16 of the prescaled value, else BF16)
     int gram_fmt = 0;

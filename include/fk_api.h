@@ -74,10 +74,10 @@ typedef enum {
  * the FP32 distance bit for bit there (fk_prune honours the flag too).  On
  * FP16-form rows of width 96, 128 or 256, BridgeView first bounds each
  * target's distance from a low-precision copy (8-bit integers with a scale
- * per row at width 96, the 16-bit operands otherwise) and computes the FP32
+ * per row at widths 96 and 256, the 16-bit operands at 128) and computes the FP32
  * distance only when the bound cannot put it beyond the pool
  * (fk_stats.d_bv_rejected).  Memory: a 16-bit copy of the rows, plus 128
- * bytes per row at width 96 in the FP16 form. */
+ * (width 96) or 384 (width 256) bytes per row in the FP16 form. */
 #define FK_FLAG_GRAM 0x8u
 #define FK_FLAG_GRAM_BF16 0x10u  /* force the bounded BF16 form                  */
 #define FK_FLAG_GRAM_F16 0x20u   /* force the bounded FP16 form                  */

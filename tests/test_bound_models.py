@@ -8,7 +8,7 @@ two models directly, with the kernels' FP32 arithmetic restated in numpy
 float32 on random, scaled and adversarial rows (exact and near duplicates,
 rows at very different norms), against float64 distances.
 
-8-bit rows (width 96): q = rint(x / s) with s = max|x| / 127, the error norm
+8-bit rows (widths 96 and 256): q = rint(x / s) with s = max|x| / 127, the error norm
 eps = |x - s q| (float64, rounded up by 1e-6) and n = |s q|^2 stored as
 float32; the kernel forms dt = (n_x + n_y) - 2 (s_x s_y) (q_x . q_y) in FP32
 with the integer dot exact, f = 2^-20 (n_x + n_y), and drops the target
@@ -65,10 +65,11 @@ def rows(kind, rng, n=600, d=96):
     return x.astype(np.float32)
 
 
+@pytest.mark.parametrize("d", [96, 256])
 @pytest.mark.parametrize("kind", ["unit", "scaled", "sparse", "clustered"])
-def test_8bit_lower_bound_never_exceeds_the_distance(kind):
+def test_8bit_lower_bound_never_exceeds_the_distance(kind, d):
     rng = np.random.default_rng(97)
-    x = rows(kind, rng)
+    x = rows(kind, rng, n=600 if d == 96 else 300, d=d)
     q, s, eps, n = quantise8(x)
     x64 = x.astype(np.float64)
     worst = -np.inf

@@ -171,8 +171,8 @@ def test_gram_full_build_identical(device, name, pg):
 @pytest.mark.parametrize("pg", [FK_PG_HNSW_LAYER, FK_PG_ALPHAPG])
 def test_gram_bound_pass_build_identical(device, d, pg):
     """FP16-form rows of widths 96, 128 and 256 with FK_FLAG_GRAM: BridgeView
-    drops targets whose low-precision bound (8-bit rows at width 96, 16-bit
-    ones at 128 and 256) puts them beyond the pool's last distance, and the
+    drops targets whose low-precision bound (8-bit rows at widths 96 and
+    256, 16-bit ones at 128) puts them beyond the pool's last distance, and the
     Gram-form prune may run; the graph, the distance counts and the
     BridgeView pairs evaluated equal the plain build's, and the bound must
     have dropped some targets."""
